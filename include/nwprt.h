@@ -1,0 +1,344 @@
+/*
+ * nwprt.h -- C ABI of the MI355X (gfx950) neural wavelet PRT relighting path.
+ *
+ * The reference renderer (AlstonXiao/neural_wavelet_prt_demo) has no FFI: its
+ * relighting path is a set of C++ methods on `RenderEngine` plus on-disk
+ * formats.  Every entry point below says which reference interface it
+ * replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Every function returns an nwprt_status; NWPRT_OK (0) is success.  On
+ *    failure nwprt_last_error() returns a message for the calling thread.
+ *    No C++ exception crosses this boundary.
+ *  - "device" pointers are HIP device addresses on the context's GPU;
+ *    "host" pointers are ordinary CPU memory.  The caller owns every buffer
+ *    it passes in; the library owns scenes, environment maps and its own
+ *    work buffers.
+ *  - `stream` is a hipStream_t passed as void*; NULL selects the context's
+ *    own stream.  Calls are asynchronous on that stream unless stated.
+ *  - One context may be used from one host thread at a time.
+ *  - Pixel buffers follow the reference G-buffer layout
+ *    (Renderer/frames.h:143-183): float4 per pixel for positions, normals,
+ *    reflection directions, kd, ks, direct light and outputs; one float per
+ *    pixel for roughness; one byte per pixel for the environment-map flag.
+ */
+#ifndef NWPRT_H
+#define NWPRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NWPRT_ABI_VERSION 1
+
+/* Wavelet count and cubemap limits of this build. */
+#define NWPRT_MAX_K 1024        /* selected wavelets per environment map */
+#define NWPRT_MAX_FACE_SIZE 128 /* cubemap face edge N (power of two)    */
+#define NWPRT_MAX_LEVELS 32     /* hash-grid levels                      */
+
+typedef enum {
+  NWPRT_OK = 0,
+  NWPRT_ERR_INVALID_ARGUMENT = 1,
+  NWPRT_ERR_IO = 2,           /* file missing / unreadable                   */
+  NWPRT_ERR_FORMAT = 3,       /* malformed .npy / .hdr / json / shape        */
+  NWPRT_ERR_DEVICE = 4,       /* HIP runtime error                           */
+  NWPRT_ERR_OUT_OF_MEMORY = 5,
+  NWPRT_ERR_UNSUPPORTED = 6,  /* valid input this build does not implement   */
+  NWPRT_ERR_STATE = 7,        /* call order violated (e.g. p2 before p1)     */
+  NWPRT_ERR_NOT_FOUND = 8     /* unknown scene / environment-map handle      */
+} nwprt_status;
+
+/* Arithmetic of the per-(pixel, wavelet) MLP. */
+typedef enum {
+  NWPRT_PREC_FP32 = 0, /* plain form: FP32 vector ALU, one pixel per lane    */
+  NWPRT_PREC_FP16 = 1, /* fused MFMA, FP16 operands, FP32 accumulation       */
+  NWPRT_PREC_BF16 = 2, /* fused MFMA, BF16 operands, FP32 accumulation       */
+  NWPRT_PREC_FP8 = 3   /* fused MFMA, FP8-E4M3 operands, per-layer scales     */
+} nwprt_precision;
+
+typedef enum { NWPRT_DTYPE_F32 = 0, NWPRT_DTYPE_F16 = 1 } nwprt_dtype;
+
+/* Which float4 buffer the RGBA8 tone map reads
+ * (Renderer/toneMap.cu:80-85: wavelet full -> with-direct, indirect -> indirect). */
+typedef enum {
+  NWPRT_TONEMAP_FULL = 0,
+  NWPRT_TONEMAP_INDIRECT = 1,
+  NWPRT_TONEMAP_NONE = 2
+} nwprt_tonemap_source;
+
+typedef enum { NWPRT_HOST = 0, NWPRT_DEVICE = 1 } nwprt_location;
+
+typedef struct nwprt_context *nwprt_ctx;
+
+/* ---------------------------------------------------------------- scene -- */
+
+/* Multiresolution hash grid of the first-hit position (tiny-cuda-nn
+ * "HashGrid" as configured by config_comp.json "vtx_enc",
+ * Renderer/tinycudann/prt_network.h:45). */
+typedef struct {
+  int32_t n_levels;             /* 16  */
+  int32_t n_features_per_level; /* 2   */
+  int32_t log2_hashmap_size;    /* 19  */
+  int32_t base_resolution;      /* 16  */
+  float per_level_scale;        /* 1.5 */
+} nwprt_hashgrid_desc;
+
+/* Host-computed per-level table.  Computed once on the host in FP32 and
+ * passed to kernels; never re-derived on the device. */
+typedef struct {
+  int32_t n_levels;
+  float scale[NWPRT_MAX_LEVELS];        /* exp2(l*log2(b))*N_min - 1 (FP32) */
+  uint32_t resolution[NWPRT_MAX_LEVELS];/* ceil(scale) + 1                  */
+  uint32_t dense[NWPRT_MAX_LEVELS];     /* 1 if resolution^3 <= 2^log2T     */
+  uint32_t size[NWPRT_MAX_LEVELS];      /* entries in the level             */
+  uint32_t offset[NWPRT_MAX_LEVELS + 1];/* prefix sum of size; [n] = total  */
+} nwprt_hashgrid_levels;
+
+/* Network topology.  The per-pair network is
+ *   z = proj(hash(pos)) (.) wavelet_feature[:, c_k]          (feature_dim)
+ *   -> Linear(feature_dim, hidden) ReLU -> (n_hidden_layers-1) x
+ *      [Linear(hidden, hidden) ReLU] -> Linear(hidden, 3) -> output act.
+ * Bias-free, as tiny-cuda-nn networks are.  This build's fused kernels
+ * implement feature_dim 32, hidden 64, 3 hidden layers. */
+typedef struct {
+  nwprt_hashgrid_desc grid;
+  nwprt_dtype hash_dtype;    /* storage type of hash-grid entries          */
+  int32_t face_size;         /* N: wavelet feature table is [F][6*N*N]      */
+  int32_t feature_dim;       /* = n_levels * n_features_per_level (32)      */
+  int32_t hidden_width;      /* 64                                          */
+  int32_t n_hidden_layers;   /* 3                                           */
+  int32_t has_projection;    /* 1: feature_dim x feature_dim projection     */
+  int32_t output_softplus;   /* 1: softplus on the 3 outputs, 0: identity   */
+  int32_t aux_dims;          /* 0, or 13: G-buffer encodings (normal,
+                                reflect, roughness, kd, ks) entering the
+                                second layer (tcnn.cu cmp_enc Identity)     */
+  float aux_pad_value;       /* value of aux inputs 13..15 (padding)        */
+} nwprt_scene_desc;
+
+/* Host-side scene parameters (all row-major). */
+typedef struct {
+  const void *hash_table;        /* levels.offset[n] entries x F, hash_dtype */
+  const float *wavelet_features; /* [feature_dim][6*N*N]                     */
+  const float *projection;       /* [feature_dim][feature_dim] (out x in)    */
+  const float *layers;           /* W1 [H][F], W2..Wn [H][H], Wout [3][H],
+                                    then, if aux_dims, Waux [H][16]          */
+} nwprt_scene_params;
+
+/* ------------------------------------------------------------- G-buffer -- */
+
+/* One frame's G-buffer (Renderer/frames.h:96-183, written by the OptiX pass
+ * at Renderer/device_functions/camera.cu:247-270).  position: bbox-normalised
+ * first hit in [0,1]^3; normal/reflect encoded as v/2+0.5; kd, ks albedos;
+ * direct: first-hit direct light; env_flag: 1 where the primary ray hit
+ * geometry.  All float4 except roughness (float) and env_flag (uint8). */
+typedef struct {
+  float *position;
+  float *normal;
+  float *reflect;
+  float *roughness;
+  float *kd;
+  float *ks;
+  float *direct;
+  uint8_t *env_flag;
+} nwprt_gbuffer;
+
+/* Outputs (Renderer/frames.h:135-141): float4 (rgb, 1) per pixel and packed
+ * RGBA8.  Any pointer may be NULL to skip that output. */
+typedef struct {
+  float *indirect;             /* fbPredictedIndirect              */
+  float *indirect_with_direct; /* fbPredictedIndirectWithDirect    */
+  uint32_t *rgba8;             /* fbFinalColor                     */
+} nwprt_frame_out;
+
+/* ---------------------------------------------------------------- core -- */
+
+int32_t nwprt_abi_version(void);
+const char *nwprt_last_error(void);
+const char *nwprt_status_string(nwprt_status s);
+
+/* Replaces the process-global TCNN state of Renderer/tinycudann/tcnn.cu:40-47
+ * (network singleton + 4 streams) with an explicit context. */
+nwprt_status nwprt_create(int device, nwprt_ctx *out);
+nwprt_status nwprt_destroy(nwprt_ctx ctx);
+/* RenderEngine::TCNNSync, Renderer/tinycudann/tcnn.cu:154-158. */
+nwprt_status nwprt_sync(nwprt_ctx ctx);
+/* The context's own stream (hipStream_t as void*). */
+nwprt_status nwprt_get_stream(nwprt_ctx ctx, void **stream);
+
+/* ------------------------------------------------------- host utilities -- */
+/* These never touch the GPU. */
+
+/* Hash-grid level table (tiny-cuda-nn grid conventions behind
+ * Renderer/tinycudann/prt_network.h:45). */
+nwprt_status nwprt_hashgrid_level_table(const nwprt_hashgrid_desc *desc,
+                                        nwprt_hashgrid_levels *out);
+/* Element counts of a scene's parameter arrays. */
+nwprt_status nwprt_scene_param_counts(const nwprt_scene_desc *desc,
+                                      size_t *hash_entries,
+                                      size_t *wavelet_feature_count,
+                                      size_t *projection_count,
+                                      size_t *layer_weight_count);
+
+/* Seeded synthetic inputs (counter-based SplitMix64 streams, Box-Muller in
+ * double).  They stand in for the OptiX G-buffer pass and the offline
+ * training / env-map data.  Outputs are host arrays sized by the caller. */
+nwprt_status nwprt_synth_scene(const nwprt_scene_desc *desc, uint64_t seed,
+                               float hash_std, float feature_std,
+                               void *hash_table, float *wavelet_features,
+                               float *projection, float *layers);
+nwprt_status nwprt_synth_gbuffer(uint64_t seed, int32_t width, int32_t height,
+                                 float background_fraction,
+                                 const nwprt_gbuffer *host_out);
+/* faces: [6][N][N][3] FP32 RGB, log-normal(0, 1.5) plus one 3x3 sun at 1e3. */
+nwprt_status nwprt_synth_cubemap(uint64_t seed, int32_t face_size,
+                                 float *faces);
+
+/* File formats.  Arrays returned through `float **` / `int32_t **` are
+ * allocated by the library and released with nwprt_free_host. */
+void nwprt_free_host(void *p);
+/* npy v1/v2/v3, dtypes <f4 <f8 <f2 <i4 <i8 <u1, C or Fortran order, converted
+ * (as Renderer/npy.h:LoadArrayFromNumpy is used) to float32 or int32. */
+nwprt_status nwprt_npy_read_f32(const char *path, float **data, int64_t *shape,
+                                int32_t *ndim, int32_t *fortran_order);
+nwprt_status nwprt_npy_read_i32(const char *path, int32_t **data,
+                                int64_t *shape, int32_t *ndim,
+                                int32_t *fortran_order);
+nwprt_status nwprt_npy_write_f32(const char *path, const float *data,
+                                 const int64_t *shape, int32_t ndim);
+nwprt_status nwprt_npy_write_i32(const char *path, const int32_t *data,
+                                 const int64_t *shape, int32_t ndim);
+/* Radiance RGBE reader (Renderer/HDRloader.cpp:30-118): rows returned in
+ * file order (top first), RGB float32. */
+nwprt_status nwprt_hdr_read(const char *path, float **rgb, int32_t *width,
+                            int32_t *height);
+/* Radiance RGBE writer (flat, run-length free scanlines), rows top first. */
+nwprt_status nwprt_hdr_write(const char *path, const float *rgb,
+                             int32_t width, int32_t height,
+                             int32_t channels);
+/* 4N x 3N cubemap cross (rows top first, as read) -> faces [6][N][N][3] with
+ * face order +X,-X,+Y,-Y,-Z,+Z and rows bottom-up
+ * (Renderer/EnvironmentMap.cpp:112-126, Renderer/HDRloader.h:21-25,
+ * Renderer/textures.cpp:72-83). */
+nwprt_status nwprt_cubemap_cross_to_faces(const float *cross, int32_t width,
+                                          int32_t height, float *faces);
+
+/* ---------------------------------------------------------------- scene -- */
+
+/* Upload a scene (hash grid, wavelet feature table, projection, MLP).
+ * Replaces RenderEngine::initTCNN's upload half (tcnn.cu:67-74). */
+nwprt_status nwprt_scene_create(nwprt_ctx ctx, const nwprt_scene_desc *desc,
+                                const nwprt_scene_params *params,
+                                int32_t *scene_id);
+/* RenderEngine::initTCNN(modelPath), Renderer/tinycudann/tcnn.cu:49-81:
+ * reads config_comp.json, vtx_enc.npy, out_prd.npy, cmp_net.npy, wve_enc.npy. */
+nwprt_status nwprt_scene_load_folder(nwprt_ctx ctx, const char *model_dir,
+                                     int32_t *scene_id);
+nwprt_status nwprt_scene_destroy(nwprt_ctx ctx, int32_t scene_id);
+nwprt_status nwprt_scene_get_desc(nwprt_ctx ctx, int32_t scene_id,
+                                  nwprt_scene_desc *desc);
+
+/* ---------------------------------------------------- environment maps -- */
+
+/* Cubemap faces [6][N][N][3] FP32 -> GPU non-standard Haar transform (FP64
+ * scaling-coefficient reduction) -> area-weighted top-K (ties: lower index).
+ * Replaces the offline producer of waveletcoeffs/ and waveletstrength/
+ * consumed at Renderer/EnvironmentMap.cpp:56-67. */
+nwprt_status nwprt_envmap_from_faces(nwprt_ctx ctx, const float *faces,
+                                     nwprt_location faces_location,
+                                     int32_t face_size, int32_t k,
+                                     int32_t *env_id);
+/* Recompute an existing map (built from faces, same N and K) from new faces:
+ * Haar + top-K into its buffers on `stream`, no allocation.  For requests
+ * that carry a rotated or fresh cubemap. */
+nwprt_status nwprt_envmap_update_faces(nwprt_ctx ctx, int32_t env_id,
+                                       const float *faces,
+                                       nwprt_location faces_location,
+                                       void *stream);
+/* Precomputed selection: indices int32[K] into [0, 6N^2) and strengths
+ * float32[K][3] (Renderer/EnvironmentMap.cpp:56-67 +
+ * RenderEngine::buildLights wavelet upload, Renderer/RenderEngine.cpp:167-175). */
+nwprt_status nwprt_envmap_from_selection(nwprt_ctx ctx,
+                                         const int32_t *indices,
+                                         const float *strengths, int32_t k,
+                                         int32_t face_size, int32_t *env_id);
+/* Environment_Map(folder, name) (Renderer/EnvironmentMap.cpp:20-68):
+ * cube/<name>.hdr is loaded; waveletcoeffs/<name>.npy + waveletstrength/
+ * <name>.npy are used when both exist, otherwise the selection is computed
+ * on the GPU with `k` wavelets. */
+nwprt_status nwprt_envmap_load_folder(nwprt_ctx ctx, const char *folder,
+                                      const char *name, int32_t k,
+                                      int32_t *env_id);
+nwprt_status nwprt_envmap_destroy(nwprt_ctx ctx, int32_t env_id);
+nwprt_status nwprt_envmap_info(nwprt_ctx ctx, int32_t env_id,
+                               int32_t *face_size, int32_t *k);
+/* Debug taps (synchronous): the selection (host int32[K], float[K][3]) and,
+ * for maps built from faces, the FP32 Haar coefficients (host [6N^2][3]). */
+nwprt_status nwprt_envmap_get_selection(nwprt_ctx ctx, int32_t env_id,
+                                        int32_t *indices, float *strengths);
+nwprt_status nwprt_envmap_get_coefficients(nwprt_ctx ctx, int32_t env_id,
+                                           float *coefficients);
+
+/* -------------------------------------------------------------- relight -- */
+
+/* Whole relight of one frame: per-pixel encoding, per-pair MLP, clamped
+ * compensated K-sum, outputs and tone map.  Replaces TCNNPass_p1 +
+ * TCNNPass_p2 (+ computeFinalPixelColors when out->rgba8 is set),
+ * Renderer/tinycudann/tcnn.cu:83-152, Renderer/toneMap.cu:73-99.
+ * G-buffer and outputs are device pointers. */
+nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
+                           const nwprt_gbuffer *gbuffer, int32_t width,
+                           int32_t height, const nwprt_frame_out *out,
+                           nwprt_precision precision,
+                           nwprt_tonemap_source tonemap, void *stream);
+
+/* Split form for a two-frame pipeline (RenderEngine::TCNNPass_p1 /
+ * TCNNPass_p2, tcnn.cu:83-152; OutputVisualizer.cpp:61-126).  p1 encodes the
+ * environment-independent per-pixel features of `frame_slot` (0 or 1); p2
+ * runs the per-pair network for an environment map on that slot.  p2 on a
+ * slot not encoded at this size/scene/precision returns NWPRT_ERR_STATE
+ * (the reference asserts, tcnn.cu:126). */
+nwprt_status nwprt_pass_p1(nwprt_ctx ctx, int32_t scene_id, int32_t frame_slot,
+                           const nwprt_gbuffer *gbuffer, int32_t width,
+                           int32_t height, nwprt_precision precision,
+                           void *stream);
+nwprt_status nwprt_pass_p2(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
+                           int32_t frame_slot, const nwprt_gbuffer *gbuffer,
+                           int32_t width, int32_t height,
+                           const nwprt_frame_out *out,
+                           nwprt_precision precision,
+                           nwprt_tonemap_source tonemap, void *stream);
+
+/* RenderEngine::computeFinalPixelColors kernel (Renderer/toneMap.cu:47-71):
+ * float4 -> RGBA8, device pointers. */
+nwprt_status nwprt_tonemap(nwprt_ctx ctx, const float *src_float4,
+                           uint32_t *dst_rgba8, int32_t width, int32_t height,
+                           void *stream);
+
+/* ------------------------------------------------------------ debug taps -- */
+
+/* Hash-grid tap: for n positions (device float4), write the 8 corner entry
+ * indices per level (device uint32[n][L][8], index within the level) and
+ * the interpolated features (device float[n][L*F]).  Either output may be
+ * NULL. */
+nwprt_status nwprt_debug_hash_encode(nwprt_ctx ctx, int32_t scene_id,
+                                     const float *positions, int32_t n,
+                                     uint32_t *indices, float *features,
+                                     void *stream);
+/* Transport tap: per-pair network output T (after output activation) for
+ * pixels [pixel_begin, pixel_begin + n_pixels) of a frame: device
+ * float[n_pixels][K][3].  Uses the same kernels as nwprt_relight. */
+nwprt_status nwprt_debug_transport(nwprt_ctx ctx, int32_t scene_id,
+                                   int32_t env_id, const nwprt_gbuffer *gbuffer,
+                                   int32_t width, int32_t height,
+                                   int32_t pixel_begin, int32_t n_pixels,
+                                   float *transport, nwprt_precision precision,
+                                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NWPRT_H */

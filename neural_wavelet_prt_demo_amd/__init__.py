@@ -1,0 +1,509 @@
+"""MI355X (gfx950) neural wavelet PRT relighting path.
+
+Python binding of ``libnwprt.so`` (C ABI declared in ``include/nwprt.h``)
+over ctypes.  The compute path is the HIP library; this module only moves
+numpy arrays in and out.  If the library is missing, every entry point
+raises -- there is no Python or CPU fallback.
+
+Names mirror the reference renderer where one exists: ``initTCNN`` loads a
+weight folder (Renderer/tinycudann/tcnn.cu:49), ``TCNNPass_p1`` /
+``TCNNPass_p2`` are the per-pixel and per-pair passes (tcnn.cu:83, :111),
+``computeFinalPixelColors`` is the tone map (Renderer/toneMap.cu:73).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import (POINTER, Structure, c_char_p, c_float, c_int, c_int32, c_int64, c_size_t,
+                    c_uint8, c_uint32, c_uint64, c_void_p)
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _hip
+from ._hip import DeviceBuffer, Event
+
+LIB_PATH = Path(__file__).resolve().with_name("libnwprt.so")
+
+MAX_K = 1024
+MAX_FACE_SIZE = 128
+MAX_LEVELS = 32
+
+# nwprt_status
+OK, ERR_INVALID_ARGUMENT, ERR_IO, ERR_FORMAT, ERR_DEVICE, ERR_OUT_OF_MEMORY, \
+    ERR_UNSUPPORTED, ERR_STATE, ERR_NOT_FOUND = range(9)
+# nwprt_precision
+PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP8 = range(4)
+PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16, "fp8": PREC_FP8}
+# nwprt_dtype
+DTYPE_F32, DTYPE_F16 = 0, 1
+# nwprt_tonemap_source
+TONEMAP_FULL, TONEMAP_INDIRECT, TONEMAP_NONE = 0, 1, 2
+HOST, DEVICE = 0, 1
+
+
+class NwprtError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"nwprt status {status}: {message}")
+        self.status = status
+
+
+class HashgridDesc(Structure):
+    _fields_ = [("n_levels", c_int32), ("n_features_per_level", c_int32),
+                ("log2_hashmap_size", c_int32), ("base_resolution", c_int32),
+                ("per_level_scale", c_float)]
+
+
+class HashgridLevels(Structure):
+    _fields_ = [("n_levels", c_int32), ("scale", c_float * MAX_LEVELS),
+                ("resolution", c_uint32 * MAX_LEVELS), ("dense", c_uint32 * MAX_LEVELS),
+                ("size", c_uint32 * MAX_LEVELS), ("offset", c_uint32 * (MAX_LEVELS + 1))]
+
+
+class SceneDesc(Structure):
+    _fields_ = [("grid", HashgridDesc), ("hash_dtype", c_int32), ("face_size", c_int32),
+                ("feature_dim", c_int32), ("hidden_width", c_int32),
+                ("n_hidden_layers", c_int32), ("has_projection", c_int32),
+                ("output_softplus", c_int32), ("aux_dims", c_int32),
+                ("aux_pad_value", c_float)]
+
+
+class SceneParams(Structure):
+    _fields_ = [("hash_table", c_void_p), ("wavelet_features", c_void_p),
+                ("projection", c_void_p), ("layers", c_void_p)]
+
+
+class GBuffer(Structure):
+    _fields_ = [("position", c_void_p), ("normal", c_void_p), ("reflect", c_void_p),
+                ("roughness", c_void_p), ("kd", c_void_p), ("ks", c_void_p),
+                ("direct", c_void_p), ("env_flag", c_void_p)]
+
+
+class FrameOut(Structure):
+    _fields_ = [("indirect", c_void_p), ("indirect_with_direct", c_void_p),
+                ("rgba8", c_void_p)]
+
+
+GBUFFER_FIELDS = {
+    "position": (np.float32, 4), "normal": (np.float32, 4), "reflect": (np.float32, 4),
+    "roughness": (np.float32, 1), "kd": (np.float32, 4), "ks": (np.float32, 4),
+    "direct": (np.float32, 4), "env_flag": (np.uint8, 1),
+}
+
+# Every symbol include/nwprt.h declares, with (restype, argtypes).
+_SIGNATURES = {
+    "nwprt_abi_version": (c_int32, []),
+    "nwprt_last_error": (c_char_p, []),
+    "nwprt_status_string": (c_char_p, [c_int]),
+    "nwprt_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "nwprt_destroy": (c_int, [c_void_p]),
+    "nwprt_sync": (c_int, [c_void_p]),
+    "nwprt_get_stream": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "nwprt_hashgrid_level_table": (c_int, [POINTER(HashgridDesc), POINTER(HashgridLevels)]),
+    "nwprt_scene_param_counts": (c_int, [POINTER(SceneDesc), POINTER(c_size_t), POINTER(c_size_t),
+                                         POINTER(c_size_t), POINTER(c_size_t)]),
+    "nwprt_synth_scene": (c_int, [POINTER(SceneDesc), c_uint64, c_float, c_float, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]),
+    "nwprt_synth_gbuffer": (c_int, [c_uint64, c_int32, c_int32, c_float, POINTER(GBuffer)]),
+    "nwprt_synth_cubemap": (c_int, [c_uint64, c_int32, c_void_p]),
+    "nwprt_free_host": (None, [c_void_p]),
+    "nwprt_npy_read_f32": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64),
+                                   POINTER(c_int32), POINTER(c_int32)]),
+    "nwprt_npy_read_i32": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64),
+                                   POINTER(c_int32), POINTER(c_int32)]),
+    "nwprt_npy_write_f32": (c_int, [c_char_p, c_void_p, POINTER(c_int64), c_int32]),
+    "nwprt_npy_write_i32": (c_int, [c_char_p, c_void_p, POINTER(c_int64), c_int32]),
+    "nwprt_hdr_read": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int32), POINTER(c_int32)]),
+    "nwprt_hdr_write": (c_int, [c_char_p, c_void_p, c_int32, c_int32, c_int32]),
+    "nwprt_cubemap_cross_to_faces": (c_int, [c_void_p, c_int32, c_int32, c_void_p]),
+    "nwprt_scene_create": (c_int, [c_void_p, POINTER(SceneDesc), POINTER(SceneParams),
+                                   POINTER(c_int32)]),
+    "nwprt_scene_load_folder": (c_int, [c_void_p, c_char_p, POINTER(c_int32)]),
+    "nwprt_scene_destroy": (c_int, [c_void_p, c_int32]),
+    "nwprt_scene_get_desc": (c_int, [c_void_p, c_int32, POINTER(SceneDesc)]),
+    "nwprt_envmap_from_faces": (c_int, [c_void_p, c_void_p, c_int, c_int32, c_int32,
+                                        POINTER(c_int32)]),
+    "nwprt_envmap_update_faces": (c_int, [c_void_p, c_int32, c_void_p, c_int, c_void_p]),
+    "nwprt_envmap_from_selection": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+                                            POINTER(c_int32)]),
+    "nwprt_envmap_load_folder": (c_int, [c_void_p, c_char_p, c_char_p, c_int32,
+                                         POINTER(c_int32)]),
+    "nwprt_envmap_destroy": (c_int, [c_void_p, c_int32]),
+    "nwprt_envmap_info": (c_int, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32)]),
+    "nwprt_envmap_get_selection": (c_int, [c_void_p, c_int32, c_void_p, c_void_p]),
+    "nwprt_envmap_get_coefficients": (c_int, [c_void_p, c_int32, c_void_p]),
+    "nwprt_relight": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32, c_int32,
+                              POINTER(FrameOut), c_int, c_int, c_void_p]),
+    "nwprt_pass_p1": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32, c_int32,
+                              c_int, c_void_p]),
+    "nwprt_pass_p2": (c_int, [c_void_p, c_int32, c_int32, c_int32, POINTER(GBuffer), c_int32,
+                              c_int32, POINTER(FrameOut), c_int, c_int, c_void_p]),
+    "nwprt_tonemap": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
+    "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
+                                        c_void_p]),
+    "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
+                                      c_int32, c_int32, c_int32, c_void_p, c_int, c_void_p]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libnwprt.so (raises if it was not built: no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `python -m neural_wavelet_prt_demo_amd.build`")
+        l = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != OK:
+        msg = lib().nwprt_last_error()
+        raise NwprtError(status, msg.decode() if msg else "")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+# ------------------------------------------------------------- host side --
+
+def config_scene_desc(face_size: int, hash_dtype: int = DTYPE_F16, *, levels: int = 16,
+                      features: int = 2, log2_hashmap_size: int = 19, base_resolution: int = 16,
+                      per_level_scale: float = 1.5, projection: bool = True,
+                      softplus: bool = True) -> SceneDesc:
+    """The network of the relighting workloads: 16 x 2 hash grid (2^19
+    entries per level, N_min 16, growth 1.5), 32 x 32 projection, MLP
+    32 -> 64 -> 64 -> 64 -> 3 with ReLU and a softplus output."""
+    d = SceneDesc()
+    d.grid = HashgridDesc(levels, features, log2_hashmap_size, base_resolution, per_level_scale)
+    d.hash_dtype = hash_dtype
+    d.face_size = face_size
+    d.feature_dim = levels * features
+    d.hidden_width = 64
+    d.n_hidden_layers = 3
+    d.has_projection = int(projection)
+    d.output_softplus = int(softplus)
+    d.aux_dims = 0
+    d.aux_pad_value = 1.0
+    return d
+
+
+def hashgrid_levels(grid: HashgridDesc) -> Dict[str, np.ndarray]:
+    lv = HashgridLevels()
+    _check(lib().nwprt_hashgrid_level_table(ctypes.byref(grid), ctypes.byref(lv)))
+    n = lv.n_levels
+    return {"scale": np.array(lv.scale[:n], np.float32),
+            "resolution": np.array(lv.resolution[:n], np.uint32),
+            "dense": np.array(lv.dense[:n], np.uint32),
+            "size": np.array(lv.size[:n], np.uint32),
+            "offset": np.array(lv.offset[:n + 1], np.uint32)}
+
+
+def scene_param_counts(desc: SceneDesc):
+    h, w, p, l = c_size_t(), c_size_t(), c_size_t(), c_size_t()
+    _check(lib().nwprt_scene_param_counts(ctypes.byref(desc), ctypes.byref(h), ctypes.byref(w),
+                                          ctypes.byref(p), ctypes.byref(l)))
+    return h.value, w.value, p.value, l.value
+
+
+def synth_scene(desc: SceneDesc, seed: int, hash_std: float = 0.1,
+                feature_std: float = 0.1) -> Dict[str, np.ndarray]:
+    """Seeded scene parameters (host arrays)."""
+    entries, nwf, nproj, nlay = scene_param_counts(desc)
+    F = desc.grid.n_features_per_level
+    hdt = np.uint16 if desc.hash_dtype == DTYPE_F16 else np.float32
+    out = {"hash_table": np.empty((entries, F), hdt),
+           "wavelet_features": np.empty((desc.feature_dim, nwf // desc.feature_dim), np.float32),
+           "projection": np.empty((desc.feature_dim, desc.feature_dim), np.float32)
+           if nproj else None,
+           "layers": np.empty(nlay, np.float32)}
+    _check(lib().nwprt_synth_scene(ctypes.byref(desc), seed, hash_std, feature_std,
+                                   _ptr(out["hash_table"]), _ptr(out["wavelet_features"]),
+                                   _ptr(out["projection"]), _ptr(out["layers"])))
+    return out
+
+
+def synth_gbuffer(seed: int, width: int, height: int,
+                  background_fraction: float = 0.0) -> Dict[str, np.ndarray]:
+    """Seeded G-buffer (host arrays in the reference layout)."""
+    P = width * height
+    g = {k: np.empty((P, c) if c > 1 else (P,), dt) for k, (dt, c) in GBUFFER_FIELDS.items()}
+    gb = GBuffer(**{k: _ptr(v) for k, v in g.items()})
+    _check(lib().nwprt_synth_gbuffer(seed, width, height, background_fraction, ctypes.byref(gb)))
+    return g
+
+
+def synth_cubemap(seed: int, face_size: int) -> np.ndarray:
+    faces = np.empty((6, face_size, face_size, 3), np.float32)
+    _check(lib().nwprt_synth_cubemap(seed, face_size, _ptr(faces)))
+    return faces
+
+
+def _take_host(ptr: c_void_p, count: int, dtype) -> np.ndarray:
+    try:
+        buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(ptr.value)
+        return np.frombuffer(bytes(buf), dtype=dtype).copy()
+    finally:
+        lib().nwprt_free_host(ptr)
+
+
+def npy_read(path: str, dtype=np.float32):
+    data = c_void_p()
+    shape = (c_int64 * 8)()
+    nd, fo = c_int32(), c_int32()
+    fn = lib().nwprt_npy_read_f32 if dtype == np.float32 else lib().nwprt_npy_read_i32
+    _check(fn(str(path).encode(), ctypes.byref(data), shape, ctypes.byref(nd), ctypes.byref(fo)))
+    shp = tuple(shape[i] for i in range(nd.value))
+    arr = _take_host(data, int(np.prod(shp)) if shp else 1, dtype)
+    return arr, shp, bool(fo.value)
+
+
+def npy_write(path: str, a: np.ndarray) -> None:
+    a = np.ascontiguousarray(a)
+    shape = (c_int64 * max(1, a.ndim))(*a.shape)
+    if a.dtype == np.float32:
+        _check(lib().nwprt_npy_write_f32(str(path).encode(), _ptr(a), shape, a.ndim))
+    elif a.dtype == np.int32:
+        _check(lib().nwprt_npy_write_i32(str(path).encode(), _ptr(a), shape, a.ndim))
+    else:
+        raise TypeError("npy_write supports float32 and int32")
+
+
+def hdr_read(path: str) -> np.ndarray:
+    data = c_void_p()
+    w, h = c_int32(), c_int32()
+    _check(lib().nwprt_hdr_read(str(path).encode(), ctypes.byref(data), ctypes.byref(w),
+                                ctypes.byref(h)))
+    return _take_host(data, w.value * h.value * 3, np.float32).reshape(h.value, w.value, 3)
+
+
+def hdr_write(path: str, img: np.ndarray) -> None:
+    img = np.ascontiguousarray(img, np.float32)
+    _check(lib().nwprt_hdr_write(str(path).encode(), _ptr(img), img.shape[1], img.shape[0],
+                                 img.shape[2]))
+
+
+def cross_to_faces(cross: np.ndarray) -> np.ndarray:
+    cross = np.ascontiguousarray(cross, np.float32)
+    h, w = cross.shape[:2]
+    faces = np.empty((6, h // 3, h // 3, 3), np.float32)
+    _check(lib().nwprt_cubemap_cross_to_faces(_ptr(cross), w, h, _ptr(faces)))
+    return faces
+
+
+# ------------------------------------------------------------ device side --
+
+class DeviceFrame:
+    """G-buffer and outputs of one frame in device memory (reference layout,
+    Renderer/frames.h:143-183)."""
+
+    def __init__(self, width: int, height: int, host: Optional[Dict[str, np.ndarray]] = None):
+        self.width, self.height = width, height
+        P = width * height
+        self.bufs = {k: DeviceBuffer(P * np.dtype(dt).itemsize * c)
+                     for k, (dt, c) in GBUFFER_FIELDS.items()}
+        self.out = {"indirect": DeviceBuffer(P * 16), "indirect_with_direct": DeviceBuffer(P * 16),
+                    "rgba8": DeviceBuffer(P * 4)}
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host: Dict[str, np.ndarray]) -> None:
+        for k, v in host.items():
+            self.bufs[k].upload(v)
+
+    def gbuffer(self) -> GBuffer:
+        return GBuffer(**{k: b.ptr for k, b in self.bufs.items()})
+
+    def frame_out(self, indirect=True, with_direct=True, rgba8=True) -> FrameOut:
+        return FrameOut(self.out["indirect"].ptr if indirect else None,
+                        self.out["indirect_with_direct"].ptr if with_direct else None,
+                        self.out["rgba8"].ptr if rgba8 else None)
+
+    def results(self) -> Dict[str, np.ndarray]:
+        P = self.width * self.height
+        return {"indirect": self.out["indirect"].download(np.float32, (P, 4)),
+                "indirect_with_direct": self.out["indirect_with_direct"].download(np.float32, (P, 4)),
+                "rgba8": self.out["rgba8"].download(np.uint32, (P,))}
+
+
+class Context:
+    """One nwprt context (device, stream, resident scenes and env maps)."""
+
+    def __init__(self, device: int = 0):
+        self.h = c_void_p()
+        _check(lib().nwprt_create(device, ctypes.byref(self.h)))
+
+    def close(self) -> None:
+        if self.h:
+            lib().nwprt_destroy(self.h)
+            self.h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        s = c_void_p()
+        _check(lib().nwprt_get_stream(self.h, ctypes.byref(s)))
+        return s.value or 0
+
+    def sync(self) -> None:
+        _check(lib().nwprt_sync(self.h))
+
+    # scenes -------------------------------------------------------------
+    def scene_create(self, desc: SceneDesc, params: Dict[str, np.ndarray]) -> int:
+        keep = {k: (None if v is None else np.ascontiguousarray(v)) for k, v in params.items()}
+        p = SceneParams(_ptr(keep["hash_table"]), _ptr(keep["wavelet_features"]),
+                        _ptr(keep.get("projection")), _ptr(keep["layers"]))
+        sid = c_int32()
+        _check(lib().nwprt_scene_create(self.h, ctypes.byref(desc), ctypes.byref(p),
+                                        ctypes.byref(sid)))
+        return sid.value
+
+    def initTCNN(self, model_dir: str) -> int:
+        """RenderEngine::initTCNN (Renderer/tinycudann/tcnn.cu:49-81)."""
+        sid = c_int32()
+        _check(lib().nwprt_scene_load_folder(self.h, str(model_dir).encode(), ctypes.byref(sid)))
+        return sid.value
+
+    def scene_desc(self, scene: int) -> SceneDesc:
+        d = SceneDesc()
+        _check(lib().nwprt_scene_get_desc(self.h, scene, ctypes.byref(d)))
+        return d
+
+    def scene_destroy(self, scene: int) -> None:
+        _check(lib().nwprt_scene_destroy(self.h, scene))
+
+    # environment maps ----------------------------------------------------
+    def envmap_from_faces(self, faces, k: int, face_size: Optional[int] = None) -> int:
+        eid = c_int32()
+        if isinstance(faces, DeviceBuffer):
+            _check(lib().nwprt_envmap_from_faces(self.h, faces.ptr, DEVICE, face_size, k,
+                                                 ctypes.byref(eid)))
+        else:
+            faces = np.ascontiguousarray(faces, np.float32)
+            _check(lib().nwprt_envmap_from_faces(self.h, _ptr(faces), HOST, faces.shape[1], k,
+                                                 ctypes.byref(eid)))
+        return eid.value
+
+    def envmap_update_faces(self, env: int, faces, stream: Optional[int] = None) -> None:
+        """Recompute a faces-built map from new faces in place (no allocation)."""
+        s = c_void_p(stream) if stream else None
+        if isinstance(faces, DeviceBuffer):
+            _check(lib().nwprt_envmap_update_faces(self.h, env, faces.ptr, DEVICE, s))
+        else:
+            faces = np.ascontiguousarray(faces, np.float32)
+            _check(lib().nwprt_envmap_update_faces(self.h, env, _ptr(faces), HOST, s))
+
+    def envmap_from_selection(self, indices: np.ndarray, strengths: np.ndarray,
+                              face_size: int) -> int:
+        idx = np.ascontiguousarray(indices, np.int32)
+        st = np.ascontiguousarray(strengths, np.float32)
+        eid = c_int32()
+        _check(lib().nwprt_envmap_from_selection(self.h, _ptr(idx), _ptr(st), len(idx), face_size,
+                                                 ctypes.byref(eid)))
+        return eid.value
+
+    def envmap_load_folder(self, folder: str, name: str, k: int) -> int:
+        """Environment_Map(folder, name) (Renderer/EnvironmentMap.cpp:20-68)."""
+        eid = c_int32()
+        _check(lib().nwprt_envmap_load_folder(self.h, str(folder).encode(), str(name).encode(), k,
+                                              ctypes.byref(eid)))
+        return eid.value
+
+    def envmap_info(self, env: int):
+        n, k = c_int32(), c_int32()
+        _check(lib().nwprt_envmap_info(self.h, env, ctypes.byref(n), ctypes.byref(k)))
+        return n.value, k.value
+
+    def envmap_selection(self, env: int):
+        n, k = self.envmap_info(env)
+        idx = np.empty(k, np.int32)
+        st = np.empty((k, 3), np.float32)
+        _check(lib().nwprt_envmap_get_selection(self.h, env, _ptr(idx), _ptr(st)))
+        return idx, st
+
+    def envmap_coefficients(self, env: int) -> np.ndarray:
+        n, _ = self.envmap_info(env)
+        c = np.empty((6, n, n, 3), np.float32)
+        _check(lib().nwprt_envmap_get_coefficients(self.h, env, _ptr(c)))
+        return c
+
+    def envmap_destroy(self, env: int) -> None:
+        _check(lib().nwprt_envmap_destroy(self.h, env))
+
+    # relight -------------------------------------------------------------
+    def relight(self, scene: int, env: int, frame: DeviceFrame, precision: int = PREC_BF16,
+                tonemap: int = TONEMAP_FULL, stream: Optional[int] = None,
+                out: Optional[FrameOut] = None) -> None:
+        gb = frame.gbuffer()
+        fo = out if out is not None else frame.frame_out()
+        _check(lib().nwprt_relight(self.h, scene, env, ctypes.byref(gb), frame.width,
+                                   frame.height, ctypes.byref(fo), precision, tonemap,
+                                   c_void_p(stream) if stream else None))
+
+    def TCNNPass_p1(self, scene: int, slot: int, frame: DeviceFrame,
+                    precision: int = PREC_BF16, stream: Optional[int] = None) -> None:
+        gb = frame.gbuffer()
+        _check(lib().nwprt_pass_p1(self.h, scene, slot, ctypes.byref(gb), frame.width,
+                                   frame.height, precision, c_void_p(stream) if stream else None))
+
+    def TCNNPass_p2(self, scene: int, env: int, slot: int, frame: DeviceFrame,
+                    precision: int = PREC_BF16, tonemap: int = TONEMAP_FULL,
+                    stream: Optional[int] = None) -> None:
+        gb = frame.gbuffer()
+        fo = frame.frame_out()
+        _check(lib().nwprt_pass_p2(self.h, scene, env, slot, ctypes.byref(gb), frame.width,
+                                   frame.height, ctypes.byref(fo), precision, tonemap,
+                                   c_void_p(stream) if stream else None))
+
+    def computeFinalPixelColors(self, src: DeviceBuffer, dst: DeviceBuffer, width: int,
+                                height: int, stream: Optional[int] = None) -> None:
+        _check(lib().nwprt_tonemap(self.h, src.ptr, dst.ptr, width, height,
+                                   c_void_p(stream) if stream else None))
+
+    # debug taps ----------------------------------------------------------
+    def debug_hash_encode(self, scene: int, positions: np.ndarray):
+        """positions: [n, 4] float32 (host).  Returns (indices [n, L, 8],
+        features [n, 32]) as host arrays."""
+        pos = np.ascontiguousarray(positions, np.float32)
+        n = pos.shape[0]
+        L = self.scene_desc(scene).grid.n_levels
+        dpos = DeviceBuffer.from_array(pos)
+        didx = DeviceBuffer(n * L * 8 * 4)
+        dfeat = DeviceBuffer(n * 32 * 4)
+        _check(lib().nwprt_debug_hash_encode(self.h, scene, dpos.ptr, n, didx.ptr, dfeat.ptr, None))
+        self.sync()
+        return didx.download(np.uint32, (n, L, 8)), dfeat.download(np.float32, (n, 32))
+
+    def debug_transport(self, scene: int, env: int, frame: DeviceFrame, begin: int, count: int,
+                        precision: int = PREC_BF16) -> np.ndarray:
+        _, k = self.envmap_info(env)
+        dt = DeviceBuffer(count * k * 3 * 4)
+        dt.zero()
+        gb = frame.gbuffer()
+        _check(lib().nwprt_debug_transport(self.h, scene, env, ctypes.byref(gb), frame.width,
+                                           frame.height, begin, count, dt.ptr, precision, None))
+        self.sync()
+        return dt.download(np.float32, (count, k, 3))
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,195 @@
+// Per-pixel encoding for gfx950: multiresolution hash-grid features of the
+// first-hit position, then the optional 32 x 32 projection.  One lane per
+// pixel, computed once per pixel (the reference encodes every pixel K times,
+// once per replicated (pixel, wavelet) column: Renderer/tinycudann/
+// utils_kernels.h:25 and prt_network.h:86-91).
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace nwprt {
+namespace {
+
+constexpr int kEncodeBlock = 256;
+
+// Spatial hash of Instant-NGP (Mueller et al. 2022): x*1 ^ y*2654435761 ^
+// z*805459861 in uint32 arithmetic.
+__device__ __forceinline__ uint32_t grid_hash(uint32_t x, uint32_t y, uint32_t z) {
+  return x ^ (y * 2654435761u) ^ (z * 805459861u);
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
+// receives a contiguous run of pixel tiles, whose coarse-level hash entries
+// then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, local = bid >> 3;
+  return x * q + (x < r ? x : r) + local;
+}
+
+template <int F, bool HF16, Elem OUT, bool DBG>
+__global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
+  constexpr int L = 32 / F;
+  __shared__ float proj[32 * 32];
+  const bool has_proj = a.projection != nullptr;
+  if (has_proj)
+    for (int i = threadIdx.x; i < 32 * 32; i += kEncodeBlock) proj[i] = a.projection[i];
+  __syncthreads();
+
+  const int p = xcd_tile(blockIdx.x, gridDim.x) * kEncodeBlock + threadIdx.x;
+  if (p >= a.n_pixels) return;
+  const bool hit = a.env_flag == nullptr || a.env_flag[p] != 0;
+
+  float feat[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) feat[i] = 0.0f;
+
+  if (hit || DBG) {
+    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)p);
+    const float x[3] = {x4.x, x4.y, x4.z};
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const float scale = a.lv.scale[l];
+      const uint32_t res = a.lv.resolution[l], size = a.lv.size[l];
+      const bool dense = a.lv.dense[l] != 0;
+      const uint32_t off = a.lv.offset[l];
+      uint32_t gi[3];
+      float fr[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float pos = fmaf(x[d], scale, 0.5f);
+        const float fl = floorf(pos);
+        gi[d] = (uint32_t)(int)fl;
+        fr[d] = pos - fl;
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t cx = gi[0] + (c & 1), cy = gi[1] + ((c >> 1) & 1),
+                       cz = gi[2] + ((c >> 2) & 1);
+        uint32_t idx;
+        if (dense) {
+          idx = cx + cy * res + cz * res * res;
+          if (idx >= size) idx %= size;
+        } else {
+          idx = grid_hash(cx, cy, cz) & (size - 1u);
+        }
+        const float w = ((c & 1) ? fr[0] : 1.0f - fr[0]) * ((c & 2) ? fr[1] : 1.0f - fr[1]) *
+                        ((c & 4) ? fr[2] : 1.0f - fr[2]);
+        const size_t e = ((size_t)off + idx) * F;
+        if (HF16) {
+          const uint16_t *h = (const uint16_t *)a.hash + e;
+          if (F == 2) {
+            const uint32_t two = *(const uint32_t *)h;
+            feat[l * F + 0] = fmaf(w, dev::f16_to_f32((uint16_t)(two & 0xffffu)), feat[l * F + 0]);
+            feat[l * F + 1] = fmaf(w, dev::f16_to_f32((uint16_t)(two >> 16)), feat[l * F + 1]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < F; ++j) feat[l * F + j] = fmaf(w, dev::f16_to_f32(h[j]), feat[l * F + j]);
+          }
+        } else {
+          const float *h = (const float *)a.hash + e;
+          if (F == 2) {
+            const float2 two = *(const float2 *)h;
+            feat[l * F + 0] = fmaf(w, two.x, feat[l * F + 0]);
+            feat[l * F + 1] = fmaf(w, two.y, feat[l * F + 1]);
+          } else {
+#pragma unroll
+            for (int j = 0; j < F; ++j) feat[l * F + j] = fmaf(w, h[j], feat[l * F + j]);
+          }
+        }
+        if (DBG) a.dbg_indices[((size_t)p * L + l) * 8 + c] = idx;
+      }
+    }
+    if (DBG && a.dbg_features) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) a.dbg_features[(size_t)p * 32 + i] = feat[i];
+    }
+  }
+  if (!hit) {
+#pragma unroll
+    for (int i = 0; i < 32; ++i) feat[i] = 0.0f;
+  }
+
+  float out[32];
+  if (has_proj) {
+#pragma unroll
+    for (int o = 0; o < 32; ++o) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int i = 0; i < 32; ++i) acc = fmaf(proj[o * 32 + i], feat[i], acc);
+      out[o] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 32; ++i) out[i] = feat[i];
+  }
+
+  if (OUT == Elem::F32) {
+    float4 *dst = (float4 *)((float *)a.features + (size_t)p * 32);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      dst[q] = make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+  } else {
+    dev::u32x4 *dst = (dev::u32x4 *)((uint16_t *)a.features + (size_t)p * 32);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      dev::u32x4 v;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const float lo = out[8 * q + 2 * d], hi = out[8 * q + 2 * d + 1];
+        v[d] = OUT == Elem::BF16 ? dev::pack_bf16(lo, hi) : dev::pack_f16(lo, hi);
+      }
+      dst[q] = v;
+    }
+  }
+}
+
+template <int F, bool HF16, bool DBG>
+void dispatch_out(const EncodeArgs &a, dim3 grid, hipStream_t s) {
+  switch (a.out_elem) {
+    case Elem::F32:
+      hipLaunchKernelGGL((encode_kernel<F, HF16, Elem::F32, DBG>), grid, dim3(kEncodeBlock), 0, s, a);
+      break;
+    case Elem::F16:
+      hipLaunchKernelGGL((encode_kernel<F, HF16, Elem::F16, DBG>), grid, dim3(kEncodeBlock), 0, s, a);
+      break;
+    case Elem::BF16:
+      hipLaunchKernelGGL((encode_kernel<F, HF16, Elem::BF16, DBG>), grid, dim3(kEncodeBlock), 0, s, a);
+      break;
+  }
+}
+
+template <int F>
+void dispatch_f(const EncodeArgs &a, dim3 grid, hipStream_t s) {
+  const bool dbg = a.dbg_indices != nullptr;
+  if (a.hash_f16) {
+    if (dbg) dispatch_out<F, true, true>(a, grid, s);
+    else dispatch_out<F, true, false>(a, grid, s);
+  } else {
+    if (dbg) dispatch_out<F, false, true>(a, grid, s);
+    else dispatch_out<F, false, false>(a, grid, s);
+  }
+}
+
+}  // namespace
+
+void launch_encode(const EncodeArgs &a, hipStream_t s) {
+  require(a.n_pixels >= 1, NWPRT_ERR_INVALID_ARGUMENT, "encode: no pixels");
+  require(a.lv.n_levels * a.n_features == 32, NWPRT_ERR_UNSUPPORTED,
+          "encode: n_levels * n_features must be 32");
+  for (int l = 0; l < a.lv.n_levels; ++l)
+    require(a.lv.dense[l] || (a.lv.size[l] & (a.lv.size[l] - 1)) == 0, NWPRT_ERR_INVALID_ARGUMENT,
+            "encode: hashed level size must be a power of two");
+  const dim3 grid((a.n_pixels + kEncodeBlock - 1) / kEncodeBlock);
+  switch (a.n_features) {
+    case 1: dispatch_f<1>(a, grid, s); break;
+    case 2: dispatch_f<2>(a, grid, s); break;
+    case 4: dispatch_f<4>(a, grid, s); break;
+    case 8: dispatch_f<8>(a, grid, s); break;
+    default: fail(NWPRT_ERR_UNSUPPORTED, "encode: n_features must be 1, 2, 4 or 8");
+  }
+  hip_check(hipGetLastError(), "encode_kernel launch");
+}
+
+}  // namespace nwprt

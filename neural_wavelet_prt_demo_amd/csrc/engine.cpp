@@ -1,0 +1,680 @@
+// Context, scene / environment-map registries and the C ABI of libnwprt.
+//
+// Replaces the TCNN half of RenderEngine (Renderer/tinycudann/tcnn.cu:40-162,
+// Renderer/RenderEngine.cpp:162-175): instead of process-global network and
+// matrices, a context owns device-resident scenes (hash grid, wavelet
+// feature table, weights), environment maps (K indices + strengths) and
+// per-frame-slot work buffers that grow and are reused.
+#include <hip/hip_runtime.h>
+
+#include <fstream>
+#include <map>
+
+#include "kernels.hpp"
+
+namespace nwprt {
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    release();
+    const hipError_t e = hipMalloc(&p, n);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      fail(NWPRT_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    hip_check(e, "hipMalloc");
+    bytes = n;
+  }
+  void upload(const void *src, size_t n) {
+    ensure(n);
+    hip_check(hipMemcpy(p, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  }
+  template <class T>
+  T *as() const { return (T *)p; }
+};
+
+struct Scene {
+  nwprt_scene_desc desc{};
+  nwprt_hashgrid_levels lv{};
+  DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16;
+};
+
+struct EnvMap {
+  int n = 0, k = 0;
+  bool has_coef = false;
+  DevBuf idx, strength, coef;
+  hipEvent_t ready = nullptr;
+  ~EnvMap() {
+    if (ready) (void)hipEventDestroy(ready);
+  }
+};
+
+// Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
+// 2: nwprt_relight / debug taps).
+struct FrameSlot {
+  DevBuf features, mk, wsel;
+  bool valid = false;
+  int scene = -1, n_pixels = 0;
+  Elem elem = Elem::F32;
+};
+
+}  // namespace
+}  // namespace nwprt
+
+struct nwprt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<int, std::unique_ptr<nwprt::Scene>> scenes;
+  std::map<int, std::unique_ptr<nwprt::EnvMap>> envs;
+  int next_id = 1;
+  nwprt::FrameSlot slots[3];
+  nwprt::DevBuf staging, scores;
+};
+
+namespace nwprt {
+namespace {
+
+const char *status_name(nwprt_status s) {
+  switch (s) {
+    case NWPRT_OK: return "ok";
+    case NWPRT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case NWPRT_ERR_IO: return "i/o error";
+    case NWPRT_ERR_FORMAT: return "format error";
+    case NWPRT_ERR_DEVICE: return "device error";
+    case NWPRT_ERR_OUT_OF_MEMORY: return "out of memory";
+    case NWPRT_ERR_UNSUPPORTED: return "unsupported";
+    case NWPRT_ERR_STATE: return "invalid call order";
+    case NWPRT_ERR_NOT_FOUND: return "unknown handle";
+  }
+  return "unknown status";
+}
+
+template <class F>
+nwprt_status guarded(F &&f) {
+  try {
+    f();
+    set_last_error("");
+    return NWPRT_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return NWPRT_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return NWPRT_ERR_INVALID_ARGUMENT;
+  }
+}
+
+void bind(nwprt_ctx ctx) {
+  require(ctx != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null context");
+  hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+}
+
+hipStream_t pick(nwprt_ctx ctx, void *stream) {
+  return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+Scene &get_scene(nwprt_ctx ctx, int id) {
+  auto it = ctx->scenes.find(id);
+  require(it != ctx->scenes.end(), NWPRT_ERR_NOT_FOUND, "unknown scene id " + std::to_string(id));
+  return *it->second;
+}
+
+EnvMap &get_env(nwprt_ctx ctx, int id) {
+  auto it = ctx->envs.find(id);
+  require(it != ctx->envs.end(), NWPRT_ERR_NOT_FOUND,
+          "unknown environment map id " + std::to_string(id));
+  return *it->second;
+}
+
+void check_aligned(const void *p, size_t a, const char *what) {
+  require(((uintptr_t)p % a) == 0, NWPRT_ERR_INVALID_ARGUMENT,
+          std::string(what) + " must be " + std::to_string(a) + "-byte aligned");
+}
+
+Elem elem_of(nwprt_precision p) {
+  switch (p) {
+    case NWPRT_PREC_FP32: return Elem::F32;
+    case NWPRT_PREC_FP16: return Elem::F16;
+    case NWPRT_PREC_BF16: return Elem::BF16;
+    case NWPRT_PREC_FP8:
+      fail(NWPRT_ERR_UNSUPPORTED, "FP8 precision is not implemented in this build");
+  }
+  fail(NWPRT_ERR_INVALID_ARGUMENT, "unknown precision");
+}
+
+void check_frame(const nwprt_gbuffer *gb, int w, int h, bool need_direct) {
+  require(gb != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null G-buffer");
+  require(w > 0 && h > 0 && (long long)w * h <= (1ll << 28), NWPRT_ERR_INVALID_ARGUMENT,
+          "frame size must be positive and at most 2^28 pixels");
+  require(gb->position != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "G-buffer position is required");
+  check_aligned(gb->position, 16, "G-buffer position");
+  if (need_direct) {
+    require(gb->direct != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "G-buffer direct is required");
+    check_aligned(gb->direct, 16, "G-buffer direct");
+  }
+}
+
+void create_scene(nwprt_ctx ctx, const nwprt_scene_desc &d, const nwprt_scene_params &prm,
+                  int *scene_id) {
+  validate_scene_desc(d);
+  require(d.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
+          "scene: aux G-buffer inputs are not implemented by this build's kernels");
+  require(prm.hash_table && prm.wavelet_features && prm.layers && scene_id,
+          NWPRT_ERR_INVALID_ARGUMENT, "scene: missing parameter array");
+  require(!d.has_projection || prm.projection, NWPRT_ERR_INVALID_ARGUMENT,
+          "scene: has_projection set but projection is null");
+  auto s = std::make_unique<Scene>();
+  s->desc = d;
+  hashgrid_levels(d.grid, s->lv);
+  const size_t entries = s->lv.offset[s->lv.n_levels];
+  const size_t esz = d.hash_dtype == NWPRT_DTYPE_F16 ? 2 : 4;
+  s->hash.upload(prm.hash_table, entries * d.grid.n_features_per_level * esz);
+  s->wfeat.upload(prm.wavelet_features,
+                  (size_t)d.feature_dim * 6 * d.face_size * d.face_size * sizeof(float));
+  if (d.has_projection) s->proj.upload(prm.projection, 32 * 32 * sizeof(float));
+  const size_t nw = scene_layer_weight_count(d);
+  s->w_f32.upload(prm.layers, nw * sizeof(float));
+  std::vector<uint16_t> tiles;
+  build_weight_tiles(prm.layers, Elem::BF16, tiles);
+  s->tiles_bf16.upload(tiles.data(), tiles.size() * 2);
+  build_weight_tiles(prm.layers, Elem::F16, tiles);
+  s->tiles_f16.upload(tiles.data(), tiles.size() * 2);
+  const int id = ctx->next_id++;
+  ctx->scenes[id] = std::move(s);
+  *scene_id = id;
+}
+
+EnvMap &new_env(nwprt_ctx ctx, int n, int k, int *env_id) {
+  auto e = std::make_unique<EnvMap>();
+  e->n = n;
+  e->k = k;
+  e->idx.ensure((size_t)k * 4);
+  e->strength.ensure((size_t)k * 12);
+  hip_check(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming), "hipEventCreate");
+  const int id = ctx->next_id++;
+  EnvMap &ref = *e;
+  ctx->envs[id] = std::move(e);
+  *env_id = id;
+  return ref;
+}
+
+void env_from_faces(nwprt_ctx ctx, const float *faces, nwprt_location loc, int n, int k,
+                    int *env_id) {
+  require(faces && env_id, NWPRT_ERR_INVALID_ARGUMENT, "envmap: null argument");
+  require(n >= 2 && n <= NWPRT_MAX_FACE_SIZE && (n & (n - 1)) == 0, NWPRT_ERR_INVALID_ARGUMENT,
+          "envmap: face size must be a power of two in [2, 128]");
+  require(k >= 1 && k <= NWPRT_MAX_K && k <= 6 * n * n, NWPRT_ERR_INVALID_ARGUMENT,
+          "envmap: K must be in [1, min(1024, 6 N^2)]");
+  const size_t m = 6ull * n * n;
+  const float *dfaces = faces;
+  if (loc == NWPRT_HOST) {
+    ctx->staging.ensure(m * 12);
+    hip_check(hipMemcpyAsync(ctx->staging.p, faces, m * 12, hipMemcpyHostToDevice, ctx->stream),
+              "hipMemcpyAsync faces");
+    dfaces = ctx->staging.as<float>();
+  } else {
+    require(loc == NWPRT_DEVICE, NWPRT_ERR_INVALID_ARGUMENT, "envmap: bad faces location");
+  }
+  int id = 0;
+  EnvMap &e = new_env(ctx, n, k, &id);
+  try {
+    e.coef.ensure(m * 12);
+    e.has_coef = true;
+    ctx->scores.ensure(m * 4);
+    launch_haar(dfaces, m * 3, e.coef.as<float>(), n, 1, ctx->stream);
+    launch_topk(e.coef.as<float>(), n, k, ctx->scores.as<uint32_t>(), e.idx.as<int32_t>(),
+                e.strength.as<float>(), ctx->stream);
+    hip_check(hipEventRecord(e.ready, ctx->stream), "hipEventRecord");
+    if (loc == NWPRT_HOST)  // the staging buffer is reused by the next call
+      hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  } catch (...) {
+    ctx->envs.erase(id);
+    throw;
+  }
+  *env_id = id;
+}
+
+void env_from_selection(nwprt_ctx ctx, const int32_t *idx, const float *str, int k, int n,
+                        int *env_id) {
+  require(idx && str && env_id, NWPRT_ERR_INVALID_ARGUMENT, "envmap: null argument");
+  require(n >= 2 && n <= NWPRT_MAX_FACE_SIZE && (n & (n - 1)) == 0, NWPRT_ERR_INVALID_ARGUMENT,
+          "envmap: face size must be a power of two in [2, 128]");
+  require(k >= 1 && k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT, "envmap: K must be in [1, 1024]");
+  for (int i = 0; i < k; ++i)
+    require(idx[i] >= 0 && idx[i] < 6 * n * n, NWPRT_ERR_INVALID_ARGUMENT,
+            "envmap: wavelet index " + std::to_string(idx[i]) + " outside [0, 6 N^2)");
+  int id = 0;
+  EnvMap &e = new_env(ctx, n, k, &id);
+  try {
+    hip_check(hipMemcpy(e.idx.p, idx, (size_t)k * 4, hipMemcpyHostToDevice), "hipMemcpy idx");
+    hip_check(hipMemcpy(e.strength.p, str, (size_t)k * 12, hipMemcpyHostToDevice),
+              "hipMemcpy strength");
+    hip_check(hipEventRecord(e.ready, ctx->stream), "hipEventRecord");
+  } catch (...) {
+    ctx->envs.erase(id);
+    throw;
+  }
+  *env_id = id;
+}
+
+void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const nwprt_gbuffer *gb,
+                 int w, int h, Elem elem, hipStream_t s, uint32_t *dbg_idx = nullptr,
+                 float *dbg_feat = nullptr) {
+  const int P = w * h;
+  slot.valid = false;
+  slot.features.ensure((size_t)P * 32 * (elem == Elem::F32 ? 4 : 2));
+  EncodeArgs a{};
+  a.n_pixels = P;
+  a.position = gb->position;
+  a.env_flag = gb->env_flag;
+  a.hash = sc.hash.p;
+  a.hash_f16 = sc.desc.hash_dtype == NWPRT_DTYPE_F16;
+  a.n_features = sc.desc.grid.n_features_per_level;
+  a.lv = sc.lv;
+  a.projection = sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
+  a.features = slot.features.p;
+  a.out_elem = elem;
+  a.dbg_indices = dbg_idx;
+  a.dbg_features = dbg_feat;
+  launch_encode(a, s);
+  slot.valid = true;
+  slot.scene = scene_id;
+  slot.n_pixels = P;
+  slot.elem = elem;
+}
+
+void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, FrameSlot &slot,
+                       const nwprt_gbuffer *gb, int w, int h, const nwprt_frame_out *out,
+                       Elem elem, nwprt_tonemap_source tm, hipStream_t s, float *dump = nullptr,
+                       int dump_begin = 0, int dump_count = 0) {
+  const int P = w * h;
+  require(slot.valid && slot.scene == scene_id && slot.n_pixels == P && slot.elem == elem,
+          NWPRT_ERR_STATE,
+          "pass p2: frame slot was not encoded (p1) for this scene, size and precision");
+  require(env.n == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
+          "environment map face size " + std::to_string(env.n) +
+              " does not match the scene's wavelet feature table (" +
+              std::to_string(sc.desc.face_size) + ")");
+  require(tm == NWPRT_TONEMAP_FULL || tm == NWPRT_TONEMAP_INDIRECT || tm == NWPRT_TONEMAP_NONE,
+          NWPRT_ERR_INVALID_ARGUMENT, "unknown tonemap source");
+  if (out) {
+    if (out->indirect) check_aligned(out->indirect, 16, "indirect output");
+    if (out->indirect_with_direct) check_aligned(out->indirect_with_direct, 16, "with-direct output");
+    if (out->rgba8) check_aligned(out->rgba8, 4, "rgba8 output");
+  }
+  hip_check(hipStreamWaitEvent(s, env.ready, 0), "hipStreamWaitEvent");
+  const int n_coef = 6 * env.n * env.n;
+  RelightArgs a{};
+  a.n_pixels = P;
+  a.k = env.k;
+  a.features = slot.features.p;
+  a.light = env.strength.as<float>();
+  a.env_flag = gb->env_flag;
+  a.direct = gb->direct;
+  a.indirect = out ? out->indirect : nullptr;
+  a.with_direct = out ? out->indirect_with_direct : nullptr;
+  a.rgba8 = out ? out->rgba8 : nullptr;
+  a.tonemap_source = (int)tm;
+  a.softplus = sc.desc.output_softplus;
+  a.dump = dump;
+  a.dump_begin = dump_begin;
+  a.dump_count = dump_count;
+  if (elem == Elem::F32) {
+    slot.wsel.ensure((size_t)env.k * 32 * 4);
+    launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
+                             env.idx.as<int32_t>(), env.k, Elem::BF16, nullptr,
+                             slot.wsel.as<float>(), s);
+    a.wsel = slot.wsel.as<float>();
+    a.w_f32 = sc.w_f32.as<float>();
+    launch_relight_plain(a, s);
+  } else {
+    slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
+    launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
+                             env.idx.as<int32_t>(), env.k, elem, slot.mk.as<uint16_t>(), nullptr,
+                             s);
+    a.mk = slot.mk.as<uint16_t>();
+    a.w_tiles = elem == Elem::BF16 ? sc.tiles_bf16.as<uint16_t>() : sc.tiles_f16.as<uint16_t>();
+    launch_relight_mfma(a, elem, s);
+  }
+}
+
+}  // namespace
+}  // namespace nwprt
+
+using namespace nwprt;
+
+extern "C" {
+
+int32_t nwprt_abi_version(void) { return NWPRT_ABI_VERSION; }
+
+const char *nwprt_status_string(nwprt_status s) { return status_name(s); }
+
+nwprt_status nwprt_create(int device, nwprt_ctx *out) {
+  return guarded([&] {
+    require(out != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null output pointer");
+    int count = 0;
+    hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+    require(device >= 0 && device < count, NWPRT_ERR_INVALID_ARGUMENT,
+            "device " + std::to_string(device) + " not present (" + std::to_string(count) +
+                " visible)");
+    auto ctx = std::make_unique<nwprt_context>();
+    ctx->device = device;
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    *out = ctx.release();
+  });
+}
+
+nwprt_status nwprt_destroy(nwprt_ctx ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipDeviceSynchronize();
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    (void)hipStreamDestroy(s);
+  });
+}
+
+nwprt_status nwprt_sync(nwprt_ctx ctx) {
+  return guarded([&] {
+    bind(ctx);
+    hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  });
+}
+
+nwprt_status nwprt_get_stream(nwprt_ctx ctx, void **stream) {
+  return guarded([&] {
+    require(ctx && stream, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    *stream = (void *)ctx->stream;
+  });
+}
+
+nwprt_status nwprt_scene_create(nwprt_ctx ctx, const nwprt_scene_desc *desc,
+                                const nwprt_scene_params *params, int32_t *scene_id) {
+  return guarded([&] {
+    bind(ctx);
+    require(desc && params, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    int id = 0;
+    create_scene(ctx, *desc, *params, &id);
+    *scene_id = id;
+  });
+}
+
+nwprt_status nwprt_scene_load_folder(nwprt_ctx ctx, const char *model_dir, int32_t *scene_id) {
+  return guarded([&] {
+    bind(ctx);
+    require(model_dir && scene_id, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    const HostScene hs = load_model_folder(model_dir);
+    nwprt_scene_params p{};
+    p.hash_table = hs.hash_table.data();
+    p.wavelet_features = hs.wavelet_features.data();
+    p.projection = hs.projection.empty() ? nullptr : hs.projection.data();
+    p.layers = hs.layers.data();
+    int id = 0;
+    create_scene(ctx, hs.desc, p, &id);
+    *scene_id = id;
+  });
+}
+
+nwprt_status nwprt_scene_destroy(nwprt_ctx ctx, int32_t scene_id) {
+  return guarded([&] {
+    bind(ctx);
+    get_scene(ctx, scene_id);
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    ctx->scenes.erase(scene_id);
+    for (auto &sl : ctx->slots)
+      if (sl.scene == scene_id) sl.valid = false;
+  });
+}
+
+nwprt_status nwprt_scene_get_desc(nwprt_ctx ctx, int32_t scene_id, nwprt_scene_desc *desc) {
+  return guarded([&] {
+    require(ctx && desc, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    *desc = get_scene(ctx, scene_id).desc;
+  });
+}
+
+nwprt_status nwprt_envmap_from_faces(nwprt_ctx ctx, const float *faces,
+                                     nwprt_location faces_location, int32_t face_size,
+                                     int32_t k, int32_t *env_id) {
+  return guarded([&] {
+    bind(ctx);
+    int id = 0;
+    env_from_faces(ctx, faces, faces_location, face_size, k, &id);
+    *env_id = id;
+  });
+}
+
+nwprt_status nwprt_envmap_update_faces(nwprt_ctx ctx, int32_t env_id, const float *faces,
+                                       nwprt_location faces_location, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    EnvMap &e = get_env(ctx, env_id);
+    require(e.has_coef, NWPRT_ERR_STATE, "environment map was not built from faces");
+    require(faces != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null faces");
+    const hipStream_t s = pick(ctx, stream);
+    const size_t m = 6ull * e.n * e.n;
+    const float *dfaces = faces;
+    if (faces_location == NWPRT_HOST) {
+      ctx->staging.ensure(m * 12);
+      hip_check(hipMemcpyAsync(ctx->staging.p, faces, m * 12, hipMemcpyHostToDevice, s),
+                "hipMemcpyAsync faces");
+      dfaces = ctx->staging.as<float>();
+    } else {
+      require(faces_location == NWPRT_DEVICE, NWPRT_ERR_INVALID_ARGUMENT, "bad faces location");
+    }
+    ctx->scores.ensure(m * 4);
+    launch_haar(dfaces, m * 3, e.coef.as<float>(), e.n, 1, s);
+    launch_topk(e.coef.as<float>(), e.n, e.k, ctx->scores.as<uint32_t>(), e.idx.as<int32_t>(),
+                e.strength.as<float>(), s);
+    hip_check(hipEventRecord(e.ready, s), "hipEventRecord");
+    if (faces_location == NWPRT_HOST) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+nwprt_status nwprt_envmap_from_selection(nwprt_ctx ctx, const int32_t *indices,
+                                         const float *strengths, int32_t k, int32_t face_size,
+                                         int32_t *env_id) {
+  return guarded([&] {
+    bind(ctx);
+    int id = 0;
+    env_from_selection(ctx, indices, strengths, k, face_size, &id);
+    *env_id = id;
+  });
+}
+
+nwprt_status nwprt_envmap_load_folder(nwprt_ctx ctx, const char *folder, const char *name,
+                                      int32_t k, int32_t *env_id) {
+  return guarded([&] {
+    bind(ctx);
+    require(folder && name && env_id, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    const std::string base(folder), nm(name);
+    const HdrImage cross = hdr_read(base + "/cube/" + nm + ".hdr");
+    const int n = cross.height / 3;
+    std::vector<float> faces((size_t)6 * n * n * 3);
+    cross_to_faces(cross.rgb.data(), cross.width, cross.height, faces.data());
+    const std::string cpath = base + "/waveletcoeffs/" + nm + ".npy";
+    const std::string spath = base + "/waveletstrength/" + nm + ".npy";
+    std::ifstream c1(cpath), c2(spath);
+    int id = 0;
+    if (c1.good() && c2.good()) {
+      const NpyArray ci = npy_read(cpath), si = npy_read(spath);
+      require(si.shape.size() == 2 && si.shape[1] == 3, NWPRT_ERR_FORMAT,
+              "waveletstrength: expected shape [K, 3]");
+      const std::vector<int32_t> idx = ci.as_i32();
+      const std::vector<float> str = si.as_f32();
+      require(idx.size() * 3 == str.size(), NWPRT_ERR_FORMAT,
+              "waveletcoeffs / waveletstrength: K mismatch");
+      env_from_selection(ctx, idx.data(), str.data(), (int)idx.size(), n, &id);
+    } else {
+      env_from_faces(ctx, faces.data(), NWPRT_HOST, n, k, &id);
+    }
+    *env_id = id;
+  });
+}
+
+nwprt_status nwprt_envmap_destroy(nwprt_ctx ctx, int32_t env_id) {
+  return guarded([&] {
+    bind(ctx);
+    get_env(ctx, env_id);
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    ctx->envs.erase(env_id);
+  });
+}
+
+nwprt_status nwprt_envmap_info(nwprt_ctx ctx, int32_t env_id, int32_t *face_size, int32_t *k) {
+  return guarded([&] {
+    require(ctx, NWPRT_ERR_INVALID_ARGUMENT, "null context");
+    const EnvMap &e = get_env(ctx, env_id);
+    if (face_size) *face_size = e.n;
+    if (k) *k = e.k;
+  });
+}
+
+nwprt_status nwprt_envmap_get_selection(nwprt_ctx ctx, int32_t env_id, int32_t *indices,
+                                        float *strengths) {
+  return guarded([&] {
+    bind(ctx);
+    EnvMap &e = get_env(ctx, env_id);
+    hip_check(hipEventSynchronize(e.ready), "hipEventSynchronize");
+    if (indices)
+      hip_check(hipMemcpy(indices, e.idx.p, (size_t)e.k * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (strengths)
+      hip_check(hipMemcpy(strengths, e.strength.p, (size_t)e.k * 12, hipMemcpyDeviceToHost),
+                "hipMemcpy");
+  });
+}
+
+nwprt_status nwprt_envmap_get_coefficients(nwprt_ctx ctx, int32_t env_id, float *coefficients) {
+  return guarded([&] {
+    bind(ctx);
+    EnvMap &e = get_env(ctx, env_id);
+    require(e.has_coef, NWPRT_ERR_STATE, "environment map was not built from faces");
+    require(coefficients, NWPRT_ERR_INVALID_ARGUMENT, "null output");
+    hip_check(hipEventSynchronize(e.ready), "hipEventSynchronize");
+    hip_check(hipMemcpy(coefficients, e.coef.p, (size_t)6 * e.n * e.n * 12, hipMemcpyDeviceToHost),
+              "hipMemcpy");
+  });
+}
+
+nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
+                           const nwprt_gbuffer *gbuffer, int32_t width, int32_t height,
+                           const nwprt_frame_out *out, nwprt_precision precision,
+                           nwprt_tonemap_source tonemap, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    Scene &sc = get_scene(ctx, scene_id);
+    EnvMap &env = get_env(ctx, env_id);
+    check_frame(gbuffer, width, height, true);
+    const Elem elem = elem_of(precision);
+    const hipStream_t s = pick(ctx, stream);
+    encode_into(ctx, sc, scene_id, ctx->slots[2], gbuffer, width, height, elem, s);
+    relight_from_slot(ctx, sc, scene_id, env, ctx->slots[2], gbuffer, width, height, out, elem,
+                      tonemap, s);
+  });
+}
+
+nwprt_status nwprt_pass_p1(nwprt_ctx ctx, int32_t scene_id, int32_t frame_slot,
+                           const nwprt_gbuffer *gbuffer, int32_t width, int32_t height,
+                           nwprt_precision precision, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    require(frame_slot == 0 || frame_slot == 1, NWPRT_ERR_INVALID_ARGUMENT,
+            "frame slot must be 0 or 1");
+    Scene &sc = get_scene(ctx, scene_id);
+    check_frame(gbuffer, width, height, false);
+    encode_into(ctx, sc, scene_id, ctx->slots[frame_slot], gbuffer, width, height,
+                elem_of(precision), pick(ctx, stream));
+  });
+}
+
+nwprt_status nwprt_pass_p2(nwprt_ctx ctx, int32_t scene_id, int32_t env_id, int32_t frame_slot,
+                           const nwprt_gbuffer *gbuffer, int32_t width, int32_t height,
+                           const nwprt_frame_out *out, nwprt_precision precision,
+                           nwprt_tonemap_source tonemap, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    require(frame_slot == 0 || frame_slot == 1, NWPRT_ERR_INVALID_ARGUMENT,
+            "frame slot must be 0 or 1");
+    Scene &sc = get_scene(ctx, scene_id);
+    EnvMap &env = get_env(ctx, env_id);
+    check_frame(gbuffer, width, height, true);
+    relight_from_slot(ctx, sc, scene_id, env, ctx->slots[frame_slot], gbuffer, width, height, out,
+                      elem_of(precision), tonemap, pick(ctx, stream));
+  });
+}
+
+nwprt_status nwprt_tonemap(nwprt_ctx ctx, const float *src_float4, uint32_t *dst_rgba8,
+                           int32_t width, int32_t height, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    require(src_float4 && dst_rgba8 && width > 0 && height > 0, NWPRT_ERR_INVALID_ARGUMENT,
+            "tonemap: bad argument");
+    check_aligned(src_float4, 16, "tonemap source");
+    launch_tonemap(src_float4, dst_rgba8, width * height, pick(ctx, stream));
+  });
+}
+
+nwprt_status nwprt_debug_hash_encode(nwprt_ctx ctx, int32_t scene_id, const float *positions,
+                                     int32_t n, uint32_t *indices, float *features,
+                                     void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    Scene &sc = get_scene(ctx, scene_id);
+    require(positions && n > 0 && (indices || features), NWPRT_ERR_INVALID_ARGUMENT,
+            "debug_hash_encode: bad argument");
+    check_aligned(positions, 16, "positions");
+    nwprt_gbuffer gb{};
+    gb.position = (float *)positions;
+    DevBuf idx_tmp;
+    uint32_t *idx = indices;
+    if (!idx) {
+      idx_tmp.ensure((size_t)n * sc.lv.n_levels * 8 * 4);
+      idx = idx_tmp.as<uint32_t>();
+    }
+    const hipStream_t s = pick(ctx, stream);
+    encode_into(ctx, sc, scene_id, ctx->slots[2], &gb, n, 1, Elem::F32, s, idx, features);
+    ctx->slots[2].valid = false;
+    if (!indices) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+nwprt_status nwprt_debug_transport(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
+                                   const nwprt_gbuffer *gbuffer, int32_t width, int32_t height,
+                                   int32_t pixel_begin, int32_t n_pixels, float *transport,
+                                   nwprt_precision precision, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    Scene &sc = get_scene(ctx, scene_id);
+    EnvMap &env = get_env(ctx, env_id);
+    check_frame(gbuffer, width, height, true);
+    require(transport && n_pixels > 0 && pixel_begin >= 0 &&
+                (long long)pixel_begin + n_pixels <= (long long)width * height,
+            NWPRT_ERR_INVALID_ARGUMENT, "debug_transport: pixel range outside the frame");
+    const Elem elem = elem_of(precision);
+    const hipStream_t s = pick(ctx, stream);
+    encode_into(ctx, sc, scene_id, ctx->slots[2], gbuffer, width, height, elem, s);
+    relight_from_slot(ctx, sc, scene_id, env, ctx->slots[2], gbuffer, width, height, nullptr,
+                      elem, NWPRT_TONEMAP_NONE, s, transport, pixel_begin, n_pixels);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+// Environment-map kernels for gfx950:
+//  - non-standard 2-D Haar transform of each cubemap face (FP64 scaling
+//    coefficients, one rounding to FP32 per output coefficient);
+//  - area-weighted top-K selection with ties broken by lower flat index;
+//  - per-relight preparation of the selected wavelets' feature rows.
+//
+// The reference only loads the results of an offline transform
+// (waveletcoeffs/, waveletstrength/: Renderer/EnvironmentMap.cpp:36-40, 56-67)
+// and gathers feature rows per pair (Renderer/tinycudann/utils_kernels.h:97-115).
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace nwprt {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    fail(NWPRT_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+constexpr int kHaarBlock = 256;
+constexpr int kHaarMaxPerThread =
+    (NWPRT_MAX_FACE_SIZE / 4) * (NWPRT_MAX_FACE_SIZE / 4) / kHaarBlock;  // 4
+
+// One 2x2 step of the orthonormal non-standard Haar transform.  a = x[2i][2j],
+// b = x[2i][2j+1], c = x[2i+1][2j], d = x[2i+1][2j+1] (row 0 = bottom row of
+// the face).  The row step then the column step, each with 1/sqrt(2), equal
+// one division by 2; the operation order below is fixed so that any FP64
+// implementation following it produces identical bits.
+struct Haar4 {
+  double s, h, v, dg;
+};
+__device__ __forceinline__ Haar4 haar2x2(double a, double b, double c, double d) {
+  Haar4 r;
+  r.s = ((a + b) + (c + d)) * 0.5;
+  r.h = ((a - b) + (c - d)) * 0.5;
+  r.v = ((a + b) - (c + d)) * 0.5;
+  r.dg = ((a - b) - (c - d)) * 0.5;
+  return r;
+}
+
+// Grid: one workgroup per (map, face, channel).  The first level reads the
+// FP32 texels from global memory; the LL quadrant (FP64) of every later level
+// lives in LDS, (N/2)^2 doubles (32 KiB at N = 128).  Mallat layout: at a
+// level whose LL block is s x s (hs = s/2), the details of 2x2 block (i, j)
+// go to (i, hs+j) [h], (hs+i, j) [v], (hs+i, hs+j) [dg]; the final LL is the
+// DC coefficient at (0, 0) = face sum / N.
+__global__ __launch_bounds__(kHaarBlock) void haar_kernel(const float *__restrict__ faces,
+                                                          size_t map_stride,
+                                                          float *__restrict__ coef, int n) {
+  extern __shared__ double ll[];
+  const int id = blockIdx.x;
+  const int ch = id % 3, face = (id / 3) % 6, map = id / 18;
+  const size_t base = (size_t)map * map_stride + (size_t)face * n * n * 3 + ch;
+  const float *src = faces + base;
+  float *dst = coef + base;
+  const int half = n / 2;
+  const int tid = threadIdx.x;
+
+  for (int q = tid; q < half * half; q += kHaarBlock) {
+    const int i = q / half, j = q % half;
+    const double a = src[((size_t)(2 * i) * n + 2 * j) * 3];
+    const double b = src[((size_t)(2 * i) * n + 2 * j + 1) * 3];
+    const double c = src[((size_t)(2 * i + 1) * n + 2 * j) * 3];
+    const double d = src[((size_t)(2 * i + 1) * n + 2 * j + 1) * 3];
+    const Haar4 r = haar2x2(a, b, c, d);
+    ll[i * half + j] = r.s;
+    dst[((size_t)i * n + half + j) * 3] = (float)r.h;
+    dst[((size_t)(half + i) * n + j) * 3] = (float)r.v;
+    dst[((size_t)(half + i) * n + half + j) * 3] = (float)r.dg;
+  }
+  __syncthreads();
+
+  for (int s = half; s >= 2; s >>= 1) {
+    const int hs = s >> 1;
+    double keep[kHaarMaxPerThread];
+#pragma unroll
+    for (int t = 0; t < kHaarMaxPerThread; ++t) {
+      const int q = tid + t * kHaarBlock;
+      if (q < hs * hs) {
+        const int i = q / hs, j = q % hs;
+        const Haar4 r = haar2x2(ll[(2 * i) * half + 2 * j], ll[(2 * i) * half + 2 * j + 1],
+                                ll[(2 * i + 1) * half + 2 * j],
+                                ll[(2 * i + 1) * half + 2 * j + 1]);
+        keep[t] = r.s;
+        dst[((size_t)i * n + hs + j) * 3] = (float)r.h;
+        dst[((size_t)(hs + i) * n + j) * 3] = (float)r.v;
+        dst[((size_t)(hs + i) * n + hs + j) * 3] = (float)r.dg;
+      }
+    }
+    __syncthreads();  // every read of this level is done before LL is overwritten
+#pragma unroll
+    for (int t = 0; t < kHaarMaxPerThread; ++t) {
+      const int q = tid + t * kHaarBlock;
+      if (q < hs * hs) ll[(q / hs) * half + (q % hs)] = keep[t];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) dst[0] = (float)ll[0];
+}
+
+// ------------------------------------------------------------------ top-K --
+
+constexpr int kTopkBlock = 1024;
+
+// Score of coefficient t: |luminance| (weights of Renderer/EnvironmentMap.cpp:149-151,
+// evaluated unfused in FP32 as there) times the support area of its basis
+// function relative to the face, 4^-j for a detail at Mallat position with
+// max(row, col) in [2^j, 2^(j+1)) and 1 for the DC term.  Scores below the
+// smallest normal float and NaNs become 0.
+__device__ __forceinline__ uint32_t score_bits(const float *__restrict__ coef, int t, int n) {
+  const float r = coef[3 * (size_t)t], g = coef[3 * (size_t)t + 1], b = coef[3 * (size_t)t + 2];
+  const float lum = __fadd_rn(__fadd_rn(__fmul_rn(r, 0.212671f), __fmul_rn(g, 0.715160f)),
+                              __fmul_rn(b, 0.072169f));
+  const int q = t % (n * n);
+  const int row = q / n, col = q % n;
+  const int m = row > col ? row : col;
+  const int j = m == 0 ? 0 : 31 - __clz(m);
+  float s = ldexpf(fabsf(lum), -2 * j);
+  if (!(s >= 1.17549435e-38f)) s = 0.0f;  // NaN, 0 and subnormal scores
+  return __float_as_uint(s);
+}
+
+// Key: score bits in the high word, (2^32 - 1 - index) in the low word, so a
+// larger key means a larger score, and for equal scores a lower index.  Keys
+// are unique, so "the K largest keys" is an exact set.
+__device__ __forceinline__ uint64_t topk_key(const uint32_t *scores, int t) {
+  return ((uint64_t)scores[t] << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)t);
+}
+
+// One workgroup: scores -> MSB-first radix select of the K-th largest key
+// (8-bit digits, early exit once a whole bucket is taken) -> compaction ->
+// bitonic sort of the K selected keys in LDS (descending).
+__global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restrict__ coef, int n,
+                                                          int K, uint32_t *__restrict__ scores,
+                                                          int32_t *__restrict__ out_idx,
+                                                          float *__restrict__ out_str) {
+  __shared__ uint32_t hist[256];
+  __shared__ uint64_t sel[NWPRT_MAX_K];
+  __shared__ uint64_t s_prefix, s_mask;
+  __shared__ uint32_t s_need, s_count;
+  __shared__ int s_done;
+  const int tid = threadIdx.x;
+  const int M = 6 * n * n;
+
+  for (int t = tid; t < M; t += kTopkBlock) scores[t] = score_bits(coef, t, n);
+  if (tid == 0) {
+    s_prefix = 0;
+    s_mask = 0;
+    s_need = (uint32_t)K;
+    s_done = 0;
+    s_count = 0;
+  }
+  __threadfence_block();
+  __syncthreads();
+
+  for (int pass = 0; pass < 8; ++pass) {
+    const int shift = 56 - 8 * pass;
+    if (tid < 256) hist[tid] = 0;
+    __syncthreads();
+    const uint64_t prefix = s_prefix, mask = s_mask;
+    for (int t = tid; t < M; t += kTopkBlock) {
+      const uint64_t key = topk_key(scores, t);
+      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t cum = 0;
+      int d = 255;
+      for (; d > 0; --d) {
+        if (cum + hist[d] >= s_need) break;
+        cum += hist[d];
+      }
+      s_need -= cum;
+      s_prefix = prefix | ((uint64_t)d << shift);
+      s_mask = mask | (0xFFull << shift);
+      s_done = hist[d] == s_need;
+    }
+    __syncthreads();
+    if (s_done) break;
+  }
+
+  const uint64_t prefix = s_prefix, mask = s_mask;
+  for (int t = tid; t < M; t += kTopkBlock) {
+    const uint64_t key = topk_key(scores, t);
+    if ((key & mask) >= prefix) {
+      const uint32_t pos = atomicAdd(&s_count, 1u);
+      if (pos < (uint32_t)NWPRT_MAX_K) sel[pos] = key;
+    }
+  }
+  __syncthreads();
+  int kp = 1;
+  while (kp < K) kp <<= 1;
+  for (int i = (int)s_count + tid; i < kp; i += kTopkBlock) sel[i] = 0;
+  __syncthreads();
+  for (int size = 2; size <= kp; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < kp; i += kTopkBlock) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const uint64_t x = sel[i], y = sel[j];
+          const bool desc = (i & size) == 0;
+          if (desc ? (x < y) : (x > y)) {
+            sel[i] = y;
+            sel[j] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < K; i += kTopkBlock) {
+    const uint32_t idx = 0xFFFFFFFFu - (uint32_t)(sel[i] & 0xFFFFFFFFull);
+    out_idx[i] = (int32_t)idx;
+    for (int c = 0; c < 3; ++c) out_str[3 * i + c] = coef[3 * (size_t)idx + c];
+  }
+}
+
+// -------------------------------------------------------- selection prep --
+
+// One workgroup per selected wavelet k.  Thread t = (m-block b, lane):
+// lane holds row m = 16 b + lane % 16 and input slots 8 g .. 8 g + 7
+// (g = lane / 16) of the first layer's A operand, pre-multiplied by the
+// wavelet feature row: W1[m][f] * wfeat[f][c_k].  This moves the per-pair
+// Hadamard product of Renderer/tinycudann/utils_kernels.h:113-114 onto the
+// weights, once per (relight, k).
+__global__ __launch_bounds__(256) void prepare_selection_kernel(
+    const float *__restrict__ w1, const float *__restrict__ wfeat, int n_coef,
+    const int32_t *__restrict__ indices, int elem, uint16_t *__restrict__ mk,
+    float *__restrict__ wsel) {
+  const int k = blockIdx.x;
+  const int t = threadIdx.x;
+  const int c = indices[k];
+  if (mk) {
+    const int b = t >> 6, lane = t & 63, g = lane >> 4;
+    const int m = 16 * b + (lane & 15);
+    float v[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int f = 8 * g + s;
+      v[s] = w1[m * 32 + f] * wfeat[(size_t)f * n_coef + c];
+    }
+    dev::u32x4 out;
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+      out[d] = elem == (int)Elem::BF16 ? dev::pack_bf16(v[2 * d], v[2 * d + 1])
+                                       : dev::pack_f16(v[2 * d], v[2 * d + 1]);
+    *(dev::u32x4 *)(mk + (((size_t)k * 4 + b) * 64 + lane) * 8) = out;
+  }
+  if (wsel && t < 32) wsel[(size_t)k * 32 + t] = wfeat[(size_t)t * n_coef + c];
+}
+
+}  // namespace
+
+void launch_haar(const float *faces, size_t map_stride, float *coef, int n, int n_maps,
+                 hipStream_t s) {
+  require(n >= 2 && n <= NWPRT_MAX_FACE_SIZE && (n & (n - 1)) == 0 && n_maps >= 1,
+          NWPRT_ERR_INVALID_ARGUMENT, "haar: bad face size");
+  const size_t lds = (size_t)(n / 2) * (n / 2) * sizeof(double);
+  hipLaunchKernelGGL(haar_kernel, dim3(n_maps * 18), dim3(kHaarBlock), lds, s, faces,
+                     map_stride, coef, n);
+  hip_check(hipGetLastError(), "haar_kernel launch");
+}
+
+void launch_topk(const float *coef, int n, int k, uint32_t *scores, int32_t *indices,
+                 float *strengths, hipStream_t s) {
+  require(k >= 1 && k <= NWPRT_MAX_K && k <= 6 * n * n, NWPRT_ERR_INVALID_ARGUMENT,
+          "topk: K must be in [1, min(1024, 6 N^2)]");
+  hipLaunchKernelGGL(topk_kernel, dim3(1), dim3(kTopkBlock), 0, s, coef, n, k, scores,
+                     indices, strengths);
+  hip_check(hipGetLastError(), "topk_kernel launch");
+}
+
+void launch_prepare_selection(const float *w1, const float *wfeat, int n_coef,
+                              const int32_t *indices, int k, Elem mk_elem, uint16_t *mk,
+                              float *wsel, hipStream_t s) {
+  require(k >= 1 && k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT, "prepare: bad K");
+  require(mk == nullptr || mk_elem == Elem::BF16 || mk_elem == Elem::F16,
+          NWPRT_ERR_INVALID_ARGUMENT, "prepare: MFMA tiles must be BF16 or FP16");
+  hipLaunchKernelGGL(prepare_selection_kernel, dim3(k), dim3(256), 0, s, w1, wfeat, n_coef,
+                     indices, (int)mk_elem, mk, wsel);
+  hip_check(hipGetLastError(), "prepare_selection_kernel launch");
+}
+
+}  // namespace nwprt

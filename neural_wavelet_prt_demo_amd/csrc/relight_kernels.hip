@@ -1,0 +1,316 @@
+// Per-(pixel, wavelet) network and per-pixel reduction for gfx950.
+//
+// Reference path being replaced (Renderer/tinycudann/tcnn.cu:111-152): a host
+// loop over image rows runs, per row, the hash grid, the per-pair Hadamard
+// product with the wavelet feature (utils_kernels.h:97-115), the out_prd and
+// cmp_net networks and a cast, materialising 16 + 3 floats per pair; then
+// computeResponse (utils_kernels.h:51-90) sums max(light_i * T_i, 0) over K.
+//
+// Here one launch does all pairs of a frame and nothing per pair touches
+// HBM: each wave owns 64 pixels and walks the K selected wavelets; the four
+// linear layers run as chained 16x16x32 MFMAs whose results stay in VGPRs,
+// and the clamped sum is carried per lane with Neumaier compensation.
+//
+// MFMA data layout (v_mfma_f32_16x16x32_{bf16,f16}; lane l, g = l/16,
+// n = l%16): the A operand holds row n, k-slots 8g..8g+7; B holds column n,
+// the same k-slots; the result D holds rows 4g..4g+3 of column n.  Weights
+// are the A operand (rows = output features) and pixels are the columns, so
+// a layer's D block (output features 16b + 4g + j) is already laid out as
+// the next layer's B operand up to a permutation of the contraction index,
+// which the host folds into the weight tiles (build_weight_tiles).
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace nwprt {
+namespace {
+
+using dev::f32x4;
+using dev::u32x4;
+
+constexpr int kRelightBlock = 256;  // 4 waves x 64 pixels
+
+template <typename E>
+struct Mfma;
+template <>
+struct Mfma<__bf16> {
+  static __device__ __forceinline__ f32x4 run(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(dev::bf16x8, a),
+                                                   __builtin_bit_cast(dev::bf16x8, b), c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ uint32_t pack(float lo, float hi) {
+    return dev::pack_bf16(lo, hi);
+  }
+};
+template <>
+struct Mfma<_Float16> {
+  static __device__ __forceinline__ f32x4 run(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(dev::f16x8, a),
+                                                  __builtin_bit_cast(dev::f16x8, b), c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ uint32_t pack(float lo, float hi) {
+    return dev::pack_f16(lo, hi);
+  }
+};
+
+// ReLU + round of two 16-row D blocks (output features 32kb + [0,32)) into
+// one B operand: slots 0..3 <- block 2kb rows 4g..4g+3, slots 4..7 <- block
+// 2kb+1 rows 4g..4g+3.
+template <typename E>
+__device__ __forceinline__ u32x4 relu_pack(f32x4 lo, f32x4 hi) {
+  u32x4 r;
+  r[0] = Mfma<E>::pack(fmaxf(lo[0], 0.f), fmaxf(lo[1], 0.f));
+  r[1] = Mfma<E>::pack(fmaxf(lo[2], 0.f), fmaxf(lo[3], 0.f));
+  r[2] = Mfma<E>::pack(fmaxf(hi[0], 0.f), fmaxf(hi[1], 0.f));
+  r[3] = Mfma<E>::pack(fmaxf(hi[2], 0.f), fmaxf(hi[3], 0.f));
+  return r;
+}
+
+__device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float r, float g,
+                                              float b) {
+  const float4 d = *(const float4 *)(a.direct + 4 * (size_t)p);
+  const float4 ind = make_float4(r, g, b, 1.0f);
+  const float4 wd = make_float4(r + d.x, g + d.y, b + d.z, 1.0f);
+  if (a.indirect) *(float4 *)(a.indirect + 4 * (size_t)p) = ind;
+  if (a.with_direct) *(float4 *)(a.with_direct + 4 * (size_t)p) = wd;
+  if (a.rgba8 && a.tonemap_source != NWPRT_TONEMAP_NONE) {
+    const float4 t = a.tonemap_source == NWPRT_TONEMAP_FULL ? wd : ind;
+    a.rgba8[p] = dev::tonemap_rgba8(t.x, t.y, t.z);
+  }
+}
+
+// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
+// pixels [64 (4 b + w), +64).  Pixel groups j = 0..3 (16 pixels each) are the
+// MFMA columns; lane l owns pixel base + l for the reduction.
+template <typename E, bool DUMP>
+__global__ __launch_bounds__(kRelightBlock, 2) void relight_mfma_kernel(RelightArgs a) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, n = lane & 15;
+  const int base = (blockIdx.x * 4 + wave) * 64;
+  const int P = a.n_pixels;
+  if (base >= P) return;
+  const int my_px = base + lane;
+  const bool mine = my_px < P;
+  const bool hit = mine && (a.env_flag == nullptr || a.env_flag[my_px] != 0);
+  const bool any_hit = __ballot(hit) != 0;
+
+  float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
+  if (any_hit) {
+    // Per-pixel features as B operands: group j, column n, k-slots 8g..8g+7.
+    const uint16_t *feat = (const uint16_t *)a.features;
+    u32x4 hb[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = base + 16 * j + n;
+      hb[j] = p < P ? *(const u32x4 *)(feat + (size_t)p * 32 + 8 * g) : u32x4{0u, 0u, 0u, 0u};
+    }
+    // Layers 2..4 as A-operand tiles: [4][2][64] each.
+    const u32x4 *wt = (const u32x4 *)a.w_tiles;
+    u32x4 w2[4][2], w3[4][2], w4[4][2];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int kb = 0; kb < 2; ++kb) {
+        w2[b][kb] = wt[(0 * 8 + b * 2 + kb) * 64 + lane];
+        w3[b][kb] = wt[(1 * 8 + b * 2 + kb) * 64 + lane];
+        w4[b][kb] = wt[(2 * 8 + b * 2 + kb) * 64 + lane];
+      }
+    const u32x4 *mk = (const u32x4 *)a.mk;
+    u32x4 a1[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) a1[b] = mk[b * 64 + lane];
+
+    for (int k = 0; k < a.k; ++k) {
+      const u32x4 cur[4] = {a1[0], a1[1], a1[2], a1[3]};
+      if (k + 1 < a.k) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) a1[b] = mk[((size_t)(k + 1) * 4 + b) * 64 + lane];
+      }
+      const float l0 = a.light[3 * k], l1 = a.light[3 * k + 1], l2 = a.light[3 * k + 2];
+      const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+      f32x4 out = zero;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        f32x4 c[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) c[b] = Mfma<E>::run(cur[b], hb[j], zero);
+        u32x4 x0 = relu_pack<E>(c[0], c[1]), x1 = relu_pack<E>(c[2], c[3]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) c[b] = Mfma<E>::run(w2[b][1], x1, Mfma<E>::run(w2[b][0], x0, zero));
+        x0 = relu_pack<E>(c[0], c[1]);
+        x1 = relu_pack<E>(c[2], c[3]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) c[b] = Mfma<E>::run(w3[b][1], x1, Mfma<E>::run(w3[b][0], x0, zero));
+        x0 = relu_pack<E>(c[0], c[1]);
+        x1 = relu_pack<E>(c[2], c[3]);
+        // Output layer: variant j carries W4 in rows 4j..4j+2, so group j's
+        // three outputs land in rows 4j.. of the shared accumulator, i.e. in
+        // lanes with g == j -- the lanes that own those pixels.
+        out = Mfma<E>::run(w4[j][1], x1, Mfma<E>::run(w4[j][0], x0, out));
+      }
+      const float lc[3] = {l0, l1, l2};
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float t = a.softplus ? dev::softplus(out[ch]) : out[ch];
+        dev::neumaier_add(sum[ch], comp[ch], fmaxf(lc[ch] * t, 0.0f));
+        if (DUMP) {
+          const int q = my_px - a.dump_begin;
+          if (mine && q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
+        }
+      }
+    }
+  }
+  if (mine) {
+    const float r = hit ? sum[0] + comp[0] : 0.f, gg = hit ? sum[1] + comp[1] : 0.f,
+                bb = hit ? sum[2] + comp[2] : 0.f;
+    write_outputs(a, my_px, r, gg, bb);
+  }
+}
+
+// Plain FP32 form: one lane per pixel walks the K wavelets with the four
+// layers on the vector ALU, weights broadcast from LDS and the selected
+// wavelet features read with uniform (scalar) loads.
+constexpr int kPlainBlock = 256;
+constexpr int kW1 = 64 * 32, kW2 = 64 * 64, kW3 = 64 * 64, kW4 = 3 * 64;
+
+template <bool DUMP>
+__global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs a) {
+  __shared__ float w[kW1 + kW2 + kW3 + kW4];
+  for (int i = threadIdx.x; i < kW1 + kW2 + kW3 + kW4; i += kPlainBlock) w[i] = a.w_f32[i];
+  __syncthreads();
+  const int p = blockIdx.x * kPlainBlock + threadIdx.x;
+  if (p >= a.n_pixels) return;
+  const bool hit = a.env_flag == nullptr || a.env_flag[p] != 0;
+  float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
+  if (hit) {
+    float h[32];
+    const float4 *fp = (const float4 *)((const float *)a.features + (size_t)p * 32);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float4 v = fp[q];
+      h[4 * q] = v.x; h[4 * q + 1] = v.y; h[4 * q + 2] = v.z; h[4 * q + 3] = v.w;
+    }
+    for (int k = 0; k < a.k; ++k) {
+      // The weight reads are loop-invariant; an opaque zero offset keeps the
+      // compiler from hoisting all ~10K of them out of the K loop.
+      int wo = 0;
+      asm volatile("" : "+s"(wo));
+      const float *W1 = w + wo, *W2 = W1 + kW1, *W3 = W2 + kW2, *W4 = W3 + kW3;
+      const float *ws = a.wsel + (size_t)k * 32;
+      float y1[64];
+#pragma unroll
+      for (int o = 0; o < 64; ++o) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 32; ++i) acc = fmaf(W1[o * 32 + i], h[i] * ws[i], acc);
+        y1[o] = fmaxf(acc, 0.f);
+      }
+      float y2[64];
+#pragma unroll
+      for (int o = 0; o < 64; ++o) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) acc = fmaf(W2[o * 64 + i], y1[i], acc);
+        y2[o] = fmaxf(acc, 0.f);
+      }
+#pragma unroll
+      for (int o = 0; o < 64; ++o) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) acc = fmaf(W3[o * 64 + i], y2[i], acc);
+        y1[o] = fmaxf(acc, 0.f);  // reuse y1 as layer-3 output
+      }
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) acc = fmaf(W4[ch * 64 + i], y1[i], acc);
+        const float t = a.softplus ? dev::softplus(acc) : acc;
+        dev::neumaier_add(sum[ch], comp[ch], fmaxf(a.light[3 * k + ch] * t, 0.0f));
+        if (DUMP) {
+          const int q = p - a.dump_begin;
+          if (q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
+        }
+      }
+    }
+  }
+  write_outputs(a, p, hit ? sum[0] + comp[0] : 0.f, hit ? sum[1] + comp[1] : 0.f,
+                hit ? sum[2] + comp[2] : 0.f);
+}
+
+__global__ __launch_bounds__(256) void tonemap_kernel(const float *__restrict__ src,
+                                                      uint32_t *__restrict__ dst, int n) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const float4 v = *(const float4 *)(src + 4 * (size_t)p);
+  dst[p] = dev::tonemap_rgba8(v.x, v.y, v.z);
+}
+
+}  // namespace
+
+// A-operand tiles of layers 2, 3 (64 x 64) and the output layer (3 x 64) for
+// the chained MFMA layout above.  Tile [b][kb][lane][s] holds
+// W[16 b + lane%16][f(kb, lane/16, s)] with the contraction permutation
+// f(kb, g, s) = 32 kb + (s < 4 ? 4 g + s : 16 + 4 g + s - 4).  The output
+// layer is stored as four variants j (in place of b): rows 4j..4j+2 hold W4,
+// the other rows are zero.
+void build_weight_tiles(const float *w, Elem elem, std::vector<uint16_t> &out) {
+  const float *W2 = w + kW1, *W3 = W2 + kW2, *W4 = W3 + kW3;
+  out.assign(3 * 4 * 2 * 64 * 8, 0);
+  auto cvt = [&](float v) {
+    return elem == Elem::BF16 ? f32_to_bf16_bits(v) : f32_to_f16_bits(v);
+  };
+  auto feat = [](int kb, int g, int s) { return 32 * kb + (s < 4 ? 4 * g + s : 16 + 4 * g + s - 4); };
+  for (int layer = 0; layer < 3; ++layer)
+    for (int b = 0; b < 4; ++b)
+      for (int kb = 0; kb < 2; ++kb)
+        for (int lane = 0; lane < 64; ++lane)
+          for (int s = 0; s < 8; ++s) {
+            const int row = lane & 15, g = lane >> 4, f = feat(kb, g, s);
+            float v = 0.f;
+            if (layer == 0) v = W2[(16 * b + row) * 64 + f];
+            else if (layer == 1) v = W3[(16 * b + row) * 64 + f];
+            else if (row >= 4 * b && row < 4 * b + 3) v = W4[(row - 4 * b) * 64 + f];
+            out[((((size_t)layer * 4 + b) * 2 + kb) * 64 + lane) * 8 + s] = cvt(v);
+          }
+}
+
+void launch_relight_mfma(const RelightArgs &a, Elem elem, hipStream_t s) {
+  require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: bad pixel count or K");
+  require(a.features && a.mk && a.light && a.w_tiles && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: missing operand");
+  const dim3 grid((a.n_pixels + kRelightBlock - 1) / kRelightBlock);
+  const bool dump = a.dump != nullptr;
+  if (elem == Elem::BF16) {
+    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<__bf16, true>), grid, dim3(kRelightBlock), 0, s, a);
+    else hipLaunchKernelGGL((relight_mfma_kernel<__bf16, false>), grid, dim3(kRelightBlock), 0, s, a);
+  } else if (elem == Elem::F16) {
+    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<_Float16, true>), grid, dim3(kRelightBlock), 0, s, a);
+    else hipLaunchKernelGGL((relight_mfma_kernel<_Float16, false>), grid, dim3(kRelightBlock), 0, s, a);
+  } else {
+    fail(NWPRT_ERR_INVALID_ARGUMENT, "relight: MFMA element must be BF16 or FP16");
+  }
+  hip_check(hipGetLastError(), "relight_mfma_kernel launch");
+}
+
+void launch_relight_plain(const RelightArgs &a, hipStream_t s) {
+  require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: bad pixel count or K");
+  require(a.features && a.wsel && a.light && a.w_f32 && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: missing operand");
+  const dim3 grid((a.n_pixels + kPlainBlock - 1) / kPlainBlock);
+  if (a.dump) hipLaunchKernelGGL((relight_plain_kernel<true>), grid, dim3(kPlainBlock), 0, s, a);
+  else hipLaunchKernelGGL((relight_plain_kernel<false>), grid, dim3(kPlainBlock), 0, s, a);
+  hip_check(hipGetLastError(), "relight_plain_kernel launch");
+}
+
+void launch_tonemap(const float *src, uint32_t *dst, int n_pixels, hipStream_t s) {
+  require(n_pixels >= 1, NWPRT_ERR_INVALID_ARGUMENT, "tonemap: no pixels");
+  hipLaunchKernelGGL(tonemap_kernel, dim3((n_pixels + 255) / 256), dim3(256), 0, s, src, dst,
+                     n_pixels);
+  hip_check(hipGetLastError(), "tonemap_kernel launch");
+}
+
+}  // namespace nwprt

@@ -1,0 +1,167 @@
+"""ctypes binding of oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg.  The product never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, Structure, c_double, c_float, c_int, c_uint32, c_void_p
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+class OrcScene(Structure):
+    _fields_ = [("n_levels", c_int), ("n_features", c_int), ("scale", c_void_p),
+                ("resolution", c_void_p), ("dense", c_void_p), ("size", c_void_p),
+                ("offset", c_void_p), ("table", c_void_p), ("n_coef", c_int),
+                ("wfeat", c_void_p), ("proj", c_void_p), ("layers", c_void_p),
+                ("feature_dim", c_int), ("hidden", c_int), ("n_hidden_layers", c_int),
+                ("softplus", c_int)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        l = ctypes.CDLL(str(LIB_PATH))
+        l.orc_hashgrid_levels.argtypes = [c_int, c_int, c_int, c_float] + [c_void_p] * 5
+        l.orc_hash_encode.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_void_p]
+        l.orc_haar_face.argtypes = [c_void_p, c_int, c_int, c_void_p]
+        l.orc_envmap_haar.argtypes = [c_void_p, c_int, c_void_p]
+        l.orc_score.argtypes = [c_void_p, c_int, c_int]
+        l.orc_score.restype = c_float
+        l.orc_topk.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p]
+        l.orc_relight_pixels.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                         c_void_p, c_int]
+        l.orc_tonemap.argtypes = [c_float, c_float, c_float]
+        l.orc_tonemap.restype = c_uint32
+        l.orc_cpu_relight_fp32.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int, c_void_p,
+                                           c_void_p, c_void_p, c_int, c_int, c_void_p, c_int]
+        l.orc_cpu_relight_fp32.restype = c_double
+        _lib = l
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_void_p)
+
+
+def hashgrid_levels(n_levels=16, log2_hashmap_size=19, base_resolution=16, per_level_scale=1.5):
+    out = {"scale": np.zeros(n_levels, np.float32), "resolution": np.zeros(n_levels, np.uint32),
+           "dense": np.zeros(n_levels, np.uint32), "size": np.zeros(n_levels, np.uint32),
+           "offset": np.zeros(n_levels + 1, np.uint32)}
+    lib().orc_hashgrid_levels(n_levels, log2_hashmap_size, base_resolution, per_level_scale,
+                              _p(out["scale"]), _p(out["resolution"]), _p(out["dense"]),
+                              _p(out["size"]), _p(out["offset"]))
+    return out
+
+
+class Scene:
+    """Host scene for the oracle: keeps every array alive for the struct."""
+
+    def __init__(self, levels: Dict[str, np.ndarray], n_features: int, table: np.ndarray,
+                 wfeat: np.ndarray, proj: Optional[np.ndarray], layers: np.ndarray,
+                 softplus: bool = True, hidden: int = 64, n_hidden_layers: int = 3):
+        self.keep = dict(levels)
+        self.keep["table"] = np.ascontiguousarray(table, np.float32)
+        self.keep["wfeat"] = np.ascontiguousarray(wfeat, np.float32)
+        self.keep["proj"] = None if proj is None else np.ascontiguousarray(proj, np.float32)
+        self.keep["layers"] = np.ascontiguousarray(layers, np.float32)
+        feature_dim = self.keep["wfeat"].shape[0]
+        k = self.keep
+        self.s = OrcScene(len(levels["scale"]), n_features, _p(k["scale"]), _p(k["resolution"]),
+                          _p(k["dense"]), _p(k["size"]), _p(k["offset"]), _p(k["table"]),
+                          k["wfeat"].shape[1], _p(k["wfeat"]), _p(k["proj"]), _p(k["layers"]),
+                          feature_dim, hidden, n_hidden_layers, int(softplus))
+
+    @classmethod
+    def from_synth(cls, levels, desc, params):
+        """From a product SceneDesc + synth_scene() arrays; FP16 tables are
+        widened with numpy (exact), not with product code."""
+        tab = params["hash_table"]
+        if tab.dtype == np.uint16:
+            tab = tab.view(np.float16).astype(np.float32)
+        return cls(levels, desc.grid.n_features_per_level, tab, params["wavelet_features"],
+                   params["projection"] if desc.has_projection else None, params["layers"],
+                   bool(desc.output_softplus))
+
+    def hash_encode(self, pos3: np.ndarray):
+        L = self.s.n_levels
+        idx = np.zeros((L, 8), np.uint32)
+        feat = np.zeros(L * self.s.n_features, np.float64)
+        p = np.ascontiguousarray(pos3, np.float32)
+        lib().orc_hash_encode(ctypes.byref(self.s), _p(p), _p(idx), _p(feat))
+        return idx, feat
+
+    def relight(self, sel, light, gbuf, pixels=None, transport=False, threads=8):
+        sel = np.ascontiguousarray(sel, np.int32)
+        light = np.ascontiguousarray(light, np.float32)
+        pix = None if pixels is None else np.ascontiguousarray(pixels, np.int32)
+        n = len(gbuf["position"]) if pix is None else len(pix)
+        ind = np.zeros((n, 3), np.float64)
+        wd = np.zeros((n, 3), np.float64)
+        tr = np.zeros((n, len(sel), 3), np.float64) if transport else None
+        flag = gbuf.get("env_flag")
+        lib().orc_relight_pixels(ctypes.byref(self.s), _p(sel), _p(light), len(sel),
+                                 _p(gbuf["position"]), _p(flag), _p(gbuf["direct"]), _p(pix), n,
+                                 _p(ind), _p(wd), _p(tr), threads)
+        return (ind, wd, tr) if transport else (ind, wd)
+
+    def cpu_relight_fp32(self, sel, light, gbuf, begin, count, threads):
+        sel = np.ascontiguousarray(sel, np.int32)
+        light = np.ascontiguousarray(light, np.float32)
+        out = np.zeros((count, 3), np.float32)
+        secs = lib().orc_cpu_relight_fp32(ctypes.byref(self.s), _p(sel), _p(light), len(sel),
+                                          _p(gbuf["position"]), _p(gbuf.get("env_flag")),
+                                          _p(gbuf["direct"]), begin, count, _p(out), threads)
+        return out, secs
+
+
+def envmap_haar(faces: np.ndarray) -> np.ndarray:
+    faces = np.ascontiguousarray(faces, np.float32)
+    coef = np.zeros_like(faces)
+    lib().orc_envmap_haar(_p(faces), faces.shape[1], _p(coef))
+    return coef
+
+
+def haar_face(plane: np.ndarray) -> np.ndarray:
+    plane = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros_like(plane)
+    lib().orc_haar_face(_p(plane), plane.shape[0], 1, _p(out))
+    return out
+
+
+def score(coef: np.ndarray, t: int) -> float:
+    coef = np.ascontiguousarray(coef, np.float32)
+    return lib().orc_score(_p(coef), t, coef.shape[1])
+
+
+def topk(coef: np.ndarray, k: int):
+    coef = np.ascontiguousarray(coef, np.float32)
+    idx = np.zeros(k, np.int32)
+    st = np.zeros((k, 3), np.float32)
+    lib().orc_topk(_p(coef), coef.shape[1], k, _p(idx), _p(st))
+    return idx, st
+
+
+def tonemap(rgb: np.ndarray) -> np.ndarray:
+    rgb = np.asarray(rgb, np.float32).reshape(-1, rgb.shape[-1])
+    return np.array([lib().orc_tonemap(float(r[0]), float(r[1]), float(r[2])) for r in rgb],
+                    np.uint32)

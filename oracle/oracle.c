@@ -1,0 +1,286 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the relighting path (checker only;
+ * see oracle.h for who may load it and how each function is pinned).
+ *
+ * Built with -ffp-contract=off: every FP64 expression below is evaluated
+ * exactly as written, and the few FP32 steps that must match the GPU bit for
+ * bit use explicit fmaf() where the kernel fuses.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------ hash grid -- */
+
+/* Level table, in FP32 exactly as the product's host code documents it:
+ * scale_l = exp2f(l * log2f(b)) * N_min - 1, res_l = ceilf(scale_l) + 1,
+ * dense iff res_l^3 <= T, size_l = min(next_multiple(res_l^3, 8), T).
+ * (tiny-cuda-nn's grid conventions behind Renderer/tinycudann/prt_network.h:45;
+ * the library itself is absent from the reference tree.) */
+void orc_hashgrid_levels(int n_levels, int log2_hashmap_size, int base_resolution,
+                         float per_level_scale, float *scale, uint32_t *resolution,
+                         uint32_t *dense, uint32_t *size, uint32_t *offset) {
+  const uint64_t T = 1ull << log2_hashmap_size;
+  const float log2_b = log2f(per_level_scale);
+  uint64_t off = 0;
+  for (int l = 0; l < n_levels; ++l) {
+    const float s = exp2f((float)l * log2_b) * (float)base_resolution - 1.0f;
+    const uint32_t r = (uint32_t)ceilf(s) + 1u;
+    const uint64_t cube = (uint64_t)r * r * r;
+    uint64_t n = cube <= T ? (cube + 7) / 8 * 8 : T;
+    if (n > T) n = T;
+    scale[l] = s;
+    resolution[l] = r;
+    dense[l] = cube <= T;
+    size[l] = (uint32_t)n;
+    offset[l] = (uint32_t)off;
+    off += n;
+  }
+  offset[n_levels] = (uint32_t)off;
+}
+
+/* Multiresolution hash encoding of one position (Mueller et al. 2022,
+ * "Instant Neural Graphics Primitives", Sec. 3): per level, the grid
+ * position fmaf(x, scale, 0.5) in FP32, its floor and fraction, the 8
+ * surrounding vertices, each mapped to an entry by the dense linear index
+ * x + y res + z res^2 (dense levels) or by the spatial hash
+ * x * 1 ^ y * 2654435761 ^ z * 805459861 (hashed levels), modulo the level
+ * size; trilinear weights and the blend in FP64.  indices: [L][8] (may be
+ * NULL), features: [L * F]. */
+void orc_hash_encode(const orc_scene *sc, const float pos[3], uint32_t *indices,
+                     double *features) {
+  const int L = sc->n_levels, F = sc->n_features;
+  for (int l = 0; l < L; ++l) {
+    uint32_t g[3];
+    float fr[3];
+    for (int d = 0; d < 3; ++d) {
+      const float p = fmaf(pos[d], sc->scale[l], 0.5f);
+      const float fl = floorf(p);
+      g[d] = (uint32_t)(int)fl;
+      fr[d] = p - fl;
+    }
+    for (int j = 0; j < F; ++j) features[l * F + j] = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t cx = g[0] + (c & 1), cy = g[1] + ((c >> 1) & 1), cz = g[2] + ((c >> 2) & 1);
+      uint32_t idx;
+      if (sc->dense[l]) {
+        const uint32_t r = sc->resolution[l];
+        idx = (cx + cy * r + cz * r * r) % sc->size[l];
+      } else {
+        idx = (cx ^ (cy * 2654435761u) ^ (cz * 805459861u)) % sc->size[l];
+      }
+      const double wx = (c & 1) ? (double)fr[0] : 1.0 - (double)fr[0];
+      const double wy = (c & 2) ? (double)fr[1] : 1.0 - (double)fr[1];
+      const double wz = (c & 4) ? (double)fr[2] : 1.0 - (double)fr[2];
+      const double w = wx * wy * wz;
+      const float *e = sc->table + ((size_t)sc->offset[l] + idx) * F;
+      for (int j = 0; j < F; ++j) features[l * F + j] += w * (double)e[j];
+      if (indices) indices[l * 8 + c] = idx;
+    }
+  }
+}
+
+/* ----------------------------------------------------------------- haar -- */
+
+/* One 2x2 step of the orthonormal non-standard Haar transform; a = x[2i][2j],
+ * b = x[2i][2j+1], c = x[2i+1][2j], d = x[2i+1][2j+1].  Row step then
+ * column step with 1/sqrt(2) each == division by 2.  The operation order is
+ * part of the specification (the GPU kernel uses the same). */
+static void haar2x2(double a, double b, double c, double d, double *s, double *h, double *v,
+                    double *dg) {
+  *s = ((a + b) + (c + d)) * 0.5;
+  *h = ((a - b) + (c - d)) * 0.5;
+  *v = ((a + b) - (c + d)) * 0.5;
+  *dg = ((a - b) - (c - d)) * 0.5;
+}
+
+/* Non-standard (Mallat pyramid) 2-D Haar transform of one N x N face
+ * channel, elements `stride` floats apart, row 0 = bottom row (as faces are
+ * stored after the reference's row flip, Renderer/HDRloader.h:21-25).  At a
+ * level whose LL block is s x s (hs = s/2) the details of block (i, j) go to
+ * (i, hs+j), (hs+i, j), (hs+i, hs+j); the last LL (face sum / N) goes to
+ * (0, 0).  Scaling coefficients stay FP64; each output is rounded to FP32
+ * once. */
+void orc_haar_face(const float *src, int n, int stride, float *dst) {
+  const int half = n / 2;
+  double *cur = (double *)malloc(sizeof(double) * (size_t)half * half);
+  double *nxt = (double *)malloc(sizeof(double) * (size_t)half * half);
+  double s, h, v, dg;
+  for (int i = 0; i < half; ++i)
+    for (int j = 0; j < half; ++j) {
+      haar2x2(src[((size_t)(2 * i) * n + 2 * j) * stride], src[((size_t)(2 * i) * n + 2 * j + 1) * stride],
+              src[((size_t)(2 * i + 1) * n + 2 * j) * stride],
+              src[((size_t)(2 * i + 1) * n + 2 * j + 1) * stride], &s, &h, &v, &dg);
+      cur[i * half + j] = s;
+      dst[((size_t)i * n + half + j) * stride] = (float)h;
+      dst[((size_t)(half + i) * n + j) * stride] = (float)v;
+      dst[((size_t)(half + i) * n + half + j) * stride] = (float)dg;
+    }
+  for (int sz = half; sz >= 2; sz /= 2) {
+    const int hs = sz / 2;
+    for (int i = 0; i < hs; ++i)
+      for (int j = 0; j < hs; ++j) {
+        haar2x2(cur[(2 * i) * half + 2 * j], cur[(2 * i) * half + 2 * j + 1],
+                cur[(2 * i + 1) * half + 2 * j], cur[(2 * i + 1) * half + 2 * j + 1], &s, &h, &v,
+                &dg);
+        nxt[i * half + j] = s;
+        dst[((size_t)i * n + hs + j) * stride] = (float)h;
+        dst[((size_t)(hs + i) * n + j) * stride] = (float)v;
+        dst[((size_t)(hs + i) * n + hs + j) * stride] = (float)dg;
+      }
+    double *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  dst[0] = (float)cur[0];
+  free(cur);
+  free(nxt);
+}
+
+/* faces / coef: [6][N][N][3]. */
+void orc_envmap_haar(const float *faces, int n, float *coef) {
+  for (int f = 0; f < 6; ++f)
+    for (int c = 0; c < 3; ++c)
+      orc_haar_face(faces + (size_t)f * n * n * 3 + c, n, 3, coef + (size_t)f * n * n * 3 + c);
+}
+
+/* ---------------------------------------------------------------- top-K -- */
+
+/* Area-weighted score of coefficient t: |luminance| with the weights and the
+ * unfused FP32 evaluation of Renderer/EnvironmentMap.cpp:149-151, times the
+ * relative support area of its basis function (4^-j for a detail at Mallat
+ * position with max(row, col) in [2^j, 2^(j+1)), 1 for the DC term).
+ * Subnormal, zero and NaN scores are 0. */
+float orc_score(const float *coef, int t, int n) {
+  const float r = coef[3 * (size_t)t], g = coef[3 * (size_t)t + 1], b = coef[3 * (size_t)t + 2];
+  const float lum = (r * 0.212671f + g * 0.715160f) + b * 0.072169f;
+  const int q = t % (n * n);
+  const int row = q / n, col = q % n;
+  const int m = row > col ? row : col;
+  int j = 0;
+  while ((2 << j) <= m) ++j;
+  float s = ldexpf(fabsf(lum), -2 * j);
+  if (!(s >= FLT_MIN)) s = 0.0f;
+  return s;
+}
+
+typedef struct {
+  float score;
+  int32_t index;
+} scored;
+
+static int by_score_then_index(const void *pa, const void *pb) {
+  const scored *a = (const scored *)pa, *b = (const scored *)pb;
+  if (a->score != b->score) return a->score > b->score ? -1 : 1;
+  return a->index < b->index ? -1 : (a->index > b->index);
+}
+
+/* The K largest scores, ties to the lower flat index, in that order;
+ * strengths are the RGB coefficients at those indices (the layout read at
+ * Renderer/EnvironmentMap.cpp:56-67). */
+void orc_topk(const float *coef, int n, int k, int32_t *indices, float *strengths) {
+  const int m = 6 * n * n;
+  scored *all = (scored *)malloc(sizeof(scored) * (size_t)m);
+  for (int t = 0; t < m; ++t) {
+    all[t].score = orc_score(coef, t, n);
+    all[t].index = t;
+  }
+  qsort(all, (size_t)m, sizeof(scored), by_score_then_index);
+  for (int i = 0; i < k; ++i) {
+    indices[i] = all[i].index;
+    for (int c = 0; c < 3; ++c) strengths[3 * i + c] = coef[3 * (size_t)all[i].index + c];
+  }
+  free(all);
+}
+
+/* -------------------------------------------------------------- relight -- */
+
+static double softplus_d(double x) { return x > 0 ? x + log1p(exp(-x)) : log1p(exp(x)); }
+
+/* Relight of selected pixels in FP64.  Per pixel p with env_flag set
+ * (Renderer/tinycudann/utils_kernels.h:72):
+ *   h = hash(pos) (orc_hash_encode), h' = proj h (or h)
+ *   for each selected wavelet k with flat index c_k
+ *     z = h' (.) wfeat[:, c_k]              (utils_kernels.h:113-114)
+ *     T = out_act(W_out relu(... relu(W1 z)))
+ *     rgb += max(light[k] * T, 0) per channel (utils_kernels.h:74-78)
+ * indirect = rgb, with_direct = rgb + direct (utils_kernels.h:80-88).
+ * Background pixels give 0.  transport (optional): [n][K][3]. */
+void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *light, int k,
+                        const float *position, const uint8_t *env_flag, const float *direct,
+                        const int32_t *pixels, int n_pixels, double *indirect,
+                        double *with_direct, double *transport, int n_threads) {
+  const int Fd = sc->feature_dim, H = sc->hidden, NH = sc->n_hidden_layers;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads > 0 ? n_threads : 1)
+  for (int q = 0; q < n_pixels; ++q) {
+    const int p = pixels ? pixels[q] : q;
+    double rgb[3] = {0, 0, 0};
+    const int hit = env_flag == NULL || env_flag[p] != 0;
+    if (hit) {
+      double feat[64], hp[64], z[64], a[128], b[128];
+      orc_hash_encode(sc, position + 4 * (size_t)p, NULL, feat);
+      for (int o = 0; o < Fd; ++o) {
+        if (sc->proj) {
+          double acc = 0;
+          for (int i = 0; i < Fd; ++i) acc += (double)sc->proj[o * Fd + i] * feat[i];
+          hp[o] = acc;
+        } else {
+          hp[o] = feat[o];
+        }
+      }
+      for (int kk = 0; kk < k; ++kk) {
+        const int c = sel[kk];
+        for (int f = 0; f < Fd; ++f) z[f] = hp[f] * (double)sc->wfeat[(size_t)f * sc->n_coef + c];
+        const float *W = sc->layers;
+        for (int o = 0; o < H; ++o) {
+          double acc = 0;
+          for (int i = 0; i < Fd; ++i) acc += (double)W[o * Fd + i] * z[i];
+          a[o] = acc > 0 ? acc : 0;
+        }
+        W += (size_t)H * Fd;
+        for (int l = 1; l < NH; ++l) {
+          for (int o = 0; o < H; ++o) {
+            double acc = 0;
+            for (int i = 0; i < H; ++i) acc += (double)W[o * H + i] * a[i];
+            b[o] = acc > 0 ? acc : 0;
+          }
+          memcpy(a, b, sizeof(double) * (size_t)H);
+          W += (size_t)H * H;
+        }
+        for (int ch = 0; ch < 3; ++ch) {
+          double acc = 0;
+          for (int i = 0; i < H; ++i) acc += (double)W[ch * H + i] * a[i];
+          const double t = sc->softplus ? softplus_d(acc) : acc;
+          const double term = (double)light[3 * kk + ch] * t;
+          rgb[ch] += term > 0 ? term : 0;
+          if (transport) transport[((size_t)q * k + kk) * 3 + ch] = t;
+        }
+      }
+    } else if (transport) {
+      for (int kk = 0; kk < 3 * k; ++kk) transport[(size_t)q * k * 3 + kk] = 0;
+    }
+    for (int ch = 0; ch < 3; ++ch) {
+      indirect[3 * q + ch] = rgb[ch];
+      with_direct[3 * q + ch] = rgb[ch] + (double)direct[4 * (size_t)p + ch];
+    }
+  }
+}
+
+/* --------------------------------------------------------------- tonemap -- */
+
+/* Renderer/toneMap.cu:58-70 in FP32: y = 1.055 * (x + 1e-6)^(1/2.4) - 0.055
+ * (one fused multiply-add, as a CUDA build contracts it), clamped to [0, 1]
+ * with NaN -> 0, byte = (uint)(y * 255.9); R | G << 8 | B << 16 | 255 << 24. */
+uint32_t orc_tonemap(float r, float g, float b) {
+  const float in[3] = {r, g, b};
+  uint32_t out = 255u << 24;
+  for (int c = 0; c < 3; ++c) {
+    float y = fmaf(1.055f, powf(in[c] + 1e-6f, 1.0f / 2.4f), -(1.055f - 1.0f));
+    y = fminf(1.0f, fmaxf(0.0f, y));
+    out |= ((uint32_t)(y * 255.9f)) << (8 * c);
+  }
+  return out;
+}

@@ -1,0 +1,86 @@
+/*
+ * oracle.h -- CPU restatement of the neural wavelet PRT relighting path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it.  The
+ * product (libnwprt) never links, calls or falls back to it.
+ *
+ * Arithmetic is FP64 except where the reference path fixes an FP32 result
+ * that later steps depend on bit for bit (hash-grid positions, scores of the
+ * top-K selection, the tone map's byte quantisation); those follow the FP32
+ * operation order stated at each function.  Citations are to the reference
+ * repository (paths relative to its root).
+ *
+ * Parity status of each function (how it is pinned independently of itself):
+ *   orc_hashgrid_levels  committed level-table fixture + FP64 evaluation
+ *                        (tests/test_oracle_hashgrid.py)
+ *   orc_hash_encode      brute-force trilinear interpolation of a dense grid,
+ *                        one-hot weights on grid vertices, the published
+ *                        spatial hash on hand-computed inputs
+ *   orc_haar_face        closed forms (constant face, delta), Parseval,
+ *                        inverse round trip, pywt-free textbook recursion
+ *   orc_topk             full sort on (score desc, index asc), ties, K = all
+ *   orc_relight_pixels   brute-force numpy FP64 network on tiny cases,
+ *                        closed forms (zero weights -> softplus(0) = ln 2,
+ *                        background -> 0, negative light -> clamped)
+ *   orc_tonemap          closed forms (0 -> 0, 1 -> 255, alpha 255, byte order)
+ * The tiny-cuda-nn conventions behind the hash grid and the MLP (level
+ * rounding, layer layout, padding values) cannot be pinned against the
+ * reference: that library is an empty submodule there ("parity unpinned"
+ * for those conventions; see the design notes).
+ */
+#ifndef NWPRT_ORACLE_H
+#define NWPRT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int n_levels, n_features;   /* L, F; L * F = feature_dim           */
+  const float *scale;         /* [L] FP32 level scales                */
+  const uint32_t *resolution; /* [L]                                  */
+  const uint32_t *dense;      /* [L]                                  */
+  const uint32_t *size;       /* [L]                                  */
+  const uint32_t *offset;     /* [L]                                  */
+  const float *table;         /* entries x F (FP16 tables pre-widened) */
+  int n_coef;                 /* 6 N^2                                */
+  const float *wfeat;         /* [feature_dim][n_coef]                */
+  const float *proj;          /* [feature_dim][feature_dim] or NULL   */
+  const float *layers;        /* W1 [H][F], W2, W3 [H][H], W4 [3][H]  */
+  int feature_dim, hidden;    /* 32, 64                               */
+  int n_hidden_layers;        /* 3                                    */
+  int softplus;
+} orc_scene;
+
+void orc_hashgrid_levels(int n_levels, int log2_hashmap_size, int base_resolution,
+                         float per_level_scale, float *scale, uint32_t *resolution,
+                         uint32_t *dense, uint32_t *size, uint32_t *offset);
+
+void orc_hash_encode(const orc_scene *sc, const float pos[3], uint32_t *indices,
+                     double *features);
+
+void orc_haar_face(const float *src, int n, int stride, float *dst);
+void orc_envmap_haar(const float *faces, int n, float *coef);
+float orc_score(const float *coef, int t, int n);
+void orc_topk(const float *coef, int n, int k, int32_t *indices, float *strengths);
+
+void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *light, int k,
+                        const float *position, const uint8_t *env_flag, const float *direct,
+                        const int32_t *pixels, int n_pixels, double *indirect,
+                        double *with_direct, double *transport, int n_threads);
+
+uint32_t orc_tonemap(float r, float g, float b);
+
+/* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
+double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
+                            const float *position, const uint8_t *env_flag,
+                            const float *direct, int pixel_begin, int n_pixels, float *indirect,
+                            int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,246 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars: bit-exact for Haar coefficients, top-K index sets (and strengths) and
+hash-grid entry indices; radiance within the tolerances stated per test
+against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from cases import make_case
+from conftest import max_rel_err, psnr
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances (radiance vs the FP64 oracle, per pixel and channel, relative
+# to max(|ref|, 1e-3 * max|ref|)).
+TOL_FP32 = 1e-4
+TOL_HALF = {"bf16": 2e-2, "fp16": 5e-3}
+MIN_PSNR_DB = 50.0
+
+
+def _gpu_relight(nw, ctx, case, precision, env=None, scene=None, tonemap=None):
+    if scene is None:
+        scene = ctx.scene_create(case.desc, case.params)
+    if env is None:
+        env = ctx.envmap_from_faces(case.faces, case.k)
+    frame = nw.DeviceFrame(case.width, case.height, case.gbuf)
+    ctx.relight(scene, env, frame, nw.PRECISIONS[precision],
+                nw.TONEMAP_FULL if tonemap is None else tonemap)
+    ctx.sync()
+    return scene, env, frame, frame.results()
+
+
+@pytest.mark.parametrize("n", [2, 4, 32, 64, 128])
+def test_haar_and_topk_bit_exact(nw, orc, ctx, n):
+    faces = nw.synth_cubemap(7 + n, n) if n >= 4 else np.random.default_rng(n).lognormal(
+        0, 1.5, (6, n, n, 3)).astype(np.float32)
+    k = min(300, 6 * n * n)
+    env = ctx.envmap_from_faces(faces, k)
+    coef = ctx.envmap_coefficients(env)
+    ref = orc.envmap_haar(faces)
+    assert np.array_equal(coef.view(np.uint32), ref.view(np.uint32))
+    idx, st = ctx.envmap_selection(env)
+    ridx, rst = orc.topk(ref, k)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(st.view(np.uint32), rst.view(np.uint32))
+    ctx.envmap_destroy(env)
+
+
+def test_topk_ties_break_by_index(nw, orc, ctx):
+    n = 16
+    faces = np.ones((6, n, n, 3), np.float32)  # all DCs equal, all details 0
+    faces[2, 3, 5] = 4.0                          # one bump on face 2
+    for k in (1, 6, 7, 40, 6 * n * n):
+        env = ctx.envmap_from_faces(faces, min(k, 1024))
+        idx, _ = ctx.envmap_selection(env)
+        ridx, _ = orc.topk(orc.envmap_haar(faces), min(k, 1024))
+        assert np.array_equal(idx, ridx), k
+        ctx.envmap_destroy(env)
+
+
+def test_hash_indices_bit_exact(nw, orc, ctx):
+    case = make_case(nw, orc, width=64, height=32, hash_dtype=nw.DTYPE_F16)
+    scene = ctx.scene_create(case.desc, case.params)
+    pos = case.gbuf["position"].copy()
+    special = np.array([[0, 0, 0, 1], [1, 1, 1, 1], [0.5, 0.5, 0.5, 1], [1, 0, 0.25, 1],
+                        [1 / 15, 2 / 15, 3 / 15, 1], [0.999999, 1e-7, 0.5, 1]], np.float32)
+    pos[:len(special)] = special
+    gidx, gfeat = ctx.debug_hash_encode(scene, pos)
+    for i in range(0, len(pos), 7):
+        ridx, rfeat = case.oscene.hash_encode(pos[i, :3])
+        assert np.array_equal(gidx[i], ridx), i
+        np.testing.assert_allclose(gfeat[i], rfeat, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_relight_config0_vs_oracle(nw, orc, ctx, precision):
+    """128 x 128, N = 32, K = 64 (the CPU-runnable configuration)."""
+    case = make_case(nw, orc)
+    scene, env, frame, res = _gpu_relight(nw, ctx, case, precision)
+    idx, st = ctx.envmap_selection(env)
+    ind, wd = case.oscene.relight(idx, st, case.gbuf)
+    got = res["indirect"][:, :3]
+    err = max_rel_err(ind, got)
+    p = psnr(ind, got)
+    print(f"{precision}: max rel err {err:.3e}, PSNR {p:.1f} dB")
+    assert np.all(res["indirect"][:, 3] == 1.0) and np.all(res["indirect_with_direct"][:, 3] == 1.0)
+    np.testing.assert_allclose(res["indirect_with_direct"][:, :3],
+                               got + case.gbuf["direct"][:, :3], rtol=1e-6, atol=1e-6)
+    if precision == "fp32":
+        assert err <= TOL_FP32
+    else:
+        assert err <= TOL_HALF[precision]
+        assert p >= MIN_PSNR_DB
+    # RGBA8 is the tone map of the with-direct output.
+    tm = orc.tonemap(res["indirect_with_direct"][:, :3])
+    diff = np.abs(tm.view(np.uint8).astype(int) - res["rgba8"].view(np.uint8).astype(int))
+    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
+def test_transport_amplified_vs_oracle(nw, orc, ctx, precision):
+    """Per-pair transport with unit-scale features, so every layer matters."""
+    case = make_case(nw, orc, width=32, height=16, k=40, hash_std=1.0, feature_std=1.0, seed=3)
+    scene = ctx.scene_create(case.desc, case.params)
+    env = ctx.envmap_from_faces(case.faces, case.k)
+    frame = nw.DeviceFrame(case.width, case.height, case.gbuf)
+    begin, count = 100, 200
+    t_gpu = ctx.debug_transport(scene, env, frame, begin, count, nw.PRECISIONS[precision])
+    idx, st = ctx.envmap_selection(env)
+    _, _, t_ref = case.oscene.relight(idx, st, case.gbuf, pixels=np.arange(begin, begin + count),
+                                      transport=True)
+    spread = float(np.std(t_ref))
+    err = np.abs(t_gpu - t_ref)
+    print(f"{precision}: T spread {spread:.3f}, max |dT| {err.max():.3e}")
+    assert spread > 0.1  # the network is not near-constant in this case
+    tol = {"fp32": 1e-4, "bf16": 0.05, "fp16": 0.01}[precision]
+    assert err.max() <= tol * max(1.0, float(np.abs(t_ref).max()))
+
+
+def test_background_pixels(nw, orc, ctx):
+    case = make_case(nw, orc, width=40, height=30, background=0.4)
+    for precision in ("fp32", "bf16"):
+        _, _, _, res = _gpu_relight(nw, ctx, case, precision)
+        bg = case.gbuf["env_flag"] == 0
+        assert bg.any() and (~bg).any()
+        assert np.all(res["indirect"][bg, :3] == 0.0)
+        assert np.array_equal(res["indirect_with_direct"][bg, :3], case.gbuf["direct"][bg, :3])
+        assert np.all(res["indirect"][~bg, :3] > 0.0)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (37, 13), (65, 3), (300, 1)])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_ragged_frames(nw, orc, ctx, shape, precision):
+    w, h = shape
+    case = make_case(nw, orc, width=w, height=h, k=17, face_size=8, seed=w * h)
+    _, env, _, res = _gpu_relight(nw, ctx, case, precision)
+    idx, st = ctx.envmap_selection(env)
+    ind, _ = case.oscene.relight(idx, st, case.gbuf)
+    tol = TOL_FP32 if precision == "fp32" else TOL_HALF[precision]
+    assert max_rel_err(ind, res["indirect"][:, :3]) <= tol
+
+
+def test_split_passes_match_relight(nw, orc, ctx):
+    case = make_case(nw, orc, width=96, height=40)
+    scene, env, frame, res = _gpu_relight(nw, ctx, case, "bf16")
+    frame2 = nw.DeviceFrame(case.width, case.height, case.gbuf)
+    ctx.TCNNPass_p1(scene, 1, frame2, nw.PREC_BF16)
+    ctx.TCNNPass_p2(scene, env, 1, frame2, nw.PREC_BF16)
+    ctx.sync()
+    res2 = frame2.results()
+    for key in res:
+        assert np.array_equal(res[key], res2[key]), key
+    # p2 on a slot encoded for another precision is a call-order error.
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.TCNNPass_p2(scene, env, 1, frame2, nw.PREC_FP32)
+    assert e.value.status == nw.ERR_STATE
+
+
+def test_selection_path_matches_faces_path(nw, orc, ctx):
+    case = make_case(nw, orc, width=64, height=64)
+    scene, env, _, res = _gpu_relight(nw, ctx, case, "bf16")
+    idx, st = orc.topk(orc.envmap_haar(case.faces), case.k)
+    env2 = ctx.envmap_from_selection(idx, st, case.desc.face_size)
+    _, _, _, res2 = _gpu_relight(nw, ctx, case, "bf16", env=env2, scene=scene)
+    for key in res:
+        assert np.array_equal(res[key], res2[key]), key
+
+
+def test_light_scaling_is_exact(nw, orc, ctx):
+    """Doubling every light coefficient doubles each clamped term exactly in
+    FP32, so the compensated sum doubles bit for bit (any size)."""
+    case = make_case(nw, orc, width=256, height=64, k=96)
+    scene = ctx.scene_create(case.desc, case.params)
+    idx, st = orc.topk(orc.envmap_haar(case.faces), case.k)
+    e1 = ctx.envmap_from_selection(idx, st, case.desc.face_size)
+    e2 = ctx.envmap_from_selection(idx, st * 2, case.desc.face_size)
+    for precision in ("fp32", "bf16", "fp16"):
+        _, _, _, r1 = _gpu_relight(nw, ctx, case, precision, env=e1, scene=scene)
+        _, _, _, r2 = _gpu_relight(nw, ctx, case, precision, env=e2, scene=scene)
+        assert np.array_equal(r1["indirect"][:, :3] * 2, r2["indirect"][:, :3]), precision
+
+
+def test_tonemap_entry_point(nw, orc, ctx):
+    rng = np.random.default_rng(5)
+    n = 4096
+    src = np.zeros((n, 4), np.float32)
+    src[:, :3] = rng.lognormal(-1, 1.5, (n, 3))
+    src[:8, :3] = [[0, 0, 0], [1, 1, 1], [1e6, 0, 0], [0, 1e-7, 2], [0.5, 0.25, 0.125],
+                   [0.0031308, 0.04, 0.9], [10, 10, 10], [1e-30, 0, 0]]
+    ds, dd = nw.DeviceBuffer.from_array(src), nw.DeviceBuffer(n * 4)
+    ctx.computeFinalPixelColors(ds, dd, 64, 64)
+    ctx.sync()
+    got = dd.download(np.uint32, (n,))
+    ref = orc.tonemap(src[:, :3])
+    diff = np.abs(got.view(np.uint8).astype(int) - ref.view(np.uint8).astype(int))
+    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    assert got[0] == 0xFF000000 and got[1] == 0xFFFFFFFF
+
+
+def test_error_paths(nw, orc, ctx):
+    case = make_case(nw, orc, width=8, height=8, k=8, face_size=8)
+    scene = ctx.scene_create(case.desc, case.params)
+    env = ctx.envmap_from_faces(case.faces, 8)
+    frame = nw.DeviceFrame(8, 8, case.gbuf)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.relight(scene + 1000, env, frame)
+    assert e.value.status == nw.ERR_NOT_FOUND
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.relight(scene, env, frame, nw.PREC_FP8)
+    assert e.value.status == nw.ERR_UNSUPPORTED
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.envmap_from_faces(case.faces, 6 * 8 * 8 + 1)
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.envmap_from_selection(np.array([6 * 64], np.int32), np.ones((1, 3), np.float32), 8)
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    other = ctx.envmap_from_faces(nw.synth_cubemap(1, 16), 8)
+    with pytest.raises(nw.NwprtError) as e:  # face size differs from the scene's table
+        ctx.relight(scene, other, frame)
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    with pytest.raises(nw.NwprtError) as e:  # slot 0 never encoded
+        ctx.TCNNPass_p2(scene, env, 0, frame)
+    assert e.value.status == nw.ERR_STATE
+
+
+def test_config1_fp16_1080p(nw, orc, ctx):
+    """1920 x 1080, N = 64, K = 128, FP16 hash grid and MLP: bit-exact hash
+    indices and top-K set, radiance on a pixel sample within tolerance."""
+    case = make_case(nw, orc, width=1920, height=1080, face_size=64, k=128,
+                     hash_dtype=nw.DTYPE_F16, seed=1)
+    scene, env, frame, res = _gpu_relight(nw, ctx, case, "fp16")
+    idx, st = ctx.envmap_selection(env)
+    ridx, rst = orc.topk(orc.envmap_haar(case.faces), case.k)
+    assert np.array_equal(idx, ridx)
+    rng = np.random.default_rng(11)
+    pix = rng.choice(case.width * case.height, 1500, replace=False)
+    gidx, _ = ctx.debug_hash_encode(scene, case.gbuf["position"][pix])
+    for j in range(0, len(pix), 25):
+        ridx_h, _ = case.oscene.hash_encode(case.gbuf["position"][pix[j], :3])
+        assert np.array_equal(gidx[j], ridx_h)
+    ind, _ = case.oscene.relight(idx, st, case.gbuf, pixels=pix)
+    got = res["indirect"][pix, :3]
+    err, p = max_rel_err(ind, got), psnr(ind, got)
+    print(f"config1 fp16: max rel err {err:.3e}, PSNR {p:.1f} dB")
+    assert err <= TOL_HALF["fp16"] and p >= MIN_PSNR_DB
