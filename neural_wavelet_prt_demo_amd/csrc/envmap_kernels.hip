@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restric
     }
   }
   for (int i = tid; i < K; i += kTopkBlock) {
-    const uint32_t idx = 0xFFFFFFFFu - (uint32_t)(sel[i] & 0xFFFFFFFFull);
+    uint32_t idx = 0xFFFFFFFFu - (uint32_t)(sel[i] & 0xFFFFFFFFull);
+    if (idx >= (uint32_t)M) idx = 0;  // unreachable: keys are unique, exactly K are kept
     out_idx[i] = (int32_t)idx;
     for (int c = 0; c < 3; ++c) out_str[3 * i + c] = coef[3 * (size_t)idx + c];
   }

@@ -1,0 +1,279 @@
+"""CPU tests of the C ABI surface and the on-disk formats.
+
+The .npy and .hdr readers are held against numpy and against the
+reference's own readers (Renderer/npy.h, Renderer/HDRloader.cpp) built
+under oracle/_ref from the reference sources; those tests skip where that
+checker is not built.
+"""
+import re
+import struct
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+REF_BIN = ROOT / "oracle" / "_ref" / "ref_formats"
+needs_ref = pytest.mark.skipif(not REF_BIN.exists(), reason="oracle/_ref checker not built")
+
+
+def test_library_exports_every_declared_symbol(nw):
+    header = (ROOT / "include" / "nwprt.h").read_text()
+    declared = set(re.findall(r"\b(nwprt_[a-z0-9_]+)\s*\(", header))
+    assert len(declared) >= 30
+    lib = nw.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+        assert name in nw._SIGNATURES, f"{name} missing from the Python binding"
+    assert lib.nwprt_abi_version() == 1
+
+
+def test_status_strings_and_last_error(nw):
+    assert nw.lib().nwprt_status_string(nw.ERR_STATE) == b"invalid call order"
+    with pytest.raises(nw.NwprtError) as e:
+        nw.npy_read("/nonexistent/file.npy")
+    assert e.value.status == nw.ERR_IO
+    assert b"cannot open" in nw.lib().nwprt_last_error()
+
+
+# ------------------------------------------------------------------- npy --
+
+@pytest.mark.parametrize("dtype", ["<f4", "<f8", "<f2", "<i4", "<i8", "|u1"])
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_npy_read_matches_numpy(nw, tmp_path, dtype, order):
+    rng = np.random.default_rng(0)
+    a = (rng.standard_normal((5, 7, 3)) * 50).astype(np.dtype(dtype))
+    a = np.asfortranarray(a) if order == "F" else a
+    p = tmp_path / "a.npy"
+    np.save(p, a)
+    vals, shape, fortran = nw.npy_read(p, np.float32)
+    assert shape == (5, 7, 3) and fortran == (order == "F")
+    flat = a.ravel(order="K" if order == "C" else "F")  # file order
+    np.testing.assert_array_equal(vals, flat.astype(np.float64).astype(np.float32))
+    if np.dtype(dtype).kind in "iu":
+        ivals, _, _ = nw.npy_read(p, np.int32)
+        np.testing.assert_array_equal(ivals, flat.astype(np.int32))
+
+
+def test_npy_write_roundtrip(nw, tmp_path):
+    a = np.arange(24, dtype=np.float32).reshape(2, 3, 4) / 7
+    nw.npy_write(tmp_path / "f.npy", a)
+    np.testing.assert_array_equal(np.load(tmp_path / "f.npy"), a)
+    b = np.arange(-5, 5, dtype=np.int32)
+    nw.npy_write(tmp_path / "i.npy", b)
+    np.testing.assert_array_equal(np.load(tmp_path / "i.npy"), b)
+
+
+def test_npy_malformed(nw, tmp_path):
+    p = tmp_path / "bad.npy"
+    p.write_bytes(b"NOTNUMPY" + b"\0" * 40)
+    with pytest.raises(nw.NwprtError) as e:
+        nw.npy_read(p)
+    assert e.value.status == nw.ERR_FORMAT
+    good = tmp_path / "g.npy"
+    np.save(good, np.ones(100, np.float32))
+    p.write_bytes(good.read_bytes()[:-17])  # truncated payload
+    with pytest.raises(nw.NwprtError) as e:
+        nw.npy_read(p)
+    assert e.value.status == nw.ERR_FORMAT
+    np.save(good, np.ones(3, np.float32).astype(">f4"))
+    with pytest.raises(nw.NwprtError) as e:
+        nw.npy_read(good)
+    assert e.value.status == nw.ERR_UNSUPPORTED
+
+
+def _ref_npy(path, mode):
+    out = Path(str(path) + ".bin")
+    subprocess.run([str(REF_BIN), mode, str(path), str(out)], check=True)
+    b = out.read_bytes()
+    nd, fo = struct.unpack("<ii", b[:8])
+    shape = struct.unpack(f"<{nd}Q", b[8:8 + 8 * nd])
+    data = np.frombuffer(b[8 + 8 * nd:], np.float32 if mode == "npyf" else np.int32)
+    return data, tuple(shape), bool(fo)
+
+
+@needs_ref
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_npy_matches_reference_reader(nw, tmp_path, order):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((64, 3)).astype(np.float32)  # a waveletstrength-like array
+    a = np.asfortranarray(a) if order == "F" else a
+    np.save(tmp_path / "s.npy", a)
+    ref, rshape, rfo = _ref_npy(tmp_path / "s.npy", "npyf")
+    got, shape, fo = nw.npy_read(tmp_path / "s.npy")
+    assert shape == rshape and fo == rfo
+    np.testing.assert_array_equal(got, ref)
+    i = rng.integers(0, 24576, 64).astype(np.int32)  # a waveletcoeffs-like array
+    np.save(tmp_path / "c.npy", i)
+    ref_i, _, _ = _ref_npy(tmp_path / "c.npy", "npyi")
+    got_i, _, _ = nw.npy_read(tmp_path / "c.npy", np.int32)
+    np.testing.assert_array_equal(got_i, ref_i)
+
+
+# ------------------------------------------------------------------- hdr --
+
+def _rgbe_encode(img):
+    """Ward's float -> RGBE (test-side, numpy)."""
+    v = img.max(axis=-1)
+    m, e = np.frexp(v)
+    scale = np.where(v > 1e-32, m * 256.0 / np.where(v > 0, v, 1), 0)
+    rgb = np.floor(img * scale[..., None]).astype(np.uint8)
+    ex = np.where(v > 1e-32, e + 128, 0).astype(np.uint8)
+    return np.concatenate([rgb, ex[..., None]], axis=-1)
+
+
+def _write_hdr(path, rgbe, mode):
+    """Writes RGBE scanlines flat, with new-style RLE, or with old-style runs."""
+    h, w, _ = rgbe.shape
+    out = bytearray(b"#?RADIANCE\n# test\nFORMAT=32-bit_rle_rgbe\n\n")
+    out += f"-Y {h} +X {w}\n".encode()
+    for y in range(h):
+        row = rgbe[y]
+        if mode == "flat":
+            out += row.tobytes()
+        elif mode == "rle":
+            out += bytes([2, 2, w >> 8, w & 255])
+            for c in range(4):
+                comp = row[:, c]
+                i = 0
+                while i < w:
+                    j = i
+                    while j < w and j - i < 127 and comp[j] == comp[i]:
+                        j += 1
+                    if j - i >= 3:
+                        out += bytes([128 + (j - i), comp[i]])
+                        i = j
+                    else:
+                        k = i
+                        while k < w and k - i < 128 and not (k + 2 < w and comp[k] == comp[k + 1] == comp[k + 2]):
+                            k += 1
+                        k = max(k, i + 1)
+                        out += bytes([k - i]) + comp[i:k].tobytes()
+                        i = k
+        elif mode == "old":
+            x = 0
+            while x < w:
+                out += row[x].tobytes()
+                run = 1
+                while x + run < w and run < 255 and np.array_equal(row[x + run], row[x]):
+                    run += 1
+                if run > 1:
+                    out += bytes([1, 1, 1, run - 1])
+                x += run
+    Path(path).write_bytes(bytes(out))
+
+
+def _decode_ref_formula(rgbe):
+    e = rgbe[..., 3].astype(np.int32) - 128
+    return (rgbe[..., :3].astype(np.float32) / np.float32(256.0)) * np.ldexp(
+        np.float32(1.0), e)[..., None].astype(np.float32)
+
+
+@pytest.mark.parametrize("mode", ["flat", "rle", "old"])
+def test_hdr_read_decodes_like_reference_formula(nw, tmp_path, mode):
+    rng = np.random.default_rng(2)
+    img = rng.lognormal(0, 2, (12, 24, 3)).astype(np.float32)
+    img[3, 5:15] = img[3, 5]  # runs
+    img[0, 0] = 0
+    rgbe = _rgbe_encode(img)
+    _write_hdr(tmp_path / "x.hdr", rgbe, mode)
+    got = nw.hdr_read(tmp_path / "x.hdr")
+    np.testing.assert_array_equal(got, _decode_ref_formula(rgbe))
+
+
+@needs_ref
+@pytest.mark.parametrize("mode", ["flat", "rle"])
+def test_hdr_matches_reference_loader(nw, tmp_path, mode):
+    rng = np.random.default_rng(3)
+    img = rng.lognormal(0, 2, (24, 32, 3)).astype(np.float32)
+    rgbe = _rgbe_encode(img)
+    _write_hdr(tmp_path / "x.hdr", rgbe, mode)
+    out = tmp_path / "x.bin"
+    subprocess.run([str(REF_BIN), "hdr", str(tmp_path / "x.hdr"), str(out)], check=True)
+    b = out.read_bytes()
+    w, h = struct.unpack("<ii", b[:8])
+    cols = np.frombuffer(b[8:8 + w * h * 12], np.float32).reshape(h, w, 3)
+    flipped = np.frombuffer(b[8 + w * h * 12:], np.float32).reshape(h, w, 3)
+    got = nw.hdr_read(tmp_path / "x.hdr")
+    np.testing.assert_array_equal(got, cols)
+    np.testing.assert_array_equal(got[::-1], flipped)  # HDRLoaderResult::get row flip
+
+
+def test_hdr_write_roundtrip(nw, tmp_path):
+    rng = np.random.default_rng(4)
+    img = rng.lognormal(0, 2, (9, 17, 3)).astype(np.float32)
+    img[0, 0] = 0
+    nw.hdr_write(tmp_path / "w.hdr", img)
+    back = nw.hdr_read(tmp_path / "w.hdr")
+    assert back[0, 0].max() == 0
+    rel = np.abs(back - img) / np.maximum(img.max(axis=-1, keepdims=True), 1e-30)
+    assert rel.max() < 1.0 / 128
+
+
+def test_hdr_malformed(nw, tmp_path):
+    p = tmp_path / "bad.hdr"
+    p.write_bytes(b"#?RADIANCE\n\n-Y 4 +X 4\n" + b"\x01" * 10)
+    with pytest.raises(nw.NwprtError) as e:
+        nw.hdr_read(p)
+    assert e.value.status == nw.ERR_FORMAT
+    p.write_bytes(b"P6\n4 4\n255\n")
+    with pytest.raises(nw.NwprtError) as e:
+        nw.hdr_read(p)
+    assert e.value.status == nw.ERR_FORMAT
+
+
+def test_cross_to_faces_layout(nw):
+    """Faces +X,-X,+Y,-Y,-Z,+Z cut from a 4N x 3N cross at (col, row) blocks
+    (2,1) (0,1) (1,2) (1,0) (1,1) (3,1) counted from the bottom, rows
+    bottom-up (Renderer/EnvironmentMap.cpp:112-126)."""
+    n = 5
+    cross = np.arange(3 * n * 4 * n * 3, dtype=np.float32).reshape(3 * n, 4 * n, 3)
+    faces = nw.cross_to_faces(cross)
+    bottom_up = cross[::-1]
+    blocks = [(2, 1), (0, 1), (1, 2), (1, 0), (1, 1), (3, 1)]
+    for f, (c, r) in enumerate(blocks):
+        np.testing.assert_array_equal(faces[f], bottom_up[r * n:(r + 1) * n, c * n:(c + 1) * n])
+    with pytest.raises(nw.NwprtError):
+        nw.cross_to_faces(np.zeros((9, 10, 3), np.float32))
+
+
+# -------------------------------------------------------------- generator --
+
+def test_generator_distributions(nw):
+    g = nw.synth_gbuffer(0, 128, 96, 0.25)
+    P = 128 * 96
+    pos = g["position"]
+    assert pos.shape == (P, 4) and np.all(pos[:, :3] >= 0) and np.all(pos[:, :3] < 1)
+    assert np.all(pos[:, 3] == 1)
+    n = g["normal"][:, :3] * 2 - 1
+    r = g["reflect"][:, :3] * 2 - 1
+    np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1, atol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(r, axis=1), 1, atol=1e-6)
+    assert np.all(np.sum(n * r, axis=1) >= -1e-6)  # reflection stays in the hemisphere
+    assert abs(np.mean(n[:, 2])) < 0.03  # uniform on the sphere
+    assert np.all(g["roughness"] >= 0.05) and np.all(g["roughness"] < 1)
+    assert 0.22 < np.mean(g["env_flag"] == 0) < 0.28
+    g2 = nw.synth_gbuffer(0, 128, 96, 0.25)
+    for k in g:
+        assert np.array_equal(g[k], g2[k])
+    assert not np.array_equal(nw.synth_gbuffer(1, 128, 96)["position"], pos)
+
+
+def test_cubemap_generator(nw):
+    f = nw.synth_cubemap(3, 64)
+    assert f.shape == (6, 64, 64, 3)
+    assert np.sum(f == 1e3) == 27  # one 3 x 3 sun, all channels
+    rest = np.log(f[f != 1e3])
+    assert abs(rest.mean()) < 0.03 and abs(rest.std() - 1.5) < 0.03
+
+
+def test_scene_generator(nw):
+    d = nw.config_scene_desc(32, nw.DTYPE_F16)
+    p = nw.synth_scene(d, 0)
+    tab = p["hash_table"].view(np.float16).astype(np.float32)
+    assert tab.shape == (6513496, 2) and abs(tab.std() - 0.1) < 1e-3
+    assert abs(p["wavelet_features"].std() - 0.1) < 2e-3
+    w1 = p["layers"][:64 * 32]
+    assert abs(w1.std() - np.sqrt(2 / 32)) < 0.01
+    assert abs(p["projection"].std() - np.sqrt(1 / 32)) < 0.01
