@@ -30,10 +30,36 @@ __device__ __forceinline__ float f16_to_f32(uint16_t h) {
   return (float)__builtin_bit_cast(_Float16, h);
 }
 
-// log(1 + e^x), written as max(x, 0) + log(1 + e^-|x|) so it neither
-// overflows nor loses the small tail.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+
+// ReLU of two floats rounded (RNE) to a packed bf16 / f16 pair: one
+// v_cvt_pk_*_f32 and one v_pk_max_i16.  Rounding commutes with ReLU, and a
+// negative bf16 / f16 value (sign bit set) is a negative int16, so an int16
+// max with 0 clears it to +0 without the NaN canonicalisation an FP max
+// needs.
+template <typename E>
+__device__ __forceinline__ uint32_t cvt_relu2(float a, float b) {
+  const f32x2 v = {a, b};
+  uint32_t r;
+  if constexpr (sizeof(E) == 2 && __is_same(E, __bf16))
+    r = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+  else
+    r = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, f16x2));
+  const i16x2 m = __builtin_elementwise_max(__builtin_bit_cast(i16x2, r), (i16x2){0, 0});
+  return __builtin_bit_cast(uint32_t, m);
+}
+
+// log(1 + e^x) = max(x, 0) + ln(1 + e^-|x|), on the hardware exp2 / log2.
+// For e^-|x| < 2^-10 the log term uses ln(1 + e) ~ e (1 - e/2), which keeps
+// the tail accurate where 1 + e rounds to 1.
 __device__ __forceinline__ float softplus(float x) {
-  return fmaxf(x, 0.0f) + __logf(1.0f + __expf(-fabsf(x)));
+  const float e = __builtin_amdgcn_exp2f(-fabsf(x) * 1.44269504088896341f);
+  const float lg = __builtin_amdgcn_logf(1.0f + e) * 0.693147180559945309f;
+  const float small = e * fmaf(-0.5f, e, 1.0f);
+  return (x > 0.0f ? x : 0.0f) + (e < 0x1p-10f ? small : lg);
 }
 
 // Neumaier-compensated accumulation: (s + c) carries the running sum.

@@ -60,10 +60,10 @@ struct Mfma<_Float16> {
 template <typename E>
 __device__ __forceinline__ u32x4 relu_pack(f32x4 lo, f32x4 hi) {
   u32x4 r;
-  r[0] = Mfma<E>::pack(fmaxf(lo[0], 0.f), fmaxf(lo[1], 0.f));
-  r[1] = Mfma<E>::pack(fmaxf(lo[2], 0.f), fmaxf(lo[3], 0.f));
-  r[2] = Mfma<E>::pack(fmaxf(hi[0], 0.f), fmaxf(hi[1], 0.f));
-  r[3] = Mfma<E>::pack(fmaxf(hi[2], 0.f), fmaxf(hi[3], 0.f));
+  r[0] = dev::cvt_relu2<E>(lo[0], lo[1]);
+  r[1] = dev::cvt_relu2<E>(lo[2], lo[3]);
+  r[2] = dev::cvt_relu2<E>(hi[0], hi[1]);
+  r[3] = dev::cvt_relu2<E>(hi[2], hi[3]);
   return r;
 }
 
@@ -82,8 +82,10 @@ __device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float
 
 // Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
 // pixels [64 (4 b + w), +64).  Pixel groups j = 0..3 (16 pixels each) are the
-// MFMA columns; lane l owns pixel base + l for the reduction.
-template <typename E, bool DUMP>
+// MFMA columns; lane l owns pixel base + l for the reduction.  GP groups are
+// carried through the layers together so that GP independent MFMA chains
+// hide each other's result latency.
+template <typename E, bool SOFTPLUS, bool DUMP, int GP = 2>
 __global__ __launch_bounds__(kRelightBlock, 2) void relight_mfma_kernel(RelightArgs a) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, n = lane & 15;
@@ -131,29 +133,51 @@ __global__ __launch_bounds__(kRelightBlock, 2) void relight_mfma_kernel(RelightA
       const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
       f32x4 out = zero;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        f32x4 c[4];
+      for (int jp = 0; jp < 4; jp += GP) {
+        f32x4 c[GP][4];
+        u32x4 x[GP][2];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) c[b] = Mfma<E>::run(cur[b], hb[j], zero);
-        u32x4 x0 = relu_pack<E>(c[0], c[1]), x1 = relu_pack<E>(c[2], c[3]);
+        for (int j = 0; j < GP; ++j)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) c[b] = Mfma<E>::run(w2[b][1], x1, Mfma<E>::run(w2[b][0], x0, zero));
-        x0 = relu_pack<E>(c[0], c[1]);
-        x1 = relu_pack<E>(c[2], c[3]);
+          for (int b = 0; b < 4; ++b) c[j][b] = Mfma<E>::run(cur[b], hb[jp + j], zero);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) c[b] = Mfma<E>::run(w3[b][1], x1, Mfma<E>::run(w3[b][0], x0, zero));
-        x0 = relu_pack<E>(c[0], c[1]);
-        x1 = relu_pack<E>(c[2], c[3]);
+        for (int j = 0; j < GP; ++j) {
+          x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
+          x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+        }
+#pragma unroll
+        for (int j = 0; j < GP; ++j)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], zero));
+#pragma unroll
+        for (int j = 0; j < GP; ++j) {
+          x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
+          x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+        }
+#pragma unroll
+        for (int j = 0; j < GP; ++j)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            c[j][b] = Mfma<E>::run(w3[b][1], x[j][1], Mfma<E>::run(w3[b][0], x[j][0], zero));
+#pragma unroll
+        for (int j = 0; j < GP; ++j) {
+          x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
+          x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+        }
         // Output layer: variant j carries W4 in rows 4j..4j+2, so group j's
         // three outputs land in rows 4j.. of the shared accumulator, i.e. in
         // lanes with g == j -- the lanes that own those pixels.
-        out = Mfma<E>::run(w4[j][1], x1, Mfma<E>::run(w4[j][0], x0, out));
+#pragma unroll
+        for (int j = 0; j < GP; ++j)
+          out = Mfma<E>::run(w4[jp + j][1], x[j][1], Mfma<E>::run(w4[jp + j][0], x[j][0], out));
       }
       const float lc[3] = {l0, l1, l2};
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        const float t = a.softplus ? dev::softplus(out[ch]) : out[ch];
-        dev::neumaier_add(sum[ch], comp[ch], fmaxf(lc[ch] * t, 0.0f));
+        const float t = SOFTPLUS ? dev::softplus(out[ch]) : out[ch];
+        const float term = lc[ch] * t;
+        dev::neumaier_add(sum[ch], comp[ch], term > 0.0f ? term : 0.0f);
         if (DUMP) {
           const int q = my_px - a.dump_begin;
           if (mine && q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
@@ -247,6 +271,19 @@ __global__ __launch_bounds__(256) void tonemap_kernel(const float *__restrict__ 
   dst[p] = dev::tonemap_rgba8(v.x, v.y, v.z);
 }
 
+template <typename E>
+void launch_mfma_variant(const RelightArgs &a, hipStream_t s) {
+  const dim3 grid((a.n_pixels + kRelightBlock - 1) / kRelightBlock), block(kRelightBlock);
+  const bool dump = a.dump != nullptr;
+  if (a.softplus) {
+    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<E, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((relight_mfma_kernel<E, true, false>), grid, block, 0, s, a);
+  } else {
+    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<E, false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((relight_mfma_kernel<E, false, false>), grid, block, 0, s, a);
+  }
+}
+
 }  // namespace
 
 // A-operand tiles of layers 2, 3 (64 x 64) and the output layer (3 x 64) for
@@ -281,14 +318,10 @@ void launch_relight_mfma(const RelightArgs &a, Elem elem, hipStream_t s) {
           "relight: bad pixel count or K");
   require(a.features && a.mk && a.light && a.w_tiles && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: missing operand");
-  const dim3 grid((a.n_pixels + kRelightBlock - 1) / kRelightBlock);
-  const bool dump = a.dump != nullptr;
   if (elem == Elem::BF16) {
-    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<__bf16, true>), grid, dim3(kRelightBlock), 0, s, a);
-    else hipLaunchKernelGGL((relight_mfma_kernel<__bf16, false>), grid, dim3(kRelightBlock), 0, s, a);
+    launch_mfma_variant<__bf16>(a, s);
   } else if (elem == Elem::F16) {
-    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<_Float16, true>), grid, dim3(kRelightBlock), 0, s, a);
-    else hipLaunchKernelGGL((relight_mfma_kernel<_Float16, false>), grid, dim3(kRelightBlock), 0, s, a);
+    launch_mfma_variant<_Float16>(a, s);
   } else {
     fail(NWPRT_ERR_INVALID_ARGUMENT, "relight: MFMA element must be BF16 or FP16");
   }
