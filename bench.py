@@ -80,6 +80,11 @@ def main():
         ctx.envmap_update_faces(env, dfaces, stream)
         ctx.relight(scene, env, frame, precision, nw.TONEMAP_FULL, stream)
 
+    # Tile autotuning for this shape (records the fastest fused-kernel
+    # variant for the (precision, K, pixels) bucket; steps then use it).
+    variant, _ = ctx.autotune_relight(scene, env, frame, prec)
+    variant_name = nw.Context.relight_variants()[variant]
+
     # --- fused MFMA path: throughput and per-step latency ------------------
     for _ in range(args.warmup):
         step()
@@ -137,7 +142,8 @@ def main():
         "data": "synthetic (seeded G-buffer, log-normal cubemap with sun, random-init network)",
         "config": {"workload": f"relight {W}x{H}, cubemap 6x{N}x{N}, K={K}, hash grid 16x2 "
                                f"2^19 FP16, MLP 32-64-64-64-3 {args.precision}",
-                   "pixels": P, "k": K, "pairs_per_step": pairs},
+                   "pixels": P, "k": K, "pairs_per_step": pairs,
+                   "tile_variant": variant_name},
         "latency_ms": {"p50": percentile(step_ms, 50), "p99": percentile(step_ms, 99),
                        "max": float(max(step_ms))},
         "wall_s": wall,

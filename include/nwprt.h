@@ -311,6 +311,29 @@ nwprt_status nwprt_pass_p2(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
                            nwprt_precision precision,
                            nwprt_tonemap_source tonemap, void *stream);
 
+/* ------------------------------------------------------- tile autotuning -- */
+/* No counterpart in the reference (its MLP tiling is fixed inside
+ * tiny-cuda-nn).  The fused MFMA kernel comes in several tile-shape
+ * variants (pixel groups per pass, output-layer weights in VGPRs or LDS,
+ * waves per SIMD).  nwprt_autotune_relight times every variant on one frame
+ * and records the fastest for that (precision, K, pixel-count) bucket;
+ * later relights falling in the bucket use it, others use variant 0. */
+int32_t nwprt_relight_variant_count(void);
+const char *nwprt_relight_variant_name(int32_t variant);
+/* Force one variant for every bucket (-1 returns to the tuned choice). */
+nwprt_status nwprt_set_relight_variant(nwprt_ctx ctx, int32_t variant);
+nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id,
+                                    int32_t env_id,
+                                    const nwprt_gbuffer *gbuffer,
+                                    int32_t width, int32_t height,
+                                    nwprt_precision precision,
+                                    int32_t *best_variant, float *best_ms);
+/* The variant a relight of this shape would use now. */
+nwprt_status nwprt_selected_relight_variant(nwprt_ctx ctx,
+                                            nwprt_precision precision,
+                                            int32_t k, int32_t n_pixels,
+                                            int32_t *variant);
+
 /* RenderEngine::computeFinalPixelColors kernel (Renderer/toneMap.cu:47-71):
  * float4 -> RGBA8, device pointers. */
 nwprt_status nwprt_tonemap(nwprt_ctx ctx, const float *src_float4,

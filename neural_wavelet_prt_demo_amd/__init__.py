@@ -138,6 +138,13 @@ _SIGNATURES = {
                               c_int, c_void_p]),
     "nwprt_pass_p2": (c_int, [c_void_p, c_int32, c_int32, c_int32, POINTER(GBuffer), c_int32,
                               c_int32, POINTER(FrameOut), c_int, c_int, c_void_p]),
+    "nwprt_relight_variant_count": (c_int32, []),
+    "nwprt_relight_variant_name": (c_char_p, [c_int32]),
+    "nwprt_set_relight_variant": (c_int, [c_void_p, c_int32]),
+    "nwprt_autotune_relight": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
+                                       c_int32, c_int, POINTER(c_int32), POINTER(c_float)]),
+    "nwprt_selected_relight_variant": (c_int, [c_void_p, c_int, c_int32, c_int32,
+                                               POINTER(c_int32)]),
     "nwprt_tonemap": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
@@ -474,6 +481,30 @@ class Context:
         _check(lib().nwprt_pass_p2(self.h, scene, env, slot, ctypes.byref(gb), frame.width,
                                    frame.height, ctypes.byref(fo), precision, tonemap,
                                    c_void_p(stream) if stream else None))
+
+    # tile tuning ---------------------------------------------------------
+    @staticmethod
+    def relight_variants():
+        n = lib().nwprt_relight_variant_count()
+        return [lib().nwprt_relight_variant_name(i).decode() for i in range(n)]
+
+    def set_relight_variant(self, variant: int) -> None:
+        _check(lib().nwprt_set_relight_variant(self.h, variant))
+
+    def autotune_relight(self, scene: int, env: int, frame: DeviceFrame,
+                         precision: int = PREC_BF16):
+        gb = frame.gbuffer()
+        best, ms = c_int32(), c_float()
+        _check(lib().nwprt_autotune_relight(self.h, scene, env, ctypes.byref(gb), frame.width,
+                                            frame.height, precision, ctypes.byref(best),
+                                            ctypes.byref(ms)))
+        return best.value, ms.value
+
+    def selected_relight_variant(self, precision: int, k: int, n_pixels: int) -> int:
+        v = c_int32()
+        _check(lib().nwprt_selected_relight_variant(self.h, precision, k, n_pixels,
+                                                    ctypes.byref(v)))
+        return v.value
 
     def computeFinalPixelColors(self, src: DeviceBuffer, dst: DeviceBuffer, width: int,
                                 height: int, stream: Optional[int] = None) -> None:

@@ -7,8 +7,10 @@
 // per-frame-slot work buffers that grow and are reused.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <fstream>
 #include <map>
+#include <tuple>
 
 #include "kernels.hpp"
 
@@ -82,6 +84,10 @@ struct nwprt_context {
   int next_id = 1;
   nwprt::FrameSlot slots[3];
   nwprt::DevBuf staging, scores;
+  // Tile tuning: forced variant (-1: none) and the autotuner's choices keyed
+  // by (element type, K rounded up to a power of two, ceil(log2 pixels)).
+  int forced_variant = -1;
+  std::map<std::tuple<int, int, int>, int> tuned;
 };
 
 namespace nwprt {
@@ -273,6 +279,19 @@ void env_from_selection(nwprt_ctx ctx, const int32_t *idx, const float *str, int
   *env_id = id;
 }
 
+std::tuple<int, int, int> tune_key(Elem elem, int k, int n_pixels) {
+  int kb = 1, pb = 0;
+  while (kb < k) kb <<= 1;
+  while ((1ll << pb) < n_pixels) ++pb;
+  return {(int)elem, kb, pb};
+}
+
+int pick_variant(nwprt_ctx ctx, Elem elem, int k, int n_pixels) {
+  if (ctx->forced_variant >= 0) return ctx->forced_variant;
+  auto it = ctx->tuned.find(tune_key(elem, k, n_pixels));
+  return it == ctx->tuned.end() ? 0 : it->second;
+}
+
 void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const nwprt_gbuffer *gb,
                  int w, int h, Elem elem, hipStream_t s, uint32_t *dbg_idx = nullptr,
                  float *dbg_feat = nullptr) {
@@ -302,7 +321,7 @@ void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const 
 void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, FrameSlot &slot,
                        const nwprt_gbuffer *gb, int w, int h, const nwprt_frame_out *out,
                        Elem elem, nwprt_tonemap_source tm, hipStream_t s, float *dump = nullptr,
-                       int dump_begin = 0, int dump_count = 0) {
+                       int dump_begin = 0, int dump_count = 0, int variant = -1) {
   const int P = w * h;
   require(slot.valid && slot.scene == scene_id && slot.n_pixels == P && slot.elem == elem,
           NWPRT_ERR_STATE,
@@ -350,7 +369,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
                              s);
     a.mk = slot.mk.as<uint16_t>();
     a.w_tiles = elem == Elem::BF16 ? sc.tiles_bf16.as<uint16_t>() : sc.tiles_f16.as<uint16_t>();
-    launch_relight_mfma(a, elem, s);
+    launch_relight_mfma(a, elem, variant >= 0 ? variant : pick_variant(ctx, elem, env.k, P), s);
   }
 }
 
@@ -619,6 +638,83 @@ nwprt_status nwprt_pass_p2(nwprt_ctx ctx, int32_t scene_id, int32_t env_id, int3
     check_frame(gbuffer, width, height, true);
     relight_from_slot(ctx, sc, scene_id, env, ctx->slots[frame_slot], gbuffer, width, height, out,
                       elem_of(precision), tonemap, pick(ctx, stream));
+  });
+}
+
+int32_t nwprt_relight_variant_count(void) { return relight_mfma_variant_count(); }
+
+const char *nwprt_relight_variant_name(int32_t variant) {
+  return relight_mfma_variant_name(variant);
+}
+
+nwprt_status nwprt_set_relight_variant(nwprt_ctx ctx, int32_t variant) {
+  return guarded([&] {
+    require(ctx != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null context");
+    require(variant >= -1 && variant < relight_mfma_variant_count(), NWPRT_ERR_INVALID_ARGUMENT,
+            "unknown relight variant " + std::to_string(variant));
+    ctx->forced_variant = variant;
+  });
+}
+
+nwprt_status nwprt_selected_relight_variant(nwprt_ctx ctx, nwprt_precision precision, int32_t k,
+                                            int32_t n_pixels, int32_t *variant) {
+  return guarded([&] {
+    require(ctx && variant && k >= 1 && n_pixels >= 1, NWPRT_ERR_INVALID_ARGUMENT,
+            "bad argument");
+    *variant = pick_variant(ctx, elem_of(precision), k, n_pixels);
+  });
+}
+
+nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
+                                    const nwprt_gbuffer *gbuffer, int32_t width, int32_t height,
+                                    nwprt_precision precision, int32_t *best_variant,
+                                    float *best_ms) {
+  return guarded([&] {
+    bind(ctx);
+    Scene &sc = get_scene(ctx, scene_id);
+    EnvMap &env = get_env(ctx, env_id);
+    check_frame(gbuffer, width, height, true);
+    const Elem elem = elem_of(precision);
+    require(elem != Elem::F32, NWPRT_ERR_INVALID_ARGUMENT,
+            "autotune: the plain FP32 form has no tile variants");
+    const hipStream_t s = ctx->stream;
+    FrameSlot &slot = ctx->slots[2];
+    encode_into(ctx, sc, scene_id, slot, gbuffer, width, height, elem, s);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    int best = 0;
+    float best_t = 0.f;
+    try {
+      for (int v = 0; v < relight_mfma_variant_count(); ++v) {
+        relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr, elem,
+                          NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+          hip_check(hipEventRecord(e0, s), "hipEventRecord");
+          relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr, elem,
+                            NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
+          hip_check(hipEventRecord(e1, s), "hipEventRecord");
+          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+          float ms = 0.f;
+          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+          t = rep == 0 ? ms : std::min(t, ms);
+        }
+        if (v == 0 || t < best_t) {
+          best = v;
+          best_t = t;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->tuned[tune_key(elem, env.k, width * height)] = best;
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = best_t;
   });
 }
 

@@ -76,7 +76,10 @@ struct RelightArgs {
   float *dump;
   int dump_begin, dump_count;
 };
-void launch_relight_mfma(const RelightArgs &a, Elem elem, hipStream_t s);
+// Tile-shape variants of the fused kernel (see relight_kernels.hip).
+int relight_mfma_variant_count();
+const char *relight_mfma_variant_name(int variant);
+void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream_t s);
 void launch_relight_plain(const RelightArgs &a, hipStream_t s);
 void launch_tonemap(const float *src, uint32_t *dst, int n_pixels, hipStream_t s);
 

@@ -80,15 +80,105 @@ __device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float
   }
 }
 
+// Tile-shape variants of the fused kernel (chosen per shape by the
+// autotuner): GP pixel groups carried through the layers together (GP
+// independent MFMA chains hide each other's result latency), the output
+// layer's A tiles in VGPRs or in LDS (LDS frees 32 VGPRs for a third wave per
+// SIMD), and the minimum waves per SIMD the register budget must allow.
+struct MfmaVariant {
+  int gp;
+  bool w4_lds;
+  int min_waves;
+  const char *name;
+};
+constexpr MfmaVariant kMfmaVariants[] = {
+    {2, false, 2, "gp2.w4reg.occ2"}, {4, false, 2, "gp4.w4reg.occ2"},
+    {2, true, 3, "gp2.w4lds.occ3"},  {4, true, 2, "gp4.w4lds.occ2"},
+    {1, true, 3, "gp1.w4lds.occ3"},
+};
+constexpr int kNumMfmaVariants = sizeof(kMfmaVariants) / sizeof(kMfmaVariants[0]);
+
+// The four layers and the clamped, compensated accumulation for one selected
+// wavelet k (A operands `a1` = its first-layer tiles) over the wave's 64
+// pixels.
+template <typename E, bool SOFTPLUS, bool DUMP, int GP, bool W4LDS>
+__device__ __forceinline__ void wavelet_step(const RelightArgs &a, int k, const u32x4 (&a1)[4],
+                                             const u32x4 (&hb)[4], const u32x4 (&w2)[4][2],
+                                             const u32x4 (&w3)[4][2], const u32x4 (&w4)[4][2],
+                                             const u32x4 *w4s, int lane, int my_px, bool mine,
+                                             float (&sum)[3], float (&comp)[3]) {
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  f32x4 out = zero;
+#pragma unroll
+  for (int jp = 0; jp < 4; jp += GP) {
+    f32x4 c[GP][4];
+    u32x4 x[GP][2];
+#pragma unroll
+    for (int j = 0; j < GP; ++j)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) c[j][b] = Mfma<E>::run(a1[b], hb[jp + j], zero);
+#pragma unroll
+    for (int j = 0; j < GP; ++j) {
+      x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
+      x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+    }
+#pragma unroll
+    for (int j = 0; j < GP; ++j)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], zero));
+#pragma unroll
+    for (int j = 0; j < GP; ++j) {
+      x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
+      x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+    }
+#pragma unroll
+    for (int j = 0; j < GP; ++j)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        c[j][b] = Mfma<E>::run(w3[b][1], x[j][1], Mfma<E>::run(w3[b][0], x[j][0], zero));
+#pragma unroll
+    for (int j = 0; j < GP; ++j) {
+      x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
+      x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+    }
+    // Output layer: variant j carries W4 in rows 4j..4j+2, so group j's
+    // three outputs land in rows 4j.. of the shared accumulator, i.e. in
+    // lanes with g == j -- the lanes that own those pixels.
+#pragma unroll
+    for (int j = 0; j < GP; ++j) {
+      const u32x4 t0 = W4LDS ? w4s[((jp + j) * 2 + 0) * 64 + lane] : w4[jp + j][0];
+      const u32x4 t1 = W4LDS ? w4s[((jp + j) * 2 + 1) * 64 + lane] : w4[jp + j][1];
+      out = Mfma<E>::run(t1, x[j][1], Mfma<E>::run(t0, x[j][0], out));
+    }
+  }
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    const float t = SOFTPLUS ? dev::softplus(out[ch]) : out[ch];
+    const float term = a.light[3 * k + ch] * t;
+    dev::neumaier_add(sum[ch], comp[ch], term > 0.0f ? term : 0.0f);
+    if (DUMP) {
+      const int q = my_px - a.dump_begin;
+      if (mine && q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
+    }
+  }
+}
+
 // Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
 // pixels [64 (4 b + w), +64).  Pixel groups j = 0..3 (16 pixels each) are the
-// MFMA columns; lane l owns pixel base + l for the reduction.  GP groups are
-// carried through the layers together so that GP independent MFMA chains
-// hide each other's result latency.
-template <typename E, bool SOFTPLUS, bool DUMP, int GP = 2>
-__global__ __launch_bounds__(kRelightBlock, 2) void relight_mfma_kernel(RelightArgs a) {
+// MFMA columns; lane l owns pixel base + l for the reduction.  The next
+// wavelet's first-layer tiles are fetched while the current one computes
+// (two register sets, K loop unrolled by two).
+template <typename E, bool SOFTPLUS, bool DUMP, int GP, bool W4LDS, int MINW>
+__global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
+  __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane >> 4, n = lane & 15;
+  const u32x4 *wt = (const u32x4 *)a.w_tiles;
+  if (W4LDS) {
+    for (int i = threadIdx.x; i < 4 * 2 * 64; i += kRelightBlock) w4s[i] = wt[2 * 8 * 64 + i];
+    __syncthreads();
+  }
   const int base = (blockIdx.x * 4 + wave) * 64;
   const int P = a.n_pixels;
   if (base >= P) return;
@@ -108,7 +198,6 @@ __global__ __launch_bounds__(kRelightBlock, 2) void relight_mfma_kernel(RelightA
       hb[j] = p < P ? *(const u32x4 *)(feat + (size_t)p * 32 + 8 * g) : u32x4{0u, 0u, 0u, 0u};
     }
     // Layers 2..4 as A-operand tiles: [4][2][64] each.
-    const u32x4 *wt = (const u32x4 *)a.w_tiles;
     u32x4 w2[4][2], w3[4][2], w4[4][2];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -116,72 +205,26 @@ __global__ __launch_bounds__(kRelightBlock, 2) void relight_mfma_kernel(RelightA
       for (int kb = 0; kb < 2; ++kb) {
         w2[b][kb] = wt[(0 * 8 + b * 2 + kb) * 64 + lane];
         w3[b][kb] = wt[(1 * 8 + b * 2 + kb) * 64 + lane];
-        w4[b][kb] = wt[(2 * 8 + b * 2 + kb) * 64 + lane];
+        if (!W4LDS) w4[b][kb] = wt[(2 * 8 + b * 2 + kb) * 64 + lane];
       }
     const u32x4 *mk = (const u32x4 *)a.mk;
-    u32x4 a1[4];
+    u32x4 ta[4], tb[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) a1[b] = mk[b * 64 + lane];
-
-    for (int k = 0; k < a.k; ++k) {
-      const u32x4 cur[4] = {a1[0], a1[1], a1[2], a1[3]};
+    for (int b = 0; b < 4; ++b) ta[b] = mk[b * 64 + lane];
+    for (int k = 0; k < a.k; k += 2) {
       if (k + 1 < a.k) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) a1[b] = mk[((size_t)(k + 1) * 4 + b) * 64 + lane];
+        for (int b = 0; b < 4; ++b) tb[b] = mk[((size_t)(k + 1) * 4 + b) * 64 + lane];
       }
-      const float l0 = a.light[3 * k], l1 = a.light[3 * k + 1], l2 = a.light[3 * k + 2];
-      const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-      f32x4 out = zero;
+      wavelet_step<E, SOFTPLUS, DUMP, GP, W4LDS>(a, k, ta, hb, w2, w3, w4, w4s, lane, my_px, mine,
+                                                  sum, comp);
+      if (k + 1 < a.k) {
+        if (k + 2 < a.k) {
 #pragma unroll
-      for (int jp = 0; jp < 4; jp += GP) {
-        f32x4 c[GP][4];
-        u32x4 x[GP][2];
-#pragma unroll
-        for (int j = 0; j < GP; ++j)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) c[j][b] = Mfma<E>::run(cur[b], hb[jp + j], zero);
-#pragma unroll
-        for (int j = 0; j < GP; ++j) {
-          x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
-          x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
+          for (int b = 0; b < 4; ++b) ta[b] = mk[((size_t)(k + 2) * 4 + b) * 64 + lane];
         }
-#pragma unroll
-        for (int j = 0; j < GP; ++j)
-#pragma unroll
-          for (int b = 0; b < 4; ++b)
-            c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], zero));
-#pragma unroll
-        for (int j = 0; j < GP; ++j) {
-          x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
-          x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
-        }
-#pragma unroll
-        for (int j = 0; j < GP; ++j)
-#pragma unroll
-          for (int b = 0; b < 4; ++b)
-            c[j][b] = Mfma<E>::run(w3[b][1], x[j][1], Mfma<E>::run(w3[b][0], x[j][0], zero));
-#pragma unroll
-        for (int j = 0; j < GP; ++j) {
-          x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
-          x[j][1] = relu_pack<E>(c[j][2], c[j][3]);
-        }
-        // Output layer: variant j carries W4 in rows 4j..4j+2, so group j's
-        // three outputs land in rows 4j.. of the shared accumulator, i.e. in
-        // lanes with g == j -- the lanes that own those pixels.
-#pragma unroll
-        for (int j = 0; j < GP; ++j)
-          out = Mfma<E>::run(w4[jp + j][1], x[j][1], Mfma<E>::run(w4[jp + j][0], x[j][0], out));
-      }
-      const float lc[3] = {l0, l1, l2};
-#pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        const float t = SOFTPLUS ? dev::softplus(out[ch]) : out[ch];
-        const float term = lc[ch] * t;
-        dev::neumaier_add(sum[ch], comp[ch], term > 0.0f ? term : 0.0f);
-        if (DUMP) {
-          const int q = my_px - a.dump_begin;
-          if (mine && q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
-        }
+        wavelet_step<E, SOFTPLUS, DUMP, GP, W4LDS>(a, k + 1, tb, hb, w2, w3, w4, w4s, lane, my_px,
+                                                    mine, sum, comp);
       }
     }
   }
@@ -271,17 +314,35 @@ __global__ __launch_bounds__(256) void tonemap_kernel(const float *__restrict__ 
   dst[p] = dev::tonemap_rgba8(v.x, v.y, v.z);
 }
 
-template <typename E>
-void launch_mfma_variant(const RelightArgs &a, hipStream_t s) {
+template <typename E, bool SP, int V>
+void launch_v(const RelightArgs &a, dim3 grid, dim3 block, hipStream_t s) {
+  constexpr MfmaVariant v = kMfmaVariants[V];
+  if (a.dump && V == 0)
+    hipLaunchKernelGGL((relight_mfma_kernel<E, SP, true, v.gp, v.w4_lds, v.min_waves>), grid, block,
+                       0, s, a);
+  else
+    hipLaunchKernelGGL((relight_mfma_kernel<E, SP, false, v.gp, v.w4_lds, v.min_waves>), grid,
+                       block, 0, s, a);
+}
+
+template <typename E, bool SP>
+void launch_sp(const RelightArgs &a, int variant, hipStream_t s) {
   const dim3 grid((a.n_pixels + kRelightBlock - 1) / kRelightBlock), block(kRelightBlock);
-  const bool dump = a.dump != nullptr;
-  if (a.softplus) {
-    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<E, true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((relight_mfma_kernel<E, true, false>), grid, block, 0, s, a);
-  } else {
-    if (dump) hipLaunchKernelGGL((relight_mfma_kernel<E, false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((relight_mfma_kernel<E, false, false>), grid, block, 0, s, a);
+  if (a.dump) variant = 0;  // the transport tap is built for the default variant only
+  switch (variant) {
+    case 0: launch_v<E, SP, 0>(a, grid, block, s); break;
+    case 1: launch_v<E, SP, 1>(a, grid, block, s); break;
+    case 2: launch_v<E, SP, 2>(a, grid, block, s); break;
+    case 3: launch_v<E, SP, 3>(a, grid, block, s); break;
+    case 4: launch_v<E, SP, 4>(a, grid, block, s); break;
+    default: fail(NWPRT_ERR_INVALID_ARGUMENT, "relight: unknown kernel variant");
   }
+}
+
+template <typename E>
+void launch_mfma_variant(const RelightArgs &a, int variant, hipStream_t s) {
+  if (a.softplus) launch_sp<E, true>(a, variant, s);
+  else launch_sp<E, false>(a, variant, s);
 }
 
 }  // namespace
@@ -313,15 +374,20 @@ void build_weight_tiles(const float *w, Elem elem, std::vector<uint16_t> &out) {
           }
 }
 
-void launch_relight_mfma(const RelightArgs &a, Elem elem, hipStream_t s) {
+int relight_mfma_variant_count() { return kNumMfmaVariants; }
+const char *relight_mfma_variant_name(int v) {
+  return v >= 0 && v < kNumMfmaVariants ? kMfmaVariants[v].name : nullptr;
+}
+
+void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream_t s) {
   require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: bad pixel count or K");
   require(a.features && a.mk && a.light && a.w_tiles && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: missing operand");
   if (elem == Elem::BF16) {
-    launch_mfma_variant<__bf16>(a, s);
+    launch_mfma_variant<__bf16>(a, variant, s);
   } else if (elem == Elem::F16) {
-    launch_mfma_variant<_Float16>(a, s);
+    launch_mfma_variant<_Float16>(a, variant, s);
   } else {
     fail(NWPRT_ERR_INVALID_ARGUMENT, "relight: MFMA element must be BF16 or FP16");
   }

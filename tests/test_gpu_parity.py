@@ -244,3 +244,24 @@ def test_config1_fp16_1080p(nw, orc, ctx):
     err, p = max_rel_err(ind, got), psnr(ind, got)
     print(f"config1 fp16: max rel err {err:.3e}, PSNR {p:.1f} dB")
     assert err <= TOL_HALF["fp16"] and p >= MIN_PSNR_DB
+
+
+def test_every_tile_variant_matches_and_autotune_records(nw, orc, ctx):
+    """All fused-kernel tile variants compute the same outputs bit for bit
+    (same per-lane operation order), and the autotuner's choice is used."""
+    case = make_case(nw, orc, width=200, height=37, k=45)
+    scene = ctx.scene_create(case.desc, case.params)
+    env = ctx.envmap_from_faces(case.faces, case.k)
+    ref = None
+    for v in range(len(nw.Context.relight_variants())):
+        ctx.set_relight_variant(v)
+        _, _, _, res = _gpu_relight(nw, ctx, case, "bf16", env=env, scene=scene)
+        if ref is None:
+            ref = res
+        for key in ref:
+            assert np.array_equal(ref[key], res[key]), (v, key)
+    ctx.set_relight_variant(-1)
+    frame = nw.DeviceFrame(case.width, case.height, case.gbuf)
+    best, ms = ctx.autotune_relight(scene, env, frame, nw.PREC_BF16)
+    assert 0 <= best < len(nw.Context.relight_variants()) and ms > 0
+    assert ctx.selected_relight_variant(nw.PREC_BF16, case.k, case.width * case.height) == best
