@@ -1,0 +1,253 @@
+// nwprt_render_engine.hpp -- RenderEngine-shaped C++ facade over the C ABI.
+//
+// Header-only; link libnwprt.so and the HIP runtime.  It keeps the method
+// names and argument meaning of the reference's TCNN interface
+// (Renderer/RenderEngine.h:78-85) so a renderer written against it can swap
+// implementations: two frame-buffer sets as fullFrameCUDA holds them
+// (Renderer/frames.h:96-183), initTCNN(modelPath), buildLights from
+// env-map folders, TCNNPass_p1 / TCNNPass_p2 / TCNNSync /
+// computeFinalPixelColors.  Errors throw std::runtime_error (the reference's
+// CUDA_CHECK convention, Renderer/optix7.h:26-37); a bad env-map id throws
+// std::overflow_error (Renderer/RenderEngine.cpp:348-349, here with the
+// bound checked as >=).
+//
+// The OptiX pass that fills the G-buffer is not part of this build: the
+// caller (or writeSyntheticGBuffer) writes fbFirstHitPos & co.  The OptiX
+// denoiser is not part of it either: with a denoiser requested or not, the
+// "Denoised" buffers receive a copy (the reference's denoiser-off path,
+// Renderer/RenderEngine.cpp:398-401), and denoiserApplied() reports false.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "nwprt.h"
+
+namespace nwprt {
+
+enum class finalColorBufferContent { full, direct, indirect };
+
+// Renderer/RenderEngine.h:29-42 (the fields the relight path reads) plus
+// the MLP precision of this build.
+struct renderConfig {
+  finalColorBufferContent renderingMode = finalColorBufferContent::full;
+  bool waveletMode = true;
+  bool finalDenoiser = false;
+  bool indirectDenoiser = false;
+  nwprt_precision precision = NWPRT_PREC_BF16;
+};
+
+// Device buffers of one frame that the relight path reads or writes.
+struct fullFrameBuffers {
+  float *fbFirstHitPos = nullptr, *fbworldNormal = nullptr, *fbFirstHitReflecDir = nullptr,
+        *fbRoughness = nullptr, *fbFirstHitKd = nullptr, *fbFirstHitKs = nullptr,
+        *fbDirectDenoised = nullptr;
+  uint8_t *fbEnvironmentMapFlag = nullptr;
+  float *fbPredictedIndirect = nullptr, *fbPredictedIndirectDenoised = nullptr,
+        *fbPredictedIndirectWithDirect = nullptr, *fbPredictedIndirectWithDirectDenoised = nullptr;
+  uint32_t *fbFinalColor = nullptr;
+};
+
+class RenderEngine {
+ public:
+  explicit RenderEngine(int device = 0) : device_(device) { check(nwprt_create(device, &ctx_)); }
+  ~RenderEngine() {
+    freeFrames();
+    if (ctx_) nwprt_destroy(ctx_);
+  }
+  RenderEngine(const RenderEngine &) = delete;
+  RenderEngine &operator=(const RenderEngine &) = delete;
+
+  // RenderEngine::resize: (re)allocate both frame-buffer sets.
+  void resize(int width, int height) {
+    if (width <= 0 || height <= 0) throw std::runtime_error("resize: bad frame size");
+    freeFrames();
+    width_ = width;
+    height_ = height;
+    const size_t P = (size_t)width * height;
+    for (auto &f : frames_) {
+      float **f4[] = {&f.fbFirstHitPos, &f.fbworldNormal, &f.fbFirstHitReflecDir,
+                      &f.fbFirstHitKd, &f.fbFirstHitKs, &f.fbDirectDenoised,
+                      &f.fbPredictedIndirect, &f.fbPredictedIndirectDenoised,
+                      &f.fbPredictedIndirectWithDirect,
+                      &f.fbPredictedIndirectWithDirectDenoised};
+      for (float **b : f4) *b = (float *)alloc(P * 16);
+      f.fbRoughness = (float *)alloc(P * 4);
+      f.fbEnvironmentMapFlag = (uint8_t *)alloc(P);
+      f.fbFinalColor = (uint32_t *)alloc(P * 4);
+    }
+  }
+  int width() const { return width_; }
+  int height() const { return height_; }
+  fullFrameBuffers &frame(int frameID) {
+    if (frameID < 0 || frameID > 1) throw std::out_of_range("frameID must be 0 or 1");
+    return frames_[frameID];
+  }
+
+  // Fill a frame's G-buffer with the seeded synthetic generator (stands in
+  // for the OptiX pass, Renderer/device_functions/camera.cu:247-270).
+  void writeSyntheticGBuffer(int frameID, uint64_t seed, float backgroundFraction = 0.f) {
+    const size_t P = (size_t)width_ * height_;
+    std::vector<float> pos(4 * P), nrm(4 * P), refl(4 * P), kd(4 * P), ks(4 * P), dir(4 * P),
+        rough(P);
+    std::vector<uint8_t> flag(P);
+    nwprt_gbuffer g{pos.data(), nrm.data(), refl.data(), rough.data(),
+                    kd.data(), ks.data(), dir.data(), flag.data()};
+    check(nwprt_synth_gbuffer(seed, width_, height_, backgroundFraction, &g));
+    fullFrameBuffers &f = frame(frameID);
+    upload(f.fbFirstHitPos, pos.data(), P * 16);
+    upload(f.fbworldNormal, nrm.data(), P * 16);
+    upload(f.fbFirstHitReflecDir, refl.data(), P * 16);
+    upload(f.fbFirstHitKd, kd.data(), P * 16);
+    upload(f.fbFirstHitKs, ks.data(), P * 16);
+    upload(f.fbDirectDenoised, dir.data(), P * 16);
+    upload(f.fbRoughness, rough.data(), P * 4);
+    upload(f.fbEnvironmentMapFlag, flag.data(), P);
+  }
+
+  // RenderEngine::initTCNN (Renderer/tinycudann/tcnn.cu:49-81).
+  void initTCNN(const std::string &modelPath) {
+    int32_t id = -1;
+    check(nwprt_scene_load_folder(ctx_, modelPath.c_str(), &id));
+    scene_ = id;
+  }
+  // Use a scene created through the C ABI instead of a weight folder.
+  void setScene(int32_t sceneId) { scene_ = sceneId; }
+  nwprt_ctx context() const { return ctx_; }
+
+  // RenderEngine::buildLights, wavelet part (Renderer/RenderEngine.cpp:
+  // 113-175): one Environment_Map(folder, name) per entry; maps without a
+  // stored selection get a GPU transform with `k` wavelets.
+  void buildLights(const std::vector<std::pair<std::string, std::string>> &envmaps, int k) {
+    destroyLights();
+    for (const auto &e : envmaps) {
+      int32_t id = -1;
+      check(nwprt_envmap_load_folder(ctx_, e.first.c_str(), e.second.c_str(), k, &id));
+      envs_.push_back(id);
+    }
+  }
+  int addEnvironmentMap(int32_t envId) {
+    envs_.push_back(envId);
+    return (int)envs_.size() - 1;
+  }
+  void destroyLights() {
+    for (int32_t id : envs_) nwprt_envmap_destroy(ctx_, id);
+    envs_.clear();
+  }
+  size_t environmentMapCount() const { return envs_.size(); }
+
+  // RenderEngine::TCNNAvaliable (tcnn.cu:160-162).
+  bool TCNNAvaliable() const { return scene_ >= 0 && !envs_.empty(); }
+
+  // RenderEngine::TCNNPass_p1 (tcnn.cu:83-108): per-pixel, env-independent
+  // part of the network for frameID.
+  void TCNNPass_p1(int frameID, hipStream_t stream = nullptr, renderConfig config = {}) {
+    nwprt_gbuffer g = gbuffer(frameID);
+    check(nwprt_pass_p1(ctx_, scene_, frameID, &g, width_, height_, config.precision, stream));
+  }
+
+  // RenderEngine::TCNNPass_p2 (tcnn.cu:111-152): per-pair network, clamped
+  // K-sum, then the denoise-or-copy into the "Denoised" buffers.
+  void TCNNPass_p2(int envMapID, int frameID, renderConfig config, hipStream_t stream = nullptr) {
+    if (envMapID < 0 || (size_t)envMapID >= envs_.size())
+      throw std::overflow_error("Envmap ID for rendering is larger than cached envmap");
+    nwprt_gbuffer g = gbuffer(frameID);
+    fullFrameBuffers &f = frame(frameID);
+    nwprt_frame_out out{f.fbPredictedIndirect, f.fbPredictedIndirectWithDirect, nullptr};
+    check(nwprt_pass_p2(ctx_, scene_, envs_[(size_t)envMapID], frameID, &g, width_, height_, &out,
+                        config.precision, NWPRT_TONEMAP_NONE, stream));
+    const size_t bytes = (size_t)width_ * height_ * 16;
+    hipStream_t s = stream ? stream : ownStream();
+    hipCheck(hipMemcpyAsync(f.fbPredictedIndirectWithDirectDenoised, f.fbPredictedIndirectWithDirect,
+                            bytes, hipMemcpyDeviceToDevice, s));
+    hipCheck(hipMemcpyAsync(f.fbPredictedIndirectDenoised, f.fbPredictedIndirect, bytes,
+                            hipMemcpyDeviceToDevice, s));
+  }
+
+  // RenderEngine::TCNNSync (tcnn.cu:154-158).
+  void TCNNSync() {
+    check(nwprt_sync(ctx_));
+    hipCheck(hipDeviceSynchronize());
+  }
+
+  // RenderEngine::computeFinalPixelColors, wavelet mode (toneMap.cu:73-99).
+  void computeFinalPixelColors(int frameID = 1, hipStream_t stream = nullptr,
+                               renderConfig config = {}) {
+    fullFrameBuffers &f = frame(frameID);
+    const float *src = config.renderingMode == finalColorBufferContent::indirect
+                           ? f.fbPredictedIndirectDenoised
+                           : f.fbPredictedIndirectWithDirectDenoised;
+    if (config.renderingMode == finalColorBufferContent::direct) src = f.fbDirectDenoised;
+    check(nwprt_tonemap(ctx_, src, f.fbFinalColor, width_, height_, stream));
+  }
+
+  // RenderEngine::downloadPixels (Renderer/RenderEngine.cpp:286-291).
+  void downloadPixels(uint32_t *hostPixels, int frameID) {
+    TCNNSync();
+    hipCheck(hipMemcpy(hostPixels, frame(frameID).fbFinalColor,
+                       (size_t)width_ * height_ * 4, hipMemcpyDeviceToHost));
+  }
+  // The two wavelet outputs of RenderEngine::downloadframe (RenderEngine.cpp:317-318).
+  void downloadPredictions(int frameID, std::vector<float> &indirect,
+                           std::vector<float> &withDirect) {
+    TCNNSync();
+    const size_t n = (size_t)width_ * height_ * 4;
+    indirect.resize(n);
+    withDirect.resize(n);
+    hipCheck(hipMemcpy(indirect.data(), frame(frameID).fbPredictedIndirectDenoised, n * 4,
+                       hipMemcpyDeviceToHost));
+    hipCheck(hipMemcpy(withDirect.data(), frame(frameID).fbPredictedIndirectWithDirectDenoised,
+                       n * 4, hipMemcpyDeviceToHost));
+  }
+  bool denoiserApplied() const { return false; }
+
+ private:
+  static void check(nwprt_status s) {
+    if (s != NWPRT_OK)
+      throw std::runtime_error(std::string("nwprt: ") + nwprt_status_string(s) + ": " +
+                               nwprt_last_error());
+  }
+  static void hipCheck(hipError_t e) {
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP: ") + hipGetErrorString(e));
+  }
+  void *alloc(size_t bytes) {
+    void *p = nullptr;
+    hipCheck(hipSetDevice(device_));
+    hipCheck(hipMalloc(&p, bytes));
+    allocations_.push_back(p);
+    return p;
+  }
+  static void upload(void *dst, const void *src, size_t bytes) {
+    hipCheck(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  }
+  void freeFrames() {
+    if (!allocations_.empty()) (void)hipDeviceSynchronize();
+    for (void *p : allocations_) (void)hipFree(p);
+    allocations_.clear();
+    frames_[0] = frames_[1] = fullFrameBuffers{};
+  }
+  nwprt_gbuffer gbuffer(int frameID) {
+    fullFrameBuffers &f = frame(frameID);
+    return nwprt_gbuffer{f.fbFirstHitPos, f.fbworldNormal, f.fbFirstHitReflecDir, f.fbRoughness,
+                         f.fbFirstHitKd,  f.fbFirstHitKs,  f.fbDirectDenoised,    f.fbEnvironmentMapFlag};
+  }
+  hipStream_t ownStream() {
+    void *s = nullptr;
+    check(nwprt_get_stream(ctx_, &s));
+    return (hipStream_t)s;
+  }
+
+  int device_;
+  nwprt_ctx ctx_ = nullptr;
+  int32_t scene_ = -1;
+  std::vector<int32_t> envs_;
+  int width_ = 0, height_ = 0;
+  fullFrameBuffers frames_[2];
+  std::vector<void *> allocations_;
+};
+
+}  // namespace nwprt

@@ -186,10 +186,12 @@ def _ptr(a: Optional[np.ndarray]):
 def config_scene_desc(face_size: int, hash_dtype: int = DTYPE_F16, *, levels: int = 16,
                       features: int = 2, log2_hashmap_size: int = 19, base_resolution: int = 16,
                       per_level_scale: float = 1.5, projection: bool = True,
-                      softplus: bool = True) -> SceneDesc:
+                      softplus: bool = True, aux: bool = False) -> SceneDesc:
     """The network of the relighting workloads: 16 x 2 hash grid (2^19
     entries per level, N_min 16, growth 1.5), 32 x 32 projection, MLP
-    32 -> 64 -> 64 -> 64 -> 3 with ReLU and a softplus output."""
+    32 -> 64 -> 64 -> 64 -> 3 with ReLU and a softplus output.  With `aux`,
+    the 13 G-buffer encodings (+3 pads) also enter the second layer, as in
+    the reference graph's cmp_net input."""
     d = SceneDesc()
     d.grid = HashgridDesc(levels, features, log2_hashmap_size, base_resolution, per_level_scale)
     d.hash_dtype = hash_dtype
@@ -199,7 +201,7 @@ def config_scene_desc(face_size: int, hash_dtype: int = DTYPE_F16, *, levels: in
     d.n_hidden_layers = 3
     d.has_projection = int(projection)
     d.output_softplus = int(softplus)
-    d.aux_dims = 0
+    d.aux_dims = 13 if aux else 0
     d.aux_pad_value = 1.0
     return d
 

@@ -18,6 +18,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libnwprt.so"
+OFFLINE = PKG / "nwprt_offline"
 
 SOURCES = [
     "host_util.cpp",
@@ -77,8 +78,17 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # The offline trajectory driver, linked against the library next to it.
+    exe_tmp = OFFLINE.with_suffix(".tmp")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-std=c++17", "-O2", "-Wall", "-x", "hip",
+           str(CSRC / "offline_main.cpp"), "-x", "none", str(LIB), "-Wl,-rpath,$ORIGIN",
+           "-o", str(exe_tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"offline driver build failed\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(exe_tmp, OFFLINE)
     if verbose:
-        print(f"[build] {LIB}")
+        print(f"[build] {LIB}\n[build] {OFFLINE}")
     return LIB
 
 

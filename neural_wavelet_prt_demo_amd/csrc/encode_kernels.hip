@@ -28,6 +28,28 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   return x * q + (x < r ? x : r) + local;
 }
 
+// Row p of a [n][W] array in element type OUT (W a multiple of 8).
+template <Elem OUT, int W>
+__device__ __forceinline__ void store_row(void *base, int p, const float *v) {
+  if (OUT == Elem::F32) {
+    float4 *dst = (float4 *)((float *)base + (size_t)p * W);
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) dst[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  } else {
+    dev::u32x4 *dst = (dev::u32x4 *)((uint16_t *)base + (size_t)p * W);
+#pragma unroll
+    for (int q = 0; q < W / 8; ++q) {
+      dev::u32x4 r;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const float lo = v[8 * q + 2 * d], hi = v[8 * q + 2 * d + 1];
+        r[d] = OUT == Elem::BF16 ? dev::pack_bf16(lo, hi) : dev::pack_f16(lo, hi);
+      }
+      dst[q] = r;
+    }
+  }
+}
+
 template <int F, bool HF16, Elem OUT, bool DBG>
 __global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
   constexpr int L = 32 / F;
@@ -125,23 +147,25 @@ __global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
     for (int i = 0; i < 32; ++i) out[i] = feat[i];
   }
 
-  if (OUT == Elem::F32) {
-    float4 *dst = (float4 *)((float *)a.features + (size_t)p * 32);
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-      dst[q] = make_float4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
-  } else {
-    dev::u32x4 *dst = (dev::u32x4 *)((uint16_t *)a.features + (size_t)p * 32);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      dev::u32x4 v;
-#pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const float lo = out[8 * q + 2 * d], hi = out[8 * q + 2 * d + 1];
-        v[d] = OUT == Elem::BF16 ? dev::pack_bf16(lo, hi) : dev::pack_f16(lo, hi);
-      }
-      dst[q] = v;
-    }
+  store_row<OUT, 32>(a.features, p, out);
+
+  if (a.aux) {
+    // The reference's per-pair input rows 3..15 (Renderer/tinycudann/
+    // utils_kernels.h:31-46, INPUT_BUFFER_* of prt_globals.h): normal and
+    // reflection as stored, log(25 r + 1) / log 26, 2 kd - 1, 2 ks - 1; then
+    // three padding inputs.  Computed once per pixel.
+    const float4 nn = *(const float4 *)(a.normal + 4 * (size_t)p);
+    const float4 rr = *(const float4 *)(a.reflect + 4 * (size_t)p);
+    const float4 kd = *(const float4 *)(a.kd + 4 * (size_t)p);
+    const float4 ks = *(const float4 *)(a.ks + 4 * (size_t)p);
+    const float rough = a.roughness[p];
+    const float rgh_den = 1.0f / logf(26.0f);
+    const float v[16] = {nn.x, nn.y, nn.z, rr.x, rr.y, rr.z,
+                         logf(fmaf(25.0f, rough, 1.0f)) * rgh_den,
+                         fmaf(2.0f, kd.x, -1.0f), fmaf(2.0f, kd.y, -1.0f), fmaf(2.0f, kd.z, -1.0f),
+                         fmaf(2.0f, ks.x, -1.0f), fmaf(2.0f, ks.y, -1.0f), fmaf(2.0f, ks.z, -1.0f),
+                         a.aux_pad, a.aux_pad, a.aux_pad};
+    store_row<OUT, 16>(a.aux, p, v);
   }
 }
 

@@ -51,7 +51,11 @@ struct DevBuf {
 struct Scene {
   nwprt_scene_desc desc{};
   nwprt_hashgrid_levels lv{};
-  DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16;
+  DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16, aux_bf16, aux_f16;
+  // FP32 aux columns of the second layer ([64][16]) inside w_f32, or null.
+  const float *waux_f32() const {
+    return desc.aux_dims ? w_f32.as<float>() + (scene_layer_weight_count(desc) - 64 * 16) : nullptr;
+  }
 };
 
 struct EnvMap {
@@ -67,7 +71,7 @@ struct EnvMap {
 // Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
 // 2: nwprt_relight / debug taps).
 struct FrameSlot {
-  DevBuf features, mk, wsel;
+  DevBuf features, aux, mk, wsel;
   bool valid = false;
   int scene = -1, n_pixels = 0;
   Elem elem = Elem::F32;
@@ -179,8 +183,6 @@ void check_frame(const nwprt_gbuffer *gb, int w, int h, bool need_direct) {
 void create_scene(nwprt_ctx ctx, const nwprt_scene_desc &d, const nwprt_scene_params &prm,
                   int *scene_id) {
   validate_scene_desc(d);
-  require(d.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
-          "scene: aux G-buffer inputs are not implemented by this build's kernels");
   require(prm.hash_table && prm.wavelet_features && prm.layers && scene_id,
           NWPRT_ERR_INVALID_ARGUMENT, "scene: missing parameter array");
   require(!d.has_projection || prm.projection, NWPRT_ERR_INVALID_ARGUMENT,
@@ -201,6 +203,13 @@ void create_scene(nwprt_ctx ctx, const nwprt_scene_desc &d, const nwprt_scene_pa
   s->tiles_bf16.upload(tiles.data(), tiles.size() * 2);
   build_weight_tiles(prm.layers, Elem::F16, tiles);
   s->tiles_f16.upload(tiles.data(), tiles.size() * 2);
+  if (d.aux_dims) {
+    const float *waux = prm.layers + (nw - 64 * 16);
+    build_aux_tiles(waux, Elem::BF16, tiles);
+    s->aux_bf16.upload(tiles.data(), tiles.size() * 2);
+    build_aux_tiles(waux, Elem::F16, tiles);
+    s->aux_f16.upload(tiles.data(), tiles.size() * 2);
+  }
   const int id = ctx->next_id++;
   ctx->scenes[id] = std::move(s);
   *scene_id = id;
@@ -311,6 +320,24 @@ void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const 
   a.out_elem = elem;
   a.dbg_indices = dbg_idx;
   a.dbg_features = dbg_feat;
+  if (sc.desc.aux_dims && dbg_idx == nullptr) {
+    const float *f4[] = {gb->normal, gb->reflect, gb->kd, gb->ks};
+    for (const float *p : f4) {
+      require(p != nullptr, NWPRT_ERR_INVALID_ARGUMENT,
+              "G-buffer normal, reflect, kd and ks are required by this scene's aux inputs");
+      check_aligned(p, 16, "G-buffer aux input");
+    }
+    require(gb->roughness != nullptr, NWPRT_ERR_INVALID_ARGUMENT,
+            "G-buffer roughness is required by this scene's aux inputs");
+    slot.aux.ensure((size_t)P * 16 * (elem == Elem::F32 ? 4 : 2));
+    a.normal = gb->normal;
+    a.reflect = gb->reflect;
+    a.roughness = gb->roughness;
+    a.kd = gb->kd;
+    a.ks = gb->ks;
+    a.aux = slot.aux.p;
+    a.aux_pad = sc.desc.aux_pad_value;
+  }
   launch_encode(a, s);
   slot.valid = true;
   slot.scene = scene_id;
@@ -354,6 +381,11 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   a.dump = dump;
   a.dump_begin = dump_begin;
   a.dump_count = dump_count;
+  if (sc.desc.aux_dims) {
+    a.aux = slot.aux.p;
+    a.waux_f32 = sc.waux_f32();
+    a.waux_tiles = elem == Elem::BF16 ? sc.aux_bf16.as<uint16_t>() : sc.aux_f16.as<uint16_t>();
+  }
   if (elem == Elem::F32) {
     slot.wsel.ensure((size_t)env.k * 32 * 4);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,

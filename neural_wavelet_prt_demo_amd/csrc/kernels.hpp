@@ -48,6 +48,11 @@ struct EncodeArgs {
   const float *projection;    // [32][32] or null (identity)
   void *features;             // [n][32] in out_elem
   Elem out_elem;
+  // Aux G-buffer encodings (null aux: none): 13 values per pixel from the
+  // G-buffer plus 3 padding values, written as [n][16] in out_elem.
+  const float *normal, *reflect, *roughness, *kd, *ks;
+  void *aux;
+  float aux_pad;
   // debug tap (null when unused)
   uint32_t *dbg_indices;      // [n][L][8]
   float *dbg_features;        // [n][32] unprojected FP32
@@ -65,6 +70,12 @@ struct RelightArgs {
   const float *light;         // [K][3]
   const uint16_t *w_tiles;    // MFMA: W2 | W3 | W4 variants, [4][2][64][8] each
   const float *w_f32;         // plain: W1 [64][32] | W2 | W3 [64][64] | W4 [3][64]
+  // Aux inputs of the second layer (null: none): per-pixel [P][16] encodings
+  // (Elem per kernel) and the second layer's aux columns, as MFMA A tiles
+  // [4][64][4] (fused) or FP32 [64][16] (plain).
+  const void *aux;
+  const uint16_t *waux_tiles;
+  const float *waux_f32;
   const uint8_t *env_flag;
   const float *direct;        // float4
   float *indirect;            // float4, may be null
@@ -83,7 +94,9 @@ void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream
 void launch_relight_plain(const RelightArgs &a, hipStream_t s);
 void launch_tonemap(const float *src, uint32_t *dst, int n_pixels, hipStream_t s);
 
-// Host-side MFMA weight tiling of layers 2..4 (see relight kernels).
+// Host-side MFMA weight tiling of layers 2..4 and of the second layer's aux
+// columns (see relight kernels).
 void build_weight_tiles(const float *w_f32, Elem elem, std::vector<uint16_t> &out);
+void build_aux_tiles(const float *waux, Elem elem, std::vector<uint16_t> &out);
 
 }  // namespace nwprt

@@ -1,0 +1,154 @@
+// nwprt_offline -- offline trajectory relight on the RenderEngine facade.
+//
+// Mirrors the wavelet half of the reference's offline mode
+// (Renderer/offlineRenderer.cpp:12-32, 70-142; flags of Renderer/main.cpp:
+// 64-89): read a trajectory file, and for each entry in [start, end) load
+// the entry's environment map from the env folder, run the two TCNN passes
+// and write predictedIndirect/<i>_<name>.hdr and predictedWithDirect/
+// <i>_<name>.hdr, rows flipped as OutputWriter::writeRGBA does
+// (Renderer/OutputWriter.cpp:24-40, 146-149).
+//
+// The OptiX G-buffer pass is not part of this build: frame i's G-buffer is
+// the seeded synthetic one with seed (--seed + i); the camera columns of the
+// trajectory are parsed and ignored.
+//
+//   nwprt_offline <results_dir> -w <model_dir> -m <env_folder> -t <trajectory>
+//                 [-s start] [-e end] [-l] [-k K] [--seed S] [--precision bf16|fp16|fp32]
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <vector>
+
+#include "../../include/nwprt_render_engine.hpp"
+
+namespace fs = std::filesystem;
+
+namespace {
+
+struct Trajectory {
+  std::string envmapFolder;
+  bool finalDenoiser = false;
+  std::vector<std::string> envmaps;
+};
+
+// Renderer/offlineRenderer.cpp:70-123: keywords before "trajectoryBegin",
+// then one "px py pz lx ly lz envname" line per frame.
+Trajectory readTrajectory(const std::string &path) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("cannot open trajectory file " + path);
+  Trajectory t;
+  bool started = false;
+  std::string line;
+  const char *keywords[] = {"samplesPerPixel", "directSamplesPerLight", "IndirectSamplesPerDirect",
+                            "finalDenoiser", "directIndirectDenoiser", "trainingMode",
+                            "waveletSamplesPerPixel", "waveletSamplesPerLight",
+                            "waveletDirectDenoiser"};
+  while (std::getline(in, line)) {
+    line.erase(std::remove_if(line.begin(), line.end(),
+                              [](char c) { return c == '\t' || c == '\r' || c == '\n'; }),
+               line.end());
+    if (line.empty()) continue;
+    std::istringstream ss(line);
+    std::vector<std::string> tok;
+    for (std::string w; std::getline(ss, w, ' ');)
+      if (!w.empty()) tok.push_back(w);
+    if (tok.empty()) continue;
+    if (tok[0] == "trajectoryBegin") {
+      started = true;
+    } else if (tok[0] == "envmapFolder" && tok.size() > 1) {
+      t.envmapFolder = tok[1];
+    } else if (tok[0] == "waveletFinalDenoiser" && tok.size() > 1) {
+      t.finalDenoiser = tok[1] == "true";
+    } else if (std::find_if(std::begin(keywords), std::end(keywords),
+                            [&](const char *k) { return tok[0] == k; }) != std::end(keywords)) {
+      continue;
+    } else {
+      if (!started) throw std::runtime_error("Unknown things before trajectory start " + tok[0]);
+      if (tok.size() < 7) throw std::runtime_error("trajectory line needs 7 fields: " + line);
+      for (int i = 0; i < 6; ++i) (void)std::stof(tok[(size_t)i]);  // camera, validated only
+      t.envmaps.push_back(tok[6]);
+    }
+  }
+  return t;
+}
+
+void writeFlipped(const fs::path &file, std::vector<float> rgba, int w, int h) {
+  for (int y = 0; y < h / 2; ++y)
+    std::swap_ranges(rgba.begin() + (size_t)y * w * 4, rgba.begin() + (size_t)(y + 1) * w * 4,
+                     rgba.begin() + (size_t)(h - 1 - y) * w * 4);
+  fs::create_directories(file.parent_path());
+  if (nwprt_hdr_write(file.string().c_str(), rgba.data(), w, h, 4) != NWPRT_OK)
+    throw std::runtime_error(std::string("writing ") + file.string() + ": " + nwprt_last_error());
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string results, model, envFolder, trajFile, precision = "bf16";
+  int start = 0, end = 2000, k = 64;
+  bool large = false;
+  uint64_t seed = 0;
+  for (int i = 1; i < argc; ++i) {
+    const std::string a = argv[i];
+    auto next = [&]() -> std::string {
+      if (i + 1 >= argc) throw std::runtime_error("missing value for " + a);
+      return argv[++i];
+    };
+    try {
+      if (a == "-w") model = next();
+      else if (a == "-m") envFolder = next();
+      else if (a == "-t") trajFile = next();
+      else if (a == "-s") start = std::stoi(next());
+      else if (a == "-e") end = std::stoi(next());
+      else if (a == "-k") k = std::stoi(next());
+      else if (a == "-l") large = true;
+      else if (a == "--seed") seed = std::stoull(next());
+      else if (a == "--precision") precision = next();
+      else if (!a.empty() && a[0] != '-' && results.empty()) results = a;
+      else throw std::runtime_error("unknown argument " + a);
+    } catch (const std::exception &e) {
+      std::cerr << "nwprt_offline: " << e.what() << "\n";
+      return 2;
+    }
+  }
+  if (results.empty() || model.empty() || trajFile.empty()) {
+    std::cerr << "usage: nwprt_offline <results_dir> -w <model_dir> -m <env_folder> -t "
+                 "<trajectory> [-s start] [-e end] [-l] [-k K] [--seed S] [--precision p]\n";
+    return 2;
+  }
+  try {
+    nwprt::renderConfig cfg;
+    cfg.precision = precision == "fp32"   ? NWPRT_PREC_FP32
+                    : precision == "fp16" ? NWPRT_PREC_FP16
+                                          : NWPRT_PREC_BF16;
+    const Trajectory traj = readTrajectory(trajFile);
+    if (envFolder.empty()) envFolder = traj.envmapFolder;
+    cfg.finalDenoiser = cfg.indirectDenoiser = traj.finalDenoiser;
+    const int w = large ? 800 : 512, h = large ? 600 : 512;  // Renderer/main.cpp:156
+    nwprt::RenderEngine engine(0);
+    engine.resize(w, h);
+    engine.initTCNN(model);
+    end = std::min<int>(end, (int)traj.envmaps.size());
+    for (int i = start; i < end; ++i) {
+      engine.buildLights({{envFolder, traj.envmaps[(size_t)i]}}, k);
+      engine.writeSyntheticGBuffer(1, seed + (uint64_t)i);
+      engine.TCNNPass_p1(1, nullptr, cfg);
+      engine.TCNNPass_p2(0, 1, cfg);
+      std::vector<float> ind, wd;
+      engine.downloadPredictions(1, ind, wd);
+      const std::string name = std::to_string(i) + "_" + traj.envmaps[(size_t)i] + ".hdr";
+      writeFlipped(fs::path(results) / "predictedIndirect" / name, ind, w, h);
+      writeFlipped(fs::path(results) / "predictedWithDirect" / name, wd, w, h);
+      std::cout << "Trajectory: " << i << " Done. Total is: " << end << std::endl;
+    }
+    engine.destroyLights();
+  } catch (const std::exception &e) {
+    std::cerr << "nwprt_offline: " << e.what() << "\n";
+    return 1;
+  }
+  return 0;
+}

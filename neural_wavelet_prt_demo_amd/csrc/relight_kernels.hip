@@ -54,6 +54,28 @@ struct Mfma<_Float16> {
   }
 };
 
+// 16x16x16 MFMA for the 16-wide aux block of the second layer: lane (g, n)
+// holds k-slots 4g..4g+3 of row / column n.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+template <typename E>
+struct Mfma16;
+template <>
+struct Mfma16<__bf16> {
+  static __device__ __forceinline__ f32x4 run(u32x2 a, u32x2 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(bf16x4, a),
+                                                     __builtin_bit_cast(bf16x4, b), c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma16<_Float16> {
+  static __device__ __forceinline__ f32x4 run(u32x2 a, u32x2 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(f16x4, a),
+                                                 __builtin_bit_cast(f16x4, b), c, 0, 0, 0);
+  }
+};
+
 // ReLU + round of two 16-row D blocks (output features 32kb + [0,32)) into
 // one B operand: slots 0..3 <- block 2kb rows 4g..4g+3, slots 4..7 <- block
 // 2kb+1 rows 4g..4g+3.
@@ -101,9 +123,10 @@ constexpr int kNumMfmaVariants = sizeof(kMfmaVariants) / sizeof(kMfmaVariants[0]
 // The four layers and the clamped, compensated accumulation for one selected
 // wavelet k (A operands `a1` = its first-layer tiles) over the wave's 64
 // pixels.
-template <typename E, bool SOFTPLUS, bool DUMP, int GP, bool W4LDS>
+template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS>
 __device__ __forceinline__ void wavelet_step(const RelightArgs &a, int k, const u32x4 (&a1)[4],
-                                             const u32x4 (&hb)[4], const u32x4 (&w2)[4][2],
+                                             const u32x4 (&hb)[4], const u32x2 (&auxb)[4],
+                                             const u32x2 (&waux)[4], const u32x4 (&w2)[4][2],
                                              const u32x4 (&w3)[4][2], const u32x4 (&w4)[4][2],
                                              const u32x4 *w4s, int lane, int my_px, bool mine,
                                              float (&sum)[3], float (&comp)[3]) {
@@ -125,8 +148,11 @@ __device__ __forceinline__ void wavelet_step(const RelightArgs &a, int k, const 
 #pragma unroll
     for (int j = 0; j < GP; ++j)
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
-        c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], zero));
+      for (int b = 0; b < 4; ++b) {
+        // Second layer; with aux inputs its 16 extra columns enter first.
+        const f32x4 c0 = AUX ? Mfma16<E>::run(waux[b], auxb[jp + j], zero) : zero;
+        c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], c0));
+      }
 #pragma unroll
     for (int j = 0; j < GP; ++j) {
       x[j][0] = relu_pack<E>(c[j][0], c[j][1]);
@@ -169,7 +195,7 @@ __device__ __forceinline__ void wavelet_step(const RelightArgs &a, int k, const 
 // MFMA columns; lane l owns pixel base + l for the reduction.  The next
 // wavelet's first-layer tiles are fetched while the current one computes
 // (two register sets, K loop unrolled by two).
-template <typename E, bool SOFTPLUS, bool DUMP, int GP, bool W4LDS, int MINW>
+template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, int MINW>
 __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
   __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -197,6 +223,16 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
       const int p = base + 16 * j + n;
       hb[j] = p < P ? *(const u32x4 *)(feat + (size_t)p * 32 + 8 * g) : u32x4{0u, 0u, 0u, 0u};
     }
+    // Aux inputs (B: group j, column n, k-slots 4g..4g+3) and the second
+    // layer's aux columns (A tiles [4][64]).
+    u32x2 auxb[4], waux[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = base + 16 * j + n;
+      auxb[j] = AUX && p < P ? *(const u32x2 *)((const uint16_t *)a.aux + (size_t)p * 16 + 4 * g)
+                             : u32x2{0u, 0u};
+      waux[j] = AUX ? ((const u32x2 *)a.waux_tiles)[j * 64 + lane] : u32x2{0u, 0u};
+    }
     // Layers 2..4 as A-operand tiles: [4][2][64] each.
     u32x4 w2[4][2], w3[4][2], w4[4][2];
 #pragma unroll
@@ -216,15 +252,15 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
 #pragma unroll
         for (int b = 0; b < 4; ++b) tb[b] = mk[((size_t)(k + 1) * 4 + b) * 64 + lane];
       }
-      wavelet_step<E, SOFTPLUS, DUMP, GP, W4LDS>(a, k, ta, hb, w2, w3, w4, w4s, lane, my_px, mine,
-                                                  sum, comp);
+      wavelet_step<E, SOFTPLUS, DUMP, AUX, GP, W4LDS>(a, k, ta, hb, auxb, waux, w2, w3, w4, w4s,
+                                                       lane, my_px, mine, sum, comp);
       if (k + 1 < a.k) {
         if (k + 2 < a.k) {
 #pragma unroll
           for (int b = 0; b < 4; ++b) ta[b] = mk[((size_t)(k + 2) * 4 + b) * 64 + lane];
         }
-        wavelet_step<E, SOFTPLUS, DUMP, GP, W4LDS>(a, k + 1, tb, hb, w2, w3, w4, w4s, lane, my_px,
-                                                    mine, sum, comp);
+        wavelet_step<E, SOFTPLUS, DUMP, AUX, GP, W4LDS>(a, k + 1, tb, hb, auxb, waux, w2, w3, w4,
+                                                         w4s, lane, my_px, mine, sum, comp);
       }
     }
   }
@@ -241,10 +277,14 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
 constexpr int kPlainBlock = 256;
 constexpr int kW1 = 64 * 32, kW2 = 64 * 64, kW3 = 64 * 64, kW4 = 3 * 64;
 
-template <bool DUMP>
+constexpr int kWaux = 64 * 16;
+
+template <bool DUMP, bool AUX>
 __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs a) {
-  __shared__ float w[kW1 + kW2 + kW3 + kW4];
+  __shared__ float w[kW1 + kW2 + kW3 + kW4 + (AUX ? kWaux : 0)];
   for (int i = threadIdx.x; i < kW1 + kW2 + kW3 + kW4; i += kPlainBlock) w[i] = a.w_f32[i];
+  if (AUX)
+    for (int i = threadIdx.x; i < kWaux; i += kPlainBlock) w[kW1 + kW2 + kW3 + kW4 + i] = a.waux_f32[i];
   __syncthreads();
   const int p = blockIdx.x * kPlainBlock + threadIdx.x;
   if (p >= a.n_pixels) return;
@@ -257,6 +297,21 @@ __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs 
     for (int q = 0; q < 8; ++q) {
       const float4 v = fp[q];
       h[4 * q] = v.x; h[4 * q + 1] = v.y; h[4 * q + 2] = v.z; h[4 * q + 3] = v.w;
+    }
+    // Aux inputs enter the second layer as a per-pixel constant term.
+    float b2[64];
+#pragma unroll
+    for (int o = 0; o < 64; ++o) b2[o] = 0.f;
+    if (AUX) {
+      const float *ax = (const float *)a.aux + (size_t)p * 16;
+      const float *Wa = w + kW1 + kW2 + kW3 + kW4;
+#pragma unroll
+      for (int o = 0; o < 64; ++o) {
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc = fmaf(Wa[o * 16 + i], ax[i], acc);
+        b2[o] = acc;
+      }
     }
     for (int k = 0; k < a.k; ++k) {
       // The weight reads are loop-invariant; an opaque zero offset keeps the
@@ -276,7 +331,7 @@ __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs 
       float y2[64];
 #pragma unroll
       for (int o = 0; o < 64; ++o) {
-        float acc = 0.f;
+        float acc = b2[o];
 #pragma unroll
         for (int i = 0; i < 64; ++i) acc = fmaf(W2[o * 64 + i], y1[i], acc);
         y2[o] = fmaxf(acc, 0.f);
@@ -317,12 +372,20 @@ __global__ __launch_bounds__(256) void tonemap_kernel(const float *__restrict__ 
 template <typename E, bool SP, int V>
 void launch_v(const RelightArgs &a, dim3 grid, dim3 block, hipStream_t s) {
   constexpr MfmaVariant v = kMfmaVariants[V];
-  if (a.dump && V == 0)
-    hipLaunchKernelGGL((relight_mfma_kernel<E, SP, true, v.gp, v.w4_lds, v.min_waves>), grid, block,
-                       0, s, a);
-  else
-    hipLaunchKernelGGL((relight_mfma_kernel<E, SP, false, v.gp, v.w4_lds, v.min_waves>), grid,
-                       block, 0, s, a);
+  if (a.aux) {
+    // Aux scenes are built for variant 0's tile shape (tap included).
+    if (a.dump)
+      hipLaunchKernelGGL((relight_mfma_kernel<E, SP, true, true, 2, false, 2>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((relight_mfma_kernel<E, SP, false, true, 2, false, 2>), grid, block, 0, s,
+                         a);
+  } else if (a.dump && V == 0) {
+    hipLaunchKernelGGL((relight_mfma_kernel<E, SP, true, false, v.gp, v.w4_lds, v.min_waves>),
+                       grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((relight_mfma_kernel<E, SP, false, false, v.gp, v.w4_lds, v.min_waves>),
+                       grid, block, 0, s, a);
+  }
 }
 
 template <typename E, bool SP>
@@ -379,11 +442,25 @@ const char *relight_mfma_variant_name(int v) {
   return v >= 0 && v < kNumMfmaVariants ? kMfmaVariants[v].name : nullptr;
 }
 
+// A tiles of the second layer's 16 aux columns for v_mfma_f32_16x16x16:
+// tile [b][lane][s] = Waux[16 b + lane%16][4 (lane/16) + s].
+void build_aux_tiles(const float *waux, Elem elem, std::vector<uint16_t> &out) {
+  out.assign(4 * 64 * 4, 0);
+  for (int b = 0; b < 4; ++b)
+    for (int lane = 0; lane < 64; ++lane)
+      for (int s = 0; s < 4; ++s) {
+        const float v = waux[(16 * b + (lane & 15)) * 16 + 4 * (lane >> 4) + s];
+        out[((size_t)b * 64 + lane) * 4 + s] =
+            elem == Elem::BF16 ? f32_to_bf16_bits(v) : f32_to_f16_bits(v);
+      }
+}
+
 void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream_t s) {
   require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: bad pixel count or K");
   require(a.features && a.mk && a.light && a.w_tiles && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: missing operand");
+  require(!a.aux || a.waux_tiles, NWPRT_ERR_INVALID_ARGUMENT, "relight: missing aux weights");
   if (elem == Elem::BF16) {
     launch_mfma_variant<__bf16>(a, variant, s);
   } else if (elem == Elem::F16) {
@@ -399,9 +476,15 @@ void launch_relight_plain(const RelightArgs &a, hipStream_t s) {
           "relight: bad pixel count or K");
   require(a.features && a.wsel && a.light && a.w_f32 && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: missing operand");
-  const dim3 grid((a.n_pixels + kPlainBlock - 1) / kPlainBlock);
-  if (a.dump) hipLaunchKernelGGL((relight_plain_kernel<true>), grid, dim3(kPlainBlock), 0, s, a);
-  else hipLaunchKernelGGL((relight_plain_kernel<false>), grid, dim3(kPlainBlock), 0, s, a);
+  const dim3 grid((a.n_pixels + kPlainBlock - 1) / kPlainBlock), block(kPlainBlock);
+  if (a.aux) {
+    require(a.waux_f32 != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "relight: missing aux weights");
+    if (a.dump) hipLaunchKernelGGL((relight_plain_kernel<true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((relight_plain_kernel<false, true>), grid, block, 0, s, a);
+  } else {
+    if (a.dump) hipLaunchKernelGGL((relight_plain_kernel<true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((relight_plain_kernel<false, false>), grid, block, 0, s, a);
+  }
   hip_check(hipGetLastError(), "relight_plain_kernel launch");
 }
 

@@ -109,6 +109,11 @@ void synth_scene(const nwprt_scene_desc &d, uint64_t seed, float hash_std,
         layers[off + i] = (float)(std_ * s.normal(i));
       off += (size_t)fan_in * fan_out;
     }
+    if (d.aux_dims) {  // second layer's 16 aux columns, He-normal over its 64 + 16 inputs
+      Stream s(seed, kLayer0 + (uint64_t)d.n_hidden_layers + 1);
+      const double std_ = std::sqrt(2.0 / (H + 16));
+      for (size_t i = 0; i < (size_t)H * 16; ++i) layers[off + i] = (float)(std_ * s.normal(i));
+    }
   }
 }
 

@@ -23,7 +23,20 @@ class OrcScene(Structure):
                 ("offset", c_void_p), ("table", c_void_p), ("n_coef", c_int),
                 ("wfeat", c_void_p), ("proj", c_void_p), ("layers", c_void_p),
                 ("feature_dim", c_int), ("hidden", c_int), ("n_hidden_layers", c_int),
-                ("softplus", c_int)]
+                ("softplus", c_int), ("aux", c_int), ("aux_pad", c_float), ("waux", c_void_p)]
+
+
+class OrcGbuffer(Structure):
+    _fields_ = [("position", c_void_p), ("normal", c_void_p), ("reflect", c_void_p),
+                ("roughness", c_void_p), ("kd", c_void_p), ("ks", c_void_p),
+                ("direct", c_void_p), ("env_flag", c_void_p)]
+
+
+def _gbuf(g):
+    keep = {k: np.ascontiguousarray(v) for k, v in g.items()}
+    return keep, OrcGbuffer(*[_p(keep.get(k)) for k in ("position", "normal", "reflect",
+                                                         "roughness", "kd", "ks", "direct",
+                                                         "env_flag")])
 
 
 _lib = None
@@ -47,13 +60,14 @@ def lib():
         l.orc_score.argtypes = [c_void_p, c_int, c_int]
         l.orc_score.restype = c_float
         l.orc_topk.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p]
-        l.orc_relight_pixels.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int, c_void_p,
-                                         c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+        l.orc_relight_pixels.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int,
+                                         POINTER(OrcGbuffer), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p, c_int]
         l.orc_tonemap.argtypes = [c_float, c_float, c_float]
         l.orc_tonemap.restype = c_uint32
-        l.orc_cpu_relight_fp32.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int, c_void_p,
-                                           c_void_p, c_void_p, c_int, c_int, c_void_p, c_int]
+        l.orc_aux_inputs.argtypes = [POINTER(OrcGbuffer), c_int, c_float, c_void_p]
+        l.orc_cpu_relight_fp32.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int,
+                                           POINTER(OrcGbuffer), c_int, c_int, c_void_p, c_int]
         l.orc_cpu_relight_fp32.restype = c_double
         _lib = l
     return _lib
@@ -78,29 +92,37 @@ class Scene:
 
     def __init__(self, levels: Dict[str, np.ndarray], n_features: int, table: np.ndarray,
                  wfeat: np.ndarray, proj: Optional[np.ndarray], layers: np.ndarray,
-                 softplus: bool = True, hidden: int = 64, n_hidden_layers: int = 3):
+                 softplus: bool = True, hidden: int = 64, n_hidden_layers: int = 3,
+                 waux: Optional[np.ndarray] = None, aux_pad: float = 1.0):
         self.keep = dict(levels)
         self.keep["table"] = np.ascontiguousarray(table, np.float32)
         self.keep["wfeat"] = np.ascontiguousarray(wfeat, np.float32)
         self.keep["proj"] = None if proj is None else np.ascontiguousarray(proj, np.float32)
         self.keep["layers"] = np.ascontiguousarray(layers, np.float32)
+        self.keep["waux"] = None if waux is None else np.ascontiguousarray(waux, np.float32)
         feature_dim = self.keep["wfeat"].shape[0]
         k = self.keep
         self.s = OrcScene(len(levels["scale"]), n_features, _p(k["scale"]), _p(k["resolution"]),
                           _p(k["dense"]), _p(k["size"]), _p(k["offset"]), _p(k["table"]),
                           k["wfeat"].shape[1], _p(k["wfeat"]), _p(k["proj"]), _p(k["layers"]),
-                          feature_dim, hidden, n_hidden_layers, int(softplus))
+                          feature_dim, hidden, n_hidden_layers, int(softplus),
+                          int(waux is not None), aux_pad, _p(k["waux"]))
 
     @classmethod
     def from_synth(cls, levels, desc, params):
         """From a product SceneDesc + synth_scene() arrays; FP16 tables are
-        widened with numpy (exact), not with product code."""
+        widened with numpy (exact), not with product code.  With aux inputs
+        the last 64 x 16 weights are the second layer's aux columns."""
         tab = params["hash_table"]
         if tab.dtype == np.uint16:
             tab = tab.view(np.float16).astype(np.float32)
+        layers = np.asarray(params["layers"], np.float32)
+        waux = None
+        if desc.aux_dims:
+            waux, layers = layers[-64 * 16:].reshape(64, 16), layers[:-64 * 16]
         return cls(levels, desc.grid.n_features_per_level, tab, params["wavelet_features"],
-                   params["projection"] if desc.has_projection else None, params["layers"],
-                   bool(desc.output_softplus))
+                   params["projection"] if desc.has_projection else None, layers,
+                   bool(desc.output_softplus), waux=waux, aux_pad=desc.aux_pad_value)
 
     def hash_encode(self, pos3: np.ndarray):
         L = self.s.n_levels
@@ -118,20 +140,26 @@ class Scene:
         ind = np.zeros((n, 3), np.float64)
         wd = np.zeros((n, 3), np.float64)
         tr = np.zeros((n, len(sel), 3), np.float64) if transport else None
-        flag = gbuf.get("env_flag")
+        keep, g = _gbuf(gbuf)
         lib().orc_relight_pixels(ctypes.byref(self.s), _p(sel), _p(light), len(sel),
-                                 _p(gbuf["position"]), _p(flag), _p(gbuf["direct"]), _p(pix), n,
-                                 _p(ind), _p(wd), _p(tr), threads)
+                                 ctypes.byref(g), _p(pix), n, _p(ind), _p(wd), _p(tr), threads)
         return (ind, wd, tr) if transport else (ind, wd)
 
     def cpu_relight_fp32(self, sel, light, gbuf, begin, count, threads):
         sel = np.ascontiguousarray(sel, np.int32)
         light = np.ascontiguousarray(light, np.float32)
         out = np.zeros((count, 3), np.float32)
+        keep, g = _gbuf(gbuf)
         secs = lib().orc_cpu_relight_fp32(ctypes.byref(self.s), _p(sel), _p(light), len(sel),
-                                          _p(gbuf["position"]), _p(gbuf.get("env_flag")),
-                                          _p(gbuf["direct"]), begin, count, _p(out), threads)
+                                          ctypes.byref(g), begin, count, _p(out), threads)
         return out, secs
+
+
+def aux_inputs(gbuf, p: int, pad: float = 1.0) -> np.ndarray:
+    keep, g = _gbuf(gbuf)
+    out = np.zeros(16, np.float32)
+    lib().orc_aux_inputs(ctypes.byref(g), p, pad, _p(out))
+    return out
 
 
 def envmap_haar(faces: np.ndarray) -> np.ndarray:

@@ -52,19 +52,27 @@ static void encode_fp32(const orc_scene *sc, const float pos[3], float *out) {
 /* Relights pixels [pixel_begin, pixel_begin + n_pixels) into indirect
  * (float3 per pixel) and returns the elapsed wall time in seconds. */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
-                            const float *position, const uint8_t *env_flag,
-                            const float *direct, int pixel_begin, int n_pixels, float *indirect,
-                            int n_threads) {
-  (void)direct;
+                            const orc_gbuffer *gb, int pixel_begin, int n_pixels,
+                            float *indirect, int n_threads) {
   const int Fd = sc->feature_dim, H = sc->hidden, NH = sc->n_hidden_layers;
   const double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(n_threads > 0 ? n_threads : 1)
   for (int q = 0; q < n_pixels; ++q) {
     const int p = pixel_begin + q;
     float rgb[3] = {0.f, 0.f, 0.f};
-    if (env_flag == NULL || env_flag[p]) {
-      float hp[64], z[64], a[128], b[128];
-      encode_fp32(sc, position + 4 * (size_t)p, hp);
+    if (gb->env_flag == NULL || gb->env_flag[p]) {
+      float hp[64], z[64], a[128], b[128], b2[128];
+      for (int o = 0; o < H; ++o) b2[o] = 0.f;
+      if (sc->aux) {
+        float ax[16];
+        orc_aux_inputs(gb, p, sc->aux_pad, ax);
+        for (int o = 0; o < H; ++o) {
+          float acc = 0.f;
+          for (int i = 0; i < 16; ++i) acc += sc->waux[o * 16 + i] * ax[i];
+          b2[o] = acc;
+        }
+      }
+      encode_fp32(sc, gb->position + 4 * (size_t)p, hp);
       for (int kk = 0; kk < k; ++kk) {
         const int c = sel[kk];
         for (int f = 0; f < Fd; ++f) z[f] = hp[f] * sc->wfeat[(size_t)f * sc->n_coef + c];
@@ -77,7 +85,7 @@ double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float
         W += (size_t)H * Fd;
         for (int l = 1; l < NH; ++l) {
           for (int o = 0; o < H; ++o) {
-            float acc = 0.f;
+            float acc = l == 1 ? b2[o] : 0.f;
             for (int i = 0; i < H; ++i) acc += W[o * H + i] * a[i];
             b[o] = acc > 0.f ? acc : 0.f;
           }

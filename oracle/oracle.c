@@ -200,6 +200,26 @@ void orc_topk(const float *coef, int n, int k, int32_t *indices, float *strength
 
 static double softplus_d(double x) { return x > 0 ? x + log1p(exp(-x)) : log1p(exp(x)); }
 
+/* Per-pixel rows 3..15 of the reference's per-pair input
+ * (Renderer/tinycudann/utils_kernels.h:31-46 with the offsets of
+ * prt_globals.h:4-13), in the FP32 the reference computes them in: normal
+ * and reflection direction as stored (already v/2 + 0.5), roughness as
+ * log(25 r + 1) * (1 / log 26) (the multiply-add fused as a CUDA build
+ * contracts it), 2 kd - 1, 2 ks - 1; then three padding inputs. */
+void orc_aux_inputs(const orc_gbuffer *gb, int p, float pad, float out[16]) {
+  const float *n = gb->normal + 4 * (size_t)p, *r = gb->reflect + 4 * (size_t)p;
+  const float *kd = gb->kd + 4 * (size_t)p, *ks = gb->ks + 4 * (size_t)p;
+  const float rgh_den = 1.0f / logf(26.0f);
+  for (int c = 0; c < 3; ++c) {
+    out[c] = n[c];
+    out[3 + c] = r[c];
+    out[7 + c] = fmaf(2.0f, kd[c], -1.0f);
+    out[10 + c] = fmaf(2.0f, ks[c], -1.0f);
+  }
+  out[6] = logf(fmaf(25.0f, gb->roughness[p], 1.0f)) * rgh_den;
+  out[13] = out[14] = out[15] = pad;
+}
+
 /* Relight of selected pixels in FP64.  Per pixel p with env_flag set
  * (Renderer/tinycudann/utils_kernels.h:72):
  *   h = hash(pos) (orc_hash_encode), h' = proj h (or h)
@@ -208,20 +228,31 @@ static double softplus_d(double x) { return x > 0 ? x + log1p(exp(-x)) : log1p(e
  *     T = out_act(W_out relu(... relu(W1 z)))
  *     rgb += max(light[k] * T, 0) per channel (utils_kernels.h:74-78)
  * indirect = rgb, with_direct = rgb + direct (utils_kernels.h:80-88).
+ * With aux inputs, waux * aux(p) joins the second layer's pre-activation
+ * (the folded cmp_net input [out_prd | cmp_enc], prt_network.h:113-139).
  * Background pixels give 0.  transport (optional): [n][K][3]. */
 void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *light, int k,
-                        const float *position, const uint8_t *env_flag, const float *direct,
-                        const int32_t *pixels, int n_pixels, double *indirect,
-                        double *with_direct, double *transport, int n_threads) {
+                        const orc_gbuffer *gb, const int32_t *pixels, int n_pixels,
+                        double *indirect, double *with_direct, double *transport, int n_threads) {
   const int Fd = sc->feature_dim, H = sc->hidden, NH = sc->n_hidden_layers;
 #pragma omp parallel for schedule(dynamic, 4) num_threads(n_threads > 0 ? n_threads : 1)
   for (int q = 0; q < n_pixels; ++q) {
     const int p = pixels ? pixels[q] : q;
     double rgb[3] = {0, 0, 0};
-    const int hit = env_flag == NULL || env_flag[p] != 0;
+    const int hit = gb->env_flag == NULL || gb->env_flag[p] != 0;
     if (hit) {
-      double feat[64], hp[64], z[64], a[128], b[128];
-      orc_hash_encode(sc, position + 4 * (size_t)p, NULL, feat);
+      double feat[64], hp[64], z[64], a[128], b[128], b2[128];
+      for (int o = 0; o < H; ++o) b2[o] = 0;
+      if (sc->aux) {
+        float ax[16];
+        orc_aux_inputs(gb, p, sc->aux_pad, ax);
+        for (int o = 0; o < H; ++o) {
+          double acc = 0;
+          for (int i = 0; i < 16; ++i) acc += (double)sc->waux[o * 16 + i] * (double)ax[i];
+          b2[o] = acc;
+        }
+      }
+      orc_hash_encode(sc, gb->position + 4 * (size_t)p, NULL, feat);
       for (int o = 0; o < Fd; ++o) {
         if (sc->proj) {
           double acc = 0;
@@ -243,7 +274,7 @@ void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *li
         W += (size_t)H * Fd;
         for (int l = 1; l < NH; ++l) {
           for (int o = 0; o < H; ++o) {
-            double acc = 0;
+            double acc = l == 1 ? b2[o] : 0;
             for (int i = 0; i < H; ++i) acc += (double)W[o * H + i] * a[i];
             b[o] = acc > 0 ? acc : 0;
           }
@@ -264,7 +295,7 @@ void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *li
     }
     for (int ch = 0; ch < 3; ++ch) {
       indirect[3 * q + ch] = rgb[ch];
-      with_direct[3 * q + ch] = rgb[ch] + (double)direct[4 * (size_t)p + ch];
+      with_direct[3 * q + ch] = rgb[ch] + (double)gb->direct[4 * (size_t)p + ch];
     }
   }
 }

@@ -53,7 +53,19 @@ typedef struct {
   int feature_dim, hidden;    /* 32, 64                               */
   int n_hidden_layers;        /* 3                                    */
   int softplus;
+  int aux;                    /* 1: 13 G-buffer encodings + 3 pads    */
+  float aux_pad;              /*    enter the second layer through    */
+  const float *waux;          /*    waux [H][16]                      */
 } orc_scene;
+
+/* G-buffer arrays (reference layout: float4 except roughness / env_flag). */
+typedef struct {
+  const float *position, *normal, *reflect, *roughness, *kd, *ks, *direct;
+  const uint8_t *env_flag;
+} orc_gbuffer;
+
+/* The 13 aux encodings of one pixel plus 3 pads (FP32). */
+void orc_aux_inputs(const orc_gbuffer *gb, int p, float pad, float out[16]);
 
 void orc_hashgrid_levels(int n_levels, int log2_hashmap_size, int base_resolution,
                          float per_level_scale, float *scale, uint32_t *resolution,
@@ -68,17 +80,15 @@ float orc_score(const float *coef, int t, int n);
 void orc_topk(const float *coef, int n, int k, int32_t *indices, float *strengths);
 
 void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *light, int k,
-                        const float *position, const uint8_t *env_flag, const float *direct,
-                        const int32_t *pixels, int n_pixels, double *indirect,
-                        double *with_direct, double *transport, int n_threads);
+                        const orc_gbuffer *gb, const int32_t *pixels, int n_pixels,
+                        double *indirect, double *with_direct, double *transport, int n_threads);
 
 uint32_t orc_tonemap(float r, float g, float b);
 
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
-                            const float *position, const uint8_t *env_flag,
-                            const float *direct, int pixel_begin, int n_pixels, float *indirect,
-                            int n_threads);
+                            const orc_gbuffer *gb, int pixel_begin, int n_pixels,
+                            float *indirect, int n_threads);
 
 #ifdef __cplusplus
 }

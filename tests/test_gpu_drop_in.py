@@ -1,0 +1,159 @@
+"""GPU tests of the drop-in boundary: aux G-buffer inputs, the reference's
+weight-folder layout (RenderEngine::initTCNN) and env-map folder layout
+(Environment_Map(folder, name))."""
+import json
+import math
+
+import numpy as np
+import pytest
+
+from cases import _f16, _faces_to_cross, _write_model_folder, make_case
+from conftest import max_rel_err, psnr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16", 2e-2), ("fp16", 5e-3)])
+def test_aux_scene_vs_oracle(nw, orc, ctx, precision, tol):
+    case = make_case(nw, orc, width=80, height=48, k=48, aux=True, hash_std=0.5,
+                     feature_std=0.5, background=0.1, seed=4)
+    scene = ctx.scene_create(case.desc, case.params)
+    env = ctx.envmap_from_faces(case.faces, case.k)
+    frame = nw.DeviceFrame(case.width, case.height, case.gbuf)
+    ctx.relight(scene, env, frame, nw.PRECISIONS[precision])
+    ctx.sync()
+    got = frame.results()["indirect"][:, :3]
+    idx, st = ctx.envmap_selection(env)
+    ref, _ = case.oscene.relight(idx, st, case.gbuf)
+    print(f"aux {precision}: max rel {max_rel_err(ref, got):.2e}, PSNR {psnr(ref, got):.1f}")
+    assert max_rel_err(ref, got) <= tol
+    # the aux inputs matter: dropping them changes the image
+    plain = orc.Scene.from_synth(case.levels, nw.config_scene_desc(32, nw.DTYPE_F32),
+                                 {**case.params, "layers": case.params["layers"][:-64 * 16]})
+    ref_noaux, _ = plain.relight(idx, st, case.gbuf)
+    assert max_rel_err(ref, ref_noaux) > 10 * tol
+
+
+def test_model_folder_matches_reference_graph(nw, orc, ctx, tmp_path):
+    rng = np.random.default_rng(9)
+    n_face, K, W, H = 8, 24, 32, 24
+    w, lv = _write_model_folder(tmp_path / "model", rng, n_face)
+    scene = ctx.initTCNN(tmp_path / "model")
+    d = ctx.scene_desc(scene)
+    assert (d.aux_dims, d.face_size, d.has_projection, d.output_softplus) == (13, n_face, 0, 1)
+    gbuf = nw.synth_gbuffer(3, W, H, 0.0)
+    faces = nw.synth_cubemap(3, n_face)
+    env = ctx.envmap_from_faces(faces, K)
+    frame = nw.DeviceFrame(W, H, gbuf)
+    ctx.relight(scene, env, frame, nw.PREC_FP32)
+    ctx.sync()
+    got = frame.results()["indirect"][:, :3]
+    sel, light = ctx.envmap_selection(env)
+
+    # Unfolded reference graph in FP64 on the FP16-rounded parameters
+    # (Renderer/tinycudann/prt_network.h:73-139, tcnn.cu:30-38).
+    olv = orc.hashgrid_levels(16, 14, 16, 1.5)
+    enc = orc.Scene(olv, 2, _f16(w["vtx"]).reshape(-1, 2), np.zeros((32, 6 * n_face ** 2)), None,
+                    np.zeros(64 * 32 + 2 * 64 * 64 + 3 * 64, np.float32))
+    o1, o2, c1, c2, c3, wve = (_f16(w[k]) for k in ("o1", "o2", "c1", "c2", "c3", "wve"))
+    ref = np.zeros((W * H, 3))
+    for p in range(W * H):
+        _, h = enc.hash_encode(gbuf["position"][p, :3])
+        z = h[None, :] * wve[:, sel].T
+        o = np.maximum(z @ o1.T, 0) @ o2.T                       # out_prd
+        n, r, kd, ks = (gbuf[k][p, :3].astype(np.float64) for k in ("normal", "reflect", "kd", "ks"))
+        rough = math.log(25 * float(gbuf["roughness"][p]) + 1) / math.log(26)
+        aux = np.concatenate([n, r, [rough], 2 * kd - 1, 2 * ks - 1, [1.0, 1.0, 1.0]])
+        x = np.concatenate([o, np.broadcast_to(aux, (K, 16))], axis=1)   # [out_prd | cmp_enc]
+        y = np.maximum(np.maximum(x @ c1.T, 0) @ c2.T, 0) @ c3[:3].T     # cmp_net
+        t = np.logaddexp(0, y)
+        ref[p] = np.maximum(light * t, 0).sum(axis=0)
+    err = max_rel_err(ref, got)
+    print(f"model folder fp32: max rel err {err:.2e}")
+    assert err <= 1e-4
+
+
+def test_model_folder_errors(nw, ctx, tmp_path):
+    rng = np.random.default_rng(1)
+    _write_model_folder(tmp_path / "m", rng, 4)
+    np.save(tmp_path / "m" / "cmp_net.npy", np.zeros(100, np.float32))
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.initTCNN(tmp_path / "m")
+    assert e.value.status == nw.ERR_FORMAT
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.initTCNN(tmp_path / "missing")
+    assert e.value.status == nw.ERR_IO
+
+
+def test_envmap_folder_layout(nw, orc, ctx, tmp_path):
+    """Environment_Map(folder, name): cube/<name>.hdr always; the selection
+    comes from waveletcoeffs/ + waveletstrength/ when both exist, else from
+    the GPU transform of the decoded cube (Renderer/EnvironmentMap.cpp:20-68)."""
+    n, K = 16, 40
+    faces = nw.synth_cubemap(5, n)
+    (tmp_path / "cube").mkdir()
+    nw.hdr_write(tmp_path / "cube" / "sky.hdr", _faces_to_cross(faces))
+    env = ctx.envmap_load_folder(tmp_path, "sky", K)
+    decoded = nw.cross_to_faces(nw.hdr_read(tmp_path / "cube" / "sky.hdr"))
+    idx, st = ctx.envmap_selection(env)
+    ridx, rst = orc.topk(orc.envmap_haar(decoded), K)
+    assert np.array_equal(idx, ridx) and np.array_equal(st, rst)
+
+    (tmp_path / "waveletcoeffs").mkdir()
+    (tmp_path / "waveletstrength").mkdir()
+    sel = np.array([5, 1, 700, 3], np.int32)
+    strength = np.arange(12, dtype=np.float32).reshape(4, 3)
+    np.save(tmp_path / "waveletcoeffs" / "sky.npy", sel)
+    np.save(tmp_path / "waveletstrength" / "sky.npy", strength)
+    env2 = ctx.envmap_load_folder(tmp_path, "sky", K)
+    assert ctx.envmap_info(env2) == (n, 4)
+    idx2, st2 = ctx.envmap_selection(env2)
+    assert np.array_equal(idx2, sel) and np.array_equal(st2, strength)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.envmap_load_folder(tmp_path, "nosuch", K)
+    assert e.value.status == nw.ERR_IO
+
+
+def test_offline_trajectory_driver(nw, ctx, tmp_path):
+    """nwprt_offline (RenderEngine facade) writes the same predictedIndirect
+    / predictedWithDirect images as a relight through the C ABI, rows
+    flipped as OutputWriter::writeRGBA does."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(nw.__file__).resolve().with_name("nwprt_offline")
+    assert exe.exists(), "nwprt_offline was not built"
+    rng = np.random.default_rng(2)
+    n_face, K = 8, 16
+    _write_model_folder(tmp_path / "model", rng, n_face)
+    env_dir = tmp_path / "env"
+    (env_dir / "cube").mkdir(parents=True)
+    for name, seed in (("a", 1), ("b", 2)):
+        nw.hdr_write(env_dir / "cube" / f"{name}.hdr", _faces_to_cross(nw.synth_cubemap(seed, n_face)))
+    traj = tmp_path / "traj.txt"
+    traj.write_text(f"envmapFolder {env_dir}\nwaveletFinalDenoiser false\ntrajectoryBegin\n"
+                    "0 1 2 0 0 0 a\n1 1 2 0 0 0 b\n3 3 3 0 0 0 a\n")
+    out = tmp_path / "out"
+    r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-t", str(traj),
+                        "-e", "2", "-k", str(K), "--seed", "10", "--precision", "fp32"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Trajectory: 1 Done" in r.stdout
+    assert not (out / "predictedIndirect" / "2_a.hdr").exists()
+    scene = ctx.initTCNN(tmp_path / "model")
+    for i, name in ((0, "a"), (1, "b")):
+        env = ctx.envmap_load_folder(env_dir, name, K)
+        frame = nw.DeviceFrame(512, 512, nw.synth_gbuffer(10 + i, 512, 512, 0.0))
+        ctx.relight(scene, env, frame, nw.PREC_FP32)
+        ctx.sync()
+        res = frame.results()
+        for key, folder in (("indirect", "predictedIndirect"),
+                            ("indirect_with_direct", "predictedWithDirect")):
+            want = tmp_path / f"want_{i}_{key}.hdr"
+            nw.hdr_write(want, res[key].reshape(512, 512, 4)[::-1])
+            got = nw.hdr_read(out / folder / f"{i}_{name}.hdr")
+            np.testing.assert_array_equal(got, nw.hdr_read(want))
+    bad = tmp_path / "bad.txt"
+    bad.write_text("0 0 0 0 0 0 a\n")
+    r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-m", str(env_dir),
+                        "-t", str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "before trajectory start" in r.stderr
