@@ -52,14 +52,19 @@ __device__ __forceinline__ uint32_t cvt_relu2(float a, float b) {
   return __builtin_bit_cast(uint32_t, m);
 }
 
-// log(1 + e^x) = max(x, 0) + ln(1 + e^-|x|), on the hardware exp2 / log2.
-// For e^-|x| < 2^-10 the log term uses ln(1 + e) ~ e (1 - e/2), which keeps
-// the tail accurate where 1 + e rounds to 1.
+// log(1 + e^x) on the hardware exp2 / log2: ln 2 * log2(1 + 2^(x log2 e)),
+// and x itself above 15 (where the two agree to FP32 precision).  Below about
+// -17, 1 + e^x rounds to 1 and the result is 0 instead of e^x < 2^-24.
 __device__ __forceinline__ float softplus(float x) {
-  const float e = __builtin_amdgcn_exp2f(-fabsf(x) * 1.44269504088896341f);
-  const float lg = __builtin_amdgcn_logf(1.0f + e) * 0.693147180559945309f;
-  const float small = e * fmaf(-0.5f, e, 1.0f);
-  return (x > 0.0f ? x : 0.0f) + (e < 0x1p-10f ? small : lg);
+  const float e = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+  const float sp = __builtin_amdgcn_logf(1.0f + e) * 0.693147180559945309f;
+  return x > 15.0f ? x : sp;
+}
+
+// max(x, 0) as an integer max on the bit pattern: negative floats are
+// negative integers, so one v_max_i32 and no NaN canonicalisation.
+__device__ __forceinline__ float relu_f32(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
 }
 
 // Neumaier-compensated accumulation: (s + c) carries the running sum.

@@ -116,20 +116,21 @@ struct MfmaVariant {
 constexpr MfmaVariant kMfmaVariants[] = {
     {2, false, 2, "gp2.w4reg.occ2"}, {4, false, 2, "gp4.w4reg.occ2"},
     {2, true, 3, "gp2.w4lds.occ3"},  {4, true, 2, "gp4.w4lds.occ2"},
-    {1, true, 3, "gp1.w4lds.occ3"},
+    {1, true, 3, "gp1.w4lds.occ3"},  {0, false, 2, "pipe.w4reg.occ2"},
+    {0, true, 2, "pipe.w4lds.occ2"},
 };
 constexpr int kNumMfmaVariants = sizeof(kMfmaVariants) / sizeof(kMfmaVariants[0]);
 
-// The four layers and the clamped, compensated accumulation for one selected
-// wavelet k (A operands `a1` = its first-layer tiles) over the wave's 64
-// pixels.
-template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS>
-__device__ __forceinline__ void wavelet_step(const RelightArgs &a, int k, const u32x4 (&a1)[4],
-                                             const u32x4 (&hb)[4], const u32x2 (&auxb)[4],
-                                             const u32x2 (&waux)[4], const u32x4 (&w2)[4][2],
-                                             const u32x4 (&w3)[4][2], const u32x4 (&w4)[4][2],
-                                             const u32x4 *w4s, int lane, int my_px, bool mine,
-                                             float (&sum)[3], float (&comp)[3]) {
+// The four layers for one selected wavelet (A operands `a1` = its
+// first-layer tiles) over the wave's 64 pixels.  Returns the output layer's
+// accumulator: lane l holds the three pre-activation outputs of pixel
+// base + l in elements 0..2.
+template <typename E, bool AUX, int GP, bool W4LDS>
+__device__ __forceinline__ f32x4 wavelet_step(const u32x4 (&a1)[4], const u32x4 (&hb)[4],
+                                              const u32x2 (&auxb)[4], const u32x2 (&waux)[4],
+                                              const u32x4 (&w2)[4][2], const u32x4 (&w3)[4][2],
+                                              const u32x4 (&w4)[4][2], const u32x4 *w4s,
+                                              int lane) {
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   f32x4 out = zero;
 #pragma unroll
@@ -178,23 +179,101 @@ __device__ __forceinline__ void wavelet_step(const RelightArgs &a, int k, const 
       out = Mfma<E>::run(t1, x[j][1], Mfma<E>::run(t0, x[j][0], out));
     }
   }
+  return out;
+}
+
+// Scheduling hint for one pipeline stage: NM MFMAs, each followed by its
+// share of the stage's NV vector-ALU instructions, so the matrix core and
+// the VALU work on different pixel groups at the same time.
+template <int NM, int NV>
+__device__ __forceinline__ void interleave_stage() {
 #pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    const float t = SOFTPLUS ? dev::softplus(out[ch]) : out[ch];
-    const float term = a.light[3 * k + ch] * t;
-    dev::neumaier_add(sum[ch], comp[ch], term > 0.0f ? term : 0.0f);
-    if (DUMP) {
-      const int q = my_px - a.dump_begin;
-      if (mine && q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
-    }
+  for (int i = 0; i < NM; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, (NV + NM - 1) / NM, 0);  // VALU
   }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Software-pipelined form of wavelet_step: the four pixel groups flow
+// through the four layers two at a time, and every stage pairs one group's
+// MFMAs (M) with another group's ReLU + pack (C):
+//   M1(0) | M1(1) C(0) | M2(0) C(1) | M2(1) C(0) | M3(0) C(1) | M3(1) C(0) |
+//   M4(0) M1(2) C(1) | M4(1) M1(3) C(2) | M2(2) C(3) | ... | M4(3)
+// Per-lane arithmetic and its order are those of wavelet_step.
+template <typename E, bool AUX, bool W4LDS>
+__device__ __forceinline__ f32x4 wavelet_step_pipelined(
+    const u32x4 (&a1)[4], const u32x4 (&hb)[4], const u32x2 (&auxb)[4], const u32x2 (&waux)[4],
+    const u32x4 (&w2)[4][2], const u32x4 (&w3)[4][2], const u32x4 (&w4)[4][2], const u32x4 *w4s,
+    int lane) {
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  f32x4 out = zero;
+  f32x4 c[4][4];
+  u32x4 x[4][2];
+  constexpr int kAux = AUX ? 4 : 0;
+  auto M1 = [&](int g) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) c[g][b] = Mfma<E>::run(a1[b], hb[g], zero);
+  };
+  // Layers 2 and 3: the four first-half-K MFMAs are issued before the four
+  // second-half ones, so no MFMA waits on the one just issued.
+  auto M2 = [&](int g) {
+    f32x4 t[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const f32x4 c0 = AUX ? Mfma16<E>::run(waux[b], auxb[g], zero) : zero;
+      t[b] = Mfma<E>::run(w2[b][0], x[g][0], c0);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) c[g][b] = Mfma<E>::run(w2[b][1], x[g][1], t[b]);
+  };
+  auto M3 = [&](int g) {
+    f32x4 t[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) t[b] = Mfma<E>::run(w3[b][0], x[g][0], zero);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) c[g][b] = Mfma<E>::run(w3[b][1], x[g][1], t[b]);
+  };
+  auto M4 = [&](int g) {
+    const u32x4 t0 = W4LDS ? w4s[(g * 2 + 0) * 64 + lane] : w4[g][0];
+    const u32x4 t1 = W4LDS ? w4s[(g * 2 + 1) * 64 + lane] : w4[g][1];
+    out = Mfma<E>::run(t1, x[g][1], Mfma<E>::run(t0, x[g][0], out));
+  };
+  auto C = [&](int g) {
+    x[g][0] = relu_pack<E>(c[g][0], c[g][1]);
+    x[g][1] = relu_pack<E>(c[g][2], c[g][3]);
+  };
+  M1(0);
+  __builtin_amdgcn_sched_barrier(0);
+  M1(1); C(0); interleave_stage<4, 16>();
+  M2(0); C(1); interleave_stage<8 + kAux, 16>();
+  M2(1); C(0); interleave_stage<8 + kAux, 16>();
+  M3(0); C(1); interleave_stage<8, 16>();
+  M3(1); C(0); interleave_stage<8, 16>();
+  M4(0); M1(2); C(1); interleave_stage<6, 16>();
+  M4(1); M1(3); C(2); interleave_stage<6, 16>();
+  M2(2); C(3); interleave_stage<8 + kAux, 16>();
+  M2(3); C(2); interleave_stage<8 + kAux, 16>();
+  M3(2); C(3); interleave_stage<8, 16>();
+  M3(3); C(2); interleave_stage<8, 16>();
+  M4(2); C(3); interleave_stage<2, 16>();
+  M4(3);
+  return out;
+}
+
+// Transport of one output (softplus or identity) and its clamped light term
+// max(light * T, 0) (Renderer/tinycudann/utils_kernels.h:74-78).
+template <bool SOFTPLUS>
+__device__ __forceinline__ float transport(float o) {
+  return SOFTPLUS ? dev::softplus(o) : o;
 }
 
 // Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
 // pixels [64 (4 b + w), +64).  Pixel groups j = 0..3 (16 pixels each) are the
 // MFMA columns; lane l owns pixel base + l for the reduction.  The next
 // wavelet's first-layer tiles are fetched while the current one computes
-// (two register sets, K loop unrolled by two).
+// (two register sets, K loop unrolled by two).  GP = 0 selects the
+// software-pipelined step.
 template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, int MINW>
 __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
   __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
@@ -247,20 +326,46 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
     u32x4 ta[4], tb[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) ta[b] = mk[b * 64 + lane];
+    auto step = [&](const u32x4(&t)[4]) -> f32x4 {
+      if constexpr (GP == 0)
+        return wavelet_step_pipelined<E, AUX, W4LDS>(t, hb, auxb, waux, w2, w3, w4, w4s, lane);
+      else
+        return wavelet_step<E, AUX, GP, W4LDS>(t, hb, auxb, waux, w2, w3, w4, w4s, lane);
+    };
+    // Wavelets are taken two at a time: the clamped terms of k and k + 1 are
+    // added, and that pair sum enters the Neumaier-compensated running sum.
     for (int k = 0; k < a.k; k += 2) {
-      if (k + 1 < a.k) {
+      const bool two = k + 1 < a.k;
+      if (two) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) tb[b] = mk[((size_t)(k + 1) * 4 + b) * 64 + lane];
       }
-      wavelet_step<E, SOFTPLUS, DUMP, AUX, GP, W4LDS>(a, k, ta, hb, auxb, waux, w2, w3, w4, w4s,
-                                                       lane, my_px, mine, sum, comp);
-      if (k + 1 < a.k) {
+      const f32x4 o0 = step(ta);
+      f32x4 o1 = {0.f, 0.f, 0.f, 0.f};
+      if (two) {
         if (k + 2 < a.k) {
 #pragma unroll
           for (int b = 0; b < 4; ++b) ta[b] = mk[((size_t)(k + 2) * 4 + b) * 64 + lane];
         }
-        wavelet_step<E, SOFTPLUS, DUMP, AUX, GP, W4LDS>(a, k + 1, tb, hb, auxb, waux, w2, w3, w4,
-                                                         w4s, lane, my_px, mine, sum, comp);
+        o1 = step(tb);
+      }
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float t0 = transport<SOFTPLUS>(o0[ch]);
+        float x = dev::relu_f32(a.light[3 * k + ch] * t0);
+        float t1 = 0.f;
+        if (two) {
+          t1 = transport<SOFTPLUS>(o1[ch]);
+          x += dev::relu_f32(a.light[3 * (k + 1) + ch] * t1);
+        }
+        dev::neumaier_add(sum[ch], comp[ch], x);
+        if (DUMP) {
+          const int q = my_px - a.dump_begin;
+          if (mine && q >= 0 && q < a.dump_count) {
+            a.dump[((size_t)q * a.k + k) * 3 + ch] = t0;
+            if (two) a.dump[((size_t)q * a.k + k + 1) * 3 + ch] = t1;
+          }
+        }
       }
     }
   }
@@ -289,7 +394,7 @@ __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs 
   const int p = blockIdx.x * kPlainBlock + threadIdx.x;
   if (p >= a.n_pixels) return;
   const bool hit = a.env_flag == nullptr || a.env_flag[p] != 0;
-  float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
+  float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f}, pend[3] = {0.f, 0.f, 0.f};
   if (hit) {
     float h[32];
     const float4 *fp = (const float4 *)((const float *)a.features + (size_t)p * 32);
@@ -349,13 +454,20 @@ __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs 
 #pragma unroll
         for (int i = 0; i < 64; ++i) acc = fmaf(W4[ch * 64 + i], y1[i], acc);
         const float t = a.softplus ? dev::softplus(acc) : acc;
-        dev::neumaier_add(sum[ch], comp[ch], fmaxf(a.light[3 * k + ch] * t, 0.0f));
+        // Pairs of wavelets: the clamped terms of k and k + 1 are added and
+        // the pair sum enters the compensated running sum (the fused kernel's
+        // order).
+        const float term = dev::relu_f32(a.light[3 * k + ch] * t);
+        if (k & 1) dev::neumaier_add(sum[ch], comp[ch], pend[ch] + term);
+        else pend[ch] = term;
         if (DUMP) {
           const int q = p - a.dump_begin;
           if (q >= 0 && q < a.dump_count) a.dump[((size_t)q * a.k + k) * 3 + ch] = t;
         }
       }
     }
+    if (a.k & 1)
+      for (int ch = 0; ch < 3; ++ch) dev::neumaier_add(sum[ch], comp[ch], pend[ch]);
   }
   write_outputs(a, p, hit ? sum[0] + comp[0] : 0.f, hit ? sum[1] + comp[1] : 0.f,
                 hit ? sum[2] + comp[2] : 0.f);
@@ -398,6 +510,8 @@ void launch_sp(const RelightArgs &a, int variant, hipStream_t s) {
     case 2: launch_v<E, SP, 2>(a, grid, block, s); break;
     case 3: launch_v<E, SP, 3>(a, grid, block, s); break;
     case 4: launch_v<E, SP, 4>(a, grid, block, s); break;
+    case 5: launch_v<E, SP, 5>(a, grid, block, s); break;
+    case 6: launch_v<E, SP, 6>(a, grid, block, s); break;
     default: fail(NWPRT_ERR_INVALID_ARGUMENT, "relight: unknown kernel variant");
   }
 }
