@@ -22,6 +22,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 FLOP_PER_PAIR = 2 * (32 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # 20,864
+# Encode pass, per pixel: position float4 (16 B) + hit flag (1 B) read,
+# 16 levels x 8 corners x 2 FP16 features gathered (512 B), 32 projected
+# features written in the MLP element type (64 B for bf16 / fp16).
+ENCODE_BYTES_PER_PIXEL = 16 + 1 + 16 * 8 * 2 * 2 + 32 * 2
+HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp8": 5000.0, "fp32": 157.3}
 METRIC = "(pixel,wavelet) MLP evals/s at 1080p K=300; p50/p99 relight latency vs plain"
 
@@ -41,7 +46,28 @@ def parse():
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB: the "
+                         "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def measured_traffic(workload_key):
+    """HBM bytes per launch of the fused kernel from the committed PMC
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    same counter's reading on a copy of known size), or None."""
+    path = ROOT / "profiles" / f"relight_mfma_kernel_{workload_key}_pmc.txt"
+    if not path.exists():
+        return None, None
+    kv = {}
+    for line in path.read_text().splitlines():
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        return float(kv["traffic_bytes_per_launch"]), str(path.relative_to(ROOT))
+    except (KeyError, ValueError):
+        return None, None
 
 
 def percentile(xs, q):
@@ -101,8 +127,19 @@ def main():
     total_ms = evs[0].elapsed_ms(evs[-1])
     value = pairs * args.steps / (total_ms / 1e3)
 
+    # --- hash-grid encode pass (p1) alone: random gathers, HBM-side -------
+    pev = [Event() for _ in range(6)]
+    p1ms = []
+    for i in range(5):
+        pev[i].record(stream)
+        ctx.TCNNPass_p1(scene, 0, frame, prec, stream)
+        pev[i + 1].record(stream)
+        ctx.sync()
+        p1ms.append(pev[i].elapsed_ms(pev[i + 1]))
+    encode_ms = float(np.mean(p1ms))
+    encode_gbs = P * ENCODE_BYTES_PER_PIXEL / (encode_ms / 1e3) / 1e9
+
     # --- dominant kernel alone: the per-pair pass (p2) with HIP events -----
-    ctx.TCNNPass_p1(scene, 0, frame, prec, stream)
     kev = [Event() for _ in range(6)]
     kms = []
     for i in range(5):
@@ -127,6 +164,15 @@ def main():
         plain_ms.append(a.elapsed_ms(b))
     plain_value = pairs / (np.mean(plain_ms) / 1e3)
 
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src, dst = DeviceBuffer(nb), DeviceBuffer(nb)
+        dst.copy_from(src, nb, stream)
+        ctx.sync()
+        del src, dst
+
+    workload_key = f"{W}x{H}_k{K}_n{N}_{args.precision}"
+    traffic, traffic_src = measured_traffic(workload_key)
     result = {
         "metric": METRIC,
         "value": value,
@@ -153,8 +199,10 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "relight_mfma_kernel (pass p2)",
                      "achieved": achieved, "peak": PEAK_TFLOPS[args.precision],
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS[args.precision],
-                     "traffic": None, "kernel_ms": kernel_ms,
-                     "flop_per_pair": FLOP_PER_PAIR},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel_ms": kernel_ms, "flop_per_pair": FLOP_PER_PAIR},
+        "encode_pass": {"bound": "hbm", "ms": encode_ms, "bytes_per_pixel": ENCODE_BYTES_PER_PIXEL,
+                        "achieved_gbs": encode_gbs, "frac": encode_gbs / HBM_PEAK_GBS},
     }
 
     # --- accuracy against the FP64 oracle on a pixel sample ---------------

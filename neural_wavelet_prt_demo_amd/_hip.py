@@ -25,6 +25,8 @@ def hip():
         _hip.hipFree.argtypes = [ctypes.c_void_p]
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         _hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+        _hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.c_int, ctypes.c_void_p]
         _hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
         _hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
@@ -87,6 +89,13 @@ class DeviceBuffer:
 
     def zero(self) -> None:
         check(hip().hipMemset(ctypes.c_void_p(self.ptr), 0, self.nbytes), "hipMemset")
+
+    def copy_from(self, src: "DeviceBuffer", nbytes: int, stream: Optional[int] = None) -> None:
+        """Device-to-device copy of ``nbytes`` on ``stream`` (a blit kernel)."""
+        assert nbytes <= self.nbytes and nbytes <= src.nbytes, "copy larger than buffer"
+        check(hip().hipMemcpyAsync(ctypes.c_void_p(self.ptr), ctypes.c_void_p(src.ptr),
+                                   ctypes.c_size_t(nbytes), D2D, ctypes.c_void_p(stream or 0)),
+              "hipMemcpyAsync D2D")
 
     def free(self) -> None:
         if self.ptr:
