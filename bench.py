@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16"])
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp16", "fp8"])
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--face-size", type=int, default=128)
@@ -109,7 +109,9 @@ def main():
     # Tile autotuning for this shape (records the fastest fused-kernel
     # variant for the (precision, K, pixels) bucket; steps then use it).
     variant, _ = ctx.autotune_relight(scene, env, frame, prec)
-    variant_name = nw.Context.relight_variants()[variant]
+    names = (nw.Context.relight_fp8_variants() if prec == nw.PREC_FP8
+             else nw.Context.relight_variants())
+    variant_name = names[variant]
 
     # --- fused MFMA path: throughput and per-step latency ------------------
     for _ in range(args.warmup):

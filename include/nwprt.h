@@ -56,7 +56,12 @@ typedef enum {
   NWPRT_PREC_FP32 = 0, /* plain form: FP32 vector ALU, one pixel per lane    */
   NWPRT_PREC_FP16 = 1, /* fused MFMA, FP16 operands, FP32 accumulation       */
   NWPRT_PREC_BF16 = 2, /* fused MFMA, BF16 operands, FP32 accumulation       */
-  NWPRT_PREC_FP8 = 3   /* fused MFMA, FP8-E4M3 operands, per-layer scales     */
+  NWPRT_PREC_FP8 = 3   /* fused MFMA: hidden layers FP8-E4M3 (OCP) with one
+                          power-of-two scale per layer, weights from their
+                          amax, activations from a BF16 calibration launch
+                          over ~8K pixels of the frame; first layer BF16;
+                          FP32 accumulation.  Scenes with aux inputs:
+                          NWPRT_ERR_UNSUPPORTED. */
 } nwprt_precision;
 
 typedef enum { NWPRT_DTYPE_F32 = 0, NWPRT_DTYPE_F16 = 1 } nwprt_dtype;
@@ -317,10 +322,16 @@ nwprt_status nwprt_pass_p2(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
  * variants (pixel groups per pass, output-layer weights in VGPRs or LDS,
  * waves per SIMD).  nwprt_autotune_relight times every variant on one frame
  * and records the fastest for that (precision, K, pixel-count) bucket;
- * later relights falling in the bucket use it, others use variant 0. */
+ * later relights falling in the bucket use it, others use variant 0.
+ * The count and names below are those of the BF16 / FP16 kernel; the FP8
+ * kernel has its own (nwprt_relight_fp8_variant_*), and autotuning or
+ * querying with NWPRT_PREC_FP8 works on those. */
 int32_t nwprt_relight_variant_count(void);
 const char *nwprt_relight_variant_name(int32_t variant);
-/* Force one variant for every bucket (-1 returns to the tuned choice). */
+int32_t nwprt_relight_fp8_variant_count(void);
+const char *nwprt_relight_fp8_variant_name(int32_t variant);
+/* Force one BF16 / FP16 variant for every bucket (-1 returns to the tuned
+ * choice). */
 nwprt_status nwprt_set_relight_variant(nwprt_ctx ctx, int32_t variant);
 nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id,
                                     int32_t env_id,
@@ -359,6 +370,45 @@ nwprt_status nwprt_debug_transport(nwprt_ctx ctx, int32_t scene_id,
                                    int32_t pixel_begin, int32_t n_pixels,
                                    float *transport, nwprt_precision precision,
                                    void *stream);
+
+/* -------------------------------------------------------------- batching -- */
+/* No counterpart in the reference, which relights one frame per call
+ * (RenderEngine::TCNNPass_p2, Renderer/tinycudann/tcnn.cu:111-152).  A batch
+ * is a fixed set of relight jobs -- each its own scene, environment map (and
+ * so K), frame size, G-buffer and outputs, all device pointers -- run as one
+ * captured HIP graph: per job the hash-grid encode and the selection tiles,
+ * then ONE persistent fused kernel whose waves pull 64-pixel work items of
+ * every job (largest K first) from a shared counter.  Each job's outputs
+ * equal, bit for bit, those of nwprt_relight with the same precision and
+ * tile variant.  The graph reads the G-buffers, environment-map selections
+ * and outputs through the pointers given here, so new contents (e.g. an
+ * nwprt_envmap_update_faces of a job's map, same K) are picked up by the
+ * next launch.  BF16 / FP16 only; scenes without aux inputs; all jobs must
+ * share the output activation (NWPRT_ERR_UNSUPPORTED otherwise). */
+typedef struct {
+  int32_t scene_id;
+  int32_t env_id;
+  nwprt_gbuffer gbuffer;
+  int32_t width, height;
+  nwprt_frame_out out;
+} nwprt_batch_job;
+
+nwprt_status nwprt_batch_create(nwprt_ctx ctx, const nwprt_batch_job *jobs,
+                                int32_t n_jobs, nwprt_precision precision,
+                                nwprt_tonemap_source tonemap,
+                                int32_t *batch_id);
+/* Launches the batch's graph on `stream` (NULL: the context stream) after
+ * the jobs' environment maps are ready.  Asynchronous. */
+nwprt_status nwprt_batch_launch(nwprt_ctx ctx, int32_t batch_id, void *stream);
+/* Times every tile variant that has a persistent form on this batch and
+ * keeps the fastest (re-captures the graph).  Synchronous. */
+nwprt_status nwprt_batch_autotune(nwprt_ctx ctx, int32_t batch_id,
+                                  int32_t *best_variant, float *best_ms);
+/* Work items (64-pixel groups), (pixel, wavelet) pairs, tile variant. */
+nwprt_status nwprt_batch_info(nwprt_ctx ctx, int32_t batch_id,
+                              int64_t *n_items, int64_t *n_pairs,
+                              int32_t *variant);
+nwprt_status nwprt_batch_destroy(nwprt_ctx ctx, int32_t batch_id);
 
 #ifdef __cplusplus
 }

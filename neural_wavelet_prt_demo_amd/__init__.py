@@ -84,6 +84,11 @@ class FrameOut(Structure):
                 ("rgba8", c_void_p)]
 
 
+class BatchJob(Structure):
+    _fields_ = [("scene_id", c_int32), ("env_id", c_int32), ("gbuffer", GBuffer),
+                ("width", c_int32), ("height", c_int32), ("out", FrameOut)]
+
+
 GBUFFER_FIELDS = {
     "position": (np.float32, 4), "normal": (np.float32, 4), "reflect": (np.float32, 4),
     "roughness": (np.float32, 1), "kd": (np.float32, 4), "ks": (np.float32, 4),
@@ -140,6 +145,8 @@ _SIGNATURES = {
                               c_int32, POINTER(FrameOut), c_int, c_int, c_void_p]),
     "nwprt_relight_variant_count": (c_int32, []),
     "nwprt_relight_variant_name": (c_char_p, [c_int32]),
+    "nwprt_relight_fp8_variant_count": (c_int32, []),
+    "nwprt_relight_fp8_variant_name": (c_char_p, [c_int32]),
     "nwprt_set_relight_variant": (c_int, [c_void_p, c_int32]),
     "nwprt_autotune_relight": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
                                        c_int32, c_int, POINTER(c_int32), POINTER(c_float)]),
@@ -150,6 +157,13 @@ _SIGNATURES = {
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
                                       c_int32, c_int32, c_int32, c_void_p, c_int, c_void_p]),
+    "nwprt_batch_create": (c_int, [c_void_p, POINTER(BatchJob), c_int32, c_int, c_int,
+                                   POINTER(c_int32)]),
+    "nwprt_batch_launch": (c_int, [c_void_p, c_int32, c_void_p]),
+    "nwprt_batch_autotune": (c_int, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_float)]),
+    "nwprt_batch_info": (c_int, [c_void_p, c_int32, POINTER(c_int64), POINTER(c_int64),
+                                 POINTER(c_int32)]),
+    "nwprt_batch_destroy": (c_int, [c_void_p, c_int32]),
 }
 
 _lib = None
@@ -490,6 +504,11 @@ class Context:
         n = lib().nwprt_relight_variant_count()
         return [lib().nwprt_relight_variant_name(i).decode() for i in range(n)]
 
+    @staticmethod
+    def relight_fp8_variants():
+        n = lib().nwprt_relight_fp8_variant_count()
+        return [lib().nwprt_relight_fp8_variant_name(i).decode() for i in range(n)]
+
     def set_relight_variant(self, variant: int) -> None:
         _check(lib().nwprt_set_relight_variant(self.h, variant))
 
@@ -537,6 +556,37 @@ class Context:
                                            frame.height, begin, count, dt.ptr, precision, None))
         self.sync()
         return dt.download(np.float32, (count, k, 3))
+
+    # ---- batches: many relight jobs as one captured graph --------------
+    def batch_create(self, jobs, precision: int = PREC_BF16,
+                     tonemap: int = TONEMAP_FULL) -> int:
+        """jobs: sequence of (scene_id, env_id, DeviceFrame); each frame's
+        own output buffers receive the job's results."""
+        arr = (BatchJob * len(jobs))()
+        for i, (scene, env, frame) in enumerate(jobs):
+            arr[i] = BatchJob(scene, env, frame.gbuffer(), frame.width, frame.height,
+                              frame.frame_out())
+        bid = c_int32()
+        _check(lib().nwprt_batch_create(self.h, arr, len(jobs), precision, tonemap,
+                                        ctypes.byref(bid)))
+        return bid.value
+
+    def batch_launch(self, batch: int, stream: Optional[int] = None) -> None:
+        _check(lib().nwprt_batch_launch(self.h, batch, c_void_p(stream) if stream else None))
+
+    def batch_autotune(self, batch: int):
+        best, ms = c_int32(), c_float()
+        _check(lib().nwprt_batch_autotune(self.h, batch, ctypes.byref(best), ctypes.byref(ms)))
+        return best.value, ms.value
+
+    def batch_info(self, batch: int):
+        items, pairs, variant = c_int64(), c_int64(), c_int32()
+        _check(lib().nwprt_batch_info(self.h, batch, ctypes.byref(items), ctypes.byref(pairs),
+                                      ctypes.byref(variant)))
+        return items.value, pairs.value, variant.value
+
+    def batch_destroy(self, batch: int) -> None:
+        _check(lib().nwprt_batch_destroy(self.h, batch))
 
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -30,6 +30,7 @@ SOURCES = [
     "envmap_kernels.hip",
     "encode_kernels.hip",
     "relight_kernels.hip",
+    "relight_fp8_kernels.hip",
 ]
 
 # Packed-FP32 VALU forms are disabled for the device code (the host pass

@@ -52,6 +52,9 @@ struct Scene {
   nwprt_scene_desc desc{};
   nwprt_hashgrid_levels lv{};
   DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16, aux_bf16, aux_f16;
+  // FP8 form: E4M3 tiles of layers 2..4 and their power-of-two exponents.
+  DevBuf tiles_fp8;
+  int fp8_exp[3] = {0, 0, 0};
   // FP32 aux columns of the second layer ([64][16]) inside w_f32, or null.
   const float *waux_f32() const {
     return desc.aux_dims ? w_f32.as<float>() + (scene_layer_weight_count(desc) - 64 * 16) : nullptr;
@@ -72,9 +75,38 @@ struct EnvMap {
 // 2: nwprt_relight / debug taps).
 struct FrameSlot {
   DevBuf features, aux, mk, wsel;
+  // FP8 form: first-layer 32x32 tiles, calibration subsample, activation maxima.
+  DevBuf mk32, calib, amax;
   bool valid = false;
   int scene = -1, n_pixels = 0;
   Elem elem = Elem::F32;
+};
+
+// A captured batch of relight jobs (nwprt_batch_*): concatenated per-job
+// features and selection tiles, the device job table and work-item list,
+// and the graph that encodes every job and runs the persistent kernel.
+struct Batch {
+  nwprt_precision prec = NWPRT_PREC_BF16;
+  nwprt_tonemap_source tm = NWPRT_TONEMAP_FULL;
+  bool softplus = true;
+  std::vector<nwprt_batch_job> jobs;
+  std::vector<int> k, p;
+  std::vector<size_t> feat_off, mk_off;  // in elements of 2 bytes
+  // What the graph captured of each job's scene and map: a launch checks
+  // that these buffers are still the ones in use.
+  std::vector<const void *> tiles_at, idx_at;
+  DevBuf features, mk, args, items, meta;  // meta: int n_items | uint counter
+  long long n_items = 0, n_pairs = 0;
+  int variant = 5;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~Batch() { drop_graph(); }
 };
 
 }  // namespace
@@ -92,6 +124,8 @@ struct nwprt_context {
   // by (element type, K rounded up to a power of two, ceil(log2 pixels)).
   int forced_variant = -1;
   std::map<std::tuple<int, int, int>, int> tuned;
+  std::map<int, std::unique_ptr<nwprt::Batch>> batches;
+  int n_cu = 0;  // compute units of the device (persistent grid size)
 };
 
 namespace nwprt {
@@ -157,16 +191,20 @@ void check_aligned(const void *p, size_t a, const char *what) {
           std::string(what) + " must be " + std::to_string(a) + "-byte aligned");
 }
 
+// Element type of the per-pixel features for a precision (the FP8 form
+// reads BF16 features: its first layer is a BF16 MFMA).
 Elem elem_of(nwprt_precision p) {
   switch (p) {
     case NWPRT_PREC_FP32: return Elem::F32;
     case NWPRT_PREC_FP16: return Elem::F16;
     case NWPRT_PREC_BF16: return Elem::BF16;
-    case NWPRT_PREC_FP8:
-      fail(NWPRT_ERR_UNSUPPORTED, "FP8 precision is not implemented in this build");
+    case NWPRT_PREC_FP8: return Elem::BF16;
   }
   fail(NWPRT_ERR_INVALID_ARGUMENT, "unknown precision");
 }
+
+// Tuning-table key of a precision: the element type, 3 for the FP8 form.
+int tune_class(nwprt_precision p) { return p == NWPRT_PREC_FP8 ? 3 : (int)elem_of(p); }
 
 void check_frame(const nwprt_gbuffer *gb, int w, int h, bool need_direct) {
   require(gb != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null G-buffer");
@@ -209,6 +247,10 @@ void create_scene(nwprt_ctx ctx, const nwprt_scene_desc &d, const nwprt_scene_pa
     s->aux_bf16.upload(tiles.data(), tiles.size() * 2);
     build_aux_tiles(waux, Elem::F16, tiles);
     s->aux_f16.upload(tiles.data(), tiles.size() * 2);
+  } else {
+    std::vector<uint8_t> t8;
+    build_fp8_weight_tiles(prm.layers, t8, s->fp8_exp);
+    s->tiles_fp8.upload(t8.data(), t8.size());
   }
   const int id = ctx->next_id++;
   ctx->scenes[id] = std::move(s);
@@ -288,25 +330,28 @@ void env_from_selection(nwprt_ctx ctx, const int32_t *idx, const float *str, int
   *env_id = id;
 }
 
-std::tuple<int, int, int> tune_key(Elem elem, int k, int n_pixels) {
+std::tuple<int, int, int> tune_key(nwprt_precision prec, int k, int n_pixels) {
   int kb = 1, pb = 0;
   while (kb < k) kb <<= 1;
   while ((1ll << pb) < n_pixels) ++pb;
-  return {(int)elem, kb, pb};
+  return {tune_class(prec), kb, pb};
 }
 
-int pick_variant(nwprt_ctx ctx, Elem elem, int k, int n_pixels) {
-  if (ctx->forced_variant >= 0) return ctx->forced_variant;
-  auto it = ctx->tuned.find(tune_key(elem, k, n_pixels));
+// Kernel variant for a relight: the forced one (BF16 / FP16 only), else the
+// autotuner's choice for this shape bucket, else 0.
+int pick_variant(nwprt_ctx ctx, nwprt_precision prec, int k, int n_pixels) {
+  if (ctx->forced_variant >= 0 && prec != NWPRT_PREC_FP8) return ctx->forced_variant;
+  auto it = ctx->tuned.find(tune_key(prec, k, n_pixels));
   return it == ctx->tuned.end() ? 0 : it->second;
 }
 
-void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const nwprt_gbuffer *gb,
-                 int w, int h, Elem elem, hipStream_t s, uint32_t *dbg_idx = nullptr,
-                 float *dbg_feat = nullptr) {
-  const int P = w * h;
-  slot.valid = false;
-  slot.features.ensure((size_t)P * 32 * (elem == Elem::F32 ? 4 : 2));
+int variant_count(nwprt_precision prec) {
+  return prec == NWPRT_PREC_FP8 ? relight_fp8_variant_count() : relight_mfma_variant_count();
+}
+
+// Hash-grid encode of P pixels of `gb` into `features` ([P][32] of elem).
+EncodeArgs encode_args(const Scene &sc, const nwprt_gbuffer *gb, int P, Elem elem,
+                       void *features) {
   EncodeArgs a{};
   a.n_pixels = P;
   a.position = gb->position;
@@ -316,8 +361,18 @@ void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const 
   a.n_features = sc.desc.grid.n_features_per_level;
   a.lv = sc.lv;
   a.projection = sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
-  a.features = slot.features.p;
+  a.features = features;
   a.out_elem = elem;
+  return a;
+}
+
+void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const nwprt_gbuffer *gb,
+                 int w, int h, Elem elem, hipStream_t s, uint32_t *dbg_idx = nullptr,
+                 float *dbg_feat = nullptr) {
+  const int P = w * h;
+  slot.valid = false;
+  slot.features.ensure((size_t)P * 32 * (elem == Elem::F32 ? 4 : 2));
+  EncodeArgs a = encode_args(sc, gb, P, elem, slot.features.p);
   a.dbg_indices = dbg_idx;
   a.dbg_features = dbg_feat;
   if (sc.desc.aux_dims && dbg_idx == nullptr) {
@@ -347,8 +402,10 @@ void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const 
 
 void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, FrameSlot &slot,
                        const nwprt_gbuffer *gb, int w, int h, const nwprt_frame_out *out,
-                       Elem elem, nwprt_tonemap_source tm, hipStream_t s, float *dump = nullptr,
-                       int dump_begin = 0, int dump_count = 0, int variant = -1) {
+                       nwprt_precision prec, nwprt_tonemap_source tm, hipStream_t s,
+                       float *dump = nullptr, int dump_begin = 0, int dump_count = 0,
+                       int variant = -1) {
+  const Elem elem = elem_of(prec);
   const int P = w * h;
   require(slot.valid && slot.scene == scene_id && slot.n_pixels == P && slot.elem == elem,
           NWPRT_ERR_STATE,
@@ -394,6 +451,36 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
     a.wsel = slot.wsel.as<float>();
     a.w_f32 = sc.w_f32.as<float>();
     launch_relight_plain(a, s);
+  } else if (prec == NWPRT_PREC_FP8) {
+    require(sc.desc.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
+            "FP8 precision does not support scenes with aux inputs");
+    // Selection tiles: 16x16 (BF16 calibration network) and 32x32 (FP8 form).
+    slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
+    slot.mk32.ensure((size_t)env.k * 4 * 64 * 16);
+    launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
+                             env.idx.as<int32_t>(), env.k, Elem::BF16, slot.mk.as<uint16_t>(),
+                             nullptr, s, slot.mk32.as<uint16_t>());
+    // Per-layer activation maxima over about 8K pixels spread over the frame.
+    constexpr int kCalibPixels = 8192;
+    const int stride = std::max(1, (P + kCalibPixels - 1) / kCalibPixels);
+    const int n_cal = (P + stride - 1) / stride;
+    slot.calib.ensure((size_t)n_cal * 64);
+    slot.amax.ensure(16);
+    launch_calib_gather(slot.features.as<uint16_t>(), gb->env_flag, P, stride,
+                        slot.calib.as<uint16_t>(), n_cal, slot.amax.as<float>(), s);
+    RelightArgs c{};
+    c.n_pixels = n_cal;
+    c.k = env.k;
+    c.features = slot.calib.p;
+    c.mk = slot.mk.as<uint16_t>();
+    c.w_tiles = sc.tiles_bf16.as<uint16_t>();
+    c.amax = slot.amax.as<float>();
+    launch_calibrate_amax(c, s);
+    a.mk = slot.mk32.as<uint16_t>();
+    a.w8_tiles = sc.tiles_fp8.as<uint8_t>();
+    for (int i = 0; i < 3; ++i) a.w8_exp[i] = sc.fp8_exp[i];
+    a.amax = slot.amax.as<float>();
+    launch_relight_fp8(a, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
   } else {
     slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
@@ -401,8 +488,163 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
                              s);
     a.mk = slot.mk.as<uint16_t>();
     a.w_tiles = elem == Elem::BF16 ? sc.tiles_bf16.as<uint16_t>() : sc.tiles_f16.as<uint16_t>();
-    launch_relight_mfma(a, elem, variant >= 0 ? variant : pick_variant(ctx, elem, env.k, P), s);
+    launch_relight_mfma(a, elem, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
   }
+}
+
+// ------------------------------------------------------------------ batch --
+
+constexpr int kDefaultBatchVariant = 5;  // pipe.w4reg.occ2
+
+Batch &get_batch(nwprt_ctx ctx, int id) {
+  auto it = ctx->batches.find(id);
+  require(it != ctx->batches.end(), NWPRT_ERR_NOT_FOUND, "unknown batch id " + std::to_string(id));
+  return *it->second;
+}
+
+const void *scene_tiles(const Scene &sc, nwprt_precision prec) {
+  return prec == NWPRT_PREC_BF16 ? sc.tiles_bf16.p : sc.tiles_f16.p;
+}
+
+// Every job's scene and environment map still exist, with the buffers the
+// graph captured; the launch stream waits for the maps.
+void check_batch_inputs(nwprt_ctx ctx, const Batch &b, hipStream_t s) {
+  for (size_t j = 0; j < b.jobs.size(); ++j) {
+    const Scene &sc = get_scene(ctx, b.jobs[j].scene_id);
+    EnvMap &env = get_env(ctx, b.jobs[j].env_id);
+    require(scene_tiles(sc, b.prec) == b.tiles_at[j] && env.idx.p == b.idx_at[j] &&
+                env.k == b.k[j],
+            NWPRT_ERR_STATE, "batch: a job's scene or environment map changed since the batch "
+                             "was created; create the batch again");
+    if (s) hip_check(hipStreamWaitEvent(s, env.ready, 0), "hipStreamWaitEvent");
+  }
+}
+
+// Records the batch graph on the context stream: per job the encode and the
+// selection tiles, then the counter reset and the persistent kernel.
+void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
+  require(relight_batch_variant_ok(variant), NWPRT_ERR_INVALID_ARGUMENT,
+          "batch: tile variant has no persistent form");
+  b.drop_graph();
+  const hipStream_t s = ctx->stream;
+  const Elem elem = elem_of(b.prec);
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+  try {
+    for (size_t j = 0; j < b.jobs.size(); ++j) {
+      const nwprt_batch_job &jb = b.jobs[j];
+      const Scene &sc = get_scene(ctx, jb.scene_id);
+      const EnvMap &env = get_env(ctx, jb.env_id);
+      launch_encode(encode_args(sc, &jb.gbuffer, b.p[j], elem,
+                                b.features.as<uint16_t>() + b.feat_off[j]),
+                    s);
+      launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * env.n * env.n,
+                               env.idx.as<int32_t>(), b.k[j], elem,
+                               b.mk.as<uint16_t>() + b.mk_off[j], nullptr, s);
+    }
+    unsigned int *counter = (unsigned int *)(b.meta.as<int>() + 1);
+    launch_reset_counter(counter, s);
+    launch_relight_batch(b.args.as<RelightArgs>(), b.items.as<BatchItem>(), b.meta.as<int>(),
+                         counter, elem, b.softplus, variant, ctx->n_cu, s);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    throw;
+  }
+  hip_check(hipStreamEndCapture(s, &b.graph), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  b.variant = variant;
+}
+
+void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_precision prec,
+                  nwprt_tonemap_source tm, int *batch_id) {
+  require(jobs && batch_id && n >= 1 && n <= 65536, NWPRT_ERR_INVALID_ARGUMENT,
+          "batch: need 1 to 65536 jobs");
+  require(prec == NWPRT_PREC_BF16 || prec == NWPRT_PREC_FP16, NWPRT_ERR_UNSUPPORTED,
+          "batch: BF16 or FP16 precision only");
+  require(tm == NWPRT_TONEMAP_FULL || tm == NWPRT_TONEMAP_INDIRECT || tm == NWPRT_TONEMAP_NONE,
+          NWPRT_ERR_INVALID_ARGUMENT, "unknown tonemap source");
+  auto b = std::make_unique<Batch>();
+  b->prec = prec;
+  b->tm = tm;
+  size_t fo = 0, mo = 0;
+  for (int j = 0; j < n; ++j) {
+    const nwprt_batch_job &jb = jobs[j];
+    const Scene &sc = get_scene(ctx, jb.scene_id);
+    EnvMap &env = get_env(ctx, jb.env_id);
+    require(sc.desc.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
+            "batch: scenes with aux inputs are not supported");
+    require(env.n == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
+            "batch: environment map face size does not match the job's scene");
+    check_frame(&jb.gbuffer, jb.width, jb.height, true);
+    if (jb.out.indirect) check_aligned(jb.out.indirect, 16, "indirect output");
+    if (jb.out.indirect_with_direct) check_aligned(jb.out.indirect_with_direct, 16, "with-direct output");
+    if (jb.out.rgba8) check_aligned(jb.out.rgba8, 4, "rgba8 output");
+    const bool sp = sc.desc.output_softplus != 0;
+    if (j == 0) b->softplus = sp;
+    require(sp == b->softplus, NWPRT_ERR_UNSUPPORTED,
+            "batch: all jobs must share the output activation");
+    const int P = jb.width * jb.height;
+    b->jobs.push_back(jb);
+    b->k.push_back(env.k);
+    b->p.push_back(P);
+    b->feat_off.push_back(fo);
+    b->mk_off.push_back(mo);
+    b->tiles_at.push_back(scene_tiles(sc, prec));
+    b->idx_at.push_back(env.idx.p);
+    fo += (size_t)P * 32;
+    mo += (size_t)env.k * 4 * 64 * 8;
+    b->n_items += (P + 63) / 64;
+    b->n_pairs += (long long)P * env.k;
+  }
+  require(b->n_items < (1ll << 31), NWPRT_ERR_INVALID_ARGUMENT, "batch: too many pixels");
+  b->features.ensure(fo * 2);
+  b->mk.ensure(mo * 2);
+  // Device job table.
+  std::vector<RelightArgs> args(n);
+  for (int j = 0; j < n; ++j) {
+    const nwprt_batch_job &jb = b->jobs[j];
+    const Scene &sc = get_scene(ctx, jb.scene_id);
+    const EnvMap &env = get_env(ctx, jb.env_id);
+    RelightArgs &a = args[j];
+    a = RelightArgs{};
+    a.n_pixels = b->p[j];
+    a.k = b->k[j];
+    a.features = b->features.as<uint16_t>() + b->feat_off[j];
+    a.mk = b->mk.as<uint16_t>() + b->mk_off[j];
+    a.light = env.strength.as<float>();
+    a.w_tiles = (const uint16_t *)scene_tiles(sc, prec);
+    a.env_flag = jb.gbuffer.env_flag;
+    a.direct = jb.gbuffer.direct;
+    a.indirect = jb.out.indirect;
+    a.with_direct = jb.out.indirect_with_direct;
+    a.rgba8 = jb.out.rgba8;
+    a.tonemap_source = (int)tm;
+    a.softplus = b->softplus;
+  }
+  b->args.upload(args.data(), args.size() * sizeof(RelightArgs));
+  // Work items: jobs in decreasing K (ties: job order), each job's 64-pixel
+  // groups in pixel order, so the costliest items are taken first.
+  std::vector<int> order(n);
+  for (int j = 0; j < n; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->k[x] > b->k[y]; });
+  std::vector<BatchItem> items;
+  items.reserve((size_t)b->n_items);
+  for (int j : order)
+    for (int base = 0; base < b->p[j]; base += 64) items.push_back(BatchItem{j, base});
+  b->items.upload(items.data(), items.size() * sizeof(BatchItem));
+  const int meta[2] = {(int)b->n_items, 0};
+  b->meta.upload(meta, sizeof(meta));
+  // The graph does not wait on the maps' events; they are complete now, and
+  // each launch makes its stream wait for them.
+  for (int j = 0; j < n; ++j)
+    hip_check(hipEventSynchronize(get_env(ctx, b->jobs[j].env_id).ready), "hipEventSynchronize");
+  capture_batch(ctx, *b, kDefaultBatchVariant);
+  const int id = ctx->next_id++;
+  ctx->batches[id] = std::move(b);
+  *batch_id = id;
 }
 
 }  // namespace
@@ -428,6 +670,8 @@ nwprt_status nwprt_create(int device, nwprt_ctx *out) {
     ctx->device = device;
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device),
+              "hipDeviceGetAttribute");
     *out = ctx.release();
   });
 }
@@ -638,8 +882,8 @@ nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
     const Elem elem = elem_of(precision);
     const hipStream_t s = pick(ctx, stream);
     encode_into(ctx, sc, scene_id, ctx->slots[2], gbuffer, width, height, elem, s);
-    relight_from_slot(ctx, sc, scene_id, env, ctx->slots[2], gbuffer, width, height, out, elem,
-                      tonemap, s);
+    relight_from_slot(ctx, sc, scene_id, env, ctx->slots[2], gbuffer, width, height, out,
+                      precision, tonemap, s);
   });
 }
 
@@ -669,7 +913,7 @@ nwprt_status nwprt_pass_p2(nwprt_ctx ctx, int32_t scene_id, int32_t env_id, int3
     EnvMap &env = get_env(ctx, env_id);
     check_frame(gbuffer, width, height, true);
     relight_from_slot(ctx, sc, scene_id, env, ctx->slots[frame_slot], gbuffer, width, height, out,
-                      elem_of(precision), tonemap, pick(ctx, stream));
+                      precision, tonemap, pick(ctx, stream));
   });
 }
 
@@ -677,6 +921,12 @@ int32_t nwprt_relight_variant_count(void) { return relight_mfma_variant_count();
 
 const char *nwprt_relight_variant_name(int32_t variant) {
   return relight_mfma_variant_name(variant);
+}
+
+int32_t nwprt_relight_fp8_variant_count(void) { return relight_fp8_variant_count(); }
+
+const char *nwprt_relight_fp8_variant_name(int32_t variant) {
+  return relight_fp8_variant_name(variant);
 }
 
 nwprt_status nwprt_set_relight_variant(nwprt_ctx ctx, int32_t variant) {
@@ -693,7 +943,7 @@ nwprt_status nwprt_selected_relight_variant(nwprt_ctx ctx, nwprt_precision preci
   return guarded([&] {
     require(ctx && variant && k >= 1 && n_pixels >= 1, NWPRT_ERR_INVALID_ARGUMENT,
             "bad argument");
-    *variant = pick_variant(ctx, elem_of(precision), k, n_pixels);
+    *variant = pick_variant(ctx, precision, k, n_pixels);
   });
 }
 
@@ -718,14 +968,14 @@ nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env
     int best = 0;
     float best_t = 0.f;
     try {
-      for (int v = 0; v < relight_mfma_variant_count(); ++v) {
-        relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr, elem,
-                          NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
+      for (int v = 0; v < variant_count(precision); ++v) {
+        relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr,
+                          precision, NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
         float t = 0.f;
         for (int rep = 0; rep < 3; ++rep) {
           hip_check(hipEventRecord(e0, s), "hipEventRecord");
-          relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr, elem,
-                            NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
+          relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr,
+                            precision, NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
           hip_check(hipEventRecord(e1, s), "hipEventRecord");
           hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
           float ms = 0.f;
@@ -744,7 +994,7 @@ nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    ctx->tuned[tune_key(elem, env.k, width * height)] = best;
+    ctx->tuned[tune_key(precision, env.k, width * height)] = best;
     if (best_variant) *best_variant = best;
     if (best_ms) *best_ms = best_t;
   });
@@ -801,7 +1051,94 @@ nwprt_status nwprt_debug_transport(nwprt_ctx ctx, int32_t scene_id, int32_t env_
     const hipStream_t s = pick(ctx, stream);
     encode_into(ctx, sc, scene_id, ctx->slots[2], gbuffer, width, height, elem, s);
     relight_from_slot(ctx, sc, scene_id, env, ctx->slots[2], gbuffer, width, height, nullptr,
-                      elem, NWPRT_TONEMAP_NONE, s, transport, pixel_begin, n_pixels);
+                      precision, NWPRT_TONEMAP_NONE, s, transport, pixel_begin, n_pixels);
+  });
+}
+
+nwprt_status nwprt_batch_create(nwprt_ctx ctx, const nwprt_batch_job *jobs, int32_t n_jobs,
+                                nwprt_precision precision, nwprt_tonemap_source tonemap,
+                                int32_t *batch_id) {
+  return guarded([&] {
+    bind(ctx);
+    int id = 0;
+    create_batch(ctx, jobs, n_jobs, precision, tonemap, &id);
+    *batch_id = id;
+  });
+}
+
+nwprt_status nwprt_batch_launch(nwprt_ctx ctx, int32_t batch_id, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    Batch &b = get_batch(ctx, batch_id);
+    const hipStream_t s = pick(ctx, stream);
+    check_batch_inputs(ctx, b, s);
+    hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+  });
+}
+
+nwprt_status nwprt_batch_autotune(nwprt_ctx ctx, int32_t batch_id, int32_t *best_variant,
+                                  float *best_ms) {
+  return guarded([&] {
+    bind(ctx);
+    Batch &b = get_batch(ctx, batch_id);
+    const hipStream_t s = ctx->stream;
+    check_batch_inputs(ctx, b, s);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    int best = -1;
+    float best_t = 0.f;
+    try {
+      for (int v = 0; v < relight_mfma_variant_count(); ++v) {
+        if (!relight_batch_variant_ok(v)) continue;
+        capture_batch(ctx, b, v);
+        hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+          hip_check(hipEventRecord(e0, s), "hipEventRecord");
+          hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+          hip_check(hipEventRecord(e1, s), "hipEventRecord");
+          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+          float ms = 0.f;
+          hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+          t = rep == 0 ? ms : std::min(t, ms);
+        }
+        if (best < 0 || t < best_t) {
+          best = v;
+          best_t = t;
+        }
+      }
+      capture_batch(ctx, b, best);
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = best_t;
+  });
+}
+
+nwprt_status nwprt_batch_info(nwprt_ctx ctx, int32_t batch_id, int64_t *n_items,
+                              int64_t *n_pairs, int32_t *variant) {
+  return guarded([&] {
+    require(ctx != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null context");
+    const Batch &b = get_batch(ctx, batch_id);
+    if (n_items) *n_items = b.n_items;
+    if (n_pairs) *n_pairs = b.n_pairs;
+    if (variant) *variant = b.variant;
+  });
+}
+
+nwprt_status nwprt_batch_destroy(nwprt_ctx ctx, int32_t batch_id) {
+  return guarded([&] {
+    bind(ctx);
+    get_batch(ctx, batch_id);
+    // A launch may still be running on any stream.
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    ctx->batches.erase(batch_id);
   });
 }
 

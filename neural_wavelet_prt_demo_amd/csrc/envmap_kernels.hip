@@ -231,10 +231,26 @@ __global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restric
 __global__ __launch_bounds__(256) void prepare_selection_kernel(
     const float *__restrict__ w1, const float *__restrict__ wfeat, int n_coef,
     const int32_t *__restrict__ indices, int elem, uint16_t *__restrict__ mk,
-    float *__restrict__ wsel) {
+    float *__restrict__ wsel, uint16_t *__restrict__ mk32) {
   const int k = blockIdx.x;
   const int t = threadIdx.x;
   const int c = indices[k];
+  if (mk32) {
+    // 32x32x16 A operand: block b = t / 128 (rows 32 b + lane % 32), K step
+    // s = (t / 64) % 2, lane holds features 16 s + 8 (lane / 32) + [0, 8).
+    const int b = t >> 7, s = (t >> 6) & 1, lane = t & 63;
+    const int m = 32 * b + (lane & 31);
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int f = 16 * s + 8 * (lane >> 5) + j;
+      v[j] = w1[m * 32 + f] * wfeat[(size_t)f * n_coef + c];
+    }
+    dev::u32x4 out;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) out[d] = dev::pack_bf16(v[2 * d], v[2 * d + 1]);
+    *(dev::u32x4 *)(mk32 + (((size_t)k * 4 + b * 2 + s) * 64 + lane) * 8) = out;
+  }
   if (mk) {
     const int b = t >> 6, lane = t & 63, g = lane >> 4;
     const int m = 16 * b + (lane & 15);
@@ -277,12 +293,12 @@ void launch_topk(const float *coef, int n, int k, uint32_t *scores, int32_t *ind
 
 void launch_prepare_selection(const float *w1, const float *wfeat, int n_coef,
                               const int32_t *indices, int k, Elem mk_elem, uint16_t *mk,
-                              float *wsel, hipStream_t s) {
+                              float *wsel, hipStream_t s, uint16_t *mk32) {
   require(k >= 1 && k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT, "prepare: bad K");
   require(mk == nullptr || mk_elem == Elem::BF16 || mk_elem == Elem::F16,
           NWPRT_ERR_INVALID_ARGUMENT, "prepare: MFMA tiles must be BF16 or FP16");
   hipLaunchKernelGGL(prepare_selection_kernel, dim3(k), dim3(256), 0, s, w1, wfeat, n_coef,
-                     indices, (int)mk_elem, mk, wsel);
+                     indices, (int)mk_elem, mk, wsel, mk32);
   hip_check(hipGetLastError(), "prepare_selection_kernel launch");
 }
 

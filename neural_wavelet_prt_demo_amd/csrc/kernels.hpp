@@ -29,11 +29,15 @@ void launch_topk(const float *coef, int n, int k, uint32_t *scores,
                  int32_t *indices, float *strengths, hipStream_t s);
 
 // Per-relight preparation of the selected wavelets:
-//  mk   (may be null): [K][4][64][8] MFMA A-operand tiles of W1 * diag(wfeat[:,c_k])
-//  wsel (may be null): [K][32] FP32 selected wavelet features.
+//  mk   (may be null): [K][4][64][8] 16x16x32 MFMA A-operand tiles of
+//                      W1 * diag(wfeat[:,c_k])
+//  wsel (may be null): [K][32] FP32 selected wavelet features
+//  mk32 (may be null): [K][2][2][64][8] BF16 32x32x16 A-operand tiles of the
+//                      same product, for the FP8 form's first layer.
 void launch_prepare_selection(const float *w1, const float *wfeat, int n_coef,
                               const int32_t *indices, int k, Elem mk_elem,
-                              uint16_t *mk, float *wsel, hipStream_t s);
+                              uint16_t *mk, float *wsel, hipStream_t s,
+                              uint16_t *mk32 = nullptr);
 
 // ---------------------------------------------------------------- encode --
 
@@ -86,6 +90,11 @@ struct RelightArgs {
   // transport tap: T for pixels [dump_begin, dump_begin + dump_count)
   float *dump;
   int dump_begin, dump_count;
+  // FP8 form: layers 2..4 as E4M3 A tiles [3][2][64][32] (W * 2^w8_exp per
+  // layer) and the calibrated activation maxima of layers 1..3 (device).
+  const uint8_t *w8_tiles;
+  int w8_exp[3];
+  float *amax;
 };
 // Tile-shape variants of the fused kernel (see relight_kernels.hip).
 int relight_mfma_variant_count();
@@ -94,9 +103,45 @@ void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream
 void launch_relight_plain(const RelightArgs &a, hipStream_t s);
 void launch_tonemap(const float *src, uint32_t *dst, int n_pixels, hipStream_t s);
 
+// Persistent batch form: work item = the 64 pixels [base, base + 64) of job
+// `job` (base < the job's pixel count).
+struct BatchItem {
+  int job, base;
+};
+// Whether a BF16/FP16 tile variant has a persistent batch form (output
+// layer in VGPRs).
+bool relight_batch_variant_ok(int variant);
+// Waves take items [0, *n_items) in order from *counter, which must be 0 at
+// launch (launch_reset_counter); the grid is n_cu x the variant's waves per
+// SIMD workgroups.  Jobs must not have aux inputs or a transport tap, and
+// all use `softplus` as their output activation.
+void launch_relight_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
+                          unsigned int *counter, Elem elem, bool softplus, int variant,
+                          int n_cu, hipStream_t s);
+void launch_reset_counter(unsigned int *counter, hipStream_t s);
+
 // Host-side MFMA weight tiling of layers 2..4 and of the second layer's aux
 // columns (see relight kernels).
 void build_weight_tiles(const float *w_f32, Elem elem, std::vector<uint16_t> &out);
 void build_aux_tiles(const float *waux, Elem elem, std::vector<uint16_t> &out);
+
+// FP8 form (relight_fp8_kernels.hip).  Layers 2..4 quantized to OCP E4M3
+// with one power-of-two scale per layer (amax mapped into (224, 448]);
+// `exps` receives the three exponents.
+void build_fp8_weight_tiles(const float *w_f32, std::vector<uint8_t> &out, int exps[3]);
+// Activation calibration: copies every `stride`-th pixel's BF16 features
+// (zeros for background pixels) into `calib` [n_cal][32] and clears amax[4].
+void launch_calib_gather(const uint16_t *features, const uint8_t *env_flag, int n_pixels,
+                         int stride, uint16_t *calib, int n_cal, float *amax, hipStream_t s);
+// Maxima of the layer 1..3 activations of the BF16 network over the
+// calibration pixels and all K wavelets (`a`: BF16 MFMA operands, features =
+// calib, n_pixels = n_cal) into a.amax[0..2].
+void launch_calibrate_amax(const RelightArgs &a, hipStream_t s);
+int relight_fp8_variant_count();
+const char *relight_fp8_variant_name(int variant);
+void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s);
+// OCP E4M3 encode (round to nearest even, saturating at +-448) / decode.
+uint8_t f32_to_e4m3(float x);
+float e4m3_to_f32(uint8_t c);
 
 }  // namespace nwprt

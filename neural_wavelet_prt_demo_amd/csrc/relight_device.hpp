@@ -1,0 +1,36 @@
+// Device pieces shared by the relight kernels (BF16/FP16 and FP8 forms).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace nwprt {
+namespace dev {
+
+// Per-pixel outputs of computeResponse (Renderer/tinycudann/utils_kernels.h:80-88):
+// indirect = (rgb, 1), indirect + direct = (rgb + direct, 1), and the RGBA8
+// tone map of the chosen one (Renderer/toneMap.cu:47-71).
+__device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float r, float g,
+                                              float b) {
+  const float4 d = *(const float4 *)(a.direct + 4 * (size_t)p);
+  const float4 ind = make_float4(r, g, b, 1.0f);
+  const float4 wd = make_float4(r + d.x, g + d.y, b + d.z, 1.0f);
+  if (a.indirect) *(float4 *)(a.indirect + 4 * (size_t)p) = ind;
+  if (a.with_direct) *(float4 *)(a.with_direct + 4 * (size_t)p) = wd;
+  if (a.rgba8 && a.tonemap_source != NWPRT_TONEMAP_NONE) {
+    const float4 t = a.tonemap_source == NWPRT_TONEMAP_FULL ? wd : ind;
+    a.rgba8[p] = tonemap_rgba8(t.x, t.y, t.z);
+  }
+}
+
+// Transport of one output (softplus or identity); its clamped light term is
+// max(light * T, 0) (Renderer/tinycudann/utils_kernels.h:74-78).
+template <bool SOFTPLUS>
+__device__ __forceinline__ float transport(float o) {
+  return SOFTPLUS ? softplus(o) : o;
+}
+
+}  // namespace dev
+}  // namespace nwprt

@@ -1,0 +1,329 @@
+// FP8 form of the fused per-(pixel, wavelet) network for gfx950.
+//
+// Same computation as the BF16 kernel (Renderer/tinycudann/tcnn.cu:111-152,
+// utils_kernels.h:51-115 replaced by one launch), with the hidden layers on
+// v_mfma_scale_f32_32x32x64_f8f6f4: OCP E4M3 operands, FP32 accumulation,
+// one power-of-two scale per layer applied by the instruction's E8M0 scale
+// operands.  The first layer (the per-wavelet Hadamard product folded into
+// W1) stays BF16 on v_mfma_f32_32x32x16_bf16: its contraction is only 32
+// long, so an FP8 K=64 instruction would be half padding.
+//
+// 32x32 MFMA layouts (gfx950, lane l, h = l / 32, c = l % 32):
+//   A: row c, contraction slots (h, j) for its bytes / elements j;
+//   B: column c, the same slots;
+//   D: column c, element v holds row 8 (v / 4) + 4 h + v % 4.
+// A hidden layer's two 32-row D blocks hold, in lane l, 32 of the 64 outputs
+// of pixel column c: exactly the 32 slots B needs from that lane.  Slot
+// (h, j) of the next layer therefore carries output
+//   pi(h, j) = 32 (j / 16) + 8 ((j % 16) / 4) + 4 h + j % 4,
+// and the host permutes the weight columns to match (build_fp8_weight_tiles).
+//
+// Scaling.  Weights: W * 2^ew quantized, ew per layer from the weight amax.
+// Activations: the maxima of the BF16 network's layer 1..3 outputs over a
+// pixel subsample (calib_amax_kernel) give ea per layer, with amax * 2^ea in
+// (112, 224]: half of E4M3's 448 in reserve, and v_med3 saturates anything
+// beyond 448 instead of letting it round to NaN.  Layer 1's 2^ea1 is applied
+// to the per-pixel features once; layer n's MFMA scales (A: 2^(ea_n - ew_n),
+// B: 2^-ea_(n-1)) leave its output already multiplied by 2^ea_n, so the ReLU
+// + convert step is v_med3 + v_cvt_pk_fp8_f32 and nothing else.  Every scale
+// is a power of two: multiplying the inputs by 2^s multiplies the result by
+// exactly 2^s.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "relight_device.hpp"
+
+namespace nwprt {
+namespace {
+
+using dev::u32x4;
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBlock = 256;  // 4 waves x 64 pixels
+
+__device__ __forceinline__ f32x16 mfma_bf16(u32x4 a, u32x4 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(dev::bf16x8, a),
+                                                 __builtin_bit_cast(dev::bf16x8, b), c, 0, 0, 0);
+}
+
+// E4M3 x E4M3 (format code 0 for both operands), A scaled by 2^(sa - 127),
+// B by 2^(sb - 127).
+__device__ __forceinline__ f32x16 mfma_fp8(i32x8 a, i32x8 b, f32x16 c, int sa, int sb) {
+  return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, sa, 0, sb);
+}
+
+// ReLU, saturation at 448 and E4M3 rounding (RNE) of a hidden layer's two D
+// blocks into the next layer's B operand: dword d holds block d / 4,
+// elements 4 (d % 4) .. + 3, i.e. slots j = 4 d .. 4 d + 3.
+__device__ __forceinline__ i32x8 relu_fp8(const f32x16 &lo, const f32x16 &hi) {
+  i32x8 x;
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    const f32x16 &src = d < 4 ? lo : hi;
+    const int v = 4 * (d & 3);
+    const float q0 = __builtin_amdgcn_fmed3f(src[v + 0], 0.f, 448.f);
+    const float q1 = __builtin_amdgcn_fmed3f(src[v + 1], 0.f, 448.f);
+    const float q2 = __builtin_amdgcn_fmed3f(src[v + 2], 0.f, 448.f);
+    const float q3 = __builtin_amdgcn_fmed3f(src[v + 3], 0.f, 448.f);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(q0, q1, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(q2, q3, w, true);
+    x[d] = w;
+  }
+  return x;
+}
+
+// Exponent e with amax * 2^e in (112, 224], from the float's bits (so it
+// moves by exactly -s when amax is multiplied by 2^s).  0 for a zero,
+// denormal or non-finite maximum; clamped to [-100, 100].
+__device__ __forceinline__ int act_exp(float amax) {
+  const uint32_t u = __float_as_uint(amax);
+  const int be = (int)((u >> 23) & 0xffu);
+  if (be == 0 || be == 255 || (u >> 31)) return 0;
+  const int E = be - 127;
+  const int e = (u & 0x7fffffu) <= 0x600000u ? 7 - E : 6 - E;
+  return min(max(e, -100), 100);
+}
+
+__device__ __forceinline__ int scale_byte(int e) { return min(max(e, 0), 254); }
+
+// Eight BF16 features times 2^e (exact: a power of two), re-rounded to BF16.
+__device__ __forceinline__ u32x4 scale_bf16x8(u32x4 v, float s) {
+  u32x4 r;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    const float lo = __uint_as_float(v[d] << 16) * s;
+    const float hi = __uint_as_float(v[d] & 0xffff0000u) * s;
+    r[d] = dev::pack_bf16(lo, hi);
+  }
+  return r;
+}
+
+// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
+// pixels [64 (4 b + w), +64) as two 32-column groups; after the output layer
+// lane l holds pixel base + l.  The next wavelet's first-layer tiles load
+// while the current one computes.
+template <bool SOFTPLUS, bool DUMP>
+__global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
+  // Output-layer variants in LDS (read once per wavelet), freeing 16 VGPRs.
+  __shared__ i32x8 w4s[2 * 64];
+  if (threadIdx.x < 2 * 64) w4s[threadIdx.x] = ((const i32x8 *)a.w8_tiles)[2 * 2 * 64 + threadIdx.x];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int h = lane >> 5, col = lane & 31;
+  const int base = (blockIdx.x * 4 + wave) * 64;
+  const int P = a.n_pixels;
+  if (base >= P) return;
+  const int my_px = base + lane;
+  const bool mine = my_px < P;
+  const bool hit = mine && (a.env_flag == nullptr || a.env_flag[my_px] != 0);
+  const bool any_hit = __ballot(hit) != 0;
+
+  float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
+  if (any_hit) {
+    const int ea1 = act_exp(a.amax[0]), ea2 = act_exp(a.amax[1]), ea3 = act_exp(a.amax[2]);
+    const int sa2 = scale_byte(127 + ea2 - a.w8_exp[0]), sb2 = scale_byte(127 - ea1);
+    const int sa3 = scale_byte(127 + ea3 - a.w8_exp[1]), sb3 = scale_byte(127 - ea2);
+    const int sa4 = scale_byte(127 - a.w8_exp[2]), sb4 = scale_byte(127 - ea3);
+    const float fs = __uint_as_float((uint32_t)(ea1 + 127) << 23);
+
+    // First-layer B operands: group g, column col, features 16 s + 8 h + [0, 8).
+    const uint16_t *feat = (const uint16_t *)a.features;
+    u32x4 f[2][2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int p = base + 32 * g + col;
+#pragma unroll
+      for (int s = 0; s < 2; ++s)
+        f[g][s] = p < P ? scale_bf16x8(*(const u32x4 *)(feat + (size_t)p * 32 + 16 * s + 8 * h), fs)
+                        : u32x4{0u, 0u, 0u, 0u};
+    }
+    const i32x8 *w8 = (const i32x8 *)a.w8_tiles;
+    i32x8 w2[2], w3[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      w2[b] = w8[(0 * 2 + b) * 64 + lane];
+      w3[b] = w8[(1 * 2 + b) * 64 + lane];
+    }
+    const u32x4 *mk = (const u32x4 *)a.mk;
+    u32x4 ta[4], tb[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ta[q] = mk[q * 64 + lane];
+
+    // The four layers of one wavelet over the wave's 64 pixels.  The output
+    // layer's variant b carries W4 in rows 4b..4b+2 and takes group b, so
+    // lane l ends with pixel base + l's outputs in elements 0..2.
+    auto step = [&](const u32x4(&t)[4]) -> f32x16 {
+      const f32x16 z = {};
+      f32x16 d[2][2];
+      i32x8 x[2];
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          d[g][b] = mfma_bf16(t[2 * b + 1], f[g][1], mfma_bf16(t[2 * b], f[g][0], z));
+#pragma unroll
+      for (int g = 0; g < 2; ++g) x[g] = relu_fp8(d[g][0], d[g][1]);
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) d[g][b] = mfma_fp8(w2[b], x[g], z, sa2, sb2);
+#pragma unroll
+      for (int g = 0; g < 2; ++g) x[g] = relu_fp8(d[g][0], d[g][1]);
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) d[g][b] = mfma_fp8(w3[b], x[g], z, sa3, sb3);
+#pragma unroll
+      for (int g = 0; g < 2; ++g) x[g] = relu_fp8(d[g][0], d[g][1]);
+      const f32x16 o = mfma_fp8(w4s[lane], x[0], z, sa4, sb4);
+      return mfma_fp8(w4s[64 + lane], x[1], o, sa4, sb4);
+    };
+    // Wavelets two at a time: the clamped terms of k and k + 1 are added and
+    // the pair sum enters the Neumaier-compensated running sum.
+    for (int k = 0; k < a.k; k += 2) {
+      const bool two = k + 1 < a.k;
+      if (two) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tb[q] = mk[((size_t)(k + 1) * 4 + q) * 64 + lane];
+      }
+      const f32x16 o0 = step(ta);
+      // Keep the two wavelets' layer chains apart: interleaved, their
+      // accumulators (64 VGPRs each) would not fit beside the operands.
+      __builtin_amdgcn_sched_barrier(0);
+      f32x16 o1 = {};
+      if (two) {
+        if (k + 2 < a.k) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) ta[q] = mk[((size_t)(k + 2) * 4 + q) * 64 + lane];
+        }
+        o1 = step(tb);
+      }
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float t0 = dev::transport<SOFTPLUS>(o0[ch]);
+        float x = dev::relu_f32(a.light[3 * k + ch] * t0);
+        float t1 = 0.f;
+        if (two) {
+          t1 = dev::transport<SOFTPLUS>(o1[ch]);
+          x += dev::relu_f32(a.light[3 * (k + 1) + ch] * t1);
+        }
+        dev::neumaier_add(sum[ch], comp[ch], x);
+        if (DUMP) {
+          const int q = my_px - a.dump_begin;
+          if (mine && q >= 0 && q < a.dump_count) {
+            a.dump[((size_t)q * a.k + k) * 3 + ch] = t0;
+            if (two) a.dump[((size_t)q * a.k + k + 1) * 3 + ch] = t1;
+          }
+        }
+      }
+    }
+  }
+  if (mine)
+    dev::write_outputs(a, my_px, hit ? sum[0] + comp[0] : 0.f, hit ? sum[1] + comp[1] : 0.f,
+                       hit ? sum[2] + comp[2] : 0.f);
+}
+
+struct Fp8Variant {
+  const char *name;
+};
+constexpr Fp8Variant kFp8Variants[] = {{"fp8.l1bf16.occ2"}};
+constexpr int kNumFp8Variants = sizeof(kFp8Variants) / sizeof(kFp8Variants[0]);
+
+// Exponent e with amax * 2^e in (224, 448], 0 for a zero or non-finite amax.
+int weight_exp(float amax) {
+  if (!(amax > 0.f) || !std::isfinite(amax)) return 0;
+  int E = 0;
+  const float m = std::frexp(amax, &E);  // amax = m 2^E, m in [0.5, 1)
+  const int e = m <= 0.875f ? 9 - E : 8 - E;
+  return std::min(std::max(e, -100), 100);
+}
+
+}  // namespace
+
+uint8_t f32_to_e4m3(float x) {
+  if (std::isnan(x)) return 0x7f;
+  const uint8_t sign = std::signbit(x) ? 0x80 : 0;
+  const float ax = std::fabs(x);
+  if (ax >= 448.f) return sign | 0x7e;  // saturate
+  if (ax < 0.015625f) {                 // subnormal range: multiples of 2^-9
+    const float q = std::nearbyint(ax * 512.f);  // 0..8; 8 is the smallest normal
+    return sign | (uint8_t)q;
+  }
+  int E = 0;
+  std::frexp(ax, &E);  // ax in [2^(E-1), 2^E)
+  const int ub = E - 1;
+  int mant = (int)std::nearbyint(std::ldexp(ax, 3 - ub)) - 8;  // 0..8
+  int exp = ub + 7;
+  if (mant == 8) {
+    mant = 0;
+    ++exp;
+  }
+  if (exp > 15 || (exp == 15 && mant == 7)) return sign | 0x7e;
+  return sign | (uint8_t)(exp << 3) | (uint8_t)mant;
+}
+
+float e4m3_to_f32(uint8_t c) {
+  const int e = (c >> 3) & 15, m = c & 7;
+  if (e == 15 && m == 7) return NAN;
+  const float v = e == 0 ? std::ldexp((float)m, -9) : std::ldexp(1.0f + m / 8.0f, e - 7);
+  return (c & 0x80) ? -v : v;
+}
+
+// Tiles [layer][b][lane][j], 32 bytes per lane: layers 2, 3 (b = 32-row
+// output block) and the output layer's two variants (b: W4 in rows
+// 4b..4b+2, zeros elsewhere); slot (lane / 32, j) carries input pi(h, j).
+void build_fp8_weight_tiles(const float *w, std::vector<uint8_t> &out, int exps[3]) {
+  constexpr int kW1 = 64 * 32, kW2 = 64 * 64, kW3 = 64 * 64;
+  const float *mats[3] = {w + kW1, w + kW1 + kW2, w + kW1 + kW2 + kW3};
+  const int rows[3] = {64, 64, 3};
+  for (int l = 0; l < 3; ++l) {
+    float amax = 0.f;
+    for (int i = 0; i < rows[l] * 64; ++i) amax = std::max(amax, std::fabs(mats[l][i]));
+    exps[l] = weight_exp(amax);
+  }
+  auto pi = [](int h, int j) { return 32 * (j >> 4) + 8 * ((j & 15) >> 2) + 4 * h + (j & 3); };
+  out.assign(3 * 2 * 64 * 32, 0);
+  for (int l = 0; l < 3; ++l) {
+    const float s = std::ldexp(1.0f, exps[l]);
+    for (int b = 0; b < 2; ++b)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int j = 0; j < 32; ++j) {
+          const int r = lane & 31, in = pi(lane >> 5, j);
+          float v = 0.f;
+          if (l < 2) v = mats[l][(32 * b + r) * 64 + in];
+          else if (r >= 4 * b && r < 4 * b + 3) v = mats[2][(r - 4 * b) * 64 + in];
+          out[(((size_t)l * 2 + b) * 64 + lane) * 32 + j] = f32_to_e4m3(v * s);
+        }
+  }
+}
+
+int relight_fp8_variant_count() { return kNumFp8Variants; }
+const char *relight_fp8_variant_name(int v) {
+  return v >= 0 && v < kNumFp8Variants ? kFp8Variants[v].name : nullptr;
+}
+
+void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s) {
+  require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: bad pixel count or K");
+  require(a.features && a.mk && a.light && a.w8_tiles && a.amax && a.direct,
+          NWPRT_ERR_INVALID_ARGUMENT, "relight: missing FP8 operand");
+  require(a.aux == nullptr, NWPRT_ERR_UNSUPPORTED,
+          "relight: the FP8 form does not support scenes with aux inputs");
+  require(variant >= 0 && variant < kNumFp8Variants, NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: unknown FP8 kernel variant");
+  const dim3 grid((a.n_pixels + kBlock - 1) / kBlock), block(kBlock);
+  if (a.softplus) {
+    if (a.dump) hipLaunchKernelGGL((relight_fp8_kernel<true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((relight_fp8_kernel<true, false>), grid, block, 0, s, a);
+  } else {
+    if (a.dump) hipLaunchKernelGGL((relight_fp8_kernel<false, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((relight_fp8_kernel<false, false>), grid, block, 0, s, a);
+  }
+  hip_check(hipGetLastError(), "relight_fp8_kernel launch");
+}
+
+}  // namespace nwprt

@@ -22,6 +22,7 @@
 
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "relight_device.hpp"
 
 namespace nwprt {
 namespace {
@@ -89,18 +90,8 @@ __device__ __forceinline__ u32x4 relu_pack(f32x4 lo, f32x4 hi) {
   return r;
 }
 
-__device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float r, float g,
-                                              float b) {
-  const float4 d = *(const float4 *)(a.direct + 4 * (size_t)p);
-  const float4 ind = make_float4(r, g, b, 1.0f);
-  const float4 wd = make_float4(r + d.x, g + d.y, b + d.z, 1.0f);
-  if (a.indirect) *(float4 *)(a.indirect + 4 * (size_t)p) = ind;
-  if (a.with_direct) *(float4 *)(a.with_direct + 4 * (size_t)p) = wd;
-  if (a.rgba8 && a.tonemap_source != NWPRT_TONEMAP_NONE) {
-    const float4 t = a.tonemap_source == NWPRT_TONEMAP_FULL ? wd : ind;
-    a.rgba8[p] = dev::tonemap_rgba8(t.x, t.y, t.z);
-  }
-}
+using dev::transport;
+using dev::write_outputs;
 
 // Tile-shape variants of the fused kernel (chosen per shape by the
 // autotuner): GP pixel groups carried through the layers together (GP
@@ -261,32 +252,18 @@ __device__ __forceinline__ f32x4 wavelet_step_pipelined(
   return out;
 }
 
-// Transport of one output (softplus or identity) and its clamped light term
-// max(light * T, 0) (Renderer/tinycudann/utils_kernels.h:74-78).
-template <bool SOFTPLUS>
-__device__ __forceinline__ float transport(float o) {
-  return SOFTPLUS ? dev::softplus(o) : o;
-}
-
-// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
-// pixels [64 (4 b + w), +64).  Pixel groups j = 0..3 (16 pixels each) are the
-// MFMA columns; lane l owns pixel base + l for the reduction.  The next
-// wavelet's first-layer tiles are fetched while the current one computes
-// (two register sets, K loop unrolled by two).  GP = 0 selects the
-// software-pipelined step.
-template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, int MINW>
-__global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
-  __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// One wave's 64 pixels [base, base + 64) of a frame (base < a.n_pixels).
+// Pixel groups j = 0..3 (16 pixels each) are the MFMA columns; lane l owns
+// pixel base + l for the reduction.  The next wavelet's first-layer tiles
+// are fetched while the current one computes (two register sets, K loop
+// unrolled by two).  GP = 0 selects the software-pipelined step; W4LDS reads
+// the output layer's tiles from `w4s` (filled by the caller).
+template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS>
+__device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int lane,
+                                             const u32x4 *w4s) {
   const int g = lane >> 4, n = lane & 15;
   const u32x4 *wt = (const u32x4 *)a.w_tiles;
-  if (W4LDS) {
-    for (int i = threadIdx.x; i < 4 * 2 * 64; i += kRelightBlock) w4s[i] = wt[2 * 8 * 64 + i];
-    __syncthreads();
-  }
-  const int base = (blockIdx.x * 4 + wave) * 64;
   const int P = a.n_pixels;
-  if (base >= P) return;
   const int my_px = base + lane;
   const bool mine = my_px < P;
   const bool hit = mine && (a.env_flag == nullptr || a.env_flag[my_px] != 0);
@@ -375,6 +352,49 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
     write_outputs(a, my_px, r, gg, bb);
   }
 }
+
+// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
+// pixels [64 (4 b + w), +64).
+template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, int MINW>
+__global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
+  __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (W4LDS) {
+    const u32x4 *wt = (const u32x4 *)a.w_tiles;
+    for (int i = threadIdx.x; i < 4 * 2 * 64; i += kRelightBlock) w4s[i] = wt[2 * 8 * 64 + i];
+    __syncthreads();
+  }
+  const int base = (blockIdx.x * 4 + wave) * 64;
+  if (base >= a.n_pixels) return;
+  relight_wave<E, SOFTPLUS, DUMP, AUX, GP, W4LDS>(a, base, lane, w4s);
+}
+
+// Persistent batch form: a machine-sized grid whose waves take work items
+// (job, first pixel of a 64-pixel group) from a shared counter until the
+// list is exhausted.  Jobs are whole relights with their own frame,
+// scene weights, selection and K; items are listed largest first so the
+// tail of the launch is short items.  Output-layer tiles stay in VGPRs (no
+// LDS, so no workgroup barrier and no scene switch cost beyond reloading).
+template <typename E, bool SOFTPLUS, int GP, int MINW>
+__global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
+    const RelightArgs *__restrict__ jobs, const BatchItem *__restrict__ items,
+    const int *__restrict__ n_items_p, unsigned int *__restrict__ counter) {
+  const int lane = threadIdx.x & 63;
+  const int n_items = *n_items_p;
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = (int)atomicAdd(counter, 1u);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= n_items) break;
+    const BatchItem item = items[it];
+    const RelightArgs a = jobs[item.job];
+    // One 64-pixel group per item: a loop over several would let the
+    // compiler hoist the group's loads and spill.
+    relight_wave<E, SOFTPLUS, false, false, GP, false>(a, item.base, lane, nullptr);
+  }
+}
+
+__global__ void reset_counter_kernel(unsigned int *counter) { *counter = 0u; }
 
 // Plain FP32 form: one lane per pixel walks the K wavelets with the four
 // layers on the vector ALU, weights broadcast from LDS and the selected
@@ -473,6 +493,94 @@ __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs 
                 hit ? sum[2] + comp[2] : 0.f);
 }
 
+// Activation calibration for the FP8 form: the BF16 network over a pixel
+// subsample (a.features, a.n_pixels) and the wavelets [k0, k0 + k_chunk) of
+// blockIdx.y, recording the maximum pre-activation output of layers 1..3
+// (the ReLU outputs share it when positive).  One atomicMax per wave and
+// layer on the float bits, which order like the values for non-negative
+// floats; the maxima start at 0.
+__global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a, int k_chunk) {
+  using E = __bf16;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane >> 4, n = lane & 15;
+  const int base = (blockIdx.x * 4 + wave) * 64;
+  const int k0 = blockIdx.y * k_chunk, k1 = min(a.k, k0 + k_chunk);
+  const int P = a.n_pixels;
+  if (base >= P || k0 >= k1) return;
+  const uint16_t *feat = (const uint16_t *)a.features;
+  u32x4 hb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int p = base + 16 * j + n;
+    hb[j] = p < P ? *(const u32x4 *)(feat + (size_t)p * 32 + 8 * g) : u32x4{0u, 0u, 0u, 0u};
+  }
+  const u32x4 *wt = (const u32x4 *)a.w_tiles;
+  u32x4 w2[4][2], w3[4][2];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      w2[b][kb] = wt[(0 * 8 + b * 2 + kb) * 64 + lane];
+      w3[b][kb] = wt[(1 * 8 + b * 2 + kb) * 64 + lane];
+    }
+  const u32x4 *mk = (const u32x4 *)a.mk;
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  float m[3] = {0.f, 0.f, 0.f};
+  auto upd = [](float &mm, f32x4 c) { mm = fmaxf(mm, fmaxf(fmaxf(c[0], c[1]), fmaxf(c[2], c[3]))); };
+  for (int k = k0; k < k1; ++k) {
+    u32x4 a1[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) a1[b] = mk[((size_t)k * 4 + b) * 64 + lane];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      f32x4 c[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        c[b] = Mfma<E>::run(a1[b], hb[j], zero);
+        upd(m[0], c[b]);
+      }
+      u32x4 x0 = relu_pack<E>(c[0], c[1]), x1 = relu_pack<E>(c[2], c[3]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        c[b] = Mfma<E>::run(w2[b][1], x1, Mfma<E>::run(w2[b][0], x0, zero));
+        upd(m[1], c[b]);
+      }
+      x0 = relu_pack<E>(c[0], c[1]);
+      x1 = relu_pack<E>(c[2], c[3]);
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        c[b] = Mfma<E>::run(w3[b][1], x1, Mfma<E>::run(w3[b][0], x0, zero));
+        upd(m[2], c[b]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float v = m[i];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    if (lane == 0) atomicMax((unsigned int *)a.amax + i, __float_as_uint(v));
+  }
+}
+
+// Calibration subsample: calibration pixel q is frame pixel q * stride, its
+// features copied (zeros for a background pixel, whose features the encode
+// pass leaves unwritten); block 0 also clears the four maxima.
+__global__ __launch_bounds__(256) void calib_gather_kernel(const uint16_t *__restrict__ feat,
+                                                           const uint8_t *__restrict__ flag,
+                                                           int n_pixels, int stride,
+                                                           uint16_t *__restrict__ calib,
+                                                           int n_cal, float *__restrict__ amax) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) amax[threadIdx.x] = 0.f;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int q = i >> 2, part = i & 3;
+  if (q >= n_cal) return;
+  const long long p = (long long)q * stride;
+  const bool hit = p < n_pixels && (flag == nullptr || flag[p] != 0);
+  const u32x4 v = hit ? *(const u32x4 *)(feat + p * 32 + 8 * part) : u32x4{0u, 0u, 0u, 0u};
+  *(u32x4 *)(calib + (size_t)q * 32 + 8 * part) = v;
+}
+
 __global__ __launch_bounds__(256) void tonemap_kernel(const float *__restrict__ src,
                                                       uint32_t *__restrict__ dst, int n) {
   const int p = blockIdx.x * 256 + threadIdx.x;
@@ -513,6 +621,26 @@ void launch_sp(const RelightArgs &a, int variant, hipStream_t s) {
     case 5: launch_v<E, SP, 5>(a, grid, block, s); break;
     case 6: launch_v<E, SP, 6>(a, grid, block, s); break;
     default: fail(NWPRT_ERR_INVALID_ARGUMENT, "relight: unknown kernel variant");
+  }
+}
+
+template <typename E, bool SP, int V>
+void launch_batch_v(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
+                    unsigned int *counter, int n_cu, hipStream_t s) {
+  constexpr MfmaVariant v = kMfmaVariants[V];
+  static_assert(!v.w4_lds, "the batch form keeps the output layer in VGPRs");
+  hipLaunchKernelGGL((relight_batch_kernel<E, SP, v.gp, v.min_waves>), dim3(n_cu * v.min_waves),
+                     dim3(kRelightBlock), 0, s, jobs, items, n_items, counter);
+}
+
+template <typename E, bool SP>
+void launch_batch_e(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
+                    unsigned int *counter, int variant, int n_cu, hipStream_t s) {
+  switch (variant) {
+    case 0: launch_batch_v<E, SP, 0>(jobs, items, n_items, counter, n_cu, s); break;
+    case 1: launch_batch_v<E, SP, 1>(jobs, items, n_items, counter, n_cu, s); break;
+    case 5: launch_batch_v<E, SP, 5>(jobs, items, n_items, counter, n_cu, s); break;
+    default: fail(NWPRT_ERR_INVALID_ARGUMENT, "batch: variant has no persistent form");
   }
 }
 
@@ -600,6 +728,53 @@ void launch_relight_plain(const RelightArgs &a, hipStream_t s) {
     else hipLaunchKernelGGL((relight_plain_kernel<false, false>), grid, block, 0, s, a);
   }
   hip_check(hipGetLastError(), "relight_plain_kernel launch");
+}
+
+bool relight_batch_variant_ok(int v) {
+  return v >= 0 && v < kNumMfmaVariants && !kMfmaVariants[v].w4_lds;
+}
+
+void launch_relight_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
+                          unsigned int *counter, Elem elem, bool softplus, int variant,
+                          int n_cu, hipStream_t s) {
+  require(jobs && items && n_items && counter && n_cu >= 1, NWPRT_ERR_INVALID_ARGUMENT,
+          "batch: missing operand");
+  if (elem == Elem::BF16) {
+    if (softplus) launch_batch_e<__bf16, true>(jobs, items, n_items, counter, variant, n_cu, s);
+    else launch_batch_e<__bf16, false>(jobs, items, n_items, counter, variant, n_cu, s);
+  } else if (elem == Elem::F16) {
+    if (softplus) launch_batch_e<_Float16, true>(jobs, items, n_items, counter, variant, n_cu, s);
+    else launch_batch_e<_Float16, false>(jobs, items, n_items, counter, variant, n_cu, s);
+  } else {
+    fail(NWPRT_ERR_INVALID_ARGUMENT, "batch: MFMA element must be BF16 or FP16");
+  }
+  hip_check(hipGetLastError(), "relight_batch_kernel launch");
+}
+
+void launch_reset_counter(unsigned int *counter, hipStream_t s) {
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, counter);
+  hip_check(hipGetLastError(), "reset_counter_kernel launch");
+}
+
+void launch_calib_gather(const uint16_t *features, const uint8_t *env_flag, int n_pixels,
+                         int stride, uint16_t *calib, int n_cal, float *amax, hipStream_t s) {
+  require(features && calib && amax && n_pixels >= 1 && stride >= 1 && n_cal >= 1 &&
+              (long long)(n_cal - 1) * stride < n_pixels,
+          NWPRT_ERR_INVALID_ARGUMENT, "calibration gather: bad subsample");
+  const int threads = 4 * n_cal;
+  hipLaunchKernelGGL(calib_gather_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, features,
+                     env_flag, n_pixels, stride, calib, n_cal, amax);
+  hip_check(hipGetLastError(), "calib_gather_kernel launch");
+}
+
+void launch_calibrate_amax(const RelightArgs &a, hipStream_t s) {
+  require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K && a.features && a.mk &&
+              a.w_tiles && a.amax,
+          NWPRT_ERR_INVALID_ARGUMENT, "calibration: missing operand");
+  constexpr int kChunk = 16;
+  const dim3 grid((a.n_pixels + kRelightBlock - 1) / kRelightBlock, (a.k + kChunk - 1) / kChunk);
+  hipLaunchKernelGGL(calib_amax_kernel, grid, dim3(kRelightBlock), 0, s, a, kChunk);
+  hip_check(hipGetLastError(), "calib_amax_kernel launch");
 }
 
 void launch_tonemap(const float *src, uint32_t *dst, int n_pixels, hipStream_t s) {
