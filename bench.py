@@ -46,6 +46,8 @@ def parse():
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched uneven-jobs leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -72,6 +74,57 @@ def measured_traffic(workload_key):
 
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
+
+
+def bench_batch(nw, ctx, prec, reps=3):
+    """32 uneven relight jobs (workloads.batch_jobs: 256^2..1080p, K 32..300,
+    own 6x64x64 cubemaps, 2 scenes) as one captured graph with the persistent
+    kernel, against the same jobs relit one call at a time."""
+    from neural_wavelet_prt_demo_amd import workloads
+    from neural_wavelet_prt_demo_amd import Event
+    specs = workloads.batch_jobs()
+    desc = nw.config_scene_desc(64, nw.DTYPE_F16)
+    scenes = [ctx.scene_create(desc, nw.synth_scene(desc, 1000 + i)) for i in range(2)]
+    jobs, keep = [], []
+    for sp in specs:
+        env = ctx.envmap_from_faces(nw.synth_cubemap(sp.seed, 64), sp.k)
+        frame = nw.DeviceFrame(sp.width, sp.height, nw.synth_gbuffer(sp.seed, sp.width, sp.height, 0.0))
+        jobs.append((scenes[sp.scene], env, frame))
+        keep.append(frame)
+    ctx.sync()
+    bid = ctx.batch_create(jobs, prec)
+    best, _ = ctx.batch_autotune(bid)
+    n_items, n_pairs, variant = ctx.batch_info(bid)
+    stream = ctx.stream
+
+    def timed(fn, n):
+        fn()
+        ctx.sync()
+        ev = [Event() for _ in range(n + 1)]
+        ev[0].record(stream)
+        for i in range(n):
+            fn()
+            ev[i + 1].record(stream)
+        ctx.sync()
+        return [ev[i].elapsed_ms(ev[i + 1]) for i in range(n)]
+
+    batch_ms = timed(lambda: ctx.batch_launch(bid), reps)
+
+    def sequential():
+        for scene, env, frame in jobs:
+            ctx.relight(scene, env, frame, prec, nw.TONEMAP_FULL, stream)
+
+    seq_ms = timed(sequential, max(1, reps - 1))
+    ctx.batch_destroy(bid)
+    b = float(np.mean(batch_ms))
+    s = float(np.mean(seq_ms))
+    return {"workload": "32 jobs: 256^2..1920x1080, K in {32,64,128,300}, 6x64x64 cubemap each, "
+                        "2 scenes", "jobs": len(specs),
+            "pixels": int(sum(sp.width * sp.height for sp in specs)), "pairs": int(n_pairs),
+            "work_items": int(n_items), "tile_variant": nw.Context.relight_variants()[variant],
+            "graph_ms": b, "evals_per_s": n_pairs / (b / 1e3),
+            "one_call_per_job_ms": s, "one_call_per_job_evals_per_s": n_pairs / (s / 1e3),
+            "speedup_vs_one_call_per_job": s / b}
 
 
 def main():
@@ -206,6 +259,10 @@ def main():
         "encode_pass": {"bound": "hbm", "ms": encode_ms, "bytes_per_pixel": ENCODE_BYTES_PER_PIXEL,
                         "achieved_gbs": encode_gbs, "frac": encode_gbs / HBM_PEAK_GBS},
     }
+
+    # --- batched uneven jobs: one graph, one persistent kernel ------------
+    if not args.no_batch and prec != nw.PREC_FP8:
+        result["batched"] = bench_batch(nw, ctx, prec)
 
     # --- accuracy against the FP64 oracle on a pixel sample ---------------
     if not args.no_accuracy:

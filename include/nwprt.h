@@ -262,6 +262,16 @@ nwprt_status nwprt_envmap_update_faces(nwprt_ctx ctx, int32_t env_id,
                                        const float *faces,
                                        nwprt_location faces_location,
                                        void *stream);
+/* No counterpart in the reference (a served request's rotated map).  As
+ * nwprt_envmap_update_faces, from `faces` (device, [6][N][N][3], not the
+ * map's own buffers) turned by `yaw` radians about +y: each texel takes the
+ * source texel nearest to its rotated centre direction, with the face and
+ * (u, v) conventions of EnvUVToDirec / EnvDirecToUV
+ * (Renderer/device_functions/cubeMap.cu:23-115).  The first call on a map
+ * allocates its rotated-faces buffer; later calls allocate nothing. */
+nwprt_status nwprt_envmap_update_rotated(nwprt_ctx ctx, int32_t env_id,
+                                         const float *faces, double yaw,
+                                         void *stream);
 /* Precomputed selection: indices int32[K] into [0, 6N^2) and strengths
  * float32[K][3] (Renderer/EnvironmentMap.cpp:56-67 +
  * RenderEngine::buildLights wavelet upload, Renderer/RenderEngine.cpp:167-175). */

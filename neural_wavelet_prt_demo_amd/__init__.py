@@ -13,8 +13,8 @@ weight folder (Renderer/tinycudann/tcnn.cu:49), ``TCNNPass_p1`` /
 from __future__ import annotations
 
 import ctypes
-from ctypes import (POINTER, Structure, c_char_p, c_float, c_int, c_int32, c_int64, c_size_t,
-                    c_uint8, c_uint32, c_uint64, c_void_p)
+from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int32, c_int64,
+                    c_size_t, c_uint8, c_uint32, c_uint64, c_void_p)
 from pathlib import Path
 from typing import Dict, Optional
 
@@ -157,6 +157,7 @@ _SIGNATURES = {
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
                                       c_int32, c_int32, c_int32, c_void_p, c_int, c_void_p]),
+    "nwprt_envmap_update_rotated": (c_int, [c_void_p, c_int32, c_void_p, c_double, c_void_p]),
     "nwprt_batch_create": (c_int, [c_void_p, POINTER(BatchJob), c_int32, c_int, c_int,
                                    POINTER(c_int32)]),
     "nwprt_batch_launch": (c_int, [c_void_p, c_int32, c_void_p]),
@@ -469,6 +470,13 @@ class Context:
         c = np.empty((6, n, n, 3), np.float32)
         _check(lib().nwprt_envmap_get_coefficients(self.h, env, _ptr(c)))
         return c
+
+    def envmap_update_rotated(self, env: int, faces: DeviceBuffer, yaw: float,
+                              stream: Optional[int] = None) -> None:
+        """Recompute a faces-built map from `faces` (device) turned by `yaw`
+        radians about +y."""
+        _check(lib().nwprt_envmap_update_rotated(self.h, env, faces.ptr, float(yaw),
+                                                 c_void_p(stream) if stream else None))
 
     def envmap_destroy(self, env: int) -> None:
         _check(lib().nwprt_envmap_destroy(self.h, env))

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <map>
 #include <tuple>
@@ -65,6 +66,10 @@ struct EnvMap {
   int n = 0, k = 0;
   bool has_coef = false;
   DevBuf idx, strength, coef;
+  // Faces-built maps: top-K work buffer and, once rotated, the rotated faces
+  // (per map, so updates of different maps on different streams never share
+  // scratch memory).
+  DevBuf scores, rotated;
   hipEvent_t ready = nullptr;
   ~EnvMap() {
     if (ready) (void)hipEventDestroy(ready);
@@ -119,7 +124,7 @@ struct nwprt_context {
   std::map<int, std::unique_ptr<nwprt::EnvMap>> envs;
   int next_id = 1;
   nwprt::FrameSlot slots[3];
-  nwprt::DevBuf staging, scores;
+  nwprt::DevBuf staging;
   // Tile tuning: forced variant (-1: none) and the autotuner's choices keyed
   // by (element type, K rounded up to a power of two, ceil(log2 pixels)).
   int forced_variant = -1;
@@ -292,10 +297,10 @@ void env_from_faces(nwprt_ctx ctx, const float *faces, nwprt_location loc, int n
   EnvMap &e = new_env(ctx, n, k, &id);
   try {
     e.coef.ensure(m * 12);
+    e.scores.ensure(m * 4);
     e.has_coef = true;
-    ctx->scores.ensure(m * 4);
     launch_haar(dfaces, m * 3, e.coef.as<float>(), n, 1, ctx->stream);
-    launch_topk(e.coef.as<float>(), n, k, ctx->scores.as<uint32_t>(), e.idx.as<int32_t>(),
+    launch_topk(e.coef.as<float>(), n, k, e.scores.as<uint32_t>(), e.idx.as<int32_t>(),
                 e.strength.as<float>(), ctx->stream);
     hip_check(hipEventRecord(e.ready, ctx->stream), "hipEventRecord");
     if (loc == NWPRT_HOST)  // the staging buffer is reused by the next call
@@ -776,12 +781,30 @@ nwprt_status nwprt_envmap_update_faces(nwprt_ctx ctx, int32_t env_id, const floa
     } else {
       require(faces_location == NWPRT_DEVICE, NWPRT_ERR_INVALID_ARGUMENT, "bad faces location");
     }
-    ctx->scores.ensure(m * 4);
     launch_haar(dfaces, m * 3, e.coef.as<float>(), e.n, 1, s);
-    launch_topk(e.coef.as<float>(), e.n, e.k, ctx->scores.as<uint32_t>(), e.idx.as<int32_t>(),
+    launch_topk(e.coef.as<float>(), e.n, e.k, e.scores.as<uint32_t>(), e.idx.as<int32_t>(),
                 e.strength.as<float>(), s);
     hip_check(hipEventRecord(e.ready, s), "hipEventRecord");
     if (faces_location == NWPRT_HOST) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  });
+}
+
+nwprt_status nwprt_envmap_update_rotated(nwprt_ctx ctx, int32_t env_id, const float *faces,
+                                         double yaw, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    EnvMap &e = get_env(ctx, env_id);
+    require(e.has_coef, NWPRT_ERR_STATE, "environment map was not built from faces");
+    require(faces != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null faces");
+    require(std::isfinite(yaw), NWPRT_ERR_INVALID_ARGUMENT, "yaw must be finite");
+    const hipStream_t s = pick(ctx, stream);
+    const size_t m = 6ull * e.n * e.n;
+    e.rotated.ensure(m * 12);
+    launch_rotate_cubemap(faces, e.rotated.as<float>(), e.n, yaw, s);
+    launch_haar(e.rotated.as<float>(), m * 3, e.coef.as<float>(), e.n, 1, s);
+    launch_topk(e.coef.as<float>(), e.n, e.k, e.scores.as<uint32_t>(), e.idx.as<int32_t>(),
+                e.strength.as<float>(), s);
+    hip_check(hipEventRecord(e.ready, s), "hipEventRecord");
   });
 }
 

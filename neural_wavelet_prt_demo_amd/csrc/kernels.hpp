@@ -28,6 +28,10 @@ void launch_haar(const float *faces, size_t map_stride, float *coef, int n,
 void launch_topk(const float *coef, int n, int k, uint32_t *scores,
                  int32_t *indices, float *strengths, hipStream_t s);
 
+// Yaw rotation (radians, about +y) of a cubemap [6][N][N][3] into `dst`
+// (nearest source texel to each rotated texel centre).
+void launch_rotate_cubemap(const float *src, float *dst, int n, double yaw, hipStream_t s);
+
 // Per-relight preparation of the selected wavelets:
 //  mk   (may be null): [K][4][64][8] 16x16x32 MFMA A-operand tiles of
 //                      W1 * diag(wfeat[:,c_k])

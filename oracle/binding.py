@@ -57,6 +57,7 @@ def lib():
         l.orc_hash_encode.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_void_p]
         l.orc_haar_face.argtypes = [c_void_p, c_int, c_int, c_void_p]
         l.orc_envmap_haar.argtypes = [c_void_p, c_int, c_void_p]
+        l.orc_rotate_cubemap.argtypes = [c_void_p, c_int, c_double, c_void_p]
         l.orc_score.argtypes = [c_void_p, c_int, c_int]
         l.orc_score.restype = c_float
         l.orc_topk.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p]
@@ -167,6 +168,13 @@ def envmap_haar(faces: np.ndarray) -> np.ndarray:
     coef = np.zeros_like(faces)
     lib().orc_envmap_haar(_p(faces), faces.shape[1], _p(coef))
     return coef
+
+
+def rotate_cubemap(faces: np.ndarray, yaw: float) -> np.ndarray:
+    faces = np.ascontiguousarray(faces, np.float32)
+    out = np.zeros_like(faces)
+    lib().orc_rotate_cubemap(_p(faces), faces.shape[1], float(yaw), _p(out))
+    return out
 
 
 def haar_face(plane: np.ndarray) -> np.ndarray:

@@ -20,6 +20,9 @@
  *   orc_haar_face        closed forms (constant face, delta), Parseval,
  *                        inverse round trip, pywt-free textbook recursion
  *   orc_topk             full sort on (score desc, index asc), ties, K = all
+ *   orc_rotate_cubemap   yaw 0 and 4 x 90 degrees = identity, 90 degrees =
+ *                        the face permutation the axes give on face-constant
+ *                        maps, 90 + 90 = 180, the +-Y faces turn in place
  *   orc_relight_pixels   brute-force numpy FP64 network on tiny cases,
  *                        closed forms (zero weights -> softplus(0) = ln 2,
  *                        background -> 0, negative light -> clamped)
@@ -76,6 +79,7 @@ void orc_hash_encode(const orc_scene *sc, const float pos[3], uint32_t *indices,
 
 void orc_haar_face(const float *src, int n, int stride, float *dst);
 void orc_envmap_haar(const float *faces, int n, float *coef);
+void orc_rotate_cubemap(const float *src, int n, double yaw, float *dst);
 float orc_score(const float *coef, int t, int n);
 void orc_topk(const float *coef, int n, int k, int32_t *indices, float *strengths);
 
