@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-jobs leg")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the served-stream leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -74,6 +76,74 @@ def measured_traffic(workload_key):
 
 def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
+
+
+def bench_serving(nw, ctx, prec, n_requests=60):
+    """Served stream: Poisson arrivals, 4 resident scenes (face 128, FP16 hash
+    grid 16 x 2^19 x 2), K = 300; 70 % of requests turn a resident 6x128x128
+    cubemap by a random yaw, 30 % bring a fresh cubemap from host memory.
+    Latency is measured from each request's scheduled arrival to the poll
+    that sees it complete; throughput over the span first arrival -> last
+    completion.  Memory budget 200 GB of request state."""
+    from neural_wavelet_prt_demo_amd import DeviceBuffer, FrameOut
+    desc = nw.config_scene_desc(128, nw.DTYPE_F16)
+    scenes = [ctx.scene_create(desc, nw.synth_scene(desc, 2000 + i)) for i in range(4)]
+    bases = [DeviceBuffer.from_array(nw.synth_cubemap(3000 + i, 128)) for i in range(4)]
+    fresh = [nw.synth_cubemap(4000 + j, 128) for j in range(8)]
+    views = {"1080p": (1920, 1080), "4k": (3840, 2160)}
+    gbufs = {v: nw.DeviceFrame(w, h, nw.synth_gbuffer(5000 + i, w, h, 0.0))
+             for i, (v, (w, h)) in enumerate(views.items())}
+    ring = 64
+    outs = {v: [(DeviceBuffer(w * h * 16), DeviceBuffer(w * h * 16), DeviceBuffer(w * h * 4))
+                for _ in range(ring)] for v, (w, h) in views.items()}
+    ctx.sync()
+
+    def stream(rate, p4k, seed):
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+        srv = ctx.server_create(budget_bytes=200 << 30, max_active=ring, precision=prec)
+        plan = []
+        for i in range(n_requests):
+            v = "4k" if rng.random() < p4k else "1080p"
+            rotated = rng.random() < 0.7
+            plan.append((v, int(rng.integers(4)), rotated, float(rng.uniform(0, 2 * np.pi)),
+                         int(rng.integers(len(fresh)))))
+        lat, arr, pairs = {}, {}, 0
+        t0 = time.perf_counter()
+        i = 0
+        while len(lat) < n_requests:
+            now = time.perf_counter() - t0
+            while i < n_requests and arrivals[i] <= now:
+                v, s, rotated, yaw, fj = plan[i]
+                o = outs[v][i % ring]
+                fo = FrameOut(o[0].ptr, o[1].ptr, o[2].ptr)
+                if rotated:
+                    rid = ctx.server_submit(srv, scenes[s], gbufs[v], faces=bases[s], yaw=yaw,
+                                            k=300, face_size=128, out=fo)
+                else:
+                    rid = ctx.server_submit(srv, scenes[s], gbufs[v], faces=fresh[fj], k=300,
+                                            face_size=128, out=fo)
+                arr[rid] = t0 + arrivals[i]
+                pairs += views[v][0] * views[v][1] * 300
+                i += 1
+            for rid, _ in ctx.server_poll(srv, wait=0):
+                lat[rid] = (time.perf_counter() - arr[rid]) * 1e3
+        span = time.perf_counter() - (t0 + arrivals[0])
+        st = ctx.server_stats(srv)
+        ctx.server_destroy(srv)
+        ms = np.array(list(lat.values()))
+        return {"rate_req_s": rate, "views": "1080p" if p4k == 0 else f"1080p + {int(p4k * 100)}% 4K",
+                "requests": n_requests, "throughput_req_s": n_requests / span,
+                "evals_per_s": pairs / span, "p50_ms": percentile(ms, 50),
+                "p99_ms": percentile(ms, 99), "max_ms": float(ms.max()),
+                "peak_active": st["peak_active"], "peak_state_gb": st["peak_bytes"] / 1e9,
+                "ticks": st["ticks"]}
+
+    runs = [stream(r, 0.0, r) for r in (30, 60, 120, 240, 480)]
+    runs.append(stream(60, 0.25, 7))
+    return {"workload": "Poisson arrivals, 4 resident scenes, K=300, 70% yaw-rotated resident "
+                        "6x128x128 cubemap / 30% fresh host cubemap, 60 requests per rate",
+            "memory_budget_gb": 200, "runs": runs}
 
 
 def bench_batch(nw, ctx, prec, reps=3):
@@ -263,6 +333,8 @@ def main():
     # --- batched uneven jobs: one graph, one persistent kernel ------------
     if not args.no_batch and prec != nw.PREC_FP8:
         result["batched"] = bench_batch(nw, ctx, prec)
+    if not args.no_serving and prec != nw.PREC_FP8:
+        result["served"] = bench_serving(nw, ctx, prec)
 
     # --- accuracy against the FP64 oracle on a pixel sample ---------------
     if not args.no_accuracy:

@@ -420,6 +420,78 @@ nwprt_status nwprt_batch_info(nwprt_ctx ctx, int32_t batch_id,
                               int32_t *variant);
 nwprt_status nwprt_batch_destroy(nwprt_ctx ctx, int32_t batch_id);
 
+/* --------------------------------------------------------------- serving -- */
+/* No counterpart in the reference (an interactive viewer relights one frame
+ * at a time, Renderer/OutputVisualizer.cpp:61-126).  A server takes relight
+ * requests as they arrive and keeps the GPU busy with all of them:
+ *  - submit() queues a request (scene, resident G-buffer, outputs, and an
+ *    environment map that is resident, a yaw rotation of a resident cubemap,
+ *    or a fresh cubemap);
+ *  - poll() retires finished work, admits queued requests in arrival order
+ *    while the admitted ones' device state fits the memory budget and a slot
+ *    is free, prepares each admitted request on a second stream (cubemap
+ *    rotation or upload, Haar + top-K, hash-grid encode, selection tiles),
+ *    and keeps up to two persistent relight launches ("ticks") in flight,
+ *    each filled in arrival order with up to tick_pairs (pixel, wavelet)
+ *    pairs of 64-pixel work items from every admitted request.
+ * A request that arrives while others run joins the next tick; its
+ * preparation overlaps their launches.  Each request's outputs equal, bit
+ * for bit, nwprt_relight of the same inputs with the same precision and tile
+ * variant.  Requests never share outputs while in flight (the caller's
+ * responsibility); G-buffers, resident maps and source cubemaps must stay
+ * valid until the request completes. */
+typedef struct {
+  uint64_t memory_budget_bytes; /* device bytes of admitted requests' state; 0: no limit */
+  int32_t max_active;           /* requests admitted at once (>= 1)         */
+  int64_t tick_pairs;           /* pairs per persistent launch; 0: 2^28     */
+  nwprt_precision precision;    /* BF16 or FP16                             */
+  nwprt_tonemap_source tonemap;
+  int32_t variant;              /* tile variant with a persistent form, -1: default */
+} nwprt_server_config;
+
+typedef enum {
+  NWPRT_ENV_RESIDENT = 0, /* env_id: an existing map (its K)                   */
+  NWPRT_ENV_ROTATED = 1,  /* faces (device) turned by yaw: Haar + top-K of K   */
+  NWPRT_ENV_FRESH = 2     /* faces (host or device): Haar + top-K of K         */
+} nwprt_env_source;
+
+typedef struct {
+  int32_t scene_id;
+  nwprt_gbuffer gbuffer;
+  int32_t width, height;
+  nwprt_frame_out out;
+  nwprt_env_source env_source;
+  int32_t env_id;
+  const float *faces;
+  nwprt_location faces_location;
+  int32_t face_size;
+  int32_t k;
+  double yaw;
+} nwprt_request;
+
+typedef struct {
+  int64_t submitted, completed, queued, active;
+  int64_t ticks, items;
+  uint64_t bytes_in_use, peak_bytes;
+  int32_t peak_active;
+} nwprt_server_stats;
+
+nwprt_status nwprt_server_create(nwprt_ctx ctx, const nwprt_server_config *config,
+                                 int32_t *server_id);
+nwprt_status nwprt_server_submit(nwprt_ctx ctx, int32_t server_id,
+                                 const nwprt_request *request,
+                                 int64_t *request_id);
+/* Advances the server and reports up to max_done completed requests (ids and
+ * submit-to-completion latency in ms, in completion order).  wait: 0 returns
+ * at once; 1 blocks until at least one request completes (or none is
+ * pending); 2 blocks until every submitted request has completed. */
+nwprt_status nwprt_server_poll(nwprt_ctx ctx, int32_t server_id, int32_t wait,
+                               int64_t *done_ids, double *latency_ms,
+                               int32_t max_done, int32_t *n_done);
+nwprt_status nwprt_server_get_stats(nwprt_ctx ctx, int32_t server_id,
+                                    nwprt_server_stats *stats);
+nwprt_status nwprt_server_destroy(nwprt_ctx ctx, int32_t server_id);
+
 #ifdef __cplusplus
 }
 #endif

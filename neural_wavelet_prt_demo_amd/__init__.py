@@ -89,6 +89,28 @@ class BatchJob(Structure):
                 ("width", c_int32), ("height", c_int32), ("out", FrameOut)]
 
 
+ENV_RESIDENT, ENV_ROTATED, ENV_FRESH = range(3)
+
+
+class ServerConfig(Structure):
+    _fields_ = [("memory_budget_bytes", c_uint64), ("max_active", c_int32),
+                ("tick_pairs", c_int64), ("precision", c_int), ("tonemap", c_int),
+                ("variant", c_int32)]
+
+
+class Request(Structure):
+    _fields_ = [("scene_id", c_int32), ("gbuffer", GBuffer), ("width", c_int32),
+                ("height", c_int32), ("out", FrameOut), ("env_source", c_int),
+                ("env_id", c_int32), ("faces", c_void_p), ("faces_location", c_int),
+                ("face_size", c_int32), ("k", c_int32), ("yaw", c_double)]
+
+
+class ServerStats(Structure):
+    _fields_ = [("submitted", c_int64), ("completed", c_int64), ("queued", c_int64),
+                ("active", c_int64), ("ticks", c_int64), ("items", c_int64),
+                ("bytes_in_use", c_uint64), ("peak_bytes", c_uint64), ("peak_active", c_int32)]
+
+
 GBUFFER_FIELDS = {
     "position": (np.float32, 4), "normal": (np.float32, 4), "reflect": (np.float32, 4),
     "roughness": (np.float32, 1), "kd": (np.float32, 4), "ks": (np.float32, 4),
@@ -165,6 +187,12 @@ _SIGNATURES = {
     "nwprt_batch_info": (c_int, [c_void_p, c_int32, POINTER(c_int64), POINTER(c_int64),
                                  POINTER(c_int32)]),
     "nwprt_batch_destroy": (c_int, [c_void_p, c_int32]),
+    "nwprt_server_create": (c_int, [c_void_p, POINTER(ServerConfig), POINTER(c_int32)]),
+    "nwprt_server_submit": (c_int, [c_void_p, c_int32, POINTER(Request), POINTER(c_int64)]),
+    "nwprt_server_poll": (c_int, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32,
+                                  POINTER(c_int32)]),
+    "nwprt_server_get_stats": (c_int, [c_void_p, c_int32, POINTER(ServerStats)]),
+    "nwprt_server_destroy": (c_int, [c_void_p, c_int32]),
 }
 
 _lib = None
@@ -595,6 +623,66 @@ class Context:
 
     def batch_destroy(self, batch: int) -> None:
         _check(lib().nwprt_batch_destroy(self.h, batch))
+
+    # ---- request server: continuous batching -----------------------------
+    def server_create(self, budget_bytes: int = 0, max_active: int = 64, tick_pairs: int = 0,
+                      precision: int = PREC_BF16, tonemap: int = TONEMAP_FULL,
+                      variant: int = -1) -> int:
+        cfg = ServerConfig(budget_bytes, max_active, tick_pairs, precision, tonemap, variant)
+        sid = c_int32()
+        _check(lib().nwprt_server_create(self.h, ctypes.byref(cfg), ctypes.byref(sid)))
+        if not hasattr(self, "_server_keep"):
+            self._server_keep = {}
+        return sid.value
+
+    def server_submit(self, server: int, scene: int, frame: DeviceFrame, *, env: int = None,
+                      faces=None, yaw: Optional[float] = None, k: int = 0,
+                      face_size: int = 0, out: Optional[FrameOut] = None) -> int:
+        """One request: env=<map id> (resident), or faces=<DeviceBuffer> with
+        yaw (rotated), or faces=<DeviceBuffer or host array> (fresh)."""
+        req = Request()
+        req.scene_id = scene
+        req.gbuffer = frame.gbuffer()
+        req.width, req.height = frame.width, frame.height
+        req.out = out if out is not None else frame.frame_out()
+        keep = None
+        if env is not None:
+            req.env_source, req.env_id = ENV_RESIDENT, env
+        else:
+            if isinstance(faces, DeviceBuffer):
+                req.faces, req.faces_location = faces.ptr, DEVICE
+            else:
+                keep = np.ascontiguousarray(faces, np.float32)  # read at admission
+                req.faces, req.faces_location = keep.ctypes.data, HOST
+            req.env_source = ENV_ROTATED if yaw is not None else ENV_FRESH
+            req.yaw = float(yaw or 0.0)
+            req.k, req.face_size = k, face_size
+        rid = c_int64()
+        _check(lib().nwprt_server_submit(self.h, server, ctypes.byref(req), ctypes.byref(rid)))
+        if keep is not None:
+            self._server_keep[(server, rid.value)] = keep
+        return rid.value
+
+    def server_poll(self, server: int, wait: int = 0, max_done: int = 4096):
+        ids = np.zeros(max_done, np.int64)
+        lat = np.zeros(max_done, np.float64)
+        n = c_int32()
+        _check(lib().nwprt_server_poll(self.h, server, wait, ids.ctypes.data, lat.ctypes.data,
+                                       max_done, ctypes.byref(n)))
+        done = [(int(ids[i]), float(lat[i])) for i in range(n.value)]
+        for rid, _ in done:
+            self._server_keep.pop((server, rid), None)
+        return done
+
+    def server_stats(self, server: int) -> Dict[str, int]:
+        st = ServerStats()
+        _check(lib().nwprt_server_get_stats(self.h, server, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in ServerStats._fields_}
+
+    def server_destroy(self, server: int) -> None:
+        _check(lib().nwprt_server_destroy(self.h, server))
+        for key in [k for k in getattr(self, "_server_keep", {}) if k[0] == server]:
+            self._server_keep.pop(key)
 
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -27,6 +27,7 @@ SOURCES = [
     "model_folder.cpp",
     "host_abi.cpp",
     "engine.cpp",
+    "server.cpp",
     "envmap_kernels.hip",
     "encode_kernels.hip",
     "relight_kernels.hip",

@@ -13,161 +13,10 @@
 #include <map>
 #include <tuple>
 
+#include "engine_state.hpp"
 #include "kernels.hpp"
 
 namespace nwprt {
-namespace {
-
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void ensure(size_t n) {
-    if (n <= bytes) return;
-    release();
-    const hipError_t e = hipMalloc(&p, n);
-    if (e == hipErrorOutOfMemory) {
-      (void)hipGetLastError();
-      fail(NWPRT_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
-    }
-    hip_check(e, "hipMalloc");
-    bytes = n;
-  }
-  void upload(const void *src, size_t n) {
-    ensure(n);
-    hip_check(hipMemcpy(p, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D");
-  }
-  template <class T>
-  T *as() const { return (T *)p; }
-};
-
-struct Scene {
-  nwprt_scene_desc desc{};
-  nwprt_hashgrid_levels lv{};
-  DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16, aux_bf16, aux_f16;
-  // FP8 form: E4M3 tiles of layers 2..4 and their power-of-two exponents.
-  DevBuf tiles_fp8;
-  int fp8_exp[3] = {0, 0, 0};
-  // FP32 aux columns of the second layer ([64][16]) inside w_f32, or null.
-  const float *waux_f32() const {
-    return desc.aux_dims ? w_f32.as<float>() + (scene_layer_weight_count(desc) - 64 * 16) : nullptr;
-  }
-};
-
-struct EnvMap {
-  int n = 0, k = 0;
-  bool has_coef = false;
-  DevBuf idx, strength, coef;
-  // Faces-built maps: top-K work buffer and, once rotated, the rotated faces
-  // (per map, so updates of different maps on different streams never share
-  // scratch memory).
-  DevBuf scores, rotated;
-  hipEvent_t ready = nullptr;
-  ~EnvMap() {
-    if (ready) (void)hipEventDestroy(ready);
-  }
-};
-
-// Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
-// 2: nwprt_relight / debug taps).
-struct FrameSlot {
-  DevBuf features, aux, mk, wsel;
-  // FP8 form: first-layer 32x32 tiles, calibration subsample, activation maxima.
-  DevBuf mk32, calib, amax;
-  bool valid = false;
-  int scene = -1, n_pixels = 0;
-  Elem elem = Elem::F32;
-};
-
-// A captured batch of relight jobs (nwprt_batch_*): concatenated per-job
-// features and selection tiles, the device job table and work-item list,
-// and the graph that encodes every job and runs the persistent kernel.
-struct Batch {
-  nwprt_precision prec = NWPRT_PREC_BF16;
-  nwprt_tonemap_source tm = NWPRT_TONEMAP_FULL;
-  bool softplus = true;
-  std::vector<nwprt_batch_job> jobs;
-  std::vector<int> k, p;
-  std::vector<size_t> feat_off, mk_off;  // in elements of 2 bytes
-  // What the graph captured of each job's scene and map: a launch checks
-  // that these buffers are still the ones in use.
-  std::vector<const void *> tiles_at, idx_at;
-  DevBuf features, mk, args, items, meta;  // meta: int n_items | uint counter
-  long long n_items = 0, n_pairs = 0;
-  int variant = 5;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-  }
-  ~Batch() { drop_graph(); }
-};
-
-}  // namespace
-}  // namespace nwprt
-
-struct nwprt_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::map<int, std::unique_ptr<nwprt::Scene>> scenes;
-  std::map<int, std::unique_ptr<nwprt::EnvMap>> envs;
-  int next_id = 1;
-  nwprt::FrameSlot slots[3];
-  nwprt::DevBuf staging;
-  // Tile tuning: forced variant (-1: none) and the autotuner's choices keyed
-  // by (element type, K rounded up to a power of two, ceil(log2 pixels)).
-  int forced_variant = -1;
-  std::map<std::tuple<int, int, int>, int> tuned;
-  std::map<int, std::unique_ptr<nwprt::Batch>> batches;
-  int n_cu = 0;  // compute units of the device (persistent grid size)
-};
-
-namespace nwprt {
-namespace {
-
-const char *status_name(nwprt_status s) {
-  switch (s) {
-    case NWPRT_OK: return "ok";
-    case NWPRT_ERR_INVALID_ARGUMENT: return "invalid argument";
-    case NWPRT_ERR_IO: return "i/o error";
-    case NWPRT_ERR_FORMAT: return "format error";
-    case NWPRT_ERR_DEVICE: return "device error";
-    case NWPRT_ERR_OUT_OF_MEMORY: return "out of memory";
-    case NWPRT_ERR_UNSUPPORTED: return "unsupported";
-    case NWPRT_ERR_STATE: return "invalid call order";
-    case NWPRT_ERR_NOT_FOUND: return "unknown handle";
-  }
-  return "unknown status";
-}
-
-template <class F>
-nwprt_status guarded(F &&f) {
-  try {
-    f();
-    set_last_error("");
-    return NWPRT_OK;
-  } catch (const Error &e) {
-    set_last_error(e.what());
-    return e.status;
-  } catch (const std::bad_alloc &) {
-    set_last_error("host allocation failed");
-    return NWPRT_ERR_OUT_OF_MEMORY;
-  } catch (const std::exception &e) {
-    set_last_error(e.what());
-    return NWPRT_ERR_INVALID_ARGUMENT;
-  }
-}
 
 void bind(nwprt_ctx ctx) {
   require(ctx != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null context");
@@ -208,9 +57,6 @@ Elem elem_of(nwprt_precision p) {
   fail(NWPRT_ERR_INVALID_ARGUMENT, "unknown precision");
 }
 
-// Tuning-table key of a precision: the element type, 3 for the FP8 form.
-int tune_class(nwprt_precision p) { return p == NWPRT_PREC_FP8 ? 3 : (int)elem_of(p); }
-
 void check_frame(const nwprt_gbuffer *gb, int w, int h, bool need_direct) {
   require(gb != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null G-buffer");
   require(w > 0 && h > 0 && (long long)w * h <= (1ll << 28), NWPRT_ERR_INVALID_ARGUMENT,
@@ -222,6 +68,53 @@ void check_frame(const nwprt_gbuffer *gb, int w, int h, bool need_direct) {
     check_aligned(gb->direct, 16, "G-buffer direct");
   }
 }
+
+void check_outputs(const nwprt_frame_out &out) {
+  if (out.indirect) check_aligned(out.indirect, 16, "indirect output");
+  if (out.indirect_with_direct) check_aligned(out.indirect_with_direct, 16, "with-direct output");
+  if (out.rgba8) check_aligned(out.rgba8, 4, "rgba8 output");
+}
+
+// Hash-grid encode of P pixels of `gb` into `features` ([P][32] of elem).
+EncodeArgs encode_args(const Scene &sc, const nwprt_gbuffer *gb, int P, Elem elem,
+                       void *features) {
+  EncodeArgs a{};
+  a.n_pixels = P;
+  a.position = gb->position;
+  a.env_flag = gb->env_flag;
+  a.hash = sc.hash.p;
+  a.hash_f16 = sc.desc.hash_dtype == NWPRT_DTYPE_F16;
+  a.n_features = sc.desc.grid.n_features_per_level;
+  a.lv = sc.lv;
+  a.projection = sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
+  a.features = features;
+  a.out_elem = elem;
+  return a;
+}
+
+const void *scene_tiles(const Scene &sc, nwprt_precision prec) {
+  return prec == NWPRT_PREC_BF16 ? sc.tiles_bf16.p : sc.tiles_f16.p;
+}
+
+namespace {
+
+const char *status_name(nwprt_status s) {
+  switch (s) {
+    case NWPRT_OK: return "ok";
+    case NWPRT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case NWPRT_ERR_IO: return "i/o error";
+    case NWPRT_ERR_FORMAT: return "format error";
+    case NWPRT_ERR_DEVICE: return "device error";
+    case NWPRT_ERR_OUT_OF_MEMORY: return "out of memory";
+    case NWPRT_ERR_UNSUPPORTED: return "unsupported";
+    case NWPRT_ERR_STATE: return "invalid call order";
+    case NWPRT_ERR_NOT_FOUND: return "unknown handle";
+  }
+  return "unknown status";
+}
+
+// Tuning-table key of a precision: the element type, 3 for the FP8 form.
+int tune_class(nwprt_precision p) { return p == NWPRT_PREC_FP8 ? 3 : (int)elem_of(p); }
 
 void create_scene(nwprt_ctx ctx, const nwprt_scene_desc &d, const nwprt_scene_params &prm,
                   int *scene_id) {
@@ -354,23 +247,6 @@ int variant_count(nwprt_precision prec) {
   return prec == NWPRT_PREC_FP8 ? relight_fp8_variant_count() : relight_mfma_variant_count();
 }
 
-// Hash-grid encode of P pixels of `gb` into `features` ([P][32] of elem).
-EncodeArgs encode_args(const Scene &sc, const nwprt_gbuffer *gb, int P, Elem elem,
-                       void *features) {
-  EncodeArgs a{};
-  a.n_pixels = P;
-  a.position = gb->position;
-  a.env_flag = gb->env_flag;
-  a.hash = sc.hash.p;
-  a.hash_f16 = sc.desc.hash_dtype == NWPRT_DTYPE_F16;
-  a.n_features = sc.desc.grid.n_features_per_level;
-  a.lv = sc.lv;
-  a.projection = sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
-  a.features = features;
-  a.out_elem = elem;
-  return a;
-}
-
 void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const nwprt_gbuffer *gb,
                  int w, int h, Elem elem, hipStream_t s, uint32_t *dbg_idx = nullptr,
                  float *dbg_feat = nullptr) {
@@ -421,11 +297,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
               std::to_string(sc.desc.face_size) + ")");
   require(tm == NWPRT_TONEMAP_FULL || tm == NWPRT_TONEMAP_INDIRECT || tm == NWPRT_TONEMAP_NONE,
           NWPRT_ERR_INVALID_ARGUMENT, "unknown tonemap source");
-  if (out) {
-    if (out->indirect) check_aligned(out->indirect, 16, "indirect output");
-    if (out->indirect_with_direct) check_aligned(out->indirect_with_direct, 16, "with-direct output");
-    if (out->rgba8) check_aligned(out->rgba8, 4, "rgba8 output");
-  }
+  if (out) check_outputs(*out);
   hip_check(hipStreamWaitEvent(s, env.ready, 0), "hipStreamWaitEvent");
   const int n_coef = 6 * env.n * env.n;
   RelightArgs a{};
@@ -507,10 +379,6 @@ Batch &get_batch(nwprt_ctx ctx, int id) {
   return *it->second;
 }
 
-const void *scene_tiles(const Scene &sc, nwprt_precision prec) {
-  return prec == NWPRT_PREC_BF16 ? sc.tiles_bf16.p : sc.tiles_f16.p;
-}
-
 // Every job's scene and environment map still exist, with the buffers the
 // graph captured; the launch stream waits for the maps.
 void check_batch_inputs(nwprt_ctx ctx, const Batch &b, hipStream_t s) {
@@ -584,9 +452,7 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     require(env.n == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
             "batch: environment map face size does not match the job's scene");
     check_frame(&jb.gbuffer, jb.width, jb.height, true);
-    if (jb.out.indirect) check_aligned(jb.out.indirect, 16, "indirect output");
-    if (jb.out.indirect_with_direct) check_aligned(jb.out.indirect_with_direct, 16, "with-direct output");
-    if (jb.out.rgba8) check_aligned(jb.out.rgba8, 4, "rgba8 output");
+    check_outputs(jb.out);
     const bool sp = sc.desc.output_softplus != 0;
     if (j == 0) b->softplus = sp;
     require(sp == b->softplus, NWPRT_ERR_UNSUPPORTED,

@@ -1,0 +1,172 @@
+// Host-side state of a libnwprt context, shared by the C ABI (engine.cpp)
+// and the request server (server.cpp): device buffers, resident scenes and
+// environment maps, frame slots, captured batches, and the helpers every
+// entry point uses (error guard, handle lookup, argument checks).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace nwprt {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    release();
+    const hipError_t e = hipMalloc(&p, n);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      fail(NWPRT_ERR_OUT_OF_MEMORY, "device allocation of " + std::to_string(n) + " bytes failed");
+    }
+    hip_check(e, "hipMalloc");
+    bytes = n;
+  }
+  void upload(const void *src, size_t n) {
+    ensure(n);
+    hip_check(hipMemcpy(p, src, n, hipMemcpyHostToDevice), "hipMemcpy H2D");
+  }
+  template <class T>
+  T *as() const { return (T *)p; }
+};
+
+struct Scene {
+  nwprt_scene_desc desc{};
+  nwprt_hashgrid_levels lv{};
+  DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16, aux_bf16, aux_f16;
+  // FP8 form: E4M3 tiles of layers 2..4 and their power-of-two exponents.
+  DevBuf tiles_fp8;
+  int fp8_exp[3] = {0, 0, 0};
+  // FP32 aux columns of the second layer ([64][16]) inside w_f32, or null.
+  const float *waux_f32() const {
+    return desc.aux_dims ? w_f32.as<float>() + (scene_layer_weight_count(desc) - 64 * 16) : nullptr;
+  }
+};
+
+struct EnvMap {
+  int n = 0, k = 0;
+  bool has_coef = false;
+  DevBuf idx, strength, coef;
+  // Faces-built maps: top-K work buffer and, once rotated, the rotated faces
+  // (per map, so updates of different maps on different streams never share
+  // scratch memory).
+  DevBuf scores, rotated;
+  hipEvent_t ready = nullptr;
+  ~EnvMap() {
+    if (ready) (void)hipEventDestroy(ready);
+  }
+};
+
+// Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
+// 2: nwprt_relight / debug taps).
+struct FrameSlot {
+  DevBuf features, aux, mk, wsel;
+  // FP8 form: first-layer 32x32 tiles, calibration subsample, activation maxima.
+  DevBuf mk32, calib, amax;
+  bool valid = false;
+  int scene = -1, n_pixels = 0;
+  Elem elem = Elem::F32;
+};
+
+// A captured batch of relight jobs (nwprt_batch_*): concatenated per-job
+// features and selection tiles, the device job table and work-item list,
+// and the graph that encodes every job and runs the persistent kernel.
+struct Batch {
+  nwprt_precision prec = NWPRT_PREC_BF16;
+  nwprt_tonemap_source tm = NWPRT_TONEMAP_FULL;
+  bool softplus = true;
+  std::vector<nwprt_batch_job> jobs;
+  std::vector<int> k, p;
+  std::vector<size_t> feat_off, mk_off;  // in elements of 2 bytes
+  // What the graph captured of each job's scene and map: a launch checks
+  // that these buffers are still the ones in use.
+  std::vector<const void *> tiles_at, idx_at;
+  DevBuf features, mk, args, items, meta;  // meta: int n_items | uint counter
+  long long n_items = 0, n_pairs = 0;
+  int variant = 5;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~Batch() { drop_graph(); }
+};
+
+// Request server (server.cpp); owned through a deleter defined there.
+struct Server;
+void destroy_server(Server *s);
+using ServerPtr = std::unique_ptr<Server, void (*)(Server *)>;
+
+}  // namespace nwprt
+
+struct nwprt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<int, std::unique_ptr<nwprt::Scene>> scenes;
+  std::map<int, std::unique_ptr<nwprt::EnvMap>> envs;
+  int next_id = 1;
+  nwprt::FrameSlot slots[3];
+  nwprt::DevBuf staging;
+  // Tile tuning: forced variant (-1: none) and the autotuner's choices keyed
+  // by (element type, K rounded up to a power of two, ceil(log2 pixels)).
+  int forced_variant = -1;
+  std::map<std::tuple<int, int, int>, int> tuned;
+  std::map<int, std::unique_ptr<nwprt::Batch>> batches;
+  std::map<int, nwprt::ServerPtr> servers;
+  int n_cu = 0;  // compute units of the device (persistent grid size)
+};
+
+namespace nwprt {
+
+// Runs f, mapping nwprt::Error and other exceptions to a status code and
+// the thread's last-error message (no exception crosses the C ABI).
+template <class F>
+nwprt_status guarded(F &&f) {
+  try {
+    f();
+    set_last_error("");
+    return NWPRT_OK;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.status;
+  } catch (const std::bad_alloc &) {
+    set_last_error("host allocation failed");
+    return NWPRT_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return NWPRT_ERR_INVALID_ARGUMENT;
+  }
+}
+
+void bind(nwprt_ctx ctx);
+hipStream_t pick(nwprt_ctx ctx, void *stream);
+Scene &get_scene(nwprt_ctx ctx, int id);
+EnvMap &get_env(nwprt_ctx ctx, int id);
+void check_aligned(const void *p, size_t a, const char *what);
+void check_frame(const nwprt_gbuffer *gb, int w, int h, bool need_direct);
+void check_outputs(const nwprt_frame_out &out);
+Elem elem_of(nwprt_precision p);
+EncodeArgs encode_args(const Scene &sc, const nwprt_gbuffer *gb, int P, Elem elem,
+                       void *features);
+const void *scene_tiles(const Scene &sc, nwprt_precision prec);
+
+}  // namespace nwprt

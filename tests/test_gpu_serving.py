@@ -1,13 +1,159 @@
 """GPU tests of the serving path: rotated environment maps (bit-exact
-against the oracle's rotation + Haar + top-K)."""
+against the oracle's rotation + Haar + top-K) and the request server
+(continuous batching): every request's outputs are the same alone and in the
+mix, and the same as nwprt_relight of its inputs, bit for bit."""
 import math
 
 import numpy as np
 import pytest
 
+from cases import make_case
 from neural_wavelet_prt_demo_amd import DeviceBuffer
 
 pytestmark = pytest.mark.gpu
+
+N = 8
+# (width, height, scene, env source, K, yaw): uneven sizes, both scenes,
+# resident / rotated / fresh (device and host) maps.
+REQUESTS = [(40, 30, 0, "resident", 17, None), (64, 48, 1, "rotated", 64, 0.7),
+            (1, 1, 0, "fresh_dev", 5, None), (100, 7, 1, "fresh_host", 130, None),
+            (33, 17, 0, "rotated", 300, -2.5), (128, 96, 1, "resident", 32, None),
+            (70, 70, 0, "rotated", 200, 3.9)]
+
+
+class _Setup:
+    def __init__(self, nw, orc, ctx):
+        self.nw, self.ctx = nw, ctx
+        cs = [make_case(nw, orc, width=8, height=8, k=8, face_size=N, seed=200 + s) for s in (0, 1)]
+        self.scenes = [ctx.scene_create(c.desc, c.params) for c in cs]
+        self.base = [nw.synth_cubemap(50 + s, N) for s in (0, 1)]
+        self.dbase = [DeviceBuffer.from_array(b) for b in self.base]
+        self.reqs = []
+        for i, (w, h, s, src, k, yaw) in enumerate(REQUESTS):
+            gb = nw.synth_gbuffer(300 + i, w, h, 0.15)
+            fresh = nw.synth_cubemap(400 + i, N)
+            r = {"w": w, "h": h, "scene": self.scenes[s], "src": src, "k": k, "yaw": yaw,
+                 "gbuf": gb, "fresh": fresh, "dfresh": DeviceBuffer.from_array(fresh),
+                 "env": ctx.envmap_from_faces(self.base[s], k) if src == "resident" else None,
+                 "s": s}
+            self.reqs.append(r)
+
+    def frame(self, r):
+        return self.nw.DeviceFrame(r["w"], r["h"], r["gbuf"])
+
+    def submit(self, server, r, frame):
+        ctx = self.ctx
+        if r["src"] == "resident":
+            return ctx.server_submit(server, r["scene"], frame, env=r["env"])
+        if r["src"] == "rotated":
+            return ctx.server_submit(server, r["scene"], frame, faces=self.dbase[r["s"]],
+                                     yaw=r["yaw"], k=r["k"], face_size=N)
+        faces = r["dfresh"] if r["src"] == "fresh_dev" else r["fresh"]
+        return ctx.server_submit(server, r["scene"], frame, faces=faces, k=r["k"], face_size=N)
+
+    def reference(self, r, variant):
+        """nwprt_relight of the same inputs with the same tile variant."""
+        ctx = self.ctx
+        if r["src"] == "resident":
+            env = r["env"]
+        elif r["src"] == "rotated":
+            env = ctx.envmap_from_faces(self.base[r["s"]], r["k"])
+            ctx.envmap_update_rotated(env, self.dbase[r["s"]], r["yaw"])
+        else:
+            env = ctx.envmap_from_faces(r["fresh"], r["k"])
+        f = self.frame(r)
+        ctx.set_relight_variant(variant)
+        ctx.relight(r["scene"], env, f, self.nw.PREC_BF16)
+        ctx.sync()
+        ctx.set_relight_variant(-1)
+        return f.results()
+
+
+def _same(a, b, what):
+    for key in ("indirect", "indirect_with_direct", "rgba8"):
+        assert np.array_equal(a[key], b[key]), (what, key)
+
+
+def test_server_alone_equals_mix(nw, orc, ctx):
+    st = _Setup(nw, orc, ctx)
+    variant = 5
+    refs = [st.reference(r, variant) for r in st.reqs]
+    # Alone: one request per server lifetime.
+    for i, r in enumerate(st.reqs):
+        srv = ctx.server_create(max_active=4, variant=variant)
+        f = st.frame(r)
+        rid = st.submit(srv, r, f)
+        done = ctx.server_poll(srv, wait=2)
+        assert [d[0] for d in done] == [rid] and done[0][1] > 0
+        _same(f.results(), refs[i], ("alone", i))
+        ctx.server_destroy(srv)
+    # Mix: all at once, small ticks so every tick holds items of several
+    # requests and requests span many ticks.
+    srv = ctx.server_create(max_active=16, tick_pairs=64 * 300 * 3, variant=variant)
+    frames = [st.frame(r) for r in st.reqs]
+    ids = [st.submit(srv, r, f) for r, f in zip(st.reqs, frames)]
+    done = ctx.server_poll(srv, wait=2)
+    assert sorted(d[0] for d in done) == sorted(ids)
+    stats = ctx.server_stats(srv)
+    print("mix stats:", stats)
+    assert stats["completed"] == len(ids) and stats["queued"] == 0 and stats["active"] == 0
+    assert stats["ticks"] > len(ids) and stats["peak_active"] > 1 and stats["bytes_in_use"] == 0
+    for i, f in enumerate(frames):
+        _same(f.results(), refs[i], ("mix", i))
+    ctx.server_destroy(srv)
+
+
+def test_server_memory_budget_admission(nw, orc, ctx):
+    """A budget that fits one request's state at a time: requests are
+    admitted one by one as earlier ones complete, and results are unchanged."""
+    st = _Setup(nw, orc, ctx)
+    variant = 5
+    big = max(r["w"] * r["h"] * 64 + r["k"] * 4096 + 6 * N * N * 40 + r["k"] * 16 for r in st.reqs)
+    srv = ctx.server_create(budget_bytes=big, max_active=8, tick_pairs=64 * 300 * 2,
+                            variant=variant)
+    frames = [st.frame(r) for r in st.reqs]
+    ids = []
+    for r, f in zip(st.reqs, frames):
+        ids.append(st.submit(srv, r, f))
+    got = []
+    while len(got) < len(ids):
+        got += ctx.server_poll(srv, wait=1)
+    assert [g[0] for g in got] == ids  # arrival order
+    stats = ctx.server_stats(srv)
+    print("budget stats:", stats, "budget", big)
+    assert stats["peak_bytes"] <= big and stats["peak_active"] < len(ids)
+    for i, f in enumerate(frames):
+        _same(f.results(), st.reference(st.reqs[i], variant), ("budget", i))
+    ctx.server_destroy(srv)
+
+
+def test_server_errors(nw, orc, ctx):
+    st = _Setup(nw, orc, ctx)
+    r = st.reqs[1]
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.server_create(precision=nw.PREC_FP32)
+    assert e.value.status == nw.ERR_UNSUPPORTED
+    srv = ctx.server_create(budget_bytes=1 << 16, max_active=2)
+    with pytest.raises(nw.NwprtError) as e:  # needs more than the whole budget
+        st.submit(srv, r, st.frame(r))
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    ctx.server_destroy(srv)
+    srv = ctx.server_create(max_active=2)
+    aux = make_case(nw, orc, width=8, height=8, k=8, face_size=N, aux=True, seed=3)
+    aux_scene = ctx.scene_create(aux.desc, aux.params)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.server_submit(srv, aux_scene, st.frame(r), faces=st.dbase[0], yaw=0.1, k=8,
+                          face_size=N)
+    assert e.value.status == nw.ERR_UNSUPPORTED
+    with pytest.raises(nw.NwprtError) as e:  # face size of another scene
+        ctx.server_submit(srv, r["scene"], st.frame(r), faces=st.dbase[0], yaw=0.1, k=8,
+                          face_size=16)
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    assert ctx.server_poll(srv, wait=2) == []
+    ctx.server_destroy(srv)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.server_poll(srv)
+    assert e.value.status == nw.ERR_NOT_FOUND
 
 
 @pytest.mark.parametrize("n,k,yaw", [(8, 40, 0.3), (64, 300, -2.1), (128, 300, math.pi / 2),
