@@ -206,9 +206,9 @@ def test_error_paths(nw, orc, ctx):
     with pytest.raises(nw.NwprtError) as e:
         ctx.relight(scene + 1000, env, frame)
     assert e.value.status == nw.ERR_NOT_FOUND
-    with pytest.raises(nw.NwprtError) as e:
-        ctx.relight(scene, env, frame, nw.PREC_FP8)
-    assert e.value.status == nw.ERR_UNSUPPORTED
+    with pytest.raises(nw.NwprtError) as e:  # no such precision
+        ctx.relight(scene, env, frame, 7)
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
     with pytest.raises(nw.NwprtError) as e:
         ctx.envmap_from_faces(case.faces, 6 * 8 * 8 + 1)
     assert e.value.status == nw.ERR_INVALID_ARGUMENT
