@@ -50,6 +50,91 @@ __device__ __forceinline__ void store_row(void *base, int p, const float *v) {
   }
 }
 
+// Trilinear hash-grid features of level l at position x, added to acc[0..F).
+// Position pos = fmaf(x, scale_l, 0.5f) in FP32; corner c takes bit 0 / 1 / 2
+// for +x / +y / +z; dense levels index x + y res + z res^2 (mod size), hashed
+// ones grid_hash & (size - 1).  idx (may be null) receives the 8 indices.
+template <int F, bool HF16>
+__device__ __forceinline__ void accumulate_level(const EncodeArgs &a, int l, const float (&x)[3],
+                                                 float *acc, uint32_t *idx_out) {
+  const float scale = a.lv.scale[l];
+  const uint32_t res = a.lv.resolution[l], size = a.lv.size[l];
+  const bool dense = a.lv.dense[l] != 0;
+  const uint32_t off = a.lv.offset[l];
+  uint32_t gi[3];
+  float fr[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float pos = fmaf(x[d], scale, 0.5f);
+    const float fl = floorf(pos);
+    gi[d] = (uint32_t)(int)fl;
+    fr[d] = pos - fl;
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t cx = gi[0] + (c & 1), cy = gi[1] + ((c >> 1) & 1), cz = gi[2] + ((c >> 2) & 1);
+    uint32_t idx;
+    if (dense) {
+      idx = cx + cy * res + cz * res * res;
+      if (idx >= size) idx %= size;
+    } else {
+      idx = grid_hash(cx, cy, cz) & (size - 1u);
+    }
+    const float w = ((c & 1) ? fr[0] : 1.0f - fr[0]) * ((c & 2) ? fr[1] : 1.0f - fr[1]) *
+                    ((c & 4) ? fr[2] : 1.0f - fr[2]);
+    const size_t e = ((size_t)off + idx) * F;
+    if (HF16) {
+      const uint16_t *h = (const uint16_t *)a.hash + e;
+      if (F == 2) {
+        const uint32_t two = *(const uint32_t *)h;
+        acc[0] = fmaf(w, dev::f16_to_f32((uint16_t)(two & 0xffffu)), acc[0]);
+        acc[1] = fmaf(w, dev::f16_to_f32((uint16_t)(two >> 16)), acc[1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < F; ++j) acc[j] = fmaf(w, dev::f16_to_f32(h[j]), acc[j]);
+      }
+    } else {
+      const float *h = (const float *)a.hash + e;
+      if (F == 2) {
+        const float2 two = *(const float2 *)h;
+        acc[0] = fmaf(w, two.x, acc[0]);
+        acc[1] = fmaf(w, two.y, acc[1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < F; ++j) acc[j] = fmaf(w, h[j], acc[j]);
+      }
+    }
+    if (idx_out) idx_out[c] = idx;
+  }
+}
+
+// Planar form (16 levels x 2 features): workgroup b encodes levels q and
+// 15 - q (q = b % 8) of pixels [256 (b / 8), +256) and writes their four
+// raw features to plane q.  Workgroups are dealt to the 8 XCDs round-robin,
+// so every workgroup of plane q runs on XCD q and that XCD's L2 only ever
+// holds these two levels' entries (at most 2 x 2^19 x 2 x 2 B = 4 MiB for
+// FP16 hashed levels).  Pairing a fine level with a coarse one evens the
+// work per plane.
+template <bool HF16, Elem OUT>
+__global__ __launch_bounds__(kEncodeBlock) void encode_planar_kernel(EncodeArgs a) {
+  const int q = blockIdx.x & 7;
+  const int p = (blockIdx.x >> 3) * kEncodeBlock + threadIdx.x;
+  if (p >= a.n_pixels) return;
+  const bool hit = a.env_flag == nullptr || a.env_flag[p] != 0;
+  float v[4] = {0.f, 0.f, 0.f, 0.f};
+  if (hit) {
+    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)p);
+    const float x[3] = {x4.x, x4.y, x4.z};
+    accumulate_level<2, HF16>(a, q, x, v, nullptr);
+    accumulate_level<2, HF16>(a, 15 - q, x, v + 2, nullptr);
+  }
+  const size_t at = (size_t)q * a.n_pixels + p;
+  if (OUT == Elem::BF16)
+    ((uint2 *)a.features)[at] = make_uint2(dev::pack_bf16(v[0], v[1]), dev::pack_bf16(v[2], v[3]));
+  else
+    ((uint2 *)a.features)[at] = make_uint2(dev::pack_f16(v[0], v[1]), dev::pack_f16(v[2], v[3]));
+}
+
 template <int F, bool HF16, Elem OUT, bool DBG>
 __global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
   constexpr int L = 32 / F;
@@ -71,58 +156,9 @@ __global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
     const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)p);
     const float x[3] = {x4.x, x4.y, x4.z};
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const float scale = a.lv.scale[l];
-      const uint32_t res = a.lv.resolution[l], size = a.lv.size[l];
-      const bool dense = a.lv.dense[l] != 0;
-      const uint32_t off = a.lv.offset[l];
-      uint32_t gi[3];
-      float fr[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const float pos = fmaf(x[d], scale, 0.5f);
-        const float fl = floorf(pos);
-        gi[d] = (uint32_t)(int)fl;
-        fr[d] = pos - fl;
-      }
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const uint32_t cx = gi[0] + (c & 1), cy = gi[1] + ((c >> 1) & 1),
-                       cz = gi[2] + ((c >> 2) & 1);
-        uint32_t idx;
-        if (dense) {
-          idx = cx + cy * res + cz * res * res;
-          if (idx >= size) idx %= size;
-        } else {
-          idx = grid_hash(cx, cy, cz) & (size - 1u);
-        }
-        const float w = ((c & 1) ? fr[0] : 1.0f - fr[0]) * ((c & 2) ? fr[1] : 1.0f - fr[1]) *
-                        ((c & 4) ? fr[2] : 1.0f - fr[2]);
-        const size_t e = ((size_t)off + idx) * F;
-        if (HF16) {
-          const uint16_t *h = (const uint16_t *)a.hash + e;
-          if (F == 2) {
-            const uint32_t two = *(const uint32_t *)h;
-            feat[l * F + 0] = fmaf(w, dev::f16_to_f32((uint16_t)(two & 0xffffu)), feat[l * F + 0]);
-            feat[l * F + 1] = fmaf(w, dev::f16_to_f32((uint16_t)(two >> 16)), feat[l * F + 1]);
-          } else {
-#pragma unroll
-            for (int j = 0; j < F; ++j) feat[l * F + j] = fmaf(w, dev::f16_to_f32(h[j]), feat[l * F + j]);
-          }
-        } else {
-          const float *h = (const float *)a.hash + e;
-          if (F == 2) {
-            const float2 two = *(const float2 *)h;
-            feat[l * F + 0] = fmaf(w, two.x, feat[l * F + 0]);
-            feat[l * F + 1] = fmaf(w, two.y, feat[l * F + 1]);
-          } else {
-#pragma unroll
-            for (int j = 0; j < F; ++j) feat[l * F + j] = fmaf(w, h[j], feat[l * F + j]);
-          }
-        }
-        if (DBG) a.dbg_indices[((size_t)p * L + l) * 8 + c] = idx;
-      }
-    }
+    for (int l = 0; l < L; ++l)
+      accumulate_level<F, HF16>(a, l, x, feat + l * F,
+                                DBG ? a.dbg_indices + ((size_t)p * L + l) * 8 : nullptr);
     if (DBG && a.dbg_features) {
 #pragma unroll
       for (int i = 0; i < 32; ++i) a.dbg_features[(size_t)p * 32 + i] = feat[i];
@@ -206,6 +242,27 @@ void launch_encode(const EncodeArgs &a, hipStream_t s) {
     require(a.lv.dense[l] || (a.lv.size[l] & (a.lv.size[l] - 1)) == 0, NWPRT_ERR_INVALID_ARGUMENT,
             "encode: hashed level size must be a power of two");
   const dim3 grid((a.n_pixels + kEncodeBlock - 1) / kEncodeBlock);
+  if (a.planar) {
+    require(a.n_features == 2 && a.aux == nullptr && a.dbg_indices == nullptr &&
+                (a.out_elem == Elem::BF16 || a.out_elem == Elem::F16),
+            NWPRT_ERR_INVALID_ARGUMENT,
+            "encode: the planar form is for 16 x 2 grids, BF16/FP16, no aux or tap");
+    require((long long)grid.x * 8 < (1ll << 31), NWPRT_ERR_INVALID_ARGUMENT, "encode: too many pixels");
+    const dim3 g8(grid.x * 8);
+    if (a.hash_f16) {
+      if (a.out_elem == Elem::BF16)
+        hipLaunchKernelGGL((encode_planar_kernel<true, Elem::BF16>), g8, dim3(kEncodeBlock), 0, s, a);
+      else
+        hipLaunchKernelGGL((encode_planar_kernel<true, Elem::F16>), g8, dim3(kEncodeBlock), 0, s, a);
+    } else {
+      if (a.out_elem == Elem::BF16)
+        hipLaunchKernelGGL((encode_planar_kernel<false, Elem::BF16>), g8, dim3(kEncodeBlock), 0, s, a);
+      else
+        hipLaunchKernelGGL((encode_planar_kernel<false, Elem::F16>), g8, dim3(kEncodeBlock), 0, s, a);
+    }
+    hip_check(hipGetLastError(), "encode_planar_kernel launch");
+    return;
+  }
   switch (a.n_features) {
     case 1: dispatch_f<1>(a, grid, s); break;
     case 2: dispatch_f<2>(a, grid, s); break;

@@ -89,11 +89,21 @@ EncodeArgs encode_args(const Scene &sc, const nwprt_gbuffer *gb, int P, Elem ele
   a.projection = sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
   a.features = features;
   a.out_elem = elem;
+  a.planar = planar_features(sc, elem);
   return a;
 }
 
 const void *scene_tiles(const Scene &sc, nwprt_precision prec) {
   return prec == NWPRT_PREC_BF16 ? sc.tiles_bf16.p : sc.tiles_f16.p;
+}
+
+bool planar_features(const Scene &sc, Elem elem) {
+  return elem != Elem::F32 && sc.desc.aux_dims == 0 && sc.desc.grid.n_levels == 16 &&
+         sc.desc.grid.n_features_per_level == 2;
+}
+
+const float *scene_projection(const Scene &sc) {
+  return sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
 }
 
 namespace {
@@ -300,10 +310,12 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   if (out) check_outputs(*out);
   hip_check(hipStreamWaitEvent(s, env.ready, 0), "hipStreamWaitEvent");
   const int n_coef = 6 * env.n * env.n;
+  const bool planar = planar_features(sc, elem);
   RelightArgs a{};
   a.n_pixels = P;
   a.k = env.k;
   a.features = slot.features.p;
+  a.features_planar = planar;
   a.light = env.strength.as<float>();
   a.env_flag = gb->env_flag;
   a.direct = gb->direct;
@@ -336,7 +348,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
     slot.mk32.ensure((size_t)env.k * 4 * 64 * 16);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
                              env.idx.as<int32_t>(), env.k, Elem::BF16, slot.mk.as<uint16_t>(),
-                             nullptr, s, slot.mk32.as<uint16_t>());
+                             nullptr, s, slot.mk32.as<uint16_t>(), planar, scene_projection(sc));
     // Per-layer activation maxima over about 8K pixels spread over the frame.
     constexpr int kCalibPixels = 8192;
     const int stride = std::max(1, (P + kCalibPixels - 1) / kCalibPixels);
@@ -344,11 +356,12 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
     slot.calib.ensure((size_t)n_cal * 64);
     slot.amax.ensure(16);
     launch_calib_gather(slot.features.as<uint16_t>(), gb->env_flag, P, stride,
-                        slot.calib.as<uint16_t>(), n_cal, slot.amax.as<float>(), s);
+                        slot.calib.as<uint16_t>(), n_cal, slot.amax.as<float>(), planar, s);
     RelightArgs c{};
     c.n_pixels = n_cal;
     c.k = env.k;
     c.features = slot.calib.p;
+    c.features_planar = planar;
     c.mk = slot.mk.as<uint16_t>();
     c.w_tiles = sc.tiles_bf16.as<uint16_t>();
     c.amax = slot.amax.as<float>();
@@ -362,7 +375,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
     slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
                              env.idx.as<int32_t>(), env.k, elem, slot.mk.as<uint16_t>(), nullptr,
-                             s);
+                             s, nullptr, planar, scene_projection(sc));
     a.mk = slot.mk.as<uint16_t>();
     a.w_tiles = elem == Elem::BF16 ? sc.tiles_bf16.as<uint16_t>() : sc.tiles_f16.as<uint16_t>();
     launch_relight_mfma(a, elem, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
@@ -413,7 +426,8 @@ void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
                     s);
       launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * env.n * env.n,
                                env.idx.as<int32_t>(), b.k[j], elem,
-                               b.mk.as<uint16_t>() + b.mk_off[j], nullptr, s);
+                               b.mk.as<uint16_t>() + b.mk_off[j], nullptr, s, nullptr,
+                               planar_features(sc, elem), scene_projection(sc));
     }
     unsigned int *counter = (unsigned int *)(b.meta.as<int>() + 1);
     launch_reset_counter(counter, s);
@@ -484,6 +498,7 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     a.n_pixels = b->p[j];
     a.k = b->k[j];
     a.features = b->features.as<uint16_t>() + b->feat_off[j];
+    a.features_planar = planar_features(sc, elem_of(prec));
     a.mk = b->mk.as<uint16_t>() + b->mk_off[j];
     a.light = env.strength.as<float>();
     a.w_tiles = (const uint16_t *)scene_tiles(sc, prec);

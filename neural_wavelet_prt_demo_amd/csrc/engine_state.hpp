@@ -168,5 +168,11 @@ Elem elem_of(nwprt_precision p);
 EncodeArgs encode_args(const Scene &sc, const nwprt_gbuffer *gb, int P, Elem elem,
                        void *features);
 const void *scene_tiles(const Scene &sc, nwprt_precision prec);
+// Whether a scene's per-pixel features take the planar layout for an MFMA
+// element type (16 x 2 grids without aux inputs; the projection then moves
+// into the selection tiles).
+bool planar_features(const Scene &sc, Elem elem);
+// Projection to fold into the selection tiles (null: none).
+const float *scene_projection(const Scene &sc);
 
 }  // namespace nwprt

@@ -233,21 +233,36 @@ __global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restric
 __global__ __launch_bounds__(256) void prepare_selection_kernel(
     const float *__restrict__ w1, const float *__restrict__ wfeat, int n_coef,
     const int32_t *__restrict__ indices, int elem, uint16_t *__restrict__ mk,
-    float *__restrict__ wsel, uint16_t *__restrict__ mk32) {
+    float *__restrict__ wsel, uint16_t *__restrict__ mk32, int planar,
+    const float *__restrict__ proj) {
+  __shared__ float wk[32], pj[32 * 32];
   const int k = blockIdx.x;
   const int t = threadIdx.x;
   const int c = indices[k];
+  if (t < 32) wk[t] = wfeat[(size_t)t * n_coef + c];
+  if (planar && proj)
+    for (int i = t; i < 32 * 32; i += 256) pj[i] = proj[i];
+  __syncthreads();
+  // First-layer weight of output row m at contraction slot j: W1[m][j] w[j]
+  // for row-layout features; for planar ones the raw feature planar_feature(j)
+  // with the projection folded in, sum_i W1[m][i] w[i] P[i][f].
+  auto weight = [&](int m, int j) -> float {
+    if (!planar) return w1[m * 32 + j] * wk[j];
+    const int f = planar_feature(j);
+    if (!proj) return w1[m * 32 + f] * wk[f];
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int i = 0; i < 32; ++i) acc = fmaf(w1[m * 32 + i] * wk[i], pj[i * 32 + f], acc);
+    return acc;
+  };
   if (mk32) {
     // 32x32x16 A operand: block b = t / 128 (rows 32 b + lane % 32), K step
-    // s = (t / 64) % 2, lane holds features 16 s + 8 (lane / 32) + [0, 8).
+    // s = (t / 64) % 2, lane holds slots 16 s + 8 (lane / 32) + [0, 8).
     const int b = t >> 7, s = (t >> 6) & 1, lane = t & 63;
     const int m = 32 * b + (lane & 31);
     float v[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int f = 16 * s + 8 * (lane >> 5) + j;
-      v[j] = w1[m * 32 + f] * wfeat[(size_t)f * n_coef + c];
-    }
+    for (int j = 0; j < 8; ++j) v[j] = weight(m, 16 * s + 8 * (lane >> 5) + j);
     dev::u32x4 out;
 #pragma unroll
     for (int d = 0; d < 4; ++d) out[d] = dev::pack_bf16(v[2 * d], v[2 * d + 1]);
@@ -258,10 +273,7 @@ __global__ __launch_bounds__(256) void prepare_selection_kernel(
     const int m = 16 * b + (lane & 15);
     float v[8];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const int f = 8 * g + s;
-      v[s] = w1[m * 32 + f] * wfeat[(size_t)f * n_coef + c];
-    }
+    for (int s = 0; s < 8; ++s) v[s] = weight(m, 8 * g + s);
     dev::u32x4 out;
 #pragma unroll
     for (int d = 0; d < 4; ++d)
@@ -269,7 +281,7 @@ __global__ __launch_bounds__(256) void prepare_selection_kernel(
                                        : dev::pack_f16(v[2 * d], v[2 * d + 1]);
     *(dev::u32x4 *)(mk + (((size_t)k * 4 + b) * 64 + lane) * 8) = out;
   }
-  if (wsel && t < 32) wsel[(size_t)k * 32 + t] = wfeat[(size_t)t * n_coef + c];
+  if (wsel && t < 32) wsel[(size_t)k * 32 + t] = wk[t];
 }
 
 // Yaw rotation of a cubemap about +y: output texel (f, r, c) takes the
@@ -362,12 +374,14 @@ void launch_topk(const float *coef, int n, int k, uint32_t *scores, int32_t *ind
 
 void launch_prepare_selection(const float *w1, const float *wfeat, int n_coef,
                               const int32_t *indices, int k, Elem mk_elem, uint16_t *mk,
-                              float *wsel, hipStream_t s, uint16_t *mk32) {
+                              float *wsel, hipStream_t s, uint16_t *mk32, bool planar,
+                              const float *projection) {
   require(k >= 1 && k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT, "prepare: bad K");
   require(mk == nullptr || mk_elem == Elem::BF16 || mk_elem == Elem::F16,
           NWPRT_ERR_INVALID_ARGUMENT, "prepare: MFMA tiles must be BF16 or FP16");
   hipLaunchKernelGGL(prepare_selection_kernel, dim3(k), dim3(256), 0, s, w1, wfeat, n_coef,
-                     indices, (int)mk_elem, mk, wsel, mk32);
+                     indices, (int)mk_elem, mk, wsel, mk32, (int)planar,
+                     planar ? projection : nullptr);
   hip_check(hipGetLastError(), "prepare_selection_kernel launch");
 }
 

@@ -38,10 +38,14 @@ void launch_rotate_cubemap(const float *src, float *dst, int n, double yaw, hipS
 //  wsel (may be null): [K][32] FP32 selected wavelet features
 //  mk32 (may be null): [K][2][2][64][8] BF16 32x32x16 A-operand tiles of the
 //                      same product, for the FP8 form's first layer.
+// planar: the MFMA tiles are for planar features -- contraction slot j
+// carries raw feature planar_feature(j), with the [32][32] projection (if
+// not null) folded in: W1 diag(wfeat[:,c_k]) P, columns permuted.
 void launch_prepare_selection(const float *w1, const float *wfeat, int n_coef,
                               const int32_t *indices, int k, Elem mk_elem,
                               uint16_t *mk, float *wsel, hipStream_t s,
-                              uint16_t *mk32 = nullptr);
+                              uint16_t *mk32 = nullptr, bool planar = false,
+                              const float *projection = nullptr);
 
 // ---------------------------------------------------------------- encode --
 
@@ -64,15 +68,31 @@ struct EncodeArgs {
   // debug tap (null when unused)
   uint32_t *dbg_indices;      // [n][L][8]
   float *dbg_features;        // [n][32] unprojected FP32
+  // Planar form (16 levels x 2 features, no aux, no tap): `features` holds
+  // the unprojected hash-grid features as 8 planes [8][n][4], plane q =
+  // (level q f0, level q f1, level 15-q f0, level 15-q f1); the projection is
+  // folded into the first-layer selection tiles instead.  Plane q is
+  // computed by workgroups that land on XCD q, so each XCD's L2 serves two
+  // levels' hash tables.
+  int planar;
 };
 void launch_encode(const EncodeArgs &a, hipStream_t s);
+
+// Raw hash-grid feature (2 level + feature) carried by slot `slot` in [0, 32)
+// of the planar layout: slot / 4 is the plane, slot % 4 its value.
+__host__ __device__ inline int planar_feature(int slot) {
+  const int plane = slot >> 2, v = slot & 3;
+  const int level = v < 2 ? plane : 15 - plane;
+  return 2 * level + (v & 1);
+}
 
 // --------------------------------------------------------------- relight --
 
 struct RelightArgs {
   int n_pixels;
   int k;
-  const void *features;       // [P][32], Elem per kernel
+  const void *features;       // [P][32], Elem per kernel; planar: [8][P][4]
+  int features_planar;        // MFMA forms: features in the planar layout
   const uint16_t *mk;         // MFMA: [K][4][64][8]
   const float *wsel;          // plain: [K][32]
   const float *light;         // [K][3]
@@ -136,7 +156,8 @@ void build_fp8_weight_tiles(const float *w_f32, std::vector<uint8_t> &out, int e
 // Activation calibration: copies every `stride`-th pixel's BF16 features
 // (zeros for background pixels) into `calib` [n_cal][32] and clears amax[4].
 void launch_calib_gather(const uint16_t *features, const uint8_t *env_flag, int n_pixels,
-                         int stride, uint16_t *calib, int n_cal, float *amax, hipStream_t s);
+                         int stride, uint16_t *calib, int n_cal, float *amax, bool planar,
+                         hipStream_t s);
 // Maxima of the layer 1..3 activations of the BF16 network over the
 // calibration pixels and all K wavelets (`a`: BF16 MFMA operands, features =
 // calib, n_pixels = n_cal) into a.amax[0..2].

@@ -25,6 +25,19 @@ __device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float
   }
 }
 
+// Eight 16-bit features of pixel p at contraction slots slot0 .. slot0 + 7
+// (slot0 a multiple of 8): one 16-byte row read, or for planar features two
+// 8-byte reads from planes slot0 / 4 and slot0 / 4 + 1 (plane stride P).
+__device__ __forceinline__ u32x4 load_features8(const RelightArgs &a, int P, int p, int slot0) {
+  if (a.features_planar) {
+    const uint2 *pl = (const uint2 *)a.features;
+    const uint2 lo = pl[(size_t)(slot0 >> 2) * P + p];
+    const uint2 hi = pl[(size_t)((slot0 >> 2) + 1) * P + p];
+    return u32x4{lo.x, lo.y, hi.x, hi.y};
+  }
+  return *(const u32x4 *)((const uint16_t *)a.features + (size_t)p * 32 + slot0);
+}
+
 // Transport of one output (softplus or identity); its clamped light term is
 // max(light * T, 0) (Renderer/tinycudann/utils_kernels.h:74-78).
 template <bool SOFTPLUS>

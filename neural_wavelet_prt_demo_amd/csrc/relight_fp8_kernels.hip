@@ -130,15 +130,14 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
     const int sa4 = scale_byte(127 - a.w8_exp[2]), sb4 = scale_byte(127 - ea3);
     const float fs = __uint_as_float((uint32_t)(ea1 + 127) << 23);
 
-    // First-layer B operands: group g, column col, features 16 s + 8 h + [0, 8).
-    const uint16_t *feat = (const uint16_t *)a.features;
+    // First-layer B operands: group g, column col, slots 16 s + 8 h + [0, 8).
     u32x4 f[2][2];
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       const int p = base + 32 * g + col;
 #pragma unroll
       for (int s = 0; s < 2; ++s)
-        f[g][s] = p < P ? scale_bf16x8(*(const u32x4 *)(feat + (size_t)p * 32 + 16 * s + 8 * h), fs)
+        f[g][s] = p < P ? scale_bf16x8(dev::load_features8(a, P, p, 16 * s + 8 * h), fs)
                         : u32x4{0u, 0u, 0u, 0u};
     }
     const i32x8 *w8 = (const i32x8 *)a.w8_tiles;

@@ -272,12 +272,11 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
   float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
   if (any_hit) {
     // Per-pixel features as B operands: group j, column n, k-slots 8g..8g+7.
-    const uint16_t *feat = (const uint16_t *)a.features;
     u32x4 hb[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int p = base + 16 * j + n;
-      hb[j] = p < P ? *(const u32x4 *)(feat + (size_t)p * 32 + 8 * g) : u32x4{0u, 0u, 0u, 0u};
+      hb[j] = p < P ? dev::load_features8(a, P, p, 8 * g) : u32x4{0u, 0u, 0u, 0u};
     }
     // Aux inputs (B: group j, column n, k-slots 4g..4g+3) and the second
     // layer's aux columns (A tiles [4][64]).
@@ -507,12 +506,11 @@ __global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a
   const int k0 = blockIdx.y * k_chunk, k1 = min(a.k, k0 + k_chunk);
   const int P = a.n_pixels;
   if (base >= P || k0 >= k1) return;
-  const uint16_t *feat = (const uint16_t *)a.features;
   u32x4 hb[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int p = base + 16 * j + n;
-    hb[j] = p < P ? *(const u32x4 *)(feat + (size_t)p * 32 + 8 * g) : u32x4{0u, 0u, 0u, 0u};
+    hb[j] = p < P ? dev::load_features8(a, P, p, 8 * g) : u32x4{0u, 0u, 0u, 0u};
   }
   const u32x4 *wt = (const u32x4 *)a.w_tiles;
   u32x4 w2[4][2], w3[4][2];
@@ -564,21 +562,31 @@ __global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a
 }
 
 // Calibration subsample: calibration pixel q is frame pixel q * stride, its
-// features copied (zeros for a background pixel, whose features the encode
-// pass leaves unwritten); block 0 also clears the four maxima.
+// features copied in the frame's layout (zeros for a background pixel, whose
+// features the encode pass leaves unwritten); block 0 also clears the four
+// maxima.  Row layout: thread i copies 16 B part i % 4 of pixel i / 4;
+// planar: 8 B of plane i % 8 of pixel i / 8.
 __global__ __launch_bounds__(256) void calib_gather_kernel(const uint16_t *__restrict__ feat,
                                                            const uint8_t *__restrict__ flag,
                                                            int n_pixels, int stride,
                                                            uint16_t *__restrict__ calib,
-                                                           int n_cal, float *__restrict__ amax) {
+                                                           int n_cal, float *__restrict__ amax,
+                                                           int planar) {
   if (blockIdx.x == 0 && threadIdx.x < 4) amax[threadIdx.x] = 0.f;
   const int i = blockIdx.x * 256 + threadIdx.x;
-  const int q = i >> 2, part = i & 3;
+  const int shift = planar ? 3 : 2;
+  const int q = i >> shift, part = i & ((1 << shift) - 1);
   if (q >= n_cal) return;
   const long long p = (long long)q * stride;
   const bool hit = p < n_pixels && (flag == nullptr || flag[p] != 0);
-  const u32x4 v = hit ? *(const u32x4 *)(feat + p * 32 + 8 * part) : u32x4{0u, 0u, 0u, 0u};
-  *(u32x4 *)(calib + (size_t)q * 32 + 8 * part) = v;
+  if (planar) {
+    const uint2 *src = (const uint2 *)feat;
+    ((uint2 *)calib)[(size_t)part * n_cal + q] =
+        hit ? src[(size_t)part * n_pixels + p] : make_uint2(0u, 0u);
+  } else {
+    const u32x4 v = hit ? *(const u32x4 *)(feat + p * 32 + 8 * part) : u32x4{0u, 0u, 0u, 0u};
+    *(u32x4 *)(calib + (size_t)q * 32 + 8 * part) = v;
+  }
 }
 
 __global__ __launch_bounds__(256) void tonemap_kernel(const float *__restrict__ src,
@@ -757,13 +765,14 @@ void launch_reset_counter(unsigned int *counter, hipStream_t s) {
 }
 
 void launch_calib_gather(const uint16_t *features, const uint8_t *env_flag, int n_pixels,
-                         int stride, uint16_t *calib, int n_cal, float *amax, hipStream_t s) {
+                         int stride, uint16_t *calib, int n_cal, float *amax, bool planar,
+                         hipStream_t s) {
   require(features && calib && amax && n_pixels >= 1 && stride >= 1 && n_cal >= 1 &&
               (long long)(n_cal - 1) * stride < n_pixels,
           NWPRT_ERR_INVALID_ARGUMENT, "calibration gather: bad subsample");
-  const int threads = 4 * n_cal;
+  const int threads = (planar ? 8 : 4) * n_cal;
   hipLaunchKernelGGL(calib_gather_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, features,
-                     env_flag, n_pixels, stride, calib, n_cal, amax);
+                     env_flag, n_pixels, stride, calib, n_cal, amax, (int)planar);
   hip_check(hipGetLastError(), "calib_gather_kernel launch");
 }
 

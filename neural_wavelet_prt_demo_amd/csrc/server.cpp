@@ -250,12 +250,14 @@ void prepare(Server &sv, Request &q) {
   }
   launch_encode(encode_args(sc, &q.r.gbuffer, q.P, elem, features), s);
   launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * q.N * q.N, idx, q.K,
-                           elem, mk, nullptr, s);
+                           elem, mk, nullptr, s, nullptr, planar_features(sc, elem),
+                           scene_projection(sc));
   RelightArgs &a = sv.jobs_host[q.slot];
   a = RelightArgs{};
   a.n_pixels = q.P;
   a.k = q.K;
   a.features = features;
+  a.features_planar = planar_features(sc, elem);
   a.mk = mk;
   a.light = strength;
   a.w_tiles = (const uint16_t *)scene_tiles(sc, sv.cfg.precision);
