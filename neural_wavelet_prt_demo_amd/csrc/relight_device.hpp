@@ -45,5 +45,41 @@ __device__ __forceinline__ float transport(float o) {
   return SOFTPLUS ? softplus(o) : o;
 }
 
+// Clamped light terms max(light * T, 0) of the MFMA forms, accumulated with
+// two VALU operations fewer per term than the literal form:
+// - softplus: T = ln 2 * log2(1 + 2^y), y = o log2(e), and T >= 0, so the
+//   term is max(light, 0) * T.  The clamp is applied to the wave-uniform
+//   light (a scalar operation) and the factor ln 2 to the pixel's final sum
+//   (`kSumScale`) instead of every term; above o = 15 the scaled transport
+//   is y itself (1 + 2^y rounds to 2^y there);
+// - identity: T = o may be negative and the term stays max(light * o, 0).
+// Every term is non-negative either way.
+template <bool SOFTPLUS>
+struct LightTerm {
+  static constexpr float kSumScale = SOFTPLUS ? 0.693147180559945309f : 1.0f;
+  // Per wavelet and channel: the light as the term uses it.
+  __device__ static __forceinline__ float light(float l) { return SOFTPLUS ? relu_f32(l) : l; }
+  // Per pair: the term divided by kSumScale.
+  __device__ static __forceinline__ float term(float light, float o) {
+    if constexpr (SOFTPLUS) {
+      const float y = o * 1.44269504088896341f;
+      const float sp = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(y));
+      return light * (o > 15.0f ? y : sp);
+    } else {
+      return relu_f32(light * o);
+    }
+  }
+};
+
+// Kahan-compensated accumulation of non-negative terms: s - c is the
+// running sum (c holds the low-order part lost by the last addition, with
+// its sign flipped).
+__device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+  const float y = x - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
 }  // namespace dev
 }  // namespace nwprt

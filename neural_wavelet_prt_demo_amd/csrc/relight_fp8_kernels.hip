@@ -182,7 +182,8 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
       return mfma_fp8(w4s[64 + lane], x[1], o, sa4, sb4);
     };
     // Wavelets two at a time: the clamped terms of k and k + 1 are added and
-    // the pair sum enters the Neumaier-compensated running sum.
+    // the pair sum enters the Kahan-compensated running sum.
+    using LT = dev::LightTerm<SOFTPLUS>;
     for (int k = 0; k < a.k; k += 2) {
       const bool two = k + 1 < a.k;
       if (two) {
@@ -203,27 +204,25 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
       }
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        const float t0 = dev::transport<SOFTPLUS>(o0[ch]);
-        float x = dev::relu_f32(a.light[3 * k + ch] * t0);
-        float t1 = 0.f;
-        if (two) {
-          t1 = dev::transport<SOFTPLUS>(o1[ch]);
-          x += dev::relu_f32(a.light[3 * (k + 1) + ch] * t1);
-        }
-        dev::neumaier_add(sum[ch], comp[ch], x);
+        float x = LT::term(LT::light(a.light[3 * k + ch]), o0[ch]);
+        if (two) x += LT::term(LT::light(a.light[3 * (k + 1) + ch]), o1[ch]);
+        dev::kahan_add(sum[ch], comp[ch], x);
         if (DUMP) {
           const int q = my_px - a.dump_begin;
           if (mine && q >= 0 && q < a.dump_count) {
-            a.dump[((size_t)q * a.k + k) * 3 + ch] = t0;
-            if (two) a.dump[((size_t)q * a.k + k + 1) * 3 + ch] = t1;
+            a.dump[((size_t)q * a.k + k) * 3 + ch] = dev::transport<SOFTPLUS>(o0[ch]);
+            if (two)
+              a.dump[((size_t)q * a.k + k + 1) * 3 + ch] = dev::transport<SOFTPLUS>(o1[ch]);
           }
         }
       }
     }
   }
-  if (mine)
-    dev::write_outputs(a, my_px, hit ? sum[0] + comp[0] : 0.f, hit ? sum[1] + comp[1] : 0.f,
-                       hit ? sum[2] + comp[2] : 0.f);
+  if (mine) {
+    constexpr float f = dev::LightTerm<SOFTPLUS>::kSumScale;
+    dev::write_outputs(a, my_px, hit ? (sum[0] - comp[0]) * f : 0.f,
+                       hit ? (sum[1] - comp[1]) * f : 0.f, hit ? (sum[2] - comp[2]) * f : 0.f);
+  }
 }
 
 struct Fp8Variant {

@@ -9,7 +9,7 @@
 // Here one launch does all pairs of a frame and nothing per pair touches
 // HBM: each wave owns 64 pixels and walks the K selected wavelets; the four
 // linear layers run as chained 16x16x32 MFMAs whose results stay in VGPRs,
-// and the clamped sum is carried per lane with Neumaier compensation.
+// and the clamped sum is carried per lane with Kahan compensation.
 //
 // MFMA data layout (v_mfma_f32_16x16x32_{bf16,f16}; lane l, g = l/16,
 // n = l%16): the A operand holds row n, k-slots 8g..8g+7; B holds column n,
@@ -309,7 +309,8 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
         return wavelet_step<E, AUX, GP, W4LDS>(t, hb, auxb, waux, w2, w3, w4, w4s, lane);
     };
     // Wavelets are taken two at a time: the clamped terms of k and k + 1 are
-    // added, and that pair sum enters the Neumaier-compensated running sum.
+    // added, and that pair sum enters the Kahan-compensated running sum.
+    using LT = dev::LightTerm<SOFTPLUS>;
     for (int k = 0; k < a.k; k += 2) {
       const bool two = k + 1 < a.k;
       if (two) {
@@ -327,27 +328,23 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
       }
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        const float t0 = transport<SOFTPLUS>(o0[ch]);
-        float x = dev::relu_f32(a.light[3 * k + ch] * t0);
-        float t1 = 0.f;
-        if (two) {
-          t1 = transport<SOFTPLUS>(o1[ch]);
-          x += dev::relu_f32(a.light[3 * (k + 1) + ch] * t1);
-        }
-        dev::neumaier_add(sum[ch], comp[ch], x);
+        float x = LT::term(LT::light(a.light[3 * k + ch]), o0[ch]);
+        if (two) x += LT::term(LT::light(a.light[3 * (k + 1) + ch]), o1[ch]);
+        dev::kahan_add(sum[ch], comp[ch], x);
         if (DUMP) {
           const int q = my_px - a.dump_begin;
           if (mine && q >= 0 && q < a.dump_count) {
-            a.dump[((size_t)q * a.k + k) * 3 + ch] = t0;
-            if (two) a.dump[((size_t)q * a.k + k + 1) * 3 + ch] = t1;
+            a.dump[((size_t)q * a.k + k) * 3 + ch] = transport<SOFTPLUS>(o0[ch]);
+            if (two) a.dump[((size_t)q * a.k + k + 1) * 3 + ch] = transport<SOFTPLUS>(o1[ch]);
           }
         }
       }
     }
   }
   if (mine) {
-    const float r = hit ? sum[0] + comp[0] : 0.f, gg = hit ? sum[1] + comp[1] : 0.f,
-                bb = hit ? sum[2] + comp[2] : 0.f;
+    constexpr float f = dev::LightTerm<SOFTPLUS>::kSumScale;
+    const float r = hit ? (sum[0] - comp[0]) * f : 0.f, gg = hit ? (sum[1] - comp[1]) * f : 0.f,
+                bb = hit ? (sum[2] - comp[2]) * f : 0.f;
     write_outputs(a, my_px, r, gg, bb);
   }
 }
