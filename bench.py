@@ -50,6 +50,8 @@ def parse():
                     help="skip the batched uneven-jobs leg")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the served-stream leg")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="skip the two-slot frame pipeline leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -144,6 +146,34 @@ def bench_serving(nw, ctx, prec, n_requests=60):
     return {"workload": "Poisson arrivals, 4 resident scenes, K=300, 70% yaw-rotated resident "
                         "6x128x128 cubemap / 30% fresh host cubemap, 60 requests per rate",
             "memory_budget_gb": 200, "runs": runs}
+
+
+def bench_pipelined(nw, ctx, scene, dfaces, frame, prec, k, n, steps, warmup):
+    """Frames through the two-slot pipeline of the reference's interactive
+    loop (Renderer/OutputVisualizer.cpp:71-123): per frame an envmap update,
+    p1 and p2 -- the work of one sequential step -- with the update and p1
+    of frame f + 1 on a second stream while p2 of frame f runs.  Timed from
+    before the first frame's update to the end of the last p2."""
+    from neural_wavelet_prt_demo_amd import Event
+    from neural_wavelet_prt_demo_amd.pipeline import FramePipeline
+    envs = [ctx.envmap_from_faces(dfaces, k, face_size=n) for _ in range(2)]
+    pipe = FramePipeline(ctx, scene, envs, prec)
+    for _ in range(max(2, warmup)):
+        pipe.submit(frame, dfaces)
+    pipe.drain()
+    start, end = Event(), Event()
+    start.record(pipe.side.handle)
+    for _ in range(steps):
+        pipe.submit(frame, dfaces)
+    end.record(ctx.stream)
+    pipe.drain()
+    ms = start.elapsed_ms(end)
+    for e in envs:
+        ctx.envmap_destroy(e)
+    pairs = frame.width * frame.height * k
+    return {"frames": steps, "ms_per_frame": ms / steps,
+            "evals_per_s": pairs * steps / (ms / 1e3),
+            "streams": "map update + p1 on a side stream, p2 on the context stream"}
 
 
 def bench_batch(nw, ctx, prec, reps=3):
@@ -252,6 +282,14 @@ def main():
     total_ms = evs[0].elapsed_ms(evs[-1])
     value = pairs * args.steps / (total_ms / 1e3)
 
+    # --- two-slot frame pipeline: the same per-frame work, with the map
+    # update and p1 of frame f + 1 overlapping p2 of frame f ---------------
+    pipelined = None
+    if not args.no_pipeline:
+        pipelined = bench_pipelined(nw, ctx, scene, dfaces, frame, prec, K, N, args.steps,
+                                    args.warmup)
+        pipelined["speedup_vs_sequential"] = (total_ms / args.steps) / pipelined["ms_per_frame"]
+
     # --- hash-grid encode pass (p1) alone: random gathers, HBM-side -------
     pev = [Event() for _ in range(6)]
     p1ms = []
@@ -329,6 +367,8 @@ def main():
         "encode_pass": {"bound": "hbm", "ms": encode_ms, "bytes_per_pixel": ENCODE_BYTES_PER_PIXEL,
                         "achieved_gbs": encode_gbs, "frac": encode_gbs / HBM_PEAK_GBS},
     }
+    if pipelined is not None:
+        result["pipelined_two_slot"] = pipelined
 
     # --- batched uneven jobs: one graph, one persistent kernel ------------
     if not args.no_batch and prec != nw.PREC_FP8:

@@ -21,7 +21,7 @@ from typing import Dict, Optional
 import numpy as np
 
 from . import _hip
-from ._hip import DeviceBuffer, Event
+from ._hip import DeviceBuffer, Event, Stream
 
 LIB_PATH = Path(__file__).resolve().with_name("libnwprt.so")
 

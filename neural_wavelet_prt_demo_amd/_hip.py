@@ -34,6 +34,9 @@ def hip():
                                              ctypes.c_void_p]
         _hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
         _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamWaitEvent.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint]
         _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
     return _hip
 
@@ -122,6 +125,11 @@ class Event:
     def synchronize(self) -> None:
         check(hip().hipEventSynchronize(ctypes.c_void_p(self.ev)), "hipEventSynchronize")
 
+    def wait(self, stream: Optional[int]) -> None:
+        """Make later work on ``stream`` wait for this event's last record."""
+        check(hip().hipStreamWaitEvent(ctypes.c_void_p(stream or 0), ctypes.c_void_p(self.ev), 0),
+              "hipStreamWaitEvent")
+
     def elapsed_ms(self, end: "Event") -> float:
         ms = ctypes.c_float(0)
         check(hip().hipEventElapsedTime(ctypes.byref(ms), ctypes.c_void_p(self.ev),
@@ -131,5 +139,25 @@ class Event:
     def __del__(self):
         try:
             hip().hipEventDestroy(ctypes.c_void_p(self.ev))
+        except Exception:
+            pass
+
+
+class Stream:
+    """A HIP stream at the default priority; destroyed when collected."""
+
+    def __init__(self):
+        st = ctypes.c_void_p()
+        check(hip().hipStreamCreate(ctypes.byref(st)), "hipStreamCreate")
+        self.handle = st.value
+
+    def synchronize(self) -> None:
+        stream_synchronize(self.handle)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                hip().hipStreamDestroy(ctypes.c_void_p(self.handle))
+                self.handle = None
         except Exception:
             pass
