@@ -19,6 +19,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_build"
 LIB = PKG / "libnwprt.so"
 OFFLINE = PKG / "nwprt_offline"
+CEILING = PKG / "mfma_ceiling"
 
 SOURCES = [
     "host_util.cpp",
@@ -89,8 +90,15 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"offline driver build failed\n{' '.join(cmd)}\n{r.stderr}")
     os.replace(exe_tmp, OFFLINE)
+    # The sustained-MFMA-rate probe (standalone; no library dependency).
+    exe_tmp = CEILING.with_suffix(".tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, str(CSRC / "mfma_ceiling.hip"), "-o", str(exe_tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"mfma_ceiling build failed\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(exe_tmp, CEILING)
     if verbose:
-        print(f"[build] {LIB}\n[build] {OFFLINE}")
+        print(f"[build] {LIB}\n[build] {OFFLINE}\n[build] {CEILING}")
     return LIB
 
 
