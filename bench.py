@@ -26,6 +26,10 @@ FLOP_PER_PAIR = 2 * (32 * 64 + 64 * 64 + 64 * 64 + 64 * 3)  # 20,864
 # 16 levels x 8 corners x 2 FP16 features gathered (512 B), 32 projected
 # features written in the MLP element type (64 B for bf16 / fp16).
 ENCODE_BYTES_PER_PIXEL = 16 + 1 + 16 * 8 * 2 * 2 + 32 * 2
+# Denoiser, per pixel and a-trous pass: colour, normal and albedo read once
+# (16 B each; the 25 taps are cache reuse) and the pass result written
+# (16 B); plus one colour read (16 B) for the HDR normalisation.
+DENOISE_BYTES_PER_PIXEL_PASS = 4 * 16
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp8": 5000.0, "fp32": 157.3}
 METRIC = "(pixel,wavelet) MLP evals/s at 1080p K=300; p50/p99 relight latency vs plain"
@@ -52,6 +56,8 @@ def parse():
                     help="skip the served-stream leg")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="skip the two-slot frame pipeline leg")
+    ap.add_argument("--no-denoise", action="store_true",
+                    help="skip the denoiser leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -156,11 +162,41 @@ def bench_serving(nw, ctx, prec, n_requests=60):
                 "peak_active": st["peak_active"], "peak_state_gb": st["peak_bytes"] / 1e9,
                 "ticks": st["ticks"]}
 
-    runs = [stream(r, 0.0, r) for r in (30, 60, 120, 240, 480)]
+    def closed_loop(n, seed):
+        """One request in flight at a time (the next is submitted when the
+        previous completes): the server's own request latency, no queueing."""
+        rng = np.random.default_rng(seed)
+        srv = ctx.server_create(budget_bytes=200 << 30, max_active=ring, precision=prec)
+        lat = []
+        for i in range(n + 2):
+            s = int(rng.integers(4))
+            o = outs["1080p"][i % ring]
+            fo = FrameOut(o[0].ptr, o[1].ptr, o[2].ptr)
+            t = time.perf_counter()
+            if rng.random() < 0.7:
+                ctx.server_submit(srv, scenes[s], gbufs["1080p"], faces=bases[s],
+                                  yaw=float(rng.uniform(0, 2 * np.pi)), k=300, face_size=128,
+                                  out=fo)
+            else:
+                ctx.server_submit(srv, scenes[s], gbufs["1080p"],
+                                  faces=fresh[int(rng.integers(len(fresh)))], k=300,
+                                  face_size=128, out=fo)
+            got = ctx.server_poll(srv, wait=2)
+            if i >= 2:  # the first two warm the pool
+                lat.append((time.perf_counter() - t) * 1e3)
+            assert len(got) == 1
+        ctx.server_destroy(srv)
+        ms = np.array(lat)
+        return {"mode": "closed loop, one request in flight", "views": "1080p", "requests": n,
+                "p50_ms": percentile(ms, 50), "p99_ms": percentile(ms, 99),
+                "max_ms": float(ms.max())}
+
+    single = closed_loop(40, 11)
+    runs = [stream(r, 0.0, r) for r in (10, 30, 60, 120, 240, 480)]
     runs.append(stream(60, 0.25, 7))
     return {"workload": "Poisson arrivals, 4 resident scenes, K=300, 70% yaw-rotated resident "
                         "6x128x128 cubemap / 30% fresh host cubemap, 60 requests per rate",
-            "memory_budget_gb": 200, "runs": runs}
+            "memory_budget_gb": 200, "closed_loop": single, "runs": runs}
 
 
 def bench_pipelined(nw, ctx, scene, dfaces, frame, prec, k, n, steps, warmup):
@@ -317,6 +353,28 @@ def main():
     encode_ms = float(np.mean(p1ms))
     encode_gbs = P * ENCODE_BYTES_PER_PIXEL / (encode_ms / 1e3) / 1e9
 
+    # --- denoiser on the relit frame (with-direct output, normal + kd guides)
+    denoise_leg = None
+    if not args.no_denoise:
+        dn_out = DeviceBuffer(P * 16)
+        dev = [Event() for _ in range(6)]
+        dms = []
+        for i in range(5):
+            dev[i].record(stream)
+            ctx.denoise(frame.out["indirect_with_direct"], frame.bufs["normal"],
+                        frame.bufs["kd"], dn_out, W, H, 0.0, None, stream)
+            dev[i + 1].record(stream)
+            ctx.sync()
+            dms.append(dev[i].elapsed_ms(dev[i + 1]))
+        dn_ms = float(np.mean(dms[1:]))
+        it = nw.Context.denoise_default_params().iterations
+        dn_bytes = P * (DENOISE_BYTES_PER_PIXEL_PASS * it + 16)
+        denoise_leg = {"bound": "hbm", "passes": it, "ms": dn_ms,
+                       "bytes_per_pixel": DENOISE_BYTES_PER_PIXEL_PASS * it + 16,
+                       "achieved_gbs": dn_bytes / (dn_ms / 1e3) / 1e9,
+                       "frac": dn_bytes / (dn_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+        del dn_out
+
     # --- dominant kernel alone: the per-pair pass (p2) with HIP events -----
     kev = [Event() for _ in range(6)]
     kms = []
@@ -385,6 +443,8 @@ def main():
     }
     if pipelined is not None:
         result["pipelined_two_slot"] = pipelined
+    if denoise_leg is not None:
+        result["denoise_pass"] = denoise_leg
     if ceiling is not None and args.precision in ("bf16", "fp16"):
         # The published peak assumes the nominal clock; with real (toggling)
         # operands a pure MFMA loop on this GPU runs power-limited below it.

@@ -361,6 +361,41 @@ nwprt_status nwprt_tonemap(nwprt_ctx ctx, const float *src_float4,
                            uint32_t *dst_rgba8, int32_t width, int32_t height,
                            void *stream);
 
+/* -------------------------------------------------------------- denoiser -- */
+/* OptixBackend::denoise(color, normal, albedo, output, blendFactor, size,
+ * stream) (Renderer/OptixBackend.cpp:841-906), as called by
+ * RenderEngine::denoiseOrMoveData with the denoiser on
+ * (Renderer/RenderEngine.cpp:390-397) and by TCNNPass_p2 on its two outputs
+ * (Renderer/tinycudann/tcnn.cu:144-148).  All images are device float4
+ * [height][width] (the reference's OptixImage2D FLOAT4 layout); the normal
+ * and albedo buffers are the guides (guideNormal / guideAlbedo,
+ * OptixBackend.cpp:783-785; world or camera normals give the same result,
+ * only differences between normals enter).  The reference's learned model
+ * is replaced by an edge-avoiding a-trous wavelet filter:
+ * - kappa = 2^-mean(log2 Y) over pixels with luminance Y > 1e-8 (the role
+ *   of optixDenoiserComputeIntensity, OptixBackend.cpp:871-877);
+ * - `iterations` passes, step 2^i, 5x5 B3-spline taps, weights
+ *   exp(-(kappa^2 |dc|^2 4^i / sigma_color^2 + |dn|^2 / sigma_normal^2 +
+ *   |da|^2 / sigma_albedo^2)), taps outside the image skipped;
+ * - RGB = blend * color + (1 - blend) * filtered; alpha copied from color
+ *   (OPTIX_DENOISER_ALPHA_MODE_COPY, OptixBackend.cpp:815).
+ * `output` must not overlap the three inputs.  Work buffers (2 images) are
+ * held by the context, like the reference's per-size denoiser state
+ * (OptixBackend::setupDenoiser, OptixBackend.cpp:772-817).  params NULL
+ * selects the defaults below. */
+typedef struct {
+  int32_t iterations;  /* 1..8 a-trous passes (default 5)                  */
+  float sigma_color;   /* colour scale, kappa-normalised (default 1.0)     */
+  float sigma_normal;  /* normal scale (default 0.3)                       */
+  float sigma_albedo;  /* albedo scale (default 0.1)                       */
+} nwprt_denoise_params;
+
+void nwprt_denoise_default_params(nwprt_denoise_params *params);
+nwprt_status nwprt_denoise(nwprt_ctx ctx, const float *color, const float *normal,
+                           const float *albedo, float *output, int32_t width,
+                           int32_t height, float blend_factor,
+                           const nwprt_denoise_params *params, void *stream);
+
 /* ------------------------------------------------------------ debug taps -- */
 
 /* Hash-grid tap: for n positions (device float4), write the 8 corner entry

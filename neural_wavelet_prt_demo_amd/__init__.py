@@ -111,6 +111,11 @@ class ServerStats(Structure):
                 ("bytes_in_use", c_uint64), ("peak_bytes", c_uint64), ("peak_active", c_int32)]
 
 
+class DenoiseParams(Structure):
+    _fields_ = [("iterations", c_int32), ("sigma_color", c_float), ("sigma_normal", c_float),
+                ("sigma_albedo", c_float)]
+
+
 GBUFFER_FIELDS = {
     "position": (np.float32, 4), "normal": (np.float32, 4), "reflect": (np.float32, 4),
     "roughness": (np.float32, 1), "kd": (np.float32, 4), "ks": (np.float32, 4),
@@ -175,6 +180,9 @@ _SIGNATURES = {
     "nwprt_selected_relight_variant": (c_int, [c_void_p, c_int, c_int32, c_int32,
                                                POINTER(c_int32)]),
     "nwprt_tonemap": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_void_p]),
+    "nwprt_denoise_default_params": (None, [POINTER(DenoiseParams)]),
+    "nwprt_denoise": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+                              c_float, POINTER(DenoiseParams), c_void_p]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -566,6 +574,24 @@ class Context:
     def computeFinalPixelColors(self, src: DeviceBuffer, dst: DeviceBuffer, width: int,
                                 height: int, stream: Optional[int] = None) -> None:
         _check(lib().nwprt_tonemap(self.h, src.ptr, dst.ptr, width, height,
+                                   c_void_p(stream) if stream else None))
+
+    # denoiser ------------------------------------------------------------
+    @staticmethod
+    def denoise_default_params() -> DenoiseParams:
+        p = DenoiseParams()
+        lib().nwprt_denoise_default_params(ctypes.byref(p))
+        return p
+
+    def denoise(self, color, normal, albedo, out, width: int, height: int,
+                blend: float = 0.0, params: Optional[DenoiseParams] = None,
+                stream: Optional[int] = None) -> None:
+        """OptixBackend::denoise (Renderer/OptixBackend.cpp:841-906) on device
+        float4 images (DeviceBuffer or raw device address)."""
+        ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+        _check(lib().nwprt_denoise(self.h, ptr(color), ptr(normal), ptr(albedo), ptr(out),
+                                   width, height, blend,
+                                   ctypes.byref(params) if params is not None else None,
                                    c_void_p(stream) if stream else None))
 
     # debug taps ----------------------------------------------------------

@@ -33,6 +33,7 @@ SOURCES = [
     "encode_kernels.hip",
     "relight_kernels.hip",
     "relight_fp8_kernels.hip",
+    "denoise_kernels.hip",
 ]
 
 # Packed-FP32 VALU forms are disabled for the device code (the host pass

@@ -1,0 +1,182 @@
+// Guided edge-avoiding a-trous denoiser: the replacement for the denoiser
+// step after the relight pass (Renderer/RenderEngine.cpp:390-403 calling
+// OptixBackend::denoise, Renderer/OptixBackend.cpp:841-906).  The
+// reference's denoiser is a closed learned model; this one is the published
+// edge-avoiding a-trous wavelet filter (Dammertz, Sewtz, Hanika, Lensch,
+// "Edge-Avoiding A-Trous Wavelet Transform for fast Global Illumination
+// Filtering", HPG 2010) guided by the same inputs the reference passes:
+// the colour, the albedo and the normal buffers (float4, guideAlbedo and
+// guideNormal on, OptixBackend.cpp:783-785), with the alpha channel copied
+// from the input (OPTIX_DENOISER_ALPHA_MODE_COPY, OptixBackend.cpp:815) and
+// the result blended with the input by blendFactor.
+//
+// The filter, for an image of P = W x H pixels:
+// - HDR normalisation: kappa = 2^-m with m the mean of log2(Y) over pixels
+//   whose luminance Y = 0.2126 R + 0.7152 G + 0.0722 B (FP64) is > 1e-8
+//   (kappa = 1 if none), so colour distances are relative to the image's
+//   log-average luminance;
+// - passes i = 0 .. L-1 with step s = 2^i over the 5 x 5 B3-spline taps
+//   h(dx) h(dy), h = (1, 4, 6, 4, 1) / 16, skipping taps outside the image:
+//     w(p, q) = h exp(-(kappa^2 |c_q - c_p|^2 4^i / sc^2
+//                      + |n_q - n_p|^2 / sn^2 + |a_q - a_p|^2 / sa^2)),
+//     c'_p = sum_q w c_q / sum_q w   (RGB; c the previous pass's output),
+//   the colour sigma halving each pass as in the paper;
+// - output RGB = blend * input + (1 - blend) * c after pass L-1, alpha =
+//   the input's alpha.
+// Passes run in FP32 (one exp2 per tap); the normalisation runs in FP64.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace nwprt {
+namespace {
+
+constexpr int kTile = 16;            // 16 x 16 pixels per workgroup
+constexpr int kIntensityBlocks = 512;
+constexpr int kIntensityThreads = 256;
+
+// Per-block partial sums of log2(Y) and counts over lit pixels.
+__global__ __launch_bounds__(kIntensityThreads) void intensity_partials_kernel(
+    const float4 *__restrict__ color, long long n, double2 *__restrict__ partials) {
+  double s = 0.0, c = 0.0;
+  for (long long p = (long long)blockIdx.x * kIntensityThreads + threadIdx.x; p < n;
+       p += (long long)kIntensityBlocks * kIntensityThreads) {
+    const float4 v = color[p];
+    const double y = 0.2126 * (double)v.x + 0.7152 * (double)v.y + 0.0722 * (double)v.z;
+    if (y > 1e-8) {
+      s += log2(y);
+      c += 1.0;
+    }
+  }
+  __shared__ double ss[kIntensityThreads], sc[kIntensityThreads];
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = kIntensityThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      ss[threadIdx.x] += ss[threadIdx.x + h];
+      sc[threadIdx.x] += sc[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = make_double2(ss[0], sc[0]);
+}
+
+// kappa = 2^-(sum / count) from the partials, in a fixed order.
+__global__ __launch_bounds__(kIntensityBlocks) void intensity_finish_kernel(
+    const double2 *__restrict__ partials, float *__restrict__ kappa) {
+  __shared__ double ss[kIntensityBlocks], sc[kIntensityBlocks];
+  const double2 v = partials[threadIdx.x];
+  ss[threadIdx.x] = v.x;
+  sc[threadIdx.x] = v.y;
+  __syncthreads();
+  for (int h = kIntensityBlocks / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      ss[threadIdx.x] += ss[threadIdx.x + h];
+      sc[threadIdx.x] += sc[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) kappa[0] = sc[0] > 0.0 ? (float)exp2(-ss[0] / sc[0]) : 1.0f;
+}
+
+__device__ __forceinline__ float sq3(float4 a, float4 b) {
+  const float x = a.x - b.x, y = a.y - b.y, z = a.z - b.z;
+  return x * x + y * y + z * z;
+}
+
+// One a-trous pass.  inv_c2 = 4^i / sc^2 (times kappa^2 on the device),
+// inv_n2 = 1 / sn^2, inv_a2 = 1 / sa^2.  LAST: blend with `orig` and write
+// `out`; otherwise write the pass result (alpha carried from `orig`).
+template <bool LAST>
+__global__ __launch_bounds__(kTile * kTile) void atrous_kernel(
+    const float4 *__restrict__ in, const float4 *__restrict__ orig,
+    const float4 *__restrict__ normal, const float4 *__restrict__ albedo,
+    const float *__restrict__ kappa, float4 *__restrict__ out, int w, int h, int step,
+    float inv_c2, float inv_n2, float inv_a2, float blend) {
+  const int x = blockIdx.x * kTile + threadIdx.x, y = blockIdx.y * kTile + threadIdx.y;
+  if (x >= w || y >= h) return;
+  const size_t p = (size_t)y * w + x;
+  const float k = kappa[0];
+  const float wc = k * k * inv_c2 * 1.44269504088896341f;  // exp(-d) = exp2(-d log2 e)
+  const float wn = inv_n2 * 1.44269504088896341f, wa = inv_a2 * 1.44269504088896341f;
+  const float4 cp = in[p], np = normal[p], ap = albedo[p];
+  constexpr float kh[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  float sr = 0.f, sg = 0.f, sb = 0.f, sw = 0.f;
+#pragma unroll
+  for (int dy = -2; dy <= 2; ++dy) {
+    const int qy = y + dy * step;
+    if (qy < 0 || qy >= h) continue;
+#pragma unroll
+    for (int dx = -2; dx <= 2; ++dx) {
+      const int qx = x + dx * step;
+      if (qx < 0 || qx >= w) continue;
+      const size_t q = (size_t)qy * w + qx;
+      const float4 cq = in[q], nq = normal[q], aq = albedo[q];
+      const float d = wc * sq3(cq, cp) + wn * sq3(nq, np) + wa * sq3(aq, ap);
+      const float wt = kh[dx + 2] * kh[dy + 2] * __builtin_amdgcn_exp2f(-d);
+      sr += wt * cq.x;
+      sg += wt * cq.y;
+      sb += wt * cq.z;
+      sw += wt;
+    }
+  }
+  // The centre tap has weight h0^2 > 0, so sw > 0.
+  const float inv = 1.0f / sw;
+  const float4 o = orig[p];
+  float4 r;
+  if (LAST) {
+    const float b1 = 1.0f - blend;
+    r = make_float4(blend * o.x + b1 * (sr * inv), blend * o.y + b1 * (sg * inv),
+                    blend * o.z + b1 * (sb * inv), o.w);
+  } else {
+    r = make_float4(sr * inv, sg * inv, sb * inv, o.w);
+  }
+  out[p] = r;
+}
+
+}  // namespace
+
+size_t denoise_scratch_bytes(int w, int h) {
+  return 2 * (size_t)w * h * sizeof(float4) + kIntensityBlocks * sizeof(double2) + 256;
+}
+
+void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s) {
+  require(a.color && a.normal && a.albedo && a.out && scratch, NWPRT_ERR_INVALID_ARGUMENT,
+          "denoise: null buffer");
+  require(a.width > 0 && a.height > 0 && a.iterations >= 1 && a.iterations <= kMaxDenoisePasses,
+          NWPRT_ERR_INVALID_ARGUMENT, "denoise: bad size or iteration count");
+  const size_t P = (size_t)a.width * a.height;
+  float4 *t0 = (float4 *)scratch, *t1 = t0 + P;
+  double2 *partials = (double2 *)(t1 + P);
+  float *kappa = (float *)(partials + kIntensityBlocks);
+  hipLaunchKernelGGL(intensity_partials_kernel, dim3(kIntensityBlocks), dim3(kIntensityThreads),
+                     0, s, (const float4 *)a.color, (long long)P, partials);
+  hip_check(hipGetLastError(), "intensity_partials_kernel launch");
+  hipLaunchKernelGGL(intensity_finish_kernel, dim3(1), dim3(kIntensityBlocks), 0, s,
+                     (const double2 *)partials, kappa);
+  hip_check(hipGetLastError(), "intensity_finish_kernel launch");
+  const dim3 grid((a.width + kTile - 1) / kTile, (a.height + kTile - 1) / kTile);
+  const dim3 block(kTile, kTile);
+  const float4 *orig = (const float4 *)a.color;
+  const float4 *src = orig;
+  const float inv_n2 = 1.0f / (a.sigma_normal * a.sigma_normal);
+  const float inv_a2 = 1.0f / (a.sigma_albedo * a.sigma_albedo);
+  for (int i = 0; i < a.iterations; ++i) {
+    const float inv_c2 = (float)(1 << (2 * i)) / (a.sigma_color * a.sigma_color);
+    const bool last = i == a.iterations - 1;
+    float4 *dst = last ? (float4 *)a.out : (i % 2 == 0 ? t0 : t1);
+    if (last)
+      hipLaunchKernelGGL(atrous_kernel<true>, grid, block, 0, s, src, orig,
+                         (const float4 *)a.normal, (const float4 *)a.albedo, kappa, dst, a.width,
+                         a.height, 1 << i, inv_c2, inv_n2, inv_a2, a.blend);
+    else
+      hipLaunchKernelGGL(atrous_kernel<false>, grid, block, 0, s, src, orig,
+                         (const float4 *)a.normal, (const float4 *)a.albedo, kappa, dst, a.width,
+                         a.height, 1 << i, inv_c2, inv_n2, inv_a2, a.blend);
+    hip_check(hipGetLastError(), "atrous_kernel launch");
+    src = dst;
+  }
+}
+
+}  // namespace nwprt

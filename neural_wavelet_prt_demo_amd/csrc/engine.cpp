@@ -915,6 +915,48 @@ nwprt_status nwprt_tonemap(nwprt_ctx ctx, const float *src_float4, uint32_t *dst
   });
 }
 
+void nwprt_denoise_default_params(nwprt_denoise_params *params) {
+  if (params) *params = nwprt_denoise_params{5, 1.0f, 0.3f, 0.1f};
+}
+
+nwprt_status nwprt_denoise(nwprt_ctx ctx, const float *color, const float *normal,
+                           const float *albedo, float *output, int32_t width, int32_t height,
+                           float blend_factor, const nwprt_denoise_params *params,
+                           void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    nwprt_denoise_params pr;
+    nwprt_denoise_default_params(&pr);
+    if (params) pr = *params;
+    require(color && normal && albedo && output, NWPRT_ERR_INVALID_ARGUMENT,
+            "denoise: null buffer");
+    require(width > 0 && height > 0 && (int64_t)width * height <= (1ll << 31),
+            NWPRT_ERR_INVALID_ARGUMENT, "denoise: bad image size");
+    require(pr.iterations >= 1 && pr.iterations <= kMaxDenoisePasses,
+            NWPRT_ERR_INVALID_ARGUMENT, "denoise: iterations must be 1..8");
+    require(pr.sigma_color > 0.f && pr.sigma_normal > 0.f && pr.sigma_albedo > 0.f &&
+                std::isfinite(pr.sigma_color) && std::isfinite(pr.sigma_normal) &&
+                std::isfinite(pr.sigma_albedo),
+            NWPRT_ERR_INVALID_ARGUMENT, "denoise: sigmas must be positive and finite");
+    require(blend_factor >= 0.f && blend_factor <= 1.f, NWPRT_ERR_INVALID_ARGUMENT,
+            "denoise: blend factor must be in [0, 1]");
+    for (const void *p : {(const void *)color, (const void *)normal, (const void *)albedo,
+                          (const void *)output})
+      check_aligned(p, 16, "denoise image");
+    const size_t bytes = (size_t)width * height * 16;
+    const char *o = (const char *)output;
+    for (const float *in : {color, normal, albedo}) {
+      const char *i = (const char *)in;
+      require(o + bytes <= i || i + bytes <= o, NWPRT_ERR_INVALID_ARGUMENT,
+              "denoise: output overlaps an input");
+    }
+    ctx->denoise_scratch.ensure(denoise_scratch_bytes(width, height));
+    DenoiseArgs a{color, normal, albedo, output, width, height, pr.iterations,
+                  pr.sigma_color, pr.sigma_normal, pr.sigma_albedo, blend_factor};
+    launch_denoise(a, ctx->denoise_scratch.p, pick(ctx, stream));
+  });
+}
+
 nwprt_status nwprt_debug_hash_encode(nwprt_ctx ctx, int32_t scene_id, const float *positions,
                                      int32_t n, uint32_t *indices, float *features,
                                      void *stream) {

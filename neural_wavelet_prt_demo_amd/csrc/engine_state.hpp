@@ -133,6 +133,7 @@ struct nwprt_context {
   std::map<int, std::unique_ptr<nwprt::Batch>> batches;
   std::map<int, nwprt::ServerPtr> servers;
   int n_cu = 0;  // compute units of the device (persistent grid size)
+  nwprt::DevBuf denoise_scratch;  // denoiser ping-pong images + normalisation
 };
 
 namespace nwprt {

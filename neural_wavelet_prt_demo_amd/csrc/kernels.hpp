@@ -127,6 +127,18 @@ void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream
 void launch_relight_plain(const RelightArgs &a, hipStream_t s);
 void launch_tonemap(const float *src, uint32_t *dst, int n_pixels, hipStream_t s);
 
+// Guided a-trous denoiser (denoise_kernels.hip).
+constexpr int kMaxDenoisePasses = 8;
+struct DenoiseArgs {
+  const float *color, *normal, *albedo;  // float4 [height][width]
+  float *out;
+  int width, height, iterations;
+  float sigma_color, sigma_normal, sigma_albedo, blend;
+};
+// Device bytes of the work buffer launch_denoise needs for a w x h image.
+size_t denoise_scratch_bytes(int w, int h);
+void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s);
+
 // Persistent batch form: work item = the 64 pixels [base, base + 64) of job
 // `job` (base < the job's pixel count).
 struct BatchItem {

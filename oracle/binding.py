@@ -65,6 +65,9 @@ def lib():
                                          POINTER(OrcGbuffer), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p, c_int]
         l.orc_tonemap.argtypes = [c_float, c_float, c_float]
+        l.orc_denoise.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double,
+                                  c_double, c_double, c_double, c_void_p]
+        l.orc_denoise.restype = None
         l.orc_tonemap.restype = c_uint32
         l.orc_aux_inputs.argtypes = [POINTER(OrcGbuffer), c_int, c_float, c_void_p]
         l.orc_cpu_relight_fp32.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int,
@@ -201,3 +204,18 @@ def tonemap(rgb: np.ndarray) -> np.ndarray:
     rgb = np.asarray(rgb, np.float32).reshape(-1, rgb.shape[-1])
     return np.array([lib().orc_tonemap(float(r[0]), float(r[1]), float(r[2])) for r in rgb],
                     np.uint32)
+
+
+def denoise(color: np.ndarray, normal: np.ndarray, albedo: np.ndarray, iterations: int = 5,
+            sigma_color: float = 1.0, sigma_normal: float = 0.3, sigma_albedo: float = 0.1,
+            blend: float = 0.0) -> np.ndarray:
+    """Guided a-trous denoiser on float4 [h][w] images; returns float64 [h][w][4]."""
+    c = np.ascontiguousarray(color, np.float32)
+    n = np.ascontiguousarray(normal, np.float32)
+    a = np.ascontiguousarray(albedo, np.float32)
+    h, w = c.shape[:2]
+    assert c.shape == (h, w, 4) and n.shape == c.shape and a.shape == c.shape
+    out = np.empty((h, w, 4), np.float64)
+    lib().orc_denoise(_p(c), _p(n), _p(a), w, h, iterations, sigma_color, sigma_normal,
+                      sigma_albedo, blend, _p(out))
+    return out

@@ -370,3 +370,77 @@ uint32_t orc_tonemap(float r, float g, float b) {
   }
   return out;
 }
+
+/* Guided edge-avoiding a-trous denoiser (Dammertz, Sewtz, Hanika, Lensch,
+ * HPG 2010), the filter that replaces the reference's closed learned
+ * denoiser behind OptixBackend::denoise (Renderer/OptixBackend.cpp:841-906):
+ * colour, normal and albedo are float4 [h][w]; kappa = 2^-mean(log2 Y) over
+ * pixels with Y = 0.2126 R + 0.7152 G + 0.0722 B > 1e-8 (1 if none);
+ * pass i (step 2^i) replaces each pixel's RGB by the weighted mean of the
+ * in-image 5 x 5 taps with weight h(dx) h(dy) exp(-(kappa^2 |dc|^2 4^i / sc^2
+ * + |dn|^2 / sn^2 + |da|^2 / sa^2)), h = (1, 4, 6, 4, 1) / 16; the output
+ * is blend * colour + (1 - blend) * the last pass, alpha = colour alpha.
+ * FP64 throughout.  out: double [h][w][4]. */
+void orc_denoise(const float *color, const float *normal, const float *albedo, int w, int h,
+                 int iterations, double sigma_color, double sigma_normal, double sigma_albedo,
+                 double blend, double *out) {
+  const size_t P = (size_t)w * h;
+  double s = 0.0, cnt = 0.0;
+  for (size_t p = 0; p < P; ++p) {
+    const double y = 0.2126 * (double)color[4 * p] + 0.7152 * (double)color[4 * p + 1] +
+                     0.0722 * (double)color[4 * p + 2];
+    if (y > 1e-8) {
+      s += log2(y);
+      cnt += 1.0;
+    }
+  }
+  const double kappa = cnt > 0.0 ? exp2(-s / cnt) : 1.0;
+  double *cur = (double *)malloc(P * 3 * sizeof(double));
+  double *nxt = (double *)malloc(P * 3 * sizeof(double));
+  for (size_t p = 0; p < P; ++p)
+    for (int c = 0; c < 3; ++c) cur[3 * p + c] = color[4 * p + c];
+  static const double kh[5] = {1.0 / 16, 4.0 / 16, 6.0 / 16, 4.0 / 16, 1.0 / 16};
+  for (int i = 0; i < iterations; ++i) {
+    const int step = 1 << i;
+    const double ic = kappa * kappa * (double)(1 << (2 * i)) / (sigma_color * sigma_color);
+    const double in_ = 1.0 / (sigma_normal * sigma_normal);
+    const double ia = 1.0 / (sigma_albedo * sigma_albedo);
+    for (int y = 0; y < h; ++y)
+      for (int x = 0; x < w; ++x) {
+        const size_t p = (size_t)y * w + x;
+        double acc[3] = {0.0, 0.0, 0.0}, ws = 0.0;
+        for (int dy = -2; dy <= 2; ++dy) {
+          const int qy = y + dy * step;
+          if (qy < 0 || qy >= h) continue;
+          for (int dx = -2; dx <= 2; ++dx) {
+            const int qx = x + dx * step;
+            if (qx < 0 || qx >= w) continue;
+            const size_t q = (size_t)qy * w + qx;
+            double dc = 0.0, dn = 0.0, da = 0.0;
+            for (int c = 0; c < 3; ++c) {
+              const double a = cur[3 * q + c] - cur[3 * p + c];
+              const double b = (double)normal[4 * q + c] - (double)normal[4 * p + c];
+              const double e = (double)albedo[4 * q + c] - (double)albedo[4 * p + c];
+              dc += a * a;
+              dn += b * b;
+              da += e * e;
+            }
+            const double wt = kh[dx + 2] * kh[dy + 2] * exp(-(ic * dc + in_ * dn + ia * da));
+            for (int c = 0; c < 3; ++c) acc[c] += wt * cur[3 * q + c];
+            ws += wt;
+          }
+        }
+        for (int c = 0; c < 3; ++c) nxt[3 * p + c] = acc[c] / ws;
+      }
+    double *t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  for (size_t p = 0; p < P; ++p) {
+    for (int c = 0; c < 3; ++c)
+      out[4 * p + c] = blend * (double)color[4 * p + c] + (1.0 - blend) * cur[3 * p + c];
+    out[4 * p + 3] = color[4 * p + 3];
+  }
+  free(cur);
+  free(nxt);
+}

@@ -27,6 +27,12 @@
  *                        closed forms (zero weights -> softplus(0) = ln 2,
  *                        background -> 0, negative light -> clamped)
  *   orc_tonemap          closed forms (0 -> 0, 1 -> 255, alpha 255, byte order)
+ *   orc_denoise          constant image and blend 1 unchanged; unbounded sigmas =
+ *                        normalised B3-spline a-trous convolution (scipy); a
+ *                        guide edge stops all flow across it; power-of-two
+ *                        colour scaling commutes; transpose symmetry; the
+ *                        learned model it replaces is closed, so parity with
+ *                        the reference's denoised images is out of reach
  * The tiny-cuda-nn conventions behind the hash grid and the MLP (level
  * rounding, layer layout, padding values) cannot be pinned against the
  * reference: that library is an empty submodule there ("parity unpinned"
@@ -88,6 +94,11 @@ void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *li
                         double *indirect, double *with_direct, double *transport, int n_threads);
 
 uint32_t orc_tonemap(float r, float g, float b);
+
+/* Guided a-trous denoiser; images float4 [h][w], out double [h][w][4]. */
+void orc_denoise(const float *color, const float *normal, const float *albedo, int w, int h,
+                 int iterations, double sigma_color, double sigma_normal, double sigma_albedo,
+                 double blend, double *out);
 
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
