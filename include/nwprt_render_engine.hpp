@@ -12,10 +12,11 @@
 // bound checked as >=).
 //
 // The OptiX pass that fills the G-buffer is not part of this build: the
-// caller (or writeSyntheticGBuffer) writes fbFirstHitPos & co.  The OptiX
-// denoiser is not part of it either: with a denoiser requested or not, the
-// "Denoised" buffers receive a copy (the reference's denoiser-off path,
-// Renderer/RenderEngine.cpp:398-401), and denoiserApplied() reports false.
+// caller (or writeSyntheticGBuffer) writes fbFirstHitPos & co.  The
+// denoiser step (RenderEngine::denoiseOrMoveData, RenderEngine.cpp:390-403)
+// runs nwprt_denoise -- a guided a-trous filter in place of the OptiX
+// learned denoiser -- with the frame's world normals and diffuse albedo
+// (fbFirstHitKd) as guides, or copies when the denoiser is off.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -151,7 +152,9 @@ class RenderEngine {
   }
 
   // RenderEngine::TCNNPass_p2 (tcnn.cu:111-152): per-pair network, clamped
-  // K-sum, then the denoise-or-copy into the "Denoised" buffers.
+  // K-sum, then the denoise-or-copy into the "Denoised" buffers
+  // (tcnn.cu:144-148: finalDenoiser for the with-direct image,
+  // indirectDenoiser for the indirect one, blend factor 0).
   void TCNNPass_p2(int envMapID, int frameID, renderConfig config, hipStream_t stream = nullptr) {
     if (envMapID < 0 || (size_t)envMapID >= envs_.size())
       throw std::overflow_error("Envmap ID for rendering is larger than cached envmap");
@@ -160,13 +163,33 @@ class RenderEngine {
     nwprt_frame_out out{f.fbPredictedIndirect, f.fbPredictedIndirectWithDirect, nullptr};
     check(nwprt_pass_p2(ctx_, scene_, envs_[(size_t)envMapID], frameID, &g, width_, height_, &out,
                         config.precision, NWPRT_TONEMAP_NONE, stream));
-    const size_t bytes = (size_t)width_ * height_ * 16;
-    hipStream_t s = stream ? stream : ownStream();
-    hipCheck(hipMemcpyAsync(f.fbPredictedIndirectWithDirectDenoised, f.fbPredictedIndirectWithDirect,
-                            bytes, hipMemcpyDeviceToDevice, s));
-    hipCheck(hipMemcpyAsync(f.fbPredictedIndirectDenoised, f.fbPredictedIndirect, bytes,
-                            hipMemcpyDeviceToDevice, s));
+    denoiseOrMoveData(frameID, config.finalDenoiser, f.fbPredictedIndirectWithDirect,
+                      f.fbPredictedIndirectWithDirectDenoised, 0.f, stream);
+    denoiseOrMoveData(frameID, config.indirectDenoiser, f.fbPredictedIndirect,
+                      f.fbPredictedIndirectDenoised, 0.f, stream);
+    denoised_ = config.finalDenoiser || config.indirectDenoiser;
   }
+
+  // RenderEngine::denoiseOrMoveData (RenderEngine.cpp:390-403): denoise
+  // `input` into `output` guided by frameID's normals and albedo, or copy.
+  void denoiseOrMoveData(int frameID, bool denoiserOn, const float *input, float *output,
+                         float blendFactor = 0.f, hipStream_t stream = nullptr) {
+    fullFrameBuffers &f = frame(frameID);
+    if (denoiserOn) {
+      check(nwprt_denoise(ctx_, input, f.fbworldNormal, f.fbFirstHitKd, output, width_, height_,
+                          blendFactor, &denoiseParams_, stream));
+    } else {
+      hipCheck(hipMemcpyAsync(output, input, (size_t)width_ * height_ * 16,
+                              hipMemcpyDeviceToDevice, stream ? stream : ownStream()));
+    }
+  }
+  // RenderEngine::denoise (RenderEngine.cpp:405-408) on caller images.
+  void denoise(const float *color, const float *normal, const float *albedo, float *output,
+               int width, int height, hipStream_t stream = nullptr) {
+    check(nwprt_denoise(ctx_, color, normal, albedo, output, width, height, 0.f,
+                        &denoiseParams_, stream));
+  }
+  void setDenoiseParams(const nwprt_denoise_params &p) { denoiseParams_ = p; }
 
   // RenderEngine::TCNNSync (tcnn.cu:154-158).
   void TCNNSync() {
@@ -203,7 +226,8 @@ class RenderEngine {
     hipCheck(hipMemcpy(withDirect.data(), frame(frameID).fbPredictedIndirectWithDirectDenoised,
                        n * 4, hipMemcpyDeviceToHost));
   }
-  bool denoiserApplied() const { return false; }
+  // Whether the last TCNNPass_p2 ran the denoiser on either output.
+  bool denoiserApplied() const { return denoised_; }
 
  private:
   static void check(nwprt_status s) {
@@ -248,6 +272,13 @@ class RenderEngine {
   int width_ = 0, height_ = 0;
   fullFrameBuffers frames_[2];
   std::vector<void *> allocations_;
+  nwprt_denoise_params denoiseParams_ = defaultDenoiseParams();
+  bool denoised_ = false;
+  static nwprt_denoise_params defaultDenoiseParams() {
+    nwprt_denoise_params p;
+    nwprt_denoise_default_params(&p);
+    return p;
+  }
 };
 
 }  // namespace nwprt

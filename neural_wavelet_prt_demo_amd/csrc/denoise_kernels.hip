@@ -85,22 +85,54 @@ __device__ __forceinline__ float sq3(float4 a, float4 b) {
   return x * x + y * y + z * z;
 }
 
-// One a-trous pass.  inv_c2 = 4^i / sc^2 (times kappa^2 on the device),
-// inv_n2 = 1 / sn^2, inv_a2 = 1 / sa^2.  LAST: blend with `orig` and write
-// `out`; otherwise write the pass result (alpha carried from `orig`).
+// One a-trous pass on an LDS tile.  With step s the taps of a pixel stay on
+// its residue class (x mod s, y mod s), where they are a dense 5 x 5
+// neighbourhood: workgroup (tx * s + rx, ty * s + ry) owns the 16 x 16
+// lattice points (tx * 16 + i, ty * 16 + j) of residue (rx, ry), i.e.
+// pixels (rx + (tx * 16 + i) s, ry + (ty * 16 + j) s), and first loads the
+// 20 x 20 halo of colour, normal and albedo into LDS.  Workgroups of the
+// same tile and neighbouring residues are adjacent in the grid, so they
+// share cache lines in L2.  The grid is 1-D, gx x gy logical workgroups
+// padded to a multiple of 8: the hardware hands workgroup b to XCD b % 8,
+// and logical workgroup (b % 8) * per + b / 8 keeps each XCD on one
+// contiguous range of tiles and residues (one L2 per XCD).  inv_c2 =
+// 4^i / sc^2 (times kappa^2 on the device), inv_n2 = 1 / sn^2, inv_a2 =
+// 1 / sa^2.  LAST: blend with `orig` and write `out`; otherwise write the
+// pass result (alpha from `orig`).
 template <bool LAST>
 __global__ __launch_bounds__(kTile * kTile) void atrous_kernel(
     const float4 *__restrict__ in, const float4 *__restrict__ orig,
     const float4 *__restrict__ normal, const float4 *__restrict__ albedo,
     const float *__restrict__ kappa, float4 *__restrict__ out, int w, int h, int step,
-    float inv_c2, float inv_n2, float inv_a2, float blend) {
-  const int x = blockIdx.x * kTile + threadIdx.x, y = blockIdx.y * kTile + threadIdx.y;
+    int gx, int gy, float inv_c2, float inv_n2, float inv_a2, float blend) {
+  constexpr int kHalo = kTile + 4;
+  __shared__ float4 sc[kHalo * kHalo], sn[kHalo * kHalo], sa[kHalo * kHalo];
+  const int per = (int)gridDim.x / 8;
+  const int lb = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (lb >= gx * gy) return;  // padding workgroup
+  const int bx = lb % gx, by = lb / gx;
+  const int rx = bx % step, tx = bx / step;
+  const int ry = by % step, ty = by / step;
+  const int x0 = rx + tx * kTile * step, y0 = ry + ty * kTile * step;  // lattice (0, 0)
+  if (x0 >= w || y0 >= h) return;  // whole workgroup outside: no barrier reached
+  const int tid = threadIdx.y * kTile + threadIdx.x;
+  for (int i = tid; i < kHalo * kHalo; i += kTile * kTile) {
+    const int qx = x0 + (i % kHalo - 2) * step, qy = y0 + (i / kHalo - 2) * step;
+    if (qx >= 0 && qx < w && qy >= 0 && qy < h) {
+      const size_t q = (size_t)qy * w + qx;
+      sc[i] = in[q];
+      sn[i] = normal[q];
+      sa[i] = albedo[q];
+    }
+  }
+  __syncthreads();
+  const int x = x0 + threadIdx.x * step, y = y0 + threadIdx.y * step;
   if (x >= w || y >= h) return;
-  const size_t p = (size_t)y * w + x;
   const float k = kappa[0];
   const float wc = k * k * inv_c2 * 1.44269504088896341f;  // exp(-d) = exp2(-d log2 e)
   const float wn = inv_n2 * 1.44269504088896341f, wa = inv_a2 * 1.44269504088896341f;
-  const float4 cp = in[p], np = normal[p], ap = albedo[p];
+  const int c0 = (threadIdx.y + 2) * kHalo + threadIdx.x + 2;
+  const float4 cp = sc[c0], np = sn[c0], ap = sa[c0];
   constexpr float kh[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
   float sr = 0.f, sg = 0.f, sb = 0.f, sw = 0.f;
 #pragma unroll
@@ -111,8 +143,8 @@ __global__ __launch_bounds__(kTile * kTile) void atrous_kernel(
     for (int dx = -2; dx <= 2; ++dx) {
       const int qx = x + dx * step;
       if (qx < 0 || qx >= w) continue;
-      const size_t q = (size_t)qy * w + qx;
-      const float4 cq = in[q], nq = normal[q], aq = albedo[q];
+      const int q = c0 + dy * kHalo + dx;
+      const float4 cq = sc[q], nq = sn[q], aq = sa[q];
       const float d = wc * sq3(cq, cp) + wn * sq3(nq, np) + wa * sq3(aq, ap);
       const float wt = kh[dx + 2] * kh[dy + 2] * __builtin_amdgcn_exp2f(-d);
       sr += wt * cq.x;
@@ -123,6 +155,7 @@ __global__ __launch_bounds__(kTile * kTile) void atrous_kernel(
   }
   // The centre tap has weight h0^2 > 0, so sw > 0.
   const float inv = 1.0f / sw;
+  const size_t p = (size_t)y * w + x;
   const float4 o = orig[p];
   float4 r;
   if (LAST) {
@@ -156,7 +189,6 @@ void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s) {
   hipLaunchKernelGGL(intensity_finish_kernel, dim3(1), dim3(kIntensityBlocks), 0, s,
                      (const double2 *)partials, kappa);
   hip_check(hipGetLastError(), "intensity_finish_kernel launch");
-  const dim3 grid((a.width + kTile - 1) / kTile, (a.height + kTile - 1) / kTile);
   const dim3 block(kTile, kTile);
   const float4 *orig = (const float4 *)a.color;
   const float4 *src = orig;
@@ -165,15 +197,20 @@ void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s) {
   for (int i = 0; i < a.iterations; ++i) {
     const float inv_c2 = (float)(1 << (2 * i)) / (a.sigma_color * a.sigma_color);
     const bool last = i == a.iterations - 1;
+    // Per residue class: ceil(ceil(W / s) / 16) x ceil(ceil(H / s) / 16) tiles.
+    const int st = 1 << i;
+    const int lw = (a.width + st - 1) / st, lh = (a.height + st - 1) / st;
+    const int gx = ((lw + kTile - 1) / kTile) * st, gy = ((lh + kTile - 1) / kTile) * st;
+    const dim3 grid((unsigned)((((long long)gx * gy + 7) / 8) * 8));
     float4 *dst = last ? (float4 *)a.out : (i % 2 == 0 ? t0 : t1);
     if (last)
       hipLaunchKernelGGL(atrous_kernel<true>, grid, block, 0, s, src, orig,
                          (const float4 *)a.normal, (const float4 *)a.albedo, kappa, dst, a.width,
-                         a.height, 1 << i, inv_c2, inv_n2, inv_a2, a.blend);
+                         a.height, st, gx, gy, inv_c2, inv_n2, inv_a2, a.blend);
     else
       hipLaunchKernelGGL(atrous_kernel<false>, grid, block, 0, s, src, orig,
                          (const float4 *)a.normal, (const float4 *)a.albedo, kappa, dst, a.width,
-                         a.height, 1 << i, inv_c2, inv_n2, inv_a2, a.blend);
+                         a.height, st, gx, gy, inv_c2, inv_n2, inv_a2, a.blend);
     hip_check(hipGetLastError(), "atrous_kernel launch");
     src = dst;
   }

@@ -157,3 +157,46 @@ def test_offline_trajectory_driver(nw, ctx, tmp_path):
     r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-m", str(env_dir),
                         "-t", str(bad)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "before trajectory start" in r.stderr
+
+
+def test_offline_driver_with_denoiser(nw, ctx, tmp_path):
+    """waveletFinalDenoiser true: the facade's TCNNPass_p2 denoises both
+    predictions (tcnn.cu:144-148) with the frame's normals and diffuse
+    albedo as guides; the written images equal a C-ABI relight followed by
+    nwprt_denoise with the default parameters."""
+    import subprocess
+    from pathlib import Path
+    from neural_wavelet_prt_demo_amd import DeviceBuffer
+    exe = Path(nw.__file__).resolve().with_name("nwprt_offline")
+    rng = np.random.default_rng(3)
+    n_face, K = 8, 16
+    _write_model_folder(tmp_path / "model", rng, n_face)
+    env_dir = tmp_path / "env"
+    (env_dir / "cube").mkdir(parents=True)
+    nw.hdr_write(env_dir / "cube" / "a.hdr", _faces_to_cross(nw.synth_cubemap(4, n_face)))
+    traj = tmp_path / "traj.txt"
+    traj.write_text(f"envmapFolder {env_dir}\nwaveletFinalDenoiser true\ntrajectoryBegin\n"
+                    "0 1 2 0 0 0 a\n")
+    out = tmp_path / "out"
+    r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-t", str(traj),
+                        "-k", str(K), "--seed", "20", "--precision", "bf16"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    scene = ctx.initTCNN(tmp_path / "model")
+    env = ctx.envmap_load_folder(env_dir, "a", K)
+    frame = nw.DeviceFrame(512, 512, nw.synth_gbuffer(20, 512, 512, 0.0))
+    ctx.relight(scene, env, frame, nw.PREC_BF16)
+    plain = None
+    for key, folder in (("indirect", "predictedIndirect"),
+                        ("indirect_with_direct", "predictedWithDirect")):
+        dn = DeviceBuffer(512 * 512 * 16)
+        ctx.denoise(frame.out[key], frame.bufs["normal"], frame.bufs["kd"], dn, 512, 512)
+        ctx.sync()
+        img = dn.download(np.float32, (512, 512, 4))
+        want = tmp_path / f"want_{key}.hdr"
+        nw.hdr_write(want, img[::-1])
+        got = nw.hdr_read(out / folder / "0_a.hdr")
+        np.testing.assert_array_equal(got, nw.hdr_read(want))
+        if key == "indirect":
+            plain = frame.results()[key].reshape(512, 512, 4)
+            assert not np.array_equal(img[..., :3], plain[..., :3])  # it did filter
