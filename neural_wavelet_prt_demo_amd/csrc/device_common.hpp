@@ -23,6 +23,17 @@ __device__ __forceinline__ uint32_t pack_f16(float lo, float hi) {
   return (uint32_t)__builtin_bit_cast(uint16_t, a) |
          ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
 }
+// pack_f16 of values that come out of an FMA chain, rounded the way the
+// code reads: first to FP32, then to FP16.  The empty asm keeps the
+// compiler from fusing the last FMA and the conversion into one
+// v_fma_mixlo/hi_f16, which rounds the exact result once and so differs
+// in rare cases -- and would make the same features depend on which kernel
+// computed them.  (Not for MFMA results: their reads need the compiler's
+// hazard tracking, which sees only real instructions.)
+__device__ __forceinline__ uint32_t pack_f16_rounded(float lo, float hi) {
+  asm volatile("" : "+v"(lo), "+v"(hi));
+  return pack_f16(lo, hi);
+}
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
   return __uint_as_float((uint32_t)h << 16);
 }

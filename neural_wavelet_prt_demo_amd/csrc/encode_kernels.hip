@@ -6,18 +6,13 @@
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
+#include "hashgrid_device.hpp"
 #include "kernels.hpp"
 
 namespace nwprt {
 namespace {
 
 constexpr int kEncodeBlock = 256;
-
-// Spatial hash of Instant-NGP (Mueller et al. 2022): x*1 ^ y*2654435761 ^
-// z*805459861 in uint32 arithmetic.
-__device__ __forceinline__ uint32_t grid_hash(uint32_t x, uint32_t y, uint32_t z) {
-  return x ^ (y * 2654435761u) ^ (z * 805459861u);
-}
 
 // Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
 // receives a contiguous run of pixel tiles, whose coarse-level hash entries
@@ -43,68 +38,10 @@ __device__ __forceinline__ void store_row(void *base, int p, const float *v) {
 #pragma unroll
       for (int d = 0; d < 4; ++d) {
         const float lo = v[8 * q + 2 * d], hi = v[8 * q + 2 * d + 1];
-        r[d] = OUT == Elem::BF16 ? dev::pack_bf16(lo, hi) : dev::pack_f16(lo, hi);
+        r[d] = OUT == Elem::BF16 ? dev::pack_bf16(lo, hi) : dev::pack_f16_rounded(lo, hi);
       }
       dst[q] = r;
     }
-  }
-}
-
-// Trilinear hash-grid features of level l at position x, added to acc[0..F).
-// Position pos = fmaf(x, scale_l, 0.5f) in FP32; corner c takes bit 0 / 1 / 2
-// for +x / +y / +z; dense levels index x + y res + z res^2 (mod size), hashed
-// ones grid_hash & (size - 1).  idx (may be null) receives the 8 indices.
-template <int F, bool HF16>
-__device__ __forceinline__ void accumulate_level(const EncodeArgs &a, int l, const float (&x)[3],
-                                                 float *acc, uint32_t *idx_out) {
-  const float scale = a.lv.scale[l];
-  const uint32_t res = a.lv.resolution[l], size = a.lv.size[l];
-  const bool dense = a.lv.dense[l] != 0;
-  const uint32_t off = a.lv.offset[l];
-  uint32_t gi[3];
-  float fr[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const float pos = fmaf(x[d], scale, 0.5f);
-    const float fl = floorf(pos);
-    gi[d] = (uint32_t)(int)fl;
-    fr[d] = pos - fl;
-  }
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const uint32_t cx = gi[0] + (c & 1), cy = gi[1] + ((c >> 1) & 1), cz = gi[2] + ((c >> 2) & 1);
-    uint32_t idx;
-    if (dense) {
-      idx = cx + cy * res + cz * res * res;
-      if (idx >= size) idx %= size;
-    } else {
-      idx = grid_hash(cx, cy, cz) & (size - 1u);
-    }
-    const float w = ((c & 1) ? fr[0] : 1.0f - fr[0]) * ((c & 2) ? fr[1] : 1.0f - fr[1]) *
-                    ((c & 4) ? fr[2] : 1.0f - fr[2]);
-    const size_t e = ((size_t)off + idx) * F;
-    if (HF16) {
-      const uint16_t *h = (const uint16_t *)a.hash + e;
-      if (F == 2) {
-        const uint32_t two = *(const uint32_t *)h;
-        acc[0] = fmaf(w, dev::f16_to_f32((uint16_t)(two & 0xffffu)), acc[0]);
-        acc[1] = fmaf(w, dev::f16_to_f32((uint16_t)(two >> 16)), acc[1]);
-      } else {
-#pragma unroll
-        for (int j = 0; j < F; ++j) acc[j] = fmaf(w, dev::f16_to_f32(h[j]), acc[j]);
-      }
-    } else {
-      const float *h = (const float *)a.hash + e;
-      if (F == 2) {
-        const float2 two = *(const float2 *)h;
-        acc[0] = fmaf(w, two.x, acc[0]);
-        acc[1] = fmaf(w, two.y, acc[1]);
-      } else {
-#pragma unroll
-        for (int j = 0; j < F; ++j) acc[j] = fmaf(w, h[j], acc[j]);
-      }
-    }
-    if (idx_out) idx_out[c] = idx;
   }
 }
 
@@ -125,14 +62,15 @@ __global__ __launch_bounds__(kEncodeBlock) void encode_planar_kernel(EncodeArgs 
   if (hit) {
     const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)p);
     const float x[3] = {x4.x, x4.y, x4.z};
-    accumulate_level<2, HF16>(a, q, x, v, nullptr);
-    accumulate_level<2, HF16>(a, 15 - q, x, v + 2, nullptr);
+    dev::accumulate_level<2, HF16>(a.lv, a.hash, q, x, v, nullptr);
+    dev::accumulate_level<2, HF16>(a.lv, a.hash, 15 - q, x, v + 2, nullptr);
   }
   const size_t at = (size_t)q * a.n_pixels + p;
   if (OUT == Elem::BF16)
     ((uint2 *)a.features)[at] = make_uint2(dev::pack_bf16(v[0], v[1]), dev::pack_bf16(v[2], v[3]));
   else
-    ((uint2 *)a.features)[at] = make_uint2(dev::pack_f16(v[0], v[1]), dev::pack_f16(v[2], v[3]));
+    ((uint2 *)a.features)[at] =
+        make_uint2(dev::pack_f16_rounded(v[0], v[1]), dev::pack_f16_rounded(v[2], v[3]));
 }
 
 template <int F, bool HF16, Elem OUT, bool DBG>
@@ -157,7 +95,7 @@ __global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
     const float x[3] = {x4.x, x4.y, x4.z};
 #pragma unroll
     for (int l = 0; l < L; ++l)
-      accumulate_level<F, HF16>(a, l, x, feat + l * F,
+      dev::accumulate_level<F, HF16>(a.lv, a.hash, l, x, feat + l * F,
                                 DBG ? a.dbg_indices + ((size_t)p * L + l) * 8 : nullptr);
     if (DBG && a.dbg_features) {
 #pragma unroll

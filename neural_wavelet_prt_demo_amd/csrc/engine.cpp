@@ -295,10 +295,11 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
                        const nwprt_gbuffer *gb, int w, int h, const nwprt_frame_out *out,
                        nwprt_precision prec, nwprt_tonemap_source tm, hipStream_t s,
                        float *dump = nullptr, int dump_begin = 0, int dump_count = 0,
-                       int variant = -1) {
+                       int variant = -1, bool fused = false) {
   const Elem elem = elem_of(prec);
   const int P = w * h;
-  require(slot.valid && slot.scene == scene_id && slot.n_pixels == P && slot.elem == elem,
+  require(fused || (slot.valid && slot.scene == scene_id && slot.n_pixels == P &&
+                    slot.elem == elem),
           NWPRT_ERR_STATE,
           "pass p2: frame slot was not encoded (p1) for this scene, size and precision");
   require(env.n == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
@@ -314,10 +315,20 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   RelightArgs a{};
   a.n_pixels = P;
   a.k = env.k;
-  a.features = slot.features.p;
+  a.features = fused ? nullptr : slot.features.p;
   a.features_planar = planar;
   a.light = env.strength.as<float>();
   a.env_flag = gb->env_flag;
+  if (fused) {
+    require(planar && prec != NWPRT_PREC_FP8, NWPRT_ERR_INVALID_ARGUMENT,
+            "relight: fused encode needs an MFMA form and a 16 x 2 grid without aux inputs");
+    check_aligned(gb->position, 16, "G-buffer position");
+    a.fused_encode = 1;
+    a.position = gb->position;
+    a.hash = sc.hash.p;
+    a.hash_f16 = sc.desc.hash_dtype == NWPRT_DTYPE_F16;
+    a.lv = sc.lv;
+  }
   a.direct = gb->direct;
   a.indirect = out ? out->indirect : nullptr;
   a.with_direct = out ? out->indirect_with_direct : nullptr;
@@ -774,6 +785,12 @@ nwprt_status nwprt_envmap_get_coefficients(nwprt_ctx ctx, int32_t env_id, float 
   });
 }
 
+// One-call MFMA relights of 16 x 2 grids without aux inputs encode inside
+// the fused kernel (each wave its own pixels) instead of a separate pass.
+bool fuse_encode(const Scene &sc, nwprt_precision prec) {
+  return prec != NWPRT_PREC_FP8 && elem_of(prec) != Elem::F32 && planar_features(sc, elem_of(prec));
+}
+
 nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
                            const nwprt_gbuffer *gbuffer, int32_t width, int32_t height,
                            const nwprt_frame_out *out, nwprt_precision precision,
@@ -785,9 +802,13 @@ nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
     check_frame(gbuffer, width, height, true);
     const Elem elem = elem_of(precision);
     const hipStream_t s = pick(ctx, stream);
-    encode_into(ctx, sc, scene_id, ctx->slots[2], gbuffer, width, height, elem, s);
+    const bool fused = fuse_encode(sc, precision);
+    if (fused)
+      ctx->slots[2].valid = false;
+    else
+      encode_into(ctx, sc, scene_id, ctx->slots[2], gbuffer, width, height, elem, s);
     relight_from_slot(ctx, sc, scene_id, env, ctx->slots[2], gbuffer, width, height, out,
-                      precision, tonemap, s);
+                      precision, tonemap, s, nullptr, 0, 0, -1, fused);
   });
 }
 
@@ -865,7 +886,12 @@ nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env
             "autotune: the plain FP32 form has no tile variants");
     const hipStream_t s = ctx->stream;
     FrameSlot &slot = ctx->slots[2];
-    encode_into(ctx, sc, scene_id, slot, gbuffer, width, height, elem, s);
+    // Tune the form nwprt_relight runs (with the fused encode where it applies).
+    const bool fused = fuse_encode(sc, precision);
+    if (fused)
+      slot.valid = false;
+    else
+      encode_into(ctx, sc, scene_id, slot, gbuffer, width, height, elem, s);
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -874,12 +900,12 @@ nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env
     try {
       for (int v = 0; v < variant_count(precision); ++v) {
         relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr,
-                          precision, NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
+                          precision, NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v, fused);
         float t = 0.f;
         for (int rep = 0; rep < 3; ++rep) {
           hip_check(hipEventRecord(e0, s), "hipEventRecord");
           relight_from_slot(ctx, sc, scene_id, env, slot, gbuffer, width, height, nullptr,
-                            precision, NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v);
+                            precision, NWPRT_TONEMAP_NONE, s, nullptr, 0, 0, v, fused);
           hip_check(hipEventRecord(e1, s), "hipEventRecord");
           hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
           float ms = 0.f;

@@ -119,6 +119,14 @@ struct RelightArgs {
   const uint8_t *w8_tiles;
   int w8_exp[3];
   float *amax;
+  // Fused encode (one-call MFMA relights of 16 x 2 grids without aux
+  // inputs): each wave computes its 64 pixels' hash-grid features from
+  // `position` in the planar slot order instead of reading `features`.
+  int fused_encode;
+  const float *position;      // float4 per pixel
+  const void *hash;           // entries x 2
+  int hash_f16;
+  nwprt_hashgrid_levels lv;
 };
 // Tile-shape variants of the fused kernel (see relight_kernels.hip).
 int relight_mfma_variant_count();

@@ -20,7 +20,10 @@
 // which the host folds into the weight tiles (build_weight_tiles).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
+#include "hashgrid_device.hpp"
 #include "kernels.hpp"
 #include "relight_device.hpp"
 
@@ -258,9 +261,51 @@ __device__ __forceinline__ f32x4 wavelet_step_pipelined(
 // are fetched while the current one computes (two register sets, K loop
 // unrolled by two).  GP = 0 selects the software-pipelined step; W4LDS reads
 // the output layer's tiles from `w4s` (filled by the caller).
-template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS>
+// Fused encode of a wave's 64 pixels into the planar B operands.  Lane l
+// computes the 16 levels of pixel base + l (level parameters uniform across
+// the wave, as in the standalone encode) with the same lookup code, packs
+// each level's two features, and stores them in the planar slot order
+// (u32 2q = level q, 2q + 1 = level 15 - q) to its 64 B of `lds`; lane
+// (g, n) then reads slots 4g .. 4g + 3 of pixel 16 j + n for group j -- the
+// words load_features8 reads from the planar feature planes.  Pixels that
+// are past the frame or not hit get zero features.
+template <typename E>
+__device__ __forceinline__ void fused_features(const RelightArgs &a, bool active, int my_px,
+                                               int lane, int g, int n, uint32_t *lds,
+                                               u32x4 (&hb)[4]) {
+  uint32_t pk[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) pk[i] = 0u;
+  if (active) {
+    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)my_px);
+    const float x[3] = {x4.x, x4.y, x4.z};
+#pragma unroll
+    for (int l = 0; l < 16; ++l) {
+      float v[2] = {0.f, 0.f};
+      if (a.hash_f16)
+        dev::accumulate_level<2, true>(a.lv, a.hash, l, x, v, nullptr);
+      else
+        dev::accumulate_level<2, false>(a.lv, a.hash, l, x, v, nullptr);
+      pk[l < 8 ? 2 * l : 2 * (15 - l) + 1] = std::is_same<E, _Float16>::value
+                                                  ? dev::pack_f16_rounded(v[0], v[1])
+                                                  : dev::pack_bf16(v[0], v[1]);
+    }
+  }
+  u32x4 *row = (u32x4 *)(lds + lane * 16);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) row[c] = u32x4{pk[4 * c], pk[4 * c + 1], pk[4 * c + 2], pk[4 * c + 3]};
+  // Lanes of one wave exchange through LDS: order the stores before the loads.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int j = 0; j < 4; ++j) hb[j] = *(const u32x4 *)(lds + (16 * j + n) * 16 + 4 * g);
+}
+
+// FUSE: the fused-encode path is compiled in (taken when a.fused_encode).
+template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, bool FUSE>
 __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int lane,
-                                             const u32x4 *w4s) {
+                                             const u32x4 *w4s, uint32_t *feat_lds) {
   const int g = lane >> 4, n = lane & 15;
   const u32x4 *wt = (const u32x4 *)a.w_tiles;
   const int P = a.n_pixels;
@@ -273,10 +318,14 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
   if (any_hit) {
     // Per-pixel features as B operands: group j, column n, k-slots 8g..8g+7.
     u32x4 hb[4];
+    if (FUSE && a.fused_encode) {
+      fused_features<E>(a, mine && hit, my_px, lane, g, n, feat_lds, hb);
+    } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int p = base + 16 * j + n;
-      hb[j] = p < P ? dev::load_features8(a, P, p, 8 * g) : u32x4{0u, 0u, 0u, 0u};
+      for (int j = 0; j < 4; ++j) {
+        const int p = base + 16 * j + n;
+        hb[j] = p < P ? dev::load_features8(a, P, p, 8 * g) : u32x4{0u, 0u, 0u, 0u};
+      }
     }
     // Aux inputs (B: group j, column n, k-slots 4g..4g+3) and the second
     // layer's aux columns (A tiles [4][64]).
@@ -354,6 +403,7 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
 template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, int MINW>
 __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
   __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
+  __shared__ uint32_t feat_lds[AUX ? 1 : 4 * 64 * 16];  // fused encode: [wave][pixel][16]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (W4LDS) {
     const u32x4 *wt = (const u32x4 *)a.w_tiles;
@@ -362,7 +412,8 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
   }
   const int base = (blockIdx.x * 4 + wave) * 64;
   if (base >= a.n_pixels) return;
-  relight_wave<E, SOFTPLUS, DUMP, AUX, GP, W4LDS>(a, base, lane, w4s);
+  relight_wave<E, SOFTPLUS, DUMP, AUX, GP, W4LDS, !AUX>(a, base, lane, w4s,
+                                                       feat_lds + (AUX ? 0 : wave * 64 * 16));
 }
 
 // Persistent batch form: a machine-sized grid whose waves take work items
@@ -386,7 +437,8 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
     const RelightArgs a = jobs[item.job];
     // One 64-pixel group per item: a loop over several would let the
     // compiler hoist the group's loads and spill.
-    relight_wave<E, SOFTPLUS, false, false, GP, false>(a, item.base, lane, nullptr);
+    relight_wave<E, SOFTPLUS, false, false, GP, false, false>(a, item.base, lane, nullptr,
+                                                              nullptr);
   }
 }
 
@@ -705,9 +757,12 @@ void build_aux_tiles(const float *waux, Elem elem, std::vector<uint16_t> &out) {
 void launch_relight_mfma(const RelightArgs &a, Elem elem, int variant, hipStream_t s) {
   require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: bad pixel count or K");
-  require(a.features && a.mk && a.light && a.w_tiles && a.direct, NWPRT_ERR_INVALID_ARGUMENT,
-          "relight: missing operand");
+  require((a.fused_encode ? a.position && a.hash : a.features != nullptr) && a.mk && a.light &&
+              a.w_tiles && a.direct,
+          NWPRT_ERR_INVALID_ARGUMENT, "relight: missing operand");
   require(!a.aux || a.waux_tiles, NWPRT_ERR_INVALID_ARGUMENT, "relight: missing aux weights");
+  require(!a.fused_encode || (!a.aux && a.lv.n_levels == 16), NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: the fused encode needs 16 x 2 grids without aux inputs");
   if (elem == Elem::BF16) {
     launch_mfma_variant<__bf16>(a, variant, s);
   } else if (elem == Elem::F16) {

@@ -141,16 +141,30 @@ def test_ragged_frames(nw, orc, ctx, shape, precision):
     assert max_rel_err(ind, res["indirect"][:, :3]) <= tol
 
 
-def test_split_passes_match_relight(nw, orc, ctx):
-    case = make_case(nw, orc, width=96, height=40)
-    scene, env, frame, res = _gpu_relight(nw, ctx, case, "bf16")
+@pytest.mark.parametrize("precision,hash_f16,shape,background",
+                         [("bf16", False, (96, 40), 0.0), ("bf16", True, (97, 41), 0.3),
+                          ("fp16", False, (33, 7), 0.2), ("fp16", True, (64, 64), 0.0)])
+def test_split_passes_match_relight(nw, orc, ctx, precision, hash_f16, shape, background):
+    """nwprt_relight encodes inside the fused kernel (each wave its own
+    pixels); p1 + p2 run the separate planar encode.  Both give the same
+    outputs bit for bit."""
+    w, h = shape
+    case = make_case(nw, orc, width=w, height=h, background=background,
+                     hash_dtype=nw.DTYPE_F16 if hash_f16 else nw.DTYPE_F32)
+    prec = nw.PRECISIONS[precision]
+    scene, env, frame, res = _gpu_relight(nw, ctx, case, precision)
     frame2 = nw.DeviceFrame(case.width, case.height, case.gbuf)
-    ctx.TCNNPass_p1(scene, 1, frame2, nw.PREC_BF16)
-    ctx.TCNNPass_p2(scene, env, 1, frame2, nw.PREC_BF16)
+    ctx.TCNNPass_p1(scene, 1, frame2, prec)
+    ctx.TCNNPass_p2(scene, env, 1, frame2, prec)
     ctx.sync()
     res2 = frame2.results()
     for key in res:
-        assert np.array_equal(res[key], res2[key]), key
+        a, b = res[key], res2[key]
+        bad = ~((a == b) | (np.isnan(a.astype(np.float64)) & np.isnan(b.astype(np.float64))))
+        where = np.argwhere(bad)[:5].tolist()
+        assert not bad.any(), (key, int(bad.sum()), where,
+                               [(a[tuple(i)], b[tuple(i)]) for i in where],
+                               bool(np.isfinite(a.astype(np.float64)).all()))
     # p2 on a slot encoded for another precision is a call-order error.
     with pytest.raises(nw.NwprtError) as e:
         ctx.TCNNPass_p2(scene, env, 1, frame2, nw.PREC_FP32)
