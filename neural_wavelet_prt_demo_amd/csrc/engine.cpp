@@ -102,6 +102,21 @@ bool planar_features(const Scene &sc, Elem elem) {
          sc.desc.grid.n_features_per_level == 2;
 }
 
+bool fuse_encode(const Scene &sc, nwprt_precision prec) {
+  return prec != NWPRT_PREC_FP8 && elem_of(prec) != Elem::F32 && planar_features(sc, elem_of(prec));
+}
+
+void set_fused_encode(RelightArgs &a, const Scene &sc, const nwprt_gbuffer *gb) {
+  check_aligned(gb->position, 16, "G-buffer position");
+  a.features = nullptr;
+  a.features_planar = 1;
+  a.fused_encode = 1;
+  a.position = gb->position;
+  a.hash = sc.hash.p;
+  a.hash_f16 = sc.desc.hash_dtype == NWPRT_DTYPE_F16;
+  a.lv = sc.lv;
+}
+
 const float *scene_projection(const Scene &sc) {
   return sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
 }
@@ -320,14 +335,9 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   a.light = env.strength.as<float>();
   a.env_flag = gb->env_flag;
   if (fused) {
-    require(planar && prec != NWPRT_PREC_FP8, NWPRT_ERR_INVALID_ARGUMENT,
+    require(fuse_encode(sc, prec), NWPRT_ERR_INVALID_ARGUMENT,
             "relight: fused encode needs an MFMA form and a 16 x 2 grid without aux inputs");
-    check_aligned(gb->position, 16, "G-buffer position");
-    a.fused_encode = 1;
-    a.position = gb->position;
-    a.hash = sc.hash.p;
-    a.hash_f16 = sc.desc.hash_dtype == NWPRT_DTYPE_F16;
-    a.lv = sc.lv;
+    set_fused_encode(a, sc, gb);
   }
   a.direct = gb->direct;
   a.indirect = out ? out->indirect : nullptr;
@@ -432,9 +442,10 @@ void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
       const nwprt_batch_job &jb = b.jobs[j];
       const Scene &sc = get_scene(ctx, jb.scene_id);
       const EnvMap &env = get_env(ctx, jb.env_id);
-      launch_encode(encode_args(sc, &jb.gbuffer, b.p[j], elem,
-                                b.features.as<uint16_t>() + b.feat_off[j]),
-                    s);
+      if (!b.fused[j])
+        launch_encode(encode_args(sc, &jb.gbuffer, b.p[j], elem,
+                                  b.features.as<uint16_t>() + b.feat_off[j]),
+                      s);
       launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * env.n * env.n,
                                env.idx.as<int32_t>(), b.k[j], elem,
                                b.mk.as<uint16_t>() + b.mk_off[j], nullptr, s, nullptr,
@@ -486,17 +497,19 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     b->jobs.push_back(jb);
     b->k.push_back(env.k);
     b->p.push_back(P);
+    const bool fused = fuse_encode(sc, prec);
+    b->fused.push_back(fused);
     b->feat_off.push_back(fo);
     b->mk_off.push_back(mo);
     b->tiles_at.push_back(scene_tiles(sc, prec));
     b->idx_at.push_back(env.idx.p);
-    fo += (size_t)P * 32;
+    if (!fused) fo += (size_t)P * 32;
     mo += (size_t)env.k * 4 * 64 * 8;
     b->n_items += (P + 63) / 64;
     b->n_pairs += (long long)P * env.k;
   }
   require(b->n_items < (1ll << 31), NWPRT_ERR_INVALID_ARGUMENT, "batch: too many pixels");
-  b->features.ensure(fo * 2);
+  if (fo) b->features.ensure(fo * 2);
   b->mk.ensure(mo * 2);
   // Device job table.
   std::vector<RelightArgs> args(n);
@@ -510,6 +523,7 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     a.k = b->k[j];
     a.features = b->features.as<uint16_t>() + b->feat_off[j];
     a.features_planar = planar_features(sc, elem_of(prec));
+    if (b->fused[j]) set_fused_encode(a, sc, &jb.gbuffer);
     a.mk = b->mk.as<uint16_t>() + b->mk_off[j];
     a.light = env.strength.as<float>();
     a.w_tiles = (const uint16_t *)scene_tiles(sc, prec);
@@ -783,12 +797,6 @@ nwprt_status nwprt_envmap_get_coefficients(nwprt_ctx ctx, int32_t env_id, float 
     hip_check(hipMemcpy(coefficients, e.coef.p, (size_t)6 * e.n * e.n * 12, hipMemcpyDeviceToHost),
               "hipMemcpy");
   });
-}
-
-// One-call MFMA relights of 16 x 2 grids without aux inputs encode inside
-// the fused kernel (each wave its own pixels) instead of a separate pass.
-bool fuse_encode(const Scene &sc, nwprt_precision prec) {
-  return prec != NWPRT_PREC_FP8 && elem_of(prec) != Elem::F32 && planar_features(sc, elem_of(prec));
 }
 
 nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,

@@ -93,6 +93,7 @@ struct Batch {
   bool softplus = true;
   std::vector<nwprt_batch_job> jobs;
   std::vector<int> k, p;
+  std::vector<char> fused;  // job encodes inside the persistent kernel
   std::vector<size_t> feat_off, mk_off;  // in elements of 2 bytes
   // What the graph captured of each job's scene and map: a launch checks
   // that these buffers are still the ones in use.
@@ -175,5 +176,9 @@ const void *scene_tiles(const Scene &sc, nwprt_precision prec);
 bool planar_features(const Scene &sc, Elem elem);
 // Projection to fold into the selection tiles (null: none).
 const float *scene_projection(const Scene &sc);
+// Whether a relight of this scene at this precision encodes inside the
+// fused kernel (MFMA forms, planar features), and the arguments that do it.
+bool fuse_encode(const Scene &sc, nwprt_precision prec);
+void set_fused_encode(RelightArgs &a, const Scene &sc, const nwprt_gbuffer *gb);
 
 }  // namespace nwprt

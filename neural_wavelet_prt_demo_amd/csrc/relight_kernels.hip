@@ -420,13 +420,16 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
 // (job, first pixel of a 64-pixel group) from a shared counter until the
 // list is exhausted.  Jobs are whole relights with their own frame,
 // scene weights, selection and K; items are listed largest first so the
-// tail of the launch is short items.  Output-layer tiles stay in VGPRs (no
-// LDS, so no workgroup barrier and no scene switch cost beyond reloading).
+// tail of the launch is short items.  Output-layer tiles stay in VGPRs, so
+// there is no workgroup barrier and no scene switch cost beyond reloading;
+// jobs with the fused encode use their wave's own 4 KB of LDS.
 template <typename E, bool SOFTPLUS, int GP, int MINW>
 __global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
     const RelightArgs *__restrict__ jobs, const BatchItem *__restrict__ items,
     const int *__restrict__ n_items_p, unsigned int *__restrict__ counter) {
+  __shared__ uint32_t feat_lds[4 * 64 * 16];  // fused encode: [wave][pixel][16]
   const int lane = threadIdx.x & 63;
+  uint32_t *my_lds = feat_lds + (threadIdx.x >> 6) * 64 * 16;
   const int n_items = *n_items_p;
   for (;;) {
     int it = 0;
@@ -437,8 +440,8 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
     const RelightArgs a = jobs[item.job];
     // One 64-pixel group per item: a loop over several would let the
     // compiler hoist the group's loads and spill.
-    relight_wave<E, SOFTPLUS, false, false, GP, false, false>(a, item.base, lane, nullptr,
-                                                              nullptr);
+    relight_wave<E, SOFTPLUS, false, false, GP, false, true>(a, item.base, lane, nullptr,
+                                                             my_lds);
   }
 }
 

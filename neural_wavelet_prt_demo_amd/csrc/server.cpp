@@ -14,7 +14,8 @@
 // preparation event before the first tick that holds its items.
 //
 // Device state per admitted request comes from a pool of exact-size blocks
-// (features, selection tiles, and for rotated / fresh maps the cubemap,
+// (selection tiles; per-pixel features only for scenes the persistent
+// kernel cannot encode itself; and for rotated / fresh maps the cubemap,
 // coefficients, scores, indices and strengths), so steady-state serving
 // allocates nothing.  The budget counts the blocks admitted requests hold.
 #include <hip/hip_runtime.h>
@@ -84,6 +85,7 @@ struct Request {
   nwprt_request r{};
   double t_submit = 0.0;
   int P = 0, K = 0, N = 0;
+  bool fused = false;  // encoded inside the persistent kernel
   int slot = -1;
   int next_base = 0;          // first pixel of the next unissued work item
   bool waited = false;        // launch stream already waits for its preparation
@@ -155,7 +157,8 @@ bool env_built(const nwprt_request &r) { return r.env_source != NWPRT_ENV_RESIDE
 
 // Device bytes a request holds while admitted.
 std::vector<size_t> block_sizes(const Request &q) {
-  std::vector<size_t> s = {(size_t)q.P * 64, (size_t)q.K * 4 * 64 * 16};
+  // Features: none when the persistent kernel encodes the request's pixels.
+  std::vector<size_t> s = {q.fused ? 0 : (size_t)q.P * 64, (size_t)q.K * 4 * 64 * 16};
   if (env_built(q.r)) {
     const size_t m = 6ull * q.N * q.N;
     s.insert(s.end(), {m * 12, m * 12, m * 4, (size_t)q.K * 4, (size_t)q.K * 12});
@@ -173,6 +176,7 @@ void validate_request(Server &sv, const nwprt_request &r, Request &q) {
   require(!sv.softplus_set || sp == sv.softplus, NWPRT_ERR_UNSUPPORTED,
           "server: all requests must share the output activation");
   q.P = r.width * r.height;
+  q.fused = fuse_encode(sc, sv.cfg.precision);
   switch (r.env_source) {
     case NWPRT_ENV_RESIDENT: {
       const EnvMap &e = get_env(sv.ctx, r.env_id);
@@ -213,8 +217,9 @@ void validate_request(Server &sv, const nwprt_request &r, Request &q) {
 }
 
 // Preparation of an admitted request on the preparation stream: its map's
-// selection, the hash-grid encode, the selection tiles and its job-table
-// entry; then the slot's event.
+// selection, the hash-grid encode (unless the persistent kernel encodes the
+// pixels itself), the selection tiles and its job-table entry; then the
+// slot's event.
 void prepare(Server &sv, Request &q) {
   const hipStream_t s = sv.prep;
   const Elem elem = elem_of(sv.cfg.precision);
@@ -248,7 +253,7 @@ void prepare(Server &sv, Request &q) {
     idx = bidx;
     strength = bstr;
   }
-  launch_encode(encode_args(sc, &q.r.gbuffer, q.P, elem, features), s);
+  if (!q.fused) launch_encode(encode_args(sc, &q.r.gbuffer, q.P, elem, features), s);
   launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * q.N * q.N, idx, q.K,
                            elem, mk, nullptr, s, nullptr, planar_features(sc, elem),
                            scene_projection(sc));
@@ -258,6 +263,7 @@ void prepare(Server &sv, Request &q) {
   a.k = q.K;
   a.features = features;
   a.features_planar = planar_features(sc, elem);
+  if (q.fused) set_fused_encode(a, sc, &q.r.gbuffer);
   a.mk = mk;
   a.light = strength;
   a.w_tiles = (const uint16_t *)scene_tiles(sc, sv.cfg.precision);
@@ -317,6 +323,10 @@ void admit(Server &sv) {
     if (sv.cfg.memory_budget_bytes && sv.stats.bytes_in_use + bytes > sv.cfg.memory_budget_bytes)
       break;
     for (size_t n : sizes) {
+      if (n == 0) {
+        q.blocks.push_back(Block{nullptr, 0});
+        continue;
+      }
       void *p = sv.pool.get(n);
       if (!p) {  // return idle blocks of other sizes to the device and retry
         sv.pool.trim();
