@@ -269,6 +269,22 @@ __device__ __forceinline__ f32x4 wavelet_step_pipelined(
 // (g, n) then reads slots 4g .. 4g + 3 of pixel 16 j + n for group j -- the
 // words load_features8 reads from the planar feature planes.  Pixels that
 // are past the frame or not hit get zero features.
+// Levels l0 .. l0 + NL - 1 of one pixel, packed into the planar slot order
+// (u32 2q = level q, 2q + 1 = level 15 - q).
+template <typename E, int NL, bool HF16>
+__device__ __forceinline__ void encode_batch(const RelightArgs &a, int l0, const float (&x)[3],
+                                             uint32_t (&pk)[16]) {
+  float v[NL][2];
+  dev::gather_levels<NL, HF16>(a.lv, a.hash, l0, x, v);
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int l = l0 + i;
+    pk[l < 8 ? 2 * l : 2 * (15 - l) + 1] = std::is_same<E, _Float16>::value
+                                                ? dev::pack_f16_rounded(v[i][0], v[i][1])
+                                                : dev::pack_bf16(v[i][0], v[i][1]);
+  }
+}
+
 template <typename E>
 __device__ __forceinline__ void fused_features(const RelightArgs &a, bool active, int my_px,
                                                int lane, int g, int n, uint32_t *lds,
@@ -279,16 +295,18 @@ __device__ __forceinline__ void fused_features(const RelightArgs &a, bool active
   if (active) {
     const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)my_px);
     const float x[3] = {x4.x, x4.y, x4.z};
+    // Four levels per batch: 32 gathers in flight before the first is used
+    // (FP16 entries; two levels of FP32 entries per half-batch).  The table
+    // type is tested per batch: that branch also keeps the compiler from
+    // hoisting the address math of all 16 levels at once (which spills).
 #pragma unroll
-    for (int l = 0; l < 16; ++l) {
-      float v[2] = {0.f, 0.f};
-      if (a.hash_f16)
-        dev::accumulate_level<2, true>(a.lv, a.hash, l, x, v, nullptr);
-      else
-        dev::accumulate_level<2, false>(a.lv, a.hash, l, x, v, nullptr);
-      pk[l < 8 ? 2 * l : 2 * (15 - l) + 1] = std::is_same<E, _Float16>::value
-                                                  ? dev::pack_f16_rounded(v[0], v[1])
-                                                  : dev::pack_bf16(v[0], v[1]);
+    for (int l0 = 0; l0 < 16; l0 += 4) {
+      if (a.hash_f16) {
+        encode_batch<E, 4, true>(a, l0, x, pk);
+      } else {
+        encode_batch<E, 2, false>(a, l0, x, pk);
+        encode_batch<E, 2, false>(a, l0 + 2, x, pk);
+      }
     }
   }
   u32x4 *row = (u32x4 *)(lds + lane * 16);
