@@ -113,12 +113,14 @@ constexpr int kTopkBlock = 1024;
 // function relative to the face, 4^-j for a detail at Mallat position with
 // max(row, col) in [2^j, 2^(j+1)) and 1 for the DC term.  Scores below the
 // smallest normal float and NaNs become 0.
+// n is a power of two (face sizes are), so the position math is shifts.
 __device__ __forceinline__ uint32_t score_bits(const float *__restrict__ coef, int t, int n) {
   const float r = coef[3 * (size_t)t], g = coef[3 * (size_t)t + 1], b = coef[3 * (size_t)t + 2];
   const float lum = __fadd_rn(__fadd_rn(__fmul_rn(r, 0.212671f), __fmul_rn(g, 0.715160f)),
                               __fmul_rn(b, 0.072169f));
-  const int q = t % (n * n);
-  const int row = q / n, col = q % n;
+  const int log2n = __ffs(n) - 1;
+  const int q = t & (n * n - 1);
+  const int row = q >> log2n, col = q & (n - 1);
   const int m = row > col ? row : col;
   const int j = m == 0 ? 0 : 31 - __clz(m);
   float s = ldexpf(fabsf(lum), -2 * j);
@@ -133,89 +135,164 @@ __device__ __forceinline__ uint64_t topk_key(const uint32_t *scores, int t) {
   return ((uint64_t)scores[t] << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)t);
 }
 
-// One workgroup: scores -> MSB-first radix select of the K-th largest key
-// (8-bit digits, early exit once a whole bucket is taken) -> compaction ->
-// bitonic sort of the K selected keys in LDS (descending).
+// Bucket of the digit at `shift` that holds the need-th largest remaining
+// key (need = *s_need on entry): with suf[d] = sum of hist[d..255], the one
+// bucket d with suf[d] >= need > suf[d + 1] extends the prefix, and *s_done
+// is set when that whole bucket is taken.  Thread t < 256 takes bucket
+// 255 - t, so suffix sums over buckets are prefix sums over threads: a
+// shuffle scan within each wave, then the totals of the waves before it
+// (`wt`, 4 words of LDS).  Every thread of the workgroup calls it, after a
+// barrier that made `hist` visible.
+__device__ void select_digit(const uint32_t *hist, uint32_t *wt, int shift, uint64_t *s_prefix,
+                             uint64_t *s_mask, uint32_t *s_need, int *s_done) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const uint32_t need = *s_need;
+  uint32_t v = 0, p = 0;
+  if (tid < 256) {
+    v = hist[255 - tid];
+    p = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t u = __shfl_up(p, off, 64);
+      if (lane >= off) p += u;
+    }
+    if (lane == 63) wt[tid >> 6] = p;
+  }
+  __syncthreads();
+  if (tid < 256) {
+    for (int w = 0; w < (tid >> 6); ++w) p += wt[w];
+    // p = suf[255 - tid]; p - v = suf[256 - tid] (0 for the top bucket).
+    if (p >= need && p - v < need) {
+      const uint32_t d = 255u - (uint32_t)tid;
+      *s_need = need - (p - v);
+      *s_prefix |= (uint64_t)d << shift;
+      *s_mask |= 0xFFull << shift;
+      *s_done = v == need - (p - v);
+    }
+  }
+  __syncthreads();
+}
+
+// One workgroup: MSB-first radix select of the K-th largest key (8-bit
+// digits, early exit once a whole bucket is taken) -> compaction -> rank
+// sort of the K selected keys in LDS (descending).
+// - Scores come from score_kernel.  The first digit's histogram is built in
+//   one private copy per wave (keys bunch up in few exponent buckets, and
+//   one shared copy would serialise their atomics).
+// - Keys above the chosen first-digit bucket are taken at once; keys in it
+//   (at most kTopkCand of them) move to LDS, and the remaining digits are
+//   resolved there.  A larger bucket falls back to passes over all keys.
+constexpr int kTopkCand = 3072;
+constexpr int kTopkWaves = kTopkBlock / 64;
+constexpr int kScoreBlock = 256;
+
+// Scores of all 6 n^2 coefficients, one thread each, across the machine
+// (the selection below then runs on one workgroup).
+__global__ __launch_bounds__(kScoreBlock) void score_kernel(const float *__restrict__ coef, int n,
+                                                            uint32_t *__restrict__ scores) {
+  const int t = blockIdx.x * kScoreBlock + threadIdx.x;
+  if (t < 6 * n * n) scores[t] = score_bits(coef, t, n);
+}
+
 __global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restrict__ coef, int n,
                                                           int K, uint32_t *__restrict__ scores,
                                                           int32_t *__restrict__ out_idx,
                                                           float *__restrict__ out_str) {
-  __shared__ uint32_t hist[256];
-  __shared__ uint64_t sel[NWPRT_MAX_K];
+  __shared__ uint32_t whist[kTopkWaves][256];
+  __shared__ uint32_t hist[256], wt[4];
+  __shared__ uint64_t sel[NWPRT_MAX_K], sorted[NWPRT_MAX_K];
+  __shared__ uint64_t cand[kTopkCand];
   __shared__ uint64_t s_prefix, s_mask;
-  __shared__ uint32_t s_need, s_count;
+  __shared__ uint32_t s_need, s_count, s_ncand;
   __shared__ int s_done;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int M = 6 * n * n;
 
-  for (int t = tid; t < M; t += kTopkBlock) scores[t] = score_bits(coef, t, n);
+  for (int i = tid; i < kTopkWaves * 256; i += kTopkBlock) (&whist[0][0])[i] = 0u;
   if (tid == 0) {
     s_prefix = 0;
     s_mask = 0;
     s_need = (uint32_t)K;
     s_done = 0;
     s_count = 0;
+    s_ncand = 0;
   }
-  __threadfence_block();
   __syncthreads();
+  for (int t = tid; t < M; t += kTopkBlock) atomicAdd(&whist[wave][scores[t] >> 24], 1u);
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t h = 0;
+    for (int w = 0; w < kTopkWaves; ++w) h += whist[w][tid];
+    hist[tid] = h;
+  }
+  __syncthreads();
+  select_digit(hist, wt, 56, &s_prefix, &s_mask, &s_need, &s_done);
+  const uint32_t d1 = (uint32_t)(s_prefix >> 56);
+  const bool in_lds = hist[d1] <= (uint32_t)kTopkCand;
+  __syncthreads();  // every thread has read hist before it is reused
 
-  for (int pass = 0; pass < 8; ++pass) {
-    const int shift = 56 - 8 * pass;
-    if (tid < 256) hist[tid] = 0;
+  // Remaining digits over `n_keys` keys from `key_at`.
+  auto resolve = [&](int n_keys, auto key_at) {
+    for (int pass = 1; pass < 8 && !s_done; ++pass) {
+      const int shift = 56 - 8 * pass;
+      if (tid < 256) hist[tid] = 0;
+      __syncthreads();
+      const uint64_t prefix = s_prefix, mask = s_mask;
+      for (int i = tid; i < n_keys; i += kTopkBlock) {
+        const uint64_t key = key_at(i);
+        if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+      }
+      __syncthreads();
+      select_digit(hist, wt, shift, &s_prefix, &s_mask, &s_need, &s_done);
+    }
+  };
+  if (in_lds) {
+    for (int t = tid; t < M; t += kTopkBlock) {
+      const uint64_t key = topk_key(scores, t);
+      const uint32_t top = (uint32_t)(key >> 56);
+      if (top > d1) {
+        const uint32_t pos = atomicAdd(&s_count, 1u);
+        if (pos < (uint32_t)NWPRT_MAX_K) sel[pos] = key;
+      } else if (top == d1) {
+        cand[atomicAdd(&s_ncand, 1u)] = key;
+      }
+    }
     __syncthreads();
+    const int nc = (int)s_ncand;
+    resolve(nc, [&](int i) { return cand[i]; });
+    const uint64_t prefix = s_prefix, mask = s_mask;
+    for (int i = tid; i < nc; i += kTopkBlock) {
+      const uint64_t key = cand[i];
+      if ((key & mask) >= prefix) {
+        const uint32_t pos = atomicAdd(&s_count, 1u);
+        if (pos < (uint32_t)NWPRT_MAX_K) sel[pos] = key;
+      }
+    }
+  } else {
+    resolve(M, [&](int t) { return topk_key(scores, t); });
     const uint64_t prefix = s_prefix, mask = s_mask;
     for (int t = tid; t < M; t += kTopkBlock) {
       const uint64_t key = topk_key(scores, t);
-      if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t cum = 0;
-      int d = 255;
-      for (; d > 0; --d) {
-        if (cum + hist[d] >= s_need) break;
-        cum += hist[d];
+      if ((key & mask) >= prefix) {
+        const uint32_t pos = atomicAdd(&s_count, 1u);
+        if (pos < (uint32_t)NWPRT_MAX_K) sel[pos] = key;
       }
-      s_need -= cum;
-      s_prefix = prefix | ((uint64_t)d << shift);
-      s_mask = mask | (0xFFull << shift);
-      s_done = hist[d] == s_need;
-    }
-    __syncthreads();
-    if (s_done) break;
-  }
-
-  const uint64_t prefix = s_prefix, mask = s_mask;
-  for (int t = tid; t < M; t += kTopkBlock) {
-    const uint64_t key = topk_key(scores, t);
-    if ((key & mask) >= prefix) {
-      const uint32_t pos = atomicAdd(&s_count, 1u);
-      if (pos < (uint32_t)NWPRT_MAX_K) sel[pos] = key;
     }
   }
   __syncthreads();
-  int kp = 1;
-  while (kp < K) kp <<= 1;
-  for (int i = (int)s_count + tid; i < kp; i += kTopkBlock) sel[i] = 0;
-  __syncthreads();
-  for (int size = 2; size <= kp; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = tid; i < kp; i += kTopkBlock) {
-        const int j = i ^ stride;
-        if (j > i) {
-          const uint64_t x = sel[i], y = sel[j];
-          const bool desc = (i & size) == 0;
-          if (desc ? (x < y) : (x > y)) {
-            sel[i] = y;
-            sel[j] = x;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  // Rank sort (descending) of the K kept keys: keys are unique, so key i goes
+  // to position #{j : key_j > key_i}.  All lanes read the same sel[j] at a
+  // time (an LDS broadcast); K <= 1024 = one key per thread.
+  const int cnt = (int)s_count < K ? (int)s_count : K;
+  for (int i = tid; i < cnt; i += kTopkBlock) {
+    const uint64_t key = sel[i];
+    int rank = 0;
+    for (int j = 0; j < cnt; ++j) rank += sel[j] > key ? 1 : 0;
+    sorted[rank] = key;
   }
+  __syncthreads();
   for (int i = tid; i < K; i += kTopkBlock) {
-    uint32_t idx = 0xFFFFFFFFu - (uint32_t)(sel[i] & 0xFFFFFFFFull);
+    uint32_t idx = 0xFFFFFFFFu - (uint32_t)(sorted[i] & 0xFFFFFFFFull);
     if (idx >= (uint32_t)M) idx = 0;  // unreachable: keys are unique, exactly K are kept
     out_idx[i] = (int32_t)idx;
     for (int c = 0; c < 3; ++c) out_str[3 * i + c] = coef[3 * (size_t)idx + c];
@@ -367,6 +444,11 @@ void launch_topk(const float *coef, int n, int k, uint32_t *scores, int32_t *ind
                  float *strengths, hipStream_t s) {
   require(k >= 1 && k <= NWPRT_MAX_K && k <= 6 * n * n, NWPRT_ERR_INVALID_ARGUMENT,
           "topk: K must be in [1, min(1024, 6 N^2)]");
+  require(n >= 2 && (n & (n - 1)) == 0, NWPRT_ERR_INVALID_ARGUMENT,
+          "topk: face size must be a power of two");
+  hipLaunchKernelGGL(score_kernel, dim3((6 * n * n + kScoreBlock - 1) / kScoreBlock),
+                     dim3(kScoreBlock), 0, s, coef, n, scores);
+  hip_check(hipGetLastError(), "score_kernel launch");
   hipLaunchKernelGGL(topk_kernel, dim3(1), dim3(kTopkBlock), 0, s, coef, n, k, scores,
                      indices, strengths);
   hip_check(hipGetLastError(), "topk_kernel launch");

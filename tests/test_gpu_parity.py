@@ -59,6 +59,24 @@ def test_topk_ties_break_by_index(nw, orc, ctx):
         ctx.envmap_destroy(env)
 
 
+def test_topk_large_tied_bucket(nw, orc, ctx):
+    """A per-pixel checkerboard gives every finest diagonal detail the same
+    score: 6 x 32^2 = 6144 equal keys at n = 64, more than the selection
+    keeps in LDS, so the kernel resolves them with passes over all keys."""
+    n = 64
+    yy, xx = np.mgrid[0:n, 0:n]
+    board = np.where((xx + yy) % 2 == 0, 1.0, -1.0).astype(np.float32)
+    faces = np.repeat(board[None, :, :, None], 6, axis=0).repeat(3, axis=3) * 5.0 + 10.0
+    faces = np.ascontiguousarray(faces, np.float32)
+    for k in (300, 1024):
+        env = ctx.envmap_from_faces(faces, k)
+        idx, st = ctx.envmap_selection(env)
+        ridx, rst = orc.topk(orc.envmap_haar(faces), k)
+        assert np.array_equal(idx, ridx), k
+        assert np.array_equal(st.view(np.uint32), rst.view(np.uint32)), k
+        ctx.envmap_destroy(env)
+
+
 def test_hash_indices_bit_exact(nw, orc, ctx):
     case = make_case(nw, orc, width=64, height=32, hash_dtype=nw.DTYPE_F16)
     scene = ctx.scene_create(case.desc, case.params)
