@@ -449,11 +449,18 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
   const int lane = threadIdx.x & 63;
   uint32_t *my_lds = feat_lds + (threadIdx.x >> 6) * 64 * 16;
   const int n_items = *n_items_p;
-  for (;;) {
+  // The next item's index is fetched while the current one runs, so the
+  // counter's round trip is off the critical path.
+  auto fetch = [&]() {
     int it = 0;
     if (lane == 0) it = (int)atomicAdd(counter, 1u);
-    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    return __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+  };
+  int next = fetch();
+  for (;;) {
+    const int it = next;
     if (it >= n_items) break;
+    next = fetch();
     const BatchItem item = items[it];
     const RelightArgs a = jobs[item.job];
     // One 64-pixel group per item: a loop over several would let the
