@@ -478,7 +478,7 @@ nwprt_status nwprt_batch_destroy(nwprt_ctx ctx, int32_t batch_id);
 typedef struct {
   uint64_t memory_budget_bytes; /* device bytes of admitted requests' state; 0: no limit */
   int32_t max_active;           /* requests admitted at once (>= 1)         */
-  int64_t tick_pairs;           /* pairs per persistent launch; 0: 2^28     */
+  int64_t tick_pairs;           /* pairs per persistent launch; 0: 2^30     */
   nwprt_precision precision;    /* BF16 or FP16                             */
   nwprt_tonemap_source tonemap;
   int32_t variant;              /* tile variant with a persistent form, -1: default */
