@@ -536,11 +536,15 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     a.softplus = b->softplus;
   }
   b->args.upload(args.data(), args.size() * sizeof(RelightArgs));
-  // Work items: jobs in decreasing K (ties: job order), each job's 64-pixel
-  // groups in pixel order, so the costliest items are taken first.
+  // Work items: jobs in decreasing K, so the costliest items are taken
+  // first; jobs of equal K grouped by scene (one hash grid in the caches at
+  // a time), then in job order; each job's 64-pixel groups in pixel order.
   std::vector<int> order(n);
   for (int j = 0; j < n; ++j) order[j] = j;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->k[x] > b->k[y]; });
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (b->k[x] != b->k[y]) return b->k[x] > b->k[y];
+    return b->jobs[x].scene_id < b->jobs[y].scene_id;
+  });
   std::vector<BatchItem> items;
   items.reserve((size_t)b->n_items);
   for (int j : order)

@@ -437,7 +437,7 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
 // Persistent batch form: a machine-sized grid whose waves take work items
 // (job, first pixel of a 64-pixel group) from a shared counter until the
 // list is exhausted.  Jobs are whole relights with their own frame,
-// scene weights, selection and K; items are listed largest first so the
+// scene weights, selection and K; items are listed by decreasing K so the
 // tail of the launch is short items.  Output-layer tiles stay in VGPRs, so
 // there is no workgroup barrier and no scene switch cost beyond reloading;
 // jobs with the fused encode use their wave's own 4 KB of LDS.
