@@ -396,6 +396,89 @@ nwprt_status nwprt_denoise(nwprt_ctx ctx, const float *color, const float *norma
                            int32_t height, float blend_factor,
                            const nwprt_denoise_params *params, void *stream);
 
+/* ---------------------------------------------------- G-buffer producer -- */
+/* The first-hit part of the OptiX pass that fills the G-buffer before the
+ * relighting path (__raygen__fovCamera, Renderer/device_functions/
+ * camera.cu:82-290; __closesthit__radiance / __miss__radiance,
+ * Renderer/device_functions/devicePrograms.cu:47-311, 517-564), as a HIP
+ * ray caster over a BVH resident in device memory.  It writes the same
+ * encodings the relighting path reads. */
+
+/* Camera::from / at / up as set by RenderEngine::setCamera
+ * (Renderer/RenderEngine.cpp:246-267): 0.8576 rad field of view, principal
+ * point at the image centre. */
+typedef struct {
+  float from[3];
+  float at[3];
+  float up[3];
+} nwprt_camera;
+
+/* What a material's PACKINFO callable writes at a first hit (e.g.
+ * Renderer/device_functions/roughPlastic.cu:9-17): kd, ks and roughness as
+ * they enter the G-buffer, and the Material::MatType (Renderer/Material.h:7;
+ * 3 = mirror, which reflects the ray instead of ending it). */
+typedef struct {
+  float kd[3];
+  float ks[3];
+  float roughness;
+  int32_t type;
+} nwprt_material;
+
+#define NWPRT_MATERIAL_MIRROR 3
+
+typedef struct {
+  float lower[3], upper[3]; /* bounds of the vertices (Scene::bounds)       */
+  int64_t triangles;
+  int64_t bvh_nodes;
+  int32_t bvh_depth;        /* internal levels, <= 32                        */
+} nwprt_mesh_info;
+
+/* Uploads a triangle mesh and builds its BVH on the host (binned SAH, at
+ * most 8 triangles per leaf, at most 32 internal levels); replaces the
+ * OptiX GAS build (OptixBackend.cpp buildAccel).  vertices [n_vertices][3];
+ * normals [n_vertices][3] or NULL (geometric normals); indices
+ * [n_triangles][3]; material_ids [n_triangles] or NULL (all material 0);
+ * materials [n_materials].  Host arrays; vertices are in world space. */
+nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_vertices,
+                               const float *normals, const int32_t *indices,
+                               int64_t n_triangles, const int32_t *material_ids,
+                               const nwprt_material *materials, int32_t n_materials,
+                               int32_t *mesh_id);
+nwprt_status nwprt_mesh_get_info(nwprt_ctx ctx, int32_t mesh_id, nwprt_mesh_info *out);
+nwprt_status nwprt_mesh_destroy(nwprt_ctx ctx, int32_t mesh_id);
+
+typedef struct {
+  int32_t max_bounce;          /* samplingParameter.maximumBounce: 1 in
+                                  wavelet mode (RenderEngine.cpp:343-346);
+                                  mirrors extend the path up to this depth */
+  const float *bounds;         /* host float[6], lower then upper, used to
+                                  normalise positions, or NULL for the
+                                  mesh's own bounds                         */
+  const float *view_env_faces; /* device [6][n][n][3] FP32 cube faces seen
+                                  by primary rays that miss (kd = albedo =
+                                  radiance), or NULL (zero)                 */
+  int32_t view_env_n;
+} nwprt_cast_params;
+
+/* One primary ray per pixel through the pixel's sample position at the
+ * jitter's mean (screen = ((ix - cx) / fx, (iy - cy) / fy)).  Writes, per
+ * pixel of a device G-buffer (any field may be NULL to skip it):
+ * - hit: position (p + 0.001 - lower) / (upper - lower) with p the first hit,
+ *   normal n / 2 + 0.5 (interpolated vertex normal, else geometric),
+ *   reflect r / 2 + 0.5 (r = reflect(ray, n)), kd, ks, roughness of its
+ *   material, env_flag 1;
+ * - miss: position at (-1e5 - lower) / (upper - lower), normal and reflect
+ *   -ray / 2 + 0.5, kd = view environment radiance (0 after a mirror),
+ *   ks 0, roughness 1, env_flag 0;
+ * - a mirror path cut by max_bounce: position -lower / (upper - lower),
+ *   normal and reflect 0.5, kd, ks, roughness 0, env_flag 0.
+ * The w components are 1.  `direct` is not written: the Monte Carlo direct
+ * light of the reference's path tracer is not produced here. */
+nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
+                                int32_t width, int32_t height,
+                                const nwprt_cast_params *params,
+                                const nwprt_gbuffer *gbuffer, void *stream);
+
 /* ------------------------------------------------------------ debug taps -- */
 
 /* Hash-grid tap: for n positions (device float4), write the 8 corner entry

@@ -116,6 +116,27 @@ class DenoiseParams(Structure):
                 ("sigma_albedo", c_float)]
 
 
+class Camera(Structure):
+    _fields_ = [("from_", c_float * 3), ("at", c_float * 3), ("up", c_float * 3)]
+
+
+class Material(Structure):
+    _fields_ = [("kd", c_float * 3), ("ks", c_float * 3), ("roughness", c_float),
+                ("type", c_int32)]
+
+
+class MeshInfo(Structure):
+    _fields_ = [("lower", c_float * 3), ("upper", c_float * 3), ("triangles", c_int64),
+                ("bvh_nodes", c_int64), ("bvh_depth", c_int32)]
+
+
+class CastParams(Structure):
+    _fields_ = [("max_bounce", c_int32), ("bounds", c_void_p), ("view_env_faces", c_void_p),
+                ("view_env_n", c_int32)]
+
+
+MATERIAL_MIRROR = 3
+
 GBUFFER_FIELDS = {
     "position": (np.float32, 4), "normal": (np.float32, 4), "reflect": (np.float32, 4),
     "roughness": (np.float32, 1), "kd": (np.float32, 4), "ks": (np.float32, 4),
@@ -183,6 +204,12 @@ _SIGNATURES = {
     "nwprt_denoise_default_params": (None, [POINTER(DenoiseParams)]),
     "nwprt_denoise": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                               c_float, POINTER(DenoiseParams), c_void_p]),
+    "nwprt_mesh_create": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
+                                  c_void_p, POINTER(Material), c_int32, POINTER(c_int32)]),
+    "nwprt_mesh_get_info": (c_int, [c_void_p, c_int32, POINTER(MeshInfo)]),
+    "nwprt_mesh_destroy": (c_int, [c_void_p, c_int32]),
+    "nwprt_gbuffer_cast": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32,
+                                   POINTER(CastParams), POINTER(GBuffer), c_void_p]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -593,6 +620,54 @@ class Context:
                                    width, height, blend,
                                    ctypes.byref(params) if params is not None else None,
                                    c_void_p(stream) if stream else None))
+
+    # G-buffer producer ---------------------------------------------------
+    def mesh_create(self, vertices: np.ndarray, indices: np.ndarray, materials,
+                    normals: Optional[np.ndarray] = None,
+                    material_ids: Optional[np.ndarray] = None) -> int:
+        """Uploads a world-space triangle mesh and builds its BVH.  materials:
+        sequence of (kd[3], ks[3], roughness, type) rows or Material."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
+        mats = (Material * len(materials))()
+        for i, m in enumerate(materials):
+            if isinstance(m, Material):
+                mats[i] = m
+            else:
+                m = np.asarray(m, np.float64).reshape(8)
+                mats[i] = Material((c_float * 3)(*m[0:3]), (c_float * 3)(*m[3:6]), float(m[6]),
+                                   int(m[7]))
+        out = c_int32()
+        _check(lib().nwprt_mesh_create(self.h, _ptr(v), len(v), _ptr(nr), _ptr(ix), len(ix),
+                                       _ptr(mid), mats, len(materials), ctypes.byref(out)))
+        return out.value
+
+    def mesh_info(self, mesh: int) -> Dict[str, object]:
+        info = MeshInfo()
+        _check(lib().nwprt_mesh_get_info(self.h, mesh, ctypes.byref(info)))
+        return {"lower": list(info.lower), "upper": list(info.upper),
+                "triangles": info.triangles, "bvh_nodes": info.bvh_nodes,
+                "bvh_depth": info.bvh_depth}
+
+    def mesh_destroy(self, mesh: int) -> None:
+        _check(lib().nwprt_mesh_destroy(self.h, mesh))
+
+    def gbuffer_cast(self, mesh: int, camera, frame, *, max_bounce: int = 1, bounds=None,
+                     env_faces=None, env_n: int = 0, stream: Optional[int] = None) -> None:
+        """Primary-ray G-buffer of `frame` (DeviceFrame) from `mesh` seen by
+        camera = (from, at, up); the frame's `direct` buffer is left as is.
+        env_faces: device [6][n][n][3] FP32 faces (DeviceBuffer or address)."""
+        f, a, u = (np.asarray(x, np.float32).reshape(3) for x in camera)
+        cam = Camera((c_float * 3)(*f), (c_float * 3)(*a), (c_float * 3)(*u))
+        b = None if bounds is None else np.ascontiguousarray(bounds, np.float32).reshape(6)
+        env = env_faces.ptr if isinstance(env_faces, DeviceBuffer) else env_faces
+        params = CastParams(max_bounce, _ptr(b), env, env_n)
+        gb = frame.gbuffer()
+        _check(lib().nwprt_gbuffer_cast(self.h, mesh, ctypes.byref(cam), frame.width,
+                                        frame.height, ctypes.byref(params), ctypes.byref(gb),
+                                        c_void_p(stream) if stream else None))
 
     # debug taps ----------------------------------------------------------
     def debug_hash_encode(self, scene: int, positions: np.ndarray):

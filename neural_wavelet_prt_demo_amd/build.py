@@ -29,11 +29,13 @@ SOURCES = [
     "host_abi.cpp",
     "engine.cpp",
     "server.cpp",
+    "mesh.cpp",
     "envmap_kernels.hip",
     "encode_kernels.hip",
     "relight_kernels.hip",
     "relight_fp8_kernels.hip",
     "denoise_kernels.hip",
+    "gbuffer_kernels.hip",
 ]
 
 # Packed-FP32 VALU forms are disabled for the device code (the host pass

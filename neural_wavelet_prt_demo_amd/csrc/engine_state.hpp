@@ -73,6 +73,17 @@ struct EnvMap {
   }
 };
 
+// A triangle mesh resident for the G-buffer producer: its BVH (two-child
+// nodes, triangles in leaf order) and the per-vertex / per-triangle data the
+// closest-hit step reads.
+struct Mesh {
+  DevBuf nodes, tris, indices, normals, tri_material, materials;
+  float lower[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
+  int64_t n_tris = 0, n_nodes = 0;
+  int depth = 0;
+  bool has_normals = false, has_tri_material = false;
+};
+
 // Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
 // 2: nwprt_relight / debug taps).
 struct FrameSlot {
@@ -135,6 +146,7 @@ struct nwprt_context {
   std::map<int, nwprt::ServerPtr> servers;
   int n_cu = 0;  // compute units of the device (persistent grid size)
   nwprt::DevBuf denoise_scratch;  // denoiser ping-pong images + normalisation
+  std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
 };
 
 namespace nwprt {

@@ -147,6 +147,55 @@ struct DenoiseArgs {
 size_t denoise_scratch_bytes(int w, int h);
 void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s);
 
+// G-buffer producer (gbuffer_kernels.hip, BVH built in mesh.cpp).
+// The traversal stack holds at most one entry per internal level.
+constexpr int kBvhMaxDepth = 32;
+constexpr int kBvhLeafMax = 8;
+// Two-child node, 64 bytes: both children's boxes sit in the parent so one
+// visit tests both.  Child reference c >= 0: internal node c; c < 0: leaf
+// ~c = first * 8 + (count - 1) over the reordered triangle array.
+struct alignas(16) BvhNode {
+  float lo0[3];
+  int32_t c0;
+  float hi0[3];
+  int32_t c1;
+  float lo1[3];
+  int32_t pad0;
+  float hi1[3];
+  int32_t pad1;
+};
+// Triangle in leaf order: A, B - A, C - A and the original triangle index.
+struct alignas(16) BvhTri {
+  float v0[3];
+  int32_t prim;
+  float e1[3];
+  int32_t pad0;
+  float e2[3];
+  int32_t pad1;
+};
+struct alignas(16) MaterialRec {
+  float kd[3];
+  float roughness;
+  float ks[3];
+  int32_t type;
+};
+struct CastArgs {
+  const BvhNode *nodes;
+  const BvhTri *tris;
+  const int32_t *indices;       // [n_tri][3] original vertex indices
+  const float *normals;         // [n_vertices][3], or null
+  const int32_t *tri_material;  // [n_tri], or null (material 0)
+  const MaterialRec *materials;
+  float cam_pos[3], cam_dir[3], cam_h[3], cam_v[3];
+  float cx, cy, fx, fy;
+  float lower[3], box[3];
+  int width, height, max_bounce;
+  const float *env_faces;  // [6][env_n][env_n][3] or null
+  int env_n;
+  nwprt_gbuffer gb;  // null fields are skipped
+};
+void launch_gbuffer_cast(const CastArgs &a, hipStream_t s);
+
 // Persistent batch form: work item = the 64 pixels [base, base + 64) of job
 // `job` (base < the job's pixel count).
 struct BatchItem {

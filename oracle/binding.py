@@ -32,6 +32,11 @@ class OrcGbuffer(Structure):
                 ("direct", c_void_p), ("env_flag", c_void_p)]
 
 
+class OrcMesh(Structure):
+    _fields_ = [("vertices", c_void_p), ("normals", c_void_p), ("indices", c_void_p),
+                ("n_tris", ctypes.c_int64), ("material_ids", c_void_p), ("materials", c_void_p)]
+
+
 def _gbuf(g):
     keep = {k: np.ascontiguousarray(v) for k, v in g.items()}
     return keep, OrcGbuffer(*[_p(keep.get(k)) for k in ("position", "normal", "reflect",
@@ -68,6 +73,9 @@ def lib():
         l.orc_denoise.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double,
                                   c_double, c_double, c_double, c_void_p]
         l.orc_denoise.restype = None
+        l.orc_gbuffer_cast.argtypes = [POINTER(OrcMesh), c_void_p, c_int, c_int, c_int, c_void_p,
+                                       c_void_p, c_int] + [c_void_p] * 9
+        l.orc_gbuffer_cast.restype = None
         l.orc_tonemap.restype = c_uint32
         l.orc_aux_inputs.argtypes = [POINTER(OrcGbuffer), c_int, c_float, c_void_p]
         l.orc_cpu_relight_fp32.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int,
@@ -218,4 +226,39 @@ def denoise(color: np.ndarray, normal: np.ndarray, albedo: np.ndarray, iteration
     out = np.empty((h, w, 4), np.float64)
     lib().orc_denoise(_p(c), _p(n), _p(a), w, h, iterations, sigma_color, sigma_normal,
                       sigma_albedo, blend, _p(out))
+    return out
+
+
+def gbuffer_cast(vertices: np.ndarray, indices: np.ndarray, materials, camera, width: int,
+                 height: int, *, normals: Optional[np.ndarray] = None,
+                 material_ids: Optional[np.ndarray] = None, max_bounce: int = 1,
+                 bounds=None, env_faces: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
+    """First-hit G-buffer by FP64 brute force.  materials: rows (kd[3], ks[3],
+    roughness, type); camera: (from, at, up); bounds: (lower, upper) or the
+    vertices' own.  Returns float64 [P][4] position / normal / reflect / kd /
+    ks, [P] roughness, uint8 env_flag, int32 prim (-1 none), float64 margin."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+    nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+    cam = np.ascontiguousarray(camera, np.float32).reshape(9)
+    if bounds is None:
+        bounds = np.concatenate([v.min(0), v.max(0)])
+    b = np.ascontiguousarray(bounds, np.float32).reshape(6)
+    env_n = 0
+    if env_faces is not None:
+        env_faces = np.ascontiguousarray(env_faces, np.float32)
+        env_n = env_faces.shape[1]
+    mesh = OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats))
+    P = width * height
+    out = {k: np.empty((P, 4), np.float64) for k in ("position", "normal", "reflect", "kd", "ks")}
+    out["roughness"] = np.empty(P, np.float64)
+    out["env_flag"] = np.empty(P, np.uint8)
+    out["prim"] = np.empty(P, np.int32)
+    out["margin"] = np.empty(P, np.float64)
+    lib().orc_gbuffer_cast(ctypes.byref(mesh), _p(cam), width, height, max_bounce, _p(b),
+                           _p(env_faces), env_n,
+                           *[_p(out[k]) for k in ("position", "normal", "reflect", "kd", "ks",
+                                                  "roughness", "env_flag", "prim", "margin")])
     return out

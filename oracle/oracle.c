@@ -444,3 +444,244 @@ void orc_denoise(const float *color, const float *normal, const float *albedo, i
   free(cur);
   free(nxt);
 }
+
+/* ------------------------------------------------------------------------ */
+/* G-buffer producer: first hits of primary rays, brute force over every
+ * triangle in FP64.  Follows __raygen__fovCamera (Renderer/device_functions/
+ * camera.cu:82-131, 233-270) with the jitter at its mean, RenderEngine::
+ * setCamera (Renderer/RenderEngine.cpp:246-267), __closesthit__radiance
+ * (Renderer/device_functions/devicePrograms.cu:47-109, 156-164, 295-308),
+ * __miss__radiance (devicePrograms.cu:517-564) and EnvDirecToUV
+ * (Renderer/device_functions/cubeMap.cu:21-92) sampled bilinearly with wrap
+ * addressing (Renderer/textures.cpp:31-36). */
+
+static void g_cross(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double g_dot(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+static void g_norm(double *a) {
+  const double l = sqrt(g_dot(a, a));
+  a[0] /= l;
+  a[1] /= l;
+  a[2] /= l;
+}
+
+static void g_env(const float *faces, int n, const double *d, double *rgb) {
+  const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
+  double m = ax, uc = d[2], vc = d[1];
+  int f = 0;
+  if (d[0] > 0 && ax >= ay && ax >= az) { m = ax; uc = d[2]; vc = d[1]; f = 0; }
+  if (!(d[0] > 0) && ax >= ay && ax >= az) { m = ax; uc = -d[2]; vc = d[1]; f = 1; }
+  if (d[1] > 0 && ay >= ax && ay >= az) { m = ay; uc = d[0]; vc = d[2]; f = 2; }
+  if (!(d[1] > 0) && ay >= ax && ay >= az) { m = ay; uc = d[0]; vc = -d[2]; f = 3; }
+  if (!(d[2] > 0) && az >= ax && az >= ay) { m = az; uc = d[0]; vc = d[1]; f = 4; }
+  if (d[2] > 0 && az >= ax && az >= ay) { m = az; uc = -d[0]; vc = d[1]; f = 5; }
+  const double u = 0.5 * (uc / m + 1.0), v = 0.5 * (vc / m + 1.0);
+  const double x = u * n - 0.5, y = v * n - 0.5;
+  const double fx = floor(x), fy = floor(y), wx = x - fx, wy = y - fy;
+  int x0 = ((int)fx % n + n) % n, y0 = ((int)fy % n + n) % n;
+  const int x1 = (x0 + 1) % n, y1 = (y0 + 1) % n;
+  const float *face = faces + (size_t)f * n * n * 3;
+  for (int c = 0; c < 3; ++c) {
+    const double p00 = face[((size_t)y0 * n + x0) * 3 + c], p01 = face[((size_t)y0 * n + x1) * 3 + c];
+    const double p10 = face[((size_t)y1 * n + x0) * 3 + c], p11 = face[((size_t)y1 * n + x1) * 3 + c];
+    const double top = p00 + wx * (p01 - p00), bot = p10 + wx * (p11 - p10);
+    rgb[c] = top + wy * (bot - top);
+  }
+}
+
+/* Closest hit of o + t d over all triangles, t > 0.01, ties to the lower
+ * index.  Returns the triangle or -1; *margin is lowered to the smallest
+ * robustness figure of this ray: the barycentric distance to an edge of any
+ * triangle that could take the hit, the relative gap to the next hit, and
+ * |det| / (|e1| |e2|) of the hit triangle (grazing incidence). */
+static int g_closest(const orc_mesh *m, const double *o, const double *d, double *t_out,
+                     double *u_out, double *v_out, double *margin) {
+  const double tmin = 1e-2;
+  double best = 1e300, second = 1e300, bu = 0, bv = 0;
+  int hit = -1;
+  for (int64_t t = 0; t < m->n_tris; ++t) {
+    const float *A = m->vertices + 3 * (size_t)m->indices[3 * t];
+    const float *B = m->vertices + 3 * (size_t)m->indices[3 * t + 1];
+    const float *C = m->vertices + 3 * (size_t)m->indices[3 * t + 2];
+    double e1[3], e2[3], s[3], p[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = (double)B[k] - A[k];
+      e2[k] = (double)C[k] - A[k];
+      s[k] = o[k] - A[k];
+    }
+    g_cross(d, e2, p);
+    const double det = g_dot(e1, p);
+    if (det == 0.0) continue;
+    const double u = g_dot(s, p) / det;
+    g_cross(s, e1, q);
+    const double v = g_dot(d, q) / det;
+    const double tt = g_dot(e2, q) / det;
+    if (!(tt > tmin)) continue;
+    const double w = 1.0 - u - v;
+    const double e = fmin(u, fmin(v, w));
+    if (e >= 0.0) {
+      if (tt < best) {
+        second = best;
+        best = tt;
+        hit = (int)t;
+        bu = u;
+        bv = v;
+      } else if (tt < second) {
+        second = tt;
+      }
+    }
+  }
+  /* Second pass for the margin: triangles whose edge passes near the ray at
+   * a distance that could take (or lose) the hit. */
+  for (int64_t t = 0; t < m->n_tris; ++t) {
+    const float *A = m->vertices + 3 * (size_t)m->indices[3 * t];
+    const float *B = m->vertices + 3 * (size_t)m->indices[3 * t + 1];
+    const float *C = m->vertices + 3 * (size_t)m->indices[3 * t + 2];
+    double e1[3], e2[3], s[3], p[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = (double)B[k] - A[k];
+      e2[k] = (double)C[k] - A[k];
+      s[k] = o[k] - A[k];
+    }
+    g_cross(d, e2, p);
+    const double det = g_dot(e1, p);
+    if (det == 0.0) continue;
+    const double u = g_dot(s, p) / det;
+    g_cross(s, e1, q);
+    const double v = g_dot(d, q) / det;
+    const double tt = g_dot(e2, q) / det;
+    if (!(tt > tmin * (1.0 - 1e-6))) continue;
+    if (hit >= 0 && tt > best * (1.0 + 1e-6) + 1e-9) continue;
+    const double e = fabs(fmin(u, fmin(v, 1.0 - u - v)));
+    if (e < *margin) *margin = e;
+    if (fabs(tt - tmin) / tmin < *margin) *margin = fabs(tt - tmin) / tmin;
+  }
+  if (hit >= 0) {
+    if (second < 1e300 && (second - best) / best < *margin) *margin = (second - best) / best;
+    const float *A = m->vertices + 3 * (size_t)m->indices[3 * hit];
+    const float *B = m->vertices + 3 * (size_t)m->indices[3 * hit + 1];
+    const float *C = m->vertices + 3 * (size_t)m->indices[3 * hit + 2];
+    double e1[3], e2[3], p[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = (double)B[k] - A[k];
+      e2[k] = (double)C[k] - A[k];
+    }
+    g_cross(d, e2, p);
+    const double g = fabs(g_dot(e1, p)) / (sqrt(g_dot(e1, e1)) * sqrt(g_dot(e2, e2)));
+    if (g < *margin) *margin = g;
+  }
+  *t_out = best;
+  *u_out = bu;
+  *v_out = bv;
+  return hit;
+}
+
+void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
+                      const float *bounds, const float *env_faces, int env_n, double *position,
+                      double *normal, double *reflect, double *kd, double *ks, double *roughness,
+                      uint8_t *env_flag, int32_t *prim, double *margin) {
+  double dir[3], up[3], hz[3], vt[3];
+  for (int a = 0; a < 3; ++a) {
+    dir[a] = (double)camera[3 + a] - camera[a];
+    up[a] = camera[6 + a];
+  }
+  g_norm(dir);
+  g_cross(dir, up, hz);
+  g_norm(hz);
+  for (int a = 0; a < 3; ++a) hz[a] *= (double)w / (double)h;
+  g_cross(hz, dir, vt);
+  g_norm(vt);
+  const double th = tan(0.5 * 0.8575560548920328);
+  const double cx = w / 2.0, cy = h / 2.0, fx = 0.5 * w / th, fy = 0.5 * h / th;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t px = 0; px < (int64_t)w * h; ++px) {
+    const int ix = (int)(px % w), iy = (int)(px / w);
+    const double sx = (ix - cx) / fx, sy = (iy - cy) / fy;
+    double d[3], o[3];
+    for (int a = 0; a < 3; ++a) {
+      d[a] = dir[a] + sx * hz[a] + sy * vt[a];
+      o[a] = camera[a];
+    }
+    g_norm(d);
+    double pos[3] = {0, 0, 0}, nrm[3] = {0, 0, 0}, rfl[3] = {0, 0, 0};
+    double kdv[3] = {0, 0, 0}, ksv[3] = {0, 0, 0}, rough = 0.0, mg = 1.0;
+    int flag = 0, first = -1;
+    for (int depth = 1;; ++depth) {
+      double t, u, v;
+      const int hit = g_closest(m, o, d, &t, &u, &v, &mg);
+      if (hit < 0) {
+        for (int a = 0; a < 3; ++a) {
+          nrm[a] = -d[a];
+          rfl[a] = -d[a];
+          pos[a] = -100000.0;
+        }
+        if (depth == 1 && env_faces) g_env(env_faces, env_n, d, kdv);
+        rough = 1.0;
+        break;
+      }
+      const int mat = m->material_ids ? m->material_ids[hit] : 0;
+      const float *mr = m->materials + 8 * (size_t)mat;
+      const int *tri = m->indices + 3 * (size_t)hit;
+      double ns[3];
+      if (m->normals) {
+        const double ww = 1.0 - u - v;
+        for (int a = 0; a < 3; ++a)
+          ns[a] = ww * m->normals[3 * (size_t)tri[0] + a] + u * m->normals[3 * (size_t)tri[1] + a] +
+                  v * m->normals[3 * (size_t)tri[2] + a];
+      } else {
+        const float *A = m->vertices + 3 * (size_t)tri[0];
+        const float *B = m->vertices + 3 * (size_t)tri[1];
+        const float *C = m->vertices + 3 * (size_t)tri[2];
+        double e1[3], e2[3];
+        for (int a = 0; a < 3; ++a) {
+          e1[a] = (double)B[a] - A[a];
+          e2[a] = (double)C[a] - A[a];
+        }
+        g_cross(e1, e2, ns);
+      }
+      g_norm(ns);
+      double p[3], r[3];
+      for (int a = 0; a < 3; ++a) p[a] = o[a] + d[a] * t + 0.001;
+      const double nd = g_dot(ns, d);
+      for (int a = 0; a < 3; ++a) r[a] = d[a] - 2.0 * nd * ns[a];
+      g_norm(r);
+      if ((int)mr[7] == 3) { /* mirror: the path continues */
+        if (depth >= max_bounce) break;
+        for (int a = 0; a < 3; ++a) {
+          o[a] = p[a];
+          d[a] = r[a];
+        }
+        continue;
+      }
+      for (int a = 0; a < 3; ++a) {
+        pos[a] = p[a];
+        nrm[a] = ns[a];
+        rfl[a] = r[a];
+        kdv[a] = mr[a];
+        ksv[a] = mr[3 + a];
+      }
+      rough = mr[6];
+      flag = 1;
+      first = hit;
+      break;
+    }
+    for (int a = 0; a < 3; ++a) {
+      position[4 * px + a] = (pos[a] - bounds[a]) / ((double)bounds[3 + a] - bounds[a]);
+      normal[4 * px + a] = nrm[a] * 0.5 + 0.5;
+      reflect[4 * px + a] = rfl[a] * 0.5 + 0.5;
+      kd[4 * px + a] = kdv[a];
+      ks[4 * px + a] = ksv[a];
+    }
+    position[4 * px + 3] = normal[4 * px + 3] = reflect[4 * px + 3] = 1.0;
+    kd[4 * px + 3] = ks[4 * px + 3] = 1.0;
+    roughness[px] = rough;
+    env_flag[px] = (uint8_t)flag;
+    prim[px] = first;
+    margin[px] = mg;
+  }
+}

@@ -33,6 +33,11 @@
  *                        colour scaling commutes; transpose symmetry; the
  *                        learned model it replaces is closed, so parity with
  *                        the reference's denoised images is out of reach
+ *   orc_gbuffer_cast     closed forms (plane at a known depth, centre-pixel ray
+ *                        = view direction, mirror at 45 degrees), an
+ *                        independent numpy edge-function intersection on
+ *                        random triangle soups, cube-face lookups at texel
+ *                        centres and axis directions, translation invariance
  * The tiny-cuda-nn conventions behind the hash grid and the MLP (level
  * rounding, layer layout, padding values) cannot be pinned against the
  * reference: that library is an empty submodule there ("parity unpinned"
@@ -99,6 +104,29 @@ uint32_t orc_tonemap(float r, float g, float b);
 void orc_denoise(const float *color, const float *normal, const float *albedo, int w, int h,
                  int iterations, double sigma_color, double sigma_normal, double sigma_albedo,
                  double blend, double *out);
+
+/* Triangle mesh for orc_gbuffer_cast.  materials: [n][8] = kd[3], ks[3],
+ * roughness, type (as a float; 3 = mirror). */
+typedef struct {
+  const float *vertices;     /* [n_vertices][3]        */
+  const float *normals;      /* [n_vertices][3] or NULL */
+  const int32_t *indices;    /* [n_tris][3]            */
+  int64_t n_tris;
+  const int32_t *material_ids; /* [n_tris] or NULL     */
+  const float *materials;
+} orc_mesh;
+
+/* First-hit G-buffer of a w x h view, FP64 brute force (one ray per pixel,
+ * jitter at its mean).  camera: from[3], at[3], up[3]; bounds: lower[3],
+ * upper[3].  Outputs double [P][4] (position, normal, reflect, kd, ks),
+ * double [P] roughness, uint8 [P] env_flag, int32 [P] first-hit triangle
+ * (-1: none) and double [P] margin: the smallest barycentric edge distance,
+ * relative depth gap or grazing figure met along the pixel's path (small
+ * values mark pixels a rounding difference can flip). */
+void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
+                      const float *bounds, const float *env_faces, int env_n, double *position,
+                      double *normal, double *reflect, double *kd, double *ks, double *roughness,
+                      uint8_t *env_flag, int32_t *prim, double *margin);
 
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,

@@ -84,3 +84,62 @@ def _faces_to_cross(faces):
     for f, (c, r) in enumerate([(2, 1), (0, 1), (1, 2), (1, 0), (1, 1), (3, 1)]):
         bottom_up[r * n:(r + 1) * n, c * n:(c + 1) * n] = faces[f]
     return cross
+
+
+def uv_sphere(r, nu, nv, centre):
+    """UV sphere: vertices, radial unit normals, CCW-outward triangles."""
+    th = np.linspace(0, np.pi, nv + 1)
+    ph = np.linspace(0, 2 * np.pi, nu, endpoint=False)
+    T_, P_ = np.meshgrid(th, ph, indexing="ij")
+    n = np.stack([np.sin(T_) * np.cos(P_), np.cos(T_), np.sin(T_) * np.sin(P_)], -1).reshape(-1, 3)
+    idx = []
+    for i in range(nv):
+        for j in range(nu):
+            a, b = i * nu + j, i * nu + (j + 1) % nu
+            idx += [[a, a + nu, b], [b, a + nu, b + nu]]
+    return (np.asarray(centre) + r * n).astype(np.float32), n.astype(np.float32), \
+        np.array(idx, np.int32)
+
+
+def mesh_scene(seed: int, *, sphere=(48, 24), soup=40, mirror=True):
+    """A seeded test scene for the G-buffer producer: a sphere with vertex
+    normals, a ground quad, a random triangle soup and (optionally) a tilted
+    mirror, with four materials.  Returns a dict of host arrays plus camera
+    and bounds; the camera sits at +z looking at the origin."""
+    rng = np.random.default_rng(seed)
+    sv, sn, si = uv_sphere(1.5, sphere[0], sphere[1], [0.3, 0.2, -1.0])
+    verts, norms, idx, mids = [sv], [sn], [si], [np.zeros(len(si), np.int32)]
+    base = len(sv)
+    g = np.array([[-8, -1.6, -9], [8, -1.6, -9], [8, -1.6, 6], [-8, -1.6, 6]], np.float32)
+    verts.append(g)
+    norms.append(np.tile([0, 1, 0], (4, 1)).astype(np.float32))
+    idx.append(np.array([[0, 2, 1], [0, 3, 2]], np.int32) + base)
+    mids.append(np.ones(2, np.int32))
+    base += 4
+    if soup:
+        c = rng.uniform([-4, -1.5, -6], [4, 3, 0], (soup, 1, 3))
+        t = (c + rng.normal(0, 0.6, (soup, 3, 3))).reshape(-1, 3).astype(np.float32)
+        verts.append(t)
+        # Per-vertex normals of the soup: each triangle's own face normal.
+        fn = np.cross(t[1::3] - t[0::3], t[2::3] - t[0::3])
+        fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+        norms.append(np.repeat(fn, 3, axis=0).astype(np.float32))
+        idx.append(np.arange(3 * soup, dtype=np.int32).reshape(-1, 3) + base)
+        mids.append(rng.integers(0, 3, soup).astype(np.int32))
+        base += 3 * soup
+    if mirror:
+        m = np.array([[-6, -1.5, -5], [-1.5, -1.5, -7.5], [-3.5, 4, -6.5]], np.float32)
+        verts.append(m)
+        fn = np.cross(m[1] - m[0], m[2] - m[0])
+        norms.append(np.tile(fn / np.linalg.norm(fn), (3, 1)).astype(np.float32))
+        idx.append(np.array([[0, 1, 2]], np.int32) + base)
+        mids.append(np.full(1, 3, np.int32))
+    mats = np.array([[0.8, 0.3, 0.2, 0.04, 0.04, 0.04, 0.3, 0],
+                     [0.5, 0.5, 0.5, 0.0, 0.0, 0.0, 1.0, 0],
+                     [0.1, 0.6, 0.9, 0.5, 0.5, 0.5, 0.05, 2],
+                     [0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 3]], np.float32)
+    v = np.concatenate(verts)
+    return {"vertices": v, "normals": np.concatenate(norms), "indices": np.concatenate(idx),
+            "material_ids": np.concatenate(mids), "materials": mats,
+            "camera": [0.5, 1.0, 6.0, 0.0, 0.0, -1.5, 0.0, 1.0, 0.0],
+            "bounds": np.concatenate([v.min(0), v.max(0)]).astype(np.float32)}

@@ -1,0 +1,228 @@
+"""GPU tests of the G-buffer producer (nwprt_mesh_create + nwprt_gbuffer_cast,
+the first-hit part of the reference's OptiX pass:
+Renderer/device_functions/camera.cu:82-290 and devicePrograms.cu:47-311,
+517-564) against the FP64 brute-force oracle (orc_gbuffer_cast).
+
+Bar: on pixels whose oracle margin is >= 1e-4 (no triangle edge, depth tie,
+t_min crossing or grazing hit within that distance, so FP32 rounding cannot
+change the first hit), env_flag, kd / ks / roughness of hits and the hit or
+miss class agree exactly; position, normal and reflect encodings agree
+within 2e-4 of max(1, |value|) (FP32 against FP64 on [0, 1] encodings and
+on the far-off miss positions; after a mirror
+bounce the reflected ray's FP32 direction error grows with the distance,
+hence the looser bound there); miss kd (bilinear environment lookup in
+FP32) within 1e-5 relative.  At most 2 % of pixels may fall under the
+margin (5 % in the dense 110K-triangle scene)."""
+import numpy as np
+import pytest
+
+from cases import mesh_scene, uv_sphere
+
+pytestmark = pytest.mark.gpu
+
+MARGIN = 1e-4
+TOL_ENC = 2e-4
+
+
+def _download(frame):
+    P = frame.width * frame.height
+    out = {}
+    for k in ("position", "normal", "reflect", "kd", "ks"):
+        out[k] = frame.bufs[k].download(np.float32, (P, 4)).astype(np.float64)
+    out["roughness"] = frame.bufs["roughness"].download(np.float32, (P,)).astype(np.float64)
+    out["env_flag"] = frame.bufs["env_flag"].download(np.uint8, (P,))
+    return out
+
+
+def _cast(nw, ctx, s, w, h, max_bounce=1, faces=None):
+    mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s.get("normals"),
+                           material_ids=s.get("material_ids"))
+    frame = nw.DeviceFrame(w, h)
+    env = None
+    if faces is not None:
+        env = nw.DeviceBuffer.from_array(np.ascontiguousarray(faces, np.float32))
+    ctx.gbuffer_cast(mesh, np.reshape(s["camera"], (3, 3)), frame, max_bounce=max_bounce,
+                     bounds=s["bounds"], env_faces=env, env_n=0 if faces is None else faces.shape[1])
+    ctx.sync()
+    got = _download(frame)
+    info = ctx.mesh_info(mesh)
+    ctx.mesh_destroy(mesh)
+    return got, info
+
+
+def _oracle(orc, s, w, h, max_bounce=1, faces=None):
+    return orc.gbuffer_cast(s["vertices"], s["indices"], s["materials"], s["camera"], w, h,
+                            normals=s.get("normals"), material_ids=s.get("material_ids"),
+                            max_bounce=max_bounce, bounds=s["bounds"], env_faces=faces)
+
+
+def _compare(got, want, tol_enc=TOL_ENC, min_safe=0.98):
+    safe = want["margin"] >= MARGIN
+    assert safe.mean() >= min_safe, safe.mean()
+    assert np.array_equal(got["env_flag"][safe], want["env_flag"][safe])
+    hit = safe & (want["env_flag"] == 1)
+    miss = safe & (want["env_flag"] == 0)
+    for k in ("kd", "ks"):
+        assert np.array_equal(got[k][hit].astype(np.float32), want[k][hit].astype(np.float32)), k
+    assert np.array_equal(got["roughness"][safe].astype(np.float32),
+                          want["roughness"][safe].astype(np.float32))
+    for k in ("position", "normal", "reflect"):
+        # Miss positions sit near -1e5 / extent: FP32 keeps them to ~1e-7
+        # relative, hence the relative term.
+        err = (np.abs(got[k][safe] - want[k][safe]) /
+               np.maximum(1.0, np.abs(want[k][safe]))).max() if safe.any() else 0.0
+        assert err <= tol_enc, (k, err)
+    if miss.any():
+        kd_w = want["kd"][miss, :3]
+        assert np.all(np.abs(got["kd"][miss, :3] - kd_w) <= 1e-5 * (np.abs(kd_w) + 1e-6))
+        assert np.array_equal(got["ks"][miss], want["ks"][miss])
+    return hit.sum(), miss.sum()
+
+
+@pytest.mark.parametrize("seed,w,h,bounce", [(0, 64, 48, 1), (1, 37, 29, 2), (2, 96, 54, 4),
+                                              (3, 1, 1, 1), (4, 1, 50, 2), (5, 50, 1, 1)])
+def test_cast_matches_oracle(nw, orc, ctx, seed, w, h, bounce):
+    s = mesh_scene(seed)
+    faces = np.random.default_rng(seed).lognormal(0, 1, (6, 8, 8, 3)).astype(np.float32)
+    got, info = _cast(nw, ctx, s, w, h, bounce, faces)
+    want = _oracle(orc, s, w, h, bounce, faces)
+    _compare(got, want, TOL_ENC if bounce == 1 else 1e-3)
+    assert 1 <= info["bvh_depth"] <= 32 and info["triangles"] == len(s["indices"])
+
+
+def test_cast_geometric_normals_and_single_material(nw, orc, ctx):
+    s = mesh_scene(11, mirror=False)
+    s.pop("normals")
+    s.pop("material_ids")
+    got, _ = _cast(nw, ctx, s, 48, 40)
+    want = _oracle(orc, s, 48, 40)
+    nh, _ = _compare(got, want)
+    assert nh > 300
+
+
+def test_cast_large_mesh_deep_bvh(nw, orc, ctx):
+    # ~110K triangles of a finely tessellated sphere plus a dense soup.
+    v, n, ix = uv_sphere(1.2, 256, 192, [0.0, 0.0, -1.0])
+    rng = np.random.default_rng(5)
+    soup = 12000
+    c = rng.uniform([-3, -2, -5], [3, 2, 1], (soup, 1, 3))
+    t = (c + rng.normal(0, 0.05, (soup, 3, 3))).reshape(-1, 3).astype(np.float32)
+    s = {"vertices": np.concatenate([v, t]),
+         "indices": np.concatenate([ix, np.arange(3 * soup, dtype=np.int32).reshape(-1, 3) +
+                                    len(v)]),
+         "materials": np.array([[0.3, 0.4, 0.5, 0.1, 0.1, 0.1, 0.5, 0]], np.float32),
+         "camera": [0.0, 0.0, 5.0, 0.0, 0.0, -1.0, 0.0, 1.0, 0.0]}
+    s["bounds"] = np.concatenate([s["vertices"].min(0), s["vertices"].max(0)])
+    got, info = _cast(nw, ctx, s, 40, 30)
+    want = _oracle(orc, s, 40, 30)
+    nh, _ = _compare(got, want, min_safe=0.95)  # 12K small triangles: edges everywhere
+    assert nh > 500 and info["triangles"] == len(s["indices"]) and info["bvh_depth"] <= 32
+
+
+@pytest.mark.parametrize("nt", [1, 5, 8, 9, 100])
+def test_cast_small_and_degenerate_meshes(nw, orc, ctx, nt):
+    # nt triangles sharing one centroid (no SAH split exists: median splits
+    # by index), scaled copies of one triangle facing the camera.
+    base = np.array([[-1, -1, 0], [1, -1, 0], [0, 1.5, 0]], np.float64)
+    v = np.concatenate([base * (0.5 + 0.02 * i) + [0, 0, -3 - 0.01 * i] for i in range(nt)])
+    s = {"vertices": v.astype(np.float32),
+         "indices": np.arange(3 * nt, dtype=np.int32).reshape(-1, 3),
+         "materials": np.array([[0.2, 0.3, 0.4, 0.0, 0.0, 0.0, 0.6, 0]], np.float32),
+         "camera": [0.0, 0.0, 2.0, 0.0, 0.0, -1.0, 0.0, 1.0, 0.0],
+         "bounds": np.array([-2, -2, -5, 2, 2, 1], np.float32)}
+    got, _ = _cast(nw, ctx, s, 24, 18)
+    want = _oracle(orc, s, 24, 18)
+    _compare(got, want)
+
+
+def test_cast_null_fields_are_skipped(nw, ctx):
+    s = mesh_scene(3)
+    mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
+                           material_ids=s["material_ids"])
+    frame = nw.DeviceFrame(16, 8)
+    sentinel = np.full((16 * 8, 4), 7.0, np.float32)
+    for k in ("normal", "kd", "direct"):
+        frame.bufs[k].upload(sentinel)
+    gb = frame.gbuffer()
+    gb.normal = None
+    gb.kd = None
+    cam = nw.Camera((nw.c_float * 3)(*s["camera"][0:3]), (nw.c_float * 3)(*s["camera"][3:6]),
+                    (nw.c_float * 3)(*s["camera"][6:9]))
+    b = np.ascontiguousarray(s["bounds"], np.float32)
+    params = nw.CastParams(1, b.ctypes.data, None, 0)
+    nw._check(nw.lib().nwprt_gbuffer_cast(ctx.h, mesh, nw.ctypes.byref(cam), 16, 8,
+                                          nw.ctypes.byref(params), nw.ctypes.byref(gb), None))
+    ctx.sync()
+    for k in ("normal", "kd", "direct"):  # skipped fields and `direct` keep their contents
+        assert np.all(frame.bufs[k].download(np.float32, (16 * 8, 4)) == 7.0), k
+    pos = frame.bufs["position"].download(np.float32, (16 * 8, 4))
+    assert np.all(pos[:, 3] == 1.0)
+    ctx.mesh_destroy(mesh)
+
+
+def test_mesh_and_cast_errors(nw, ctx):
+    s = mesh_scene(0)
+    v, ix, m = s["vertices"], s["indices"], s["materials"]
+    bad = [lambda: ctx.mesh_create(v, ix + len(v), m),                     # index out of range
+           lambda: ctx.mesh_create(v, ix, []),                             # no materials
+           lambda: ctx.mesh_create(v, ix, m, material_ids=np.full(len(ix), 9)),
+           lambda: ctx.mesh_create(np.where(np.arange(v.size).reshape(v.shape) == 5, np.nan, v),
+                                   ix, m)]
+    for f in bad:
+        with pytest.raises(nw.NwprtError) as e:
+            f()
+        assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    flat = np.array([[0, 0, -2], [1, 0, -2], [0, 1, -2]], np.float32)
+    mesh = ctx.mesh_create(flat, [[0, 1, 2]], m[:1])
+    frame = nw.DeviceFrame(8, 8)
+    cam = [[0, 0, 0], [0, 0, -1], [0, 1, 0]]
+    with pytest.raises(nw.NwprtError):        # flat mesh needs explicit bounds
+        ctx.gbuffer_cast(mesh, cam, frame)
+    with pytest.raises(nw.NwprtError):        # from == at
+        ctx.gbuffer_cast(mesh, [[0, 0, 0], [0, 0, 0], [0, 1, 0]], frame, bounds=[-1] * 3 + [1] * 3)
+    with pytest.raises(nw.NwprtError):        # up parallel to the view
+        ctx.gbuffer_cast(mesh, [[0, 0, 0], [0, 1, 0], [0, 1, 0]], frame, bounds=[-1] * 3 + [1] * 3)
+    with pytest.raises(nw.NwprtError):
+        ctx.gbuffer_cast(mesh, cam, frame, max_bounce=0, bounds=[-1] * 3 + [1] * 3)
+    ctx.gbuffer_cast(mesh, cam, frame, bounds=[-1, -1, -3, 1, 1, 0])
+    ctx.sync()
+    ctx.mesh_destroy(mesh)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.mesh_info(mesh)
+    assert e.value.status == nw.ERR_NOT_FOUND
+
+
+def test_cast_feeds_relight(nw, orc, ctx):
+    """The producer's G-buffer drives the relighting path unchanged: relight
+    on the cast frame matches the oracle relight of the same (downloaded)
+    G-buffer."""
+    from cases import make_case
+    w, h = 48, 32
+    case = make_case(nw, orc, width=w, height=h, k=64, seed=4)
+    s = mesh_scene(4)
+    mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
+                           material_ids=s["material_ids"])
+    frame = nw.DeviceFrame(w, h)
+    frame.bufs["direct"].upload(case.gbuf["direct"])
+    ctx.gbuffer_cast(mesh, np.reshape(s["camera"], (3, 3)), frame, bounds=s["bounds"])
+    scene = ctx.scene_create(case.desc, case.params)
+    env = ctx.envmap_from_faces(case.faces, case.k)
+    ctx.relight(scene, env, frame, nw.PREC_FP32, nw.TONEMAP_FULL)
+    ctx.sync()
+    res = frame.results()
+    g = _download(frame)
+    P = w * h
+    gb = {k: g[k].astype(np.float32) for k in ("position", "normal", "reflect", "kd", "ks")}
+    gb["roughness"] = g["roughness"].astype(np.float32)
+    gb["env_flag"] = g["env_flag"]
+    gb["direct"] = case.gbuf["direct"]
+    idx, st = ctx.envmap_selection(env)
+    ind, _ = case.oscene.relight(idx, st, gb)
+    assert 0 < g["env_flag"].sum() < P
+    ref = ind
+    got = res["indirect"][:, :3].astype(np.float64)
+    den = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+    assert np.max(np.abs(got - ref) / den) <= 1e-4
+    ctx.scene_destroy(scene)
+    ctx.envmap_destroy(env)
+    ctx.mesh_destroy(mesh)
