@@ -30,6 +30,9 @@ ENCODE_BYTES_PER_PIXEL = 16 + 1 + 16 * 8 * 2 * 2 + 32 * 2
 # (16 B each; the 25 taps are cache reuse) and the pass result written
 # (16 B); plus one colour read (16 B) for the HDR normalisation.
 DENOISE_BYTES_PER_PIXEL_PASS = 4 * 16
+# G-buffer producer: bytes written per pixel (position, normal, reflect,
+# kd, ks float4; roughness float; env flag byte).
+GBUFFER_WRITE_BYTES_PER_PIXEL = 5 * 16 + 4 + 1
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "fp8": 5000.0, "fp32": 157.3}
 METRIC = "(pixel,wavelet) MLP evals/s at 1080p K=300; p50/p99 relight latency vs plain"
@@ -58,6 +61,8 @@ def parse():
                     help="skip the two-slot frame pipeline leg")
     ap.add_argument("--no-denoise", action="store_true",
                     help="skip the denoiser leg")
+    ap.add_argument("--no-gbuffer", action="store_true",
+                    help="skip the G-buffer producer leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -200,6 +205,80 @@ def bench_serving(nw, ctx, prec, n_requests=60):
     return {"workload": "Poisson arrivals, 4 resident scenes, K=300, 70% yaw-rotated resident "
                         "6x128x128 cubemap / 30% fresh host cubemap, 60 requests per rate",
             "memory_budget_gb": 200, "closed_loop": single, "runs": runs}
+
+
+def gbuffer_bench_mesh():
+    """Synthetic scene for the G-buffer producer leg: a 768 x 672 heightfield
+    (1,032,192 triangles, per-vertex normals) under a 512 x 256 UV sphere
+    (262,144 triangles), three materials; ~1.29 M triangles in all."""
+    gx, gz = 768, 672
+    xs = np.linspace(-8.0, 8.0, gx + 1)
+    zs = np.linspace(-9.0, 5.0, gz + 1)
+    X, Z = np.meshgrid(xs, zs)
+    Y = -1.5 + 0.35 * np.sin(0.9 * X) * np.cos(0.7 * Z) + 0.1 * np.sin(3.1 * X + 2.3 * Z)
+    tv = np.stack([X, Y, Z], -1).reshape(-1, 3)
+    dYdx = 0.35 * 0.9 * np.cos(0.9 * X) * np.cos(0.7 * Z) + 0.31 * np.cos(3.1 * X + 2.3 * Z)
+    dYdz = -0.35 * 0.7 * np.sin(0.9 * X) * np.sin(0.7 * Z) + 0.23 * np.cos(3.1 * X + 2.3 * Z)
+    tn = np.stack([-dYdx, np.ones_like(X), -dYdz], -1).reshape(-1, 3)
+    tn /= np.linalg.norm(tn, axis=1, keepdims=True)
+    i0 = (np.arange(gz)[:, None] * (gx + 1) + np.arange(gx)[None, :]).reshape(-1)
+    ti = np.concatenate([np.stack([i0, i0 + gx + 1, i0 + 1], 1),
+                         np.stack([i0 + 1, i0 + gx + 1, i0 + gx + 2], 1)])
+    nu, nv = 512, 256
+    th = np.linspace(0, np.pi, nv + 1)
+    ph = np.linspace(0, 2 * np.pi, nu, endpoint=False)
+    T, PH = np.meshgrid(th, ph, indexing="ij")
+    sn = np.stack([np.sin(T) * np.cos(PH), np.cos(T), np.sin(T) * np.sin(PH)], -1).reshape(-1, 3)
+    sv = np.array([0.5, 0.2, -2.0]) + 1.6 * sn
+    a = (np.arange(nv)[:, None] * nu + np.arange(nu)[None, :]).reshape(-1)
+    b = (np.arange(nv)[:, None] * nu + (np.arange(nu)[None, :] + 1) % nu).reshape(-1)
+    si = np.concatenate([np.stack([a, a + nu, b], 1), np.stack([b, a + nu, b + nu], 1)]) + len(tv)
+    verts = np.concatenate([tv, sv]).astype(np.float32)
+    norms = np.concatenate([tn, sn]).astype(np.float32)
+    idx = np.concatenate([ti, si]).astype(np.int32)
+    mids = np.concatenate([(np.arange(len(ti)) // 4096) % 2,
+                           np.full(len(si), 2)]).astype(np.int32)
+    mats = [[0.45, 0.4, 0.3, 0.04, 0.04, 0.04, 0.6, 0], [0.3, 0.45, 0.25, 0.04, 0.04, 0.04, 0.8, 0],
+            [0.8, 0.2, 0.2, 0.5, 0.5, 0.5, 0.15, 0]]
+    return verts, norms, idx, mids, mats
+
+
+def bench_gbuffer(nw, ctx, W, H, stream, dfaces, n_face):
+    """G-buffer producer leg: one primary ray per pixel at W x H over the
+    ~1.29 M-triangle synthetic scene (BVH built on the host once)."""
+    from neural_wavelet_prt_demo_amd import Event
+    verts, norms, idx, mids, mats = gbuffer_bench_mesh()
+    t0 = time.perf_counter()
+    mesh = ctx.mesh_create(verts, idx, mats, normals=norms, material_ids=mids)
+    build_s = time.perf_counter() - t0
+    info = ctx.mesh_info(mesh)
+    gframe = nw.DeviceFrame(W, H)
+    cam = [[0.0, 2.5, 7.5], [0.0, -0.8, -2.0], [0.0, 1.0, 0.0]]
+    cast = lambda: ctx.gbuffer_cast(mesh, cam, gframe, env_faces=dfaces, env_n=n_face,
+                                    stream=stream)
+    for _ in range(3):
+        cast()
+    ctx.sync()
+    ev = [Event() for _ in range(11)]
+    ev[0].record(stream)
+    for i in range(10):
+        cast()
+        ev[i + 1].record(stream)
+    ctx.sync()
+    ms = [ev[i].elapsed_ms(ev[i + 1]) for i in range(10)]
+    cast_ms = float(np.mean(ms))
+    flags = gframe.bufs["env_flag"].download(np.uint8, (W * H,))
+    wb = W * H * GBUFFER_WRITE_BYTES_PER_PIXEL
+    out = {"workload": f"primary rays {W}x{H} over {info['triangles']} triangles "
+                       f"(heightfield + sphere), BVH {info['bvh_nodes']} nodes, depth "
+                       f"{info['bvh_depth']}",
+           "bvh_build_s": build_s, "ms": cast_ms, "p99_ms": percentile(ms, 99),
+           "rays_per_s": W * H / (cast_ms / 1e3), "hit_fraction": float(flags.mean()),
+           "write_bytes_per_pixel": GBUFFER_WRITE_BYTES_PER_PIXEL,
+           "write_gbs": wb / (cast_ms / 1e3) / 1e9}
+    ctx.mesh_destroy(mesh)
+    del gframe
+    return out
 
 
 def bench_pipelined(nw, ctx, scene, dfaces, frame, prec, k, n, steps, warmup):
@@ -384,6 +463,11 @@ def main():
                        "frac": dn_bytes / (dn_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
         del dn_out
 
+    # --- G-buffer producer (the step before the path) ----------------------
+    gbuffer_leg = None
+    if not args.no_gbuffer:
+        gbuffer_leg = bench_gbuffer(nw, ctx, W, H, stream, dfaces, N)
+
     # --- dominant kernel alone: the per-pair pass (p2) with HIP events -----
     kev = [Event() for _ in range(6)]
     kms = []
@@ -470,6 +554,8 @@ def main():
         result["pipelined_two_slot"] = pipelined
     if denoise_leg is not None:
         result["denoise_pass"] = denoise_leg
+    if gbuffer_leg is not None:
+        result["gbuffer_cast"] = gbuffer_leg
     if ceiling is not None and args.precision in ("bf16", "fp16"):
         # The published peak assumes the nominal clock; with real (toggling)
         # operands a pure MFMA loop on this GPU runs power-limited below it.
