@@ -445,6 +445,16 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
                                const nwprt_material *materials, int32_t n_materials,
                                int32_t *mesh_id);
 nwprt_status nwprt_mesh_get_info(nwprt_ctx ctx, int32_t mesh_id, nwprt_mesh_info *out);
+/* Host utility (no GPU): Wavefront OBJ (+ MTL) reader with what loadOBJ
+ * (Renderer/Model.cpp:73-146) keeps: polygons fanned into triangles, one
+ * vertex per distinct (position, normal) corner, materials as cookTorrance
+ * (type 2: kd = Kd, ks = Ks, roughness = (1 / Ns)^2, 0 for Ns = 0; faces
+ * without a material: kd = ks = 0.5, roughness 1).  *normals is NULL when
+ * the file has none.  Arrays are released with nwprt_free_host. */
+nwprt_status nwprt_obj_read(const char *path, float **vertices, int64_t *n_vertices,
+                            float **normals, int32_t **indices, int64_t *n_triangles,
+                            int32_t **material_ids, nwprt_material **materials,
+                            int32_t *n_materials);
 nwprt_status nwprt_mesh_destroy(nwprt_ctx ctx, int32_t mesh_id);
 
 typedef struct {

@@ -110,6 +110,36 @@ class RenderEngine {
     upload(f.fbEnvironmentMapFlag, flag.data(), P);
   }
 
+  // Scene geometry for render(): a mesh from nwprt_mesh_create, and the
+  // bounds that normalise first-hit positions (scene->bounds,
+  // Renderer/RenderEngine.cpp:104-105); null bounds: the mesh's own.
+  void setMesh(int32_t meshId, const float *bounds = nullptr) {
+    mesh_ = meshId;
+    hasBounds_ = bounds != nullptr;
+    if (bounds)
+      for (int i = 0; i < 6; ++i) bounds_[i] = bounds[i];
+  }
+  // RenderEngine::setCamera (Renderer/RenderEngine.cpp:246-267).
+  void setCamera(const nwprt_camera &camera) { camera_ = camera; }
+
+  // RenderEngine::render (Renderer/RenderEngine.cpp:328-353), G-buffer part
+  // in wavelet mode (maximumBounce 1): first-hit encodings of frameID from
+  // the mesh and camera.  viewEnvFaces (device [6][n][n][3], optional) gives
+  // the kd of pixels that see the environment.  The direct-light buffer is
+  // not written.
+  void render(int frameID, hipStream_t stream = nullptr, const float *viewEnvFaces = nullptr,
+              int viewEnvN = 0) {
+    nwprt_gbuffer g = gbuffer(frameID);
+    nwprt_cast_params p{1, hasBounds_ ? bounds_ : nullptr, viewEnvFaces, viewEnvN};
+    check(nwprt_gbuffer_cast(ctx_, mesh_, &camera_, width_, height_, &p, &g, stream));
+  }
+
+  // Zero frameID's direct-light buffer (for G-buffers from render(), which
+  // does not produce the direct term).
+  void clearDirectLight(int frameID) {
+    hipCheck(hipMemset(frame(frameID).fbDirectDenoised, 0, (size_t)width_ * height_ * 16));
+  }
+
   // RenderEngine::initTCNN (Renderer/tinycudann/tcnn.cu:49-81).
   void initTCNN(const std::string &modelPath) {
     int32_t id = -1;
@@ -268,6 +298,10 @@ class RenderEngine {
   int device_;
   nwprt_ctx ctx_ = nullptr;
   int32_t scene_ = -1;
+  int32_t mesh_ = -1;
+  nwprt_camera camera_{{0, 0, 0}, {0, 0, -1}, {0, 1, 0}};
+  float bounds_[6] = {0, 0, 0, 0, 0, 0};
+  bool hasBounds_ = false;
   std::vector<int32_t> envs_;
   int width_ = 0, height_ = 0;
   fullFrameBuffers frames_[2];

@@ -207,6 +207,9 @@ _SIGNATURES = {
     "nwprt_mesh_create": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64,
                                   c_void_p, POINTER(Material), c_int32, POINTER(c_int32)]),
     "nwprt_mesh_get_info": (c_int, [c_void_p, c_int32, POINTER(MeshInfo)]),
+    "nwprt_obj_read": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
+                               POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
+                               POINTER(c_void_p), POINTER(c_int32)]),
     "nwprt_mesh_destroy": (c_int, [c_void_p, c_int32]),
     "nwprt_gbuffer_cast": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32,
                                    POINTER(CastParams), POINTER(GBuffer), c_void_p]),
@@ -327,6 +330,27 @@ def synth_gbuffer(seed: int, width: int, height: int,
     gb = GBuffer(**{k: _ptr(v) for k, v in g.items()})
     _check(lib().nwprt_synth_gbuffer(seed, width, height, background_fraction, ctypes.byref(gb)))
     return g
+
+
+def obj_read(path: str) -> Dict[str, np.ndarray]:
+    """Wavefront OBJ (+ MTL) as mesh arrays: vertices [n][3], normals [n][3]
+    or None, indices [t][3], material_ids [t], materials [m][8] rows
+    (kd[3], ks[3], roughness, type)."""
+    v, nr, ix, mid, mats = (c_void_p() for _ in range(5))
+    nv, nt, nm = c_int64(), c_int64(), c_int32()
+    _check(lib().nwprt_obj_read(str(path).encode(), ctypes.byref(v), ctypes.byref(nv),
+                                ctypes.byref(nr), ctypes.byref(ix), ctypes.byref(nt),
+                                ctypes.byref(mid), ctypes.byref(mats), ctypes.byref(nm)))
+    out = {"vertices": _take_host(v, nv.value * 3, np.float32).reshape(-1, 3),
+           "normals": (_take_host(nr, nv.value * 3, np.float32).reshape(-1, 3)
+                       if nr.value else None),
+           "indices": _take_host(ix, nt.value * 3, np.int32).reshape(-1, 3),
+           "material_ids": _take_host(mid, nt.value, np.int32)}
+    raw = _take_host(mats, nm.value * 8, np.float32).reshape(-1, 8)
+    # The struct's last field is an int32 type: reinterpret that column.
+    out["materials"] = np.concatenate(
+        [raw[:, :7], raw[:, 7:8].view(np.int32).astype(np.float32)], axis=1)
+    return out
 
 
 def synth_cubemap(seed: int, face_size: int) -> np.ndarray:

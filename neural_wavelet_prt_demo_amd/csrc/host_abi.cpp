@@ -1,6 +1,10 @@
 // C ABI of the host-only utilities (no GPU use): level table, parameter
-// counts, seeded inputs and file formats.
+// counts, seeded inputs and file formats (npy, hdr, cube crosses, OBJ).
+#include <cmath>
 #include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
 
 #include "nwprt_internal.hpp"
 
@@ -161,3 +165,182 @@ nwprt_status nwprt_cubemap_cross_to_faces(const float *cross, int32_t width, int
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ OBJ --
+// Wavefront OBJ + MTL reader for the G-buffer producer's meshes, following
+// what loadOBJ (Renderer/Model.cpp:73-146, tinyobjloader with triangulation)
+// keeps: positions, normals, polygon faces fanned into triangles, one
+// vertex per distinct (position, normal) pair, and per-face materials made
+// into cookTorrance materials (kd = Kd, ks = Ks, roughness = (1 / Ns)^2, 0
+// for Ns = 0; faces without a material get cookTorranceMaterial's defaults
+// kd = ks = 0.5, roughness 1, Renderer/Material.h:136-143).  Texture maps
+// are not read.
+namespace {
+
+struct ObjMesh {
+  std::vector<float> vertices, normals;
+  std::vector<int32_t> indices, material_ids;
+  std::vector<nwprt_material> materials;
+  bool has_normals = false;
+};
+
+nwprt_material cook_torrance(float kd0, float kd1, float kd2, float ks0, float ks1, float ks2,
+                             float ns) {
+  nwprt_material m{};
+  m.kd[0] = kd0, m.kd[1] = kd1, m.kd[2] = kd2;
+  m.ks[0] = ks0, m.ks[1] = ks1, m.ks[2] = ks2;
+  m.roughness = ns == 0.f ? 0.f : (float)std::pow(1.0 / ns, 2.0);
+  m.type = 2;  // MatType::cookTorrance
+  return m;
+}
+
+std::vector<std::string> split_ws(const std::string &line) {
+  std::vector<std::string> out;
+  std::istringstream ss(line);
+  for (std::string w; ss >> w;) out.push_back(w);
+  return out;
+}
+
+void read_mtl(const std::string &path, std::map<std::string, int> &names,
+              std::vector<nwprt_material> &mats) {
+  std::ifstream in(path);
+  require((bool)in, NWPRT_ERR_IO, "obj: cannot open material library " + path);
+  int cur = -1;
+  float ns = 0.f;
+  auto close = [&] {
+    if (cur >= 0) mats[(size_t)cur].roughness = ns == 0.f ? 0.f : (float)std::pow(1.0 / ns, 2.0);
+  };
+  for (std::string line; std::getline(in, line);) {
+    const auto t = split_ws(line);
+    if (t.empty() || t[0][0] == '#') continue;
+    if (t[0] == "newmtl" && t.size() > 1) {
+      close();
+      names[t[1]] = cur = (int)mats.size();
+      // tinyobjloader's material defaults: Kd = Ks = 0, Ns = 1.
+      mats.push_back(cook_torrance(0, 0, 0, 0, 0, 0, 1.f));
+      ns = 1.f;
+    } else if (cur >= 0 && (t[0] == "Kd" || t[0] == "Ks") && t.size() >= 4) {
+      float *dst = t[0] == "Kd" ? mats[(size_t)cur].kd : mats[(size_t)cur].ks;
+      for (int k = 0; k < 3; ++k) dst[k] = std::stof(t[(size_t)k + 1]);
+    } else if (cur >= 0 && t[0] == "Ns" && t.size() >= 2) {
+      ns = std::stof(t[1]);
+    }
+  }
+  close();
+}
+
+ObjMesh read_obj(const std::string &path) {
+  std::ifstream in(path);
+  require((bool)in, NWPRT_ERR_IO, "obj: cannot open " + path);
+  const std::string dir = [&] {
+    const size_t s = path.find_last_of('/');
+    return s == std::string::npos ? std::string() : path.substr(0, s + 1);
+  }();
+  std::vector<float> pos, nrm;
+  std::map<std::string, int> names;
+  std::vector<nwprt_material> mats;
+  std::map<std::pair<int, int>, int32_t> known;
+  ObjMesh m;
+  int cur_mat = -1, default_mat = -1;
+  bool any_normal = false;
+  std::vector<std::pair<int, int>> corner_keys;
+  std::vector<int> face_mat;
+  auto resolve = [](long v, size_t count) -> int {
+    const long n = (long)count;
+    const long i = v > 0 ? v - 1 : n + v;
+    require(v != 0 && i >= 0 && i < n, NWPRT_ERR_FORMAT, "obj: index out of range");
+    return (int)i;
+  };
+  size_t line_no = 0;
+  for (std::string line; std::getline(in, line);) {
+    ++line_no;
+    const auto t = split_ws(line);
+    if (t.empty() || t[0][0] == '#') continue;
+    try {
+      if (t[0] == "v" && t.size() >= 4) {
+        for (int k = 1; k <= 3; ++k) pos.push_back(std::stof(t[(size_t)k]));
+      } else if (t[0] == "vn" && t.size() >= 4) {
+        for (int k = 1; k <= 3; ++k) nrm.push_back(std::stof(t[(size_t)k]));
+      } else if (t[0] == "mtllib" && t.size() > 1) {
+        read_mtl(dir + t[1], names, mats);
+      } else if (t[0] == "usemtl" && t.size() > 1) {
+        auto it = names.find(t[1]);
+        cur_mat = it == names.end() ? -1 : it->second;
+      } else if (t[0] == "f" && t.size() >= 4) {
+        std::vector<std::pair<int, int>> poly;
+        for (size_t c = 1; c < t.size(); ++c) {
+          const std::string &s = t[c];
+          const size_t a = s.find('/');
+          const int vi = resolve(std::stol(s.substr(0, a)), pos.size() / 3);
+          int ni = -1;
+          if (a != std::string::npos) {
+            const size_t b = s.find('/', a + 1);
+            if (b != std::string::npos && b + 1 < s.size())
+              ni = resolve(std::stol(s.substr(b + 1)), nrm.size() / 3);
+          }
+          if (ni >= 0) any_normal = true;
+          poly.emplace_back(vi, ni);
+        }
+        for (size_t c = 1; c + 1 < poly.size(); ++c) {
+          corner_keys.push_back(poly[0]);
+          corner_keys.push_back(poly[c]);
+          corner_keys.push_back(poly[c + 1]);
+          face_mat.push_back(cur_mat);
+        }
+      }
+    } catch (const Error &) {
+      throw;
+    } catch (const std::exception &) {
+      fail(NWPRT_ERR_FORMAT, "obj: bad line " + std::to_string(line_no) + " of " + path);
+    }
+  }
+  require(!face_mat.empty(), NWPRT_ERR_FORMAT, "obj: no faces in " + path);
+  for (int &fm : face_mat)
+    if (fm < 0) {
+      if (default_mat < 0) {
+        default_mat = (int)mats.size();
+        mats.push_back(cook_torrance(0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 1.f));
+      }
+      fm = default_mat;
+    }
+  for (const auto &key : corner_keys) {
+    auto it = known.find(key);
+    if (it == known.end()) {
+      const int32_t id = (int32_t)(m.vertices.size() / 3);
+      for (int k = 0; k < 3; ++k) m.vertices.push_back(pos[3 * (size_t)key.first + k]);
+      if (any_normal)
+        for (int k = 0; k < 3; ++k)
+          // Corners without a normal take the zero vector (geometric
+          // fallback is per triangle, not per vertex).
+          m.normals.push_back(key.second >= 0 ? nrm[3 * (size_t)key.second + k] : 0.f);
+      it = known.emplace(key, id).first;
+    }
+    m.indices.push_back(it->second);
+  }
+  m.material_ids.assign(face_mat.begin(), face_mat.end());
+  m.materials = mats;
+  m.has_normals = any_normal;
+  return m;
+}
+
+}  // namespace
+
+extern "C" nwprt_status nwprt_obj_read(const char *path, float **vertices, int64_t *n_vertices,
+                                       float **normals, int32_t **indices, int64_t *n_triangles,
+                                       int32_t **material_ids, nwprt_material **materials,
+                                       int32_t *n_materials) {
+  return host_guarded([&] {
+    require(path && vertices && n_vertices && normals && indices && n_triangles &&
+                material_ids && materials && n_materials,
+            NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    const ObjMesh m = read_obj(path);
+    *vertices = copy_out(m.vertices);
+    *normals = m.has_normals ? copy_out(m.normals) : nullptr;
+    *indices = copy_out(m.indices);
+    *material_ids = copy_out(m.material_ids);
+    *materials = copy_out(m.materials);
+    *n_vertices = (int64_t)(m.vertices.size() / 3);
+    *n_triangles = (int64_t)(m.indices.size() / 3);
+    *n_materials = (int32_t)m.materials.size();
+  });
+}

@@ -8,12 +8,18 @@
 // <i>_<name>.hdr, rows flipped as OutputWriter::writeRGBA does
 // (Renderer/OutputWriter.cpp:24-40, 146-149).
 //
-// The OptiX G-buffer pass is not part of this build: frame i's G-buffer is
-// the seeded synthetic one with seed (--seed + i); the camera columns of the
-// trajectory are parsed and ignored.
+// G-buffers: with --obj, frame i's G-buffer comes from the ray caster over
+// the OBJ mesh, seen from the trajectory's camera (from = columns 0-2, at =
+// columns 3-5, up +y, Renderer/offlineRenderer.cpp:118), positions
+// normalised by the mesh's bounds; its direct-light buffer is zero (the
+// reference's Monte Carlo direct term is not produced), so
+// predictedWithDirect equals predictedIndirect.  Without --obj, frame i's
+// G-buffer is the seeded synthetic one with seed (--seed + i) and the camera
+// columns are parsed and ignored.
 //
 //   nwprt_offline <results_dir> -w <model_dir> -m <env_folder> -t <trajectory>
 //                 [-s start] [-e end] [-l] [-k K] [--seed S] [--precision bf16|fp16|fp32]
+//                 [--obj scene.obj]
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +39,7 @@ struct Trajectory {
   std::string envmapFolder;
   bool finalDenoiser = false;
   std::vector<std::string> envmaps;
+  std::vector<nwprt_camera> cameras;
 };
 
 // Renderer/offlineRenderer.cpp:70-123: keywords before "trajectoryBegin",
@@ -69,7 +76,13 @@ Trajectory readTrajectory(const std::string &path) {
     } else {
       if (!started) throw std::runtime_error("Unknown things before trajectory start " + tok[0]);
       if (tok.size() < 7) throw std::runtime_error("trajectory line needs 7 fields: " + line);
-      for (int i = 0; i < 6; ++i) (void)std::stof(tok[(size_t)i]);  // camera, validated only
+      nwprt_camera cam{};
+      for (int i = 0; i < 3; ++i) {
+        cam.from[i] = std::stof(tok[(size_t)i]);
+        cam.at[i] = std::stof(tok[(size_t)i + 3]);
+      }
+      cam.up[1] = 1.f;
+      t.cameras.push_back(cam);
       t.envmaps.push_back(tok[6]);
     }
   }
@@ -88,7 +101,7 @@ void writeFlipped(const fs::path &file, std::vector<float> rgba, int w, int h) {
 }  // namespace
 
 int main(int argc, char **argv) {
-  std::string results, model, envFolder, trajFile, precision = "bf16";
+  std::string results, model, envFolder, trajFile, precision = "bf16", obj;
   int start = 0, end = 2000, k = 64;
   bool large = false;
   uint64_t seed = 0;
@@ -108,6 +121,7 @@ int main(int argc, char **argv) {
       else if (a == "-l") large = true;
       else if (a == "--seed") seed = std::stoull(next());
       else if (a == "--precision") precision = next();
+      else if (a == "--obj") obj = next();
       else if (!a.empty() && a[0] != '-' && results.empty()) results = a;
       else throw std::runtime_error("unknown argument " + a);
     } catch (const std::exception &e) {
@@ -117,7 +131,8 @@ int main(int argc, char **argv) {
   }
   if (results.empty() || model.empty() || trajFile.empty()) {
     std::cerr << "usage: nwprt_offline <results_dir> -w <model_dir> -m <env_folder> -t "
-                 "<trajectory> [-s start] [-e end] [-l] [-k K] [--seed S] [--precision p]\n";
+                 "<trajectory> [-s start] [-e end] [-l] [-k K] [--seed S] [--precision p] "
+                 "[--obj scene.obj]\n";
     return 2;
   }
   try {
@@ -132,10 +147,33 @@ int main(int argc, char **argv) {
     nwprt::RenderEngine engine(0);
     engine.resize(w, h);
     engine.initTCNN(model);
+    int32_t mesh = -1;
+    if (!obj.empty()) {
+      float *v = nullptr, *n = nullptr;
+      int32_t *ix = nullptr, *mid = nullptr, nm = 0;
+      nwprt_material *mats = nullptr;
+      int64_t nv = 0, nt = 0;
+      auto need = [](nwprt_status st, const char *what) {
+        if (st != NWPRT_OK) throw std::runtime_error(std::string(what) + ": " + nwprt_last_error());
+      };
+      need(nwprt_obj_read(obj.c_str(), &v, &nv, &n, &ix, &nt, &mid, &mats, &nm), "reading OBJ");
+      const nwprt_status st =
+          nwprt_mesh_create(engine.context(), v, nv, n, ix, nt, mid, mats, nm, &mesh);
+      for (void *p : {(void *)v, (void *)n, (void *)ix, (void *)mid, (void *)mats})
+        nwprt_free_host(p);
+      need(st, "uploading the mesh");
+      engine.setMesh(mesh);
+      engine.clearDirectLight(1);
+    }
     end = std::min<int>(end, (int)traj.envmaps.size());
     for (int i = start; i < end; ++i) {
       engine.buildLights({{envFolder, traj.envmaps[(size_t)i]}}, k);
-      engine.writeSyntheticGBuffer(1, seed + (uint64_t)i);
+      if (mesh >= 0) {
+        engine.setCamera(traj.cameras[(size_t)i]);
+        engine.render(1);
+      } else {
+        engine.writeSyntheticGBuffer(1, seed + (uint64_t)i);
+      }
       engine.TCNNPass_p1(1, nullptr, cfg);
       engine.TCNNPass_p2(0, 1, cfg);
       std::vector<float> ind, wd;
@@ -146,6 +184,7 @@ int main(int argc, char **argv) {
       std::cout << "Trajectory: " << i << " Done. Total is: " << end << std::endl;
     }
     engine.destroyLights();
+    if (mesh >= 0) nwprt_mesh_destroy(engine.context(), mesh);
   } catch (const std::exception &e) {
     std::cerr << "nwprt_offline: " << e.what() << "\n";
     return 1;

@@ -277,3 +277,42 @@ def test_scene_generator(nw):
     w1 = p["layers"][:64 * 32]
     assert abs(w1.std() - np.sqrt(2 / 32)) < 0.01
     assert abs(p["projection"].std() - np.sqrt(1 / 32)) < 0.01
+
+
+def test_obj_reader(nw, tmp_path):
+    """nwprt_obj_read keeps what the reference's loadOBJ keeps
+    (Renderer/Model.cpp:73-146): polygons fanned, one vertex per distinct
+    (position, normal) corner, cookTorrance materials from the MTL
+    (roughness = (1 / Ns)^2), defaults for faces without a material."""
+    (tmp_path / "m.mtl").write_text(
+        "newmtl red\nKd 0.8 0.1 0.1\nKs 0.2 0.2 0.2\nNs 4\n"
+        "newmtl flat\nKd 0.3 0.3 0.3\nNs 0\n")
+    (tmp_path / "s.obj").write_text(
+        "# test\nmtllib m.mtl\n"
+        "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0 0 1\n"
+        "vn 0 0 1\nvn 0 1 0\n"
+        "f 1 2 3\n"                      # before any usemtl: default material
+        "usemtl red\nf 1//1 2//1 3//1 4//1\n"   # quad -> 2 triangles
+        "usemtl flat\nf -5/1/2 -4/1/2 -1/1/2\n")  # negative indices, v/vt/vn
+    m = nw.obj_read(tmp_path / "s.obj")
+    assert m["indices"].shape == (4, 3)
+    # Distinct (position, normal) corners: (1,-),(2,-),(3,-),(1,1),(2,1),(3,1),(4,1),(1,2),(2,2),(5,2)
+    assert len(m["vertices"]) == 10 and m["normals"] is not None
+    v, ix = m["vertices"], m["indices"]
+    np.testing.assert_array_equal(v[ix[1]], [[0, 0, 0], [1, 0, 0], [1, 1, 0]])
+    np.testing.assert_array_equal(v[ix[2]], [[0, 0, 0], [1, 1, 0], [0, 1, 0]])
+    np.testing.assert_array_equal(v[ix[3]], [[0, 0, 0], [1, 0, 0], [0, 0, 1]])
+    np.testing.assert_array_equal(m["normals"][ix[3]], [[0, 1, 0]] * 3)
+    np.testing.assert_array_equal(m["normals"][ix[0]], [[0, 0, 0]] * 3)  # corners without vn
+    mats, mid = m["materials"], m["material_ids"]
+    assert list(mid) == [2, 0, 0, 1]
+    np.testing.assert_allclose(mats[0], [0.8, 0.1, 0.1, 0.2, 0.2, 0.2, 1 / 16, 2], rtol=1e-6)
+    np.testing.assert_allclose(mats[1], [0.3, 0.3, 0.3, 0, 0, 0, 0, 2])
+    np.testing.assert_allclose(mats[2], [0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1, 2])
+    (tmp_path / "bad.obj").write_text("v 0 0 0\nf 1 2 3\n")
+    with pytest.raises(nw.NwprtError) as e:
+        nw.obj_read(tmp_path / "bad.obj")
+    assert e.value.status == nw.ERR_FORMAT
+    with pytest.raises(nw.NwprtError) as e:
+        nw.obj_read(tmp_path / "missing.obj")
+    assert e.value.status == nw.ERR_IO

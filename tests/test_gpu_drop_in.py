@@ -200,3 +200,68 @@ def test_offline_driver_with_denoiser(nw, ctx, tmp_path):
         if key == "indirect":
             plain = frame.results()[key].reshape(512, 512, 4)
             assert not np.array_equal(img[..., :3], plain[..., :3])  # it did filter
+
+
+def test_offline_driver_with_obj_scene(nw, ctx, tmp_path):
+    """--obj: frame i's G-buffer comes from the ray caster over the OBJ mesh
+    seen from trajectory camera i (up +y, Renderer/offlineRenderer.cpp:118),
+    with a zero direct term; the written images equal obj_read ->
+    mesh_create -> gbuffer_cast -> relight through the C ABI."""
+    import subprocess
+    from pathlib import Path
+    from cases import mesh_scene
+    exe = Path(nw.__file__).resolve().with_name("nwprt_offline")
+    rng = np.random.default_rng(6)
+    n_face, K = 8, 16
+    _write_model_folder(tmp_path / "model", rng, n_face)
+    env_dir = tmp_path / "env"
+    (env_dir / "cube").mkdir(parents=True)
+    nw.hdr_write(env_dir / "cube" / "a.hdr", _faces_to_cross(nw.synth_cubemap(5, n_face)))
+    s = mesh_scene(8, mirror=False)
+    lines = ["mtllib scene.mtl"]
+    lines += [f"v {a:.9g} {b:.9g} {c:.9g}" for a, b, c in s["vertices"]]
+    lines += [f"vn {a:.9g} {b:.9g} {c:.9g}" for a, b, c in s["normals"]]
+    cur = None
+    for t, m in zip(s["indices"] + 1, s["material_ids"]):
+        if m != cur:
+            lines.append(f"usemtl m{m}")
+            cur = m
+        lines.append("f " + " ".join(f"{i}//{i}" for i in t))
+    (tmp_path / "scene.obj").write_text("\n".join(lines) + "\n")
+    mtl = []
+    for i, row in enumerate(s["materials"]):
+        ns = 1.0 / np.sqrt(row[6]) if row[6] > 0 else 0.0
+        mtl += [f"newmtl m{i}", "Kd %.9g %.9g %.9g" % tuple(row[0:3]),
+                "Ks %.9g %.9g %.9g" % tuple(row[3:6]), f"Ns {ns:.9g}"]
+    (tmp_path / "scene.mtl").write_text("\n".join(mtl) + "\n")
+    cams = [(0.5, 1.0, 6.0, 0.0, 0.0, -1.5), (-2.0, 2.5, 5.0, 0.5, -0.5, -2.0)]
+    traj = tmp_path / "traj.txt"
+    traj.write_text(f"envmapFolder {env_dir}\ntrajectoryBegin\n" +
+                    "".join(" ".join(map(str, c)) + " a\n" for c in cams))
+    out = tmp_path / "out"
+    r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-t", str(traj),
+                        "-k", str(K), "--precision", "fp32", "--obj", str(tmp_path / "scene.obj")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = nw.obj_read(tmp_path / "scene.obj")
+    assert np.all(m["materials"][:, 7] == 2)  # cookTorrance
+    mesh = ctx.mesh_create(m["vertices"], m["indices"], m["materials"], normals=m["normals"],
+                           material_ids=m["material_ids"])
+    scene = ctx.initTCNN(tmp_path / "model")
+    env = ctx.envmap_load_folder(env_dir, "a", K)
+    for i, c in enumerate(cams):
+        frame = nw.DeviceFrame(512, 512)
+        frame.bufs["direct"].upload(np.zeros((512 * 512, 4), np.float32))
+        ctx.gbuffer_cast(mesh, (c[0:3], c[3:6], (0, 1, 0)), frame)
+        ctx.relight(scene, env, frame, nw.PREC_FP32)
+        ctx.sync()
+        flags = frame.bufs["env_flag"].download(np.uint8, (512 * 512,))
+        assert 0.05 < flags.mean() < 1.0
+        res = frame.results()
+        for key, folder in (("indirect", "predictedIndirect"),
+                            ("indirect_with_direct", "predictedWithDirect")):
+            want = tmp_path / f"want_{i}_{key}.hdr"
+            nw.hdr_write(want, res[key].reshape(512, 512, 4)[::-1])
+            got = nw.hdr_read(out / folder / f"{i}_a.hdr")
+            np.testing.assert_array_equal(got, nw.hdr_read(want))
+    ctx.mesh_destroy(mesh)
