@@ -468,6 +468,13 @@ typedef struct {
                                   by primary rays that miss (kd = albedo =
                                   radiance), or NULL (zero)                 */
   int32_t view_env_n;
+  const float *light_env_faces;/* device [6][n][n][3] lighting environment
+                                  for the direct term, or NULL: `direct` is
+                                  then not written                          */
+  int32_t light_env_n;         /* 1..128                                    */
+  int32_t direct_samples;      /* estimates averaged per pixel, each one
+                                  environment sample + one BSDF sample      */
+  uint32_t seed;               /* random stream of the direct term          */
 } nwprt_cast_params;
 
 /* One primary ray per pixel through the pixel's sample position at the
@@ -482,8 +489,17 @@ typedef struct {
  *   ks 0, roughness 1, env_flag 0;
  * - a mirror path cut by max_bounce: position -lower / (upper - lower),
  *   normal and reflect 0.5, kd, ks, roughness 0, env_flag 0.
- * The w components are 1.  `direct` is not written: the Monte Carlo direct
- * light of the reference's path tracer is not produced here. */
+ * The w components are 1.
+ * direct (when light_env_faces is given and gbuffer->direct is not NULL):
+ * the closest-hit program's direct term in wavelet mode
+ * (devicePrograms.cu:165-280): per sample, one environment-importance
+ * sample (face and texel tables of EnvironmentMap.cpp:153-190) and one BSDF
+ * sample, each shadow-tested and weighted by the power heuristic, averaged
+ * over direct_samples; misses store the view radiance, as kd.  Materials
+ * must be cookTorrance (type 2) or mirrors (NWPRT_ERR_UNSUPPORTED
+ * otherwise).  Random numbers come from a counter-based generator keyed by
+ * (seed, pixel, sample), not the reference's per-pixel LCG: estimates agree
+ * with the reference's in expectation, not sample by sample. */
 nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
                                 int32_t width, int32_t height,
                                 const nwprt_cast_params *params,

@@ -132,7 +132,8 @@ class MeshInfo(Structure):
 
 class CastParams(Structure):
     _fields_ = [("max_bounce", c_int32), ("bounds", c_void_p), ("view_env_faces", c_void_p),
-                ("view_env_n", c_int32)]
+                ("view_env_n", c_int32), ("light_env_faces", c_void_p), ("light_env_n", c_int32),
+                ("direct_samples", c_int32), ("seed", c_uint32)]
 
 
 MATERIAL_MIRROR = 3
@@ -679,15 +680,21 @@ class Context:
         _check(lib().nwprt_mesh_destroy(self.h, mesh))
 
     def gbuffer_cast(self, mesh: int, camera, frame, *, max_bounce: int = 1, bounds=None,
-                     env_faces=None, env_n: int = 0, stream: Optional[int] = None) -> None:
+                     env_faces=None, env_n: int = 0, light_faces=None, light_n: int = 0,
+                     direct_samples: int = 1, seed: int = 0,
+                     stream: Optional[int] = None) -> None:
         """Primary-ray G-buffer of `frame` (DeviceFrame) from `mesh` seen by
-        camera = (from, at, up); the frame's `direct` buffer is left as is.
-        env_faces: device [6][n][n][3] FP32 faces (DeviceBuffer or address)."""
+        camera = (from, at, up).  env_faces: view environment, device
+        [6][n][n][3] FP32 faces (DeviceBuffer or address).  light_faces: the
+        lighting environment of the direct term (written to the frame's
+        `direct` buffer), or None to leave that buffer as is."""
         f, a, u = (np.asarray(x, np.float32).reshape(3) for x in camera)
         cam = Camera((c_float * 3)(*f), (c_float * 3)(*a), (c_float * 3)(*u))
         b = None if bounds is None else np.ascontiguousarray(bounds, np.float32).reshape(6)
         env = env_faces.ptr if isinstance(env_faces, DeviceBuffer) else env_faces
-        params = CastParams(max_bounce, _ptr(b), env, env_n)
+        light = light_faces.ptr if isinstance(light_faces, DeviceBuffer) else light_faces
+        params = CastParams(max_bounce, _ptr(b), env, env_n, light, light_n, direct_samples,
+                            seed & 0xFFFFFFFF)
         gb = frame.gbuffer()
         _check(lib().nwprt_gbuffer_cast(self.h, mesh, ctypes.byref(cam), frame.width,
                                         frame.height, ctypes.byref(params), ctypes.byref(gb),

@@ -82,6 +82,7 @@ struct Mesh {
   int64_t n_tris = 0, n_nodes = 0;
   int depth = 0;
   bool has_normals = false, has_tri_material = false;
+  bool direct_ok = true;  // every material has a direct-light BSDF here
 };
 
 // Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
@@ -147,6 +148,7 @@ struct nwprt_context {
   int n_cu = 0;  // compute units of the device (persistent grid size)
   nwprt::DevBuf denoise_scratch;  // denoiser ping-pong images + normalisation
   std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
+  nwprt::DevBuf env_tables;  // direct-light sampling tables of the last cast
 };
 
 namespace nwprt {

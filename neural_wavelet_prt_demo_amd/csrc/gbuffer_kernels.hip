@@ -167,7 +167,415 @@ __device__ inline float4 enc(float3 v) {
   return make_float4(v.x * 0.5f + 0.5f, v.y * 0.5f + 0.5f, v.z * 0.5f + 0.5f, 1.0f);
 }
 
+// Any hit along o + t d, t in (kTmin, kTmax): the shadow-ray query
+// (OPTIX_RAY_FLAG_TERMINATE_ON_FIRST_HIT, devicePrograms.cu:176-196).
+__device__ inline bool occluded(const BvhNode *__restrict__ nodes, const BvhTri *__restrict__ tris,
+                                float3 o, float3 d, int *stack) {
+  auto safe_inv = [](float x) { return 1.0f / (fabsf(x) > 1e-30f ? x : copysignf(1e-30f, x)); };
+  const float3 inv = v3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+  int node = 0, sp = 0;
+  for (;;) {
+    const float4 *nr = reinterpret_cast<const float4 *>(nodes + node);
+    const float4 n0 = nr[0], n1 = nr[1], n2 = nr[2], n3 = nr[3];
+    const int c0 = __float_as_int(n0.w), c1 = __float_as_int(n1.w);
+    float t0, t1;
+    bool h0 = box_entry(v3(n0), v3(n1), o, inv, kTmax, t0);
+    bool h1 = box_entry(v3(n2), v3(n3), o, inv, kTmax, t1);
+    Hit h{kTmax, 0.0f, 0.0f, INT_MAX, 0};
+    if (h0 && c0 < 0) {
+      intersect_leaf(tris, ~c0, o, d, h);
+      h0 = false;
+    }
+    if (h1 && c1 < 0) {
+      intersect_leaf(tris, ~c1, o, d, h);
+      h1 = false;
+    }
+    if (h.prim != INT_MAX) return true;
+    if (h0 && h1) {
+      if (sp < kBvhMaxDepth) stack[64 * sp++] = c1;
+      node = c0;
+    } else if (h0) {
+      node = c0;
+    } else if (h1) {
+      node = c1;
+    } else {
+      if (sp == 0) return false;
+      node = stack[64 * --sp];
+    }
+  }
+}
+
+// ------------------------------------------------------------ direct light
+// The closest-hit program's direct term in wavelet mode (maximumBounce 1,
+// devicePrograms.cu:165-280): per light sample, one environment-importance
+// sample and one BSDF sample, each shadow-tested and weighted by the power
+// heuristic, for the cookTorrance BSDF (cookTorrance.cu:26-195).  Random
+// numbers come from a counter-based generator (SplitMix64's finaliser over
+// seed, pixel and a per-sample counter) so a host restatement can replay
+// every sample.
+
+__device__ inline float rng_uniform(uint32_t seed, uint32_t pixel, uint32_t counter) {
+  uint64_t z = (((uint64_t)seed << 32) | pixel) + 0x9E3779B97F4A7C15ull * ((uint64_t)counter + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (float)(uint32_t)(z >> 40) * 5.9604644775390625e-8f;  // 2^-24
+}
+
+__device__ inline float luminance(float3 c) {
+  return __fadd_rn(__fadd_rn(__fmul_rn(c.x, 0.212671f), __fmul_rn(c.y, 0.715160f)),
+                   __fmul_rn(c.z, 0.072169f));
+}
+
+// Orthonormal frame around a unit normal (Frisvad 2012), as the reference's
+// to_local / to_world use it (shaderUtils.h:16-46).
+__device__ inline void frame(float3 n, float3 &t, float3 &b) {
+  if (n.z < -1.0f + 1e-6f) {
+    t = v3(0.0f, -1.0f, 0.0f);
+    b = v3(-1.0f, 0.0f, 0.0f);
+  } else {
+    const float a = 1.0f / (1.0f + n.z), c = -n.x * n.y * a;
+    t = v3(1.0f - n.x * n.x * a, c, -n.x);
+    b = v3(c, 1.0f - n.y * n.y * a, -n.y);
+  }
+}
+__device__ inline float3 to_local(float3 d, float3 n) {
+  n = normalize(n);
+  float3 t, b;
+  frame(n, t, b);
+  return v3(dot(d, t), dot(d, b), dot(d, n));
+}
+__device__ inline float3 to_world(float3 d, float3 n) {
+  n = normalize(n);
+  float3 t, b;
+  frame(n, t, b);
+  if (!(n.z < -1.0f + 1e-6f)) {
+    t = normalize(t);
+    b = normalize(b);
+  }
+  return add(add(scale(t, d.x), scale(b, d.y)), scale(n, d.z));
+}
+
+// Trowbridge-Reitz (GTR2) distribution and its Smith masking term.
+__device__ inline float gtr2(float3 h, float ax, float ay) {
+  const float x = h.x / ax, y = h.y / ay, l2 = x * x + y * y + h.z * h.z;
+  return 1.0f / (3.14159265358979f * ax * ay * l2 * l2);
+}
+__device__ inline float smith_gtr2(float3 v, float ax, float ay) {
+  const float lam = (-1.0f + sqrtf(1.0f + (v.x * v.x * ax * ax + v.y * v.y * ay * ay) / (v.z * v.z))) * 0.5f;
+  return 1.0f / (1.0f + lam);
+}
+
+struct Surface {
+  float3 kd, ks;
+  float rough;
+  float3 ns, ng;  // unit shading and geometric normals
+};
+
+__device__ float3 ct_eval(const Surface &s, float3 l, float3 v) {
+  if (dot(s.ng, v) < 0.0f || dot(s.ng, l) < 0.0f) return v3(0.0f, 0.0f, 0.0f);
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  if (luminance(s.ks) + luminance(s.kd) <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+  float3 h = normalize(add(v, l));
+  const float n_in = dot(n, v), n_out = dot(n, l);
+  const float alpha = fmaxf(0.0025f, s.rough * s.rough);
+  if (dot(h, n) < 0.0f) h = scale(h, -1.0f);
+  const float h_out = dot(h, l);
+  const float sch = powf(fmaxf(1.0f - h_out, 0.0f), 5.0f);
+  const float d = gtr2(to_local(h, n), alpha, alpha);
+  const float g = smith_gtr2(to_local(v, n), alpha, alpha) * smith_gtr2(to_local(l, n), alpha, alpha);
+  const float k = d * g / (4.0f * n_in * n_out);
+  const float so = powf(1.0f - n_out, 5.0f), si = powf(1.0f - n_in, 5.0f);
+  const float base = 1.05f * (1.0f - so) * (1.0f - si) / 3.14159265358979f;
+  float r[3];
+  const float kd[3] = {s.kd.x, s.kd.y, s.kd.z}, ks[3] = {s.ks.x, s.ks.y, s.ks.z};
+  for (int c = 0; c < 3; ++c) {
+    const float f = ks[c] + (1.0f - ks[c]) * sch;
+    r[c] = ((1.0f - ks[c]) * base * kd[c] + f * k) * n_out;
+  }
+  return v3(r[0], r[1], r[2]);
+}
+
+__device__ float ct_pdf(const Surface &s, float3 l, float3 v) {
+  if (dot(s.ng, v) < 0.0f || dot(s.ng, l) < 0.0f) return 0.0f;
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  const float3 h = normalize(add(v, l));
+  const float n_h = dot(n, h), n_in = dot(n, v), n_out = dot(n, l);
+  if (n_out <= 0.0f || n_h <= 0.0f) return 0.0f;
+  const float ls = luminance(s.ks), lr = luminance(s.kd);
+  if (ls + lr <= 0.0f) return 0.0f;
+  const float r = fminf(fmaxf(s.rough, 0.05f), 1.0f), a = r * r;
+  float spec = ls / (ls + lr);
+  const float diff = 1.0f - spec;
+  spec *= smith_gtr2(to_local(v, n), a, a) * gtr2(to_local(h, n), a, a) / (4.0f * n_in);
+  return spec + diff * n_out / 3.14159265358979f;
+}
+
+// Lobe choice by luminance, then a cosine-weighted direction or Heitz's
+// (2018) sample of the visible GGX normals; returns x = -2 for a black BSDF.
+__device__ float3 ct_sample(const Surface &s, float3 v, float rw, float ru, float rv) {
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  const float ls = luminance(s.ks), lr = luminance(s.kd);
+  if (ls + lr <= 0.0f) return v3(-2.0f, -2.0f, -2.0f);
+  const float spec = ls / (ls + lr);
+  if (rw > spec) {
+    const float phi = 2.0f * 3.14159265358979f * ru;
+    const float tmp = sqrtf(fminf(fmaxf(1.0f - rv, 0.0f), 1.0f));
+    return to_world(v3(cosf(phi) * tmp, sinf(phi) * tmp, sqrtf(fminf(fmaxf(rv, 0.0f), 1.0f))), n);
+  }
+  float3 li = to_local(v, n);
+  const bool below = li.z < 0.0f;
+  if (below) li = scale(li, -1.0f);
+  const float r = fminf(fmaxf(s.rough, 0.05f), 1.0f), a = r * r;
+  const float3 hemi = normalize(v3(a * li.x, a * li.y, li.z));
+  const float rr = sqrtf(ru), phi = 2.0f * 3.14159265358979f * rv;
+  const float t1 = rr * cosf(phi);
+  float t2 = rr * sinf(phi);
+  const float sm = (1.0f + hemi.z) * 0.5f;
+  t2 = (1.0f - sm) * sqrtf(1.0f - t1 * t1) + sm * t2;
+  const float3 disk = v3(t1, t2, sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2)));
+  const float3 hn = to_world(disk, hemi);
+  const float3 micro = normalize(v3(a * hn.x, a * hn.y, fmaxf(0.0f, hn.z)));
+  const float3 hw = normalize(to_world(micro, n));
+  const float3 out = normalize(add(scale(v, -1.0f), scale(hw, 2.0f * dot(v, hw))));
+  return below ? scale(out, -1.0f) : out;
+}
+
+// Cube face and (u, v) of a direction (EnvDirecToUV, later tests win).
+__device__ inline int dir_to_face(float3 d, float &u, float &v) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  float m = ax, uc = d.z, vc = d.y;
+  int f = 0;
+  if (d.x > 0.0f && ax >= ay && ax >= az) { m = ax; uc = d.z; vc = d.y; f = 0; }
+  if (!(d.x > 0.0f) && ax >= ay && ax >= az) { m = ax; uc = -d.z; vc = d.y; f = 1; }
+  if (d.y > 0.0f && ay >= ax && ay >= az) { m = ay; uc = d.x; vc = d.z; f = 2; }
+  if (!(d.y > 0.0f) && ay >= ax && ay >= az) { m = ay; uc = d.x; vc = -d.z; f = 3; }
+  if (!(d.z > 0.0f) && az >= ax && az >= ay) { m = az; uc = d.x; vc = d.y; f = 4; }
+  if (d.z > 0.0f && az >= ax && az >= ay) { m = az; uc = -d.x; vc = d.y; f = 5; }
+  u = 0.5f * (uc / m + 1.0f);
+  v = 0.5f * (vc / m + 1.0f);
+  return f;
+}
+
+// EnvUVToDirec (cubeMap.cu:94-110).
+__device__ inline float3 face_to_dir(int f, float u, float v) {
+  const float uc = 2.0f * u - 1.0f, vc = 2.0f * v - 1.0f;
+  float3 d;
+  switch (f) {
+    case 0: d = v3(1.0f, vc, uc); break;
+    case 1: d = v3(-1.0f, vc, -uc); break;
+    case 2: d = v3(uc, 1.0f, vc); break;
+    case 3: d = v3(uc, -1.0f, -vc); break;
+    case 4: d = v3(uc, vc, -1.0f); break;
+    default: d = v3(-uc, vc, 1.0f); break;
+  }
+  return normalize(d);
+}
+
+// Table layout (env_table_doubles): pmf[6] cdf[7] face_lum[6] | per face.
+__host__ __device__ inline size_t env_face_doubles(int n) {
+  return (size_t)n * (n + 1) + (size_t)n * n + (n + 1) + n;
+}
+constexpr int kTabHead = 19;
+
+// First index i in [0, count) with cdf[i] > r (count when none), as the
+// linear searches of cubeMap.cu:115-164 find it.
+__device__ inline int first_above(const double *cdf, int count, float r) {
+  int lo = 0, hi = count;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] > (double)r) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// __direct_callable__sampleEnvironmapLight (cubeMap.cu:166-174).
+__device__ float3 env_sample(const double *tab, int n, float r0, float r1, float r2) {
+  const int f = min(max(first_above(tab + 6, 6, r0) - 1, 0), 5);
+  const double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
+  const double *cdf_rows = base, *cdf_m = base + (size_t)n * (n + 1) + (size_t)n * n;
+  const int y = min(max(first_above(cdf_m, n, r2) - 1, 0), n - 1);
+  double dy = (double)r2 - cdf_m[y];
+  if (cdf_m[y + 1] - cdf_m[y] > 0) dy /= (cdf_m[y + 1] - cdf_m[y]);
+  const double *cdf = cdf_rows + (size_t)y * (n + 1);
+  const int x = min(max(first_above(cdf, n, r1) - 1, 0), n - 1);
+  double dx = (double)r1 - cdf[x];
+  if (cdf[x + 1] - cdf[x] > 0) dx /= (cdf[x + 1] - cdf[x]);
+  return face_to_dir(f, (float)((x + dx) / n), (float)((y + dy) / n));
+}
+
+// __direct_callable__pdf (cubeMap.cu:178-186) with pdf_envmap
+// (cubeMap.cu:9-19), in the same FP32 operations: a density per unit face
+// (u, v) area times the face probability.
+__device__ float env_pdf(const double *tab, int n, float3 d) {
+  float u, v;
+  const int f = dir_to_face(d, u, v);
+  const double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
+  const double *pdf_rows = base + (size_t)n * (n + 1);
+  const double *pdf_m = pdf_rows + (size_t)n * n + (n + 1);
+  const int x = (int)fminf(fmaxf(u * (float)n, 0.0f), (float)(n - 1));
+  const int y = (int)fminf(fmaxf(v * (float)n, 0.0f), (float)(n - 1));
+  const float py = (float)pdf_m[y], px = (float)pdf_rows[(size_t)y * n + x];
+  const float p = __fmul_rn(__fmul_rn(__fmul_rn(py, px), (float)n), (float)n);
+  return __fmul_rn((float)tab[f], p);
+}
+
+__device__ inline bool finite3(float3 c) {
+  return !(isnan(c.x) || isnan(c.y) || isnan(c.z));
+}
+
+// Direct light at a first hit p seen along d (unit), averaged over the
+// pixel's samples.
+__device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, float3 d,
+                               uint32_t pixel, int *stack) {
+  const float3 v = scale(d, -1.0f);
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < a.direct_samples; ++k) {
+    const uint32_t c = (uint32_t)k * 8u;
+    // Environment-importance sample, power-heuristic weight.
+    {
+      const float3 l = env_sample(a.env_tab, a.light_n, rng_uniform(a.seed, pixel, c),
+                                  rng_uniform(a.seed, pixel, c + 1), rng_uniform(a.seed, pixel, c + 2));
+      const float pe = env_pdf(a.env_tab, a.light_n, l);
+      if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
+        const float pb = ct_pdf(s, l, v);
+        const float3 f = ct_eval(s, l, v);
+        const float3 le = env_radiance(a.light_faces, a.light_n, l);
+        const float w = pe / fmaxf(pe * pe + pb * pb, 1e-8f);
+        const float3 r = v3(f.x * le.x * w, f.y * le.y * w, f.z * le.z * w);
+        if (finite3(r)) {
+          acc[0] += r.x;
+          acc[1] += r.y;
+          acc[2] += r.z;
+        }
+      }
+    }
+    // BSDF sample, power-heuristic weight.
+    {
+      const float3 l = ct_sample(s, v, rng_uniform(a.seed, pixel, c + 3),
+                                 rng_uniform(a.seed, pixel, c + 4), rng_uniform(a.seed, pixel, c + 5));
+      if (l.x != -2.0f) {
+        const float pb = ct_pdf(s, l, v);
+        const float pe = env_pdf(a.env_tab, a.light_n, l);
+        if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
+          const float3 f = ct_eval(s, l, v);
+          const float3 le = env_radiance(a.light_faces, a.light_n, l);
+          const float w = pb / fmaxf(pe * pe + pb * pb, 1e-8f);
+          const float3 r = v3(f.x * le.x * w, f.y * le.y * w, f.z * le.z * w);
+          if (finite3(r)) {
+            acc[0] += r.x;
+            acc[1] += r.y;
+            acc[2] += r.z;
+          }
+        }
+      }
+    }
+  }
+  const float inv = 1.0f / (float)a.direct_samples;
+  return v3(acc[0] * inv, acc[1] * inv, acc[2] * inv);
+}
+
+// Sampling tables of one cube face per workgroup, one thread per row
+// (Environment_Map::init_sampling_dist, EnvironmentMap.cpp:153-178, and
+// make_table_dist_2d, table_dist.cpp:43-118).  Table row r, column c holds
+// texel (r, c) of the face, weighted by TexelCoordSolidAngle evaluated at
+// ((r + 0.5) / n, (c + 0.5) / n) as the reference calls it
+// (EnvironmentMap.cpp:128-147, 167-174).
+__global__ __launch_bounds__(128) void env_tables_kernel(const float *__restrict__ faces, int n,
+                                                        double *__restrict__ tab) {
+  const int f = blockIdx.x, r = threadIdx.x;
+  double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
+  double *cdf_rows = base, *pdf_rows = base + (size_t)n * (n + 1);
+  double *cdf_m = pdf_rows + (size_t)n * n, *pdf_m = cdf_m + (n + 1);
+  __shared__ double row_int[128], row_lum[128];
+  if (r < n) {
+    const float *px = faces + ((size_t)f * n + r) * n * 3;
+    double *cr = cdf_rows + (size_t)r * (n + 1);
+    double *pr = pdf_rows + (size_t)r * n;
+    const float kScale = __fdiv_rn(__fmul_rn(2.0f, 3.14159265358979323846f), 3.0f);
+    double lum_sum = 0.0;
+    cr[0] = 0.0;
+    for (int c = 0; c < n; ++c) {
+      const double U = (2.0 * ((double)((r + 0.5) / n) + 0.5) / (double)n) - 1.0;
+      const double V = (2.0 * ((double)((c + 0.5) / n) + 0.5) / (double)n) - 1.0;
+      const double inv = 1.0 / n;
+      auto area = [](double x, double y) { return atan2(x * y, sqrt(x * x + y * y + 1.0)); };
+      const double sa = area(U - inv, V - inv) - area(U - inv, V + inv) - area(U + inv, V - inv) +
+                        area(U + inv, V + inv);
+      const float saf = (float)sa;
+      const float3 t = v3(px[3 * c], px[3 * c + 1], px[3 * c + 2]);
+      const float3 q = v3(__fdiv_rn(__fmul_rn(fabsf(t.x), saf), kScale),
+                          __fdiv_rn(__fmul_rn(fabsf(t.y), saf), kScale),
+                          __fdiv_rn(__fmul_rn(fabsf(t.z), saf), kScale));
+      const double fv = (double)luminance(q);
+      pr[c] = fv;
+      cr[c + 1] = cr[c] + fv;
+      lum_sum += (double)luminance(t);
+    }
+    const double integral = cr[n];
+    for (int c = 0; c < n; ++c) {
+      if (integral > 0) {
+        cr[c] /= integral;
+        pr[c] /= integral;
+      } else {
+        cr[c] = 0.0;
+        pr[c] = 0.0;
+      }
+    }
+    if (!(integral > 0)) cr[n] = 0.0;
+    row_int[r] = cr[n];
+    row_lum[r] = lum_sum;
+  }
+  __syncthreads();
+  if (r == 0) {
+    cdf_m[0] = 0.0;
+    double lum = 0.0;
+    for (int y = 0; y < n; ++y) {
+      cdf_m[y + 1] = cdf_m[y] + row_int[y];
+      lum += row_lum[y];
+    }
+    const double total = cdf_m[n];
+    for (int y = 0; y < n; ++y) {
+      if (total > 0) {
+        cdf_m[y] /= total;
+        pdf_m[y] = row_int[y] / total;
+      } else {
+        pdf_m[y] = 1.0 / n;
+        cdf_m[y] = (double)y / n;
+      }
+    }
+    cdf_m[n] = 1.0;
+    tab[13 + f] = lum;
+  }
+  __syncthreads();
+  if (r < n) {
+    double *cr = cdf_rows + (size_t)r * (n + 1);
+    cr[n] = cr[n - 1] > 0 ? 1.0 : 0.0;
+  }
+}
+
+// Face distribution (EnvironmentMap.cpp:180-189, make_table_dist_1d,
+// table_dist.cpp:4-27); an all-black map gets the uniform distribution.
+__global__ void env_face_table_kernel(double *__restrict__ tab) {
+  if (threadIdx.x != 0) return;
+  double total = 0.0, f[6];
+  for (int i = 0; i < 6; ++i) total += tab[13 + i];
+  for (int i = 0; i < 6; ++i) f[i] = total > 0 ? tab[13 + i] / total : 0.0;
+  double cdf[7];
+  cdf[0] = 0.0;
+  for (int i = 0; i < 6; ++i) cdf[i + 1] = cdf[i] + f[i];
+  const double t = cdf[6];
+  for (int i = 0; i < 6; ++i) {
+    tab[i] = t > 0 ? f[i] / t : 1.0 / 6.0;
+    tab[6 + i] = t > 0 ? cdf[i] / t : i / 6.0;
+  }
+  tab[12] = t > 0 ? t : 1.0;
+}
+
 // One 8 x 8 pixel tile per 64-lane workgroup (coherent primary rays).
+// DIRECT: the direct-light estimate is computed and stored (a separate
+// instance, so the plain cast keeps its lower register count).
+template <bool DIRECT>
 __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
   __shared__ int stack_lds[kBvhMaxDepth * 64];
   const int lane = threadIdx.x;
@@ -185,9 +593,10 @@ __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
   // What the pixel stores; defaults are the cleared state of a path that
   // ended on a mirror without a first hit.
   float3 pos = v3(0.0f, 0.0f, 0.0f), nrm = v3(0.0f, 0.0f, 0.0f), refl = nrm;
-  float3 kd = nrm, ks = nrm;
+  float3 kd = nrm, ks = nrm, direct = nrm;
   float rough = 0.0f;
   unsigned char flag = 0;
+  constexpr bool want_direct = DIRECT;
   for (int depth = 1;; ++depth) {
     const Hit h = trace(a.nodes, a.tris, o, d, stack);
     if (h.prim == INT_MAX) {
@@ -195,6 +604,8 @@ __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
       refl = nrm;
       pos = v3(-100000.0f, -100000.0f, -100000.0f);
       if (depth == 1 && a.env_faces) kd = env_radiance(a.env_faces, a.env_n, d);
+      // The first ray's miss colour is also its direct term (camera.cu:145).
+      direct = kd;
       ks = v3(0.0f, 0.0f, 0.0f);
       rough = 1.0f;
       break;
@@ -225,6 +636,10 @@ __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
     ks = v3(mat.ks);
     rough = mat.roughness;
     flag = 1;
+    if constexpr (want_direct) {
+      const Surface sf{kd, ks, rough, ns, ng};
+      direct = direct_light(a, sf, p, d, (uint32_t)(iy * a.width + ix), stack);
+    }
     break;
   }
 
@@ -240,9 +655,22 @@ __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
   if (g.ks) reinterpret_cast<float4 *>(g.ks)[px] = make_float4(ks.x, ks.y, ks.z, 1.0f);
   if (g.roughness) g.roughness[px] = rough;
   if (g.env_flag) g.env_flag[px] = flag;
+  if constexpr (want_direct)
+    reinterpret_cast<float4 *>(g.direct)[px] = make_float4(direct.x, direct.y, direct.z, 1.0f);
 }
 
 }  // namespace
+
+size_t env_table_doubles(int n) { return kTabHead + 6 * env_face_doubles(n); }
+
+void launch_env_tables(const float *faces, int n, double *tab, hipStream_t s) {
+  require(faces && tab && n >= 1 && n <= 128, NWPRT_ERR_INVALID_ARGUMENT,
+          "env tables: face size must be in [1, 128]");
+  hipLaunchKernelGGL(env_tables_kernel, dim3(6), dim3(128), 0, s, faces, n, tab);
+  hip_check(hipGetLastError(), "env_tables_kernel launch");
+  hipLaunchKernelGGL(env_face_table_kernel, dim3(1), dim3(64), 0, s, tab);
+  hip_check(hipGetLastError(), "env_face_table_kernel launch");
+}
 
 void launch_gbuffer_cast(const CastArgs &a, hipStream_t s) {
   require(a.nodes && a.tris && a.indices && a.materials, NWPRT_ERR_INVALID_ARGUMENT,
@@ -251,8 +679,14 @@ void launch_gbuffer_cast(const CastArgs &a, hipStream_t s) {
           "gbuffer_cast: bad size or bounce count");
   require(!a.env_faces || a.env_n > 0, NWPRT_ERR_INVALID_ARGUMENT,
           "gbuffer_cast: environment face size must be positive");
+  require(!(a.gb.direct && a.light_faces) ||
+              (a.env_tab && a.light_n >= 1 && a.light_n <= 128 && a.direct_samples >= 1),
+          NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: bad direct-light arguments");
   const long long tiles = (long long)((a.width + 7) / 8) * ((a.height + 7) / 8);
-  hipLaunchKernelGGL(gbuffer_cast_kernel, dim3((unsigned)tiles), dim3(64), 0, s, a);
+  if (a.gb.direct && a.light_faces)
+    hipLaunchKernelGGL(gbuffer_cast_kernel<true>, dim3((unsigned)tiles), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(gbuffer_cast_kernel<false>, dim3((unsigned)tiles), dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "gbuffer_cast_kernel launch");
 }
 

@@ -193,7 +193,19 @@ struct CastArgs {
   const float *env_faces;  // [6][env_n][env_n][3] or null
   int env_n;
   nwprt_gbuffer gb;  // null fields are skipped
+  // Direct light (gb.direct and light_faces non-null): light_faces
+  // [6][light_n][light_n][3] with its sampling tables env_tab (layout of
+  // env_table_doubles), direct_samples estimates per pixel, RNG seed.
+  const float *light_faces;
+  const double *env_tab;
+  int light_n, direct_samples;
+  uint32_t seed;
 };
+// Doubles of the environment sampling tables for face size n: face pmf[6],
+// face cdf[7], then per face cdf_rows[n][n+1], pdf_rows[n][n],
+// cdf_marginals[n+1], pdf_marginals[n].
+size_t env_table_doubles(int n);
+void launch_env_tables(const float *faces, int n, double *tab, hipStream_t s);
 void launch_gbuffer_cast(const CastArgs &a, hipStream_t s);
 
 // Persistent batch form: work item = the 64 pixels [base, base + 64) of job

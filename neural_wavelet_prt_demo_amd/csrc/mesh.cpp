@@ -308,6 +308,8 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     mesh->depth = bld.max_depth;
     mesh->has_normals = normals != nullptr;
     mesh->has_tri_material = material_ids != nullptr;
+    for (const MaterialRec &mr : mats)
+      if (mr.type != 2 && mr.type != NWPRT_MATERIAL_MIRROR) mesh->direct_ok = false;
     const int id = ctx->next_id++;
     ctx->meshes[id] = std::move(mesh);
     *mesh_id = id;
@@ -412,7 +414,26 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
     a.env_faces = params->view_env_faces;
     a.env_n = params->view_env_n;
     a.gb = *gbuffer;
-    a.gb.direct = nullptr;
+    if (params->light_env_faces && gbuffer->direct) {
+      require(params->light_env_n >= 1 && params->light_env_n <= NWPRT_MAX_FACE_SIZE,
+              NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: light face size must be in [1, 128]");
+      require(params->direct_samples >= 1 && params->direct_samples <= (1 << 20),
+              NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: direct_samples must be in [1, 2^20]");
+      require(m.direct_ok, NWPRT_ERR_UNSUPPORTED,
+              "gbuffer_cast: the direct term supports cookTorrance (type 2) and mirror "
+              "materials only");
+      check_aligned(gbuffer->direct, 16, "G-buffer direct");
+      ctx->env_tables.ensure(env_table_doubles(params->light_env_n) * sizeof(double));
+      launch_env_tables(params->light_env_faces, params->light_env_n,
+                        ctx->env_tables.as<double>(), pick(ctx, stream));
+      a.light_faces = params->light_env_faces;
+      a.env_tab = ctx->env_tables.as<double>();
+      a.light_n = params->light_env_n;
+      a.direct_samples = params->direct_samples;
+      a.seed = params->seed;
+    } else {
+      a.gb.direct = nullptr;
+    }
     for (const void *p : {(const void *)gbuffer->position, (const void *)gbuffer->normal,
                           (const void *)gbuffer->reflect, (const void *)gbuffer->kd,
                           (const void *)gbuffer->ks})

@@ -76,6 +76,22 @@ def lib():
         l.orc_gbuffer_cast.argtypes = [POINTER(OrcMesh), c_void_p, c_int, c_int, c_int, c_void_p,
                                        c_void_p, c_int] + [c_void_p] * 9
         l.orc_gbuffer_cast.restype = None
+        l.orc_gbuffer_cast_direct.argtypes = [POINTER(OrcMesh), c_void_p, c_int, c_int, c_int,
+                                              c_void_p, c_void_p, c_int, c_void_p, c_int, c_int,
+                                              c_uint32] + [c_void_p] * 10
+        l.orc_gbuffer_cast_direct.restype = None
+        l.orc_env_table_doubles.argtypes = [c_int]
+        l.orc_env_table_doubles.restype = ctypes.c_size_t
+        l.orc_env_tables.argtypes = [c_void_p, c_int, c_void_p]
+        l.orc_env_tables.restype = None
+        l.orc_ct_eval.argtypes = [c_void_p] * 6
+        l.orc_ct_eval.restype = None
+        l.orc_ct_pdf.argtypes = [c_void_p] * 5
+        l.orc_ct_pdf.restype = c_double
+        l.orc_env_pdf.argtypes = [c_void_p, c_int, c_void_p]
+        l.orc_env_pdf.restype = c_double
+        l.orc_rng_uniform.argtypes = [c_uint32, c_uint32, c_uint32]
+        l.orc_rng_uniform.restype = c_float
         l.orc_tonemap.restype = c_uint32
         l.orc_aux_inputs.argtypes = [POINTER(OrcGbuffer), c_int, c_float, c_void_p]
         l.orc_cpu_relight_fp32.argtypes = [POINTER(OrcScene), c_void_p, c_void_p, c_int,
@@ -232,7 +248,9 @@ def denoise(color: np.ndarray, normal: np.ndarray, albedo: np.ndarray, iteration
 def gbuffer_cast(vertices: np.ndarray, indices: np.ndarray, materials, camera, width: int,
                  height: int, *, normals: Optional[np.ndarray] = None,
                  material_ids: Optional[np.ndarray] = None, max_bounce: int = 1,
-                 bounds=None, env_faces: Optional[np.ndarray] = None) -> Dict[str, np.ndarray]:
+                 bounds=None, env_faces: Optional[np.ndarray] = None,
+                 light_faces: Optional[np.ndarray] = None, samples: int = 1,
+                 seed: int = 0) -> Dict[str, np.ndarray]:
     """First-hit G-buffer by FP64 brute force.  materials: rows (kd[3], ks[3],
     roughness, type); camera: (from, at, up); bounds: (lower, upper) or the
     vertices' own.  Returns float64 [P][4] position / normal / reflect / kd /
@@ -257,8 +275,59 @@ def gbuffer_cast(vertices: np.ndarray, indices: np.ndarray, materials, camera, w
     out["env_flag"] = np.empty(P, np.uint8)
     out["prim"] = np.empty(P, np.int32)
     out["margin"] = np.empty(P, np.float64)
-    lib().orc_gbuffer_cast(ctypes.byref(mesh), _p(cam), width, height, max_bounce, _p(b),
-                           _p(env_faces), env_n,
-                           *[_p(out[k]) for k in ("position", "normal", "reflect", "kd", "ks",
-                                                  "roughness", "env_flag", "prim", "margin")])
+    keys = ("position", "normal", "reflect", "kd", "ks", "roughness", "env_flag", "prim",
+            "margin")
+    if light_faces is None:
+        lib().orc_gbuffer_cast(ctypes.byref(mesh), _p(cam), width, height, max_bounce, _p(b),
+                               _p(env_faces), env_n, *[_p(out[k]) for k in keys])
+        return out
+    light = np.ascontiguousarray(light_faces, np.float32)
+    out["direct"] = np.empty((P, 4), np.float64)
+    lib().orc_gbuffer_cast_direct(ctypes.byref(mesh), _p(cam), width, height, max_bounce, _p(b),
+                                  _p(env_faces), env_n, _p(light), light.shape[1], samples,
+                                  seed & 0xFFFFFFFF, *[_p(out[k]) for k in keys + ("direct",)])
     return out
+
+
+def env_tables(faces: np.ndarray) -> Dict[str, np.ndarray]:
+    """Environment sampling tables of [6][n][n][3] faces, split by field."""
+    f = np.ascontiguousarray(faces, np.float32)
+    n = f.shape[1]
+    t = np.empty(lib().orc_env_table_doubles(n), np.float64)
+    lib().orc_env_tables(_p(f), n, _p(t))
+    per = n * (n + 1) + n * n + (n + 1) + n
+    out = {"face_pmf": t[0:6], "face_cdf": t[6:13], "face_lum": t[13:19], "raw": t}
+    faces_ = t[19:].reshape(6, per)
+    out["cdf_rows"] = faces_[:, :n * (n + 1)].reshape(6, n, n + 1)
+    out["pdf_rows"] = faces_[:, n * (n + 1):n * (n + 1) + n * n].reshape(6, n, n)
+    out["cdf_marginals"] = faces_[:, n * (2 * n + 1):n * (2 * n + 1) + n + 1]
+    out["pdf_marginals"] = faces_[:, n * (2 * n + 1) + n + 1:]
+    return out
+
+
+def rng_uniform(seed: int, pixel: int, counter: int) -> float:
+    return lib().orc_rng_uniform(seed & 0xFFFFFFFF, pixel, counter)
+
+
+def _d3(x):
+    return np.ascontiguousarray(x, np.float64).reshape(3)
+
+
+def ct_eval(mat, ns, ng, l, v) -> np.ndarray:
+    """cookTorrance BSDF value times the cosine, for mat = (kd, ks, roughness, type)."""
+    m = np.ascontiguousarray(mat, np.float32).reshape(8)
+    a = [_d3(x) for x in (ns, ng, l, v)]
+    out = np.empty(3, np.float64)
+    lib().orc_ct_eval(_p(m), *[_p(x) for x in a], _p(out))
+    return out
+
+
+def ct_pdf(mat, ns, ng, l, v) -> float:
+    m = np.ascontiguousarray(mat, np.float32).reshape(8)
+    a = [_d3(x) for x in (ns, ng, l, v)]
+    return lib().orc_ct_pdf(_p(m), *[_p(x) for x in a])
+
+
+def env_pdf(tables: Dict[str, np.ndarray], n: int, d) -> float:
+    dd = _d3(d)
+    return lib().orc_env_pdf(_p(tables["raw"]), n, _p(dd))

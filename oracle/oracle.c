@@ -13,6 +13,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
 /* ------------------------------------------------------------ hash grid -- */
 
 /* Level table, in FP32 exactly as the product's host code documents it:
@@ -581,10 +585,15 @@ static int g_closest(const orc_mesh *m, const double *o, const double *d, double
   return hit;
 }
 
-void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
-                      const float *bounds, const float *env_faces, int env_n, double *position,
-                      double *normal, double *reflect, double *kd, double *ks, double *roughness,
-                      uint8_t *env_flag, int32_t *prim, double *margin) {
+static void g_direct(const orc_mesh *m, const double *tab, int n, const float *light,
+                     const double *p, const double *d, const double *ns, const double *ng,
+                     const float *mat, int samples, uint32_t seed, uint32_t pixel, double *out);
+
+static void g_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
+                   const float *bounds, const float *env_faces, int env_n, double *position,
+                   double *normal, double *reflect, double *kd, double *ks, double *roughness,
+                   uint8_t *env_flag, int32_t *prim, double *margin, const float *light,
+                   const double *tab, int light_n, int samples, uint32_t seed, double *direct) {
   double dir[3], up[3], hz[3], vt[3];
   for (int a = 0; a < 3; ++a) {
     dir[a] = (double)camera[3 + a] - camera[a];
@@ -610,6 +619,7 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
     g_norm(d);
     double pos[3] = {0, 0, 0}, nrm[3] = {0, 0, 0}, rfl[3] = {0, 0, 0};
     double kdv[3] = {0, 0, 0}, ksv[3] = {0, 0, 0}, rough = 0.0, mg = 1.0;
+    double dir_l[3] = {0, 0, 0};
     int flag = 0, first = -1;
     for (int depth = 1;; ++depth) {
       double t, u, v;
@@ -621,13 +631,26 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
           pos[a] = -100000.0;
         }
         if (depth == 1 && env_faces) g_env(env_faces, env_n, d, kdv);
+        for (int a = 0; a < 3; ++a) dir_l[a] = kdv[a];
         rough = 1.0;
         break;
       }
       const int mat = m->material_ids ? m->material_ids[hit] : 0;
       const float *mr = m->materials + 8 * (size_t)mat;
       const int *tri = m->indices + 3 * (size_t)hit;
-      double ns[3];
+      double ns[3], ng[3];
+      {
+        const float *A = m->vertices + 3 * (size_t)tri[0];
+        const float *B = m->vertices + 3 * (size_t)tri[1];
+        const float *C = m->vertices + 3 * (size_t)tri[2];
+        double e1[3], e2[3];
+        for (int a = 0; a < 3; ++a) {
+          e1[a] = (double)B[a] - A[a];
+          e2[a] = (double)C[a] - A[a];
+        }
+        g_cross(e1, e2, ng);
+        g_norm(ng);
+      }
       if (m->normals) {
         const double ww = 1.0 - u - v;
         for (int a = 0; a < 3; ++a)
@@ -668,6 +691,8 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
       rough = mr[6];
       flag = 1;
       first = hit;
+      if (direct && light)
+        g_direct(m, tab, light_n, light, p, d, ns, ng, mr, samples, seed, (uint32_t)px, dir_l);
       break;
     }
     for (int a = 0; a < 3; ++a) {
@@ -683,5 +708,431 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
     env_flag[px] = (uint8_t)flag;
     prim[px] = first;
     margin[px] = mg;
+    if (direct)
+      for (int a = 0; a < 4; ++a) direct[4 * px + a] = a < 3 ? dir_l[a] : 1.0;
   }
 }
+
+void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
+                      const float *bounds, const float *env_faces, int env_n, double *position,
+                      double *normal, double *reflect, double *kd, double *ks, double *roughness,
+                      uint8_t *env_flag, int32_t *prim, double *margin) {
+  g_cast(m, camera, w, h, max_bounce, bounds, env_faces, env_n, position, normal, reflect, kd, ks,
+         roughness, env_flag, prim, margin, NULL, NULL, 0, 0, 0, NULL);
+}
+
+/* ---- direct light --------------------------------------------------------
+ * The closest-hit program's direct term in wavelet mode (maximumBounce 1,
+ * Renderer/device_functions/devicePrograms.cu:165-280) for the cookTorrance
+ * BSDF (Renderer/device_functions/cookTorrance.cu:26-195), replaying the
+ * product's counter-based random stream: per sample, an environment sample
+ * (Renderer/device_functions/cubeMap.cu:115-186 over the tables of
+ * Renderer/EnvironmentMap.cpp:128-190 and Renderer/table_dist.cpp:4-130)
+ * and a BSDF sample, each shadow-tested by brute force, power-heuristic
+ * weights.  Directions, BSDF and weights in FP64; the tables' texel weights
+ * in the reference's FP32 operations; random numbers are the same FP32
+ * values the product draws. */
+
+float orc_rng_uniform(uint32_t seed, uint32_t pixel, uint32_t counter) {
+  uint64_t z = (((uint64_t)seed << 32) | pixel) + 0x9E3779B97F4A7C15ull * ((uint64_t)counter + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (float)(uint32_t)(z >> 40) * 5.9604644775390625e-8f;
+}
+
+static float g_lumf(float x, float y, float z) {
+  const float a = x * 0.212671f, b = y * 0.715160f, c = z * 0.072169f;
+  return (a + b) + c;
+}
+
+static size_t g_face_doubles(int n) {
+  return (size_t)n * (n + 1) + (size_t)n * n + (size_t)(n + 1) + (size_t)n;
+}
+
+size_t orc_env_table_doubles(int n) { return 19 + 6 * g_face_doubles(n); }
+
+static double g_area(double x, double y) { return atan2(x * y, sqrt(x * x + y * y + 1.0)); }
+
+void orc_env_tables(const float *faces, int n, double *tab) {
+  const float k = (2.0f * 3.14159265358979323846f) / 3.0f;
+  for (int f = 0; f < 6; ++f) {
+    double *base = tab + 19 + (size_t)f * g_face_doubles(n);
+    double *cdf_rows = base, *pdf_rows = base + (size_t)n * (n + 1);
+    double *cdf_m = pdf_rows + (size_t)n * n, *pdf_m = cdf_m + (n + 1);
+    double *row_int = (double *)malloc(sizeof(double) * n), lum = 0.0;
+    for (int r = 0; r < n; ++r) {
+      double *cr = cdf_rows + (size_t)r * (n + 1), *pr = pdf_rows + (size_t)r * n, ls = 0.0;
+      cr[0] = 0.0;
+      for (int c = 0; c < n; ++c) {
+        /* TexelCoordSolidAngle((r + 0.5) / n, (c + 0.5) / n, n) */
+        const double U = 2.0 * ((r + 0.5) / n + 0.5) / n - 1.0;
+        const double V = 2.0 * ((c + 0.5) / n + 0.5) / n - 1.0;
+        const double iv = 1.0 / n;
+        const double sa = g_area(U - iv, V - iv) - g_area(U - iv, V + iv) -
+                          g_area(U + iv, V - iv) + g_area(U + iv, V + iv);
+        const float saf = (float)sa;
+        const float *t = faces + (((size_t)f * n + r) * n + c) * 3;
+        const float qx = (fabsf(t[0]) * saf) / k, qy = (fabsf(t[1]) * saf) / k,
+                    qz = (fabsf(t[2]) * saf) / k;
+        const double fv = (double)g_lumf(qx, qy, qz);
+        pr[c] = fv;
+        cr[c + 1] = cr[c] + fv;
+        ls += (double)g_lumf(t[0], t[1], t[2]);
+      }
+      const double integral = cr[n];
+      for (int c = 0; c < n; ++c) {
+        if (integral > 0) {
+          cr[c] /= integral;
+          pr[c] /= integral;
+        } else {
+          cr[c] = pr[c] = 0.0;
+        }
+      }
+      if (!(integral > 0)) cr[n] = 0.0;
+      row_int[r] = cr[n];
+      lum += ls;
+    }
+    cdf_m[0] = 0.0;
+    for (int y = 0; y < n; ++y) cdf_m[y + 1] = cdf_m[y] + row_int[y];
+    const double total = cdf_m[n];
+    for (int y = 0; y < n; ++y) {
+      if (total > 0) {
+        cdf_m[y] /= total;
+        pdf_m[y] = row_int[y] / total;
+      } else {
+        pdf_m[y] = 1.0 / n;
+        cdf_m[y] = (double)y / n;
+      }
+    }
+    cdf_m[n] = 1.0;
+    for (int r = 0; r < n; ++r) {
+      double *cr = cdf_rows + (size_t)r * (n + 1);
+      cr[n] = cr[n - 1] > 0 ? 1.0 : 0.0;
+    }
+    tab[13 + f] = lum;
+    free(row_int);
+  }
+  double total = 0.0, fr[6], cdf[7];
+  for (int i = 0; i < 6; ++i) total += tab[13 + i];
+  for (int i = 0; i < 6; ++i) fr[i] = total > 0 ? tab[13 + i] / total : 0.0;
+  cdf[0] = 0.0;
+  for (int i = 0; i < 6; ++i) cdf[i + 1] = cdf[i] + fr[i];
+  const double t = cdf[6];
+  for (int i = 0; i < 6; ++i) {
+    tab[i] = t > 0 ? fr[i] / t : 1.0 / 6.0;
+    tab[6 + i] = t > 0 ? cdf[i] / t : i / 6.0;
+  }
+  tab[12] = t > 0 ? t : 1.0;
+}
+
+static int g_first_above(const double *cdf, int count, float r) {
+  for (int i = 0; i < count; ++i)
+    if (cdf[i] > (double)r) return i;
+  return count;
+}
+
+static int g_clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static void g_face_to_dir(int f, double u, double v, double *d) {
+  const double uc = 2.0 * u - 1.0, vc = 2.0 * v - 1.0;
+  switch (f) {
+    case 0: d[0] = 1; d[1] = vc; d[2] = uc; break;
+    case 1: d[0] = -1; d[1] = vc; d[2] = -uc; break;
+    case 2: d[0] = uc; d[1] = 1; d[2] = vc; break;
+    case 3: d[0] = uc; d[1] = -1; d[2] = -vc; break;
+    case 4: d[0] = uc; d[1] = vc; d[2] = -1; break;
+    default: d[0] = -uc; d[1] = vc; d[2] = 1; break;
+  }
+  g_norm(d);
+}
+
+static void g_env_sample(const double *tab, int n, float r0, float r1, float r2, double *d) {
+  const int f = g_clampi(g_first_above(tab + 6, 6, r0) - 1, 0, 5);
+  const double *base = tab + 19 + (size_t)f * g_face_doubles(n);
+  const double *cdf_m = base + (size_t)n * (n + 1) + (size_t)n * n;
+  const int y = g_clampi(g_first_above(cdf_m, n, r2) - 1, 0, n - 1);
+  double dy = (double)r2 - cdf_m[y];
+  if (cdf_m[y + 1] - cdf_m[y] > 0) dy /= (cdf_m[y + 1] - cdf_m[y]);
+  const double *cdf = base + (size_t)y * (n + 1);
+  const int x = g_clampi(g_first_above(cdf, n, r1) - 1, 0, n - 1);
+  double dx = (double)r1 - cdf[x];
+  if (cdf[x + 1] - cdf[x] > 0) dx /= (cdf[x + 1] - cdf[x]);
+  /* the sampled (u, v) is returned as FP32 (cubeMap.cu:164) */
+  g_face_to_dir(f, (double)(float)((x + dx) / n), (double)(float)((y + dy) / n), d);
+}
+
+static int g_dir_to_face(const double *d, double *u, double *v) {
+  const double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
+  double m = ax, uc = d[2], vc = d[1];
+  int f = 0;
+  if (d[0] > 0 && ax >= ay && ax >= az) { m = ax; uc = d[2]; vc = d[1]; f = 0; }
+  if (!(d[0] > 0) && ax >= ay && ax >= az) { m = ax; uc = -d[2]; vc = d[1]; f = 1; }
+  if (d[1] > 0 && ay >= ax && ay >= az) { m = ay; uc = d[0]; vc = d[2]; f = 2; }
+  if (!(d[1] > 0) && ay >= ax && ay >= az) { m = ay; uc = d[0]; vc = -d[2]; f = 3; }
+  if (!(d[2] > 0) && az >= ax && az >= ay) { m = az; uc = d[0]; vc = d[1]; f = 4; }
+  if (d[2] > 0 && az >= ax && az >= ay) { m = az; uc = -d[0]; vc = d[1]; f = 5; }
+  *u = 0.5 * (uc / m + 1.0);
+  *v = 0.5 * (vc / m + 1.0);
+  return f;
+}
+
+static double g_env_pdf(const double *tab, int n, const double *d) {
+  double u, v;
+  const int f = g_dir_to_face(d, &u, &v);
+  const double *base = tab + 19 + (size_t)f * g_face_doubles(n);
+  const double *pdf_rows = base + (size_t)n * (n + 1);
+  const double *pdf_m = pdf_rows + (size_t)n * n + (n + 1);
+  const int x = (int)fmin(fmax(u * n, 0.0), (double)(n - 1));
+  const int y = (int)fmin(fmax(v * n, 0.0), (double)(n - 1));
+  return tab[f] * pdf_m[y] * pdf_rows[(size_t)y * n + x] * n * n;
+}
+
+static double g_lum(const double *c) {
+  return c[0] * 0.212671 + c[1] * 0.715160 + c[2] * 0.072169;
+}
+
+/* Orthonormal frame of Frisvad (2012) around unit n. */
+static void g_frame(const double *n, double *t, double *b) {
+  if (n[2] < -1.0 + 1e-6) {
+    t[0] = 0; t[1] = -1; t[2] = 0;
+    b[0] = -1; b[1] = 0; b[2] = 0;
+  } else {
+    const double a = 1.0 / (1.0 + n[2]), c = -n[0] * n[1] * a;
+    t[0] = 1.0 - n[0] * n[0] * a; t[1] = c; t[2] = -n[0];
+    b[0] = c; b[1] = 1.0 - n[1] * n[1] * a; b[2] = -n[1];
+  }
+}
+static void g_to_local(const double *d, const double *n0, double *out) {
+  double n[3] = {n0[0], n0[1], n0[2]}, t[3], b[3];
+  g_norm(n);
+  g_frame(n, t, b);
+  out[0] = g_dot(d, t);
+  out[1] = g_dot(d, b);
+  out[2] = g_dot(d, n);
+}
+static void g_to_world(const double *d, const double *n0, double *out) {
+  double n[3] = {n0[0], n0[1], n0[2]}, t[3], b[3];
+  g_norm(n);
+  g_frame(n, t, b);
+  if (!(n[2] < -1.0 + 1e-6)) {
+    g_norm(t);
+    g_norm(b);
+  }
+  for (int a = 0; a < 3; ++a) out[a] = t[a] * d[0] + b[a] * d[1] + n[a] * d[2];
+}
+static double g_gtr2(const double *h, double ax, double ay) {
+  const double x = h[0] / ax, y = h[1] / ay, l2 = x * x + y * y + h[2] * h[2];
+  return 1.0 / (M_PI * ax * ay * l2 * l2);
+}
+static double g_smith(const double *v, double ax, double ay) {
+  const double lam =
+      (-1.0 + sqrt(1.0 + (v[0] * v[0] * ax * ax + v[1] * v[1] * ay * ay) / (v[2] * v[2]))) / 2.0;
+  return 1.0 / (1.0 + lam);
+}
+
+typedef struct {
+  double kd[3], ks[3], rough, ns[3], ng[3];
+} g_surface;
+
+static void g_shading_normal(const g_surface *s, const double *v, double *n) {
+  const double sg = g_dot(s->ns, v) < 0 ? -1.0 : 1.0;
+  for (int a = 0; a < 3; ++a) n[a] = sg * s->ns[a];
+}
+
+static void g_ct_eval(const g_surface *s, const double *l, const double *v, double *out) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (g_dot(s->ng, v) < 0 || g_dot(s->ng, l) < 0) return;
+  double n[3], h[3], hl[3], vl[3], ll[3];
+  g_shading_normal(s, v, n);
+  if (g_lum(s->ks) + g_lum(s->kd) <= 0) return;
+  for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+  g_norm(h);
+  const double n_in = g_dot(n, v), n_out = g_dot(n, l);
+  const double alpha = fmax(0.0025, s->rough * s->rough);
+  if (g_dot(h, n) < 0)
+    for (int a = 0; a < 3; ++a) h[a] = -h[a];
+  const double h_out = g_dot(h, l);
+  const double sch = pow(fmax(1.0 - h_out, 0.0), 5.0);
+  g_to_local(h, n, hl);
+  g_to_local(v, n, vl);
+  g_to_local(l, n, ll);
+  const double k = g_gtr2(hl, alpha, alpha) * g_smith(vl, alpha, alpha) *
+                   g_smith(ll, alpha, alpha) / (4.0 * n_in * n_out);
+  const double base = 1.05 * (1.0 - pow(1.0 - n_out, 5.0)) * (1.0 - pow(1.0 - n_in, 5.0)) / M_PI;
+  for (int c = 0; c < 3; ++c) {
+    const double f = s->ks[c] + (1.0 - s->ks[c]) * sch;
+    out[c] = ((1.0 - s->ks[c]) * base * s->kd[c] + f * k) * n_out;
+  }
+}
+
+static double g_ct_pdf(const g_surface *s, const double *l, const double *v) {
+  if (g_dot(s->ng, v) < 0 || g_dot(s->ng, l) < 0) return 0.0;
+  double n[3], h[3], hl[3], vl[3];
+  g_shading_normal(s, v, n);
+  for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+  g_norm(h);
+  const double n_h = g_dot(n, h), n_in = g_dot(n, v), n_out = g_dot(n, l);
+  if (n_out <= 0 || n_h <= 0) return 0.0;
+  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
+  if (ls + lr <= 0) return 0.0;
+  const double r = fmin(fmax(s->rough, 0.05), 1.0), a = r * r;
+  double spec = ls / (ls + lr);
+  const double diff = 1.0 - spec;
+  g_to_local(v, n, vl);
+  g_to_local(h, n, hl);
+  spec *= g_smith(vl, a, a) * g_gtr2(hl, a, a) / (4.0 * n_in);
+  return spec + diff * n_out / M_PI;
+}
+
+/* Returns 0 for a black BSDF (the reference's (-2, -2, -2)). */
+static int g_ct_sample(const g_surface *s, const double *v, float rw, float ru, float rv,
+                       double *out) {
+  double n[3];
+  g_shading_normal(s, v, n);
+  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
+  if (ls + lr <= 0) return 0;
+  const double spec = ls / (ls + lr);
+  if ((double)rw > spec) {
+    const double phi = 2.0 * M_PI * ru, tmp = sqrt(fmin(fmax(1.0 - rv, 0.0), 1.0));
+    const double loc[3] = {cos(phi) * tmp, sin(phi) * tmp, sqrt(fmin(fmax((double)rv, 0.0), 1.0))};
+    g_to_world(loc, n, out);
+    return 1;
+  }
+  double li[3];
+  g_to_local(v, n, li);
+  const int below = li[2] < 0;
+  if (below)
+    for (int a = 0; a < 3; ++a) li[a] = -li[a];
+  const double r = fmin(fmax(s->rough, 0.05), 1.0), a = r * r;
+  double hemi[3] = {a * li[0], a * li[1], li[2]};
+  g_norm(hemi);
+  const double rr = sqrt((double)ru), phi = 2.0 * M_PI * rv;
+  const double t1 = rr * cos(phi);
+  double t2 = rr * sin(phi);
+  const double sm = (1.0 + hemi[2]) / 2.0;
+  t2 = (1.0 - sm) * sqrt(1.0 - t1 * t1) + sm * t2;
+  const double disk[3] = {t1, t2, sqrt(fmax(0.0, 1.0 - t1 * t1 - t2 * t2))};
+  double hn[3], micro[3], hw[3];
+  g_to_world(disk, hemi, hn);
+  micro[0] = a * hn[0];
+  micro[1] = a * hn[1];
+  micro[2] = fmax(0.0, hn[2]);
+  g_norm(micro);
+  g_to_world(micro, n, hw);
+  g_norm(hw);
+  const double vh = g_dot(v, hw);
+  for (int k = 0; k < 3; ++k) out[k] = -v[k] + 2.0 * vh * hw[k];
+  g_norm(out);
+  if (below)
+    for (int k = 0; k < 3; ++k) out[k] = -out[k];
+  return 1;
+}
+
+static int g_occluded(const orc_mesh *m, const double *o, const double *d) {
+  for (int64_t t = 0; t < m->n_tris; ++t) {
+    const float *A = m->vertices + 3 * (size_t)m->indices[3 * t];
+    const float *B = m->vertices + 3 * (size_t)m->indices[3 * t + 1];
+    const float *C = m->vertices + 3 * (size_t)m->indices[3 * t + 2];
+    double e1[3], e2[3], s[3], p[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = (double)B[k] - A[k];
+      e2[k] = (double)C[k] - A[k];
+      s[k] = o[k] - A[k];
+    }
+    g_cross(d, e2, p);
+    const double det = g_dot(e1, p);
+    if (det == 0.0) continue;
+    const double u = g_dot(s, p) / det;
+    if (!(u >= 0 && u <= 1)) continue;
+    g_cross(s, e1, q);
+    const double v = g_dot(d, q) / det;
+    if (!(v >= 0 && u + v <= 1)) continue;
+    if (g_dot(e2, q) / det > 1e-2) return 1;
+  }
+  return 0;
+}
+
+static void g_direct(const orc_mesh *m, const double *tab, int n, const float *light,
+                     const double *p, const double *d, const double *ns, const double *ng,
+                     const float *mat, int samples, uint32_t seed, uint32_t pixel, double *out) {
+  g_surface s;
+  for (int a = 0; a < 3; ++a) {
+    s.kd[a] = mat[a];
+    s.ks[a] = mat[3 + a];
+    s.ns[a] = ns[a];
+    s.ng[a] = ng[a];
+  }
+  s.rough = mat[6];
+  const double v[3] = {-d[0], -d[1], -d[2]};
+  double acc[3] = {0, 0, 0};
+  for (int k = 0; k < samples; ++k) {
+    const uint32_t c = (uint32_t)k * 8u;
+    double l[3], f[3], le[3];
+    g_env_sample(tab, n, orc_rng_uniform(seed, pixel, c), orc_rng_uniform(seed, pixel, c + 1),
+                 orc_rng_uniform(seed, pixel, c + 2), l);
+    double pe = g_env_pdf(tab, n, l);
+    if (pe > 1e-4 && !g_occluded(m, p, l)) {
+      const double pb = g_ct_pdf(&s, l, v);
+      g_ct_eval(&s, l, v, f);
+      g_env(light, n, l, le);
+      const double w = pe / fmax(pe * pe + pb * pb, 1e-8);
+      for (int a = 0; a < 3; ++a) {
+        const double r = f[a] * le[a] * w;
+        if (!isnan(r)) acc[a] += r;
+      }
+    }
+    if (g_ct_sample(&s, v, orc_rng_uniform(seed, pixel, c + 3),
+                    orc_rng_uniform(seed, pixel, c + 4), orc_rng_uniform(seed, pixel, c + 5), l)) {
+      const double pb = g_ct_pdf(&s, l, v);
+      pe = g_env_pdf(tab, n, l);
+      if (pe > 1e-4 && !g_occluded(m, p, l)) {
+        g_ct_eval(&s, l, v, f);
+        g_env(light, n, l, le);
+        const double w = pb / fmax(pe * pe + pb * pb, 1e-8);
+        for (int a = 0; a < 3; ++a) {
+          const double r = f[a] * le[a] * w;
+          if (!isnan(r)) acc[a] += r;
+        }
+      }
+    }
+  }
+  for (int a = 0; a < 3; ++a) out[a] = acc[a] / samples;
+}
+
+void orc_gbuffer_cast_direct(const orc_mesh *m, const float *camera, int w, int h,
+                             int max_bounce, const float *bounds, const float *env_faces,
+                             int env_n, const float *light, int light_n, int samples,
+                             uint32_t seed, double *position, double *normal, double *reflect,
+                             double *kd, double *ks, double *roughness, uint8_t *env_flag,
+                             int32_t *prim, double *margin, double *direct) {
+  double *tab = (double *)malloc(sizeof(double) * orc_env_table_doubles(light_n));
+  orc_env_tables(light, light_n, tab);
+  g_cast(m, camera, w, h, max_bounce, bounds, env_faces, env_n, position, normal, reflect, kd, ks,
+         roughness, env_flag, prim, margin, light, tab, light_n, samples, seed, direct);
+  free(tab);
+}
+
+/* Test taps of the direct-light pieces (mat: kd[3], ks[3], roughness, type). */
+static void g_surface_of(const float *mat, const double *ns, const double *ng, g_surface *s) {
+  for (int a = 0; a < 3; ++a) {
+    s->kd[a] = mat[a];
+    s->ks[a] = mat[3 + a];
+    s->ns[a] = ns[a];
+    s->ng[a] = ng[a];
+  }
+  s->rough = mat[6];
+}
+void orc_ct_eval(const float *mat, const double *ns, const double *ng, const double *l,
+                 const double *v, double *out) {
+  g_surface s;
+  g_surface_of(mat, ns, ng, &s);
+  g_ct_eval(&s, l, v, out);
+}
+double orc_ct_pdf(const float *mat, const double *ns, const double *ng, const double *l,
+                  const double *v) {
+  g_surface s;
+  g_surface_of(mat, ns, ng, &s);
+  return g_ct_pdf(&s, l, v);
+}
+double orc_env_pdf(const double *tab, int n, const double *d) { return g_env_pdf(tab, n, d); }

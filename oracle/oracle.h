@@ -38,6 +38,11 @@
  *                        independent numpy edge-function intersection on
  *                        random triangle soups, cube-face lookups at texel
  *                        centres and axis directions, translation invariance
+ *   orc_gbuffer_cast_direct  table invariants and inverse-CDF round trips;
+ *                        the estimator's mean over many samples against
+ *                        an independent quadrature of its expectation
+ *                        (diffuse cookTorrance, no occluders); closed forms
+ *                        of the BSDF (white furnace bound, reciprocity)
  * The tiny-cuda-nn conventions behind the hash grid and the MLP (level
  * rounding, layer layout, padding values) cannot be pinned against the
  * reference: that library is an empty submodule there ("parity unpinned"
@@ -46,6 +51,7 @@
 #ifndef NWPRT_ORACLE_H
 #define NWPRT_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -127,6 +133,30 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
                       const float *bounds, const float *env_faces, int env_n, double *position,
                       double *normal, double *reflect, double *kd, double *ks, double *roughness,
                       uint8_t *env_flag, int32_t *prim, double *margin);
+
+/* The same plus the direct term (FP64, replaying the product's random
+ * stream; light faces [6][light_n][light_n][3], `samples` estimates per
+ * pixel); direct: double [P][4]. */
+void orc_gbuffer_cast_direct(const orc_mesh *m, const float *camera, int w, int h,
+                             int max_bounce, const float *bounds, const float *env_faces,
+                             int env_n, const float *light, int light_n, int samples,
+                             uint32_t seed, double *position, double *normal, double *reflect,
+                             double *kd, double *ks, double *roughness, uint8_t *env_flag,
+                             int32_t *prim, double *margin, double *direct);
+/* Environment sampling tables (layout: face pmf[6], face cdf[7], face
+ * luminance[6], then per face cdf_rows[n][n+1], pdf_rows[n][n],
+ * cdf_marginals[n+1], pdf_marginals[n]). */
+size_t orc_env_table_doubles(int n);
+void orc_env_tables(const float *faces, int n, double *tab);
+/* Pieces of the direct term for tests: cookTorrance eval (with the cosine)
+ * and pdf, and the environment pdf per unit face area. */
+void orc_ct_eval(const float *mat, const double *ns, const double *ng, const double *l,
+                 const double *v, double *out);
+double orc_ct_pdf(const float *mat, const double *ns, const double *ng, const double *l,
+                  const double *v);
+double orc_env_pdf(const double *tab, int n, const double *d);
+/* The product's counter-based uniform in [0, 1). */
+float orc_rng_uniform(uint32_t seed, uint32_t pixel, uint32_t counter);
 
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
