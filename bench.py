@@ -210,7 +210,8 @@ def bench_serving(nw, ctx, prec, n_requests=60):
 def gbuffer_bench_mesh():
     """Synthetic scene for the G-buffer producer leg: a 768 x 672 heightfield
     (1,032,192 triangles, per-vertex normals) under a 512 x 256 UV sphere
-    (262,144 triangles), three materials; ~1.29 M triangles in all."""
+    (262,144 triangles), three cookTorrance materials; ~1.29 M triangles in
+    all."""
     gx, gz = 768, 672
     xs = np.linspace(-8.0, 8.0, gx + 1)
     zs = np.linspace(-9.0, 5.0, gz + 1)
@@ -238,8 +239,8 @@ def gbuffer_bench_mesh():
     idx = np.concatenate([ti, si]).astype(np.int32)
     mids = np.concatenate([(np.arange(len(ti)) // 4096) % 2,
                            np.full(len(si), 2)]).astype(np.int32)
-    mats = [[0.45, 0.4, 0.3, 0.04, 0.04, 0.04, 0.6, 0], [0.3, 0.45, 0.25, 0.04, 0.04, 0.04, 0.8, 0],
-            [0.8, 0.2, 0.2, 0.5, 0.5, 0.5, 0.15, 0]]
+    mats = [[0.45, 0.4, 0.3, 0.04, 0.04, 0.04, 0.6, 2], [0.3, 0.45, 0.25, 0.04, 0.04, 0.04, 0.8, 2],
+            [0.8, 0.2, 0.2, 0.5, 0.5, 0.5, 0.15, 2]]
     return verts, norms, idx, mids, mats
 
 
@@ -276,6 +277,21 @@ def bench_gbuffer(nw, ctx, W, H, stream, dfaces, n_face):
            "rays_per_s": W * H / (cast_ms / 1e3), "hit_fraction": float(flags.mean()),
            "write_bytes_per_pixel": GBUFFER_WRITE_BYTES_PER_PIXEL,
            "write_gbs": wb / (cast_ms / 1e3) / 1e9}
+    # The same cast with the direct-light term: per pixel one environment
+    # and one BSDF sample, each with a shadow ray, lit by the bench cubemap.
+    for spp in (1, 4):
+        castd = lambda: ctx.gbuffer_cast(mesh, cam, gframe, env_faces=dfaces, env_n=n_face,
+                                         light_faces=dfaces, light_n=n_face, direct_samples=spp,
+                                         seed=7, stream=stream)
+        castd()
+        ctx.sync()
+        ev[0].record(stream)
+        for i in range(5):
+            castd()
+            ev[i + 1].record(stream)
+        ctx.sync()
+        dms = float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(5)]))
+        out[f"with_direct_{spp}spp_ms"] = dms
     ctx.mesh_destroy(mesh)
     del gframe
     return out

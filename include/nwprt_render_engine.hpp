@@ -125,12 +125,16 @@ class RenderEngine {
   // RenderEngine::render (Renderer/RenderEngine.cpp:328-353), G-buffer part
   // in wavelet mode (maximumBounce 1): first-hit encodings of frameID from
   // the mesh and camera.  viewEnvFaces (device [6][n][n][3], optional) gives
-  // the kd of pixels that see the environment.  The direct-light buffer is
-  // not written.
+  // the kd of pixels that see the environment.  With lightEnvFaces the
+  // direct-light buffer gets `directSamples` estimates per pixel
+  // (samplesPerPixel x directSamplesPerLight of renderConfig); without it,
+  // that buffer is not written.
   void render(int frameID, hipStream_t stream = nullptr, const float *viewEnvFaces = nullptr,
-              int viewEnvN = 0) {
+              int viewEnvN = 0, const float *lightEnvFaces = nullptr, int lightEnvN = 0,
+              int directSamples = 1, uint32_t seed = 0) {
     nwprt_gbuffer g = gbuffer(frameID);
-    nwprt_cast_params p{1, hasBounds_ ? bounds_ : nullptr, viewEnvFaces, viewEnvN};
+    nwprt_cast_params p{1,          hasBounds_ ? bounds_ : nullptr, viewEnvFaces, viewEnvN,
+                        lightEnvFaces, lightEnvN,                   directSamples, seed};
     check(nwprt_gbuffer_cast(ctx_, mesh_, &camera_, width_, height_, &p, &g, stream));
   }
 

@@ -475,66 +475,85 @@ __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, fl
   return v3(acc[0] * inv, acc[1] * inv, acc[2] * inv);
 }
 
-// Sampling tables of one cube face per workgroup, one thread per row
-// (Environment_Map::init_sampling_dist, EnvironmentMap.cpp:153-178, and
-// make_table_dist_2d, table_dist.cpp:43-118).  Table row r, column c holds
-// texel (r, c) of the face, weighted by TexelCoordSolidAngle evaluated at
-// ((r + 0.5) / n, (c + 0.5) / n) as the reference calls it
-// (EnvironmentMap.cpp:128-147, 167-174).
-__global__ __launch_bounds__(128) void env_tables_kernel(const float *__restrict__ faces, int n,
-                                                        double *__restrict__ tab) {
+// Sampling tables (Environment_Map::init_sampling_dist,
+// EnvironmentMap.cpp:153-178, and make_table_dist_2d, table_dist.cpp:43-118).
+// Table row r, column c of a face holds texel (r, c), weighted by
+// TexelCoordSolidAngle evaluated at ((r + 0.5) / n, (c + 0.5) / n) as the
+// reference calls it (EnvironmentMap.cpp:128-147, 167-174).
+// Pass 1: one workgroup per (face, row), one lane per texel; the row's
+// prefix sum runs in order on lane 0.  The row integral and the row's raw
+// luminance sum are parked in the face's marginal arrays for pass 2.
+__global__ __launch_bounds__(128) void env_rows_kernel(const float *__restrict__ faces, int n,
+                                                      double *__restrict__ tab) {
+  const int f = blockIdx.x / n, r = blockIdx.x % n, c = threadIdx.x;
+  double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
+  double *cr = base + (size_t)r * (n + 1);
+  double *pr = base + (size_t)n * (n + 1) + (size_t)r * n;
+  double *cdf_m = base + (size_t)n * (n + 1) + (size_t)n * n, *pdf_m = cdf_m + (n + 1);
+  __shared__ double fv[128], lv[128];
+  if (c < n) {
+    const double U = (2.0 * ((double)((r + 0.5) / n) + 0.5) / (double)n) - 1.0;
+    const double V = (2.0 * ((double)((c + 0.5) / n) + 0.5) / (double)n) - 1.0;
+    const double inv = 1.0 / n;
+    auto area = [](double x, double y) { return atan2(x * y, sqrt(x * x + y * y + 1.0)); };
+    const double sa = area(U - inv, V - inv) - area(U - inv, V + inv) - area(U + inv, V - inv) +
+                      area(U + inv, V + inv);
+    const float saf = (float)sa;
+    const float kScale = __fdiv_rn(__fmul_rn(2.0f, 3.14159265358979323846f), 3.0f);
+    const float *px = faces + (((size_t)f * n + r) * n + c) * 3;
+    const float3 t = v3(px[0], px[1], px[2]);
+    const float3 q = v3(__fdiv_rn(__fmul_rn(fabsf(t.x), saf), kScale),
+                        __fdiv_rn(__fmul_rn(fabsf(t.y), saf), kScale),
+                        __fdiv_rn(__fmul_rn(fabsf(t.z), saf), kScale));
+    fv[c] = (double)luminance(q);
+    lv[c] = (double)luminance(t);
+  }
+  __syncthreads();
+  if (c == 0) {
+    double acc = 0.0, lum = 0.0;
+    cr[0] = 0.0;
+    for (int x = 0; x < n; ++x) {
+      acc += fv[x];
+      cr[x + 1] = acc;
+      lum += lv[x];
+    }
+    pdf_m[r] = acc;  // row integral, read by pass 2
+    cdf_m[r] = lum;  // row luminance, read by pass 2
+  }
+  __syncthreads();
+  if (c < n) {
+    const double integral = cr[n];
+    if (integral > 0) {
+      cr[c] /= integral;
+      pr[c] = fv[c] / integral;
+    } else {
+      cr[c] = 0.0;
+      pr[c] = 0.0;
+    }
+  }
+}
+
+// Pass 2: one workgroup per face: marginals in row order, the face's
+// luminance, and each row's last CDF entry set to 1 (0 for a black row).
+__global__ __launch_bounds__(128) void env_marginals_kernel(int n, double *__restrict__ tab) {
   const int f = blockIdx.x, r = threadIdx.x;
   double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
-  double *cdf_rows = base, *pdf_rows = base + (size_t)n * (n + 1);
-  double *cdf_m = pdf_rows + (size_t)n * n, *pdf_m = cdf_m + (n + 1);
+  double *cdf_m = base + (size_t)n * (n + 1) + (size_t)n * n, *pdf_m = cdf_m + (n + 1);
   __shared__ double row_int[128], row_lum[128];
   if (r < n) {
-    const float *px = faces + ((size_t)f * n + r) * n * 3;
-    double *cr = cdf_rows + (size_t)r * (n + 1);
-    double *pr = pdf_rows + (size_t)r * n;
-    const float kScale = __fdiv_rn(__fmul_rn(2.0f, 3.14159265358979323846f), 3.0f);
-    double lum_sum = 0.0;
-    cr[0] = 0.0;
-    for (int c = 0; c < n; ++c) {
-      const double U = (2.0 * ((double)((r + 0.5) / n) + 0.5) / (double)n) - 1.0;
-      const double V = (2.0 * ((double)((c + 0.5) / n) + 0.5) / (double)n) - 1.0;
-      const double inv = 1.0 / n;
-      auto area = [](double x, double y) { return atan2(x * y, sqrt(x * x + y * y + 1.0)); };
-      const double sa = area(U - inv, V - inv) - area(U - inv, V + inv) - area(U + inv, V - inv) +
-                        area(U + inv, V + inv);
-      const float saf = (float)sa;
-      const float3 t = v3(px[3 * c], px[3 * c + 1], px[3 * c + 2]);
-      const float3 q = v3(__fdiv_rn(__fmul_rn(fabsf(t.x), saf), kScale),
-                          __fdiv_rn(__fmul_rn(fabsf(t.y), saf), kScale),
-                          __fdiv_rn(__fmul_rn(fabsf(t.z), saf), kScale));
-      const double fv = (double)luminance(q);
-      pr[c] = fv;
-      cr[c + 1] = cr[c] + fv;
-      lum_sum += (double)luminance(t);
-    }
-    const double integral = cr[n];
-    for (int c = 0; c < n; ++c) {
-      if (integral > 0) {
-        cr[c] /= integral;
-        pr[c] /= integral;
-      } else {
-        cr[c] = 0.0;
-        pr[c] = 0.0;
-      }
-    }
-    if (!(integral > 0)) cr[n] = 0.0;
-    row_int[r] = cr[n];
-    row_lum[r] = lum_sum;
+    row_int[r] = pdf_m[r];
+    row_lum[r] = cdf_m[r];
   }
   __syncthreads();
   if (r == 0) {
+    double lum = 0.0, acc = 0.0;
     cdf_m[0] = 0.0;
-    double lum = 0.0;
     for (int y = 0; y < n; ++y) {
-      cdf_m[y + 1] = cdf_m[y] + row_int[y];
+      acc += row_int[y];
+      cdf_m[y + 1] = acc;
       lum += row_lum[y];
     }
-    const double total = cdf_m[n];
+    const double total = acc;
     for (int y = 0; y < n; ++y) {
       if (total > 0) {
         cdf_m[y] /= total;
@@ -547,9 +566,8 @@ __global__ __launch_bounds__(128) void env_tables_kernel(const float *__restrict
     cdf_m[n] = 1.0;
     tab[13 + f] = lum;
   }
-  __syncthreads();
   if (r < n) {
-    double *cr = cdf_rows + (size_t)r * (n + 1);
+    double *cr = base + (size_t)r * (n + 1);
     cr[n] = cr[n - 1] > 0 ? 1.0 : 0.0;
   }
 }
@@ -666,8 +684,10 @@ size_t env_table_doubles(int n) { return kTabHead + 6 * env_face_doubles(n); }
 void launch_env_tables(const float *faces, int n, double *tab, hipStream_t s) {
   require(faces && tab && n >= 1 && n <= 128, NWPRT_ERR_INVALID_ARGUMENT,
           "env tables: face size must be in [1, 128]");
-  hipLaunchKernelGGL(env_tables_kernel, dim3(6), dim3(128), 0, s, faces, n, tab);
-  hip_check(hipGetLastError(), "env_tables_kernel launch");
+  hipLaunchKernelGGL(env_rows_kernel, dim3(6 * n), dim3(128), 0, s, faces, n, tab);
+  hip_check(hipGetLastError(), "env_rows_kernel launch");
+  hipLaunchKernelGGL(env_marginals_kernel, dim3(6), dim3(128), 0, s, n, tab);
+  hip_check(hipGetLastError(), "env_marginals_kernel launch");
   hipLaunchKernelGGL(env_face_table_kernel, dim3(1), dim3(64), 0, s, tab);
   hip_check(hipGetLastError(), "env_face_table_kernel launch");
 }

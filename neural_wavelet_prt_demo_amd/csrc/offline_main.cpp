@@ -11,11 +11,13 @@
 // G-buffers: with --obj, frame i's G-buffer comes from the ray caster over
 // the OBJ mesh, seen from the trajectory's camera (from = columns 0-2, at =
 // columns 3-5, up +y, Renderer/offlineRenderer.cpp:118), positions
-// normalised by the mesh's bounds; its direct-light buffer is zero (the
-// reference's Monte Carlo direct term is not produced), so
-// predictedWithDirect equals predictedIndirect.  Without --obj, frame i's
-// G-buffer is the seeded synthetic one with seed (--seed + i) and the camera
-// columns are parsed and ignored.
+// normalised by the mesh's bounds.  The frame's environment
+// (<env folder>/cube/<name>.hdr) is both the view and the lighting map, and
+// the direct-light buffer gets samplesPerPixel x directSamplesPerLight
+// estimates per pixel (trajectory keywords, defaults 1024 x 1 as
+// renderConfig's, Renderer/RenderEngine.h:30-31) with random stream
+// (--seed + i).  Without --obj, frame i's G-buffer is the seeded synthetic
+// one with seed (--seed + i) and the camera columns are parsed and ignored.
 //
 //   nwprt_offline <results_dir> -w <model_dir> -m <env_folder> -t <trajectory>
 //                 [-s start] [-e end] [-l] [-k K] [--seed S] [--precision bf16|fp16|fp32]
@@ -38,6 +40,7 @@ namespace {
 struct Trajectory {
   std::string envmapFolder;
   bool finalDenoiser = false;
+  int samplesPerPixel = 1024, directSamplesPerLight = 1;
   std::vector<std::string> envmaps;
   std::vector<nwprt_camera> cameras;
 };
@@ -50,7 +53,7 @@ Trajectory readTrajectory(const std::string &path) {
   Trajectory t;
   bool started = false;
   std::string line;
-  const char *keywords[] = {"samplesPerPixel", "directSamplesPerLight", "IndirectSamplesPerDirect",
+  const char *keywords[] = {"IndirectSamplesPerDirect",
                             "finalDenoiser", "directIndirectDenoiser", "trainingMode",
                             "waveletSamplesPerPixel", "waveletSamplesPerLight",
                             "waveletDirectDenoiser"};
@@ -68,6 +71,10 @@ Trajectory readTrajectory(const std::string &path) {
       started = true;
     } else if (tok[0] == "envmapFolder" && tok.size() > 1) {
       t.envmapFolder = tok[1];
+    } else if (tok[0] == "samplesPerPixel" && tok.size() > 1) {
+      t.samplesPerPixel = std::stoi(tok[1]);
+    } else if (tok[0] == "directSamplesPerLight" && tok.size() > 1) {
+      t.directSamplesPerLight = std::stoi(tok[1]);
     } else if (tok[0] == "waveletFinalDenoiser" && tok.size() > 1) {
       t.finalDenoiser = tok[1] == "true";
     } else if (std::find_if(std::begin(keywords), std::end(keywords),
@@ -163,14 +170,39 @@ int main(int argc, char **argv) {
         nwprt_free_host(p);
       need(st, "uploading the mesh");
       engine.setMesh(mesh);
-      engine.clearDirectLight(1);
     }
+    float *envFaces = nullptr;  // device cube faces of the current frame's map
+    int envN = 0;
     end = std::min<int>(end, (int)traj.envmaps.size());
     for (int i = start; i < end; ++i) {
       engine.buildLights({{envFolder, traj.envmaps[(size_t)i]}}, k);
       if (mesh >= 0) {
+        const std::string cross =
+            (fs::path(envFolder) / "cube" / (traj.envmaps[(size_t)i] + ".hdr")).string();
+        float *rgb = nullptr;
+        int32_t cw = 0, ch = 0;
+        if (nwprt_hdr_read(cross.c_str(), &rgb, &cw, &ch) != NWPRT_OK)
+          throw std::runtime_error(std::string("reading ") + cross + ": " + nwprt_last_error());
+        const int n = cw / 4;
+        std::vector<float> faces((size_t)6 * n * n * 3);
+        const nwprt_status cs = nwprt_cubemap_cross_to_faces(rgb, cw, ch, faces.data());
+        nwprt_free_host(rgb);
+        if (cs != NWPRT_OK)
+          throw std::runtime_error(std::string("cube cross ") + cross + ": " + nwprt_last_error());
+        if (n != envN) {
+          if (envFaces) (void)hipFree(envFaces);
+          envFaces = nullptr;
+          if (hipMalloc(&envFaces, faces.size() * sizeof(float)) != hipSuccess)
+            throw std::runtime_error("device allocation of the environment faces failed");
+          envN = n;
+        }
+        if (hipMemcpy(envFaces, faces.data(), faces.size() * sizeof(float),
+                      hipMemcpyHostToDevice) != hipSuccess)
+          throw std::runtime_error("uploading the environment faces failed");
         engine.setCamera(traj.cameras[(size_t)i]);
-        engine.render(1);
+        engine.render(1, nullptr, envFaces, envN, envFaces, envN,
+                      traj.samplesPerPixel * traj.directSamplesPerLight,
+                      (uint32_t)(seed + (uint64_t)i));
       } else {
         engine.writeSyntheticGBuffer(1, seed + (uint64_t)i);
       }
@@ -185,6 +217,7 @@ int main(int argc, char **argv) {
     }
     engine.destroyLights();
     if (mesh >= 0) nwprt_mesh_destroy(engine.context(), mesh);
+    if (envFaces) (void)hipFree(envFaces);
   } catch (const std::exception &e) {
     std::cerr << "nwprt_offline: " << e.what() << "\n";
     return 1;

@@ -205,8 +205,10 @@ def test_offline_driver_with_denoiser(nw, ctx, tmp_path):
 def test_offline_driver_with_obj_scene(nw, ctx, tmp_path):
     """--obj: frame i's G-buffer comes from the ray caster over the OBJ mesh
     seen from trajectory camera i (up +y, Renderer/offlineRenderer.cpp:118),
-    with a zero direct term; the written images equal obj_read ->
-    mesh_create -> gbuffer_cast -> relight through the C ABI."""
+    lit for the direct term by the frame's own cube map with
+    samplesPerPixel x directSamplesPerLight samples and stream i; the written
+    images equal obj_read -> mesh_create -> gbuffer_cast -> relight through
+    the C ABI."""
     import subprocess
     from pathlib import Path
     from cases import mesh_scene
@@ -236,8 +238,8 @@ def test_offline_driver_with_obj_scene(nw, ctx, tmp_path):
     (tmp_path / "scene.mtl").write_text("\n".join(mtl) + "\n")
     cams = [(0.5, 1.0, 6.0, 0.0, 0.0, -1.5), (-2.0, 2.5, 5.0, 0.5, -0.5, -2.0)]
     traj = tmp_path / "traj.txt"
-    traj.write_text(f"envmapFolder {env_dir}\ntrajectoryBegin\n" +
-                    "".join(" ".join(map(str, c)) + " a\n" for c in cams))
+    traj.write_text(f"envmapFolder {env_dir}\nsamplesPerPixel 3\ndirectSamplesPerLight 2\n"
+                    "trajectoryBegin\n" + "".join(" ".join(map(str, c)) + " a\n" for c in cams))
     out = tmp_path / "out"
     r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-t", str(traj),
                         "-k", str(K), "--precision", "fp32", "--obj", str(tmp_path / "scene.obj")],
@@ -249,14 +251,19 @@ def test_offline_driver_with_obj_scene(nw, ctx, tmp_path):
                            material_ids=m["material_ids"])
     scene = ctx.initTCNN(tmp_path / "model")
     env = ctx.envmap_load_folder(env_dir, "a", K)
+    faces = nw.cross_to_faces(nw.hdr_read(env_dir / "cube" / "a.hdr"))
+    dfaces = nw.DeviceBuffer.from_array(np.ascontiguousarray(faces, np.float32))
     for i, c in enumerate(cams):
         frame = nw.DeviceFrame(512, 512)
-        frame.bufs["direct"].upload(np.zeros((512 * 512, 4), np.float32))
-        ctx.gbuffer_cast(mesh, (c[0:3], c[3:6], (0, 1, 0)), frame)
+        ctx.gbuffer_cast(mesh, (c[0:3], c[3:6], (0, 1, 0)), frame, env_faces=dfaces,
+                         env_n=n_face, light_faces=dfaces, light_n=n_face, direct_samples=6,
+                         seed=i)
         ctx.relight(scene, env, frame, nw.PREC_FP32)
         ctx.sync()
         flags = frame.bufs["env_flag"].download(np.uint8, (512 * 512,))
         assert 0.05 < flags.mean() < 1.0
+        direct = frame.bufs["direct"].download(np.float32, (512 * 512, 4))
+        assert (direct[flags == 1, :3] > 0).any()
         res = frame.results()
         for key, folder in (("indirect", "predictedIndirect"),
                             ("indirect_with_direct", "predictedWithDirect")):
