@@ -209,7 +209,8 @@ __device__ inline bool occluded(const BvhNode *__restrict__ nodes, const BvhTri 
 // The closest-hit program's direct term in wavelet mode (maximumBounce 1,
 // devicePrograms.cu:165-280): per light sample, one environment-importance
 // sample and one BSDF sample, each shadow-tested and weighted by the power
-// heuristic, for the cookTorrance BSDF (cookTorrance.cu:26-195).  Random
+// heuristic, for the cookTorrance (cookTorrance.cu:26-195) and roughPlastic
+// BSDFs.  Random
 // numbers come from a counter-based generator (SplitMix64's finaliser over
 // seed, pixel and a per-sample counter) so a host restatement can replay
 // every sample.
@@ -270,6 +271,7 @@ struct Surface {
   float3 kd, ks;
   float rough;
   float3 ns, ng;  // unit shading and geometric normals
+  int type;       // 0 roughPlastic, 2 cookTorrance
 };
 
 __device__ float3 ct_eval(const Surface &s, float3 l, float3 v) {
@@ -339,6 +341,87 @@ __device__ float3 ct_sample(const Surface &s, float3 v, float rw, float ru, floa
   const float3 hw = normalize(to_world(micro, n));
   const float3 out = normalize(add(scale(v, -1.0f), scale(hw, 2.0f * dot(v, hw))));
   return below ? scale(out, -1.0f) : out;
+}
+
+// roughPlastic (roughPlastic.cu:25-174): a GGX coat over a diffuse base,
+// both weighted by the dielectric Fresnel term at the material's index of
+// refraction, which this build fixes at the reference's default
+// 1.49 / 1.000277 (Material.h:94-96).
+constexpr float kEta = 1.49f / 1.000277f;
+
+__device__ inline float fresnel_dielectric(float n_dot_i, float eta) {
+  const float t2 = 1.0f - (1.0f - n_dot_i * n_dot_i) / (eta * eta);
+  if (t2 < 0.0f) return 1.0f;
+  const float nt = sqrtf(t2), ni = fabsf(n_dot_i);
+  const float rs = (ni - eta * nt) / (ni + eta * nt), rp = (eta * ni - nt) / (eta * ni + nt);
+  return (rs * rs + rp * rp) * 0.5f;
+}
+
+__device__ float3 rp_eval(const Surface &s, float3 l, float3 v) {
+  // (the reference computes, then ignores, the geometric-normal test here)
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  const float3 h = normalize(add(v, l));
+  const float n_h = dot(n, h), n_in = dot(n, v), n_out = dot(n, l);
+  if (n_out <= 0.0f || n_h <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
+  const float r = fminf(fmaxf(s.rough, 0.01f), 1.0f), a = r * r;
+  const float fo = fresnel_dielectric(dot(h, l), kEta);
+  const float d = gtr2(to_local(h, n), a, a);
+  const float g = smith_gtr2(to_local(v, n), a, a) * smith_gtr2(to_local(l, n), a, a);
+  const float k = g * fo * d / (4.0f * n_in * n_out);
+  const float fi = fresnel_dielectric(dot(h, v), kEta);
+  const float base = (1.0f - fo) * (1.0f - fi) / 3.14159265358979f;
+  return v3((s.kd.x * base + s.ks.x * k) * n_out, (s.kd.y * base + s.ks.y * k) * n_out,
+            (s.kd.z * base + s.ks.z * k) * n_out);
+}
+
+__device__ float rp_pdf(const Surface &s, float3 l, float3 v) {
+  if (dot(s.ng, v) < 0.0f || dot(s.ng, l) < 0.0f) return 0.0f;
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  const float3 h = normalize(add(v, l));
+  const float n_h = dot(n, h), n_in = dot(n, v), n_out = dot(n, l);
+  if (n_out <= 0.0f || n_h <= 0.0f) return 0.0f;
+  const float ls = luminance(s.ks), lr = luminance(s.kd);
+  if (ls + lr <= 0.0f) return 0.0f;
+  const float r = fminf(fmaxf(s.rough, 0.01f), 1.0f), a = r * r;
+  float spec = ls / (ls + lr);
+  const float diff = 1.0f - spec;
+  spec *= smith_gtr2(to_local(v, n), a, a) * gtr2(to_local(h, n), a, a) / (4.0f * n_in);
+  return spec + diff * n_out / 3.14159265358979f;
+}
+
+// Two uniforms: z1 picks the lobe and is then reused for the direction, as
+// the reference does.
+__device__ float3 rp_sample(const Surface &s, float3 v, float z1, float z2) {
+  if (dot(s.ng, v) < 0.0f) return v3(-2.0f, -2.0f, -2.0f);
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  const float ls = luminance(s.ks), lr = luminance(s.kd);
+  if (ls + lr <= 0.0f) return v3(-2.0f, -2.0f, -2.0f);
+  const float spec = ls / (ls + lr);
+  if (z1 > spec) {
+    const float phi = 2.0f * 3.14159265358979f * z1;
+    const float tmp = sqrtf(fminf(fmaxf(1.0f - z2, 0.0f), 1.0f));
+    return to_world(v3(cosf(phi) * tmp, sinf(phi) * tmp, sqrtf(fminf(fmaxf(z2, 0.0f), 1.0f))), n);
+  }
+  const float3 li = to_local(v, n);
+  const float r = fminf(fmaxf(s.rough, 0.01f), 1.0f), a = r * r;
+  const float3 hemi = normalize(v3(a * li.x, a * li.y, li.z));
+  const float rr = sqrtf(z1), phi = 2.0f * 3.14159265358979f * z2;
+  const float t1 = rr * cosf(phi);
+  float t2 = rr * sinf(phi);
+  const float sm = (1.0f + hemi.z) * 0.5f;
+  t2 = (1.0f - sm) * sqrtf(1.0f - t1 * t1) + sm * t2;
+  const float3 disk = v3(t1, t2, sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2)));
+  const float3 hn = to_world(disk, hemi);
+  const float3 micro = normalize(v3(a * hn.x, a * hn.y, fmaxf(0.0f, hn.z)));
+  const float3 hw = normalize(to_world(micro, n));
+  return normalize(add(scale(v, -1.0f), scale(hw, 2.0f * dot(v, hw))));
+}
+
+__device__ inline float3 bsdf_eval(const Surface &s, float3 l, float3 v) {
+  return s.type == 0 ? rp_eval(s, l, v) : ct_eval(s, l, v);
+}
+__device__ inline float bsdf_pdf(const Surface &s, float3 l, float3 v) {
+  return s.type == 0 ? rp_pdf(s, l, v) : ct_pdf(s, l, v);
 }
 
 // Cube face and (u, v) of a direction (EnvDirecToUV, later tests win).
@@ -438,8 +521,8 @@ __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, fl
                                   rng_uniform(a.seed, pixel, c + 1), rng_uniform(a.seed, pixel, c + 2));
       const float pe = env_pdf(a.env_tab, a.light_n, l);
       if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
-        const float pb = ct_pdf(s, l, v);
-        const float3 f = ct_eval(s, l, v);
+        const float pb = bsdf_pdf(s, l, v);
+        const float3 f = bsdf_eval(s, l, v);
         const float3 le = env_radiance(a.light_faces, a.light_n, l);
         const float w = pe / fmaxf(pe * pe + pb * pb, 1e-8f);
         const float3 r = v3(f.x * le.x * w, f.y * le.y * w, f.z * le.z * w);
@@ -452,13 +535,16 @@ __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, fl
     }
     // BSDF sample, power-heuristic weight.
     {
-      const float3 l = ct_sample(s, v, rng_uniform(a.seed, pixel, c + 3),
-                                 rng_uniform(a.seed, pixel, c + 4), rng_uniform(a.seed, pixel, c + 5));
+      const float3 l =
+          s.type == 0 ? rp_sample(s, v, rng_uniform(a.seed, pixel, c + 3),
+                                  rng_uniform(a.seed, pixel, c + 4))
+                      : ct_sample(s, v, rng_uniform(a.seed, pixel, c + 3),
+                                  rng_uniform(a.seed, pixel, c + 4), rng_uniform(a.seed, pixel, c + 5));
       if (l.x != -2.0f) {
-        const float pb = ct_pdf(s, l, v);
+        const float pb = bsdf_pdf(s, l, v);
         const float pe = env_pdf(a.env_tab, a.light_n, l);
         if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
-          const float3 f = ct_eval(s, l, v);
+          const float3 f = bsdf_eval(s, l, v);
           const float3 le = env_radiance(a.light_faces, a.light_n, l);
           const float w = pb / fmaxf(pe * pe + pb * pb, 1e-8f);
           const float3 r = v3(f.x * le.x * w, f.y * le.y * w, f.z * le.z * w);
@@ -655,7 +741,7 @@ __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
     rough = mat.roughness;
     flag = 1;
     if constexpr (want_direct) {
-      const Surface sf{kd, ks, rough, ns, ng};
+      const Surface sf{kd, ks, rough, ns, ng, mat.type};
       direct = direct_light(a, sf, p, d, (uint32_t)(iy * a.width + ix), stack);
     }
     break;

@@ -724,7 +724,8 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
 /* ---- direct light --------------------------------------------------------
  * The closest-hit program's direct term in wavelet mode (maximumBounce 1,
  * Renderer/device_functions/devicePrograms.cu:165-280) for the cookTorrance
- * BSDF (Renderer/device_functions/cookTorrance.cu:26-195), replaying the
+ * (Renderer/device_functions/cookTorrance.cu:26-195) and roughPlastic BSDFs
+ * (type from the material row), replaying the
  * product's counter-based random stream: per sample, an environment sample
  * (Renderer/device_functions/cubeMap.cu:115-186 over the tables of
  * Renderer/EnvironmentMap.cpp:128-190 and Renderer/table_dist.cpp:4-130)
@@ -933,6 +934,7 @@ static double g_smith(const double *v, double ax, double ay) {
 
 typedef struct {
   double kd[3], ks[3], rough, ns[3], ng[3];
+  int type; /* 0 roughPlastic, 2 cookTorrance */
 } g_surface;
 
 static void g_shading_normal(const g_surface *s, const double *v, double *n) {
@@ -1029,6 +1031,107 @@ static int g_ct_sample(const g_surface *s, const double *v, float rw, float ru, 
   return 1;
 }
 
+/* roughPlastic (Renderer/device_functions/roughPlastic.cu:25-174), index
+ * of refraction at the reference default 1.49 / 1.000277
+ * (Renderer/Material.h:94-96). */
+static const double g_eta = (double)(1.49f / 1.000277f);
+
+static double g_fresnel(double n_dot_i, double eta) {
+  const double t2 = 1.0 - (1.0 - n_dot_i * n_dot_i) / (eta * eta);
+  if (t2 < 0) return 1.0;
+  const double nt = sqrt(t2), ni = fabs(n_dot_i);
+  const double rs = (ni - eta * nt) / (ni + eta * nt), rp = (eta * ni - nt) / (eta * ni + nt);
+  return (rs * rs + rp * rp) / 2.0;
+}
+
+static void g_rp_eval(const g_surface *s, const double *l, const double *v, double *out) {
+  out[0] = out[1] = out[2] = 0.0;
+  double n[3], h[3], hl[3], vl[3], ll[3];
+  g_shading_normal(s, v, n);
+  for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+  g_norm(h);
+  const double n_h = g_dot(n, h), n_in = g_dot(n, v), n_out = g_dot(n, l);
+  if (n_out <= 0 || n_h <= 0) return;
+  const double r = fmin(fmax(s->rough, 0.01), 1.0), a = r * r;
+  const double fo = g_fresnel(g_dot(h, l), g_eta), fi = g_fresnel(g_dot(h, v), g_eta);
+  g_to_local(h, n, hl);
+  g_to_local(v, n, vl);
+  g_to_local(l, n, ll);
+  const double k = g_smith(vl, a, a) * g_smith(ll, a, a) * fo * g_gtr2(hl, a, a) /
+                   (4.0 * n_in * n_out);
+  const double base = (1.0 - fo) * (1.0 - fi) / M_PI;
+  for (int c = 0; c < 3; ++c) out[c] = (s->kd[c] * base + s->ks[c] * k) * n_out;
+}
+
+static double g_rp_pdf(const g_surface *s, const double *l, const double *v) {
+  if (g_dot(s->ng, v) < 0 || g_dot(s->ng, l) < 0) return 0.0;
+  double n[3], h[3], hl[3], vl[3];
+  g_shading_normal(s, v, n);
+  for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+  g_norm(h);
+  const double n_h = g_dot(n, h), n_in = g_dot(n, v), n_out = g_dot(n, l);
+  if (n_out <= 0 || n_h <= 0) return 0.0;
+  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
+  if (ls + lr <= 0) return 0.0;
+  const double r = fmin(fmax(s->rough, 0.01), 1.0), a = r * r;
+  double spec = ls / (ls + lr);
+  const double diff = 1.0 - spec;
+  g_to_local(v, n, vl);
+  g_to_local(h, n, hl);
+  spec *= g_smith(vl, a, a) * g_gtr2(hl, a, a) / (4.0 * n_in);
+  return spec + diff * n_out / M_PI;
+}
+
+/* z1 chooses the lobe and is reused for the direction, as the reference
+ * does.  Returns 0 for the reference's (-2, -2, -2). */
+static int g_rp_sample(const g_surface *s, const double *v, float z1, float z2, double *out) {
+  if (g_dot(s->ng, v) < 0) return 0;
+  double n[3];
+  g_shading_normal(s, v, n);
+  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
+  if (ls + lr <= 0) return 0;
+  const double spec = ls / (ls + lr);
+  if ((double)z1 > spec) {
+    const double phi = 2.0 * M_PI * z1, tmp = sqrt(fmin(fmax(1.0 - z2, 0.0), 1.0));
+    const double loc[3] = {cos(phi) * tmp, sin(phi) * tmp, sqrt(fmin(fmax((double)z2, 0.0), 1.0))};
+    g_to_world(loc, n, out);
+    return 1;
+  }
+  double li[3];
+  g_to_local(v, n, li);
+  const double r = fmin(fmax(s->rough, 0.01), 1.0), a = r * r;
+  double hemi[3] = {a * li[0], a * li[1], li[2]};
+  g_norm(hemi);
+  const double rr = sqrt((double)z1), phi = 2.0 * M_PI * z2;
+  const double t1 = rr * cos(phi);
+  double t2 = rr * sin(phi);
+  const double sm = (1.0 + hemi[2]) / 2.0;
+  t2 = (1.0 - sm) * sqrt(1.0 - t1 * t1) + sm * t2;
+  const double disk[3] = {t1, t2, sqrt(fmax(0.0, 1.0 - t1 * t1 - t2 * t2))};
+  double hn[3], micro[3], hw[3];
+  g_to_world(disk, hemi, hn);
+  micro[0] = a * hn[0];
+  micro[1] = a * hn[1];
+  micro[2] = fmax(0.0, hn[2]);
+  g_norm(micro);
+  g_to_world(micro, n, hw);
+  g_norm(hw);
+  const double vh = g_dot(v, hw);
+  for (int k = 0; k < 3; ++k) out[k] = -v[k] + 2.0 * vh * hw[k];
+  g_norm(out);
+  return 1;
+}
+
+static void g_bsdf_eval(const g_surface *s, const double *l, const double *v, double *out) {
+  if (s->type == 0)
+    g_rp_eval(s, l, v, out);
+  else
+    g_ct_eval(s, l, v, out);
+}
+static double g_bsdf_pdf(const g_surface *s, const double *l, const double *v) {
+  return s->type == 0 ? g_rp_pdf(s, l, v) : g_ct_pdf(s, l, v);
+}
+
 static int g_occluded(const orc_mesh *m, const double *o, const double *d) {
   for (int64_t t = 0; t < m->n_tris; ++t) {
     const float *A = m->vertices + 3 * (size_t)m->indices[3 * t];
@@ -1064,6 +1167,7 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
     s.ng[a] = ng[a];
   }
   s.rough = mat[6];
+  s.type = (int)mat[7];
   const double v[3] = {-d[0], -d[1], -d[2]};
   double acc[3] = {0, 0, 0};
   for (int k = 0; k < samples; ++k) {
@@ -1073,8 +1177,8 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
                  orc_rng_uniform(seed, pixel, c + 2), l);
     double pe = g_env_pdf(tab, n, l);
     if (pe > 1e-4 && !g_occluded(m, p, l)) {
-      const double pb = g_ct_pdf(&s, l, v);
-      g_ct_eval(&s, l, v, f);
+      const double pb = g_bsdf_pdf(&s, l, v);
+      g_bsdf_eval(&s, l, v, f);
       g_env(light, n, l, le);
       const double w = pe / fmax(pe * pe + pb * pb, 1e-8);
       for (int a = 0; a < 3; ++a) {
@@ -1082,12 +1186,16 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
         if (!isnan(r)) acc[a] += r;
       }
     }
-    if (g_ct_sample(&s, v, orc_rng_uniform(seed, pixel, c + 3),
-                    orc_rng_uniform(seed, pixel, c + 4), orc_rng_uniform(seed, pixel, c + 5), l)) {
-      const double pb = g_ct_pdf(&s, l, v);
+    const int ok = s.type == 0 ? g_rp_sample(&s, v, orc_rng_uniform(seed, pixel, c + 3),
+                                             orc_rng_uniform(seed, pixel, c + 4), l)
+                               : g_ct_sample(&s, v, orc_rng_uniform(seed, pixel, c + 3),
+                                             orc_rng_uniform(seed, pixel, c + 4),
+                                             orc_rng_uniform(seed, pixel, c + 5), l);
+    if (ok) {
+      const double pb = g_bsdf_pdf(&s, l, v);
       pe = g_env_pdf(tab, n, l);
       if (pe > 1e-4 && !g_occluded(m, p, l)) {
-        g_ct_eval(&s, l, v, f);
+        g_bsdf_eval(&s, l, v, f);
         g_env(light, n, l, le);
         const double w = pb / fmax(pe * pe + pb * pb, 1e-8);
         for (int a = 0; a < 3; ++a) {
@@ -1122,17 +1230,18 @@ static void g_surface_of(const float *mat, const double *ns, const double *ng, g
     s->ng[a] = ng[a];
   }
   s->rough = mat[6];
+  s->type = (int)mat[7];
 }
 void orc_ct_eval(const float *mat, const double *ns, const double *ng, const double *l,
                  const double *v, double *out) {
   g_surface s;
   g_surface_of(mat, ns, ng, &s);
-  g_ct_eval(&s, l, v, out);
+  g_bsdf_eval(&s, l, v, out);
 }
 double orc_ct_pdf(const float *mat, const double *ns, const double *ng, const double *l,
                   const double *v) {
   g_surface s;
   g_surface_of(mat, ns, ng, &s);
-  return g_ct_pdf(&s, l, v);
+  return g_bsdf_pdf(&s, l, v);
 }
 double orc_env_pdf(const double *tab, int n, const double *d) { return g_env_pdf(tab, n, d); }

@@ -148,8 +148,9 @@ void orc_gbuffer_cast_direct(const orc_mesh *m, const float *camera, int w, int 
  * cdf_marginals[n+1], pdf_marginals[n]). */
 size_t orc_env_table_doubles(int n);
 void orc_env_tables(const float *faces, int n, double *tab);
-/* Pieces of the direct term for tests: cookTorrance eval (with the cosine)
- * and pdf, and the environment pdf per unit face area. */
+/* Pieces of the direct term for tests: BSDF eval (with the cosine) and pdf
+ * of the material row's type (0 roughPlastic, 2 cookTorrance), and the
+ * environment pdf per unit face area. */
 void orc_ct_eval(const float *mat, const double *ns, const double *ng, const double *l,
                  const double *v, double *out);
 double orc_ct_pdf(const float *mat, const double *ns, const double *ng, const double *l,

@@ -228,23 +228,27 @@ def test_cast_feeds_relight(nw, orc, ctx):
     ctx.mesh_destroy(mesh)
 
 
-def _direct_scene(seed, mirror=True):
+def _direct_scene(seed, mirror=True, cook_torrance=True):
     s = mesh_scene(seed, mirror=mirror)
-    mats = s["materials"].copy()
-    mats[mats[:, 7] != 3, 7] = 2  # cookTorrance everywhere but the mirror
-    s["materials"] = mats
-    return s
+    if cook_torrance:
+        mats = s["materials"].copy()
+        mats[mats[:, 7] != 3, 7] = 2  # cookTorrance everywhere but the mirror
+        s["materials"] = mats
+    return s  # otherwise roughPlastic x 2, cookTorrance, mirror
 
 
-@pytest.mark.parametrize("seed,w,h,samples,bounce", [(0, 32, 24, 64, 1), (1, 29, 17, 16, 2),
-                                                      (2, 1, 1, 256, 1)])
-def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce):
+@pytest.mark.parametrize("seed,w,h,samples,bounce,ct", [(0, 32, 24, 64, 1, True),
+                                                         (1, 29, 17, 16, 2, True),
+                                                         (2, 1, 1, 256, 1, True),
+                                                         (3, 32, 24, 64, 1, False),
+                                                         (4, 23, 19, 32, 2, False)])
+def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce, ct):
     """The direct term draws the same counter-based random numbers as the
     oracle's FP64 replay, so per pixel the two estimates differ only by
     rounding, except where a sample's shadow ray or table bin flips at a
     boundary (rare).  Bar: 98 % of pixels within 1e-3 of
     (|oracle| + 1e-2 max), every pixel within 0.25 max."""
-    s = _direct_scene(seed)
+    s = _direct_scene(seed, cook_torrance=ct)
     rng = np.random.default_rng(seed)
     light = rng.lognormal(0, 1, (6, 8, 8, 3)).astype(np.float32)
     light[2, 3, 4] = 200.0  # a sun on +y
@@ -279,8 +283,10 @@ def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce):
 
 
 def test_direct_light_errors_and_skips(nw, ctx):
-    s = mesh_scene(0)  # roughPlastic materials: no direct-light BSDF here
-    mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
+    s = mesh_scene(0)
+    mats = s["materials"].copy()
+    mats[0, 7] = 1  # DisneyBSDF: no direct-light BSDF here
+    mesh = ctx.mesh_create(s["vertices"], s["indices"], mats, normals=s["normals"],
                            material_ids=s["material_ids"])
     frame = nw.DeviceFrame(8, 8)
     light = nw.DeviceBuffer.from_array(np.ones((6, 4, 4, 3), np.float32))

@@ -6,8 +6,9 @@ independent quadrature of its expectation.
 
 The estimator restates the closest-hit program's direct term in wavelet
 mode (Renderer/device_functions/devicePrograms.cu:165-280) with the
-cookTorrance BSDF (Renderer/device_functions/cookTorrance.cu:26-195) and the
-environment tables of Renderer/EnvironmentMap.cpp:128-190 and
+cookTorrance and roughPlastic BSDFs
+(Renderer/device_functions/cookTorrance.cu:26-195, roughPlastic.cu:25-174)
+and the environment tables of Renderer/EnvironmentMap.cpp:128-190 and
 Renderer/table_dist.cpp:4-130.  Its pdf is per unit face (u, v) area
 (cubeMap.cu:9-19), not per solid angle, so its expectation is the
 quadrature below, not the rendering-equation integral."""
@@ -88,9 +89,10 @@ def _grid(n, s):
     return U.ravel(), V.ravel(), J.ravel(), 1.0 / (m * m)
 
 
-def test_bsdf_reciprocity_and_diffuse_pdf_normalisation():
+@pytest.mark.parametrize("kind", [0, 2])  # roughPlastic, cookTorrance
+def test_bsdf_reciprocity_and_diffuse_pdf_normalisation(kind):
     rng = np.random.default_rng(3)
-    mat = [0.6, 0.4, 0.2, 0.3, 0.3, 0.3, 0.35, 2]
+    mat = [0.6, 0.4, 0.2, 0.3, 0.3, 0.3, 0.35, kind]
     for _ in range(20):
         a, b = rng.normal(size=3), rng.normal(size=3)
         a[1], b[1] = abs(a[1]) + 0.05, abs(b[1]) + 0.05
@@ -103,20 +105,24 @@ def test_bsdf_reciprocity_and_diffuse_pdf_normalisation():
     v = np.array([0.3, 0.8, 0.2])
     v /= np.linalg.norm(v)
     tot = 0.0
+    diffuse = MAT[:7] + [kind]
     for f in range(6):
         for d, j in zip(_face_dir(f, U, V), J):
-            tot += orc.ct_pdf(MAT, UP, UP, d, v) * j * du
+            tot += orc.ct_pdf(diffuse, UP, UP, d, v) * j * du
     assert tot == pytest.approx(1.0, abs=2e-3)
 
 
-def test_estimator_mean_matches_quadrature_of_its_expectation():
-    """Diffuse cookTorrance (ks = 0: cosine sampling, pdf exact) on an
-    open ground plane (nothing above it to shadow), smooth environment.
+@pytest.mark.parametrize("kind", [0, 2])  # roughPlastic, cookTorrance
+def test_estimator_mean_matches_quadrature_of_its_expectation(kind):
+    """Diffuse roughPlastic / cookTorrance (ks = 0: cosine sampling, pdf
+    exact) on an open ground plane (nothing above it to shadow), smooth
+    environment.
     E = integral over faces of pe^2 g / max(pe^2 + pb^2, 1e-8) d(u, v)
       + integral of pb^2 g / max(pe^2 + pb^2, 1e-8) d(omega), pe > 1e-4,
     with g = BSDF x cosine x radiance; the mean of 8 x 2000 replayed
     samples must lie within 5 standard errors (+ 1e-3) of it."""
     n = 8
+    mat = MAT[:7] + [kind]
     rng = np.random.default_rng(11)
     light = rng.lognormal(0, 0.5, (6, n, n, 3)).astype(np.float32)
     ground = np.array([[-50, 0, -50], [50, 0, -50], [50, 0, 50], [-50, 0, 50]], np.float32)
@@ -124,7 +130,7 @@ def test_estimator_mean_matches_quadrature_of_its_expectation():
     cam = [0.0, 2.0, 3.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0]
     means = []
     for seed in range(8):
-        g = orc.gbuffer_cast(ground, ix, [MAT], cam, 2, 2, bounds=[-60, -1, -60, 60, 1, 60],
+        g = orc.gbuffer_cast(ground, ix, [mat], cam, 2, 2, bounds=[-60, -1, -60, 60, 1, 60],
                              light_faces=light, samples=2000, seed=seed)
         assert g["env_flag"][3] == 1
         means.append(g["direct"][3, :3])
@@ -142,10 +148,10 @@ def test_estimator_mean_matches_quadrature_of_its_expectation():
             pe = orc.env_pdf(t, n, d)
             if pe <= 1e-4:
                 continue
-            gv = orc.ct_eval(MAT, UP, UP, d, view) * le
+            gv = orc.ct_eval(mat, UP, UP, d, view) * le
             if not gv.any():
                 continue
-            pb = orc.ct_pdf(MAT, UP, UP, d, view)
+            pb = orc.ct_pdf(mat, UP, UP, d, view)
             den = max(pe * pe + pb * pb, 1e-8)
             e += (pe * pe * gv / den + pb * pb * gv / den * j) * du
     assert np.all(np.abs(est - e) <= 5 * sem + 1e-3 * e), (est, e, sem)
