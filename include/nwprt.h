@@ -496,9 +496,9 @@ typedef struct {
  * sample (face and texel tables of EnvironmentMap.cpp:153-190) and one BSDF
  * sample, each shadow-tested and weighted by the power heuristic, averaged
  * over direct_samples; misses store the view radiance, as kd.  Materials
- * must be roughPlastic (type 0, index of refraction fixed at the reference
- * default 1.49 / 1.000277), cookTorrance (type 2) or mirrors
- * (NWPRT_ERR_UNSUPPORTED otherwise).  Random numbers come from a counter-based generator keyed by
+ * may be roughPlastic (type 0, index of refraction fixed at the reference
+ * default 1.49 / 1.000277), cookTorrance (2), mirrors (3) or simple_ggx
+ * (4); DisneyBSDF (1) gives NWPRT_ERR_UNSUPPORTED.  Random numbers come from a counter-based generator keyed by
  * (seed, pixel, sample), not the reference's per-pixel LCG: estimates agree
  * with the reference's in expectation, not sample by sample. */
 nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,

@@ -209,8 +209,8 @@ __device__ inline bool occluded(const BvhNode *__restrict__ nodes, const BvhTri 
 // The closest-hit program's direct term in wavelet mode (maximumBounce 1,
 // devicePrograms.cu:165-280): per light sample, one environment-importance
 // sample and one BSDF sample, each shadow-tested and weighted by the power
-// heuristic, for the cookTorrance (cookTorrance.cu:26-195) and roughPlastic
-// BSDFs.  Random
+// heuristic, for the cookTorrance (cookTorrance.cu:26-195), roughPlastic
+// and simple_ggx BSDFs.  Random
 // numbers come from a counter-based generator (SplitMix64's finaliser over
 // seed, pixel and a per-sample counter) so a host restatement can replay
 // every sample.
@@ -271,7 +271,7 @@ struct Surface {
   float3 kd, ks;
   float rough;
   float3 ns, ng;  // unit shading and geometric normals
-  int type;       // 0 roughPlastic, 2 cookTorrance
+  int type;       // 0 roughPlastic, 2 cookTorrance, 4 simple_ggx
 };
 
 __device__ float3 ct_eval(const Surface &s, float3 l, float3 v) {
@@ -298,28 +298,40 @@ __device__ float3 ct_eval(const Surface &s, float3 l, float3 v) {
   return v3(r[0], r[1], r[2]);
 }
 
+// Specular-lobe probability: by luminance (cookTorrance), or
+// clamp(1 - roughness, 0.05, 0.75) for simple_ggx (simple_ggx.cu:54-55).
+__device__ inline bool spec_probability(const Surface &s, float &spec) {
+  if (s.type == 4) {
+    spec = fminf(fmaxf(1.0f - fminf(fmaxf(s.rough, 0.05f), 1.0f), 0.05f), 0.75f);
+    return true;
+  }
+  const float ls = luminance(s.ks), lr = luminance(s.kd);
+  if (ls + lr <= 0.0f) return false;
+  spec = ls / (ls + lr);
+  return true;
+}
+
 __device__ float ct_pdf(const Surface &s, float3 l, float3 v) {
   if (dot(s.ng, v) < 0.0f || dot(s.ng, l) < 0.0f) return 0.0f;
   const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
   const float3 h = normalize(add(v, l));
   const float n_h = dot(n, h), n_in = dot(n, v), n_out = dot(n, l);
   if (n_out <= 0.0f || n_h <= 0.0f) return 0.0f;
-  const float ls = luminance(s.ks), lr = luminance(s.kd);
-  if (ls + lr <= 0.0f) return 0.0f;
+  float spec;
+  if (!spec_probability(s, spec)) return 0.0f;
   const float r = fminf(fmaxf(s.rough, 0.05f), 1.0f), a = r * r;
-  float spec = ls / (ls + lr);
   const float diff = 1.0f - spec;
   spec *= smith_gtr2(to_local(v, n), a, a) * gtr2(to_local(h, n), a, a) / (4.0f * n_in);
   return spec + diff * n_out / 3.14159265358979f;
 }
 
-// Lobe choice by luminance, then a cosine-weighted direction or Heitz's
-// (2018) sample of the visible GGX normals; returns x = -2 for a black BSDF.
+// Lobe choice (spec_probability), then a cosine-weighted direction or
+// Heitz's (2018) sample of the visible GGX normals; returns x = -2 for a
+// black BSDF.  Shared by cookTorrance and simple_ggx (simple_ggx.cu:129-177).
 __device__ float3 ct_sample(const Surface &s, float3 v, float rw, float ru, float rv) {
   const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
-  const float ls = luminance(s.ks), lr = luminance(s.kd);
-  if (ls + lr <= 0.0f) return v3(-2.0f, -2.0f, -2.0f);
-  const float spec = ls / (ls + lr);
+  float spec;
+  if (!spec_probability(s, spec)) return v3(-2.0f, -2.0f, -2.0f);
   if (rw > spec) {
     const float phi = 2.0f * 3.14159265358979f * ru;
     const float tmp = sqrtf(fminf(fmaxf(1.0f - rv, 0.0f), 1.0f));
@@ -417,8 +429,29 @@ __device__ float3 rp_sample(const Surface &s, float3 v, float z1, float z2) {
   return normalize(add(scale(v, -1.0f), scale(hw, 2.0f * dot(v, hw))));
 }
 
+// simple_ggx's value (simple_ggx.cu:74-126): a Schlick-Fresnel GGX
+// specular term with F0 = 0.04 and Schlick-GGX geometry k = (a + 2r + 1) / 8,
+// cosines clamped to [1e-6, 1], plus a Lambertian kd / pi.
+__device__ float3 sg_eval(const Surface &s, float3 l, float3 v) {
+  if (dot(s.ng, v) < 0.0f || dot(s.ng, l) < 0.0f) return v3(0.0f, 0.0f, 0.0f);
+  const float3 n = dot(s.ns, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  const float3 h = normalize(add(v, l));
+  auto c01 = [](float x) { return fminf(fmaxf(x, 1e-6f), 1.0f); };
+  const float n_l = c01(dot(n, l)), n_v = c01(dot(n, v)), n_h = c01(dot(n, h)), v_h = c01(dot(h, v));
+  const float a = s.rough * s.rough, a2 = a * a;
+  const float k = (float)(((double)a + 2.0 * s.rough + 1.0) / 8.0);
+  const float fmi = (float)((-5.55473 * (double)v_h - 6.98316) * (double)v_h);
+  const float frac = (0.04f + 0.96f * powf(2.0f, fmi)) * a2;
+  const float nom0 = n_h * n_h * (a2 - 1.0f) + 1.0f;
+  const float nom1 = n_v * (1.0f - k) + k, nom2 = n_l * (1.0f - k) + k;
+  const float nom = fminf(fmaxf(4.0f * 3.14159265358979f * nom0 * nom0 * nom1 * nom2, 1e-6f),
+                          4.0f * 3.14159265358979f);
+  const float sp = frac / nom * n_l, dif = n_l / 3.14159265358979f;
+  return v3(sp + s.kd.x * dif, sp + s.kd.y * dif, sp + s.kd.z * dif);
+}
+
 __device__ inline float3 bsdf_eval(const Surface &s, float3 l, float3 v) {
-  return s.type == 0 ? rp_eval(s, l, v) : ct_eval(s, l, v);
+  return s.type == 0 ? rp_eval(s, l, v) : s.type == 4 ? sg_eval(s, l, v) : ct_eval(s, l, v);
 }
 __device__ inline float bsdf_pdf(const Surface &s, float3 l, float3 v) {
   return s.type == 0 ? rp_pdf(s, l, v) : ct_pdf(s, l, v);

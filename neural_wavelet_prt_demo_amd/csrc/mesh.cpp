@@ -309,7 +309,7 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     mesh->has_normals = normals != nullptr;
     mesh->has_tri_material = material_ids != nullptr;
     for (const MaterialRec &mr : mats)
-      if (mr.type != 0 && mr.type != 2 && mr.type != NWPRT_MATERIAL_MIRROR) mesh->direct_ok = false;
+      if (mr.type == 1) mesh->direct_ok = false;  // DisneyBSDF
     const int id = ctx->next_id++;
     ctx->meshes[id] = std::move(mesh);
     *mesh_id = id;
@@ -420,8 +420,7 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
       require(params->direct_samples >= 1 && params->direct_samples <= (1 << 20),
               NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: direct_samples must be in [1, 2^20]");
       require(m.direct_ok, NWPRT_ERR_UNSUPPORTED,
-              "gbuffer_cast: the direct term supports roughPlastic (type 0), cookTorrance "
-              "(type 2) and mirror materials only");
+              "gbuffer_cast: the direct term does not support DisneyBSDF (type 1) materials");
       check_aligned(gbuffer->direct, 16, "G-buffer direct");
       ctx->env_tables.ensure(env_table_doubles(params->light_env_n) * sizeof(double));
       launch_env_tables(params->light_env_faces, params->light_env_n,

@@ -724,8 +724,8 @@ void orc_gbuffer_cast(const orc_mesh *m, const float *camera, int w, int h, int 
 /* ---- direct light --------------------------------------------------------
  * The closest-hit program's direct term in wavelet mode (maximumBounce 1,
  * Renderer/device_functions/devicePrograms.cu:165-280) for the cookTorrance
- * (Renderer/device_functions/cookTorrance.cu:26-195) and roughPlastic BSDFs
- * (type from the material row), replaying the
+ * (Renderer/device_functions/cookTorrance.cu:26-195), roughPlastic and
+ * simple_ggx BSDFs (type from the material row), replaying the
  * product's counter-based random stream: per sample, an environment sample
  * (Renderer/device_functions/cubeMap.cu:115-186 over the tables of
  * Renderer/EnvironmentMap.cpp:128-190 and Renderer/table_dist.cpp:4-130)
@@ -934,7 +934,7 @@ static double g_smith(const double *v, double ax, double ay) {
 
 typedef struct {
   double kd[3], ks[3], rough, ns[3], ng[3];
-  int type; /* 0 roughPlastic, 2 cookTorrance */
+  int type; /* 0 roughPlastic, 2 cookTorrance, 4 simple_ggx */
 } g_surface;
 
 static void g_shading_normal(const g_surface *s, const double *v, double *n) {
@@ -968,6 +968,21 @@ static void g_ct_eval(const g_surface *s, const double *l, const double *v, doub
   }
 }
 
+/* Specular-lobe probability: by luminance (cookTorrance), or
+ * clamp(1 - roughness, 0.05, 0.75) for simple_ggx
+ * (Renderer/device_functions/simple_ggx.cu:54-55).  0: black BSDF. */
+static int g_spec_probability(const g_surface *s, double *spec) {
+  if (s->type == 4) {
+    *spec = fmin(fmax(1.0 - (double)fminf(fmaxf((float)s->rough, 0.05f), 1.0f), 0.05), 0.75);
+    *spec = (double)(float)*spec;
+    return 1;
+  }
+  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
+  if (ls + lr <= 0) return 0;
+  *spec = ls / (ls + lr);
+  return 1;
+}
+
 static double g_ct_pdf(const g_surface *s, const double *l, const double *v) {
   if (g_dot(s->ng, v) < 0 || g_dot(s->ng, l) < 0) return 0.0;
   double n[3], h[3], hl[3], vl[3];
@@ -976,10 +991,9 @@ static double g_ct_pdf(const g_surface *s, const double *l, const double *v) {
   g_norm(h);
   const double n_h = g_dot(n, h), n_in = g_dot(n, v), n_out = g_dot(n, l);
   if (n_out <= 0 || n_h <= 0) return 0.0;
-  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
-  if (ls + lr <= 0) return 0.0;
+  double spec;
+  if (!g_spec_probability(s, &spec)) return 0.0;
   const double r = fmin(fmax(s->rough, 0.05), 1.0), a = r * r;
-  double spec = ls / (ls + lr);
   const double diff = 1.0 - spec;
   g_to_local(v, n, vl);
   g_to_local(h, n, hl);
@@ -990,11 +1004,9 @@ static double g_ct_pdf(const g_surface *s, const double *l, const double *v) {
 /* Returns 0 for a black BSDF (the reference's (-2, -2, -2)). */
 static int g_ct_sample(const g_surface *s, const double *v, float rw, float ru, float rv,
                        double *out) {
-  double n[3];
+  double n[3], spec;
   g_shading_normal(s, v, n);
-  const double ls = g_lum(s->ks), lr = g_lum(s->kd);
-  if (ls + lr <= 0) return 0;
-  const double spec = ls / (ls + lr);
+  if (!g_spec_probability(s, &spec)) return 0;
   if ((double)rw > spec) {
     const double phi = 2.0 * M_PI * ru, tmp = sqrt(fmin(fmax(1.0 - rv, 0.0), 1.0));
     const double loc[3] = {cos(phi) * tmp, sin(phi) * tmp, sqrt(fmin(fmax((double)rv, 0.0), 1.0))};
@@ -1122,9 +1134,30 @@ static int g_rp_sample(const g_surface *s, const double *v, float z1, float z2, 
   return 1;
 }
 
+/* simple_ggx's value (Renderer/device_functions/simple_ggx.cu:74-126). */
+static void g_sg_eval(const g_surface *s, const double *l, const double *v, double *out) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (g_dot(s->ng, v) < 0 || g_dot(s->ng, l) < 0) return;
+  double n[3], h[3];
+  g_shading_normal(s, v, n);
+  for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+  g_norm(h);
+  const double n_l = fmin(fmax(g_dot(n, l), 1e-6), 1.0), n_v = fmin(fmax(g_dot(n, v), 1e-6), 1.0);
+  const double n_h = fmin(fmax(g_dot(n, h), 1e-6), 1.0), v_h = fmin(fmax(g_dot(h, v), 1e-6), 1.0);
+  const double a = s->rough * s->rough, a2 = a * a, k = (a + 2.0 * s->rough + 1.0) / 8.0;
+  const double fmi = (-5.55473 * v_h - 6.98316) * v_h;
+  const double frac = (0.04 + 0.96 * pow(2.0, fmi)) * a2;
+  const double nom0 = n_h * n_h * (a2 - 1.0) + 1.0;
+  const double nom = fmin(fmax(4.0 * M_PI * nom0 * nom0 * (n_v * (1 - k) + k) * (n_l * (1 - k) + k),
+                               1e-6), 4.0 * M_PI);
+  for (int c = 0; c < 3; ++c) out[c] = frac / nom * n_l + s->kd[c] / M_PI * n_l;
+}
+
 static void g_bsdf_eval(const g_surface *s, const double *l, const double *v, double *out) {
   if (s->type == 0)
     g_rp_eval(s, l, v, out);
+  else if (s->type == 4)
+    g_sg_eval(s, l, v, out);
   else
     g_ct_eval(s, l, v, out);
 }

@@ -228,27 +228,30 @@ def test_cast_feeds_relight(nw, orc, ctx):
     ctx.mesh_destroy(mesh)
 
 
-def _direct_scene(seed, mirror=True, cook_torrance=True):
+def _direct_scene(seed, mirror=True, kinds=None):
+    """kinds: BSDF types for the three non-mirror materials (None keeps the
+    scene's roughPlastic x 2 + cookTorrance)."""
     s = mesh_scene(seed, mirror=mirror)
-    if cook_torrance:
+    if kinds is not None:
         mats = s["materials"].copy()
-        mats[mats[:, 7] != 3, 7] = 2  # cookTorrance everywhere but the mirror
+        mats[:3, 7] = kinds
         s["materials"] = mats
-    return s  # otherwise roughPlastic x 2, cookTorrance, mirror
+    return s
 
 
-@pytest.mark.parametrize("seed,w,h,samples,bounce,ct", [(0, 32, 24, 64, 1, True),
-                                                         (1, 29, 17, 16, 2, True),
-                                                         (2, 1, 1, 256, 1, True),
-                                                         (3, 32, 24, 64, 1, False),
-                                                         (4, 23, 19, 32, 2, False)])
-def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce, ct):
+@pytest.mark.parametrize("seed,w,h,samples,bounce,kinds", [(0, 32, 24, 64, 1, (2, 2, 2)),
+                                                            (1, 29, 17, 16, 2, (2, 2, 2)),
+                                                            (2, 1, 1, 256, 1, (2, 2, 2)),
+                                                            (3, 32, 24, 64, 1, None),
+                                                            (4, 23, 19, 32, 2, None),
+                                                            (5, 32, 24, 48, 1, (4, 0, 4))])
+def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce, kinds):
     """The direct term draws the same counter-based random numbers as the
     oracle's FP64 replay, so per pixel the two estimates differ only by
     rounding, except where a sample's shadow ray or table bin flips at a
     boundary (rare).  Bar: 98 % of pixels within 1e-3 of
     (|oracle| + 1e-2 max), every pixel within 0.25 max."""
-    s = _direct_scene(seed, cook_torrance=ct)
+    s = _direct_scene(seed, kinds=kinds)
     rng = np.random.default_rng(seed)
     light = rng.lognormal(0, 1, (6, 8, 8, 3)).astype(np.float32)
     light[2, 3, 4] = 200.0  # a sun on +y
@@ -295,7 +298,7 @@ def test_direct_light_errors_and_skips(nw, ctx):
                          light_faces=light, light_n=4)
     assert e.value.status == nw.ERR_UNSUPPORTED
     ctx.mesh_destroy(mesh)
-    s = _direct_scene(0)
+    s = _direct_scene(0, kinds=(2, 2, 2))
     mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
                            material_ids=s["material_ids"])
     for kw in ({"light_n": 0}, {"light_n": 200}, {"light_n": 4, "direct_samples": 0}):
