@@ -506,6 +506,30 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
                                 const nwprt_cast_params *params,
                                 const nwprt_gbuffer *gbuffer, void *stream);
 
+/* ---------------------------------------------------------------- SH-PRT -- */
+/* RenderEngine::shRender (Renderer/RenderEngine.cpp:459-486) with
+ * shLoadTMatrix's transfer matrices (:448-457) and an environment's SH
+ * coefficients (Environment_Map::load_sh, EnvironmentMap.cpp:71-78): the
+ * SH-PRT baseline beside wavelet PRT.  For the first hit of each pixel's
+ * primary ray (as nwprt_gbuffer_cast casts it), each of the hit triangle's
+ * vertices gets |sum_k R_k * B_k| with R = T_v L (T_v the vertex's
+ * [nc][nc][3] transfer matrix, row k, column j; L the [nc][3] light
+ * vector; nc = (degree + 1)^2) and B the Monte Carlo projection of the
+ * BSDF onto pbrt's real SH basis at the vertex (vertex normal, direction to
+ * the camera, brdf_samples BSDF samples weighted f / pdf,
+ * devicePrograms.cu:312-501); the pixel interpolates the three colours
+ * barycentrically.  Misses show the view environment (or 0).  Vertex
+ * colours are computed once per triangle corner with a random stream keyed
+ * by (seed, vertex, material), not once per pixel: the same expectation.
+ * transfer: device [n_vertices][nc][nc][3]; light_sh: device [nc][3];
+ * color: device float4 [height][width].  The mesh needs vertex normals;
+ * materials must be roughPlastic, cookTorrance or simple_ggx. */
+nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
+                             int32_t width, int32_t height, const float *transfer,
+                             const float *light_sh, int32_t degree, int32_t brdf_samples,
+                             uint32_t seed, const float *view_env_faces, int32_t view_env_n,
+                             float *color, void *stream);
+
 /* ------------------------------------------------------------ debug taps -- */
 
 /* Hash-grid tap: for n positions (device float4), write the 8 corner entry

@@ -214,6 +214,9 @@ _SIGNATURES = {
     "nwprt_mesh_destroy": (c_int, [c_void_p, c_int32]),
     "nwprt_gbuffer_cast": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32,
                                    POINTER(CastParams), POINTER(GBuffer), c_void_p]),
+    "nwprt_sh_render": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32, c_void_p,
+                                c_void_p, c_int32, c_int32, c_uint32, c_void_p, c_int32, c_void_p,
+                                c_void_p]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -699,6 +702,20 @@ class Context:
         _check(lib().nwprt_gbuffer_cast(self.h, mesh, ctypes.byref(cam), frame.width,
                                         frame.height, ctypes.byref(params), ctypes.byref(gb),
                                         c_void_p(stream) if stream else None))
+
+    def sh_render(self, mesh: int, camera, width: int, height: int, transfer, light_sh,
+                  degree: int, out, *, brdf_samples: int = 64, seed: int = 0, env_faces=None,
+                  env_n: int = 0, stream: Optional[int] = None) -> None:
+        """SH-PRT render (RenderEngine::shRender) into `out` (device float4
+        [height][width]); transfer / light_sh / env_faces are device buffers
+        (DeviceBuffer or address)."""
+        ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+        f, a, u = (np.asarray(x, np.float32).reshape(3) for x in camera)
+        cam = Camera((c_float * 3)(*f), (c_float * 3)(*a), (c_float * 3)(*u))
+        _check(lib().nwprt_sh_render(self.h, mesh, ctypes.byref(cam), width, height,
+                                     ptr(transfer), ptr(light_sh), degree, brdf_samples,
+                                     seed & 0xFFFFFFFF, ptr(env_faces), env_n, ptr(out),
+                                     c_void_p(stream) if stream else None))
 
     # debug taps ----------------------------------------------------------
     def debug_hash_encode(self, scene: int, positions: np.ndarray):

@@ -77,12 +77,13 @@ struct EnvMap {
 // nodes, triangles in leaf order) and the per-vertex / per-triangle data the
 // closest-hit step reads.
 struct Mesh {
-  DevBuf nodes, tris, indices, normals, tri_material, materials;
+  DevBuf nodes, tris, indices, vertices, normals, tri_material, materials;
   float lower[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
-  int64_t n_tris = 0, n_nodes = 0;
+  int64_t n_tris = 0, n_nodes = 0, n_vertices = 0;
   int depth = 0;
   bool has_normals = false, has_tri_material = false;
   bool direct_ok = true;  // every material has a direct-light BSDF here
+  bool sh_ok = true;      // every material has an SH-render BSDF here
 };
 
 // Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
@@ -149,6 +150,7 @@ struct nwprt_context {
   nwprt::DevBuf denoise_scratch;  // denoiser ping-pong images + normalisation
   std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
   nwprt::DevBuf env_tables;  // direct-light sampling tables of the last cast
+  nwprt::DevBuf sh_response, sh_corners;  // SH-PRT render work buffers
 };
 
 namespace nwprt {

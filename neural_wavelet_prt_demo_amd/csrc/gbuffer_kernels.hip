@@ -796,6 +796,191 @@ __global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
     reinterpret_cast<float4 *>(g.direct)[px] = make_float4(direct.x, direct.y, direct.z, 1.0f);
 }
 
+// ------------------------------------------------------------------ SH-PRT
+// The reference's SH-PRT render (RenderEngine::shRender,
+// RenderEngine.cpp:459-486, and __closesthit__SHradiance,
+// devicePrograms.cu:312-501): each hit triangle's three vertex colours are
+// |sum_k R_k (x) B_k| with R = T_v L the vertex's light response (transfer
+// matrix times the environment's SH vector) and B the Monte Carlo SH
+// projection of the BSDF at the vertex (normal and view direction of the
+// vertex, f / pdf weights); the pixel interpolates them barycentrically.
+// Here the vertex colours are computed once per triangle corner (keyed by
+// vertex and material) instead of once per pixel: the same expectation.
+
+constexpr int kShCoeffs = (kShMaxDegree + 1) * (kShMaxDegree + 1);
+
+// Real spherical harmonics as pbrt defines them (shaderUtils.h:230-337):
+// index l (l + 1) + m; associated Legendre functions with the
+// Condon-Shortley phase; K = sqrt((2l + 1) / (4 pi) (l - |m|)! / (l + |m|)!);
+// sqrt(2) K P_l^|m| sin(|m| phi) for m < 0, K P_l^0, sqrt(2) K P_l^m
+// cos(m phi) for m > 0.  Degree 8 (81 values), fully unrolled.
+__device__ inline void sh_eval(float3 w, float (&y)[kShCoeffs]) {
+  constexpr int L = kShMaxDegree;
+  auto idx = [](int l, int m) { return l * l + l + m; };
+  const float x = w.z;
+#pragma unroll
+  for (int k = 0; k < kShCoeffs; ++k) y[k] = 0.0f;
+  y[idx(0, 0)] = 1.0f;
+  y[idx(1, 0)] = x;
+#pragma unroll
+  for (int l = 2; l <= L; ++l)
+    y[idx(l, 0)] = ((2 * l - 1) * x * y[idx(l - 1, 0)] - (l - 1) * y[idx(l - 2, 0)]) / l;
+  {
+    float neg = -1.0f, dfact = 1.0f;
+    const float xroot = sqrtf(fmaxf(0.0f, 1.0f - x * x));
+    float xpow = xroot;
+#pragma unroll
+    for (int l = 1; l <= L; ++l) {
+      y[idx(l, l)] = neg * dfact * xpow;
+      neg = -neg;
+      dfact *= 2 * l + 1;
+      xpow *= xroot;
+    }
+  }
+#pragma unroll
+  for (int l = 2; l <= L; ++l) y[idx(l, l - 1)] = x * (2 * l - 1) * y[idx(l - 1, l - 1)];
+#pragma unroll
+  for (int l = 3; l <= L; ++l)
+#pragma unroll
+    for (int m = 1; m <= l - 2; ++m)
+      y[idx(l, m)] = ((2 * (l - 1) + 1) * x * y[idx(l - 1, m)] - (l - 1 + m) * y[idx(l - 2, m)]) /
+                     (l - m);
+  float sins[L + 1], coss[L + 1];
+  const float xy = sqrtf(fmaxf(0.0f, 1.0f - x * x));
+  {
+    const float sp = xy == 0.0f ? 0.0f : w.y / xy, cp = xy == 0.0f ? 1.0f : w.x / xy;
+    float si = 0.0f, ci = 1.0f;
+#pragma unroll
+    for (int i = 0; i <= L; ++i) {
+      sins[i] = xy == 0.0f ? 0.0f : si;
+      coss[i] = xy == 0.0f ? 1.0f : ci;
+      const float os = si;
+      si = si * cp + ci * sp;
+      ci = ci * cp - os * sp;
+    }
+  }
+  const float sqrt2 = 1.41421356237f;
+#pragma unroll
+  for (int l = 0; l <= L; ++l) {
+#pragma unroll
+    for (int m = -l; m <= l; ++m) {
+      const int am = m < 0 ? -m : m;
+      float inv = 1.0f;  // (l - |m|)! / (l + |m|)!
+#pragma unroll
+      for (int t = l - am + 1; t <= l + am; ++t) inv /= (float)t;
+      const float k = sqrtf((2.0f * l + 1.0f) * 0.0795774715459f * inv);
+      if (m < 0) y[idx(l, m)] = sqrt2 * k * y[idx(l, am)] * sins[am];
+    }
+    y[idx(l, 0)] *= sqrtf((2.0f * l + 1.0f) * 0.0795774715459f);
+#pragma unroll
+    for (int m = 1; m <= l; ++m) {
+      float inv = 1.0f;
+#pragma unroll
+      for (int t = l - m + 1; t <= l + m; ++t) inv /= (float)t;
+      y[idx(l, m)] *= sqrt2 * sqrtf((2.0f * l + 1.0f) * 0.0795774715459f * inv) * coss[m];
+    }
+  }
+}
+
+// R[v][k] = sum_j T[v][k][j] (x) L[j], one thread per (vertex, k).
+__global__ __launch_bounds__(256) void sh_light_response_kernel(ShArgs a) {
+  const int nc = (a.degree + 1) * (a.degree + 1);
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n_vertices * nc) return;
+  const long long v = i / nc;
+  const int k = (int)(i % nc);
+  const float *t = a.transfer + ((size_t)v * nc + k) * nc * 3;
+  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+  for (int j = 0; j < nc; ++j) {
+    r0 = fmaf(t[3 * j], a.light_sh[3 * j], r0);
+    r1 = fmaf(t[3 * j + 1], a.light_sh[3 * j + 1], r1);
+    r2 = fmaf(t[3 * j + 2], a.light_sh[3 * j + 2], r2);
+  }
+  float *o = a.light_response + ((size_t)v * nc + k) * 3;
+  o[0] = r0;
+  o[1] = r1;
+  o[2] = r2;
+}
+
+// Colour of each triangle corner: the vertex's BSDF SH projection (normal
+// and view direction of the vertex, devicePrograms.cu:446-478) dotted with
+// its light response, then |.| per channel.
+__global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
+  const long long c = (long long)blockIdx.x * 64 + threadIdx.x;
+  if (c >= a.n_tris * 3) return;
+  const long long tri = c / 3;
+  const int vi = a.indices[c];
+  const int mid = a.tri_material ? a.tri_material[tri] : 0;
+  const MaterialRec mat = a.materials[mid];
+  const float3 pv = v3(a.vertices + 3 * (size_t)vi);
+  const float3 n = normalize(v3(a.normals + 3 * (size_t)vi));
+  const float3 view = normalize(sub(v3(a.cam_pos), pv));
+  const Surface sf{v3(mat.kd), v3(mat.ks), mat.roughness, n, n, mat.type};
+  const int nc = (a.degree + 1) * (a.degree + 1);
+  const float *lr = a.light_response + (size_t)vi * nc * 3;
+  const uint32_t seed = a.seed ^ (0x9E3779B9u * (uint32_t)(mid + 1));
+  float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+  int valid = 0;
+  for (int k = 0; k < a.samples; ++k) {
+    const uint32_t q = (uint32_t)k * 4u;
+    const float3 l =
+        sf.type == 0 ? rp_sample(sf, view, rng_uniform(seed, (uint32_t)vi, q),
+                                 rng_uniform(seed, (uint32_t)vi, q + 1))
+                     : ct_sample(sf, view, rng_uniform(seed, (uint32_t)vi, q),
+                                 rng_uniform(seed, (uint32_t)vi, q + 1),
+                                 rng_uniform(seed, (uint32_t)vi, q + 2));
+    if (l.x == -2.0f) continue;
+    const float pdf = bsdf_pdf(sf, l, view);
+    if (pdf < 1e-6f) continue;
+    const float3 f = bsdf_eval(sf, l, view);
+    if (!finite3(f)) continue;
+    float y[kShCoeffs];
+    sh_eval(l, y);
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kShCoeffs; ++j) {
+      if (j < nc) {
+        d0 = fmaf(y[j], lr[3 * j], d0);
+        d1 = fmaf(y[j], lr[3 * j + 1], d1);
+        d2 = fmaf(y[j], lr[3 * j + 2], d2);
+      }
+    }
+    acc0 += f.x / pdf * d0;
+    acc1 += f.y / pdf * d1;
+    acc2 += f.z / pdf * d2;
+    ++valid;
+  }
+  const float inv = valid > 0 ? 1.0f / (float)valid : 0.0f;
+  float *o = a.corner_color + 3 * (size_t)c;
+  o[0] = fabsf(acc0 * inv);
+  o[1] = fabsf(acc1 * inv);
+  o[2] = fabsf(acc2 * inv);
+}
+
+// Per pixel: the primary ray's first hit interpolates its triangle's corner
+// colours; a miss shows the view environment (devicePrograms.cu:519-522).
+__global__ __launch_bounds__(64) void sh_pixel_kernel(ShArgs a) {
+  __shared__ int stack_lds[kBvhMaxDepth * 64];
+  const int lane = threadIdx.x;
+  const int tiles_x = (a.width + 7) >> 3;
+  const int ix = (int)(blockIdx.x % (unsigned)tiles_x) * 8 + (lane & 7);
+  const int iy = (int)(blockIdx.x / (unsigned)tiles_x) * 8 + (lane >> 3);
+  if (ix >= a.width || iy >= a.height) return;
+  const float sx = ((float)ix - a.cx) / a.fx, sy = ((float)iy - a.cy) / a.fy;
+  const float3 d = normalize(add(add(v3(a.cam_dir), scale(v3(a.cam_h), sx)), scale(v3(a.cam_v), sy)));
+  const Hit h = trace(a.nodes, a.tris, v3(a.cam_pos), d, stack_lds + lane);
+  float3 c = v3(0.0f, 0.0f, 0.0f);
+  if (h.prim == INT_MAX) {
+    if (a.env_faces) c = env_radiance(a.env_faces, a.env_n, d);
+  } else {
+    const float *cc = a.corner_color + 9 * (size_t)h.prim;
+    const float w = 1.0f - h.u - h.v;
+    c = v3(w * cc[0] + h.u * cc[3] + h.v * cc[6], w * cc[1] + h.u * cc[4] + h.v * cc[7],
+           w * cc[2] + h.u * cc[5] + h.v * cc[8]);
+  }
+  reinterpret_cast<float4 *>(a.color)[(size_t)iy * a.width + ix] = make_float4(c.x, c.y, c.z, 1.0f);
+}
+
 }  // namespace
 
 size_t env_table_doubles(int n) { return kTabHead + 6 * env_face_doubles(n); }
@@ -827,6 +1012,30 @@ void launch_gbuffer_cast(const CastArgs &a, hipStream_t s) {
   else
     hipLaunchKernelGGL(gbuffer_cast_kernel<false>, dim3((unsigned)tiles), dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "gbuffer_cast_kernel launch");
+}
+
+}  // namespace nwprt
+
+namespace nwprt {
+
+void launch_sh_render(const ShArgs &a, hipStream_t s) {
+  require(a.nodes && a.tris && a.indices && a.vertices && a.normals && a.materials &&
+              a.transfer && a.light_sh && a.light_response && a.corner_color && a.color,
+          NWPRT_ERR_INVALID_ARGUMENT, "sh_render: null buffer");
+  require(a.degree >= 0 && a.degree <= kShMaxDegree && a.samples >= 1 && a.width > 0 &&
+              a.height > 0 && a.n_tris > 0 && a.n_vertices > 0,
+          NWPRT_ERR_INVALID_ARGUMENT, "sh_render: bad degree, sample count or size");
+  const int nc = (a.degree + 1) * (a.degree + 1);
+  const long long lr_threads = a.n_vertices * nc;
+  hipLaunchKernelGGL(sh_light_response_kernel, dim3((unsigned)((lr_threads + 255) / 256)),
+                     dim3(256), 0, s, a);
+  hip_check(hipGetLastError(), "sh_light_response_kernel launch");
+  hipLaunchKernelGGL(sh_corner_kernel, dim3((unsigned)((a.n_tris * 3 + 63) / 64)), dim3(64), 0, s,
+                     a);
+  hip_check(hipGetLastError(), "sh_corner_kernel launch");
+  const long long tiles = (long long)((a.width + 7) / 8) * ((a.height + 7) / 8);
+  hipLaunchKernelGGL(sh_pixel_kernel, dim3((unsigned)tiles), dim3(64), 0, s, a);
+  hip_check(hipGetLastError(), "sh_pixel_kernel launch");
 }
 
 }  // namespace nwprt

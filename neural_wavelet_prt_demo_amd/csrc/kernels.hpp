@@ -208,6 +208,33 @@ size_t env_table_doubles(int n);
 void launch_env_tables(const float *faces, int n, double *tab, hipStream_t s);
 void launch_gbuffer_cast(const CastArgs &a, hipStream_t s);
 
+// SH-PRT render (gbuffer_kernels.hip): per-vertex light response, per
+// triangle-corner colour, per-pixel interpolation.
+constexpr int kShMaxDegree = 8;
+struct ShArgs {
+  const BvhNode *nodes;
+  const BvhTri *tris;
+  const int32_t *indices;       // [n_tri][3]
+  const float *vertices;        // [n_vertices][3]
+  const float *normals;         // [n_vertices][3]
+  const int32_t *tri_material;  // or null
+  const MaterialRec *materials;
+  int64_t n_tris, n_vertices;
+  const float *transfer;        // [n_vertices][nc][nc][3], nc = (degree + 1)^2
+  const float *light_sh;        // [nc][3]
+  int degree, samples;
+  uint32_t seed;
+  float *light_response;        // scratch [n_vertices][nc][3]
+  float *corner_color;          // scratch [n_tri][3][3]
+  float cam_pos[3], cam_dir[3], cam_h[3], cam_v[3];
+  float cx, cy, fx, fy;
+  int width, height;
+  const float *env_faces;       // view environment or null
+  int env_n;
+  float *color;                 // float4 [height][width]
+};
+void launch_sh_render(const ShArgs &a, hipStream_t s);
+
 // Persistent batch form: work item = the 64 pixels [base, base + 64) of job
 // `job` (base < the job's pixel count).
 struct BatchItem {

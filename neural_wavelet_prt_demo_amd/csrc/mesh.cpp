@@ -1,6 +1,6 @@
-// Triangle meshes and the G-buffer producer's host side: BVH construction,
-// upload, camera set-up and the nwprt_mesh_* / nwprt_gbuffer_cast entry
-// points.  The reference builds an OptiX acceleration structure and launches
+// Triangle meshes and the host side of the G-buffer producer and the SH-PRT
+// render: BVH construction, upload, camera set-up and the nwprt_mesh_* /
+// nwprt_gbuffer_cast / nwprt_sh_render entry points.  The reference builds an OptiX acceleration structure and launches
 // its raygen program (OptixBackend::render, Renderer/OptixBackend.cpp:821-839);
 // here the BVH is built on the host and traversed by gbuffer_kernels.hip.
 //
@@ -225,6 +225,48 @@ bool finite3(const float *p) {
   return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]);
 }
 
+// RenderEngine::setCamera (Renderer/RenderEngine.cpp:246-267), in double:
+// direction, horizontal = aspect x unit(direction x up), vertical =
+// unit(horizontal x direction), principal point at the image centre and a
+// 0.8576 rad field of view.
+void camera_basis(const nwprt_camera &c, int width, int height, float *pos, float *dir_out,
+                  float *h_out, float *v_out, float &cx, float &cy, float &fx, float &fy) {
+  double dir[3], up[3], h[3], v[3];
+  for (int a = 0; a < 3; ++a) {
+    dir[a] = (double)c.at[a] - c.from[a];
+    up[a] = c.up[a];
+  }
+  auto norm = [](double *x) {
+    const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    require(l > 0 && std::isfinite(l), NWPRT_ERR_INVALID_ARGUMENT,
+            "degenerate camera (from == at, or up parallel to the view)");
+    for (int a = 0; a < 3; ++a) x[a] /= l;
+  };
+  auto cross = [](const double *x, const double *y, double *z) {
+    z[0] = x[1] * y[2] - x[2] * y[1];
+    z[1] = x[2] * y[0] - x[0] * y[2];
+    z[2] = x[0] * y[1] - x[1] * y[0];
+  };
+  norm(dir);
+  cross(dir, up, h);
+  norm(h);
+  const double aspect = (double)width / (double)height;
+  for (int a = 0; a < 3; ++a) h[a] *= aspect;
+  cross(h, dir, v);
+  norm(v);
+  for (int k = 0; k < 3; ++k) {
+    pos[k] = c.from[k];
+    dir_out[k] = (float)dir[k];
+    h_out[k] = (float)h[k];
+    v_out[k] = (float)v[k];
+  }
+  const double tan_half = std::tan(0.5 * 0.8575560548920328);
+  cx = (float)(width / 2.0);
+  cy = (float)(height / 2.0);
+  fx = (float)(0.5 * width / tan_half);
+  fy = (float)(0.5 * height / tan_half);
+}
+
 }  // namespace
 
 extern "C" {
@@ -297,6 +339,7 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     mesh->nodes.upload(bld.nodes.data(), bld.nodes.size() * sizeof(BvhNode));
     mesh->tris.upload(tris.data(), tris.size() * sizeof(BvhTri));
     mesh->indices.upload(indices, (size_t)n_triangles * 3 * sizeof(int32_t));
+    mesh->vertices.upload(vertices, (size_t)n_vertices * 3 * sizeof(float));
     if (normals) mesh->normals.upload(normals, (size_t)n_vertices * 3 * sizeof(float));
     if (material_ids)
       mesh->tri_material.upload(material_ids, (size_t)n_triangles * sizeof(int32_t));
@@ -304,12 +347,22 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     std::memcpy(mesh->lower, all.lo, sizeof all.lo);
     std::memcpy(mesh->upper, all.hi, sizeof all.hi);
     mesh->n_tris = n_triangles;
+    mesh->n_vertices = n_vertices;
     mesh->n_nodes = (int64_t)bld.nodes.size();
     mesh->depth = bld.max_depth;
     mesh->has_normals = normals != nullptr;
     mesh->has_tri_material = material_ids != nullptr;
-    for (const MaterialRec &mr : mats)
-      if (mr.type == 1) mesh->direct_ok = false;  // DisneyBSDF
+    // Capabilities of the materials the triangles use.
+    std::vector<char> used(mats.size(), 0);
+    if (material_ids)
+      for (int64_t i = 0; i < n_triangles; ++i) used[(size_t)material_ids[i]] = 1;
+    else
+      used[0] = 1;
+    for (size_t i = 0; i < mats.size(); ++i) {
+      if (!used[i]) continue;
+      if (mats[i].type == 1) mesh->direct_ok = false;  // DisneyBSDF
+      if (mats[i].type == 1 || mats[i].type == NWPRT_MATERIAL_MIRROR) mesh->sh_ok = false;
+    }
     const int id = ctx->next_id++;
     ctx->meshes[id] = std::move(mesh);
     *mesh_id = id;
@@ -355,49 +408,15 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
     require(finite3(camera->from) && finite3(camera->at) && finite3(camera->up),
             NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: non-finite camera");
 
-    // RenderEngine::setCamera (Renderer/RenderEngine.cpp:246-267), in double.
-    double dir[3], up[3], h[3], v[3];
-    for (int a = 0; a < 3; ++a) {
-      dir[a] = (double)camera->at[a] - camera->from[a];
-      up[a] = camera->up[a];
-    }
-    auto norm = [](double *x) {
-      const double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-      require(l > 0 && std::isfinite(l), NWPRT_ERR_INVALID_ARGUMENT,
-              "gbuffer_cast: degenerate camera (from == at, or up parallel to the view)");
-      for (int a = 0; a < 3; ++a) x[a] /= l;
-    };
-    auto cross = [](const double *x, const double *y, double *z) {
-      z[0] = x[1] * y[2] - x[2] * y[1];
-      z[1] = x[2] * y[0] - x[0] * y[2];
-      z[2] = x[0] * y[1] - x[1] * y[0];
-    };
-    norm(dir);
-    cross(dir, up, h);
-    norm(h);
-    const double aspect = (double)width / (double)height;
-    for (int a = 0; a < 3; ++a) h[a] *= aspect;
-    cross(h, dir, v);
-    norm(v);
-
     CastArgs a{};
+    camera_basis(*camera, width, height, a.cam_pos, a.cam_dir, a.cam_h, a.cam_v, a.cx, a.cy, a.fx,
+                 a.fy);
     a.nodes = m.nodes.as<BvhNode>();
     a.tris = m.tris.as<BvhTri>();
     a.indices = m.indices.as<int32_t>();
     a.normals = m.has_normals ? m.normals.as<float>() : nullptr;
     a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
     a.materials = m.materials.as<MaterialRec>();
-    for (int k = 0; k < 3; ++k) {
-      a.cam_pos[k] = camera->from[k];
-      a.cam_dir[k] = (float)dir[k];
-      a.cam_h[k] = (float)h[k];
-      a.cam_v[k] = (float)v[k];
-    }
-    const double tan_half = std::tan(0.5 * 0.8575560548920328);
-    a.cx = (float)(width / 2.0);
-    a.cy = (float)(height / 2.0);
-    a.fx = (float)(0.5 * width / tan_half);
-    a.fy = (float)(0.5 * height / tan_half);
     const float *lo = params->bounds ? params->bounds : m.lower;
     const float *hi = params->bounds ? params->bounds + 3 : m.upper;
     for (int k = 0; k < 3; ++k) {
@@ -439,6 +458,62 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
       check_aligned(p, 16, "G-buffer float4 field");
     check_aligned(gbuffer->roughness, 4, "G-buffer roughness");
     launch_gbuffer_cast(a, pick(ctx, stream));
+  });
+}
+
+nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
+                             int32_t width, int32_t height, const float *transfer,
+                             const float *light_sh, int32_t degree, int32_t brdf_samples,
+                             uint32_t seed, const float *view_env_faces, int32_t view_env_n,
+                             float *color, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    const Mesh &m = get_mesh(ctx, mesh_id);
+    require(camera && transfer && light_sh && color, NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_render: null argument");
+    require(width > 0 && height > 0 && (int64_t)width * height <= (1ll << 31),
+            NWPRT_ERR_INVALID_ARGUMENT, "sh_render: bad image size");
+    require(degree >= 0 && degree <= kShMaxDegree, NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_render: SH degree must be in [0, 8]");
+    require(brdf_samples >= 1 && brdf_samples <= (1 << 20), NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_render: brdf_samples must be in [1, 2^20]");
+    require(!view_env_faces || (view_env_n >= 1 && view_env_n <= NWPRT_MAX_FACE_SIZE),
+            NWPRT_ERR_INVALID_ARGUMENT, "sh_render: bad view environment size");
+    require(m.has_normals, NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_render: the mesh needs vertex normals");
+    require(m.sh_ok, NWPRT_ERR_UNSUPPORTED,
+            "sh_render: materials must be roughPlastic, cookTorrance or simple_ggx");
+    require(finite3(camera->from) && finite3(camera->at) && finite3(camera->up),
+            NWPRT_ERR_INVALID_ARGUMENT, "sh_render: non-finite camera");
+    check_aligned(color, 16, "sh_render colour");
+    ShArgs a{};
+    camera_basis(*camera, width, height, a.cam_pos, a.cam_dir, a.cam_h, a.cam_v, a.cx, a.cy, a.fx,
+                 a.fy);
+    const int nc = (degree + 1) * (degree + 1);
+    ctx->sh_response.ensure((size_t)m.n_vertices * nc * 3 * sizeof(float));
+    ctx->sh_corners.ensure((size_t)m.n_tris * 9 * sizeof(float));
+    a.nodes = m.nodes.as<BvhNode>();
+    a.tris = m.tris.as<BvhTri>();
+    a.indices = m.indices.as<int32_t>();
+    a.vertices = m.vertices.as<float>();
+    a.normals = m.normals.as<float>();
+    a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
+    a.materials = m.materials.as<MaterialRec>();
+    a.n_tris = m.n_tris;
+    a.n_vertices = m.n_vertices;
+    a.transfer = transfer;
+    a.light_sh = light_sh;
+    a.degree = degree;
+    a.samples = brdf_samples;
+    a.seed = seed;
+    a.light_response = ctx->sh_response.as<float>();
+    a.corner_color = ctx->sh_corners.as<float>();
+    a.width = width;
+    a.height = height;
+    a.env_faces = view_env_faces;
+    a.env_n = view_env_n;
+    a.color = color;
+    launch_sh_render(a, pick(ctx, stream));
   });
 }
 

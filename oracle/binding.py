@@ -90,6 +90,12 @@ def lib():
         l.orc_ct_pdf.restype = c_double
         l.orc_env_pdf.argtypes = [c_void_p, c_int, c_void_p]
         l.orc_env_pdf.restype = c_double
+        l.orc_sh_eval.argtypes = [c_void_p, c_void_p]
+        l.orc_sh_eval.restype = None
+        l.orc_sh_render.argtypes = [POINTER(OrcMesh), c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                    c_int, c_int, c_uint32, c_void_p, c_int, c_void_p, c_void_p,
+                                    c_void_p]
+        l.orc_sh_render.restype = None
         l.orc_rng_uniform.argtypes = [c_uint32, c_uint32, c_uint32]
         l.orc_rng_uniform.restype = c_float
         l.orc_tonemap.restype = c_uint32
@@ -331,3 +337,36 @@ def ct_pdf(mat, ns, ng, l, v) -> float:
 def env_pdf(tables: Dict[str, np.ndarray], n: int, d) -> float:
     dd = _d3(d)
     return lib().orc_env_pdf(_p(tables["raw"]), n, _p(dd))
+
+
+def sh_eval(w) -> np.ndarray:
+    """pbrt's real SH basis (degree 8, 81 values) at unit direction w."""
+    ww = _d3(w)
+    out = np.empty(81, np.float64)
+    lib().orc_sh_eval(_p(ww), _p(out))
+    return out
+
+
+def sh_render(vertices, indices, normals, materials, camera, width: int, height: int,
+              transfer, light_sh, degree: int, *, material_ids=None, samples: int = 16,
+              seed: int = 0, env_faces=None) -> Dict[str, np.ndarray]:
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+    nr = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+    cam = np.ascontiguousarray(camera, np.float32).reshape(9)
+    t = np.ascontiguousarray(transfer, np.float32)
+    ls = np.ascontiguousarray(light_sh, np.float32)
+    env_n = 0
+    if env_faces is not None:
+        env_faces = np.ascontiguousarray(env_faces, np.float32)
+        env_n = env_faces.shape[1]
+    mesh = OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats))
+    P = width * height
+    out = {"color": np.empty((P, 4), np.float64), "prim": np.empty(P, np.int32),
+           "margin": np.empty(P, np.float64)}
+    lib().orc_sh_render(ctypes.byref(mesh), _p(cam), width, height, _p(t), _p(ls), degree,
+                        samples, seed & 0xFFFFFFFF, _p(env_faces), env_n, _p(out["color"]),
+                        _p(out["prim"]), _p(out["margin"]))
+    return out

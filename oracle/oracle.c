@@ -1278,3 +1278,165 @@ double orc_ct_pdf(const float *mat, const double *ns, const double *ng, const do
   return g_bsdf_pdf(&s, l, v);
 }
 double orc_env_pdf(const double *tab, int n, const double *d) { return g_env_pdf(tab, n, d); }
+
+/* ---- SH-PRT render -------------------------------------------------------
+ * RenderEngine::shRender (Renderer/RenderEngine.cpp:459-486) and
+ * __closesthit__SHradiance (Renderer/device_functions/devicePrograms.cu:
+ * 312-501) in FP64, replaying the product's random stream: vertex colours
+ * per triangle corner (stream keyed by seed, vertex and material), pixels
+ * interpolate their first hit's corner colours. */
+
+/* pbrt's real SH basis of degree 8 (Renderer/device_functions/
+ * shaderUtils.h:230-337) at unit w: 81 values, index l (l + 1) + m. */
+void orc_sh_eval(const double *w, double *y) {
+  const int L = 8;
+#define IDX(l, m) ((l) * (l) + (l) + (m))
+  const double x = w[2];
+  for (int k = 0; k < 81; ++k) y[k] = 0.0;
+  y[IDX(0, 0)] = 1.0;
+  y[IDX(1, 0)] = x;
+  for (int l = 2; l <= L; ++l)
+    y[IDX(l, 0)] = ((2 * l - 1) * x * y[IDX(l - 1, 0)] - (l - 1) * y[IDX(l - 2, 0)]) / l;
+  double neg = -1.0, dfact = 1.0;
+  const double xroot = sqrt(fmax(0.0, 1.0 - x * x));
+  double xpow = xroot;
+  for (int l = 1; l <= L; ++l) {
+    y[IDX(l, l)] = neg * dfact * xpow;
+    neg = -neg;
+    dfact *= 2 * l + 1;
+    xpow *= xroot;
+  }
+  for (int l = 2; l <= L; ++l) y[IDX(l, l - 1)] = x * (2 * l - 1) * y[IDX(l - 1, l - 1)];
+  for (int l = 3; l <= L; ++l)
+    for (int m = 1; m <= l - 2; ++m)
+      y[IDX(l, m)] = ((2 * (l - 1) + 1) * x * y[IDX(l - 1, m)] - (l - 1 + m) * y[IDX(l - 2, m)]) /
+                     (l - m);
+  double sins[9], coss[9];
+  const double xy = sqrt(fmax(0.0, 1.0 - x * x));
+  if (xy == 0.0) {
+    for (int i = 0; i <= L; ++i) {
+      sins[i] = 0.0;
+      coss[i] = 1.0;
+    }
+  } else {
+    const double sp = w[1] / xy, cp = w[0] / xy;
+    double si = 0.0, ci = 1.0;
+    for (int i = 0; i <= L; ++i) {
+      sins[i] = si;
+      coss[i] = ci;
+      const double os = si;
+      si = si * cp + ci * sp;
+      ci = ci * cp - os * sp;
+    }
+  }
+  for (int l = 0; l <= L; ++l) {
+    for (int m = -l; m < 0; ++m) {
+      double inv = 1.0;
+      for (int t = l + m + 1; t <= l - m; ++t) inv /= t;
+      y[IDX(l, m)] = sqrt(2.0) * sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * inv) * y[IDX(l, -m)] *
+                     sins[-m];
+    }
+    y[IDX(l, 0)] *= sqrt((2.0 * l + 1.0) / (4.0 * M_PI));
+    for (int m = 1; m <= l; ++m) {
+      double inv = 1.0;
+      for (int t = l - m + 1; t <= l + m; ++t) inv /= t;
+      y[IDX(l, m)] *= sqrt(2.0) * sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * inv) * coss[m];
+    }
+  }
+#undef IDX
+}
+
+void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const float *transfer,
+                   const float *light_sh, int degree, int samples, uint32_t seed,
+                   const float *env_faces, int env_n, double *color, int32_t *prim,
+                   double *margin) {
+  const int nc = (degree + 1) * (degree + 1);
+  double *corner = (double *)calloc((size_t)m->n_tris * 9, sizeof(double));
+  const double cam[3] = {camera[0], camera[1], camera[2]};
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t c = 0; c < m->n_tris * 3; ++c) {
+    const int64_t tri = c / 3;
+    const int vi = m->indices[c];
+    const int mid = m->material_ids ? m->material_ids[tri] : 0;
+    const float *mat = m->materials + 8 * (size_t)mid;
+    g_surface s;
+    double nrm[3], view[3];
+    for (int a = 0; a < 3; ++a) {
+      nrm[a] = m->normals[3 * (size_t)vi + a];
+      view[a] = cam[a] - (double)m->vertices[3 * (size_t)vi + a];
+    }
+    g_norm(nrm);
+    g_norm(view);
+    g_surface_of(mat, nrm, nrm, &s);
+    /* light response R_k = sum_j T[v][k][j] L[j] */
+    double lr[81][3];
+    for (int k = 0; k < nc; ++k)
+      for (int ch = 0; ch < 3; ++ch) {
+        double acc = 0.0;
+        for (int j = 0; j < nc; ++j)
+          acc += (double)transfer[(((size_t)vi * nc + k) * nc + j) * 3 + ch] *
+                 (double)light_sh[3 * j + ch];
+        lr[k][ch] = acc;
+      }
+    const uint32_t sd = seed ^ (0x9E3779B9u * (uint32_t)(mid + 1));
+    double acc[3] = {0, 0, 0}, y[81], l[3], f[3];
+    int valid = 0;
+    for (int k = 0; k < samples; ++k) {
+      const uint32_t q = (uint32_t)k * 4u;
+      const int ok = s.type == 0
+                         ? g_rp_sample(&s, view, orc_rng_uniform(sd, (uint32_t)vi, q),
+                                       orc_rng_uniform(sd, (uint32_t)vi, q + 1), l)
+                         : g_ct_sample(&s, view, orc_rng_uniform(sd, (uint32_t)vi, q),
+                                       orc_rng_uniform(sd, (uint32_t)vi, q + 1),
+                                       orc_rng_uniform(sd, (uint32_t)vi, q + 2), l);
+      if (!ok) continue;
+      const double pdf = g_bsdf_pdf(&s, l, view);
+      if (pdf < 1e-6) continue;
+      g_bsdf_eval(&s, l, view, f);
+      if (isnan(f[0]) || isnan(f[1]) || isnan(f[2])) continue;
+      orc_sh_eval(l, y);
+      for (int ch = 0; ch < 3; ++ch) {
+        double d = 0.0;
+        for (int j = 0; j < nc; ++j) d += y[j] * lr[j][ch];
+        acc[ch] += f[ch] / pdf * d;
+      }
+      ++valid;
+    }
+    for (int ch = 0; ch < 3; ++ch) corner[9 * tri + 3 * (c % 3) + ch] = valid ? fabs(acc[ch] / valid) : 0.0;
+  }
+  /* primary rays, as orc_gbuffer_cast casts them */
+  double dir[3], up[3], hz[3], vt[3];
+  for (int a = 0; a < 3; ++a) {
+    dir[a] = (double)camera[3 + a] - camera[a];
+    up[a] = camera[6 + a];
+  }
+  g_norm(dir);
+  g_cross(dir, up, hz);
+  g_norm(hz);
+  for (int a = 0; a < 3; ++a) hz[a] *= (double)w / (double)h;
+  g_cross(hz, dir, vt);
+  g_norm(vt);
+  const double th = tan(0.5 * 0.8575560548920328);
+  const double cx = w / 2.0, cy = h / 2.0, fx = 0.5 * w / th, fy = 0.5 * h / th;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t px = 0; px < (int64_t)w * h; ++px) {
+    const int ix = (int)(px % w), iy = (int)(px / w);
+    const double sx = (ix - cx) / fx, sy = (iy - cy) / fy;
+    double d[3];
+    for (int a = 0; a < 3; ++a) d[a] = dir[a] + sx * hz[a] + sy * vt[a];
+    g_norm(d);
+    double t, u, v, mg = 1.0, c[3] = {0, 0, 0};
+    const int hit = g_closest(m, cam, d, &t, &u, &v, &mg);
+    if (hit < 0) {
+      if (env_faces) g_env(env_faces, env_n, d, c);
+    } else {
+      const double *cc = corner + 9 * (size_t)hit;
+      for (int ch = 0; ch < 3; ++ch) c[ch] = (1.0 - u - v) * cc[ch] + u * cc[3 + ch] + v * cc[6 + ch];
+    }
+    for (int ch = 0; ch < 3; ++ch) color[4 * px + ch] = c[ch];
+    color[4 * px + 3] = 1.0;
+    prim[px] = hit;
+    margin[px] = mg;
+  }
+  free(corner);
+}

@@ -43,6 +43,11 @@
  *                        an independent quadrature of its expectation
  *                        (diffuse cookTorrance, no occluders); closed forms
  *                        of the BSDF (white furnace bound, reciprocity)
+ *   orc_sh_eval          orthonormality by quadrature; closed forms of
+ *                        bands 0-2 (Condon-Shortley phase)
+ *   orc_sh_render        a single-entry transfer matrix (orientation) with
+ *                        a diffuse BSDF against quadrature of the BSDF's
+ *                        albedo
  * The tiny-cuda-nn conventions behind the hash grid and the MLP (level
  * rounding, layer layout, padding values) cannot be pinned against the
  * reference: that library is an empty submodule there ("parity unpinned"
@@ -158,6 +163,16 @@ double orc_ct_pdf(const float *mat, const double *ns, const double *ng, const do
 double orc_env_pdf(const double *tab, int n, const double *d);
 /* The product's counter-based uniform in [0, 1). */
 float orc_rng_uniform(uint32_t seed, uint32_t pixel, uint32_t counter);
+
+/* pbrt's real SH basis of degree 8 at unit w: 81 values, index l (l+1)+m. */
+void orc_sh_eval(const double *w, double *y);
+/* SH-PRT render (FP64, replaying the product's stream).  The mesh needs
+ * normals.  transfer [n_vertices][nc][nc][3], light_sh [nc][3],
+ * nc = (degree+1)^2; color double [P][4], prim int32 [P], margin [P]. */
+void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const float *transfer,
+                   const float *light_sh, int degree, int samples, uint32_t seed,
+                   const float *env_faces, int env_n, double *color, int32_t *prim,
+                   double *margin);
 
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
