@@ -62,7 +62,7 @@ def parse():
     ap.add_argument("--no-denoise", action="store_true",
                     help="skip the denoiser leg")
     ap.add_argument("--no-gbuffer", action="store_true",
-                    help="skip the G-buffer producer leg")
+                    help="skip the G-buffer producer and SH-PRT legs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB: the "
                          "known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -297,6 +297,60 @@ def bench_gbuffer(nw, ctx, W, H, stream, dfaces, n_face):
     return out
 
 
+def bench_sh(nw, ctx, W, H, stream, dfaces, n_face, samples=64):
+    """SH-PRT render leg (RenderEngine::shRender baseline): degree 8 (81 x 81
+    transfer matrices, the reference's tested degree) over a 256 x 128 UV
+    sphere on a ground quad (65,538 triangles, 33,028 vertices: 2.6 GB of
+    transfer matrices), 1080p, `samples` BSDF samples per triangle corner."""
+    from neural_wavelet_prt_demo_amd import DeviceBuffer, Event
+    nu, nv = 256, 128
+    th = np.linspace(0, np.pi, nv + 1)
+    ph = np.linspace(0, 2 * np.pi, nu, endpoint=False)
+    T_, P_ = np.meshgrid(th, ph, indexing="ij")
+    sn = np.stack([np.sin(T_) * np.cos(P_), np.cos(T_), np.sin(T_) * np.sin(P_)], -1).reshape(-1, 3)
+    sv = np.array([0.0, 0.2, -2.0]) + 1.6 * sn
+    a = (np.arange(nv)[:, None] * nu + np.arange(nu)[None, :]).reshape(-1)
+    b = (np.arange(nv)[:, None] * nu + (np.arange(nu)[None, :] + 1) % nu).reshape(-1)
+    si = np.concatenate([np.stack([a, a + nu, b], 1), np.stack([b, a + nu, b + nu], 1)])
+    g = np.array([[-8, -1.4, -9], [8, -1.4, -9], [8, -1.4, 5], [-8, -1.4, 5]])
+    verts = np.concatenate([sv, g]).astype(np.float32)
+    norms = np.concatenate([sn, np.tile([0.0, 1.0, 0.0], (4, 1))]).astype(np.float32)
+    base = len(sv)
+    idx = np.concatenate([si, np.array([[0, 2, 1], [0, 3, 2]]) + base]).astype(np.int32)
+    mids = np.concatenate([np.zeros(len(si)), np.ones(2)]).astype(np.int32)
+    mats = [[0.8, 0.3, 0.2, 0.04, 0.04, 0.04, 0.3, 2], [0.5, 0.5, 0.5, 0.04, 0.04, 0.04, 0.8, 0]]
+    mesh = ctx.mesh_create(verts, idx, mats, normals=norms, material_ids=mids)
+    nc = 81
+    T = np.zeros((len(verts), nc, nc, 3), np.float32)
+    T[:, np.arange(nc), np.arange(nc), :] = 1.0  # unshadowed transfer: R = L
+    dT = DeviceBuffer.from_array(T)
+    del T
+    rng = np.random.default_rng(5)
+    L = rng.normal(0, 0.2, (nc, 3)).astype(np.float32)
+    L[0] = [2.0, 1.8, 1.6]
+    dL = DeviceBuffer.from_array(L)
+    out = DeviceBuffer(W * H * 16)
+    cam = [[0.0, 2.5, 7.5], [0.0, -0.8, -2.0], [0.0, 1.0, 0.0]]
+    run = lambda: ctx.sh_render(mesh, cam, W, H, dT, dL, 8, out, brdf_samples=samples, seed=1,
+                                env_faces=dfaces, env_n=n_face, stream=stream)
+    run()
+    ctx.sync()
+    ev = [Event() for _ in range(6)]
+    ev[0].record(stream)
+    for i in range(5):
+        run()
+        ev[i + 1].record(stream)
+    ctx.sync()
+    ms = float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(5)]))
+    res = {"workload": f"SH-PRT degree 8, {W}x{H}, {len(idx)} triangles, {len(verts)} vertices, "
+                       f"{samples} BSDF samples per corner",
+           "ms": ms, "transfer_bytes": len(verts) * nc * nc * 12,
+           "transfer_read_gbs": len(verts) * nc * nc * 12 / (ms / 1e3) / 1e9}
+    ctx.mesh_destroy(mesh)
+    del dT
+    return res
+
+
 def bench_pipelined(nw, ctx, scene, dfaces, frame, prec, k, n, steps, warmup):
     """Frames through the two-slot pipeline of the reference's interactive
     loop (Renderer/OutputVisualizer.cpp:71-123): per frame an envmap update,
@@ -480,9 +534,10 @@ def main():
         del dn_out
 
     # --- G-buffer producer (the step before the path) ----------------------
-    gbuffer_leg = None
+    gbuffer_leg = sh_leg = None
     if not args.no_gbuffer:
         gbuffer_leg = bench_gbuffer(nw, ctx, W, H, stream, dfaces, N)
+        sh_leg = bench_sh(nw, ctx, W, H, stream, dfaces, N)
 
     # --- dominant kernel alone: the per-pair pass (p2) with HIP events -----
     kev = [Event() for _ in range(6)]
@@ -572,6 +627,8 @@ def main():
         result["denoise_pass"] = denoise_leg
     if gbuffer_leg is not None:
         result["gbuffer_cast"] = gbuffer_leg
+    if sh_leg is not None:
+        result["sh_prt_render"] = sh_leg
     if ceiling is not None and args.precision in ("bf16", "fp16"):
         # The published peak assumes the nominal clock; with real (toggling)
         # operands a pure MFMA loop on this GPU runs power-limited below it.

@@ -882,24 +882,39 @@ __device__ inline void sh_eval(float3 w, float (&y)[kShCoeffs]) {
   }
 }
 
-// R[v][k] = sum_j T[v][k][j] (x) L[j], one thread per (vertex, k).
+// R[v][k] = sum_j T[v][k][j] (x) L[j].  One wavefront per vertex walks its
+// nc rows; a row's nc x 3 floats are read by the 64 lanes together
+// (coalesced) and multiplied by L flattened the same way, then reduced
+// across the wave per channel.  Four vertices per 256-thread workgroup.
+__device__ inline float wave_sum(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
 __global__ __launch_bounds__(256) void sh_light_response_kernel(ShArgs a) {
-  const int nc = (a.degree + 1) * (a.degree + 1);
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= a.n_vertices * nc) return;
-  const long long v = i / nc;
-  const int k = (int)(i % nc);
-  const float *t = a.transfer + ((size_t)v * nc + k) * nc * 3;
-  float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-  for (int j = 0; j < nc; ++j) {
-    r0 = fmaf(t[3 * j], a.light_sh[3 * j], r0);
-    r1 = fmaf(t[3 * j + 1], a.light_sh[3 * j + 1], r1);
-    r2 = fmaf(t[3 * j + 2], a.light_sh[3 * j + 2], r2);
+  const int nc = (a.degree + 1) * (a.degree + 1), row = nc * 3;
+  const int lane = threadIdx.x & 63;
+  const long long v = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= a.n_vertices) return;  // whole waves leave together; no barriers below
+  const float *t = a.transfer + (size_t)v * nc * row;
+  float *o = a.light_response + (size_t)v * row;
+  for (int k = 0; k < nc; ++k) {
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int f = lane; f < row; f += 64) {
+      const float x = t[(size_t)k * row + f] * a.light_sh[f];
+      const int c = f % 3;
+      acc[0] += c == 0 ? x : 0.0f;
+      acc[1] += c == 1 ? x : 0.0f;
+      acc[2] += c == 2 ? x : 0.0f;
+    }
+    const float r0 = wave_sum(acc[0]), r1 = wave_sum(acc[1]), r2 = wave_sum(acc[2]);
+    if (lane == 0) {
+      o[3 * k] = r0;
+      o[3 * k + 1] = r1;
+      o[3 * k + 2] = r2;
+    }
   }
-  float *o = a.light_response + ((size_t)v * nc + k) * 3;
-  o[0] = r0;
-  o[1] = r1;
-  o[2] = r2;
 }
 
 // Colour of each triangle corner: the vertex's BSDF SH projection (normal
@@ -1025,9 +1040,7 @@ void launch_sh_render(const ShArgs &a, hipStream_t s) {
   require(a.degree >= 0 && a.degree <= kShMaxDegree && a.samples >= 1 && a.width > 0 &&
               a.height > 0 && a.n_tris > 0 && a.n_vertices > 0,
           NWPRT_ERR_INVALID_ARGUMENT, "sh_render: bad degree, sample count or size");
-  const int nc = (a.degree + 1) * (a.degree + 1);
-  const long long lr_threads = a.n_vertices * nc;
-  hipLaunchKernelGGL(sh_light_response_kernel, dim3((unsigned)((lr_threads + 255) / 256)),
+  hipLaunchKernelGGL(sh_light_response_kernel, dim3((unsigned)((a.n_vertices + 3) / 4)),
                      dim3(256), 0, s, a);
   hip_check(hipGetLastError(), "sh_light_response_kernel launch");
   hipLaunchKernelGGL(sh_corner_kernel, dim3((unsigned)((a.n_tris * 3 + 63) / 64)), dim3(64), 0, s,
