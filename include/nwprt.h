@@ -500,7 +500,9 @@ typedef struct {
  * default 1.49 / 1.000277), cookTorrance (2), mirrors (3) or simple_ggx
  * (4); DisneyBSDF (1) gives NWPRT_ERR_UNSUPPORTED.  Random numbers come from a counter-based generator keyed by
  * (seed, pixel, sample), not the reference's per-pixel LCG: estimates agree
- * with the reference's in expectation, not sample by sample. */
+ * with the reference's in expectation, not sample by sample.  The sampling
+ * tables live in a work buffer of the context: casts with a lighting map
+ * issued on different streams must not overlap. */
 nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
                                 int32_t width, int32_t height,
                                 const nwprt_cast_params *params,
@@ -523,7 +525,9 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
  * by (seed, vertex, material), not once per pixel: the same expectation.
  * transfer: device [n_vertices][nc][nc][3]; light_sh: device [nc][3];
  * color: device float4 [height][width].  The mesh needs vertex normals;
- * materials must be roughPlastic, cookTorrance or simple_ggx. */
+ * materials must be roughPlastic, cookTorrance or simple_ggx.  The light
+ * responses and corner colours live in work buffers of the context: calls
+ * issued on different streams must not overlap. */
 nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
                              int32_t width, int32_t height, const float *transfer,
                              const float *light_sh, int32_t degree, int32_t brdf_samples,
