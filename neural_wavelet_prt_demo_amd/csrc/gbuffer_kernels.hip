@@ -712,8 +712,10 @@ __global__ void env_face_table_kernel(double *__restrict__ tab) {
 // One 8 x 8 pixel tile per 64-lane workgroup (coherent primary rays).
 // DIRECT: the direct-light estimate is computed and stored (a separate
 // instance, so the plain cast keeps its lower register count).
+// At most 128 VGPRs (4 waves per SIMD, the LDS stack allows 5).
 template <bool DIRECT>
-__global__ __launch_bounds__(64) void gbuffer_cast_kernel(CastArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gbuffer_cast_kernel(
+    CastArgs a) {
   __shared__ int stack_lds[kBvhMaxDepth * 64];
   const int lane = threadIdx.x;
   const int tiles_x = (a.width + 7) >> 3;
