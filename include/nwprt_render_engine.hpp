@@ -138,8 +138,8 @@ class RenderEngine {
     check(nwprt_gbuffer_cast(ctx_, mesh_, &camera_, width_, height_, &p, &g, stream));
   }
 
-  // Zero frameID's direct-light buffer (for G-buffers from render(), which
-  // does not produce the direct term).
+  // Zero frameID's direct-light buffer (for G-buffers from render() without
+  // a lighting map, which leaves that buffer as it was).
   void clearDirectLight(int frameID) {
     hipCheck(hipMemset(frame(frameID).fbDirectDenoised, 0, (size_t)width_ * height_ * 16));
   }
