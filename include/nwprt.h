@@ -430,11 +430,12 @@ typedef struct {
   float lower[3], upper[3]; /* bounds of the vertices (Scene::bounds)       */
   int64_t triangles;
   int64_t bvh_nodes;
-  int32_t bvh_depth;        /* internal levels, <= 32                        */
+  int32_t bvh_depth;        /* internal levels, <= 24                        */
 } nwprt_mesh_info;
 
 /* Uploads a triangle mesh and builds its BVH on the host (binned SAH, at
- * most 8 triangles per leaf, at most 32 internal levels); replaces the
+ * most 8 triangles per leaf, at most 24 internal levels, so at most 2^27
+ * triangles); replaces the
  * OptiX GAS build (OptixBackend.cpp buildAccel).  vertices [n_vertices][3];
  * normals [n_vertices][3] or NULL (geometric normals); indices
  * [n_triangles][3]; material_ids [n_triangles] or NULL (all material 0);

@@ -149,7 +149,7 @@ void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s);
 
 // G-buffer producer (gbuffer_kernels.hip, BVH built in mesh.cpp).
 // The traversal stack holds at most one entry per internal level.
-constexpr int kBvhMaxDepth = 32;
+constexpr int kBvhMaxDepth = 24;
 constexpr int kBvhLeafMax = 8;
 // Two-child node, 64 bytes: both children's boxes sit in the parent so one
 // visit tests both.  Child reference c >= 0: internal node c; c < 0: leaf

@@ -87,7 +87,7 @@ def test_cast_matches_oracle(nw, orc, ctx, seed, w, h, bounce):
     got, info = _cast(nw, ctx, s, w, h, bounce, faces)
     want = _oracle(orc, s, w, h, bounce, faces)
     _compare(got, want, TOL_ENC if bounce == 1 else 1e-3)
-    assert 1 <= info["bvh_depth"] <= 32 and info["triangles"] == len(s["indices"])
+    assert 1 <= info["bvh_depth"] <= 24 and info["triangles"] == len(s["indices"])
 
 
 def test_cast_geometric_normals_and_single_material(nw, orc, ctx):
@@ -116,7 +116,7 @@ def test_cast_large_mesh_deep_bvh(nw, orc, ctx):
     got, info = _cast(nw, ctx, s, 40, 30)
     want = _oracle(orc, s, 40, 30)
     nh, _ = _compare(got, want, min_safe=0.95)  # 12K small triangles: edges everywhere
-    assert nh > 500 and info["triangles"] == len(s["indices"]) and info["bvh_depth"] <= 32
+    assert nh > 500 and info["triangles"] == len(s["indices"]) and info["bvh_depth"] <= 24
 
 
 @pytest.mark.parametrize("nt", [1, 5, 8, 9, 100])
