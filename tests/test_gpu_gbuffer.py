@@ -307,3 +307,73 @@ def test_direct_light_errors_and_skips(nw, ctx):
                              light_faces=light, **kw)
         assert e.value.status == nw.ERR_INVALID_ARGUMENT
     ctx.mesh_destroy(mesh)
+
+
+def test_full_size_properties(nw, ctx):
+    """1080p over the 1.29 M-triangle bench scene, where a brute-force oracle
+    is out of reach: size-independent properties.  Casts are bitwise
+    deterministic; hit positions lie in the bounds (plus the 0.001 offset);
+    normals and reflections are unit vectors and obey the reflection law
+    against the pinhole ray; the direct term is finite, non-negative,
+    repeatable for a seed and different for another; the SH render is
+    finite and non-negative."""
+    import bench
+    verts, norms, idx, mids, mats = bench.gbuffer_bench_mesh()
+    mesh = ctx.mesh_create(verts, idx, mats, normals=norms, material_ids=mids)
+    info = ctx.mesh_info(mesh)
+    W, H = 1920, 1080
+    cam = ([0.0, 2.5, 7.5], [0.0, -0.8, -2.0], [0.0, 1.0, 0.0])
+    faces = nw.synth_cubemap(3, 32)
+    df = nw.DeviceBuffer.from_array(faces)
+    f1, f2 = nw.DeviceFrame(W, H), nw.DeviceFrame(W, H)
+    for f, seed in ((f1, 5), (f2, 5)):
+        ctx.gbuffer_cast(mesh, cam, f, env_faces=df, env_n=32, light_faces=df, light_n=32,
+                         direct_samples=2, seed=seed)
+    ctx.sync()
+    P = W * H
+    get = lambda f, k, c=4: f.bufs[k].download(np.float32, (P, c) if c > 1 else (P,))
+    for k in ("position", "normal", "reflect", "kd", "direct"):
+        assert np.array_equal(get(f1, k), get(f2, k)), k
+    flag = f1.bufs["env_flag"].download(np.uint8, (P,)).astype(bool)
+    assert 0.3 < flag.mean() < 0.9
+    lo, hi = np.array(info["lower"]), np.array(info["upper"])
+    pos = get(f1, "position")[flag, :3]
+    slack = 0.0011 / (hi - lo)
+    assert np.all(pos >= -1e-5) and np.all(pos <= 1 + slack + 1e-5)
+    n = get(f1, "normal")[flag, :3] * 2 - 1
+    r = get(f1, "reflect")[flag, :3] * 2 - 1
+    assert np.allclose(np.linalg.norm(n, axis=1), 1, atol=1e-5)
+    assert np.allclose(np.linalg.norm(r, axis=1), 1, atol=1e-5)
+    # Reflection law: r = d - 2 (d . n) n with d the pixel's pinhole ray.
+    iy, ix = np.divmod(np.arange(P)[flag], W)
+    fwd = np.array(cam[1]) - np.array(cam[0])
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, cam[2])
+    right = right / np.linalg.norm(right) * (W / H)
+    up = np.cross(right, fwd)
+    up /= np.linalg.norm(up)
+    t = np.tan(0.5 * 0.8575560548920328)
+    sx, sy = (ix - W / 2) / (0.5 * W / t), (iy - H / 2) / (0.5 * H / t)
+    d = fwd + sx[:, None] * right + sy[:, None] * up
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    want = d - 2 * np.sum(d * n, axis=1, keepdims=True) * n
+    assert np.abs(want - r).max() < 2e-4
+    dl = get(f1, "direct")[:, :3]
+    assert np.all(np.isfinite(dl)) and np.all(dl >= 0) and dl[flag].mean() > 0
+    ctx.gbuffer_cast(mesh, cam, f2, env_faces=df, env_n=32, light_faces=df, light_n=32,
+                     direct_samples=2, seed=6)
+    ctx.sync()
+    assert not np.array_equal(get(f2, "direct"), get(f1, "direct"))
+    # SH render at degree 2 over the same scene (identity transfer).
+    nc = 9
+    T = np.zeros((len(verts), nc, nc, 3), np.float32)
+    T[:, np.arange(nc), np.arange(nc), :] = 1.0
+    L = np.zeros((nc, 3), np.float32)
+    L[0] = 1.0
+    out = nw.DeviceBuffer(P * 16)
+    ctx.sh_render(mesh, cam, W, H, nw.DeviceBuffer.from_array(T), nw.DeviceBuffer.from_array(L),
+                  2, out, brdf_samples=8, seed=1, env_faces=df, env_n=32)
+    ctx.sync()
+    img = out.download(np.float32, (P, 4))
+    assert np.all(np.isfinite(img)) and np.all(img[:, :3] >= 0) and img[flag, :3].mean() > 0
+    ctx.mesh_destroy(mesh)
