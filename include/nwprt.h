@@ -456,6 +456,15 @@ nwprt_status nwprt_obj_read(const char *path, float **vertices, int64_t *n_verti
                             float **normals, int32_t **indices, int64_t *n_triangles,
                             int32_t **material_ids, nwprt_material **materials,
                             int32_t *n_materials);
+/* Host utility (no GPU): Stanford PLY reader (ASCII or binary little-endian)
+ * with what loadPly (Renderer/Model.cpp:152-220) keeps: vertex x y z,
+ * optional nx ny nz, face vertex_indices (polygons fanned).  One material:
+ * cookTorranceMaterial's defaults (Model.cpp:231-235).  Same outputs as
+ * nwprt_obj_read. */
+nwprt_status nwprt_ply_read(const char *path, float **vertices, int64_t *n_vertices,
+                            float **normals, int32_t **indices, int64_t *n_triangles,
+                            int32_t **material_ids, nwprt_material **materials,
+                            int32_t *n_materials);
 nwprt_status nwprt_mesh_destroy(nwprt_ctx ctx, int32_t mesh_id);
 
 typedef struct {

@@ -211,6 +211,9 @@ _SIGNATURES = {
     "nwprt_obj_read": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int32)]),
+    "nwprt_ply_read": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
+                               POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
+                               POINTER(c_void_p), POINTER(c_int32)]),
     "nwprt_mesh_destroy": (c_int, [c_void_p, c_int32]),
     "nwprt_gbuffer_cast": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32,
                                    POINTER(CastParams), POINTER(GBuffer), c_void_p]),
@@ -340,9 +343,19 @@ def obj_read(path: str) -> Dict[str, np.ndarray]:
     """Wavefront OBJ (+ MTL) as mesh arrays: vertices [n][3], normals [n][3]
     or None, indices [t][3], material_ids [t], materials [m][8] rows
     (kd[3], ks[3], roughness, type)."""
+    return _mesh_read(lib().nwprt_obj_read, path)
+
+
+def ply_read(path: str) -> Dict[str, np.ndarray]:
+    """Stanford PLY as the same mesh arrays as obj_read (one default
+    cookTorrance material)."""
+    return _mesh_read(lib().nwprt_ply_read, path)
+
+
+def _mesh_read(fn, path) -> Dict[str, np.ndarray]:
     v, nr, ix, mid, mats = (c_void_p() for _ in range(5))
     nv, nt, nm = c_int64(), c_int64(), c_int32()
-    _check(lib().nwprt_obj_read(str(path).encode(), ctypes.byref(v), ctypes.byref(nv),
+    _check(fn(str(path).encode(), ctypes.byref(v), ctypes.byref(nv),
                                 ctypes.byref(nr), ctypes.byref(ix), ctypes.byref(nt),
                                 ctypes.byref(mid), ctypes.byref(mats), ctypes.byref(nm)))
     out = {"vertices": _take_host(v, nv.value * 3, np.float32).reshape(-1, 3),

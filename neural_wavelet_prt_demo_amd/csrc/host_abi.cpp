@@ -325,6 +325,196 @@ ObjMesh read_obj(const std::string &path) {
 
 }  // namespace
 
+// ------------------------------------------------------------------ PLY --
+// Stanford PLY reader for the same mesh arrays, following what loadPly
+// (Renderer/Model.cpp:152-220, tinyply) keeps: vertex x y z, optional
+// nx ny nz, and the face element's vertex_indices.  ASCII and binary
+// little-endian files; any scalar property type; other elements and
+// properties are skipped.  Faces with more than three corners are fanned
+// (loadPly reads triangles only).  A PLY file has no material: the mesh gets
+// cookTorranceMaterial's defaults, as loadPly's caller gives it
+// (Model.cpp:231-235).
+namespace {
+
+struct PlyProp {
+  std::string name, type, count_type;  // count_type non-empty: a list
+};
+struct PlyElem {
+  std::string name;
+  int64_t count = 0;
+  std::vector<PlyProp> props;
+};
+
+int ply_type_size(const std::string &t) {
+  if (t == "char" || t == "uchar" || t == "int8" || t == "uint8") return 1;
+  if (t == "short" || t == "ushort" || t == "int16" || t == "uint16") return 2;
+  if (t == "int" || t == "uint" || t == "float" || t == "int32" || t == "uint32" ||
+      t == "float32")
+    return 4;
+  if (t == "double" || t == "float64") return 8;
+  fail(NWPRT_ERR_FORMAT, "ply: unknown property type " + t);
+}
+
+double ply_binary_value(const std::string &t, const unsigned char *p) {
+  const int n = ply_type_size(t);
+  if (t == "float" || t == "float32") { float v; std::memcpy(&v, p, 4); return v; }
+  if (t == "double" || t == "float64") { double v; std::memcpy(&v, p, 8); return v; }
+  const bool sgn = t == "char" || t == "int8" || t == "short" || t == "int16" || t == "int" ||
+                   t == "int32";
+  if (n == 1) return sgn ? (double)(int8_t)p[0] : (double)p[0];
+  if (n == 2) {
+    uint16_t v;
+    std::memcpy(&v, p, 2);
+    return sgn ? (double)(int16_t)v : (double)v;
+  }
+  uint32_t v;
+  std::memcpy(&v, p, 4);
+  return sgn ? (double)(int32_t)v : (double)v;
+}
+
+ObjMesh read_ply(const std::string &path) {
+  std::ifstream in(path, std::ios::binary);
+  require((bool)in, NWPRT_ERR_IO, "ply: cannot open " + path);
+  std::string line;
+  require((bool)std::getline(in, line) && line.rfind("ply", 0) == 0, NWPRT_ERR_FORMAT,
+          "ply: missing magic in " + path);
+  bool binary = false;
+  std::vector<PlyElem> elems;
+  for (;;) {
+    require((bool)std::getline(in, line), NWPRT_ERR_FORMAT, "ply: header not terminated");
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    const auto t = split_ws(line);
+    if (t.empty() || t[0] == "comment" || t[0] == "obj_info") continue;
+    if (t[0] == "end_header") break;
+    if (t[0] == "format") {
+      require(t.size() >= 2, NWPRT_ERR_FORMAT, "ply: bad format line");
+      if (t[1] == "binary_little_endian") binary = true;
+      else require(t[1] == "ascii", NWPRT_ERR_FORMAT, "ply: unsupported format " + t[1]);
+    } else if (t[0] == "element" && t.size() >= 3) {
+      PlyElem e;
+      e.name = t[1];
+      e.count = std::stoll(t[2]);
+      require(e.count >= 0, NWPRT_ERR_FORMAT, "ply: negative element count");
+      elems.push_back(e);
+    } else if (t[0] == "property" && !elems.empty()) {
+      PlyProp pr;
+      if (t.size() >= 5 && t[1] == "list") {
+        pr.count_type = t[2];
+        pr.type = t[3];
+        pr.name = t[4];
+      } else {
+        require(t.size() >= 3, NWPRT_ERR_FORMAT, "ply: bad property line");
+        pr.type = t[1];
+        pr.name = t[2];
+      }
+      ply_type_size(pr.type);
+      if (!pr.count_type.empty()) ply_type_size(pr.count_type);
+      elems.back().props.push_back(pr);
+    }
+  }
+  std::vector<double> pos, nrm;
+  std::vector<std::vector<int64_t>> faces;
+  bool has_normal = false;
+  for (const PlyElem &e : elems) {
+    int ix = -1, iy = -1, iz = -1, inx = -1, iny = -1, inz = -1, ifc = -1;
+    for (size_t k = 0; k < e.props.size(); ++k) {
+      const std::string &n = e.props[k].name;
+      if (n == "x") ix = (int)k;
+      if (n == "y") iy = (int)k;
+      if (n == "z") iz = (int)k;
+      if (n == "nx") inx = (int)k;
+      if (n == "ny") iny = (int)k;
+      if (n == "nz") inz = (int)k;
+      if ((n == "vertex_indices" || n == "vertex_index") && !e.props[k].count_type.empty())
+        ifc = (int)k;
+    }
+    const bool is_vertex = e.name == "vertex", is_face = e.name == "face";
+    if (is_vertex) {
+      require(ix >= 0 && iy >= 0 && iz >= 0, NWPRT_ERR_FORMAT, "ply: vertex needs x y z");
+      has_normal = inx >= 0 && iny >= 0 && inz >= 0;
+    }
+    if (is_face) require(ifc >= 0, NWPRT_ERR_FORMAT, "ply: face needs vertex_indices");
+    std::vector<double> vals(e.props.size());
+    std::vector<int64_t> list;
+    for (int64_t r = 0; r < e.count; ++r) {
+      std::vector<std::string> tok;
+      size_t tk = 0;
+      if (!binary) {
+        require((bool)std::getline(in, line), NWPRT_ERR_FORMAT, "ply: truncated ascii data");
+        tok = split_ws(line);
+      }
+      auto next = [&](const std::string &type) -> double {
+        if (binary) {
+          unsigned char b[8];
+          const int n = ply_type_size(type);
+          in.read((char *)b, n);
+          require((bool)in, NWPRT_ERR_FORMAT, "ply: truncated binary data");
+          return ply_binary_value(type, b);
+        }
+        require(tk < tok.size(), NWPRT_ERR_FORMAT, "ply: short ascii record");
+        return std::stod(tok[tk++]);
+      };
+      for (size_t k = 0; k < e.props.size(); ++k) {
+        const PlyProp &pr = e.props[k];
+        if (pr.count_type.empty()) {
+          vals[k] = next(pr.type);
+        } else {
+          const double c = next(pr.count_type);
+          require(c >= 0 && c < 1e6, NWPRT_ERR_FORMAT, "ply: bad list length");
+          list.clear();
+          for (int64_t j = 0; j < (int64_t)c; ++j) list.push_back((int64_t)next(pr.type));
+          if (is_face && (int)k == ifc) faces.push_back(list);
+        }
+      }
+      if (is_vertex) {
+        pos.insert(pos.end(), {vals[ix], vals[iy], vals[iz]});
+        if (has_normal) nrm.insert(nrm.end(), {vals[inx], vals[iny], vals[inz]});
+      }
+    }
+  }
+  const int64_t nv = (int64_t)(pos.size() / 3);
+  ObjMesh m;
+  for (double v : pos) m.vertices.push_back((float)v);
+  for (double v : nrm) m.normals.push_back((float)v);
+  m.has_normals = has_normal;
+  for (const auto &f : faces) {
+    require(f.size() >= 3, NWPRT_ERR_FORMAT, "ply: face with fewer than 3 corners");
+    for (int64_t c : f)
+      require(c >= 0 && c < nv, NWPRT_ERR_FORMAT, "ply: vertex index out of range");
+    for (size_t c = 1; c + 1 < f.size(); ++c) {
+      m.indices.push_back((int32_t)f[0]);
+      m.indices.push_back((int32_t)f[c]);
+      m.indices.push_back((int32_t)f[c + 1]);
+      m.material_ids.push_back(0);
+    }
+  }
+  require(!m.indices.empty(), NWPRT_ERR_FORMAT, "ply: no faces in " + path);
+  m.materials.push_back(cook_torrance(0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 1.f));
+  return m;
+}
+
+}  // namespace
+
+extern "C" nwprt_status nwprt_ply_read(const char *path, float **vertices, int64_t *n_vertices,
+                                       float **normals, int32_t **indices, int64_t *n_triangles,
+                                       int32_t **material_ids, nwprt_material **materials,
+                                       int32_t *n_materials) {
+  return host_guarded([&] {
+    require(path && vertices && n_vertices && normals && indices && n_triangles &&
+                material_ids && materials && n_materials,
+            NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    const ObjMesh m = read_ply(path);
+    *vertices = copy_out(m.vertices);
+    *normals = m.has_normals ? copy_out(m.normals) : nullptr;
+    *indices = copy_out(m.indices);
+    *material_ids = copy_out(m.material_ids);
+    *materials = copy_out(m.materials);
+    *n_vertices = (int64_t)(m.vertices.size() / 3);
+    *n_triangles = (int64_t)(m.indices.size() / 3);
+    *n_materials = (int32_t)m.materials.size();
+  });
+}
+
 extern "C" nwprt_status nwprt_obj_read(const char *path, float **vertices, int64_t *n_vertices,
                                        float **normals, int32_t **indices, int64_t *n_triangles,
                                        int32_t **material_ids, nwprt_material **materials,

@@ -8,8 +8,8 @@
 // <i>_<name>.hdr, rows flipped as OutputWriter::writeRGBA does
 // (Renderer/OutputWriter.cpp:24-40, 146-149).
 //
-// G-buffers: with --obj, frame i's G-buffer comes from the ray caster over
-// the OBJ mesh, seen from the trajectory's camera (from = columns 0-2, at =
+// G-buffers: with --obj (or --mesh; a .ply file is read as PLY), frame i's
+// G-buffer comes from the ray caster over the mesh, seen from the trajectory's camera (from = columns 0-2, at =
 // columns 3-5, up +y, Renderer/offlineRenderer.cpp:118), positions
 // normalised by the mesh's bounds.  The frame's environment
 // (<env folder>/cube/<name>.hdr) is both the view and the lighting map, and
@@ -128,7 +128,7 @@ int main(int argc, char **argv) {
       else if (a == "-l") large = true;
       else if (a == "--seed") seed = std::stoull(next());
       else if (a == "--precision") precision = next();
-      else if (a == "--obj") obj = next();
+      else if (a == "--obj" || a == "--mesh") obj = next();
       else if (!a.empty() && a[0] != '-' && results.empty()) results = a;
       else throw std::runtime_error("unknown argument " + a);
     } catch (const std::exception &e) {
@@ -163,7 +163,10 @@ int main(int argc, char **argv) {
       auto need = [](nwprt_status st, const char *what) {
         if (st != NWPRT_OK) throw std::runtime_error(std::string(what) + ": " + nwprt_last_error());
       };
-      need(nwprt_obj_read(obj.c_str(), &v, &nv, &n, &ix, &nt, &mid, &mats, &nm), "reading OBJ");
+      const bool ply = obj.size() > 4 && obj.compare(obj.size() - 4, 4, ".ply") == 0;
+      need((ply ? nwprt_ply_read : nwprt_obj_read)(obj.c_str(), &v, &nv, &n, &ix, &nt, &mid,
+                                                   &mats, &nm),
+           "reading the mesh");
       const nwprt_status st =
           nwprt_mesh_create(engine.context(), v, nv, n, ix, nt, mid, mats, nm, &mesh);
       for (void *p : {(void *)v, (void *)n, (void *)ix, (void *)mid, (void *)mats})

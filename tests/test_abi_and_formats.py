@@ -316,3 +316,48 @@ def test_obj_reader(nw, tmp_path):
     with pytest.raises(nw.NwprtError) as e:
         nw.obj_read(tmp_path / "missing.obj")
     assert e.value.status == nw.ERR_IO
+
+
+def test_ply_reader(nw, tmp_path):
+    """nwprt_ply_read keeps what loadPly keeps (Renderer/Model.cpp:152-220):
+    x y z, optional nx ny nz, face vertex_indices; ASCII and binary
+    little-endian, any scalar types, unknown properties and elements
+    skipped, polygons fanned, one default cookTorrance material."""
+    (tmp_path / "a.ply").write_text(
+        "ply\nformat ascii 1.0\ncomment test\nelement vertex 5\nproperty float x\n"
+        "property float y\nproperty float z\nproperty float nx\nproperty float ny\n"
+        "property float nz\nelement face 2\nproperty list uchar int vertex_indices\nend_header\n"
+        "0 0 0 0 0 1\n1 0 0 0 0 1\n1 1 0 0 0 1\n0 1 0 0 0 1\n0 0 1 1 0 0\n"
+        "4 0 1 2 3\n3 0 1 4\n")
+    m = nw.ply_read(tmp_path / "a.ply")
+    assert m["vertices"].shape == (5, 3) and m["normals"].shape == (5, 3)
+    np.testing.assert_array_equal(m["indices"], [[0, 1, 2], [0, 2, 3], [0, 1, 4]])
+    np.testing.assert_array_equal(m["normals"][4], [1, 0, 0])
+    np.testing.assert_allclose(m["materials"], [[0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1, 2]])
+    assert list(m["material_ids"]) == [0, 0, 0]
+    # Binary little-endian: doubles, an extra property and an extra element.
+    hdr = ("ply\nformat binary_little_endian 1.0\nelement vertex 3\nproperty double x\n"
+           "property double y\nproperty double z\nproperty uchar red\n"
+           "element edge 1\nproperty int a\nproperty int b\n"
+           "element face 1\nproperty list uchar uint vertex_indices\nend_header\n").encode()
+    body = b"".join(struct.pack("<dddB", *v) for v in ((0, 0, 0, 7), (2, 0, 0, 8), (0, 3, 0, 9)))
+    body += struct.pack("<ii", 0, 1) + struct.pack("<BIII", 3, 2, 1, 0)
+    (tmp_path / "b.ply").write_bytes(hdr + body)
+    m = nw.ply_read(tmp_path / "b.ply")
+    assert m["normals"] is None
+    np.testing.assert_array_equal(m["vertices"], [[0, 0, 0], [2, 0, 0], [0, 3, 0]])
+    np.testing.assert_array_equal(m["indices"], [[2, 1, 0]])
+    for name, text, status in (
+            ("be.ply", "ply\nformat binary_big_endian 1.0\nelement vertex 0\nend_header\n",
+             nw.ERR_FORMAT),
+            ("oob.ply", "ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\n"
+                        "property float y\nproperty float z\nelement face 1\n"
+                        "property list uchar int vertex_indices\nend_header\n"
+                        "0 0 0\n1 0 0\n0 1 0\n3 0 1 5\n", nw.ERR_FORMAT)):
+        (tmp_path / name).write_text(text)
+        with pytest.raises(nw.NwprtError) as e:
+            nw.ply_read(tmp_path / name)
+        assert e.value.status == status
+    with pytest.raises(nw.NwprtError) as e:
+        nw.ply_read(tmp_path / "missing.ply")
+    assert e.value.status == nw.ERR_IO
