@@ -52,7 +52,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the oracle check of the GPU results (part of the CPU-baseline "
+                         "leg; --no-cpu-baseline skips it too)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-jobs leg")
     ap.add_argument("--no-serving", action="store_true",
@@ -642,8 +644,9 @@ def main():
     if not args.no_serving and prec != nw.PREC_FP8:
         result["served"] = bench_serving(nw, ctx, prec)
 
-    # --- accuracy against the FP64 oracle on a pixel sample ---------------
-    if not args.no_accuracy:
+    # --- CPU-baseline leg, part 1: the FP64 oracle checks the GPU results on
+    # a pixel sample (outside every timed region) ---------------------------
+    if not args.no_cpu_baseline and not args.no_accuracy:
         from oracle import binding as orc
         idx, st = ctx.envmap_selection(env)
         res = frame.results()  # last run: plain FP32
@@ -664,7 +667,8 @@ def main():
                          "psnr_db": float(10 * np.log10(ref.max() ** 2 / mse)) if mse > 0 else None}
         result["accuracy_vs_fp64_oracle"] = {"sample_pixels": len(pix), **acc}
 
-    # --- CPU baseline: the oracle's FP32 port on the host cores -----------
+    # --- CPU-baseline leg, part 2: the oracle's FP32 port timed on the host
+    # cores ------------------------------------------------------------------
     if not args.no_cpu_baseline:
         from oracle import binding as orc
         idx, st = ctx.envmap_selection(env)
