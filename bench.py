@@ -303,7 +303,8 @@ def bench_sh(nw, ctx, W, H, stream, dfaces, n_face, samples=64):
     """SH-PRT render leg (RenderEngine::shRender baseline): degree 8 (81 x 81
     transfer matrices, the reference's tested degree) over a 256 x 128 UV
     sphere on a ground quad (65,538 triangles, 33,028 vertices: 2.6 GB of
-    transfer matrices), 1080p, `samples` BSDF samples per triangle corner."""
+    transfer matrices), 1080p, `samples` BSDF samples per distinct (vertex,
+    material) pair of the triangle corners."""
     from neural_wavelet_prt_demo_amd import DeviceBuffer, Event
     nu, nv = 256, 128
     th = np.linspace(0, np.pi, nv + 1)
@@ -345,7 +346,7 @@ def bench_sh(nw, ctx, W, H, stream, dfaces, n_face, samples=64):
     ctx.sync()
     ms = float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(5)]))
     res = {"workload": f"SH-PRT degree 8, {W}x{H}, {len(idx)} triangles, {len(verts)} vertices, "
-                       f"{samples} BSDF samples per corner",
+                       f"{samples} BSDF samples per vertex",
            "ms": ms, "transfer_bytes": len(verts) * nc * nc * 12,
            "transfer_read_gbs": len(verts) * nc * nc * 12 / (ms / 1e3) / 1e9}
     ctx.mesh_destroy(mesh)

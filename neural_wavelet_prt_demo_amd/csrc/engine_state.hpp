@@ -78,6 +78,10 @@ struct EnvMap {
 // closest-hit step reads.
 struct Mesh {
   DevBuf nodes, tris, indices, vertices, normals, tri_material, materials;
+  // Distinct (vertex, material) pairs of the triangle corners, for the SH
+  // render: corner_key[3 t + c] -> pair; key_vertex / key_material per pair.
+  DevBuf corner_key, key_vertex, key_material;
+  int64_t n_keys = 0;
   float lower[3] = {0, 0, 0}, upper[3] = {0, 0, 0};
   int64_t n_tris = 0, n_nodes = 0, n_vertices = 0;
   int depth = 0;
@@ -150,7 +154,7 @@ struct nwprt_context {
   nwprt::DevBuf denoise_scratch;  // denoiser ping-pong images + normalisation
   std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
   nwprt::DevBuf env_tables;  // direct-light sampling tables of the last cast
-  nwprt::DevBuf sh_response, sh_corners;  // SH-PRT render work buffers
+  nwprt::DevBuf sh_response, sh_colors;  // SH-PRT render work buffers
 };
 
 namespace nwprt {

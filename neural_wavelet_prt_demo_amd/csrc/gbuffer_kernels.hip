@@ -806,8 +806,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gb
 // matrix times the environment's SH vector) and B the Monte Carlo SH
 // projection of the BSDF at the vertex (normal and view direction of the
 // vertex, f / pdf weights); the pixel interpolates them barycentrically.
-// Here the vertex colours are computed once per triangle corner (keyed by
-// vertex and material) instead of once per pixel: the same expectation.
+// Here the vertex colours are computed once per distinct (vertex, material)
+// pair of the triangle corners instead of once per pixel: the same
+// expectation.
 
 constexpr int kShCoeffs = (kShMaxDegree + 1) * (kShMaxDegree + 1);
 
@@ -919,15 +920,21 @@ __global__ __launch_bounds__(256) void sh_light_response_kernel(ShArgs a) {
   }
 }
 
-// Colour of each triangle corner: the vertex's BSDF SH projection (normal
-// and view direction of the vertex, devicePrograms.cu:446-478) dotted with
-// its light response, then |.| per channel.
+// Colour of each distinct (vertex, material) pair of the triangle corners:
+// the vertex's BSDF SH projection (normal and view direction of the vertex,
+// devicePrograms.cu:446-478) dotted with its light response, then |.| per
+// channel.  kShPairLanes lanes share one pair and take every kShPairLanes-th
+// sample, so that the few tens of thousands of pairs of a mesh still fill the
+// chip; their partial sums meet in a shuffle reduction.
+constexpr int kShPairLanes = 16;
+
 __global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
-  const long long c = (long long)blockIdx.x * 64 + threadIdx.x;
-  if (c >= a.n_tris * 3) return;
-  const long long tri = c / 3;
-  const int vi = a.indices[c];
-  const int mid = a.tri_material ? a.tri_material[tri] : 0;
+  const int g = threadIdx.x & (kShPairLanes - 1);
+  const long long u = (long long)blockIdx.x * (64 / kShPairLanes) + threadIdx.x / kShPairLanes;
+  // Lanes past the last pair stay for the shuffles and sample nothing.
+  const bool live = u < a.n_keys;
+  const int vi = live ? a.key_vertex[u] : 0;
+  const int mid = live ? a.key_material[u] : 0;
   const MaterialRec mat = a.materials[mid];
   const float3 pv = v3(a.vertices + 3 * (size_t)vi);
   const float3 n = normalize(v3(a.normals + 3 * (size_t)vi));
@@ -938,7 +945,7 @@ __global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
   const uint32_t seed = a.seed ^ (0x9E3779B9u * (uint32_t)(mid + 1));
   float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
   int valid = 0;
-  for (int k = 0; k < a.samples; ++k) {
+  for (int k = g; live && k < a.samples; k += kShPairLanes) {
     const uint32_t q = (uint32_t)k * 4u;
     const float3 l =
         sf.type == 0 ? rp_sample(sf, view, rng_uniform(seed, (uint32_t)vi, q),
@@ -967,8 +974,16 @@ __global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
     acc2 += f.z / pdf * d2;
     ++valid;
   }
+#pragma unroll
+  for (int o = kShPairLanes / 2; o > 0; o >>= 1) {
+    acc0 += __shfl_xor(acc0, o, kShPairLanes);
+    acc1 += __shfl_xor(acc1, o, kShPairLanes);
+    acc2 += __shfl_xor(acc2, o, kShPairLanes);
+    valid += __shfl_xor(valid, o, kShPairLanes);
+  }
+  if (!live || g != 0) return;
   const float inv = valid > 0 ? 1.0f / (float)valid : 0.0f;
-  float *o = a.corner_color + 3 * (size_t)c;
+  float *o = a.key_color + 3 * (size_t)u;
   o[0] = fabsf(acc0 * inv);
   o[1] = fabsf(acc1 * inv);
   o[2] = fabsf(acc2 * inv);
@@ -990,10 +1005,12 @@ __global__ __launch_bounds__(64) void sh_pixel_kernel(ShArgs a) {
   if (h.prim == INT_MAX) {
     if (a.env_faces) c = env_radiance(a.env_faces, a.env_n, d);
   } else {
-    const float *cc = a.corner_color + 9 * (size_t)h.prim;
+    const int32_t *ck = a.corner_key + 3 * (size_t)h.prim;
+    const float *c0 = a.key_color + 3 * (size_t)ck[0], *c1 = a.key_color + 3 * (size_t)ck[1],
+                *c2 = a.key_color + 3 * (size_t)ck[2];
     const float w = 1.0f - h.u - h.v;
-    c = v3(w * cc[0] + h.u * cc[3] + h.v * cc[6], w * cc[1] + h.u * cc[4] + h.v * cc[7],
-           w * cc[2] + h.u * cc[5] + h.v * cc[8]);
+    c = v3(w * c0[0] + h.u * c1[0] + h.v * c2[0], w * c0[1] + h.u * c1[1] + h.v * c2[1],
+           w * c0[2] + h.u * c1[2] + h.v * c2[2]);
   }
   reinterpret_cast<float4 *>(a.color)[(size_t)iy * a.width + ix] = make_float4(c.x, c.y, c.z, 1.0f);
 }
@@ -1037,7 +1054,8 @@ namespace nwprt {
 
 void launch_sh_render(const ShArgs &a, hipStream_t s) {
   require(a.nodes && a.tris && a.indices && a.vertices && a.normals && a.materials &&
-              a.transfer && a.light_sh && a.light_response && a.corner_color && a.color,
+              a.transfer && a.light_sh && a.light_response && a.corner_key && a.key_vertex &&
+              a.key_material && a.n_keys > 0 && a.key_color && a.color,
           NWPRT_ERR_INVALID_ARGUMENT, "sh_render: null buffer");
   require(a.degree >= 0 && a.degree <= kShMaxDegree && a.samples >= 1 && a.width > 0 &&
               a.height > 0 && a.n_tris > 0 && a.n_vertices > 0,
@@ -1045,8 +1063,10 @@ void launch_sh_render(const ShArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(sh_light_response_kernel, dim3((unsigned)((a.n_vertices + 3) / 4)),
                      dim3(256), 0, s, a);
   hip_check(hipGetLastError(), "sh_light_response_kernel launch");
-  hipLaunchKernelGGL(sh_corner_kernel, dim3((unsigned)((a.n_tris * 3 + 63) / 64)), dim3(64), 0, s,
-                     a);
+  constexpr int pairs_per_block = 64 / kShPairLanes;
+  hipLaunchKernelGGL(sh_corner_kernel,
+                     dim3((unsigned)((a.n_keys + pairs_per_block - 1) / pairs_per_block)),
+                     dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "sh_corner_kernel launch");
   const long long tiles = (long long)((a.width + 7) / 8) * ((a.height + 7) / 8);
   hipLaunchKernelGGL(sh_pixel_kernel, dim3((unsigned)tiles), dim3(64), 0, s, a);

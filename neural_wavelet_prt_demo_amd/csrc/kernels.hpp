@@ -225,7 +225,12 @@ struct ShArgs {
   int degree, samples;
   uint32_t seed;
   float *light_response;        // scratch [n_vertices][nc][3]
-  float *corner_color;          // scratch [n_tri][3][3]
+  // Distinct (vertex, material) pairs of the triangle corners: corner c of
+  // triangle t uses pair corner_key[3 t + c]; pair u is (key_vertex[u],
+  // key_material[u]).
+  const int32_t *corner_key, *key_vertex, *key_material;
+  int64_t n_keys;
+  float *key_color;             // scratch [n_keys][3]
   float cam_pos[3], cam_dir[3], cam_h[3], cam_v[3];
   float cx, cy, fx, fy;
   int width, height;

@@ -344,6 +344,32 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     if (material_ids)
       mesh->tri_material.upload(material_ids, (size_t)n_triangles * sizeof(int32_t));
     mesh->materials.upload(mats.data(), mats.size() * sizeof(MaterialRec));
+    {
+      // Corners that share a vertex and a material get the same SH colour:
+      // number the distinct (vertex, material) pairs once, here.
+      const int64_t n_corners = n_triangles * 3;
+      std::vector<uint64_t> pair(n_corners);
+      for (int64_t c = 0; c < n_corners; ++c) {
+        const uint32_t mid = material_ids ? (uint32_t)material_ids[c / 3] : 0u;
+        pair[c] = ((uint64_t)(uint32_t)indices[c] << 32) | mid;
+      }
+      std::vector<uint64_t> keys(pair);
+      std::sort(keys.begin(), keys.end());
+      keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+      std::vector<int32_t> corner_key(n_corners), key_vertex(keys.size()),
+          key_material(keys.size());
+      for (int64_t c = 0; c < n_corners; ++c)
+        corner_key[c] =
+            (int32_t)(std::lower_bound(keys.begin(), keys.end(), pair[c]) - keys.begin());
+      for (size_t u = 0; u < keys.size(); ++u) {
+        key_vertex[u] = (int32_t)(keys[u] >> 32);
+        key_material[u] = (int32_t)(keys[u] & 0xffffffffu);
+      }
+      mesh->corner_key.upload(corner_key.data(), corner_key.size() * sizeof(int32_t));
+      mesh->key_vertex.upload(key_vertex.data(), key_vertex.size() * sizeof(int32_t));
+      mesh->key_material.upload(key_material.data(), key_material.size() * sizeof(int32_t));
+      mesh->n_keys = (int64_t)keys.size();
+    }
     std::memcpy(mesh->lower, all.lo, sizeof all.lo);
     std::memcpy(mesh->upper, all.hi, sizeof all.hi);
     mesh->n_tris = n_triangles;
@@ -491,7 +517,7 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
                  a.fy);
     const int nc = (degree + 1) * (degree + 1);
     ctx->sh_response.ensure((size_t)m.n_vertices * nc * 3 * sizeof(float));
-    ctx->sh_corners.ensure((size_t)m.n_tris * 9 * sizeof(float));
+    ctx->sh_colors.ensure((size_t)m.n_keys * 3 * sizeof(float));
     a.nodes = m.nodes.as<BvhNode>();
     a.tris = m.tris.as<BvhTri>();
     a.indices = m.indices.as<int32_t>();
@@ -507,7 +533,11 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
     a.samples = brdf_samples;
     a.seed = seed;
     a.light_response = ctx->sh_response.as<float>();
-    a.corner_color = ctx->sh_corners.as<float>();
+    a.corner_key = m.corner_key.as<int32_t>();
+    a.key_vertex = m.key_vertex.as<int32_t>();
+    a.key_material = m.key_material.as<int32_t>();
+    a.n_keys = m.n_keys;
+    a.key_color = ctx->sh_colors.as<float>();
     a.width = width;
     a.height = height;
     a.env_faces = view_env_faces;
