@@ -895,6 +895,15 @@ __device__ inline float wave_sum(float x) {
   return x;
 }
 
+// A row of the transfer matrix is nc x 3 <= 243 floats: at most 4 per lane.
+// The loop keeps kShRowsInFlight rows' loads issued ahead of the row being
+// reduced, so that a wave has several rows of HBM reads outstanding while
+// its cross-lane sums run.
+constexpr int kShRowSlots = 4;
+constexpr int kShRowsInFlight = 4;
+static_assert((kShMaxDegree + 1) * (kShMaxDegree + 1) * 3 <= 64 * kShRowSlots,
+              "a transfer-matrix row must fit the wave's load slots");
+
 __global__ __launch_bounds__(256) void sh_light_response_kernel(ShArgs a) {
   const int nc = (a.degree + 1) * (a.degree + 1), row = nc * 3;
   const int lane = threadIdx.x & 63;
@@ -902,20 +911,47 @@ __global__ __launch_bounds__(256) void sh_light_response_kernel(ShArgs a) {
   if (v >= a.n_vertices) return;  // whole waves leave together; no barriers below
   const float *t = a.transfer + (size_t)v * nc * row;
   float *o = a.light_response + (size_t)v * row;
-  for (int k = 0; k < nc; ++k) {
-    float acc[3] = {0.0f, 0.0f, 0.0f};
-    for (int f = lane; f < row; f += 64) {
-      const float x = t[(size_t)k * row + f] * a.light_sh[f];
-      const int c = f % 3;
-      acc[0] += c == 0 ? x : 0.0f;
-      acc[1] += c == 1 ? x : 0.0f;
-      acc[2] += c == 2 ? x : 0.0f;
+  float l[kShRowSlots];
+  int c[kShRowSlots];
+#pragma unroll
+  for (int s = 0; s < kShRowSlots; ++s) {
+    const int f = lane + 64 * s;
+    l[s] = f < row ? a.light_sh[f] : 0.0f;
+    c[s] = f % 3;
+  }
+  float buf[kShRowsInFlight][kShRowSlots];
+  auto load_row = [&](int k, float(&r)[kShRowSlots]) {
+#pragma unroll
+    for (int s = 0; s < kShRowSlots; ++s) {
+      const int f = lane + 64 * s;
+      r[s] = (k < nc && f < row) ? t[(size_t)k * row + f] : 0.0f;
     }
-    const float r0 = wave_sum(acc[0]), r1 = wave_sum(acc[1]), r2 = wave_sum(acc[2]);
-    if (lane == 0) {
-      o[3 * k] = r0;
-      o[3 * k + 1] = r1;
-      o[3 * k + 2] = r2;
+  };
+#pragma unroll
+  for (int d = 0; d < kShRowsInFlight; ++d) load_row(d, buf[d]);
+  for (int k0 = 0; k0 < nc; k0 += kShRowsInFlight) {
+#pragma unroll
+    for (int d = 0; d < kShRowsInFlight; ++d) {
+      float cur[kShRowSlots];
+#pragma unroll
+      for (int s = 0; s < kShRowSlots; ++s) cur[s] = buf[d][s];
+      load_row(k0 + kShRowsInFlight + d, buf[d]);
+      const int k = k0 + d;
+      if (k >= nc) continue;  // wave-uniform
+      float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+#pragma unroll
+      for (int s = 0; s < kShRowSlots; ++s) {
+        const float x = cur[s] * l[s];
+        acc0 += c[s] == 0 ? x : 0.0f;
+        acc1 += c[s] == 1 ? x : 0.0f;
+        acc2 += c[s] == 2 ? x : 0.0f;
+      }
+      const float r0 = wave_sum(acc0), r1 = wave_sum(acc1), r2 = wave_sum(acc2);
+      if (lane == 0) {
+        o[3 * k] = r0;
+        o[3 * k + 1] = r1;
+        o[3 * k + 2] = r2;
+      }
     }
   }
 }
