@@ -531,12 +531,13 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
  * the camera, brdf_samples BSDF samples weighted f / pdf,
  * devicePrograms.cu:312-501); the pixel interpolates the three colours
  * barycentrically.  Misses show the view environment (or 0).  Vertex
- * colours are computed once per triangle corner with a random stream keyed
- * by (seed, vertex, material), not once per pixel: the same expectation.
+ * colours are computed once per distinct (vertex, material) pair of the
+ * triangle corners with a random stream keyed by (seed, vertex, material),
+ * not once per pixel: the same expectation.
  * transfer: device [n_vertices][nc][nc][3]; light_sh: device [nc][3];
  * color: device float4 [height][width].  The mesh needs vertex normals;
  * materials must be roughPlastic, cookTorrance or simple_ggx.  The light
- * responses and corner colours live in work buffers of the context: calls
+ * responses and vertex colours live in work buffers of the context: calls
  * issued on different streams must not overlap. */
 nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
                              int32_t width, int32_t height, const float *transfer,

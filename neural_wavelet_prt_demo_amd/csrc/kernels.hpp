@@ -208,8 +208,9 @@ size_t env_table_doubles(int n);
 void launch_env_tables(const float *faces, int n, double *tab, hipStream_t s);
 void launch_gbuffer_cast(const CastArgs &a, hipStream_t s);
 
-// SH-PRT render (gbuffer_kernels.hip): per-vertex light response, per
-// triangle-corner colour, per-pixel interpolation.
+// SH-PRT render (gbuffer_kernels.hip): per-vertex light response, colour
+// per (vertex, material) pair of the triangle corners, per-pixel
+// interpolation.
 constexpr int kShMaxDegree = 8;
 struct ShArgs {
   const BvhNode *nodes;
