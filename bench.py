@@ -119,10 +119,11 @@ def bench_serving(nw, ctx, prec, n_requests=60):
     that sees it complete; throughput over the span first arrival -> last
     completion.  Memory budget 200 GB of request state."""
     from neural_wavelet_prt_demo_amd import DeviceBuffer, FrameOut
+    from neural_wavelet_prt_demo_amd import workloads
     desc = nw.config_scene_desc(128, nw.DTYPE_F16)
-    scenes = [ctx.scene_create(desc, nw.synth_scene(desc, 2000 + i)) for i in range(4)]
-    bases = [DeviceBuffer.from_array(nw.synth_cubemap(3000 + i, 128)) for i in range(4)]
-    fresh = [nw.synth_cubemap(4000 + j, 128) for j in range(8)]
+    scenes = [ctx.scene_create(desc, nw.synth_scene(desc, s)) for s in workloads.SERVED_SCENE_SEEDS]
+    bases = [DeviceBuffer.from_array(nw.synth_cubemap(s, 128)) for s in workloads.SERVED_BASE_SEEDS]
+    fresh = [nw.synth_cubemap(s, 128) for s in workloads.SERVED_FRESH_SEEDS]
     views = {"1080p": (1920, 1080), "4k": (3840, 2160)}
     gbufs = {v: nw.DeviceFrame(w, h, nw.synth_gbuffer(5000 + i, w, h, 0.0))
              for i, (v, (w, h)) in enumerate(views.items())}
@@ -390,7 +391,7 @@ def bench_batch(nw, ctx, prec, reps=3):
     from neural_wavelet_prt_demo_amd import Event
     specs = workloads.batch_jobs()
     desc = nw.config_scene_desc(64, nw.DTYPE_F16)
-    scenes = [ctx.scene_create(desc, nw.synth_scene(desc, 1000 + i)) for i in range(2)]
+    scenes = [ctx.scene_create(desc, nw.synth_scene(desc, s)) for s in workloads.BATCH_SCENE_SEEDS]
     jobs, keep = [], []
     for sp in specs:
         env = ctx.envmap_from_faces(nw.synth_cubemap(sp.seed, 64), sp.k)

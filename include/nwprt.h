@@ -622,8 +622,10 @@ nwprt_status nwprt_batch_destroy(nwprt_ctx ctx, int32_t batch_id);
  * preparation overlaps their launches.  Each request's outputs equal, bit
  * for bit, nwprt_relight of the same inputs with the same precision and tile
  * variant.  Requests never share outputs while in flight (the caller's
- * responsibility); G-buffers, resident maps and source cubemaps must stay
- * valid until the request completes. */
+ * responsibility); G-buffers and source cubemaps must stay valid until the
+ * request completes, and the library keeps the request's scene and resident
+ * map from being destroyed or updated until then.  A resident map's
+ * strengths are copied at preparation. */
 typedef struct {
   uint64_t memory_budget_bytes; /* device bytes of admitted requests' state; 0: no limit */
   int32_t max_active;           /* requests admitted at once (>= 1)         */
@@ -658,6 +660,7 @@ typedef struct {
   int64_t ticks, items;
   uint64_t bytes_in_use, peak_bytes;
   int32_t peak_active;
+  int64_t failed; /* requests reported done with a negative latency */
 } nwprt_server_stats;
 
 nwprt_status nwprt_server_create(nwprt_ctx ctx, const nwprt_server_config *config,
@@ -668,7 +671,13 @@ nwprt_status nwprt_server_submit(nwprt_ctx ctx, int32_t server_id,
 /* Advances the server and reports up to max_done completed requests (ids and
  * submit-to-completion latency in ms, in completion order).  wait: 0 returns
  * at once; 1 blocks until at least one request completes (or none is
- * pending); 2 blocks until every submitted request has completed. */
+ * pending); 2 blocks until every submitted request has completed.  A request
+ * that could not be run (its device state could not be allocated with no
+ * other request admitted, or its preparation failed) is reported done with
+ * latency -status (negative); it gives back everything it held, and the
+ * queue moves on.  Scenes and resident maps that a queued or running
+ * request reads cannot be destroyed or updated in place until it is
+ * reported (NWPRT_ERR_STATE). */
 nwprt_status nwprt_server_poll(nwprt_ctx ctx, int32_t server_id, int32_t wait,
                                int64_t *done_ids, double *latency_ms,
                                int32_t max_done, int32_t *n_done);

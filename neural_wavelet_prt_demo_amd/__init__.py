@@ -108,7 +108,8 @@ class Request(Structure):
 class ServerStats(Structure):
     _fields_ = [("submitted", c_int64), ("completed", c_int64), ("queued", c_int64),
                 ("active", c_int64), ("ticks", c_int64), ("items", c_int64),
-                ("bytes_in_use", c_uint64), ("peak_bytes", c_uint64), ("peak_active", c_int32)]
+                ("bytes_in_use", c_uint64), ("peak_bytes", c_uint64), ("peak_active", c_int32),
+                ("failed", c_int64)]
 
 
 class DenoiseParams(Structure):

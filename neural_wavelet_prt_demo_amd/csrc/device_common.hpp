@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tonemap_table.hpp"
+
 namespace nwprt {
 namespace dev {
 
@@ -85,21 +87,23 @@ __device__ __forceinline__ void neumaier_add(float &s, float &c, float x) {
   s = t;
 }
 
-// sRGB-style tone map of Renderer/toneMap.cu:58-70 for one pixel:
-// y = clamp01(1.055 * (x + 1e-6)^(1/2.4) - 0.055), byte = (uint)(y * 255.9),
-// packed R | G << 8 | B << 16 | 255 << 24.  The multiply-subtract is one
-// fused multiply-add, as a CUDA build with default contraction emits.
-__device__ __forceinline__ uint32_t tonemap_rgba8(float r, float g, float b) {
-  const float coeff = 1.055f, inv_exp = 1.0f / 2.4f, off = coeff - 1.0f;
-  const float in[3] = {r, g, b};
-  uint32_t out = 255u << 24;
+// Tone-map byte of one channel (Renderer/toneMap.cu:58-70): the number of
+// thresholds T_b <= x, where T_b is the smallest FP32 x whose curve value
+// 1.055 (x + 1e-6)^(1/2.4) - 0.055, taken in FP64, reaches b / 255.9 --
+// i.e. (uint)(255.9 * clamp01(curve)) without the rounding of a float pow.
+// Eight steps of binary search over the table; NaN and negative inputs give
+// 0 (every comparison is false), +inf gives 255.
+__device__ __forceinline__ uint32_t tonemap_byte(float x) {
+  uint32_t b = 0;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    float y = fmaf(coeff, powf(in[c] + 1e-6f, inv_exp), -off);
-    y = fminf(1.0f, fmaxf(0.0f, y));  // fmaxf maps NaN to 0
-    out |= ((uint32_t)(y * 255.9f)) << (8 * c);
-  }
-  return out;
+  for (uint32_t step = 128; step >= 1; step >>= 1)
+    if (x >= kTonemapThresholds[b + step]) b += step;
+  return b;
+}
+
+// sRGB-style tone map of one pixel: R | G << 8 | B << 16 | 255 << 24.
+__device__ __forceinline__ uint32_t tonemap_rgba8(float r, float g, float b) {
+  return tonemap_byte(r) | (tonemap_byte(g) << 8) | (tonemap_byte(b) << 16) | (255u << 24);
 }
 
 }  // namespace dev

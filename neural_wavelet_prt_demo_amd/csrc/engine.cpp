@@ -103,7 +103,7 @@ bool planar_features(const Scene &sc, Elem elem) {
 }
 
 bool fuse_encode(const Scene &sc, nwprt_precision prec) {
-  return prec != NWPRT_PREC_FP8 && elem_of(prec) != Elem::F32 && planar_features(sc, elem_of(prec));
+  return elem_of(prec) != Elem::F32 && planar_features(sc, elem_of(prec));
 }
 
 void set_fused_encode(RelightArgs &a, const Scene &sc, const nwprt_gbuffer *gb) {
@@ -119,6 +119,74 @@ void set_fused_encode(RelightArgs &a, const Scene &sc, const nwprt_gbuffer *gb) 
 
 const float *scene_projection(const Scene &sc) {
   return sc.desc.has_projection ? sc.proj.as<float>() : nullptr;
+}
+
+void set_aux_encode(EncodeArgs &e, const Scene &sc, const nwprt_gbuffer *gb, void *aux) {
+  const float *f4[] = {gb->normal, gb->reflect, gb->kd, gb->ks};
+  for (const float *p : f4) {
+    require(p != nullptr, NWPRT_ERR_INVALID_ARGUMENT,
+            "G-buffer normal, reflect, kd and ks are required by this scene's aux inputs");
+    check_aligned(p, 16, "G-buffer aux input");
+  }
+  require(gb->roughness != nullptr, NWPRT_ERR_INVALID_ARGUMENT,
+          "G-buffer roughness is required by this scene's aux inputs");
+  e.normal = gb->normal;
+  e.reflect = gb->reflect;
+  e.roughness = gb->roughness;
+  e.kd = gb->kd;
+  e.ks = gb->ks;
+  e.aux = aux;
+  e.aux_pad = sc.desc.aux_pad_value;
+}
+
+void set_aux_relight(RelightArgs &a, const Scene &sc, nwprt_precision prec, const void *aux) {
+  a.aux = aux;
+  a.waux_f32 = sc.waux_f32();
+  a.waux_tiles = prec == NWPRT_PREC_FP16 ? sc.aux_f16.as<uint16_t>() : sc.aux_bf16.as<uint16_t>();
+  a.waux32_tiles = sc.aux32_bf16.as<uint16_t>();
+}
+
+size_t fp8_mk_bytes(int k) { return (size_t)k * 4 * 64 * 16; }
+
+size_t fp8_calib_bytes(const Scene &sc, int n_pixels) {
+  return (size_t)fp8_calib_count(n_pixels) * (64 + (sc.desc.aux_dims ? 32 : 0));
+}
+
+void prepare_fp8(const Scene &sc, const int32_t *idx, int k, const RelightArgs &a,
+                 const Fp8Work &w, hipStream_t s) {
+  const int P = a.n_pixels;
+  const bool planar = planar_features(sc, Elem::BF16);
+  launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(),
+                           6 * sc.desc.face_size * sc.desc.face_size, idx, k, Elem::BF16, w.mk16,
+                           nullptr, s, w.mk32, planar, scene_projection(sc));
+  // Per-layer activation maxima over about 16K pixels spread over the frame.
+  const int stride = fp8_calib_stride(P), n_cal = fp8_calib_count(P);
+  uint16_t *calib_aux = sc.desc.aux_dims ? w.calib + (size_t)n_cal * 32 : nullptr;
+  if (a.fused_encode)
+    launch_calib_encode(a, stride, w.calib, n_cal, w.amax, s);
+  else
+    launch_calib_gather((const uint16_t *)a.features, (const uint16_t *)a.aux, a.env_flag, P,
+                        stride, w.calib, calib_aux, n_cal, w.amax, planar, s);
+  RelightArgs c{};
+  c.n_pixels = n_cal;
+  c.k = k;
+  c.features = w.calib;
+  c.features_planar = planar;
+  c.mk = w.mk16;
+  c.w_tiles = sc.tiles_bf16.as<uint16_t>();
+  c.amax = w.amax;
+  if (sc.desc.aux_dims) {
+    c.aux = calib_aux;
+    c.waux_tiles = sc.aux_bf16.as<uint16_t>();
+  }
+  launch_calibrate_amax(c, s);
+}
+
+void set_fp8_relight(RelightArgs &a, const Scene &sc, const Fp8Work &w) {
+  a.mk = w.mk32;
+  a.w8_tiles = sc.tiles_fp8.as<uint8_t>();
+  for (int i = 0; i < 3; ++i) a.w8_exp[i] = sc.fp8_exp[i];
+  a.amax = w.amax;
 }
 
 namespace {
@@ -170,11 +238,12 @@ void create_scene(nwprt_ctx ctx, const nwprt_scene_desc &d, const nwprt_scene_pa
     s->aux_bf16.upload(tiles.data(), tiles.size() * 2);
     build_aux_tiles(waux, Elem::F16, tiles);
     s->aux_f16.upload(tiles.data(), tiles.size() * 2);
-  } else {
-    std::vector<uint8_t> t8;
-    build_fp8_weight_tiles(prm.layers, t8, s->fp8_exp);
-    s->tiles_fp8.upload(t8.data(), t8.size());
+    build_fp8_aux_tiles(waux, tiles);
+    s->aux32_bf16.upload(tiles.data(), tiles.size() * 2);
   }
+  std::vector<uint8_t> t8;
+  build_fp8_weight_tiles(prm.layers, t8, s->fp8_exp);
+  s->tiles_fp8.upload(t8.data(), t8.size());
   const int id = ctx->next_id++;
   ctx->scenes[id] = std::move(s);
   *scene_id = id;
@@ -187,6 +256,7 @@ EnvMap &new_env(nwprt_ctx ctx, int n, int k, int *env_id) {
   e->idx.ensure((size_t)k * 4);
   e->strength.ensure((size_t)k * 12);
   hip_check(hipEventCreateWithFlags(&e->ready, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&e->last_use, hipEventDisableTiming), "hipEventCreate");
   const int id = ctx->next_id++;
   EnvMap &ref = *e;
   ctx->envs[id] = std::move(e);
@@ -282,22 +352,8 @@ void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const 
   a.dbg_indices = dbg_idx;
   a.dbg_features = dbg_feat;
   if (sc.desc.aux_dims && dbg_idx == nullptr) {
-    const float *f4[] = {gb->normal, gb->reflect, gb->kd, gb->ks};
-    for (const float *p : f4) {
-      require(p != nullptr, NWPRT_ERR_INVALID_ARGUMENT,
-              "G-buffer normal, reflect, kd and ks are required by this scene's aux inputs");
-      check_aligned(p, 16, "G-buffer aux input");
-    }
-    require(gb->roughness != nullptr, NWPRT_ERR_INVALID_ARGUMENT,
-            "G-buffer roughness is required by this scene's aux inputs");
     slot.aux.ensure((size_t)P * 16 * (elem == Elem::F32 ? 4 : 2));
-    a.normal = gb->normal;
-    a.reflect = gb->reflect;
-    a.roughness = gb->roughness;
-    a.kd = gb->kd;
-    a.ks = gb->ks;
-    a.aux = slot.aux.p;
-    a.aux_pad = sc.desc.aux_pad_value;
+    set_aux_encode(a, sc, gb, slot.aux.p);
   }
   launch_encode(a, s);
   slot.valid = true;
@@ -348,11 +404,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   a.dump = dump;
   a.dump_begin = dump_begin;
   a.dump_count = dump_count;
-  if (sc.desc.aux_dims) {
-    a.aux = slot.aux.p;
-    a.waux_f32 = sc.waux_f32();
-    a.waux_tiles = elem == Elem::BF16 ? sc.aux_bf16.as<uint16_t>() : sc.aux_f16.as<uint16_t>();
-  }
+  if (sc.desc.aux_dims) set_aux_relight(a, sc, prec, slot.aux.p);
   if (elem == Elem::F32) {
     slot.wsel.ensure((size_t)env.k * 32 * 4);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
@@ -362,35 +414,17 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
     a.w_f32 = sc.w_f32.as<float>();
     launch_relight_plain(a, s);
   } else if (prec == NWPRT_PREC_FP8) {
-    require(sc.desc.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
-            "FP8 precision does not support scenes with aux inputs");
-    // Selection tiles: 16x16 (BF16 calibration network) and 32x32 (FP8 form).
-    slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
-    slot.mk32.ensure((size_t)env.k * 4 * 64 * 16);
-    launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
-                             env.idx.as<int32_t>(), env.k, Elem::BF16, slot.mk.as<uint16_t>(),
-                             nullptr, s, slot.mk32.as<uint16_t>(), planar, scene_projection(sc));
-    // Per-layer activation maxima over about 8K pixels spread over the frame.
-    constexpr int kCalibPixels = 8192;
-    const int stride = std::max(1, (P + kCalibPixels - 1) / kCalibPixels);
-    const int n_cal = (P + stride - 1) / stride;
-    slot.calib.ensure((size_t)n_cal * 64);
+    slot.mk.ensure(fp8_mk_bytes(env.k));
+    slot.mk32.ensure(fp8_mk_bytes(env.k));
+    slot.calib.ensure(fp8_calib_bytes(sc, P));
     slot.amax.ensure(16);
-    launch_calib_gather(slot.features.as<uint16_t>(), gb->env_flag, P, stride,
-                        slot.calib.as<uint16_t>(), n_cal, slot.amax.as<float>(), planar, s);
-    RelightArgs c{};
-    c.n_pixels = n_cal;
-    c.k = env.k;
-    c.features = slot.calib.p;
-    c.features_planar = planar;
-    c.mk = slot.mk.as<uint16_t>();
-    c.w_tiles = sc.tiles_bf16.as<uint16_t>();
-    c.amax = slot.amax.as<float>();
-    launch_calibrate_amax(c, s);
-    a.mk = slot.mk32.as<uint16_t>();
-    a.w8_tiles = sc.tiles_fp8.as<uint8_t>();
-    for (int i = 0; i < 3; ++i) a.w8_exp[i] = sc.fp8_exp[i];
-    a.amax = slot.amax.as<float>();
+    Fp8Work w;
+    w.mk16 = slot.mk.as<uint16_t>();
+    w.mk32 = slot.mk32.as<uint16_t>();
+    w.calib = slot.calib.as<uint16_t>();
+    w.amax = slot.amax.as<float>();
+    prepare_fp8(sc, env.idx.as<int32_t>(), env.k, a, w, s);
+    set_fp8_relight(a, sc, w);
     launch_relight_fp8(a, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
   } else {
     slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
@@ -401,6 +435,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
     a.w_tiles = elem == Elem::BF16 ? sc.tiles_bf16.as<uint16_t>() : sc.tiles_f16.as<uint16_t>();
     launch_relight_mfma(a, elem, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
   }
+  hip_check(hipEventRecord(env.last_use, s), "hipEventRecord");
 }
 
 // ------------------------------------------------------------------ batch --
@@ -419,22 +454,36 @@ void check_batch_inputs(nwprt_ctx ctx, const Batch &b, hipStream_t s) {
   for (size_t j = 0; j < b.jobs.size(); ++j) {
     const Scene &sc = get_scene(ctx, b.jobs[j].scene_id);
     EnvMap &env = get_env(ctx, b.jobs[j].env_id);
-    require(scene_tiles(sc, b.prec) == b.tiles_at[j] && env.idx.p == b.idx_at[j] &&
-                env.k == b.k[j],
+    const void *tiles = b.prec == NWPRT_PREC_FP8 ? sc.tiles_fp8.p : scene_tiles(sc, b.prec);
+    require(tiles == b.tiles_at[j] && env.idx.p == b.idx_at[j] && env.k == b.k[j],
             NWPRT_ERR_STATE, "batch: a job's scene or environment map changed since the batch "
                              "was created; create the batch again");
     if (s) hip_check(hipStreamWaitEvent(s, env.ready, 0), "hipStreamWaitEvent");
   }
 }
 
-// Records the batch graph on the context stream: per job the encode and the
-// selection tiles, then the counter reset and the persistent kernel.
+// Work buffers of job j of a batch (FP8 form).
+Fp8Work batch_fp8_work(const Batch &b, size_t j) {
+  Fp8Work w;
+  w.mk16 = (uint16_t *)(b.mk.as<char>() + b.mk_off[j]);
+  w.mk32 = (uint16_t *)(b.mk32.as<char>() + b.mk32_off[j]);
+  w.calib = (uint16_t *)(b.calib.as<char>() + b.calib_off[j]);
+  w.amax = (float *)(b.amax.as<char>() + b.amax_off[j]);
+  return w;
+}
+
+// Records the batch graph on the context stream: per job the standalone
+// encode (jobs the persistent kernel does not encode itself) and the
+// selection tiles -- for the FP8 form also the calibration sample and the
+// activation maxima -- then the counter reset and the persistent kernel.
 void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
-  require(relight_batch_variant_ok(variant), NWPRT_ERR_INVALID_ARGUMENT,
+  const bool fp8 = b.prec == NWPRT_PREC_FP8;
+  require(fp8 || relight_batch_variant_ok(variant), NWPRT_ERR_INVALID_ARGUMENT,
           "batch: tile variant has no persistent form");
   b.drop_graph();
   const hipStream_t s = ctx->stream;
   const Elem elem = elem_of(b.prec);
+  const std::vector<RelightArgs> &args = b.host_args;
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
   try {
@@ -442,19 +491,29 @@ void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
       const nwprt_batch_job &jb = b.jobs[j];
       const Scene &sc = get_scene(ctx, jb.scene_id);
       const EnvMap &env = get_env(ctx, jb.env_id);
-      if (!b.fused[j])
-        launch_encode(encode_args(sc, &jb.gbuffer, b.p[j], elem,
-                                  b.features.as<uint16_t>() + b.feat_off[j]),
-                      s);
-      launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * env.n * env.n,
-                               env.idx.as<int32_t>(), b.k[j], elem,
-                               b.mk.as<uint16_t>() + b.mk_off[j], nullptr, s, nullptr,
-                               planar_features(sc, elem), scene_projection(sc));
+      if (!b.fused[j]) {
+        EncodeArgs e = encode_args(sc, &jb.gbuffer, b.p[j], elem,
+                                   b.features.as<char>() + b.feat_off[j]);
+        if (b.aux[j]) set_aux_encode(e, sc, &jb.gbuffer, b.auxbuf.as<char>() + b.aux_off[j]);
+        launch_encode(e, s);
+      }
+      if (fp8) {
+        prepare_fp8(sc, env.idx.as<int32_t>(), b.k[j], args[j], batch_fp8_work(b, j), s);
+      } else {
+        launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * env.n * env.n,
+                                 env.idx.as<int32_t>(), b.k[j], elem,
+                                 (uint16_t *)(b.mk.as<char>() + b.mk_off[j]), nullptr, s, nullptr,
+                                 planar_features(sc, elem), scene_projection(sc));
+      }
     }
     unsigned int *counter = (unsigned int *)(b.meta.as<int>() + 1);
     launch_reset_counter(counter, s);
-    launch_relight_batch(b.args.as<RelightArgs>(), b.items.as<BatchItem>(), b.meta.as<int>(),
-                         counter, elem, b.softplus, variant, ctx->n_cu, s);
+    if (fp8)
+      launch_relight_fp8_batch(b.args.as<RelightArgs>(), b.items.as<BatchItem>(),
+                               b.meta.as<int>(), counter, b.softplus, b.has_aux, ctx->n_cu, s);
+    else
+      launch_relight_batch(b.args.as<RelightArgs>(), b.items.as<BatchItem>(), b.meta.as<int>(),
+                           counter, elem, b.softplus, b.has_aux, variant, ctx->n_cu, s);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(s, &g);
@@ -464,27 +523,29 @@ void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
   }
   hip_check(hipStreamEndCapture(s, &b.graph), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-  b.variant = variant;
+  b.variant = fp8 ? 0 : variant;
 }
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_precision prec,
                   nwprt_tonemap_source tm, int *batch_id) {
   require(jobs && batch_id && n >= 1 && n <= 65536, NWPRT_ERR_INVALID_ARGUMENT,
           "batch: need 1 to 65536 jobs");
-  require(prec == NWPRT_PREC_BF16 || prec == NWPRT_PREC_FP16, NWPRT_ERR_UNSUPPORTED,
-          "batch: BF16 or FP16 precision only");
+  require(prec == NWPRT_PREC_BF16 || prec == NWPRT_PREC_FP16 || prec == NWPRT_PREC_FP8,
+          NWPRT_ERR_UNSUPPORTED, "batch: BF16, FP16 or FP8 precision only");
   require(tm == NWPRT_TONEMAP_FULL || tm == NWPRT_TONEMAP_INDIRECT || tm == NWPRT_TONEMAP_NONE,
           NWPRT_ERR_INVALID_ARGUMENT, "unknown tonemap source");
+  const bool fp8 = prec == NWPRT_PREC_FP8;
+  const Elem elem = elem_of(prec);
   auto b = std::make_unique<Batch>();
   b->prec = prec;
   b->tm = tm;
-  size_t fo = 0, mo = 0;
+  size_t fo = 0, ao = 0, mo = 0, m32 = 0, co = 0, am = 0;
   for (int j = 0; j < n; ++j) {
     const nwprt_batch_job &jb = jobs[j];
     const Scene &sc = get_scene(ctx, jb.scene_id);
     EnvMap &env = get_env(ctx, jb.env_id);
-    require(sc.desc.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
-            "batch: scenes with aux inputs are not supported");
     require(env.n == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
             "batch: environment map face size does not match the job's scene");
     check_frame(&jb.gbuffer, jb.width, jb.height, true);
@@ -494,39 +555,61 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     require(sp == b->softplus, NWPRT_ERR_UNSUPPORTED,
             "batch: all jobs must share the output activation");
     const int P = jb.width * jb.height;
+    const bool fused = fuse_encode(sc, prec), aux = sc.desc.aux_dims != 0;
+    if (aux) {  // checks the G-buffer's aux inputs
+      EncodeArgs e{};
+      set_aux_encode(e, sc, &jb.gbuffer, nullptr);
+    }
     b->jobs.push_back(jb);
     b->k.push_back(env.k);
     b->p.push_back(P);
-    const bool fused = fuse_encode(sc, prec);
     b->fused.push_back(fused);
+    b->aux.push_back(aux);
+    b->has_aux = b->has_aux || aux;
     b->feat_off.push_back(fo);
+    b->aux_off.push_back(ao);
     b->mk_off.push_back(mo);
-    b->tiles_at.push_back(scene_tiles(sc, prec));
+    b->mk32_off.push_back(m32);
+    b->calib_off.push_back(co);
+    b->amax_off.push_back(am);
+    b->tiles_at.push_back(fp8 ? sc.tiles_fp8.p : scene_tiles(sc, prec));
     b->idx_at.push_back(env.idx.p);
-    if (!fused) fo += (size_t)P * 32;
-    mo += (size_t)env.k * 4 * 64 * 8;
+    if (!fused) fo += align256((size_t)P * 64);
+    if (aux) ao += align256((size_t)P * 32);
+    mo += align256((size_t)env.k * 4 * 64 * 16);
+    if (fp8) {
+      m32 += align256(fp8_mk_bytes(env.k));
+      co += align256(fp8_calib_bytes(sc, P));
+      am += 256;
+    }
     b->n_items += (P + 63) / 64;
     b->n_pairs += (long long)P * env.k;
   }
   require(b->n_items < (1ll << 31), NWPRT_ERR_INVALID_ARGUMENT, "batch: too many pixels");
-  if (fo) b->features.ensure(fo * 2);
-  b->mk.ensure(mo * 2);
+  if (fo) b->features.ensure(fo);
+  if (ao) b->auxbuf.ensure(ao);
+  b->mk.ensure(mo);
+  if (fp8) {
+    b->mk32.ensure(m32);
+    b->calib.ensure(co);
+    b->amax.ensure(am);
+  }
   // Device job table.
-  std::vector<RelightArgs> args(n);
+  b->host_args.assign(n, RelightArgs{});
   for (int j = 0; j < n; ++j) {
     const nwprt_batch_job &jb = b->jobs[j];
     const Scene &sc = get_scene(ctx, jb.scene_id);
     const EnvMap &env = get_env(ctx, jb.env_id);
-    RelightArgs &a = args[j];
-    a = RelightArgs{};
+    RelightArgs &a = b->host_args[j];
     a.n_pixels = b->p[j];
     a.k = b->k[j];
-    a.features = b->features.as<uint16_t>() + b->feat_off[j];
-    a.features_planar = planar_features(sc, elem_of(prec));
+    a.features = b->fused[j] ? nullptr : b->features.as<char>() + b->feat_off[j];
+    a.features_planar = planar_features(sc, elem);
     if (b->fused[j]) set_fused_encode(a, sc, &jb.gbuffer);
-    a.mk = b->mk.as<uint16_t>() + b->mk_off[j];
+    if (b->aux[j]) set_aux_relight(a, sc, prec, b->auxbuf.as<char>() + b->aux_off[j]);
+    a.mk = (const uint16_t *)(b->mk.as<char>() + b->mk_off[j]);
     a.light = env.strength.as<float>();
-    a.w_tiles = (const uint16_t *)scene_tiles(sc, prec);
+    a.w_tiles = (const uint16_t *)scene_tiles(sc, fp8 ? NWPRT_PREC_BF16 : prec);
     a.env_flag = jb.gbuffer.env_flag;
     a.direct = jb.gbuffer.direct;
     a.indirect = jb.out.indirect;
@@ -535,7 +618,16 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
     a.tonemap_source = (int)tm;
     a.softplus = b->softplus;
   }
-  b->args.upload(args.data(), args.size() * sizeof(RelightArgs));
+  if (fp8) {
+    // The preparation reads each job's arguments as they are here (features
+    // and fused-encode fields); the job table gets the FP8 fields.
+    std::vector<RelightArgs> dev_args = b->host_args;
+    for (int j = 0; j < n; ++j)
+      set_fp8_relight(dev_args[j], get_scene(ctx, b->jobs[j].scene_id), batch_fp8_work(*b, j));
+    b->args.upload(dev_args.data(), dev_args.size() * sizeof(RelightArgs));
+  } else {
+    b->args.upload(b->host_args.data(), b->host_args.size() * sizeof(RelightArgs));
+  }
   // Work items: jobs in decreasing K, so the costliest items are taken
   // first; jobs of equal K grouped by scene (one hash grid in the caches at
   // a time), then in job order; each job's 64-pixel groups in pixel order.
@@ -647,7 +739,8 @@ nwprt_status nwprt_scene_load_folder(nwprt_ctx ctx, const char *model_dir, int32
 nwprt_status nwprt_scene_destroy(nwprt_ctx ctx, int32_t scene_id) {
   return guarded([&] {
     bind(ctx);
-    get_scene(ctx, scene_id);
+    require(get_scene(ctx, scene_id).server_refs == 0, NWPRT_ERR_STATE,
+            "scene is used by a queued or running server request");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     ctx->scenes.erase(scene_id);
     for (auto &sl : ctx->slots)
@@ -679,8 +772,11 @@ nwprt_status nwprt_envmap_update_faces(nwprt_ctx ctx, int32_t env_id, const floa
     bind(ctx);
     EnvMap &e = get_env(ctx, env_id);
     require(e.has_coef, NWPRT_ERR_STATE, "environment map was not built from faces");
+    require(e.server_refs == 0, NWPRT_ERR_STATE,
+            "environment map is used by a queued or running server request");
     require(faces != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null faces");
     const hipStream_t s = pick(ctx, stream);
+    hip_check(hipStreamWaitEvent(s, e.last_use, 0), "hipStreamWaitEvent");
     const size_t m = 6ull * e.n * e.n;
     const float *dfaces = faces;
     if (faces_location == NWPRT_HOST) {
@@ -705,9 +801,12 @@ nwprt_status nwprt_envmap_update_rotated(nwprt_ctx ctx, int32_t env_id, const fl
     bind(ctx);
     EnvMap &e = get_env(ctx, env_id);
     require(e.has_coef, NWPRT_ERR_STATE, "environment map was not built from faces");
+    require(e.server_refs == 0, NWPRT_ERR_STATE,
+            "environment map is used by a queued or running server request");
     require(faces != nullptr, NWPRT_ERR_INVALID_ARGUMENT, "null faces");
     require(std::isfinite(yaw), NWPRT_ERR_INVALID_ARGUMENT, "yaw must be finite");
     const hipStream_t s = pick(ctx, stream);
+    hip_check(hipStreamWaitEvent(s, e.last_use, 0), "hipStreamWaitEvent");
     const size_t m = 6ull * e.n * e.n;
     e.rotated.ensure(m * 12);
     launch_rotate_cubemap(faces, e.rotated.as<float>(), e.n, yaw, s);
@@ -762,7 +861,8 @@ nwprt_status nwprt_envmap_load_folder(nwprt_ctx ctx, const char *folder, const c
 nwprt_status nwprt_envmap_destroy(nwprt_ctx ctx, int32_t env_id) {
   return guarded([&] {
     bind(ctx);
-    get_env(ctx, env_id);
+    require(get_env(ctx, env_id).server_refs == 0, NWPRT_ERR_STATE,
+            "environment map is used by a queued or running server request");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     ctx->envs.erase(env_id);
   });
@@ -1057,6 +1157,8 @@ nwprt_status nwprt_batch_launch(nwprt_ctx ctx, int32_t batch_id, void *stream) {
     const hipStream_t s = pick(ctx, stream);
     check_batch_inputs(ctx, b, s);
     hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
+    for (const nwprt_batch_job &jb : b.jobs)
+      hip_check(hipEventRecord(get_env(ctx, jb.env_id).last_use, s), "hipEventRecord");
   });
 }
 
@@ -1075,6 +1177,7 @@ nwprt_status nwprt_batch_autotune(nwprt_ctx ctx, int32_t batch_id, int32_t *best
     try {
       for (int v = 0; v < relight_mfma_variant_count(); ++v) {
         if (!relight_batch_variant_ok(v)) continue;
+        if (b.prec == NWPRT_PREC_FP8 && v > 0) break;  // one persistent FP8 form
         capture_batch(ctx, b, v);
         hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
         float t = 0.f;

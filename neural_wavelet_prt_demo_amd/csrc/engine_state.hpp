@@ -50,9 +50,13 @@ struct Scene {
   nwprt_scene_desc desc{};
   nwprt_hashgrid_levels lv{};
   DevBuf hash, wfeat, proj, w_f32, tiles_bf16, tiles_f16, aux_bf16, aux_f16;
-  // FP8 form: E4M3 tiles of layers 2..4 and their power-of-two exponents.
-  DevBuf tiles_fp8;
+  // FP8 form: E4M3 tiles of layers 2..4 and their power-of-two exponents;
+  // with aux inputs, the aux columns as BF16 32x32x16 tiles.
+  DevBuf tiles_fp8, aux32_bf16;
   int fp8_exp[3] = {0, 0, 0};
+  // Queued or running server requests that use this scene: destroy refuses
+  // while any do.
+  int server_refs = 0;
   // FP32 aux columns of the second layer ([64][16]) inside w_f32, or null.
   const float *waux_f32() const {
     return desc.aux_dims ? w_f32.as<float>() + (scene_layer_weight_count(desc) - 64 * 16) : nullptr;
@@ -67,9 +71,16 @@ struct EnvMap {
   // (per map, so updates of different maps on different streams never share
   // scratch memory).
   DevBuf scores, rotated;
-  hipEvent_t ready = nullptr;
+  // ready: the selection is written (readers wait on it); last_use: the last
+  // relight or batch launch that reads it was queued (in-place updates wait
+  // on it, so a reader on another stream never sees a half-written one).
+  hipEvent_t ready = nullptr, last_use = nullptr;
+  // Queued or running server requests that read this map: destroy and
+  // in-place updates refuse while any do.
+  int server_refs = 0;
   ~EnvMap() {
     if (ready) (void)hipEventDestroy(ready);
+    if (last_use) (void)hipEventDestroy(last_use);
   }
 };
 
@@ -94,7 +105,8 @@ struct Mesh {
 // 2: nwprt_relight / debug taps).
 struct FrameSlot {
   DevBuf features, aux, mk, wsel;
-  // FP8 form: first-layer 32x32 tiles, calibration subsample, activation maxima.
+  // FP8 form: first-layer 32x32 tiles, calibration subsample (features, then
+  // aux inputs), activation maxima.
   DevBuf mk32, calib, amax;
   bool valid = false;
   int scene = -1, n_pixels = 0;
@@ -111,11 +123,19 @@ struct Batch {
   std::vector<nwprt_batch_job> jobs;
   std::vector<int> k, p;
   std::vector<char> fused;  // job encodes inside the persistent kernel
-  std::vector<size_t> feat_off, mk_off;  // in elements of 2 bytes
+  std::vector<char> aux;    // job's scene has aux inputs
+  // Per-job offsets in bytes: features, aux inputs, selection tiles, and for
+  // the FP8 form the 32x32 tiles, calibration sample and maxima.
+  std::vector<size_t> feat_off, aux_off, mk_off, mk32_off, calib_off, amax_off;
+  bool has_aux = false;
   // What the graph captured of each job's scene and map: a launch checks
   // that these buffers are still the ones in use.
   std::vector<const void *> tiles_at, idx_at;
-  DevBuf features, mk, args, items, meta;  // meta: int n_items | uint counter
+  DevBuf features, auxbuf, mk, mk32, calib, amax, args, items,
+      meta;  // meta: int n_items | uint counter
+  // Host copy of the job table before the FP8 fields (the preparation reads
+  // each job's features and fused-encode fields from it).
+  std::vector<RelightArgs> host_args;
   long long n_items = 0, n_pairs = 0;
   int variant = 5;
   hipGraph_t graph = nullptr;
@@ -200,5 +220,26 @@ const float *scene_projection(const Scene &sc);
 // fused kernel (MFMA forms, planar features), and the arguments that do it.
 bool fuse_encode(const Scene &sc, nwprt_precision prec);
 void set_fused_encode(RelightArgs &a, const Scene &sc, const nwprt_gbuffer *gb);
+// Aux encodings of a scene with aux inputs: checks that the G-buffer holds
+// them and points the encode at `aux` ([P][16] of the encode's element).
+void set_aux_encode(EncodeArgs &e, const Scene &sc, const nwprt_gbuffer *gb, void *aux);
+// Aux fields of a relight (features' aux inputs and the scene's aux tiles).
+void set_aux_relight(RelightArgs &a, const Scene &sc, nwprt_precision prec, const void *aux);
+
+// FP8 form of one relight: its work buffers, the per-relight preparation
+// (selection tiles of both shapes, calibration sample -- encoded from the
+// G-buffer for a fused relight, else gathered from a.features / a.aux --
+// and activation maxima, on stream s), and its relight arguments.
+struct Fp8Work {
+  uint16_t *mk16 = nullptr;      // [K][4][64][8] BF16 16x16x32 tiles (calibration network)
+  uint16_t *mk32 = nullptr;      // [K][2][2][64][8] BF16 32x32x16 tiles (FP8 first layer)
+  uint16_t *calib = nullptr;     // calibration features, then (aux scenes) aux inputs
+  float *amax = nullptr;         // 4 floats
+};
+size_t fp8_mk_bytes(int k);
+size_t fp8_calib_bytes(const Scene &sc, int n_pixels);
+void prepare_fp8(const Scene &sc, const int32_t *idx, int k, const RelightArgs &a,
+                 const Fp8Work &w, hipStream_t s);
+void set_fp8_relight(RelightArgs &a, const Scene &sc, const Fp8Work &w);
 
 }  // namespace nwprt

@@ -115,10 +115,13 @@ struct RelightArgs {
   float *dump;
   int dump_begin, dump_count;
   // FP8 form: layers 2..4 as E4M3 A tiles [3][2][64][32] (W * 2^w8_exp per
-  // layer) and the calibrated activation maxima of layers 1..3 (device).
+  // layer), the calibrated activation maxima of layers 1..3 (device), and
+  // with aux inputs the second layer's aux columns as BF16 32x32x16 A tiles
+  // [2][64][8].
   const uint8_t *w8_tiles;
   int w8_exp[3];
   float *amax;
+  const uint16_t *waux32_tiles;
   // Fused encode (one-call MFMA relights of 16 x 2 grids without aux
   // inputs): each wave computes its 64 pixels' hash-grid features from
   // `position` in the planar slot order instead of reading `features`.
@@ -251,11 +254,11 @@ struct BatchItem {
 bool relight_batch_variant_ok(int variant);
 // Waves take items [0, *n_items) in order from *counter, which must be 0 at
 // launch (launch_reset_counter); the grid is n_cu x the variant's waves per
-// SIMD workgroups.  Jobs must not have aux inputs or a transport tap, and
-// all use `softplus` as their output activation.
+// SIMD workgroups.  Jobs have no transport tap and all use `softplus` as
+// their output activation; has_aux: some job has aux inputs.
 void launch_relight_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
-                          unsigned int *counter, Elem elem, bool softplus, int variant,
-                          int n_cu, hipStream_t s);
+                          unsigned int *counter, Elem elem, bool softplus, bool has_aux,
+                          int variant, int n_cu, hipStream_t s);
 void launch_reset_counter(unsigned int *counter, hipStream_t s);
 
 // Host-side MFMA weight tiling of layers 2..4 and of the second layer's aux
@@ -267,18 +270,41 @@ void build_aux_tiles(const float *waux, Elem elem, std::vector<uint16_t> &out);
 // with one power-of-two scale per layer (amax mapped into (224, 448]);
 // `exps` receives the three exponents.
 void build_fp8_weight_tiles(const float *w_f32, std::vector<uint8_t> &out, int exps[3]);
+// Calibration subsample of a frame of n_pixels: calibration pixel q is frame
+// pixel q * stride, about kFp8CalibPixels of them.
+constexpr int kFp8CalibPixels = 16384;
+inline int fp8_calib_stride(int n_pixels) {
+  return n_pixels <= kFp8CalibPixels ? 1 : (n_pixels + kFp8CalibPixels - 1) / kFp8CalibPixels;
+}
+inline int fp8_calib_count(int n_pixels) {
+  const int s = fp8_calib_stride(n_pixels);
+  return (n_pixels + s - 1) / s;
+}
 // Activation calibration: copies every `stride`-th pixel's BF16 features
-// (zeros for background pixels) into `calib` [n_cal][32] and clears amax[4].
-void launch_calib_gather(const uint16_t *features, const uint8_t *env_flag, int n_pixels,
-                         int stride, uint16_t *calib, int n_cal, float *amax, bool planar,
-                         hipStream_t s);
+// (zeros for background pixels) into `calib` [n_cal][32] (planar: the planar
+// layout with n_cal pixels), and with `aux` its aux inputs into `calib_aux`
+// [n_cal][16]; clears amax[4].
+void launch_calib_gather(const uint16_t *features, const uint16_t *aux, const uint8_t *env_flag,
+                         int n_pixels, int stride, uint16_t *calib, uint16_t *calib_aux,
+                         int n_cal, float *amax, bool planar, hipStream_t s);
+// The same calibration sample encoded from the G-buffer positions (`enc`:
+// the fused-encode fields of a relight, its env_flag and n_pixels), planar
+// BF16 features bit-identical to the standalone encode's; clears amax[4].
+void launch_calib_encode(const RelightArgs &enc, int stride, uint16_t *calib, int n_cal,
+                         float *amax, hipStream_t s);
 // Maxima of the layer 1..3 activations of the BF16 network over the
 // calibration pixels and all K wavelets (`a`: BF16 MFMA operands, features =
-// calib, n_pixels = n_cal) into a.amax[0..2].
+// calib, aux = calib_aux with waux_tiles, n_pixels = n_cal) into a.amax[0..2].
 void launch_calibrate_amax(const RelightArgs &a, hipStream_t s);
 int relight_fp8_variant_count();
 const char *relight_fp8_variant_name(int variant);
 void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s);
+// Persistent FP8 batch form (items and counter as launch_relight_batch);
+// has_aux: some job has aux inputs.
+void launch_relight_fp8_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
+                              unsigned int *counter, bool softplus, bool has_aux, int n_cu,
+                              hipStream_t s);
+void build_fp8_aux_tiles(const float *waux, std::vector<uint16_t> &out);
 // OCP E4M3 encode (round to nearest even, saturating at +-448) / decode.
 uint8_t f32_to_e4m3(float x);
 float e4m3_to_f32(uint8_t c);

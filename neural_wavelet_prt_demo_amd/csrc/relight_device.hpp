@@ -3,7 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.hpp"
+#include "hashgrid_device.hpp"
 #include "kernels.hpp"
 
 namespace nwprt {
@@ -79,6 +82,62 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
   const float t = s + y;
   c = (t - s) - y;
   s = t;
+}
+
+// Levels l0 .. l0 + NL - 1 of one pixel, packed into the planar slot order
+// (u32 2q = level q, 2q + 1 = level 15 - q).
+template <typename E, int NL, bool HF16>
+__device__ __forceinline__ void encode_batch(const RelightArgs &a, int l0, const float (&x)[3],
+                                             uint32_t (&pk)[16]) {
+  float v[NL][2];
+  gather_levels<NL, HF16>(a.lv, a.hash, l0, x, v);
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int l = l0 + i;
+    pk[l < 8 ? 2 * l : 2 * (15 - l) + 1] = std::is_same<E, _Float16>::value
+                                                ? pack_f16_rounded(v[i][0], v[i][1])
+                                                : pack_bf16(v[i][0], v[i][1]);
+  }
+}
+
+// Fused encode of a wave's 64 pixels: lane l computes the 16 levels of
+// pixel my_px (level parameters uniform across the wave, as in the
+// standalone encode, with the same lookup code), packs each level's two
+// features in the planar slot order and stores the 16 words to its 64 B row
+// of `lds` ([64 pixels][16 words]).  Inactive lanes (past the frame or not
+// hit) store zeros.  On return every lane may read any row: the stores are
+// ordered before the caller's loads by a wavefront fence (no workgroup
+// barrier, so waves that exit early are harmless).
+template <typename E>
+__device__ __forceinline__ void fused_encode_lds(const RelightArgs &a, bool active, int my_px,
+                                                 int lane, uint32_t *lds) {
+  uint32_t pk[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) pk[i] = 0u;
+  if (active) {
+    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)my_px);
+    const float x[3] = {x4.x, x4.y, x4.z};
+    // Four levels per batch: 32 gathers in flight before the first is used
+    // (FP16 entries; two levels of FP32 entries per half-batch).  The table
+    // type is tested per batch: that branch also keeps the compiler from
+    // hoisting the address math of all 16 levels at once (which spills).
+#pragma unroll
+    for (int l0 = 0; l0 < 16; l0 += 4) {
+      if (a.hash_f16) {
+        encode_batch<E, 4, true>(a, l0, x, pk);
+      } else {
+        encode_batch<E, 2, false>(a, l0, x, pk);
+        encode_batch<E, 2, false>(a, l0 + 2, x, pk);
+      }
+    }
+  }
+  u32x4 *row = (u32x4 *)(lds + lane * 16);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) row[c] = u32x4{pk[4 * c], pk[4 * c + 1], pk[4 * c + 2], pk[4 * c + 3]};
+  // Lanes of one wave exchange through LDS: order the stores before the loads.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 }  // namespace dev

@@ -20,9 +20,11 @@
 //
 // Scaling.  Weights: W * 2^ew quantized, ew per layer from the weight amax.
 // Activations: the maxima of the BF16 network's layer 1..3 outputs over a
-// pixel subsample (calib_amax_kernel) give ea per layer, with amax * 2^ea in
-// (112, 224]: half of E4M3's 448 in reserve, and v_med3 saturates anything
-// beyond 448 instead of letting it round to NaN.  Layer 1's 2^ea1 is applied
+// pixel subsample of about 16K pixels (calib_amax_kernel) give ea per layer,
+// with amax * 2^ea in (56, 112]: activations up to 4x the sampled maximum
+// still convert exactly, and v_med3 saturates anything beyond 448 instead of
+// letting it round to NaN.  (Only values below 2^-6 -- under 3e-4 of the
+// sampled maximum -- lose bits to the reserve, as E4M3 subnormals.)  Layer 1's 2^ea1 is applied
 // to the per-pixel features once; layer n's MFMA scales (A: 2^(ea_n - ew_n),
 // B: 2^-ea_(n-1)) leave its output already multiplied by 2^ea_n, so the ReLU
 // + convert step is v_med3 + v_cvt_pk_fp8_f32 and nothing else.  Every scale
@@ -33,6 +35,7 @@
 #include <cmath>
 
 #include "device_common.hpp"
+#include "hashgrid_device.hpp"
 #include "kernels.hpp"
 #include "relight_device.hpp"
 
@@ -76,7 +79,7 @@ __device__ __forceinline__ i32x8 relu_fp8(const f32x16 &lo, const f32x16 &hi) {
   return x;
 }
 
-// Exponent e with amax * 2^e in (112, 224], from the float's bits (so it
+// Exponent e with amax * 2^e in (56, 112], from the float's bits (so it
 // moves by exactly -s when amax is multiplied by 2^s).  0 for a zero,
 // denormal or non-finite maximum; clamped to [-100, 100].
 __device__ __forceinline__ int act_exp(float amax) {
@@ -84,7 +87,7 @@ __device__ __forceinline__ int act_exp(float amax) {
   const int be = (int)((u >> 23) & 0xffu);
   if (be == 0 || be == 255 || (u >> 31)) return 0;
   const int E = be - 127;
-  const int e = (u & 0x7fffffu) <= 0x600000u ? 7 - E : 6 - E;
+  const int e = (u & 0x7fffffu) <= 0x600000u ? 6 - E : 5 - E;
   return min(max(e, -100), 100);
 }
 
@@ -102,21 +105,21 @@ __device__ __forceinline__ u32x4 scale_bf16x8(u32x4 v, float s) {
   return r;
 }
 
-// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
-// pixels [64 (4 b + w), +64) as two 32-column groups; after the output layer
-// lane l holds pixel base + l.  The next wavelet's first-layer tiles load
-// while the current one computes.
-template <bool SOFTPLUS, bool DUMP>
-__global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
-  // Output-layer variants in LDS (read once per wavelet), freeing 16 VGPRs.
-  __shared__ i32x8 w4s[2 * 64];
-  if (threadIdx.x < 2 * 64) w4s[threadIdx.x] = ((const i32x8 *)a.w8_tiles)[2 * 2 * 64 + threadIdx.x];
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// One wave's 64 pixels [base, base + 64) of a frame (base < a.n_pixels), as
+// two 32-column groups; after the output layer lane l holds pixel base + l.
+// The next wavelet's first-layer tiles load while the current one computes.
+// `w4s`: the wave's own copy of the output layer's two tiles (lane l reads
+// only entries l and 64 + l, which it stored itself).  AUX: the second
+// layer's 16 aux columns enter as a BF16 32x32x16 MFMA accumulated onto the
+// FP8 layer's result (the last link of its chain); its B operand, the
+// pixel's aux inputs, is pre-multiplied by 2^ea2, so it adds in the scaled
+// domain of that result.  FUSE: the fused-encode path is compiled in (taken
+// when a.fused_encode).
+template <bool SOFTPLUS, bool DUMP, bool AUX, bool FUSE>
+__device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base, int lane,
+                                                 const i32x8 *w4s, uint32_t *feat_lds) {
   const int h = lane >> 5, col = lane & 31;
-  const int base = (blockIdx.x * 4 + wave) * 64;
   const int P = a.n_pixels;
-  if (base >= P) return;
   const int my_px = base + lane;
   const bool mine = my_px < P;
   const bool hit = mine && (a.env_flag == nullptr || a.env_flag[my_px] != 0);
@@ -132,13 +135,35 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
 
     // First-layer B operands: group g, column col, slots 16 s + 8 h + [0, 8).
     u32x4 f[2][2];
+    if (FUSE && a.fused_encode) {
+      dev::fused_encode_lds<__bf16>(a, hit, my_px, lane, feat_lds);
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          f[g][s] = scale_bf16x8(*(const u32x4 *)(feat_lds + (32 * g + col) * 16 + 8 * s + 4 * h),
+                                 fs);
+    } else {
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const int p = base + 32 * g + col;
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+          f[g][s] = p < P ? scale_bf16x8(dev::load_features8(a, P, p, 16 * s + 8 * h), fs)
+                          : u32x4{0u, 0u, 0u, 0u};
+      }
+    }
+    // Aux inputs (B: group g, column col, inputs 8 h .. 8 h + 7, times
+    // 2^ea2) and the second layer's aux columns (A tiles [2][64]).
+    u32x4 auxb[2], waux[2];
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       const int p = base + 32 * g + col;
-#pragma unroll
-      for (int s = 0; s < 2; ++s)
-        f[g][s] = p < P ? scale_bf16x8(dev::load_features8(a, P, p, 16 * s + 8 * h), fs)
-                        : u32x4{0u, 0u, 0u, 0u};
+      auxb[g] = AUX && p < P
+                    ? scale_bf16x8(*(const u32x4 *)((const uint16_t *)a.aux + (size_t)p * 16 + 8 * h),
+                                   __uint_as_float((uint32_t)(ea2 + 127) << 23))
+                    : u32x4{0u, 0u, 0u, 0u};
+      waux[g] = AUX ? ((const u32x4 *)a.waux32_tiles)[g * 64 + lane] : u32x4{0u, 0u, 0u, 0u};
     }
     const i32x8 *w8 = (const i32x8 *)a.w8_tiles;
     i32x8 w2[2], w3[2];
@@ -169,7 +194,10 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
 #pragma unroll
       for (int g = 0; g < 2; ++g)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) d[g][b] = mfma_fp8(w2[b], x[g], z, sa2, sb2);
+        for (int b = 0; b < 2; ++b) {
+          d[g][b] = mfma_fp8(w2[b], x[g], z, sa2, sb2);
+          if (AUX) d[g][b] = mfma_bf16(waux[b], auxb[g], d[g][b]);
+        }
 #pragma unroll
       for (int g = 0; g < 2; ++g) x[g] = relu_fp8(d[g][0], d[g][1]);
 #pragma unroll
@@ -222,6 +250,61 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
     constexpr float f = dev::LightTerm<SOFTPLUS>::kSumScale;
     dev::write_outputs(a, my_px, hit ? (sum[0] - comp[0]) * f : 0.f,
                        hit ? (sum[1] - comp[1]) * f : 0.f, hit ? (sum[2] - comp[2]) * f : 0.f);
+  }
+}
+
+// The wave's copy of the output layer's two tiles (lane l: entries l, 64 + l).
+__device__ __forceinline__ void load_w4(const RelightArgs &a, i32x8 *w4s, int lane) {
+  const i32x8 *src = (const i32x8 *)a.w8_tiles + 2 * 2 * 64;
+  w4s[lane] = src[lane];
+  w4s[64 + lane] = src[64 + lane];
+}
+
+// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
+// pixels [64 (4 b + w), +64).
+template <bool SOFTPLUS, bool DUMP, bool AUX>
+__global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
+  __shared__ i32x8 w4s[4][2 * 64];
+  __shared__ uint32_t feat_lds[AUX ? 1 : 4 * 64 * 16];  // fused encode: [wave][pixel][16]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int base = (blockIdx.x * 4 + wave) * 64;
+  if (base >= a.n_pixels) return;
+  load_w4(a, w4s[wave], lane);
+  relight_fp8_wave<SOFTPLUS, DUMP, AUX, !AUX>(a, base, lane, w4s[wave],
+                                              feat_lds + (AUX ? 0 : wave * 64 * 16));
+}
+
+// Persistent batch form (as relight_batch_kernel): a machine-sized grid
+// whose waves take (job, 64-pixel group) items from a shared counter.  Each
+// item reloads its job's output-layer tiles into the wave's own LDS copy.
+// HAS_AUX: jobs with aux inputs take the aux path (their features come from
+// the standalone encode); the others use the fused encode.
+template <bool SOFTPLUS, bool HAS_AUX>
+__global__ __launch_bounds__(kBlock, 2) void relight_fp8_batch_kernel(
+    const RelightArgs *__restrict__ jobs, const BatchItem *__restrict__ items,
+    const int *__restrict__ n_items_p, unsigned int *__restrict__ counter) {
+  __shared__ i32x8 w4s[4][2 * 64];
+  __shared__ uint32_t feat_lds[4 * 64 * 16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t *my_lds = feat_lds + wave * 64 * 16;
+  const int n_items = *n_items_p;
+  auto fetch = [&]() {
+    int it = 0;
+    if (lane == 0) it = (int)atomicAdd(counter, 1u);
+    return __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+  };
+  int next = fetch();
+  for (;;) {
+    const int it = next;
+    if (it >= n_items) break;
+    next = fetch();
+    const BatchItem item = items[it];
+    const RelightArgs a = jobs[item.job];
+    load_w4(a, w4s[wave], lane);
+    if (HAS_AUX && a.aux)
+      relight_fp8_wave<SOFTPLUS, false, true, false>(a, item.base, lane, w4s[wave], my_lds);
+    else
+      relight_fp8_wave<SOFTPLUS, false, false, true>(a, item.base, lane, w4s[wave], my_lds);
   }
 }
 
@@ -307,21 +390,71 @@ const char *relight_fp8_variant_name(int v) {
 void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s) {
   require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: bad pixel count or K");
-  require(a.features && a.mk && a.light && a.w8_tiles && a.amax && a.direct,
+  require((a.fused_encode ? a.position && a.hash : a.features != nullptr) && a.mk && a.light &&
+              a.w8_tiles && a.amax && a.direct,
           NWPRT_ERR_INVALID_ARGUMENT, "relight: missing FP8 operand");
-  require(a.aux == nullptr, NWPRT_ERR_UNSUPPORTED,
-          "relight: the FP8 form does not support scenes with aux inputs");
+  require(!a.aux || (a.waux32_tiles && !a.fused_encode), NWPRT_ERR_INVALID_ARGUMENT,
+          "relight: missing FP8 aux weights");
   require(variant >= 0 && variant < kNumFp8Variants, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: unknown FP8 kernel variant");
   const dim3 grid((a.n_pixels + kBlock - 1) / kBlock), block(kBlock);
+  const bool aux = a.aux != nullptr;
+#define NWPRT_FP8_LAUNCH(SP, DUMP, AUX) \
+  hipLaunchKernelGGL((relight_fp8_kernel<SP, DUMP, AUX>), grid, block, 0, s, a)
   if (a.softplus) {
-    if (a.dump) hipLaunchKernelGGL((relight_fp8_kernel<true, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((relight_fp8_kernel<true, false>), grid, block, 0, s, a);
+    if (aux) {
+      if (a.dump) NWPRT_FP8_LAUNCH(true, true, true);
+      else NWPRT_FP8_LAUNCH(true, false, true);
+    } else {
+      if (a.dump) NWPRT_FP8_LAUNCH(true, true, false);
+      else NWPRT_FP8_LAUNCH(true, false, false);
+    }
   } else {
-    if (a.dump) hipLaunchKernelGGL((relight_fp8_kernel<false, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((relight_fp8_kernel<false, false>), grid, block, 0, s, a);
+    if (aux) {
+      if (a.dump) NWPRT_FP8_LAUNCH(false, true, true);
+      else NWPRT_FP8_LAUNCH(false, false, true);
+    } else {
+      if (a.dump) NWPRT_FP8_LAUNCH(false, true, false);
+      else NWPRT_FP8_LAUNCH(false, false, false);
+    }
   }
+#undef NWPRT_FP8_LAUNCH
   hip_check(hipGetLastError(), "relight_fp8_kernel launch");
+}
+
+void launch_relight_fp8_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
+                              unsigned int *counter, bool softplus, bool has_aux, int n_cu,
+                              hipStream_t s) {
+  require(jobs && items && n_items && counter && n_cu >= 1, NWPRT_ERR_INVALID_ARGUMENT,
+          "batch: missing operand");
+  const dim3 grid(n_cu * 2), block(kBlock);
+  if (softplus) {
+    if (has_aux)
+      hipLaunchKernelGGL((relight_fp8_batch_kernel<true, true>), grid, block, 0, s, jobs, items,
+                         n_items, counter);
+    else
+      hipLaunchKernelGGL((relight_fp8_batch_kernel<true, false>), grid, block, 0, s, jobs, items,
+                         n_items, counter);
+  } else {
+    if (has_aux)
+      hipLaunchKernelGGL((relight_fp8_batch_kernel<false, true>), grid, block, 0, s, jobs, items,
+                         n_items, counter);
+    else
+      hipLaunchKernelGGL((relight_fp8_batch_kernel<false, false>), grid, block, 0, s, jobs, items,
+                         n_items, counter);
+  }
+  hip_check(hipGetLastError(), "relight_fp8_batch_kernel launch");
+}
+
+// A tiles of the second layer's 16 aux columns for the FP8 form's BF16
+// v_mfma_f32_32x32x16: tile [b][lane][s] = Waux[32 b + lane%32][8 (lane/32) + s].
+void build_fp8_aux_tiles(const float *waux, std::vector<uint16_t> &out) {
+  out.assign(2 * 64 * 8, 0);
+  for (int b = 0; b < 2; ++b)
+    for (int lane = 0; lane < 64; ++lane)
+      for (int s = 0; s < 8; ++s)
+        out[((size_t)b * 64 + lane) * 8 + s] =
+            f32_to_bf16_bits(waux[(32 * b + (lane & 31)) * 16 + 8 * (lane >> 5) + s]);
 }
 
 }  // namespace nwprt

@@ -144,9 +144,10 @@ __device__ __forceinline__ f32x4 wavelet_step(const u32x4 (&a1)[4], const u32x4 
     for (int j = 0; j < GP; ++j)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        // Second layer; with aux inputs its 16 extra columns enter first.
-        const f32x4 c0 = AUX ? Mfma16<E>::run(waux[b], auxb[jp + j], zero) : zero;
-        c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], c0));
+        // Second layer; with aux inputs its 16 extra columns enter last, as
+        // one more link of the same accumulation chain.
+        c[j][b] = Mfma<E>::run(w2[b][1], x[j][1], Mfma<E>::run(w2[b][0], x[j][0], zero));
+        if (AUX) c[j][b] = Mfma16<E>::run(waux[b], auxb[jp + j], c[j][b]);
       }
 #pragma unroll
     for (int j = 0; j < GP; ++j) {
@@ -214,12 +215,12 @@ __device__ __forceinline__ f32x4 wavelet_step_pipelined(
   auto M2 = [&](int g) {
     f32x4 t[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const f32x4 c0 = AUX ? Mfma16<E>::run(waux[b], auxb[g], zero) : zero;
-      t[b] = Mfma<E>::run(w2[b][0], x[g][0], c0);
-    }
+    for (int b = 0; b < 4; ++b) t[b] = Mfma<E>::run(w2[b][0], x[g][0], zero);
 #pragma unroll
-    for (int b = 0; b < 4; ++b) c[g][b] = Mfma<E>::run(w2[b][1], x[g][1], t[b]);
+    for (int b = 0; b < 4; ++b) {
+      c[g][b] = Mfma<E>::run(w2[b][1], x[g][1], t[b]);
+      if (AUX) c[g][b] = Mfma16<E>::run(waux[b], auxb[g], c[g][b]);
+    }
   };
   auto M3 = [&](int g) {
     f32x4 t[4];
@@ -255,71 +256,26 @@ __device__ __forceinline__ f32x4 wavelet_step_pipelined(
   return out;
 }
 
+// Fused encode of a wave's 64 pixels into the planar B operands: each lane
+// encodes its own pixel into the wave's LDS rows (dev::fused_encode_lds);
+// lane (g, n) then reads slots 4g .. 4g + 3 of pixel 16 j + n for group j --
+// the words load_features8 reads from the planar feature planes.  Pixels
+// that are past the frame or not hit get zero features.
+template <typename E>
+__device__ __forceinline__ void fused_features(const RelightArgs &a, bool active, int my_px,
+                                               int lane, int g, int n, uint32_t *lds,
+                                               u32x4 (&hb)[4]) {
+  dev::fused_encode_lds<E>(a, active, my_px, lane, lds);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) hb[j] = *(const u32x4 *)(lds + (16 * j + n) * 16 + 4 * g);
+}
+
 // One wave's 64 pixels [base, base + 64) of a frame (base < a.n_pixels).
 // Pixel groups j = 0..3 (16 pixels each) are the MFMA columns; lane l owns
 // pixel base + l for the reduction.  The next wavelet's first-layer tiles
 // are fetched while the current one computes (two register sets, K loop
 // unrolled by two).  GP = 0 selects the software-pipelined step; W4LDS reads
 // the output layer's tiles from `w4s` (filled by the caller).
-// Fused encode of a wave's 64 pixels into the planar B operands.  Lane l
-// computes the 16 levels of pixel base + l (level parameters uniform across
-// the wave, as in the standalone encode) with the same lookup code, packs
-// each level's two features, and stores them in the planar slot order
-// (u32 2q = level q, 2q + 1 = level 15 - q) to its 64 B of `lds`; lane
-// (g, n) then reads slots 4g .. 4g + 3 of pixel 16 j + n for group j -- the
-// words load_features8 reads from the planar feature planes.  Pixels that
-// are past the frame or not hit get zero features.
-// Levels l0 .. l0 + NL - 1 of one pixel, packed into the planar slot order
-// (u32 2q = level q, 2q + 1 = level 15 - q).
-template <typename E, int NL, bool HF16>
-__device__ __forceinline__ void encode_batch(const RelightArgs &a, int l0, const float (&x)[3],
-                                             uint32_t (&pk)[16]) {
-  float v[NL][2];
-  dev::gather_levels<NL, HF16>(a.lv, a.hash, l0, x, v);
-#pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const int l = l0 + i;
-    pk[l < 8 ? 2 * l : 2 * (15 - l) + 1] = std::is_same<E, _Float16>::value
-                                                ? dev::pack_f16_rounded(v[i][0], v[i][1])
-                                                : dev::pack_bf16(v[i][0], v[i][1]);
-  }
-}
-
-template <typename E>
-__device__ __forceinline__ void fused_features(const RelightArgs &a, bool active, int my_px,
-                                               int lane, int g, int n, uint32_t *lds,
-                                               u32x4 (&hb)[4]) {
-  uint32_t pk[16];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) pk[i] = 0u;
-  if (active) {
-    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)my_px);
-    const float x[3] = {x4.x, x4.y, x4.z};
-    // Four levels per batch: 32 gathers in flight before the first is used
-    // (FP16 entries; two levels of FP32 entries per half-batch).  The table
-    // type is tested per batch: that branch also keeps the compiler from
-    // hoisting the address math of all 16 levels at once (which spills).
-#pragma unroll
-    for (int l0 = 0; l0 < 16; l0 += 4) {
-      if (a.hash_f16) {
-        encode_batch<E, 4, true>(a, l0, x, pk);
-      } else {
-        encode_batch<E, 2, false>(a, l0, x, pk);
-        encode_batch<E, 2, false>(a, l0 + 2, x, pk);
-      }
-    }
-  }
-  u32x4 *row = (u32x4 *)(lds + lane * 16);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) row[c] = u32x4{pk[4 * c], pk[4 * c + 1], pk[4 * c + 2], pk[4 * c + 3]};
-  // Lanes of one wave exchange through LDS: order the stores before the loads.
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-  for (int j = 0; j < 4; ++j) hb[j] = *(const u32x4 *)(lds + (16 * j + n) * 16 + 4 * g);
-}
-
 // FUSE: the fused-encode path is compiled in (taken when a.fused_encode).
 template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, bool FUSE>
 __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int lane,
@@ -440,8 +396,10 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
 // scene weights, selection and K; items are listed by decreasing K so the
 // tail of the launch is short items.  Output-layer tiles stay in VGPRs, so
 // there is no workgroup barrier and no scene switch cost beyond reloading;
-// jobs with the fused encode use their wave's own 4 KB of LDS.
-template <typename E, bool SOFTPLUS, int GP, int MINW>
+// jobs with the fused encode use their wave's own 4 KB of LDS.  HAS_AUX
+// compiles in the path of jobs with aux inputs (features and aux inputs from
+// the standalone encode).
+template <typename E, bool SOFTPLUS, bool HAS_AUX, int GP, int MINW>
 __global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
     const RelightArgs *__restrict__ jobs, const BatchItem *__restrict__ items,
     const int *__restrict__ n_items_p, unsigned int *__restrict__ counter) {
@@ -464,9 +422,13 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_batch_kernel(
     const BatchItem item = items[it];
     const RelightArgs a = jobs[item.job];
     // One 64-pixel group per item: a loop over several would let the
-    // compiler hoist the group's loads and spill.
-    relight_wave<E, SOFTPLUS, false, false, GP, false, true>(a, item.base, lane, nullptr,
-                                                             my_lds);
+    // compiler hoist the group's loads and spill.  Jobs with aux inputs take
+    // the tile shape their single relights use (two groups at a time).
+    if (HAS_AUX && a.aux)
+      relight_wave<E, SOFTPLUS, false, true, 2, false, false>(a, item.base, lane, nullptr, my_lds);
+    else
+      relight_wave<E, SOFTPLUS, false, false, GP, false, true>(a, item.base, lane, nullptr,
+                                                               my_lds);
   }
 }
 
@@ -575,6 +537,7 @@ __global__ __launch_bounds__(kPlainBlock) void relight_plain_kernel(RelightArgs 
 // (the ReLU outputs share it when positive).  One atomicMax per wave and
 // layer on the float bits, which order like the values for non-negative
 // floats; the maxima start at 0.
+template <bool AUX>
 __global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a, int k_chunk) {
   using E = __bf16;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -588,6 +551,14 @@ __global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a
   for (int j = 0; j < 4; ++j) {
     const int p = base + 16 * j + n;
     hb[j] = p < P ? dev::load_features8(a, P, p, 8 * g) : u32x4{0u, 0u, 0u, 0u};
+  }
+  u32x2 auxb[4], waux[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int p = base + 16 * j + n;
+    auxb[j] = AUX && p < P ? *(const u32x2 *)((const uint16_t *)a.aux + (size_t)p * 16 + 4 * g)
+                           : u32x2{0u, 0u};
+    waux[j] = AUX ? ((const u32x2 *)a.waux_tiles)[j * 64 + lane] : u32x2{0u, 0u};
   }
   const u32x4 *wt = (const u32x4 *)a.w_tiles;
   u32x4 w2[4][2], w3[4][2];
@@ -618,6 +589,7 @@ __global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         c[b] = Mfma<E>::run(w2[b][1], x1, Mfma<E>::run(w2[b][0], x0, zero));
+        if (AUX) c[b] = Mfma16<E>::run(waux[b], auxb[j], c[b]);
         upd(m[1], c[b]);
       }
       x0 = relu_pack<E>(c[0], c[1]);
@@ -639,14 +611,18 @@ __global__ __launch_bounds__(kRelightBlock) void calib_amax_kernel(RelightArgs a
 }
 
 // Calibration subsample: calibration pixel q is frame pixel q * stride, its
-// features copied in the frame's layout (zeros for a background pixel, whose
-// features the encode pass leaves unwritten); block 0 also clears the four
-// maxima.  Row layout: thread i copies 16 B part i % 4 of pixel i / 4;
-// planar: 8 B of plane i % 8 of pixel i / 8.
+// features copied in the frame's layout (zeros for a background pixel, as
+// the encode pass writes them) and, with aux inputs, its 16 aux values
+// (zeros for a background pixel, whose activations the relight discards);
+// block 0 also clears the four maxima.  Row layout: thread i copies 16 B
+// part i % 4 of pixel i / 4 (and 8 B part i % 4 of its aux row); planar:
+// 8 B of plane i % 8 of pixel i / 8.
 __global__ __launch_bounds__(256) void calib_gather_kernel(const uint16_t *__restrict__ feat,
+                                                           const uint16_t *__restrict__ aux,
                                                            const uint8_t *__restrict__ flag,
                                                            int n_pixels, int stride,
                                                            uint16_t *__restrict__ calib,
+                                                           uint16_t *__restrict__ calib_aux,
                                                            int n_cal, float *__restrict__ amax,
                                                            int planar) {
   if (blockIdx.x == 0 && threadIdx.x < 4) amax[threadIdx.x] = 0.f;
@@ -663,6 +639,41 @@ __global__ __launch_bounds__(256) void calib_gather_kernel(const uint16_t *__res
   } else {
     const u32x4 v = hit ? *(const u32x4 *)(feat + p * 32 + 8 * part) : u32x4{0u, 0u, 0u, 0u};
     *(u32x4 *)(calib + (size_t)q * 32 + 8 * part) = v;
+    if (aux)
+      *(uint2 *)(calib_aux + (size_t)q * 16 + 4 * part) =
+          hit ? *(const uint2 *)(aux + p * 16 + 4 * part) : make_uint2(0u, 0u);
+  }
+}
+
+// The calibration subsample encoded from positions: thread q computes the 16
+// levels of frame pixel q * stride with the standalone planar encode's
+// arithmetic (plane r = levels r and 15 - r, each accumulated from zero) and
+// writes the planar layout [8][n_cal][4] in BF16; background pixels get
+// zeros.  Block 0 clears the four maxima.
+template <bool HF16>
+__global__ __launch_bounds__(256) void calib_encode_kernel(RelightArgs a, int stride,
+                                                           uint16_t *__restrict__ calib,
+                                                           int n_cal, float *__restrict__ amax) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) amax[threadIdx.x] = 0.f;
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= n_cal) return;
+  const long long p = (long long)q * stride;
+  const bool hit = p < a.n_pixels && (a.env_flag == nullptr || a.env_flag[p] != 0);
+  float x[3] = {0.f, 0.f, 0.f};
+  if (hit) {
+    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)p);
+    x[0] = x4.x;
+    x[1] = x4.y;
+    x[2] = x4.z;
+  }
+  uint2 *dst = (uint2 *)calib;
+  for (int r = 0; r < 8; ++r) {
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if (hit) {
+      dev::accumulate_level<2, HF16>(a.lv, a.hash, r, x, v, nullptr);
+      dev::accumulate_level<2, HF16>(a.lv, a.hash, 15 - r, x, v + 2, nullptr);
+    }
+    dst[(size_t)r * n_cal + q] = make_uint2(dev::pack_bf16(v[0], v[1]), dev::pack_bf16(v[2], v[3]));
   }
 }
 
@@ -711,20 +722,26 @@ void launch_sp(const RelightArgs &a, int variant, hipStream_t s) {
 
 template <typename E, bool SP, int V>
 void launch_batch_v(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
-                    unsigned int *counter, int n_cu, hipStream_t s) {
+                    unsigned int *counter, bool has_aux, int n_cu, hipStream_t s) {
   constexpr MfmaVariant v = kMfmaVariants[V];
   static_assert(!v.w4_lds, "the batch form keeps the output layer in VGPRs");
-  hipLaunchKernelGGL((relight_batch_kernel<E, SP, v.gp, v.min_waves>), dim3(n_cu * v.min_waves),
-                     dim3(kRelightBlock), 0, s, jobs, items, n_items, counter);
+  if (has_aux)
+    hipLaunchKernelGGL((relight_batch_kernel<E, SP, true, v.gp, v.min_waves>),
+                       dim3(n_cu * v.min_waves), dim3(kRelightBlock), 0, s, jobs, items, n_items,
+                       counter);
+  else
+    hipLaunchKernelGGL((relight_batch_kernel<E, SP, false, v.gp, v.min_waves>),
+                       dim3(n_cu * v.min_waves), dim3(kRelightBlock), 0, s, jobs, items, n_items,
+                       counter);
 }
 
 template <typename E, bool SP>
 void launch_batch_e(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
-                    unsigned int *counter, int variant, int n_cu, hipStream_t s) {
+                    unsigned int *counter, bool has_aux, int variant, int n_cu, hipStream_t s) {
   switch (variant) {
-    case 0: launch_batch_v<E, SP, 0>(jobs, items, n_items, counter, n_cu, s); break;
-    case 1: launch_batch_v<E, SP, 1>(jobs, items, n_items, counter, n_cu, s); break;
-    case 5: launch_batch_v<E, SP, 5>(jobs, items, n_items, counter, n_cu, s); break;
+    case 0: launch_batch_v<E, SP, 0>(jobs, items, n_items, counter, has_aux, n_cu, s); break;
+    case 1: launch_batch_v<E, SP, 1>(jobs, items, n_items, counter, has_aux, n_cu, s); break;
+    case 5: launch_batch_v<E, SP, 5>(jobs, items, n_items, counter, has_aux, n_cu, s); break;
     default: fail(NWPRT_ERR_INVALID_ARGUMENT, "batch: variant has no persistent form");
   }
 }
@@ -823,16 +840,20 @@ bool relight_batch_variant_ok(int v) {
 }
 
 void launch_relight_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
-                          unsigned int *counter, Elem elem, bool softplus, int variant,
-                          int n_cu, hipStream_t s) {
+                          unsigned int *counter, Elem elem, bool softplus, bool has_aux,
+                          int variant, int n_cu, hipStream_t s) {
   require(jobs && items && n_items && counter && n_cu >= 1, NWPRT_ERR_INVALID_ARGUMENT,
           "batch: missing operand");
   if (elem == Elem::BF16) {
-    if (softplus) launch_batch_e<__bf16, true>(jobs, items, n_items, counter, variant, n_cu, s);
-    else launch_batch_e<__bf16, false>(jobs, items, n_items, counter, variant, n_cu, s);
+    if (softplus)
+      launch_batch_e<__bf16, true>(jobs, items, n_items, counter, has_aux, variant, n_cu, s);
+    else
+      launch_batch_e<__bf16, false>(jobs, items, n_items, counter, has_aux, variant, n_cu, s);
   } else if (elem == Elem::F16) {
-    if (softplus) launch_batch_e<_Float16, true>(jobs, items, n_items, counter, variant, n_cu, s);
-    else launch_batch_e<_Float16, false>(jobs, items, n_items, counter, variant, n_cu, s);
+    if (softplus)
+      launch_batch_e<_Float16, true>(jobs, items, n_items, counter, has_aux, variant, n_cu, s);
+    else
+      launch_batch_e<_Float16, false>(jobs, items, n_items, counter, has_aux, variant, n_cu, s);
   } else {
     fail(NWPRT_ERR_INVALID_ARGUMENT, "batch: MFMA element must be BF16 or FP16");
   }
@@ -844,25 +865,44 @@ void launch_reset_counter(unsigned int *counter, hipStream_t s) {
   hip_check(hipGetLastError(), "reset_counter_kernel launch");
 }
 
-void launch_calib_gather(const uint16_t *features, const uint8_t *env_flag, int n_pixels,
-                         int stride, uint16_t *calib, int n_cal, float *amax, bool planar,
-                         hipStream_t s) {
+void launch_calib_gather(const uint16_t *features, const uint16_t *aux, const uint8_t *env_flag,
+                         int n_pixels, int stride, uint16_t *calib, uint16_t *calib_aux,
+                         int n_cal, float *amax, bool planar, hipStream_t s) {
   require(features && calib && amax && n_pixels >= 1 && stride >= 1 && n_cal >= 1 &&
-              (long long)(n_cal - 1) * stride < n_pixels,
+              (long long)(n_cal - 1) * stride < n_pixels && (!aux || (calib_aux && !planar)),
           NWPRT_ERR_INVALID_ARGUMENT, "calibration gather: bad subsample");
   const int threads = (planar ? 8 : 4) * n_cal;
   hipLaunchKernelGGL(calib_gather_kernel, dim3((threads + 255) / 256), dim3(256), 0, s, features,
-                     env_flag, n_pixels, stride, calib, n_cal, amax, (int)planar);
+                     aux, env_flag, n_pixels, stride, calib, calib_aux, n_cal, amax, (int)planar);
   hip_check(hipGetLastError(), "calib_gather_kernel launch");
+}
+
+void launch_calib_encode(const RelightArgs &enc, int stride, uint16_t *calib, int n_cal,
+                         float *amax, hipStream_t s) {
+  require(enc.position && enc.hash && calib && amax && enc.n_pixels >= 1 && stride >= 1 &&
+              n_cal >= 1 && (long long)(n_cal - 1) * stride < enc.n_pixels &&
+              enc.lv.n_levels == 16,
+          NWPRT_ERR_INVALID_ARGUMENT, "calibration encode: bad subsample");
+  const dim3 grid((n_cal + 255) / 256);
+  if (enc.hash_f16)
+    hipLaunchKernelGGL(calib_encode_kernel<true>, grid, dim3(256), 0, s, enc, stride, calib, n_cal,
+                       amax);
+  else
+    hipLaunchKernelGGL(calib_encode_kernel<false>, grid, dim3(256), 0, s, enc, stride, calib,
+                       n_cal, amax);
+  hip_check(hipGetLastError(), "calib_encode_kernel launch");
 }
 
 void launch_calibrate_amax(const RelightArgs &a, hipStream_t s) {
   require(a.n_pixels >= 1 && a.k >= 1 && a.k <= NWPRT_MAX_K && a.features && a.mk &&
-              a.w_tiles && a.amax,
+              a.w_tiles && a.amax && (!a.aux || a.waux_tiles),
           NWPRT_ERR_INVALID_ARGUMENT, "calibration: missing operand");
   constexpr int kChunk = 16;
   const dim3 grid((a.n_pixels + kRelightBlock - 1) / kRelightBlock, (a.k + kChunk - 1) / kChunk);
-  hipLaunchKernelGGL(calib_amax_kernel, grid, dim3(kRelightBlock), 0, s, a, kChunk);
+  if (a.aux)
+    hipLaunchKernelGGL(calib_amax_kernel<true>, grid, dim3(kRelightBlock), 0, s, a, kChunk);
+  else
+    hipLaunchKernelGGL(calib_amax_kernel<false>, grid, dim3(kRelightBlock), 0, s, a, kChunk);
   hip_check(hipGetLastError(), "calib_amax_kernel launch");
 }
 

@@ -13,11 +13,13 @@
 // never idles between requests.  The launch stream waits for a request's
 // preparation event before the first tick that holds its items.
 //
-// Device state per admitted request comes from a pool of exact-size blocks
-// (selection tiles; per-pixel features only for scenes the persistent
-// kernel cannot encode itself; and for rotated / fresh maps the cubemap,
-// coefficients, scores, indices and strengths), so steady-state serving
-// allocates nothing.  The budget counts the blocks admitted requests hold.
+// Device state per admitted request is one block from a pool of exact-size
+// blocks (selection tiles and the K strengths; per-pixel features and aux
+// inputs only for scenes the persistent kernel cannot encode itself; for the
+// FP8 form the 32x32 tiles, calibration sample and maxima; and for rotated /
+// fresh maps the cubemap, coefficients, scores and indices), so steady-state
+// serving allocates nothing.  The budget counts the blocks admitted requests
+// hold.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,17 +85,30 @@ struct Block {
   size_t n = 0;
 };
 
+// Byte offsets of a request's buffers inside its block (npos: absent).
+struct Layout {
+  static constexpr size_t npos = ~(size_t)0;
+  size_t features = npos, aux = npos, mk = 0, strength = 0;
+  size_t mk32 = npos, calib = npos, amax = npos;             // FP8 form
+  size_t faces = npos, coef = npos, scores = npos, idx = npos;  // built maps
+  size_t total = 0;
+};
+
 struct Request {
   int64_t id = 0;
   nwprt_request r{};
   double t_submit = 0.0;
   int P = 0, K = 0, N = 0;
   bool fused = false;  // encoded inside the persistent kernel
+  bool aux = false;    // the scene has aux inputs
   int slot = -1;
   int next_base = 0;          // first pixel of the next unissued work item
   bool waited = false;        // launch stream already waits for its preparation
-  std::vector<Block> blocks;  // pooled device state
+  Layout lay;
+  Block block;                // pooled device state
   size_t bytes = 0;
+  template <class T>
+  T *at(size_t off) const { return (T *)((char *)block.p + off); }
 };
 
 struct Tick {
@@ -139,8 +154,17 @@ struct Server {
     for (hipEvent_t e : prep_done)
       if (e) (void)hipEventDestroy(e);
     if (jobs_host) (void)hipHostFree(jobs_host);
-    for (auto &kv : requests)
-      for (Block &b : kv.second->blocks) pool.put(b.p, b.n);
+    for (auto &kv : requests) {
+      pool.put(kv.second->block.p, kv.second->block.n);
+      // The context destroys its servers before its scenes and maps.
+      const nwprt_request &r = kv.second->r;
+      auto sc = ctx->scenes.find(r.scene_id);
+      if (sc != ctx->scenes.end()) --sc->second->server_refs;
+      if (r.env_source == NWPRT_ENV_RESIDENT) {
+        auto e = ctx->envs.find(r.env_id);
+        if (e != ctx->envs.end()) --e->second->server_refs;
+      }
+    }
     if (prep) (void)hipStreamDestroy(prep);
     if (launch) (void)hipStreamDestroy(launch);
   }
@@ -158,23 +182,45 @@ Server &get_server(nwprt_ctx ctx, int id) {
 
 bool env_built(const nwprt_request &r) { return r.env_source != NWPRT_ENV_RESIDENT; }
 
-// Device bytes a request holds while admitted.
-std::vector<size_t> block_sizes(const Request &q) {
-  // Features: none when the persistent kernel encodes the request's pixels.
-  std::vector<size_t> s = {q.fused ? 0 : (size_t)q.P * 64, (size_t)q.K * 4 * 64 * 16};
+// The request's buffers, each 256-byte aligned, in one block.
+Layout request_layout(const Server &sv, const Scene &sc, const Request &q) {
+  Layout l;
+  size_t o = 0;
+  auto take = [&](size_t n) {
+    const size_t at = o;
+    o += (n + 255) & ~(size_t)255;
+    return at;
+  };
+  const bool fp8 = sv.cfg.precision == NWPRT_PREC_FP8;
+  if (!q.fused) l.features = take((size_t)q.P * 64);
+  if (q.aux) l.aux = take((size_t)q.P * 32);
+  l.mk = take((size_t)q.K * 4 * 64 * 16);
+  l.strength = take((size_t)q.K * 12);
+  if (fp8) {
+    l.mk32 = take(fp8_mk_bytes(q.K));
+    l.calib = take(fp8_calib_bytes(sc, q.P));
+    l.amax = take(16);
+  }
   if (env_built(q.r)) {
     const size_t m = 6ull * q.N * q.N;
-    s.insert(s.end(), {m * 12, m * 12, m * 4, (size_t)q.K * 4, (size_t)q.K * 12});
+    l.faces = take(m * 12);
+    l.coef = take(m * 12);
+    l.scores = take(m * 4);
+    l.idx = take((size_t)q.K * 4);
   }
-  return s;
+  l.total = o;
+  return l;
 }
 
 void validate_request(Server &sv, const nwprt_request &r, Request &q) {
   const Scene &sc = get_scene(sv.ctx, r.scene_id);
-  require(sc.desc.aux_dims == 0, NWPRT_ERR_UNSUPPORTED,
-          "server: scenes with aux inputs are not supported");
   check_frame(&r.gbuffer, r.width, r.height, true);
   check_outputs(r.out);
+  q.aux = sc.desc.aux_dims != 0;
+  if (q.aux) {  // checks the G-buffer's aux inputs
+    EncodeArgs e{};
+    set_aux_encode(e, sc, &r.gbuffer, nullptr);
+  }
   const bool sp = sc.desc.output_softplus != 0;
   require(!sv.softplus_set || sp == sv.softplus, NWPRT_ERR_UNSUPPORTED,
           "server: all requests must share the output activation");
@@ -209,8 +255,8 @@ void validate_request(Server &sv, const nwprt_request &r, Request &q) {
   }
   require(q.N == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
           "server: cubemap face size does not match the scene's wavelet feature table");
-  size_t bytes = 0;
-  for (size_t n : block_sizes(q)) bytes += n;
+  q.lay = request_layout(sv, sc, q);
+  const size_t bytes = q.lay.total;
   require(!sv.cfg.memory_budget_bytes || bytes <= sv.cfg.memory_budget_bytes,
           NWPRT_ERR_INVALID_ARGUMENT,
           "server: the request needs " + std::to_string(bytes) +
@@ -221,27 +267,26 @@ void validate_request(Server &sv, const nwprt_request &r, Request &q) {
 
 // Preparation of an admitted request on the preparation stream: its map's
 // selection, the hash-grid encode (unless the persistent kernel encodes the
-// pixels itself), the selection tiles and its job-table entry; then the
-// slot's event.
+// pixels itself), the selection tiles (and for the FP8 form the calibration)
+// and its job-table entry; then the slot's event.
 void prepare(Server &sv, Request &q) {
   const hipStream_t s = sv.prep;
-  const Elem elem = elem_of(sv.cfg.precision);
+  const nwprt_precision prec = sv.cfg.precision;
+  const Elem elem = elem_of(prec);
   const Scene &sc = get_scene(sv.ctx, q.r.scene_id);
-  void *features = q.blocks[0].p;
-  uint16_t *mk = (uint16_t *)q.blocks[1].p;
+  const Layout &l = q.lay;
+  float *strength = q.at<float>(l.strength);
   const int32_t *idx = nullptr;
-  const float *strength = nullptr;
   if (!env_built(q.r)) {
     EnvMap &e = get_env(sv.ctx, q.r.env_id);
     hip_check(hipStreamWaitEvent(s, e.ready, 0), "hipStreamWaitEvent");
     idx = e.idx.as<int32_t>();
-    strength = e.strength.as<float>();
+    hip_check(hipMemcpyAsync(strength, e.strength.p, (size_t)q.K * 12, hipMemcpyDeviceToDevice, s),
+              "hipMemcpyAsync strengths");
   } else {
     const size_t m = 6ull * q.N * q.N;
-    float *faces = (float *)q.blocks[2].p, *coef = (float *)q.blocks[3].p;
-    uint32_t *scores = (uint32_t *)q.blocks[4].p;
-    int32_t *bidx = (int32_t *)q.blocks[5].p;
-    float *bstr = (float *)q.blocks[6].p;
+    float *faces = q.at<float>(l.faces), *coef = q.at<float>(l.coef);
+    int32_t *bidx = q.at<int32_t>(l.idx);
     const float *src = q.r.faces;
     if (q.r.env_source == NWPRT_ENV_ROTATED) {
       launch_rotate_cubemap(q.r.faces, faces, q.N, q.r.yaw, s);
@@ -252,14 +297,16 @@ void prepare(Server &sv, Request &q) {
       src = faces;
     }
     launch_haar(src, m * 3, coef, q.N, 1, s);
-    launch_topk(coef, q.N, q.K, scores, bidx, bstr, s);
+    launch_topk(coef, q.N, q.K, q.at<uint32_t>(l.scores), bidx, strength, s);
     idx = bidx;
-    strength = bstr;
   }
-  if (!q.fused) launch_encode(encode_args(sc, &q.r.gbuffer, q.P, elem, features), s);
-  launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * q.N * q.N, idx, q.K,
-                           elem, mk, nullptr, s, nullptr, planar_features(sc, elem),
-                           scene_projection(sc));
+  void *features = q.fused ? nullptr : q.at<void>(l.features);
+  void *aux = q.aux ? q.at<void>(l.aux) : nullptr;
+  if (!q.fused) {
+    EncodeArgs e = encode_args(sc, &q.r.gbuffer, q.P, elem, features);
+    if (q.aux) set_aux_encode(e, sc, &q.r.gbuffer, aux);
+    launch_encode(e, s);
+  }
   RelightArgs &a = sv.jobs_host[q.slot];
   a = RelightArgs{};
   a.n_pixels = q.P;
@@ -267,9 +314,10 @@ void prepare(Server &sv, Request &q) {
   a.features = features;
   a.features_planar = planar_features(sc, elem);
   if (q.fused) set_fused_encode(a, sc, &q.r.gbuffer);
-  a.mk = mk;
+  if (q.aux) set_aux_relight(a, sc, prec, aux);
+  a.mk = q.at<uint16_t>(l.mk);
   a.light = strength;
-  a.w_tiles = (const uint16_t *)scene_tiles(sc, sv.cfg.precision);
+  a.w_tiles = (const uint16_t *)scene_tiles(sc, prec == NWPRT_PREC_FP8 ? NWPRT_PREC_BF16 : prec);
   a.env_flag = q.r.gbuffer.env_flag;
   a.direct = q.r.gbuffer.direct;
   a.indirect = q.r.out.indirect;
@@ -277,6 +325,19 @@ void prepare(Server &sv, Request &q) {
   a.rgba8 = q.r.out.rgba8;
   a.tonemap_source = (int)sv.cfg.tonemap;
   a.softplus = sv.softplus;
+  if (prec == NWPRT_PREC_FP8) {
+    Fp8Work w;
+    w.mk16 = q.at<uint16_t>(l.mk);
+    w.mk32 = q.at<uint16_t>(l.mk32);
+    w.calib = q.at<uint16_t>(l.calib);
+    w.amax = q.at<float>(l.amax);
+    prepare_fp8(sc, idx, q.K, a, w, s);
+    set_fp8_relight(a, sc, w);
+  } else {
+    launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), 6 * q.N * q.N, idx, q.K,
+                             elem, q.at<uint16_t>(l.mk), nullptr, s, nullptr,
+                             planar_features(sc, elem), scene_projection(sc));
+  }
   hip_check(hipMemcpyAsync(sv.jobs.as<RelightArgs>() + q.slot, &a, sizeof(RelightArgs),
                            hipMemcpyHostToDevice, s),
             "hipMemcpyAsync job");
@@ -284,12 +345,29 @@ void prepare(Server &sv, Request &q) {
 }
 
 void release(Server &sv, Request &q) {
-  for (Block &b : q.blocks) sv.pool.put(b.p, b.n);
-  q.blocks.clear();
+  sv.pool.put(q.block.p, q.block.n);
+  q.block = Block{};
   sv.stats.bytes_in_use -= q.bytes;
   q.bytes = 0;
   if (q.slot >= 0) sv.free_slots.push_back(q.slot);
   q.slot = -1;
+}
+
+// The scene and resident map a request reads stay alive (destroy and
+// in-place updates refuse) from submit until it completes or fails.
+void add_refs(Server &sv, const nwprt_request &r, int d) {
+  get_scene(sv.ctx, r.scene_id).server_refs += d;
+  if (!env_built(r)) get_env(sv.ctx, r.env_id).server_refs += d;
+}
+
+// A request whose admission or preparation failed: its resources are
+// returned, and it is reported done with latency -status.
+void fail_request(Server &sv, Request &q, nwprt_status st) {
+  release(sv, q);
+  add_refs(sv, q.r, -1);
+  sv.done.emplace_back(q.id, -(double)st);
+  ++sv.stats.failed;
+  sv.requests.erase(q.id);
 }
 
 void retire(Server &sv, bool block) {
@@ -308,6 +386,7 @@ void retire(Server &sv, bool block) {
       sv.done.emplace_back(q->id, (tnow - q->t_submit) * 1e3);
       sv.active.erase(std::find(sv.active.begin(), sv.active.end(), q));
       release(sv, *q);
+      add_refs(sv, q->r, -1);
       ++sv.stats.completed;
       sv.requests.erase(q->id);
     }
@@ -317,41 +396,42 @@ void retire(Server &sv, bool block) {
   }
 }
 
+// Admits queued requests in arrival order while a slot is free and the
+// budget allows.  Admission is transactional: a request whose device state
+// cannot be allocated (with nothing admitted that could free some) or whose
+// preparation fails gives back its blocks, slot and budget and is reported
+// as failed, so the queue always moves on.
 void admit(Server &sv) {
   while (!sv.queue.empty() && !sv.free_slots.empty()) {
     Request &q = *sv.queue.front();
-    const std::vector<size_t> sizes = block_sizes(q);
-    size_t bytes = 0;
-    for (size_t n : sizes) bytes += n;
+    const size_t bytes = q.lay.total;
     if (sv.cfg.memory_budget_bytes && sv.stats.bytes_in_use + bytes > sv.cfg.memory_budget_bytes)
       break;
-    for (size_t n : sizes) {
-      if (n == 0) {
-        q.blocks.push_back(Block{nullptr, 0});
-        continue;
-      }
-      void *p = sv.pool.get(n);
+    try {
+      void *p = sv.pool.get(bytes);
       if (!p) {  // return idle blocks of other sizes to the device and retry
         sv.pool.trim();
-        p = sv.pool.get(n);
+        p = sv.pool.get(bytes);
       }
       if (!p) {
-        // Device memory exhausted: keep the request queued until admitted
-        // requests complete; with none admitted nothing would ever free.
-        for (Block &b : q.blocks) sv.pool.put(b.p, b.n);
-        q.blocks.clear();
+        // Device memory exhausted: wait for admitted requests to complete;
+        // with none admitted nothing would ever free.
         require(!sv.active.empty(), NWPRT_ERR_OUT_OF_MEMORY,
                 "server: device memory for a request's state is not available");
         return;
       }
-      q.blocks.push_back(Block{p, n});
+      q.block = Block{p, bytes};
+      q.bytes = bytes;
+      sv.stats.bytes_in_use += bytes;
+      q.slot = sv.free_slots.back();
+      sv.free_slots.pop_back();
+      prepare(sv, q);
+    } catch (const Error &e) {
+      sv.queue.pop_front();
+      fail_request(sv, q, e.status);
+      continue;
     }
-    q.bytes = bytes;
-    sv.stats.bytes_in_use += bytes;
     sv.stats.peak_bytes = std::max(sv.stats.peak_bytes, sv.stats.bytes_in_use);
-    q.slot = sv.free_slots.back();
-    sv.free_slots.pop_back();
-    prepare(sv, q);
     sv.active.push_back(&q);
     sv.queue.pop_front();
     sv.stats.peak_active = std::max(sv.stats.peak_active, (int32_t)sv.active.size());
@@ -366,10 +446,12 @@ void launch_ticks(Server &sv) {
     if (t.in_flight) break;  // the ring is full (cannot happen with kInFlight < kTickSlots)
     int n = 0;
     int64_t pairs = 0;
+    bool has_aux = false;
     std::vector<Request *> touched;
     for (Request *q : sv.active) {
       if (q->next_base >= q->P) continue;
       touched.push_back(q);
+      has_aux = has_aux || q->aux;
       while (q->next_base < q->P && n < kTickItems && pairs < budget) {
         t.items_host[n++] = BatchItem{q->slot, q->next_base};
         q->next_base += 64;
@@ -392,9 +474,14 @@ void launch_ticks(Server &sv) {
               "hipMemcpyAsync count");
     unsigned int *counter = (unsigned int *)(t.meta.as<int>() + 1);
     launch_reset_counter(counter, sv.launch);
-    launch_relight_batch(sv.jobs.as<RelightArgs>(), t.items.as<BatchItem>(), t.meta.as<int>(),
-                         counter, elem_of(sv.cfg.precision), sv.softplus, sv.variant,
-                         sv.ctx->n_cu, sv.launch);
+    if (sv.cfg.precision == NWPRT_PREC_FP8)
+      launch_relight_fp8_batch(sv.jobs.as<RelightArgs>(), t.items.as<BatchItem>(),
+                               t.meta.as<int>(), counter, sv.softplus, has_aux, sv.ctx->n_cu,
+                               sv.launch);
+    else
+      launch_relight_batch(sv.jobs.as<RelightArgs>(), t.items.as<BatchItem>(), t.meta.as<int>(),
+                           counter, elem_of(sv.cfg.precision), sv.softplus, has_aux, sv.variant,
+                           sv.ctx->n_cu, sv.launch);
     hip_check(hipEventRecord(t.done, sv.launch), "hipEventRecord");
     t.in_flight = true;
     sv.in_flight.push_back(&t);
@@ -423,8 +510,9 @@ nwprt_status nwprt_server_create(nwprt_ctx ctx, const nwprt_server_config *confi
     bind(ctx);
     require(config && server_id, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
     const nwprt_server_config &c = *config;
-    require(c.precision == NWPRT_PREC_BF16 || c.precision == NWPRT_PREC_FP16,
-            NWPRT_ERR_UNSUPPORTED, "server: BF16 or FP16 precision only");
+    require(c.precision == NWPRT_PREC_BF16 || c.precision == NWPRT_PREC_FP16 ||
+                c.precision == NWPRT_PREC_FP8,
+            NWPRT_ERR_UNSUPPORTED, "server: BF16, FP16 or FP8 precision only");
     require(c.max_active >= 1 && c.max_active <= 4096, NWPRT_ERR_INVALID_ARGUMENT,
             "server: max_active must be in [1, 4096]");
     require(c.tick_pairs >= 0, NWPRT_ERR_INVALID_ARGUMENT, "server: negative tick_pairs");
@@ -472,6 +560,7 @@ nwprt_status nwprt_server_submit(nwprt_ctx ctx, int32_t server_id, const nwprt_r
     auto q = std::make_unique<Request>();
     q->r = *request;
     validate_request(sv, *request, *q);
+    add_refs(sv, q->r, +1);
     q->id = sv.next_id++;
     q->t_submit = now_s();
     sv.queue.push_back(q.get());

@@ -70,6 +70,8 @@ def lib():
                                          POINTER(OrcGbuffer), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p, c_int]
         l.orc_tonemap.argtypes = [c_float, c_float, c_float]
+        l.orc_tonemap_thresholds.argtypes = [c_void_p]
+        l.orc_tonemap_thresholds.restype = None
         l.orc_denoise.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double,
                                   c_double, c_double, c_double, c_void_p]
         l.orc_denoise.restype = None
@@ -234,6 +236,13 @@ def tonemap(rgb: np.ndarray) -> np.ndarray:
     rgb = np.asarray(rgb, np.float32).reshape(-1, rgb.shape[-1])
     return np.array([lib().orc_tonemap(float(r[0]), float(r[1]), float(r[2])) for r in rgb],
                     np.uint32)
+
+
+def tonemap_thresholds() -> np.ndarray:
+    """The oracle's 255 FP32 byte thresholds T_1 .. T_255 of the tone map."""
+    out = np.zeros(255, np.float32)
+    lib().orc_tonemap_thresholds(_p(out))
+    return out
 
 
 def denoise(color: np.ndarray, normal: np.ndarray, albedo: np.ndarray, iterations: int = 5,

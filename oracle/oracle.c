@@ -361,18 +361,44 @@ void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *li
 
 /* --------------------------------------------------------------- tonemap -- */
 
-/* Renderer/toneMap.cu:58-70 in FP32: y = 1.055 * (x + 1e-6)^(1/2.4) - 0.055
- * (one fused multiply-add, as a CUDA build contracts it), clamped to [0, 1]
- * with NaN -> 0, byte = (uint)(y * 255.9); R | G << 8 | B << 16 | 255 << 24. */
-uint32_t orc_tonemap(float r, float g, float b) {
-  const float in[3] = {r, g, b};
-  uint32_t out = 255u << 24;
-  for (int c = 0; c < 3; ++c) {
-    float y = fmaf(1.055f, powf(in[c] + 1e-6f, 1.0f / 2.4f), -(1.055f - 1.0f));
-    y = fminf(1.0f, fmaxf(0.0f, y));
-    out |= ((uint32_t)(y * 255.9f)) << (8 * c);
+/* Renderer/toneMap.cu:58-70: y = clamp01(1.055 * (x + 1e-6)^(1/2.4) - 0.055),
+ * byte = (uint)(y * 255.9), packed R | G << 8 | B << 16 | 255 << 24.  The
+ * byte is the step function of the curve taken in FP64: T_b (b = 1..255) is
+ * the smallest FP32 x with 1.055 * pow((double)x + 1e-6, 1/2.4) - 0.055 >=
+ * b / 255.9, and byte(x) = #{b : x >= T_b} (NaN and x < T_1 give 0).  The
+ * reference's FP32 powf agrees with it except within a few ulps of a T_b. */
+static double tm_curve(float x) { return 1.055 * pow((double)x + 1e-6, 1.0 / 2.4) - 0.055; }
+
+static float tm_table[256];
+static int tm_ready = 0;
+
+static void tm_build(void) {
+  if (tm_ready) return;
+  for (int b = 1; b < 256; ++b) {
+    const double want = (double)b / 255.9;
+    /* start from the inverse curve, then walk to the first float that reaches b */
+    float x = (float)(pow((want + 0.055) / 1.055, 2.4) - 1e-6);
+    while (tm_curve(x) < want) x = nextafterf(x, INFINITY);
+    while (tm_curve(nextafterf(x, -INFINITY)) >= want) x = nextafterf(x, -INFINITY);
+    tm_table[b] = x;
   }
-  return out;
+  tm_ready = 1;
+}
+
+void orc_tonemap_thresholds(float *out255) {
+  tm_build();
+  for (int b = 1; b < 256; ++b) out255[b - 1] = tm_table[b];
+}
+
+static uint32_t tm_byte(float x) {
+  uint32_t n = 0;
+  for (int b = 1; b < 256; ++b) n += (x >= tm_table[b]);
+  return n;
+}
+
+uint32_t orc_tonemap(float r, float g, float b) {
+  tm_build();
+  return tm_byte(r) | (tm_byte(g) << 8) | (tm_byte(b) << 16) | (255u << 24);
 }
 
 /* Guided edge-avoiding a-trous denoiser (Dammertz, Sewtz, Hanika, Lensch,

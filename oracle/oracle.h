@@ -26,7 +26,11 @@
  *   orc_relight_pixels   brute-force numpy FP64 network on tiny cases,
  *                        closed forms (zero weights -> softplus(0) = ln 2,
  *                        background -> 0, negative light -> clamped)
- *   orc_tonemap          closed forms (0 -> 0, 1 -> 255, alpha 255, byte order)
+ *   orc_tonemap          closed forms (0 -> 0, 1 -> 255, alpha 255, byte order);
+ *                        every threshold checked against the FP64 curve in
+ *                        Python (it reaches b / 255.9, the float below does
+ *                        not); the reference's FP32 formula in numpy agrees
+ *                        away from the thresholds
  *   orc_denoise          constant image and blend 1 unchanged; unbounded sigmas =
  *                        normalised B3-spline a-trous convolution (scipy); a
  *                        guide edge stops all flow across it; power-of-two
@@ -110,6 +114,7 @@ void orc_relight_pixels(const orc_scene *sc, const int32_t *sel, const float *li
                         double *indirect, double *with_direct, double *transport, int n_threads);
 
 uint32_t orc_tonemap(float r, float g, float b);
+void orc_tonemap_thresholds(float *out255);
 
 /* Guided a-trous denoiser; images float4 [h][w], out double [h][w][4]. */
 void orc_denoise(const float *color, const float *normal, const float *albedo, int w, int h,

@@ -13,7 +13,8 @@ from conftest import max_rel_err, psnr
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16", 2e-2), ("fp16", 5e-3)])
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-4), ("bf16", 2e-2), ("fp16", 5e-3),
+                                           ("fp8", 0.1)])
 def test_aux_scene_vs_oracle(nw, orc, ctx, precision, tol):
     case = make_case(nw, orc, width=80, height=48, k=48, aux=True, hash_std=0.5,
                      feature_std=0.5, background=0.1, seed=4)
@@ -27,6 +28,12 @@ def test_aux_scene_vs_oracle(nw, orc, ctx, precision, tol):
     ref, _ = case.oscene.relight(idx, st, case.gbuf)
     print(f"aux {precision}: max rel {max_rel_err(ref, got):.2e}, PSNR {psnr(ref, got):.1f}")
     assert max_rel_err(ref, got) <= tol
+    # FP8: the aux term, the same for all K wavelets, dominates this random
+    # network's hidden activations, so FP8's quantization errors are alike
+    # across a pixel's K terms and do not average out (BF16 loses the same
+    # ~30 dB on this scene against scenes without aux inputs).
+    if precision != "fp32":
+        assert psnr(ref, got) >= (40.0 if precision == "fp8" else 50.0)
     # the aux inputs matter: dropping them changes the image
     plain = orc.Scene.from_synth(case.levels, nw.config_scene_desc(32, nw.DTYPE_F32),
                                  {**case.params, "layers": case.params["layers"][:-64 * 16]})

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 # Radiance vs the FP64 oracle (per pixel and channel, relative to
 # max(|ref|, 1e-3 max|ref|)) and PSNR on linear radiance.
 TOL_FP8 = 0.1
-MIN_PSNR_FP8_DB = 40.0
+MIN_PSNR_FP8_DB = 50.0
 
 
 def _relight(nw, ctx, case, precision, scene=None):
@@ -93,14 +93,26 @@ def test_fp8_background_and_ragged(nw, orc, ctx):
         assert max_rel_err(ref, got) <= TOL_FP8, (w, h)
 
 
+def test_fp8_fused_encode_equals_p1_p2(nw, orc, ctx):
+    """The one-call FP8 relight encodes inside the kernel (and encodes its
+    calibration sample itself); p1 + p2 read the standalone encode's
+    features.  Both give the same outputs bit for bit, FP16 and FP32 tables,
+    ragged frames with background pixels."""
+    for hd, (w, h), bg in ((nw.DTYPE_F16, (97, 41), 0.2), (nw.DTYPE_F32, (64, 64), 0.0),
+                           (nw.DTYPE_F16, (1, 1), 0.0)):
+        case = make_case(nw, orc, width=w, height=h, k=37, face_size=16, hash_dtype=hd,
+                         background=bg, seed=w + h)
+        scene, env, fused = _relight(nw, ctx, case, "fp8")
+        frame = nw.DeviceFrame(w, h, case.gbuf)
+        ctx.TCNNPass_p1(scene, 0, frame, nw.PREC_FP8)
+        ctx.TCNNPass_p2(scene, env, 0, frame, nw.PREC_FP8, nw.TONEMAP_FULL)
+        ctx.sync()
+        split = frame.results()
+        for key in fused:
+            assert np.array_equal(fused[key], split[key]), (w, h, key)
+
+
 def test_fp8_errors_and_autotune(nw, orc, ctx):
-    aux = make_case(nw, orc, width=16, height=8, k=8, aux=True, seed=2)
-    scene = ctx.scene_create(aux.desc, aux.params)
-    env = ctx.envmap_from_faces(aux.faces, aux.k)
-    frame = nw.DeviceFrame(aux.width, aux.height, aux.gbuf)
-    with pytest.raises(nw.NwprtError) as e:
-        ctx.relight(scene, env, frame, nw.PREC_FP8)
-    assert e.value.status == nw.ERR_UNSUPPORTED
     case = make_case(nw, orc, width=64, height=32, k=24, seed=5)
     scene = ctx.scene_create(case.desc, case.params)
     env = ctx.envmap_from_faces(case.faces, case.k)

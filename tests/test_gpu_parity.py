@@ -110,10 +110,9 @@ def test_relight_config0_vs_oracle(nw, orc, ctx, precision):
     else:
         assert err <= TOL_HALF[precision]
         assert p >= MIN_PSNR_DB
-    # RGBA8 is the tone map of the with-direct output.
+    # RGBA8 is the tone map of the with-direct output, byte for byte.
     tm = orc.tonemap(res["indirect_with_direct"][:, :3])
-    diff = np.abs(tm.view(np.uint8).astype(int) - res["rgba8"].view(np.uint8).astype(int))
-    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    assert np.array_equal(tm, res["rgba8"])
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16", "fp16"])
@@ -225,9 +224,27 @@ def test_tonemap_entry_point(nw, orc, ctx):
     ctx.sync()
     got = dd.download(np.uint32, (n,))
     ref = orc.tonemap(src[:, :3])
-    diff = np.abs(got.view(np.uint8).astype(int) - ref.view(np.uint8).astype(int))
-    assert diff.max() <= 1 and np.mean(diff > 0) < 1e-3
+    assert np.array_equal(got, ref)
     assert got[0] == 0xFF000000 and got[1] == 0xFFFFFFFF
+
+
+def test_tonemap_thresholds_exact(nw, orc, ctx):
+    """Every byte boundary, the float below it, NaN, +-inf and negatives:
+    the device byte equals the oracle's."""
+    t = orc.tonemap_thresholds()
+    below = np.nextafter(t, np.float32(-np.inf))
+    above = np.nextafter(t, np.float32(np.inf))
+    special = np.array([np.nan, np.inf, -np.inf, -1.0, -0.0, 0.0, 1e-38, 3.4e38], np.float32)
+    vals = np.concatenate([t, below, above, special])
+    vals = np.concatenate([vals, np.zeros((-len(vals)) % 3, np.float32)])
+    n = len(vals) // 3
+    src = np.ones((n, 4), np.float32)
+    src[:, :3] = vals.reshape(n, 3)
+    ds, dd = nw.DeviceBuffer.from_array(src), nw.DeviceBuffer(n * 4)
+    ctx.computeFinalPixelColors(ds, dd, n, 1)
+    ctx.sync()
+    got = dd.download(np.uint32, (n,))
+    assert np.array_equal(got, orc.tonemap(src[:, :3]))
 
 
 def test_error_paths(nw, orc, ctx):

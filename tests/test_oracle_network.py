@@ -105,6 +105,68 @@ def test_tonemap_closed_forms(orc):
     assert orc.tonemap(np.array([[x, x, x]], np.float32))[0] & 0xFF == want
 
 
+def _curve64(x):
+    return 1.055 * math.pow(float(x) + 1e-6, 1.0 / 2.4) - 0.055
+
+
+def test_tonemap_thresholds_pinned(orc):
+    """Each oracle threshold T_b is the first FP32 value whose FP64 curve
+    reaches b / 255.9: the curve at T_b does, at the float just below it
+    does not (checked here with Python's own arithmetic)."""
+    t = orc.tonemap_thresholds()
+    assert np.all(np.diff(t.astype(np.float64)) > 0)
+    for b, x in enumerate(t, start=1):
+        below = np.nextafter(x, np.float32(-np.inf))
+        assert _curve64(x) >= b / 255.9 > _curve64(below), b
+    # the byte of each threshold is b, the float below it b - 1
+    tm = orc.tonemap(np.stack([t, t, t], 1)) & 0xFF
+    assert np.array_equal(tm, np.arange(1, 256))
+    below = np.nextafter(t, np.float32(-np.inf))
+    tm = orc.tonemap(np.stack([below, below, below], 1)) & 0xFF
+    assert np.array_equal(tm, np.arange(0, 255))
+
+
+def test_tonemap_matches_reference_fp32_formula(orc):
+    """Away from the thresholds the byte equals the reference's FP32 formula
+    (toneMap.cu:58-70) evaluated in numpy float32; a dropped 1e-6 offset, a
+    wrong exponent or a 256 scale would move many bytes."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(0, 1.2, 20000), 10 ** rng.uniform(-5, 0.2, 20000)])
+    x = x.astype(np.float32)
+    f = np.float32
+    y = f(1.055) * np.power(x + f(1e-6), f(1.0) / f(2.4)) - f(0.055)
+    ref = (np.clip(y, f(0), f(1)) * f(255.9)).astype(np.uint32)
+    got = orc.tonemap(np.stack([x, x, x], 1)) & 0xFF
+    t = orc.tonemap_thresholds().astype(np.float64)
+    near = np.min(np.abs(x[:, None].astype(np.float64) - t[None, :]) /
+                  np.maximum(t[None, :], 1e-30), axis=1) < 1e-5
+    assert np.array_equal(got[~near], ref[~near])
+    assert np.all(np.abs(got[near].astype(int) - ref[near].astype(int)) <= 1)
+    assert (~near).sum() > 39000
+
+
+def test_tonemap_table_header_is_current():
+    """The committed device table equals the oracle's thresholds and a fresh
+    run of its generator."""
+    import importlib.util
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1] / "neural_wavelet_prt_demo_amd" / "csrc"
+    spec = importlib.util.spec_from_file_location("gen_tm", root / "gen_tonemap_table.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    assert (root / "tonemap_table.hpp").read_text() == gen.render()
+
+
+def test_tonemap_table_equals_oracle(orc):
+    import re
+    from pathlib import Path
+    hdr = (Path(__file__).resolve().parents[1] / "neural_wavelet_prt_demo_amd" / "csrc" /
+           "tonemap_table.hpp").read_text()
+    vals = [float.fromhex(v) for v in re.findall(r"^\s+(0x[0-9a-fp.+-]+)f,", hdr, re.M)]
+    assert len(vals) == 255
+    assert np.array_equal(np.array(vals, np.float32), orc.tonemap_thresholds())
+
+
 def test_golden_small_relight(nw, orc):
     """Regression pin: the committed reduced relight (generator inputs, Haar,
     top-K, FP64 radiance) is reproduced by this tree's generator and oracle."""
