@@ -309,6 +309,21 @@ nwprt_status nwprt_relight(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
                            nwprt_precision precision,
                            nwprt_tonemap_source tonemap, void *stream);
 
+/* Allocation-free relight.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * outputs as nwprt_relight -- equal bit for bit -- for BF16, FP16 and FP8 on
+ * scenes with a 16 x 2 hash grid and no aux inputs (NWPRT_ERR_UNSUPPORTED
+ * otherwise).  Its work buffers live in a 10 MiB array that each device
+ * holds until the process ends (enough for any K <= 1024), so it calls no
+ * hipMalloc or hipFree.  Calls that overlap on one GPU share that array and
+ * race: serialise them. */
+nwprt_status nwprt_relight_noalloc(nwprt_ctx ctx, int32_t scene_id, int32_t env_id,
+                                   const nwprt_gbuffer *gbuffer, int32_t width,
+                                   int32_t height, const nwprt_frame_out *out,
+                                   nwprt_precision precision,
+                                   nwprt_tonemap_source tonemap, void *stream);
+
 /* Split form for a two-frame pipeline (RenderEngine::TCNNPass_p1 /
  * TCNNPass_p2, tcnn.cu:83-152; OutputVisualizer.cpp:61-126).  p1 encodes the
  * environment-independent per-pixel features of `frame_slot` (0 or 1); p2

@@ -189,6 +189,8 @@ _SIGNATURES = {
     "nwprt_envmap_get_coefficients": (c_int, [c_void_p, c_int32, c_void_p]),
     "nwprt_relight": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32, c_int32,
                               POINTER(FrameOut), c_int, c_int, c_void_p]),
+    "nwprt_relight_noalloc": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
+                                      c_int32, POINTER(FrameOut), c_int, c_int, c_void_p]),
     "nwprt_pass_p1": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32, c_int32,
                               c_int, c_void_p]),
     "nwprt_pass_p2": (c_int, [c_void_p, c_int32, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -595,6 +597,17 @@ class Context:
         _check(lib().nwprt_relight(self.h, scene, env, ctypes.byref(gb), frame.width,
                                    frame.height, ctypes.byref(fo), precision, tonemap,
                                    c_void_p(stream) if stream else None))
+
+    def relight_noalloc(self, scene: int, env: int, frame: DeviceFrame,
+                        precision: int = PREC_BF16, tonemap: int = TONEMAP_FULL,
+                        stream: Optional[int] = None, out: Optional[FrameOut] = None) -> None:
+        """nwprt_relight_noalloc: the fused-encode relight with the library's
+        per-device scratch array as its only work memory."""
+        gb = frame.gbuffer()
+        fo = out if out is not None else frame.frame_out()
+        _check(lib().nwprt_relight_noalloc(self.h, scene, env, ctypes.byref(gb), frame.width,
+                                           frame.height, ctypes.byref(fo), precision, tonemap,
+                                           c_void_p(stream) if stream else None))
 
     def TCNNPass_p1(self, scene: int, slot: int, frame: DeviceFrame,
                     precision: int = PREC_BF16, stream: Optional[int] = None) -> None:

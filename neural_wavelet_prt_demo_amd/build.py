@@ -36,6 +36,7 @@ SOURCES = [
     "relight_fp8_kernels.hip",
     "denoise_kernels.hip",
     "gbuffer_kernels.hip",
+    "noalloc.hip",
 ]
 
 # Packed-FP32 VALU forms are disabled for the device code (the host pass

@@ -362,17 +362,13 @@ void encode_into(nwprt_ctx ctx, Scene &sc, int scene_id, FrameSlot &slot, const 
   slot.elem = elem;
 }
 
-void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, FrameSlot &slot,
-                       const nwprt_gbuffer *gb, int w, int h, const nwprt_frame_out *out,
-                       nwprt_precision prec, nwprt_tonemap_source tm, hipStream_t s,
-                       float *dump = nullptr, int dump_begin = 0, int dump_count = 0,
-                       int variant = -1, bool fused = false) {
+}  // namespace
+
+void relight_with(nwprt_ctx ctx, const Scene &sc, EnvMap &env, const nwprt_gbuffer *gb, int P,
+                  const nwprt_frame_out *out, nwprt_precision prec, nwprt_tonemap_source tm,
+                  hipStream_t s, bool fused, const RelightWork &w, float *dump, int dump_begin,
+                  int dump_count, int variant) {
   const Elem elem = elem_of(prec);
-  const int P = w * h;
-  require(fused || (slot.valid && slot.scene == scene_id && slot.n_pixels == P &&
-                    slot.elem == elem),
-          NWPRT_ERR_STATE,
-          "pass p2: frame slot was not encoded (p1) for this scene, size and precision");
   require(env.n == sc.desc.face_size, NWPRT_ERR_INVALID_ARGUMENT,
           "environment map face size " + std::to_string(env.n) +
               " does not match the scene's wavelet feature table (" +
@@ -386,7 +382,7 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   RelightArgs a{};
   a.n_pixels = P;
   a.k = env.k;
-  a.features = fused ? nullptr : slot.features.p;
+  a.features = fused ? nullptr : w.features;
   a.features_planar = planar;
   a.light = env.strength.as<float>();
   a.env_flag = gb->env_flag;
@@ -404,38 +400,68 @@ void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, Fram
   a.dump = dump;
   a.dump_begin = dump_begin;
   a.dump_count = dump_count;
-  if (sc.desc.aux_dims) set_aux_relight(a, sc, prec, slot.aux.p);
+  if (sc.desc.aux_dims) set_aux_relight(a, sc, prec, w.aux);
   if (elem == Elem::F32) {
-    slot.wsel.ensure((size_t)env.k * 32 * 4);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
-                             env.idx.as<int32_t>(), env.k, Elem::BF16, nullptr,
-                             slot.wsel.as<float>(), s);
-    a.wsel = slot.wsel.as<float>();
+                             env.idx.as<int32_t>(), env.k, Elem::BF16, nullptr, w.wsel, s);
+    a.wsel = w.wsel;
     a.w_f32 = sc.w_f32.as<float>();
     launch_relight_plain(a, s);
   } else if (prec == NWPRT_PREC_FP8) {
-    slot.mk.ensure(fp8_mk_bytes(env.k));
-    slot.mk32.ensure(fp8_mk_bytes(env.k));
-    slot.calib.ensure(fp8_calib_bytes(sc, P));
-    slot.amax.ensure(16);
-    Fp8Work w;
-    w.mk16 = slot.mk.as<uint16_t>();
-    w.mk32 = slot.mk32.as<uint16_t>();
-    w.calib = slot.calib.as<uint16_t>();
-    w.amax = slot.amax.as<float>();
-    prepare_fp8(sc, env.idx.as<int32_t>(), env.k, a, w, s);
-    set_fp8_relight(a, sc, w);
+    Fp8Work f;
+    f.mk16 = w.mk;
+    f.mk32 = w.mk32;
+    f.calib = w.calib;
+    f.amax = w.amax;
+    prepare_fp8(sc, env.idx.as<int32_t>(), env.k, a, f, s);
+    set_fp8_relight(a, sc, f);
     launch_relight_fp8(a, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
   } else {
-    slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
     launch_prepare_selection(sc.w_f32.as<float>(), sc.wfeat.as<float>(), n_coef,
-                             env.idx.as<int32_t>(), env.k, elem, slot.mk.as<uint16_t>(), nullptr,
-                             s, nullptr, planar, scene_projection(sc));
-    a.mk = slot.mk.as<uint16_t>();
+                             env.idx.as<int32_t>(), env.k, elem, w.mk, nullptr, s, nullptr, planar,
+                             scene_projection(sc));
+    a.mk = w.mk;
     a.w_tiles = elem == Elem::BF16 ? sc.tiles_bf16.as<uint16_t>() : sc.tiles_f16.as<uint16_t>();
     launch_relight_mfma(a, elem, variant >= 0 ? variant : pick_variant(ctx, prec, env.k, P), s);
   }
   hip_check(hipEventRecord(env.last_use, s), "hipEventRecord");
+}
+
+namespace {
+
+// A relight from a frame slot: the slot's features (p1) unless fused, and
+// the slot's work buffers, grown as needed.
+void relight_from_slot(nwprt_ctx ctx, Scene &sc, int scene_id, EnvMap &env, FrameSlot &slot,
+                       const nwprt_gbuffer *gb, int w, int h, const nwprt_frame_out *out,
+                       nwprt_precision prec, nwprt_tonemap_source tm, hipStream_t s,
+                       float *dump = nullptr, int dump_begin = 0, int dump_count = 0,
+                       int variant = -1, bool fused = false) {
+  const Elem elem = elem_of(prec);
+  const int P = w * h;
+  require(fused || (slot.valid && slot.scene == scene_id && slot.n_pixels == P &&
+                    slot.elem == elem),
+          NWPRT_ERR_STATE,
+          "pass p2: frame slot was not encoded (p1) for this scene, size and precision");
+  RelightWork wk;
+  wk.features = fused ? nullptr : slot.features.p;
+  wk.aux = sc.desc.aux_dims ? slot.aux.p : nullptr;
+  if (elem == Elem::F32) {
+    slot.wsel.ensure((size_t)env.k * 32 * 4);
+    wk.wsel = slot.wsel.as<float>();
+  } else {
+    slot.mk.ensure((size_t)env.k * 4 * 64 * 16);
+    wk.mk = slot.mk.as<uint16_t>();
+    if (prec == NWPRT_PREC_FP8) {
+      slot.mk32.ensure(fp8_mk_bytes(env.k));
+      slot.calib.ensure(fp8_calib_bytes(sc, P));
+      slot.amax.ensure(16);
+      wk.mk32 = slot.mk32.as<uint16_t>();
+      wk.calib = slot.calib.as<uint16_t>();
+      wk.amax = slot.amax.as<float>();
+    }
+  }
+  relight_with(ctx, sc, env, gb, P, out, prec, tm, s, fused, wk, dump, dump_begin, dump_count,
+               variant);
 }
 
 // ------------------------------------------------------------------ batch --
@@ -676,6 +702,7 @@ nwprt_status nwprt_create(int device, nwprt_ctx *out) {
     auto ctx = std::make_unique<nwprt_context>();
     ctx->device = device;
     hip_check(hipSetDevice(device), "hipSetDevice");
+    scratch_init(device);
     hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device),
               "hipDeviceGetAttribute");

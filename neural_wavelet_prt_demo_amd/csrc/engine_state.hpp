@@ -242,4 +242,31 @@ void prepare_fp8(const Scene &sc, const int32_t *idx, int k, const RelightArgs &
                  const Fp8Work &w, hipStream_t s);
 void set_fp8_relight(RelightArgs &a, const Scene &sc, const Fp8Work &w);
 
+// Device work buffers of one relight, provided by the caller of
+// relight_with: per-pixel features (unfused relights) and aux inputs
+// (scenes with aux inputs), the MFMA selection tiles (FP8: the 16x16 tiles
+// of the calibration network) or, for the plain FP32 form, the selected
+// wavelet features; and the FP8 form's 32x32 tiles, calibration sample and
+// maxima.  Sizes: mk K x 4096 B, wsel K x 128 B, mk32 fp8_mk_bytes(K),
+// calib fp8_calib_bytes(scene, P), amax 16 B.
+struct RelightWork {
+  const void *features = nullptr;
+  const void *aux = nullptr;
+  uint16_t *mk = nullptr;
+  float *wsel = nullptr;
+  uint16_t *mk32 = nullptr, *calib = nullptr;
+  float *amax = nullptr;
+};
+// One relight of P pixels on stream s with the given work buffers (no
+// allocation): selection tiles, FP8 calibration where it applies, the
+// relight kernel, and the map's last-use event.
+void relight_with(nwprt_ctx ctx, const Scene &sc, EnvMap &env, const nwprt_gbuffer *gb, int P,
+                  const nwprt_frame_out *out, nwprt_precision prec, nwprt_tonemap_source tm,
+                  hipStream_t s, bool fused, const RelightWork &w, float *dump = nullptr,
+                  int dump_begin = 0, int dump_count = 0, int variant = -1);
+
+// The allocation-free entry point's per-device scratch array (noalloc.hip):
+// resolves its address on the current device, once per device.
+void scratch_init(int device);
+
 }  // namespace nwprt
