@@ -57,6 +57,10 @@ def parse():
                          "leg; --no-cpu-baseline skips it too)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched uneven-jobs leg")
+    ap.add_argument("--serving-requests", type=int, default=600,
+                    help="requests per Poisson rate in the served-stream leg")
+    ap.add_argument("--no-fp8", action="store_true",
+                    help="skip the FP8 legs (steps, batch, served stream) of a BF16 run")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the served-stream leg")
     ap.add_argument("--no-pipeline", action="store_true",
@@ -111,13 +115,16 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
 
 
-def bench_serving(nw, ctx, prec, n_requests=60):
+def bench_serving(nw, ctx, prec, n_requests=600, rates=(15, 20, 25, 30, 60, 120),
+                  mix_4k=True):
     """Served stream: Poisson arrivals, 4 resident scenes (face 128, FP16 hash
     grid 16 x 2^19 x 2), K = 300; 70 % of requests turn a resident 6x128x128
     cubemap by a random yaw, 30 % bring a fresh cubemap from host memory.
     Latency is measured from each request's scheduled arrival to the poll
     that sees it complete; throughput over the span first arrival -> last
-    completion.  Memory budget 200 GB of request state."""
+    completion.  Memory budget 200 GB of request state.  Each rate runs
+    n_requests requests; max_rate_p99_le_16ms is the highest of the rates
+    whose p99 is at most 16 ms."""
     from neural_wavelet_prt_demo_amd import DeviceBuffer, FrameOut
     from neural_wavelet_prt_demo_amd import workloads
     desc = nw.config_scene_desc(128, nw.DTYPE_F16)
@@ -203,11 +210,19 @@ def bench_serving(nw, ctx, prec, n_requests=60):
                 "max_ms": float(ms.max())}
 
     single = closed_loop(40, 11)
-    runs = [stream(r, 0.0, r) for r in (10, 30, 60, 120, 240, 480)]
-    runs.append(stream(60, 0.25, 7))
+    runs = []
+    for r in rates:
+        runs.append(stream(r, 0.0, r))
+        print(f"[bench] served {prec} {r} req/s: p99 {runs[-1]['p99_ms']:.1f} ms",
+              file=sys.stderr, flush=True)
+    if mix_4k:
+        runs.append(stream(60, 0.25, 7))
+    ok = [r["rate_req_s"] for r in runs if r["views"] == "1080p" and r["p99_ms"] <= 16.0]
     return {"workload": "Poisson arrivals, 4 resident scenes, K=300, 70% yaw-rotated resident "
-                        "6x128x128 cubemap / 30% fresh host cubemap, 60 requests per rate",
-            "memory_budget_gb": 200, "closed_loop": single, "runs": runs}
+                        f"6x128x128 cubemap / 30% fresh host cubemap, {n_requests} requests "
+                        "per rate",
+            "memory_budget_gb": 200, "closed_loop": single, "runs": runs,
+            "max_rate_p99_le_16ms": max(ok) if ok else None}
 
 
 def gbuffer_bench_mesh():
@@ -428,10 +443,54 @@ def bench_batch(nw, ctx, prec, reps=3):
     return {"workload": "32 jobs: 256^2..1920x1080, K in {32,64,128,300}, 6x64x64 cubemap each, "
                         "2 scenes", "jobs": len(specs),
             "pixels": int(sum(sp.width * sp.height for sp in specs)), "pairs": int(n_pairs),
-            "work_items": int(n_items), "tile_variant": nw.Context.relight_variants()[variant],
+            "work_items": int(n_items),
+            "tile_variant": ("fp8 persistent" if prec == nw.PREC_FP8
+                             else nw.Context.relight_variants()[variant]),
             "graph_ms": b, "evals_per_s": n_pairs / (b / 1e3),
             "one_call_per_job_ms": s, "one_call_per_job_evals_per_s": n_pairs / (s / 1e3),
             "speedup_vs_one_call_per_job": s / b}
+
+
+def bench_haar(nw, ctx, stream, n=128, batch_maps=64, reps=20):
+    """The Haar transform alone (nwprt_envmap_haar): one 6 x n x n map, as a
+    served request runs it, and batch_maps maps in one call.  Algorithmic
+    bytes: the FP32 texels read once and the coefficients written once,
+    2 x 6 n^2 x 3 x 4 B per map."""
+    from neural_wavelet_prt_demo_amd import DeviceBuffer, Event
+    rng = np.random.default_rng(9)
+    faces = rng.lognormal(0, 1.5, (batch_maps, 6, n, n, 3)).astype(np.float32)
+    dfaces, dcoef = DeviceBuffer.from_array(faces), DeviceBuffer(faces.nbytes)
+    per_map = 2 * 6 * n * n * 3 * 4
+
+    def timed(n_maps):
+        ctx.envmap_haar(dfaces, n, n_maps, dcoef, stream)
+        ctx.sync()
+        ev = [Event() for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for i in range(reps):
+            ctx.envmap_haar(dfaces, n, n_maps, dcoef, stream)
+            ev[i + 1].record(stream)
+        ctx.sync()
+        return float(np.mean([ev[i].elapsed_ms(ev[i + 1]) for i in range(reps)]))
+
+    one, many = timed(1), timed(batch_maps)
+    del dfaces, dcoef
+    return {"bound": "hbm", "face_size": n, "bytes_per_map": per_map,
+            "one_map_us": one * 1e3, "one_map_gbs": per_map / (one / 1e3) / 1e9,
+            "one_map_frac": per_map / (one / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "maps_per_call": batch_maps, "batch_ms": many,
+            "batch_gbs": batch_maps * per_map / (many / 1e3) / 1e9,
+            "batch_frac": batch_maps * per_map / (many / 1e3) / 1e9 / HBM_PEAK_GBS}
+
+
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -474,26 +533,47 @@ def main():
     variant_name = names[variant]
 
     # --- fused MFMA path: throughput and per-step latency ------------------
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    evs = [Event() for _ in range(args.steps + 1)]
-    mids = [Event() for _ in range(args.steps)]  # map updated, relight starts
-    t0 = time.perf_counter()
-    evs[0].record(stream)
-    for i in range(args.steps):
-        ctx.envmap_update_faces(env, dfaces, stream)
-        mids[i].record(stream)
-        ctx.relight(scene, env, frame, prec, nw.TONEMAP_FULL, stream)
-        evs[i + 1].record(stream)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    step_ms = [evs[i].elapsed_ms(evs[i + 1]) for i in range(args.steps)]
-    total_ms = evs[0].elapsed_ms(evs[-1])
+    def timed_steps(precision, steps, warmup):
+        for _ in range(warmup):
+            step(precision)
+        ctx.sync()
+        evs = [Event() for _ in range(steps + 1)]
+        mids = [Event() for _ in range(steps)]  # map updated, relight starts
+        t0 = time.perf_counter()
+        evs[0].record(stream)
+        for i in range(steps):
+            ctx.envmap_update_faces(env, dfaces, stream)
+            mids[i].record(stream)
+            ctx.relight(scene, env, frame, precision, nw.TONEMAP_FULL, stream)
+            evs[i + 1].record(stream)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        step_ms = [evs[i].elapsed_ms(evs[i + 1]) for i in range(steps)]
+        total_ms = evs[0].elapsed_ms(evs[-1])
+        # The timed steps' dominant kernel: the fused-encode relight (plus
+        # its selection prep, and for FP8 the calibration), timed live
+        # between the events of every step.
+        live = float(np.mean([mids[i].elapsed_ms(evs[i + 1]) for i in range(steps)]))
+        return step_ms, total_ms, wall, live
+
+    step_ms, total_ms, wall, relight_live_ms = timed_steps(prec, args.steps, args.warmup)
     value = pairs * args.steps / (total_ms / 1e3)
-    # The timed steps' dominant kernel: the fused-encode relight (plus its
-    # 20 us selection prep), timed live between the events of every step.
-    relight_live_ms = float(np.mean([mids[i].elapsed_ms(evs[i + 1]) for i in range(args.steps)]))
+
+    # --- the FP8 form's steps (same work, FP8-E4M3 hidden layers) ----------
+    fp8_leg = None
+    if prec != nw.PREC_FP8 and not args.no_fp8:
+        v8, _ = ctx.autotune_relight(scene, env, frame, nw.PREC_FP8)
+        s8, t8, _, live8 = timed_steps(nw.PREC_FP8, args.steps, args.warmup)
+        fp8_leg = {"ms_per_step": t8 / args.steps, "evals_per_s": pairs * args.steps / (t8 / 1e3),
+                   "latency_ms": {"p50": percentile(s8, 50), "p99": percentile(s8, 99)},
+                   "relight_live_ms": live8,
+                   "relight_tflops": pairs * FLOP_PER_PAIR / (live8 / 1e3) / 1e12,
+                   "relight_frac_of_fp8_peak": pairs * FLOP_PER_PAIR / (live8 / 1e3) / 1e12 /
+                   PEAK_TFLOPS["fp8"],
+                   "note": "relight interval = selection tiles + calibration (16K pixels x K) + "
+                           "relight_fp8_kernel with the fused encode",
+                   "tile_variant": nw.Context.relight_fp8_variants()[v8],
+                   "speedup_vs_" + args.precision: (total_ms / args.steps) / (t8 / args.steps)}
 
     # --- two-slot frame pipeline: the same per-frame work, with the map
     # update and p1 of frame f + 1 overlapping p2 of frame f ---------------
@@ -640,28 +720,45 @@ def main():
             "tflops": ceiling, "frac": result["roofline"]["achieved"] / ceiling,
             "source": ceiling_src}
 
+    if fp8_leg is not None:
+        result["fp8_form"] = fp8_leg
+    result["haar"] = bench_haar(nw, ctx, stream, N)
+
     # --- batched uneven jobs: one graph, one persistent kernel ------------
-    if not args.no_batch and prec != nw.PREC_FP8:
+    with_fp8 = prec != nw.PREC_FP8 and not args.no_fp8
+    if not args.no_batch:
         result["batched"] = bench_batch(nw, ctx, prec)
-    if not args.no_serving and prec != nw.PREC_FP8:
-        result["served"] = bench_serving(nw, ctx, prec)
+        if with_fp8:
+            result["batched_fp8"] = bench_batch(nw, ctx, nw.PREC_FP8)
+    if not args.no_serving:
+        result["served"] = bench_serving(nw, ctx, prec, args.serving_requests)
+        if with_fp8:
+            result["served_fp8"] = bench_serving(nw, ctx, nw.PREC_FP8, args.serving_requests,
+                                                 rates=(20, 30, 60), mix_4k=False)
 
     # --- CPU-baseline leg, part 1: the FP64 oracle checks the GPU results on
     # a pixel sample (outside every timed region) ---------------------------
     if not args.no_cpu_baseline and not args.no_accuracy:
         from oracle import binding as orc
         idx, st = ctx.envmap_selection(env)
-        res = frame.results()  # last run: plain FP32
+        ctx.relight(scene, env, frame, nw.PREC_FP32, nw.TONEMAP_FULL, stream)
+        ctx.sync()
+        res = frame.results()
         ctx.relight(scene, env, frame, prec, nw.TONEMAP_FULL, stream)
         ctx.sync()
         res_fast = frame.results()
+        forms = [("fp32", res), (args.precision, res_fast)]
+        if with_fp8:
+            ctx.relight(scene, env, frame, nw.PREC_FP8, nw.TONEMAP_FULL, stream)
+            ctx.sync()
+            forms.append(("fp8", frame.results()))
         rng = np.random.default_rng(0)
         pix = rng.choice(P, 1000, replace=False)
         lv = orc.hashgrid_levels()
         oscene = orc.Scene.from_synth(lv, desc, params)
         ref, _ = oscene.relight(idx, st, gbuf, pixels=pix, threads=min(16, os.cpu_count() or 1))
         acc = {}
-        for name, r in (("fp32", res), (args.precision, res_fast)):
+        for name, r in forms:
             got = r["indirect"][pix, :3].astype(np.float64)
             mse = float(np.mean((got - ref) ** 2))
             den = np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
@@ -681,7 +778,7 @@ def main():
         rows = int(max(1, min(H, args.cpu_seconds / max(secs, 1e-6))))
         _, secs = oscene.cpu_relight_fp32(idx, st, gbuf, 0, rows * W, threads)
         result["cpu_baseline"] = {"value": rows * W * K / secs, "unit": "evals/s",
-                                  "cores": threads, "kind": "port",
+                                  "cores": threads, "kind": "port", "cpu_model": cpu_model(),
                                   "sample": f"first {rows} rows x {W} px of the same frame, "
                                             f"K={K}, FP32 fused per pixel, {secs:.1f} s"}
     ctx.close()

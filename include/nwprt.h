@@ -286,6 +286,19 @@ nwprt_status nwprt_envmap_from_selection(nwprt_ctx ctx,
 nwprt_status nwprt_envmap_load_folder(nwprt_ctx ctx, const char *folder,
                                       const char *name, int32_t k,
                                       int32_t *env_id);
+/* Non-standard 2-D Haar transform of n_maps cubemaps: faces
+ * [n_maps][6][N][N][3] FP32 in, coefficients in the same layout out (device
+ * pointers, not overlapping), Mallat layout per face and channel, FP64
+ * scaling coefficients, each output rounded to FP32 once -- the transform
+ * whose flat indices the reference's waveletcoeffs/ files hold.  The
+ * reference runs it offline and only loads its results
+ * (Renderer/EnvironmentMap.cpp:36-40, 56-67); this is that producer, and the
+ * first step of nwprt_envmap_from_faces.  Asynchronous on `stream`; it uses
+ * a work buffer of the context, so calls on one context must not overlap on
+ * different streams. */
+nwprt_status nwprt_envmap_haar(nwprt_ctx ctx, const float *faces, int32_t face_size,
+                               int32_t n_maps, float *coefficients, void *stream);
+
 nwprt_status nwprt_envmap_destroy(nwprt_ctx ctx, int32_t env_id);
 nwprt_status nwprt_envmap_info(nwprt_ctx ctx, int32_t env_id,
                                int32_t *face_size, int32_t *k);
@@ -356,7 +369,7 @@ const char *nwprt_relight_variant_name(int32_t variant);
 int32_t nwprt_relight_fp8_variant_count(void);
 const char *nwprt_relight_fp8_variant_name(int32_t variant);
 /* Force one BF16 / FP16 variant for every bucket (-1 returns to the tuned
- * choice). */
+ * choice); FP8 relights use it too when it also names an FP8 variant. */
 nwprt_status nwprt_set_relight_variant(nwprt_ctx ctx, int32_t variant);
 nwprt_status nwprt_autotune_relight(nwprt_ctx ctx, int32_t scene_id,
                                     int32_t env_id,
@@ -647,7 +660,8 @@ typedef struct {
   int64_t tick_pairs;           /* pairs per persistent launch; 0: 2^30     */
   nwprt_precision precision;    /* BF16 or FP16                             */
   nwprt_tonemap_source tonemap;
-  int32_t variant;              /* tile variant with a persistent form, -1: default */
+  int32_t variant;              /* tile variant with a persistent form (FP8: an FP8
+                                   variant), -1: default */
 } nwprt_server_config;
 
 typedef enum {

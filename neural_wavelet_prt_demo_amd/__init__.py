@@ -183,6 +183,7 @@ _SIGNATURES = {
                                             POINTER(c_int32)]),
     "nwprt_envmap_load_folder": (c_int, [c_void_p, c_char_p, c_char_p, c_int32,
                                          POINTER(c_int32)]),
+    "nwprt_envmap_haar": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_void_p, c_void_p]),
     "nwprt_envmap_destroy": (c_int, [c_void_p, c_int32]),
     "nwprt_envmap_info": (c_int, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "nwprt_envmap_get_selection": (c_int, [c_void_p, c_int32, c_void_p, c_void_p]),
@@ -584,6 +585,12 @@ class Context:
         radians about +y."""
         _check(lib().nwprt_envmap_update_rotated(self.h, env, faces.ptr, float(yaw),
                                                  c_void_p(stream) if stream else None))
+
+    def envmap_haar(self, faces: "DeviceBuffer", face_size: int, n_maps: int,
+                    coef: "DeviceBuffer", stream: Optional[int] = None) -> None:
+        """Haar transform of n_maps device cubemaps into `coef` (same layout)."""
+        _check(lib().nwprt_envmap_haar(self.h, faces.ptr, face_size, n_maps, coef.ptr,
+                                       c_void_p(stream) if stream else None))
 
     def envmap_destroy(self, env: int) -> None:
         _check(lib().nwprt_envmap_destroy(self.h, env))

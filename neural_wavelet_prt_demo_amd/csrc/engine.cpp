@@ -285,10 +285,10 @@ void env_from_faces(nwprt_ctx ctx, const float *faces, nwprt_location loc, int n
   EnvMap &e = new_env(ctx, n, k, &id);
   try {
     e.coef.ensure(m * 12);
-    e.scores.ensure(m * 4);
+    e.scores.ensure(envmap_work_bytes(n, 1));
     e.has_coef = true;
-    launch_haar(dfaces, m * 3, e.coef.as<float>(), n, 1, ctx->stream);
-    launch_topk(e.coef.as<float>(), n, k, e.scores.as<uint32_t>(), e.idx.as<int32_t>(),
+    launch_haar(dfaces, m * 3, e.coef.as<float>(), n, 1, e.scores.p, ctx->stream);
+    launch_topk(e.coef.as<float>(), n, k, e.scores.p, e.idx.as<int32_t>(),
                 e.strength.as<float>(), ctx->stream);
     hip_check(hipEventRecord(e.ready, ctx->stream), "hipEventRecord");
     if (loc == NWPRT_HOST)  // the staging buffer is reused by the next call
@@ -333,9 +333,12 @@ std::tuple<int, int, int> tune_key(nwprt_precision prec, int k, int n_pixels) {
 // Kernel variant for a relight: the forced one (BF16 / FP16 only), else the
 // autotuner's choice for this shape bucket, else 0.
 int pick_variant(nwprt_ctx ctx, nwprt_precision prec, int k, int n_pixels) {
-  if (ctx->forced_variant >= 0 && prec != NWPRT_PREC_FP8) return ctx->forced_variant;
+  if (ctx->forced_variant >= 0 &&
+      (prec != NWPRT_PREC_FP8 || ctx->forced_variant < relight_fp8_variant_count()))
+    return ctx->forced_variant;
   auto it = ctx->tuned.find(tune_key(prec, k, n_pixels));
-  return it == ctx->tuned.end() ? 0 : it->second;
+  if (it != ctx->tuned.end()) return it->second;
+  return prec == NWPRT_PREC_FP8 ? kDefaultFp8Variant : 0;
 }
 
 int variant_count(nwprt_precision prec) {
@@ -504,8 +507,9 @@ Fp8Work batch_fp8_work(const Batch &b, size_t j) {
 // activation maxima -- then the counter reset and the persistent kernel.
 void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
   const bool fp8 = b.prec == NWPRT_PREC_FP8;
-  require(fp8 || relight_batch_variant_ok(variant), NWPRT_ERR_INVALID_ARGUMENT,
-          "batch: tile variant has no persistent form");
+  require(fp8 ? variant >= 0 && variant < relight_fp8_variant_count()
+              : relight_batch_variant_ok(variant),
+          NWPRT_ERR_INVALID_ARGUMENT, "batch: tile variant has no persistent form");
   b.drop_graph();
   const hipStream_t s = ctx->stream;
   const Elem elem = elem_of(b.prec);
@@ -536,7 +540,8 @@ void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
     launch_reset_counter(counter, s);
     if (fp8)
       launch_relight_fp8_batch(b.args.as<RelightArgs>(), b.items.as<BatchItem>(),
-                               b.meta.as<int>(), counter, b.softplus, b.has_aux, ctx->n_cu, s);
+                               b.meta.as<int>(), counter, b.softplus, b.has_aux, variant,
+                               ctx->n_cu, s);
     else
       launch_relight_batch(b.args.as<RelightArgs>(), b.items.as<BatchItem>(), b.meta.as<int>(),
                            counter, elem, b.softplus, b.has_aux, variant, ctx->n_cu, s);
@@ -549,7 +554,7 @@ void capture_batch(nwprt_ctx ctx, Batch &b, int variant) {
   }
   hip_check(hipStreamEndCapture(s, &b.graph), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-  b.variant = fp8 ? 0 : variant;
+  b.variant = variant;
 }
 
 size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -674,7 +679,7 @@ void create_batch(nwprt_ctx ctx, const nwprt_batch_job *jobs, int n, nwprt_preci
   // each launch makes its stream wait for them.
   for (int j = 0; j < n; ++j)
     hip_check(hipEventSynchronize(get_env(ctx, b->jobs[j].env_id).ready), "hipEventSynchronize");
-  capture_batch(ctx, *b, kDefaultBatchVariant);
+  capture_batch(ctx, *b, fp8 ? kDefaultFp8Variant : kDefaultBatchVariant);
   const int id = ctx->next_id++;
   ctx->batches[id] = std::move(b);
   *batch_id = id;
@@ -814,8 +819,8 @@ nwprt_status nwprt_envmap_update_faces(nwprt_ctx ctx, int32_t env_id, const floa
     } else {
       require(faces_location == NWPRT_DEVICE, NWPRT_ERR_INVALID_ARGUMENT, "bad faces location");
     }
-    launch_haar(dfaces, m * 3, e.coef.as<float>(), e.n, 1, s);
-    launch_topk(e.coef.as<float>(), e.n, e.k, e.scores.as<uint32_t>(), e.idx.as<int32_t>(),
+    launch_haar(dfaces, m * 3, e.coef.as<float>(), e.n, 1, e.scores.p, s);
+    launch_topk(e.coef.as<float>(), e.n, e.k, e.scores.p, e.idx.as<int32_t>(),
                 e.strength.as<float>(), s);
     hip_check(hipEventRecord(e.ready, s), "hipEventRecord");
     if (faces_location == NWPRT_HOST) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -837,8 +842,8 @@ nwprt_status nwprt_envmap_update_rotated(nwprt_ctx ctx, int32_t env_id, const fl
     const size_t m = 6ull * e.n * e.n;
     e.rotated.ensure(m * 12);
     launch_rotate_cubemap(faces, e.rotated.as<float>(), e.n, yaw, s);
-    launch_haar(e.rotated.as<float>(), m * 3, e.coef.as<float>(), e.n, 1, s);
-    launch_topk(e.coef.as<float>(), e.n, e.k, e.scores.as<uint32_t>(), e.idx.as<int32_t>(),
+    launch_haar(e.rotated.as<float>(), m * 3, e.coef.as<float>(), e.n, 1, e.scores.p, s);
+    launch_topk(e.coef.as<float>(), e.n, e.k, e.scores.p, e.idx.as<int32_t>(),
                 e.strength.as<float>(), s);
     hip_check(hipEventRecord(e.ready, s), "hipEventRecord");
   });
@@ -882,6 +887,26 @@ nwprt_status nwprt_envmap_load_folder(nwprt_ctx ctx, const char *folder, const c
       env_from_faces(ctx, faces.data(), NWPRT_HOST, n, k, &id);
     }
     *env_id = id;
+  });
+}
+
+nwprt_status nwprt_envmap_haar(nwprt_ctx ctx, const float *faces, int32_t face_size,
+                               int32_t n_maps, float *coefficients, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    require(faces && coefficients, NWPRT_ERR_INVALID_ARGUMENT, "haar: null buffer");
+    require(face_size >= 2 && face_size <= NWPRT_MAX_FACE_SIZE && (face_size & (face_size - 1)) == 0,
+            NWPRT_ERR_INVALID_ARGUMENT, "haar: face size must be a power of two in [2, 128]");
+    require(n_maps >= 1 && n_maps <= 65535, NWPRT_ERR_INVALID_ARGUMENT,
+            "haar: n_maps must be in [1, 65535]");
+    const size_t m = 6ull * face_size * face_size;
+    const char *f = (const char *)faces, *c = (const char *)coefficients;
+    const size_t bytes = m * 12 * n_maps;
+    require(c + bytes <= f || f + bytes <= c, NWPRT_ERR_INVALID_ARGUMENT,
+            "haar: coefficients overlap the faces");
+    ctx->haar_work.ensure(envmap_work_bytes(face_size, n_maps));
+    launch_haar(faces, m * 3, coefficients, face_size, n_maps, ctx->haar_work.p,
+                pick(ctx, stream));
   });
 }
 
@@ -1202,9 +1227,10 @@ nwprt_status nwprt_batch_autotune(nwprt_ctx ctx, int32_t batch_id, int32_t *best
     int best = -1;
     float best_t = 0.f;
     try {
-      for (int v = 0; v < relight_mfma_variant_count(); ++v) {
-        if (!relight_batch_variant_ok(v)) continue;
-        if (b.prec == NWPRT_PREC_FP8 && v > 0) break;  // one persistent FP8 form
+      const bool fp8 = b.prec == NWPRT_PREC_FP8;
+      const int nv = fp8 ? relight_fp8_variant_count() : relight_mfma_variant_count();
+      for (int v = 0; v < nv; ++v) {
+        if (!fp8 && !relight_batch_variant_ok(v)) continue;
         capture_batch(ctx, b, v);
         hip_check(hipGraphLaunch(b.exec, s), "hipGraphLaunch");
         float t = 0.f;

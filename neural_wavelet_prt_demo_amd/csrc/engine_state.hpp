@@ -67,7 +67,7 @@ struct EnvMap {
   int n = 0, k = 0;
   bool has_coef = false;
   DevBuf idx, strength, coef;
-  // Faces-built maps: top-K work buffer and, once rotated, the rotated faces
+  // Faces-built maps: Haar / top-K work buffer and, once rotated, the rotated faces
   // (per map, so updates of different maps on different streams never share
   // scratch memory).
   DevBuf scores, rotated;
@@ -175,6 +175,7 @@ struct nwprt_context {
   std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
   nwprt::DevBuf env_tables;  // direct-light sampling tables of the last cast
   nwprt::DevBuf sh_response, sh_colors;  // SH-PRT render work buffers
+  nwprt::DevBuf haar_work;  // nwprt_envmap_haar's tile scaling coefficients
 };
 
 namespace nwprt {

@@ -23,10 +23,6 @@ void hip_check(hipError_t e, const char *what) {
 
 namespace {
 
-constexpr int kHaarBlock = 256;
-constexpr int kHaarMaxPerThread =
-    (NWPRT_MAX_FACE_SIZE / 4) * (NWPRT_MAX_FACE_SIZE / 4) / kHaarBlock;  // 4
-
 // One 2x2 step of the orthonormal non-standard Haar transform.  a = x[2i][2j],
 // b = x[2i][2j+1], c = x[2i+1][2j], d = x[2i+1][2j+1] (row 0 = bottom row of
 // the face).  The row step then the column step, each with 1/sqrt(2), equal
@@ -44,64 +40,111 @@ __device__ __forceinline__ Haar4 haar2x2(double a, double b, double c, double d)
   return r;
 }
 
-// Grid: one workgroup per (map, face, channel).  The first level reads the
-// FP32 texels from global memory; the LL quadrant (FP64) of every later level
-// lives in LDS, (N/2)^2 doubles (32 KiB at N = 128).  Mallat layout: at a
-// level whose LL block is s x s (hs = s/2), the details of 2x2 block (i, j)
-// go to (i, hs+j) [h], (hs+i, j) [v], (hs+i, hs+j) [dg]; the final LL is the
-// DC coefficient at (0, 0) = face sum / N.
-__global__ __launch_bounds__(kHaarBlock) void haar_kernel(const float *__restrict__ faces,
-                                                          size_t map_stride,
-                                                          float *__restrict__ coef, int n) {
-  extern __shared__ double ll[];
-  const int id = blockIdx.x;
-  const int ch = id % 3, face = (id / 3) % 6, map = id / 18;
-  const size_t base = (size_t)map * map_stride + (size_t)face * n * n * 3 + ch;
+// Mallat layout: at a level whose LL block is s x s (hs = s / 2), the details
+// of 2x2 block (i, j) go to (i, hs + j) [h], (hs + i, j) [v] and
+// (hs + i, hs + j) [dg]; the final LL is the DC coefficient at (0, 0) = face
+// sum / N.  Each output is rounded to FP32 once.
+__device__ __forceinline__ void put_details(float *dst, int n, int hs, int i, int j,
+                                            const Haar4 &r) {
+  dst[((size_t)i * n + hs + j) * 3] = (float)r.h;
+  dst[((size_t)(hs + i) * n + j) * 3] = (float)r.v;
+  dst[((size_t)(hs + i) * n + hs + j) * 3] = (float)r.dg;
+}
+
+// The transform in two kernels.  A face splits into tiles of T x T texels
+// (T = min(N, 32)); the levels inside a tile need only its own texels.
+// haar_tile_kernel: one workgroup per (map, face, tile), all three channels:
+//   level 1 reads the tile's FP32 texels (one 2x2 block and its three
+//   channels per thread, rows read as contiguous 24-byte pairs), then the LL
+//   values (FP64, in LDS) go down to one per tile and channel, which goes to
+//   `ll_out` [map][face][tile][3] (FP64).
+// haar_top_kernel: one workgroup per (map, face) runs the levels above the
+//   tiles on that (N/T) x (N/T) LL block and writes the DC.
+// The per-block arithmetic is haar2x2 on the same FP64 values whatever the
+// tiling, so the coefficients are those of a whole-face transform, bit for
+// bit.  At N = 128 that is 96 workgroups per map instead of one per (face,
+// channel).
+constexpr int kHaarTile = 32;
+constexpr int kHaarTileBlock = (kHaarTile / 2) * (kHaarTile / 2);  // 256
+
+__global__ __launch_bounds__(kHaarTileBlock) void haar_tile_kernel(const float *__restrict__ faces,
+                                                                   size_t map_stride,
+                                                                   float *__restrict__ coef,
+                                                                   int n,
+                                                                   double *__restrict__ ll_out) {
+  __shared__ double ll[3][kHaarTileBlock];
+  const int T = n < kHaarTile ? n : kHaarTile, tpf = n / T, h1 = T / 2;
+  const int tile = blockIdx.x, face = blockIdx.y, map = blockIdx.z;
+  const int ty = tile / tpf, tx = tile % tpf;
+  const size_t base = (size_t)map * map_stride + (size_t)face * n * n * 3;
   const float *src = faces + base;
   float *dst = coef + base;
-  const int half = n / 2;
   const int tid = threadIdx.x;
-
-  for (int q = tid; q < half * half; q += kHaarBlock) {
-    const int i = q / half, j = q % half;
-    const double a = src[((size_t)(2 * i) * n + 2 * j) * 3];
-    const double b = src[((size_t)(2 * i) * n + 2 * j + 1) * 3];
-    const double c = src[((size_t)(2 * i + 1) * n + 2 * j) * 3];
-    const double d = src[((size_t)(2 * i + 1) * n + 2 * j + 1) * 3];
-    const Haar4 r = haar2x2(a, b, c, d);
-    ll[i * half + j] = r.s;
-    dst[((size_t)i * n + half + j) * 3] = (float)r.h;
-    dst[((size_t)(half + i) * n + j) * 3] = (float)r.v;
-    dst[((size_t)(half + i) * n + half + j) * 3] = (float)r.dg;
+  if (tid < h1 * h1) {
+    const int i = tid / h1, j = tid % h1;
+    const int gi = ty * h1 + i, gj = tx * h1 + j;
+    const float *r0 = src + ((size_t)(2 * gi) * n + 2 * gj) * 3;
+    const float *r1 = r0 + (size_t)n * 3;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const Haar4 r = haar2x2(r0[ch], r0[3 + ch], r1[ch], r1[3 + ch]);
+      ll[ch][i * h1 + j] = r.s;
+      put_details(dst + ch, n, n / 2, gi, gj, r);
+    }
   }
   __syncthreads();
-
-  for (int s = half; s >= 2; s >>= 1) {
-    const int hs = s >> 1;
-    double keep[kHaarMaxPerThread];
-#pragma unroll
-    for (int t = 0; t < kHaarMaxPerThread; ++t) {
-      const int q = tid + t * kHaarBlock;
-      if (q < hs * hs) {
-        const int i = q / hs, j = q % hs;
-        const Haar4 r = haar2x2(ll[(2 * i) * half + 2 * j], ll[(2 * i) * half + 2 * j + 1],
-                                ll[(2 * i + 1) * half + 2 * j],
-                                ll[(2 * i + 1) * half + 2 * j + 1]);
-        keep[t] = r.s;
-        dst[((size_t)i * n + hs + j) * 3] = (float)r.h;
-        dst[((size_t)(hs + i) * n + j) * 3] = (float)r.v;
-        dst[((size_t)(hs + i) * n + hs + j) * 3] = (float)r.dg;
-      }
+  // Local LL size ls -> ls / 2; the global LL at this level is tpf * ls.
+  for (int ls = h1; ls >= 2; ls >>= 1) {
+    const int hl = ls >> 1, hs = tpf * hl;
+    const bool act = tid < 3 * hl * hl;
+    const int ch = act ? tid / (hl * hl) : 0, q = tid % (hl * hl);
+    const int i = q / hl, j = q % hl;
+    double keep = 0.0;
+    if (act) {
+      const double *L = ll[ch];
+      const Haar4 r = haar2x2(L[(2 * i) * h1 + 2 * j], L[(2 * i) * h1 + 2 * j + 1],
+                              L[(2 * i + 1) * h1 + 2 * j], L[(2 * i + 1) * h1 + 2 * j + 1]);
+      keep = r.s;
+      put_details(dst + ch, n, hs, ty * hl + i, tx * hl + j, r);
     }
     __syncthreads();  // every read of this level is done before LL is overwritten
-#pragma unroll
-    for (int t = 0; t < kHaarMaxPerThread; ++t) {
-      const int q = tid + t * kHaarBlock;
-      if (q < hs * hs) ll[(q / hs) * half + (q % hs)] = keep[t];
-    }
+    if (act) ll[ch][i * h1 + j] = keep;
     __syncthreads();
   }
-  if (tid == 0) dst[0] = (float)ll[0];
+  if (tid < 3)
+    ll_out[(((size_t)map * 6 + face) * tpf * tpf + tile) * 3 + tid] = ll[tid][0];
+}
+
+__global__ __launch_bounds__(64) void haar_top_kernel(const double *__restrict__ ll_in,
+                                                      size_t map_stride,
+                                                      float *__restrict__ coef, int n) {
+  constexpr int kMaxTpf = NWPRT_MAX_FACE_SIZE / kHaarTile;  // 4
+  __shared__ double ll[3][kMaxTpf * kMaxTpf];
+  const int T = n < kHaarTile ? n : kHaarTile, tpf = n / T;
+  const int face = blockIdx.x, map = blockIdx.y;
+  float *dst = coef + (size_t)map * map_stride + (size_t)face * n * n * 3;
+  const int tid = threadIdx.x;
+  const double *src = ll_in + ((size_t)map * 6 + face) * tpf * tpf * 3;
+  if (tid < 3 * tpf * tpf) ll[tid % 3][tid / 3] = src[tid];
+  __syncthreads();
+  for (int ls = tpf; ls >= 2; ls >>= 1) {
+    const int hs = ls >> 1;
+    const bool act = tid < 3 * hs * hs;
+    const int ch = act ? tid / (hs * hs) : 0, q = tid % (hs * hs);
+    const int i = q / hs, j = q % hs;
+    double keep = 0.0;
+    if (act) {
+      const double *L = ll[ch];
+      const Haar4 r = haar2x2(L[(2 * i) * tpf + 2 * j], L[(2 * i) * tpf + 2 * j + 1],
+                              L[(2 * i + 1) * tpf + 2 * j], L[(2 * i + 1) * tpf + 2 * j + 1]);
+      keep = r.s;
+      put_details(dst + ch, n, hs, i, j, r);
+    }
+    __syncthreads();
+    if (act) ll[ch][i * tpf + j] = keep;
+    __syncthreads();
+  }
+  if (tid < 3) dst[tid] = (float)ll[tid][0];
 }
 
 // ------------------------------------------------------------------ top-K --
@@ -176,39 +219,55 @@ __device__ void select_digit(const uint32_t *hist, uint32_t *wt, int shift, uint
 // One workgroup: MSB-first radix select of the K-th largest key (8-bit
 // digits, early exit once a whole bucket is taken) -> compaction -> rank
 // sort of the K selected keys in LDS (descending).
-// - Scores come from score_kernel.  The first digit's histogram is built in
-//   one private copy per wave (keys bunch up in few exponent buckets, and
-//   one shared copy would serialise their atomics).
+// - Scores and the first digit's per-workgroup histograms come from
+//   score_kernel; this kernel sums the histograms.
 // - Keys above the chosen first-digit bucket are taken at once; keys in it
 //   (at most kTopkCand of them) move to LDS, and the remaining digits are
 //   resolved there.  A larger bucket falls back to passes over all keys.
 constexpr int kTopkCand = 3072;
-constexpr int kTopkWaves = kTopkBlock / 64;
 constexpr int kScoreBlock = 256;
 
-// Scores of all 6 n^2 coefficients, one thread each, across the machine
-// (the selection below then runs on one workgroup).
+// Scores of all 6 n^2 coefficients across the machine, kScoreKeys per
+// workgroup, and each workgroup's histogram of the keys' first digit (the
+// top byte of the score) written to wg_hist[blockIdx.x][256] -- per wave in
+// LDS first, since float exponents bunch keys into a few buckets.  The
+// selection below then starts from these histograms on one workgroup.
 __global__ __launch_bounds__(kScoreBlock) void score_kernel(const float *__restrict__ coef, int n,
-                                                            uint32_t *__restrict__ scores) {
-  const int t = blockIdx.x * kScoreBlock + threadIdx.x;
-  if (t < 6 * n * n) scores[t] = score_bits(coef, t, n);
+                                                            uint32_t *__restrict__ scores,
+                                                            uint32_t *__restrict__ wg_hist) {
+  __shared__ uint32_t wh[kScoreBlock / 64][256];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  for (int i = tid; i < (kScoreBlock / 64) * 256; i += kScoreBlock) (&wh[0][0])[i] = 0u;
+  __syncthreads();
+  const int M = 6 * n * n;
+  const int t0 = blockIdx.x * kScoreKeys;
+  for (int t = t0 + tid; t < t0 + kScoreKeys && t < M; t += kScoreBlock) {
+    const uint32_t sc = score_bits(coef, t, n);
+    scores[t] = sc;
+    atomicAdd(&wh[wave][sc >> 24], 1u);
+  }
+  __syncthreads();
+  uint32_t h = 0;
+#pragma unroll
+  for (int w = 0; w < kScoreBlock / 64; ++w) h += wh[w][tid];
+  wg_hist[(size_t)blockIdx.x * 256 + tid] = h;
 }
 
 __global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restrict__ coef, int n,
                                                           int K, uint32_t *__restrict__ scores,
+                                                          const uint32_t *__restrict__ wg_hist,
+                                                          int n_wg,
                                                           int32_t *__restrict__ out_idx,
                                                           float *__restrict__ out_str) {
-  __shared__ uint32_t whist[kTopkWaves][256];
   __shared__ uint32_t hist[256], wt[4];
   __shared__ uint64_t sel[NWPRT_MAX_K], sorted[NWPRT_MAX_K];
   __shared__ uint64_t cand[kTopkCand];
   __shared__ uint64_t s_prefix, s_mask;
   __shared__ uint32_t s_need, s_count, s_ncand;
   __shared__ int s_done;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x;
   const int M = 6 * n * n;
 
-  for (int i = tid; i < kTopkWaves * 256; i += kTopkBlock) (&whist[0][0])[i] = 0u;
   if (tid == 0) {
     s_prefix = 0;
     s_mask = 0;
@@ -217,12 +276,9 @@ __global__ __launch_bounds__(kTopkBlock) void topk_kernel(const float *__restric
     s_count = 0;
     s_ncand = 0;
   }
-  __syncthreads();
-  for (int t = tid; t < M; t += kTopkBlock) atomicAdd(&whist[wave][scores[t] >> 24], 1u);
-  __syncthreads();
   if (tid < 256) {
     uint32_t h = 0;
-    for (int w = 0; w < kTopkWaves; ++w) h += whist[w][tid];
+    for (int w = 0; w < n_wg; ++w) h += wg_hist[(size_t)w * 256 + tid];
     hist[tid] = h;
   }
   __syncthreads();
@@ -430,27 +486,40 @@ void launch_rotate_cubemap(const float *src, float *dst, int n, double yaw, hipS
   hip_check(hipGetLastError(), "rotate_cubemap_kernel launch");
 }
 
-void launch_haar(const float *faces, size_t map_stride, float *coef, int n, int n_maps,
-                 hipStream_t s) {
-  require(n >= 2 && n <= NWPRT_MAX_FACE_SIZE && (n & (n - 1)) == 0 && n_maps >= 1,
-          NWPRT_ERR_INVALID_ARGUMENT, "haar: bad face size");
-  const size_t lds = (size_t)(n / 2) * (n / 2) * sizeof(double);
-  hipLaunchKernelGGL(haar_kernel, dim3(n_maps * 18), dim3(kHaarBlock), lds, s, faces,
-                     map_stride, coef, n);
-  hip_check(hipGetLastError(), "haar_kernel launch");
+size_t envmap_work_bytes(int n, int n_maps) {
+  const size_t m = 6ull * n * n;
+  const size_t topk = (m + (m + kScoreKeys - 1) / kScoreKeys * 256) * 4;
+  const int T = n < kHaarTile ? n : kHaarTile;
+  const size_t haar = (size_t)n_maps * 6 * (n / T) * (n / T) * 3 * 8;
+  return topk > haar ? topk : haar;
 }
 
-void launch_topk(const float *coef, int n, int k, uint32_t *scores, int32_t *indices,
+void launch_haar(const float *faces, size_t map_stride, float *coef, int n, int n_maps,
+                 void *work, hipStream_t s) {
+  require(n >= 2 && n <= NWPRT_MAX_FACE_SIZE && (n & (n - 1)) == 0 && n_maps >= 1 && work &&
+              n_maps <= 65535,
+          NWPRT_ERR_INVALID_ARGUMENT, "haar: bad face size or map count");
+  const int T = n < kHaarTile ? n : kHaarTile, tpf = n / T;
+  hipLaunchKernelGGL(haar_tile_kernel, dim3(tpf * tpf, 6, n_maps), dim3(kHaarTileBlock), 0, s,
+                     faces, map_stride, coef, n, (double *)work);
+  hip_check(hipGetLastError(), "haar_tile_kernel launch");
+  hipLaunchKernelGGL(haar_top_kernel, dim3(6, n_maps), dim3(64), 0, s, (const double *)work,
+                     map_stride, coef, n);
+  hip_check(hipGetLastError(), "haar_top_kernel launch");
+}
+
+void launch_topk(const float *coef, int n, int k, void *work, int32_t *indices,
                  float *strengths, hipStream_t s) {
   require(k >= 1 && k <= NWPRT_MAX_K && k <= 6 * n * n, NWPRT_ERR_INVALID_ARGUMENT,
           "topk: K must be in [1, min(1024, 6 N^2)]");
-  require(n >= 2 && (n & (n - 1)) == 0, NWPRT_ERR_INVALID_ARGUMENT,
+  require(n >= 2 && (n & (n - 1)) == 0 && work, NWPRT_ERR_INVALID_ARGUMENT,
           "topk: face size must be a power of two");
-  hipLaunchKernelGGL(score_kernel, dim3((6 * n * n + kScoreBlock - 1) / kScoreBlock),
-                     dim3(kScoreBlock), 0, s, coef, n, scores);
+  const int m = 6 * n * n, n_wg = (m + kScoreKeys - 1) / kScoreKeys;
+  uint32_t *scores = (uint32_t *)work, *wg_hist = scores + m;
+  hipLaunchKernelGGL(score_kernel, dim3(n_wg), dim3(kScoreBlock), 0, s, coef, n, scores, wg_hist);
   hip_check(hipGetLastError(), "score_kernel launch");
-  hipLaunchKernelGGL(topk_kernel, dim3(1), dim3(kTopkBlock), 0, s, coef, n, k, scores,
-                     indices, strengths);
+  hipLaunchKernelGGL(topk_kernel, dim3(1), dim3(kTopkBlock), 0, s, coef, n, k, scores, wg_hist,
+                     n_wg, indices, strengths);
   hip_check(hipGetLastError(), "topk_kernel launch");
 }
 

@@ -16,16 +16,22 @@ enum class Elem : int { F32 = 0, F16 = 1, BF16 = 2 };
 
 // ---------------------------------------------------------------- envmap --
 
+// Work buffer of the Haar transform (n_maps maps) and of the top-K (one map)
+// at face size n: envmap_work_bytes(n, n_maps) bytes.  The two may share one
+// buffer when they run in stream order.
+constexpr int kScoreKeys = 2048;  // keys per score workgroup
+size_t envmap_work_bytes(int n, int n_maps);
+
 // Non-standard 2-D Haar of `n_maps` cubemaps (faces [6][N][N][3] FP32 each,
-// maps `map_stride` floats apart) into coefficients of the same layout.
-// One workgroup per (map, face, channel); scaling coefficients in FP64.
+// maps `map_stride` floats apart) into coefficients of the same layout:
+// tiles of 32 x 32 texels per workgroup, then the levels above the tiles;
+// scaling coefficients in FP64.
 void launch_haar(const float *faces, size_t map_stride, float *coef, int n,
-                 int n_maps, hipStream_t s);
+                 int n_maps, void *work, hipStream_t s);
 
 // Area-weighted top-K of one map's coefficients: indices int32[K] in
 // descending (score, then ascending index) order and strengths float[K][3].
-// `scores` is a uint32[6N^2] work buffer.
-void launch_topk(const float *coef, int n, int k, uint32_t *scores,
+void launch_topk(const float *coef, int n, int k, void *work,
                  int32_t *indices, float *strengths, hipStream_t s);
 
 // Yaw rotation (radians, about +y) of a cubemap [6][N][N][3] into `dst`
@@ -297,13 +303,17 @@ void launch_calib_encode(const RelightArgs &enc, int stride, uint16_t *calib, in
 // calib, aux = calib_aux with waux_tiles, n_pixels = n_cal) into a.amax[0..2].
 void launch_calibrate_amax(const RelightArgs &a, hipStream_t s);
 int relight_fp8_variant_count();
+// FP8 variant used when none was tuned or chosen (batches, servers, untuned
+// shapes): the software-pipelined form.
+constexpr int kDefaultFp8Variant = 1;
 const char *relight_fp8_variant_name(int variant);
 void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s);
 // Persistent FP8 batch form (items and counter as launch_relight_batch);
-// has_aux: some job has aux inputs.
+// has_aux: some job has aux inputs; variant: an FP8 variant (all have a
+// persistent form).
 void launch_relight_fp8_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
-                              unsigned int *counter, bool softplus, bool has_aux, int n_cu,
-                              hipStream_t s);
+                              unsigned int *counter, bool softplus, bool has_aux, int variant,
+                              int n_cu, hipStream_t s);
 void build_fp8_aux_tiles(const float *waux, std::vector<uint16_t> &out);
 // OCP E4M3 encode (round to nearest even, saturating at +-448) / decode.
 uint8_t f32_to_e4m3(float x);

@@ -84,6 +84,19 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
   s = t;
 }
 
+// Scheduling hint for one pipeline stage: NM MFMAs, each followed by its
+// share of the stage's NV vector-ALU instructions, so the matrix core and
+// the VALU work on different pixel groups at the same time.
+template <int NM, int NV>
+__device__ __forceinline__ void interleave_stage() {
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, (NV + NM - 1) / NM, 0);  // VALU
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // Levels l0 .. l0 + NL - 1 of one pixel, packed into the planar slot order
 // (u32 2q = level q, 2q + 1 = level 15 - q).
 template <typename E, int NL, bool HF16>

@@ -72,7 +72,10 @@ __device__ __forceinline__ i32x8 relu_fp8(const f32x16 &lo, const f32x16 &hi) {
     const float q1 = __builtin_amdgcn_fmed3f(src[v + 1], 0.f, 448.f);
     const float q2 = __builtin_amdgcn_fmed3f(src[v + 2], 0.f, 448.f);
     const float q3 = __builtin_amdgcn_fmed3f(src[v + 3], 0.f, 448.f);
-    int w = __builtin_amdgcn_cvt_pk_fp8_f32(q0, q1, 0, false);
+    // The first convert fills the low word; its high word is replaced by the
+    // second, so its "old" operand may be anything: q0's own bits let the
+    // result take q0's register instead of a zeroed one (no move).
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(q0, q1, __float_as_int(q0), false);
     w = __builtin_amdgcn_cvt_pk_fp8_f32(q2, q3, w, true);
     x[d] = w;
   }
@@ -115,7 +118,7 @@ __device__ __forceinline__ u32x4 scale_bf16x8(u32x4 v, float s) {
 // pixel's aux inputs, is pre-multiplied by 2^ea2, so it adds in the scaled
 // domain of that result.  FUSE: the fused-encode path is compiled in (taken
 // when a.fused_encode).
-template <bool SOFTPLUS, bool DUMP, bool AUX, bool FUSE>
+template <bool SOFTPLUS, bool DUMP, bool AUX, bool FUSE, bool PIPE>
 __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base, int lane,
                                                  const i32x8 *w4s, uint32_t *feat_lds) {
   const int h = lane >> 5, col = lane & 31;
@@ -180,7 +183,48 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
     // The four layers of one wavelet over the wave's 64 pixels.  The output
     // layer's variant b carries W4 in rows 4b..4b+2 and takes group b, so
     // lane l ends with pixel base + l's outputs in elements 0..2.
+    // Software-pipelined form (PIPE): the two pixel groups go through the
+    // layers one stage apart, and every stage pairs one group's MFMAs with
+    // the other group's ReLU + convert:
+    //   M1(0) | M1(1) R(0) | M2(0) R(1) | M2(1) R(0) | M3(0) R(1) |
+    //   M3(1) R(0) | M4(0) R(1) | M4(1)
+    // Per-lane arithmetic and its order are the plain step's.
+    auto step_pipe = [&](const u32x4(&t)[4]) -> f32x16 {
+      const f32x16 z = {};
+      f32x16 d[2][2], o = z;
+      i32x8 x[2];
+      auto M1 = [&](int g) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          d[g][b] = mfma_bf16(t[2 * b + 1], f[g][1], mfma_bf16(t[2 * b], f[g][0], z));
+      };
+      auto R = [&](int g) { x[g] = relu_fp8(d[g][0], d[g][1]); };
+      auto M2 = [&](int g) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          d[g][b] = mfma_fp8(w2[b], x[g], z, sa2, sb2);
+          if (AUX) d[g][b] = mfma_bf16(waux[b], auxb[g], d[g][b]);
+        }
+      };
+      auto M3 = [&](int g) {
+#pragma unroll
+        for (int b = 0; b < 2; ++b) d[g][b] = mfma_fp8(w3[b], x[g], z, sa3, sb3);
+      };
+      auto M4 = [&](int g) { o = mfma_fp8(w4s[64 * g + lane], x[g], o, sa4, sb4); };
+      constexpr int kR = 48;  // VALU of one group's ReLU + convert
+      M1(0);
+      __builtin_amdgcn_sched_barrier(0);
+      M1(1); R(0); dev::interleave_stage<4, kR>();
+      M2(0); R(1); dev::interleave_stage<AUX ? 4 : 2, kR>();
+      M2(1); R(0); dev::interleave_stage<AUX ? 4 : 2, kR>();
+      M3(0); R(1); dev::interleave_stage<2, kR>();
+      M3(1); R(0); dev::interleave_stage<2, kR>();
+      M4(0); R(1); dev::interleave_stage<1, kR>();
+      M4(1);
+      return o;
+    };
     auto step = [&](const u32x4(&t)[4]) -> f32x16 {
+      if constexpr (PIPE) return step_pipe(t);
       const f32x16 z = {};
       f32x16 d[2][2];
       i32x8 x[2];
@@ -262,7 +306,7 @@ __device__ __forceinline__ void load_w4(const RelightArgs &a, i32x8 *w4s, int la
 
 // Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
 // pixels [64 (4 b + w), +64).
-template <bool SOFTPLUS, bool DUMP, bool AUX>
+template <bool SOFTPLUS, bool DUMP, bool AUX, bool PIPE>
 __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
   __shared__ i32x8 w4s[4][2 * 64];
   __shared__ uint32_t feat_lds[AUX ? 1 : 4 * 64 * 16];  // fused encode: [wave][pixel][16]
@@ -270,8 +314,8 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
   const int base = (blockIdx.x * 4 + wave) * 64;
   if (base >= a.n_pixels) return;
   load_w4(a, w4s[wave], lane);
-  relight_fp8_wave<SOFTPLUS, DUMP, AUX, !AUX>(a, base, lane, w4s[wave],
-                                              feat_lds + (AUX ? 0 : wave * 64 * 16));
+  relight_fp8_wave<SOFTPLUS, DUMP, AUX, !AUX, PIPE>(a, base, lane, w4s[wave],
+                                                    feat_lds + (AUX ? 0 : wave * 64 * 16));
 }
 
 // Persistent batch form (as relight_batch_kernel): a machine-sized grid
@@ -279,7 +323,7 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
 // item reloads its job's output-layer tiles into the wave's own LDS copy.
 // HAS_AUX: jobs with aux inputs take the aux path (their features come from
 // the standalone encode); the others use the fused encode.
-template <bool SOFTPLUS, bool HAS_AUX>
+template <bool SOFTPLUS, bool HAS_AUX, bool PIPE>
 __global__ __launch_bounds__(kBlock, 2) void relight_fp8_batch_kernel(
     const RelightArgs *__restrict__ jobs, const BatchItem *__restrict__ items,
     const int *__restrict__ n_items_p, unsigned int *__restrict__ counter) {
@@ -302,16 +346,16 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_batch_kernel(
     const RelightArgs a = jobs[item.job];
     load_w4(a, w4s[wave], lane);
     if (HAS_AUX && a.aux)
-      relight_fp8_wave<SOFTPLUS, false, true, false>(a, item.base, lane, w4s[wave], my_lds);
+      relight_fp8_wave<SOFTPLUS, false, true, false, PIPE>(a, item.base, lane, w4s[wave], my_lds);
     else
-      relight_fp8_wave<SOFTPLUS, false, false, true>(a, item.base, lane, w4s[wave], my_lds);
+      relight_fp8_wave<SOFTPLUS, false, false, true, PIPE>(a, item.base, lane, w4s[wave], my_lds);
   }
 }
 
 struct Fp8Variant {
   const char *name;
 };
-constexpr Fp8Variant kFp8Variants[] = {{"fp8.l1bf16.occ2"}};
+constexpr Fp8Variant kFp8Variants[] = {{"fp8.l1bf16.occ2"}, {"fp8.pipe.occ2"}};
 constexpr int kNumFp8Variants = sizeof(kFp8Variants) / sizeof(kFp8Variants[0]);
 
 // Exponent e with amax * 2^e in (224, 448], 0 for a zero or non-finite amax.
@@ -398,51 +442,48 @@ void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s) {
   require(variant >= 0 && variant < kNumFp8Variants, NWPRT_ERR_INVALID_ARGUMENT,
           "relight: unknown FP8 kernel variant");
   const dim3 grid((a.n_pixels + kBlock - 1) / kBlock), block(kBlock);
-  const bool aux = a.aux != nullptr;
-#define NWPRT_FP8_LAUNCH(SP, DUMP, AUX) \
-  hipLaunchKernelGGL((relight_fp8_kernel<SP, DUMP, AUX>), grid, block, 0, s, a)
-  if (a.softplus) {
+  const bool aux = a.aux != nullptr, sp = a.softplus != 0, dump = a.dump != nullptr;
+  auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, s, a); };
+#define NWPRT_FP8_PICK(SP, DUMP, AUX)                               \
+  if (variant == 1) go(relight_fp8_kernel<SP, DUMP, AUX, true>);    \
+  else go(relight_fp8_kernel<SP, DUMP, AUX, false>);
+  if (sp) {
     if (aux) {
-      if (a.dump) NWPRT_FP8_LAUNCH(true, true, true);
-      else NWPRT_FP8_LAUNCH(true, false, true);
+      if (dump) { NWPRT_FP8_PICK(true, true, true) } else { NWPRT_FP8_PICK(true, false, true) }
     } else {
-      if (a.dump) NWPRT_FP8_LAUNCH(true, true, false);
-      else NWPRT_FP8_LAUNCH(true, false, false);
+      if (dump) { NWPRT_FP8_PICK(true, true, false) } else { NWPRT_FP8_PICK(true, false, false) }
     }
   } else {
     if (aux) {
-      if (a.dump) NWPRT_FP8_LAUNCH(false, true, true);
-      else NWPRT_FP8_LAUNCH(false, false, true);
+      if (dump) { NWPRT_FP8_PICK(false, true, true) } else { NWPRT_FP8_PICK(false, false, true) }
     } else {
-      if (a.dump) NWPRT_FP8_LAUNCH(false, true, false);
-      else NWPRT_FP8_LAUNCH(false, false, false);
+      if (dump) { NWPRT_FP8_PICK(false, true, false) } else { NWPRT_FP8_PICK(false, false, false) }
     }
   }
-#undef NWPRT_FP8_LAUNCH
+#undef NWPRT_FP8_PICK
   hip_check(hipGetLastError(), "relight_fp8_kernel launch");
 }
 
 void launch_relight_fp8_batch(const RelightArgs *jobs, const BatchItem *items, const int *n_items,
-                              unsigned int *counter, bool softplus, bool has_aux, int n_cu,
-                              hipStream_t s) {
+                              unsigned int *counter, bool softplus, bool has_aux, int variant,
+                              int n_cu, hipStream_t s) {
   require(jobs && items && n_items && counter && n_cu >= 1, NWPRT_ERR_INVALID_ARGUMENT,
           "batch: missing operand");
+  require(variant >= 0 && variant < kNumFp8Variants, NWPRT_ERR_INVALID_ARGUMENT,
+          "batch: unknown FP8 kernel variant");
   const dim3 grid(n_cu * 2), block(kBlock);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, jobs, items, n_items, counter);
+  };
+#define NWPRT_FP8B_PICK(SP, AUX)                                 \
+  if (variant == 1) go(relight_fp8_batch_kernel<SP, AUX, true>); \
+  else go(relight_fp8_batch_kernel<SP, AUX, false>);
   if (softplus) {
-    if (has_aux)
-      hipLaunchKernelGGL((relight_fp8_batch_kernel<true, true>), grid, block, 0, s, jobs, items,
-                         n_items, counter);
-    else
-      hipLaunchKernelGGL((relight_fp8_batch_kernel<true, false>), grid, block, 0, s, jobs, items,
-                         n_items, counter);
+    if (has_aux) { NWPRT_FP8B_PICK(true, true) } else { NWPRT_FP8B_PICK(true, false) }
   } else {
-    if (has_aux)
-      hipLaunchKernelGGL((relight_fp8_batch_kernel<false, true>), grid, block, 0, s, jobs, items,
-                         n_items, counter);
-    else
-      hipLaunchKernelGGL((relight_fp8_batch_kernel<false, false>), grid, block, 0, s, jobs, items,
-                         n_items, counter);
+    if (has_aux) { NWPRT_FP8B_PICK(false, true) } else { NWPRT_FP8B_PICK(false, false) }
   }
+#undef NWPRT_FP8B_PICK
   hip_check(hipGetLastError(), "relight_fp8_batch_kernel launch");
 }
 

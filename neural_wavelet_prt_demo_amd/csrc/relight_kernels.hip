@@ -177,19 +177,6 @@ __device__ __forceinline__ f32x4 wavelet_step(const u32x4 (&a1)[4], const u32x4 
   return out;
 }
 
-// Scheduling hint for one pipeline stage: NM MFMAs, each followed by its
-// share of the stage's NV vector-ALU instructions, so the matrix core and
-// the VALU work on different pixel groups at the same time.
-template <int NM, int NV>
-__device__ __forceinline__ void interleave_stage() {
-#pragma unroll
-  for (int i = 0; i < NM; ++i) {
-    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);                  // MFMA
-    __builtin_amdgcn_sched_group_barrier(0x002, (NV + NM - 1) / NM, 0);  // VALU
-  }
-  __builtin_amdgcn_sched_barrier(0);
-}
-
 // Software-pipelined form of wavelet_step: the four pixel groups flow
 // through the four layers two at a time, and every stage pairs one group's
 // MFMAs (M) with another group's ReLU + pack (C):
@@ -240,18 +227,18 @@ __device__ __forceinline__ f32x4 wavelet_step_pipelined(
   };
   M1(0);
   __builtin_amdgcn_sched_barrier(0);
-  M1(1); C(0); interleave_stage<4, 16>();
-  M2(0); C(1); interleave_stage<8 + kAux, 16>();
-  M2(1); C(0); interleave_stage<8 + kAux, 16>();
-  M3(0); C(1); interleave_stage<8, 16>();
-  M3(1); C(0); interleave_stage<8, 16>();
-  M4(0); M1(2); C(1); interleave_stage<6, 16>();
-  M4(1); M1(3); C(2); interleave_stage<6, 16>();
-  M2(2); C(3); interleave_stage<8 + kAux, 16>();
-  M2(3); C(2); interleave_stage<8 + kAux, 16>();
-  M3(2); C(3); interleave_stage<8, 16>();
-  M3(3); C(2); interleave_stage<8, 16>();
-  M4(2); C(3); interleave_stage<2, 16>();
+  M1(1); C(0); dev::interleave_stage<4, 16>();
+  M2(0); C(1); dev::interleave_stage<8 + kAux, 16>();
+  M2(1); C(0); dev::interleave_stage<8 + kAux, 16>();
+  M3(0); C(1); dev::interleave_stage<8, 16>();
+  M3(1); C(0); dev::interleave_stage<8, 16>();
+  M4(0); M1(2); C(1); dev::interleave_stage<6, 16>();
+  M4(1); M1(3); C(2); dev::interleave_stage<6, 16>();
+  M2(2); C(3); dev::interleave_stage<8 + kAux, 16>();
+  M2(3); C(2); dev::interleave_stage<8 + kAux, 16>();
+  M3(2); C(3); dev::interleave_stage<8, 16>();
+  M3(3); C(2); dev::interleave_stage<8, 16>();
+  M4(2); C(3); dev::interleave_stage<2, 16>();
   M4(3);
   return out;
 }

@@ -205,7 +205,7 @@ Layout request_layout(const Server &sv, const Scene &sc, const Request &q) {
     const size_t m = 6ull * q.N * q.N;
     l.faces = take(m * 12);
     l.coef = take(m * 12);
-    l.scores = take(m * 4);
+    l.scores = take(envmap_work_bytes(q.N, 1));
     l.idx = take((size_t)q.K * 4);
   }
   l.total = o;
@@ -296,8 +296,8 @@ void prepare(Server &sv, Request &q) {
                 "hipMemcpyAsync faces");
       src = faces;
     }
-    launch_haar(src, m * 3, coef, q.N, 1, s);
-    launch_topk(coef, q.N, q.K, q.at<uint32_t>(l.scores), bidx, strength, s);
+    launch_haar(src, m * 3, coef, q.N, 1, q.at<void>(l.scores), s);
+    launch_topk(coef, q.N, q.K, q.at<void>(l.scores), bidx, strength, s);
     idx = bidx;
   }
   void *features = q.fused ? nullptr : q.at<void>(l.features);
@@ -476,8 +476,8 @@ void launch_ticks(Server &sv) {
     launch_reset_counter(counter, sv.launch);
     if (sv.cfg.precision == NWPRT_PREC_FP8)
       launch_relight_fp8_batch(sv.jobs.as<RelightArgs>(), t.items.as<BatchItem>(),
-                               t.meta.as<int>(), counter, sv.softplus, has_aux, sv.ctx->n_cu,
-                               sv.launch);
+                               t.meta.as<int>(), counter, sv.softplus, has_aux, sv.variant,
+                               sv.ctx->n_cu, sv.launch);
     else
       launch_relight_batch(sv.jobs.as<RelightArgs>(), t.items.as<BatchItem>(), t.meta.as<int>(),
                            counter, elem_of(sv.cfg.precision), sv.softplus, has_aux, sv.variant,
@@ -519,9 +519,10 @@ nwprt_status nwprt_server_create(nwprt_ctx ctx, const nwprt_server_config *confi
     require(c.tonemap == NWPRT_TONEMAP_FULL || c.tonemap == NWPRT_TONEMAP_INDIRECT ||
                 c.tonemap == NWPRT_TONEMAP_NONE,
             NWPRT_ERR_INVALID_ARGUMENT, "unknown tonemap source");
-    const int variant = c.variant < 0 ? kDefaultVariant : c.variant;
-    require(relight_batch_variant_ok(variant), NWPRT_ERR_INVALID_ARGUMENT,
-            "server: tile variant has no persistent form");
+    const bool fp8 = c.precision == NWPRT_PREC_FP8;
+    const int variant = c.variant >= 0 ? c.variant : fp8 ? kDefaultFp8Variant : kDefaultVariant;
+    require(fp8 ? variant < relight_fp8_variant_count() : relight_batch_variant_ok(variant),
+            NWPRT_ERR_INVALID_ARGUMENT, "server: tile variant has no persistent form");
     ServerPtr sv(new Server(), destroy_server);
     sv->ctx = ctx;
     sv->cfg = c;

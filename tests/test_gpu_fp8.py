@@ -112,6 +112,30 @@ def test_fp8_fused_encode_equals_p1_p2(nw, orc, ctx):
             assert np.array_equal(fused[key], split[key]), (w, h, key)
 
 
+def test_fp8_variants_agree(nw, orc, ctx):
+    """The plain and the software-pipelined FP8 kernels run the same
+    per-lane arithmetic in the same order: identical outputs, also with aux
+    inputs."""
+    for aux in (False, True):
+        case = make_case(nw, orc, width=150, height=37, k=45, aux=aux, seed=21)
+        scene = ctx.scene_create(case.desc, case.params)
+        env = ctx.envmap_from_faces(case.faces, case.k)
+        ref = None
+        try:
+            for v in range(len(nw.Context.relight_fp8_variants())):
+                ctx.set_relight_variant(v)
+                frame = nw.DeviceFrame(case.width, case.height, case.gbuf)
+                ctx.relight(scene, env, frame, nw.PREC_FP8)
+                ctx.sync()
+                res = frame.results()
+                if ref is None:
+                    ref = res
+                for key in ref:
+                    assert np.array_equal(ref[key], res[key]), (aux, v, key)
+        finally:
+            ctx.set_relight_variant(-1)
+
+
 def test_fp8_errors_and_autotune(nw, orc, ctx):
     case = make_case(nw, orc, width=64, height=32, k=24, seed=5)
     scene = ctx.scene_create(case.desc, case.params)

@@ -47,6 +47,27 @@ def test_haar_and_topk_bit_exact(nw, orc, ctx, n):
     ctx.envmap_destroy(env)
 
 
+@pytest.mark.parametrize("n,n_maps", [(2, 3), (8, 5), (32, 2), (64, 3), (128, 7)])
+def test_haar_many_maps_bit_exact(nw, orc, ctx, n, n_maps):
+    """nwprt_envmap_haar over several maps at once (tiles of 32 x 32 texels,
+    then the levels above them) equals the oracle's whole-face transform of
+    each map, bit for bit."""
+    from neural_wavelet_prt_demo_amd import DeviceBuffer
+    rng = np.random.default_rng(n * 10 + n_maps)
+    faces = rng.lognormal(0, 1.5, (n_maps, 6, n, n, 3)).astype(np.float32)
+    faces[0, 2, n // 2, n // 3] = 1e3  # a sun texel
+    dfaces = DeviceBuffer.from_array(faces)
+    dcoef = DeviceBuffer(faces.nbytes)
+    ctx.envmap_haar(dfaces, n, n_maps, dcoef)
+    ctx.sync()
+    got = dcoef.download(np.float32, faces.shape)
+    for m in range(n_maps):
+        assert np.array_equal(got[m].view(np.uint32), orc.envmap_haar(faces[m]).view(np.uint32)), m
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.envmap_haar(dfaces, n, n_maps, dfaces)   # in place
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+
+
 def test_topk_ties_break_by_index(nw, orc, ctx):
     n = 16
     faces = np.ones((6, n, n, 3), np.float32)  # all DCs equal, all details 0

@@ -106,7 +106,7 @@ def _same(a, b, what):
 @pytest.mark.parametrize("precision", ["bf16", "fp8"])
 def test_server_alone_equals_mix(nw, orc, ctx, precision):
     st = _Setup(nw, orc, ctx)
-    variant = 5
+    variant = 1 if precision == "fp8" else 5  # the pipelined form of each
     prec = nw.PRECISIONS[precision]
     refs = [st.reference(r, variant, precision) for r in st.reqs]
     # Alone: one request per server lifetime.
@@ -147,9 +147,10 @@ def test_server_memory_budget_admission(nw, orc, ctx):
     st = _Setup(nw, orc, ctx)
     variant = 5
     # the largest request's block: features and aux inputs, tiles, strengths,
-    # cubemap / coefficients / scores / indices, each rounded up to 256 bytes
+    # cubemap / coefficients / scores (+ one 1 KB score histogram) / indices,
+    # each rounded up to 256 bytes
     big = max(r["w"] * r["h"] * 96 + r["k"] * 4096 + 6 * N * N * 28 + r["k"] * 16 + 8 * 256
-              for r in st.reqs)
+              + 1024 for r in st.reqs)
     srv = ctx.server_create(budget_bytes=big, max_active=8, tick_pairs=64 * 300 * 2,
                             variant=variant)
     frames = [st.frame(r) for r in st.reqs]
