@@ -459,6 +459,7 @@ typedef struct {
   int64_t triangles;
   int64_t bvh_nodes;
   int32_t bvh_depth;        /* internal levels, <= 24                        */
+  int64_t vertices;
 } nwprt_mesh_info;
 
 /* Uploads a triangle mesh and builds its BVH on the host (binned SAH, at
@@ -572,6 +573,39 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
                              const float *light_sh, int32_t degree, int32_t brdf_samples,
                              uint32_t seed, const float *view_env_faces, int32_t view_env_n,
                              float *color, void *stream);
+
+/* RenderEngine::shComputeTMatrix (Renderer/RenderEngine.cpp:410-446) with
+ * its ray generation program __raygen__ComputeTMatrix
+ * (Renderer/device_functions/cameraShTmatrix.cu:87-178): the transfer
+ * matrices nwprt_sh_render reads, for the vertices [vertex_begin,
+ * vertex_begin + vertex_count) (vertex_count -1: to the last vertex).
+ * basis_faces: device [nc][6][face_n][face_n][3] FP32, the nc = (degree+1)^2
+ * basis maps in order (SHRenderer.cpp:12-22 loads them from a folder of
+ * .npy crosses in natural order); map i is at once the view, lighting and
+ * sampling environment of its pass.  Per vertex, `samples` paths (4096 in
+ * the reference) start at the vertex position in uniform sphere directions
+ * and follow __closesthit__radiance / __miss__radiance
+ * (devicePrograms.cu:47-311, 517-564): one environment-importance light
+ * sample per non-mirror hit (power heuristic; max_bounce 1 adds the BSDF
+ * light sample), BSDF continuations, mirrors reflect, Russian roulette after
+ * depth 3, at most max_bounce rays (12 in the reference, 1..15 here); the
+ * environment radiance that reached the vertex on the first ray, or on the
+ * second after a mirror, is removed.  transfer: device
+ * [n_vertices][nc * nc][3], row k * nc + i = SH coefficient k (of the
+ * filtered basis 1 / (1 + 0.005 l^2 (l+1)^2) Y_k at the start direction)
+ * of the indirect radiance under basis map i, times 4 pi / samples; rows of
+ * other vertices are left alone.  Random numbers come from the
+ * counter-based generator keyed by (seed, vertex), the same for every
+ * basis map, as the reference's per-vertex stream is: estimates agree with
+ * the reference's in expectation, not sample by sample.  Materials:
+ * roughPlastic, cookTorrance, mirror, simple_ggx (DisneyBSDF gives
+ * NWPRT_ERR_UNSUPPORTED).  The sampling tables and path records live in work
+ * buffers of the context: calls issued on different streams must not
+ * overlap. */
+nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const float *basis_faces,
+                                       int32_t face_n, int32_t degree, int32_t samples,
+                                       int32_t max_bounce, uint32_t seed, int64_t vertex_begin,
+                                       int64_t vertex_count, float *transfer, void *stream);
 
 /* ------------------------------------------------------------ debug taps -- */
 

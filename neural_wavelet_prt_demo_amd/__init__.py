@@ -13,6 +13,8 @@ weight folder (Renderer/tinycudann/tcnn.cu:49), ``TCNNPass_p1`` /
 from __future__ import annotations
 
 import ctypes
+import os
+import re
 from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int, c_int32, c_int64,
                     c_size_t, c_uint8, c_uint32, c_uint64, c_void_p)
 from pathlib import Path
@@ -128,7 +130,7 @@ class Material(Structure):
 
 class MeshInfo(Structure):
     _fields_ = [("lower", c_float * 3), ("upper", c_float * 3), ("triangles", c_int64),
-                ("bvh_nodes", c_int64), ("bvh_depth", c_int32)]
+                ("bvh_nodes", c_int64), ("bvh_depth", c_int32), ("vertices", c_int64)]
 
 
 class CastParams(Structure):
@@ -224,6 +226,9 @@ _SIGNATURES = {
     "nwprt_sh_render": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32, c_void_p,
                                 c_void_p, c_int32, c_int32, c_uint32, c_void_p, c_int32, c_void_p,
                                 c_void_p]),
+    "nwprt_sh_compute_transfer": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
+                                          c_int32, c_uint32, ctypes.c_int64, ctypes.c_int64,
+                                          c_void_p, c_void_p]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -430,6 +435,96 @@ def cross_to_faces(cross: np.ndarray) -> np.ndarray:
     faces = np.empty((6, h // 3, h // 3, 3), np.float32)
     _check(lib().nwprt_cubemap_cross_to_faces(_ptr(cross), w, h, _ptr(faces)))
     return faces
+
+
+def _natural_key(name: str):
+    """compareNat (Renderer/files.h:27-58): digit runs compare as numbers
+    and sort before letters; letters compare case-insensitively."""
+    key = []
+    for part in re.split(r"(\d+)", name):
+        if not part:
+            continue
+        if part.isdigit():
+            key.append((0, int(part), ""))
+        else:
+            key.extend((1, 0, ch.upper()) for ch in part)
+    return key
+
+
+def sh_eval_real(dirs: np.ndarray, degree: int = 8) -> np.ndarray:
+    """pbrt's real SH basis (Renderer/device_functions/shaderUtils.h:230-337)
+    at unit directions dirs [..., 3]: [..., (degree + 1)^2] float64, index
+    l (l + 1) + m; associated Legendre functions with the Condon-Shortley
+    phase, sqrt(2) K P_l^|m| sin(|m| phi) for m < 0, K P_l^0 for m = 0,
+    sqrt(2) K P_l^m cos(m phi) for m > 0."""
+    d = np.asarray(dirs, np.float64)
+    x, y, z = d[..., 0], d[..., 1], d[..., 2]
+    L = degree
+    s = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    P = {}
+    P[(0, 0)] = np.ones_like(z)
+    for l in range(1, L + 1):
+        P[(l, l)] = -(2 * l - 1) * s * P[(l - 1, l - 1)]
+    for l in range(1, L + 1):
+        P[(l, l - 1)] = (2 * l - 1) * z * P[(l - 1, l - 1)]
+    for l in range(2, L + 1):
+        for m in range(0, l - 1):
+            P[(l, m)] = ((2 * l - 1) * z * P[(l - 1, m)] - (l - 1 + m) * P[(l - 2, m)]) / (l - m)
+    phi = np.arctan2(y, x)
+    out = np.empty(d.shape[:-1] + ((L + 1) ** 2,), np.float64)
+    for l in range(L + 1):
+        for m in range(-l, l + 1):
+            am = abs(m)
+            k = np.sqrt((2 * l + 1) / (4 * np.pi) * np.prod(
+                np.arange(l - am + 1, l + am + 1, dtype=np.float64)) ** -1.0)
+            if m < 0:
+                v = np.sqrt(2.0) * k * P[(l, am)] * np.sin(am * phi)
+            elif m == 0:
+                v = k * P[(l, 0)]
+            else:
+                v = np.sqrt(2.0) * k * P[(l, m)] * np.cos(m * phi)
+            out[..., l * l + l + m] = v
+    return out
+
+
+def cube_texel_dirs(n: int) -> np.ndarray:
+    """Unit directions of the texel centres of [6][n][n] cube faces (face
+    order +X, -X, +Y, -Y, -Z, +Z; EnvUVToDirec, Renderer/device_functions/
+    cubeMap.cu:94-110, at ((x + 0.5) / n, (y + 0.5) / n)): [6][n][n][3]."""
+    g = (np.arange(n) + 0.5) / n * 2.0 - 1.0
+    vc, uc = np.meshgrid(g, g, indexing="ij")
+    one = np.ones_like(uc)
+    faces = [(one, vc, uc), (-one, vc, -uc), (uc, one, vc), (uc, -one, -vc), (uc, vc, -one),
+             (-uc, vc, one)]
+    d = np.stack([np.stack(f, -1) for f in faces])
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def sh_basis_faces(degree: int, n: int) -> np.ndarray:
+    """Basis maps for the transfer computation: map k holds the real SH
+    basis function Y_k at every texel centre, in all three channels:
+    float32 [(degree + 1)^2][6][n][n][3]."""
+    y = sh_eval_real(cube_texel_dirs(n), degree)          # [6][n][n][nc]
+    return np.ascontiguousarray(np.repeat(np.moveaxis(y, -1, 0)[..., None], 3, -1),
+                                np.float32)
+
+
+def load_sh_basis_folder(folder: str, degree: int) -> np.ndarray:
+    """The basis maps of SHRenderer (Renderer/SHRenderer.cpp:12-22): the
+    .npy cubemap crosses of `folder` in natural order, each read as
+    Environment_Map reads one (EnvironmentMap.cpp:90-103, rows top first);
+    returns the first (degree + 1)^2 as float32 [nc][6][n][n][3]."""
+    nc = (degree + 1) ** 2
+    names = sorted((f for f in os.listdir(folder) if f.endswith(".npy")),
+                   key=lambda f: _natural_key(os.path.join(folder, f)))
+    if len(names) < nc:
+        raise NwprtError(ERR_INVALID_ARGUMENT,
+                         f"{folder}: {len(names)} basis maps, degree {degree} needs {nc}")
+    maps = []
+    for f in names[:nc]:
+        data, shape, _ = npy_read(os.path.join(folder, f))
+        maps.append(cross_to_faces(data.reshape(shape)))
+    return np.stack(maps)
 
 
 # ------------------------------------------------------------ device side --
@@ -711,7 +806,7 @@ class Context:
         _check(lib().nwprt_mesh_get_info(self.h, mesh, ctypes.byref(info)))
         return {"lower": list(info.lower), "upper": list(info.upper),
                 "triangles": info.triangles, "bvh_nodes": info.bvh_nodes,
-                "bvh_depth": info.bvh_depth}
+                "bvh_depth": info.bvh_depth, "vertices": info.vertices}
 
     def mesh_destroy(self, mesh: int) -> None:
         _check(lib().nwprt_mesh_destroy(self.h, mesh))
@@ -750,6 +845,27 @@ class Context:
                                      ptr(transfer), ptr(light_sh), degree, brdf_samples,
                                      seed & 0xFFFFFFFF, ptr(env_faces), env_n, ptr(out),
                                      c_void_p(stream) if stream else None))
+
+    def sh_compute_transfer(self, mesh: int, basis, face_n: int, degree: int, out=None, *,
+                            samples: int = 4096, max_bounce: int = 12, seed: int = 0,
+                            vertex_begin: int = 0, vertex_count: int = -1,
+                            stream: Optional[int] = None) -> "DeviceBuffer":
+        """Transfer matrices (RenderEngine::shComputeTMatrix) of the mesh's
+        vertices under the (degree + 1)^2 basis maps `basis` (device buffer
+        [nc][6][n][n][3] float32, or a host array that is uploaded) into
+        `out` (device [n_vertices][nc * nc][3] float32; allocated when
+        None), which is returned."""
+        ptr = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+        nc = (degree + 1) ** 2
+        if isinstance(basis, np.ndarray):
+            basis = DeviceBuffer.from_array(np.ascontiguousarray(basis, np.float32))
+        if out is None:
+            out = DeviceBuffer(self.mesh_info(mesh)["vertices"] * nc * nc * 3 * 4)
+        _check(lib().nwprt_sh_compute_transfer(self.h, mesh, ptr(basis), face_n, degree, samples,
+                                               max_bounce, seed & 0xFFFFFFFF, vertex_begin,
+                                               vertex_count, ptr(out),
+                                               c_void_p(stream) if stream else None))
+        return out
 
     # debug taps ----------------------------------------------------------
     def debug_hash_encode(self, scene: int, positions: np.ndarray):

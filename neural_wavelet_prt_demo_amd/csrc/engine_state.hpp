@@ -175,6 +175,9 @@ struct nwprt_context {
   std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
   nwprt::DevBuf env_tables;  // direct-light sampling tables of the last cast
   nwprt::DevBuf sh_response, sh_colors;  // SH-PRT render work buffers
+  // SH transfer computation: the basis maps' sampling tables and the path
+  // records, bases and per-(sample, map) radiance of one launch.
+  nwprt::DevBuf sh_tables, sh_work;
   nwprt::DevBuf haar_work;  // nwprt_envmap_haar's tile scaling coefficients
 };
 

@@ -1051,6 +1051,357 @@ __global__ __launch_bounds__(64) void sh_pixel_kernel(ShArgs a) {
   reinterpret_cast<float4 *>(a.color)[(size_t)iy * a.width + ix] = make_float4(c.x, c.y, c.z, 1.0f);
 }
 
+
+// ------------------------------------------------------------ SH transfer
+// RenderEngine::shComputeTMatrix (RenderEngine.cpp:410-446) with its ray
+// generation program __raygen__ComputeTMatrix (cameraShTmatrix.cu:87-178):
+// per vertex, `samples` paths start at the vertex in uniform sphere
+// directions; under each basis map i (at once the view, lighting and
+// sampling environment) a path gathers one environment-importance light
+// sample per non-mirror hit and the environment radiance where it leaves
+// the scene; the radiance that came straight from the environment (or
+// through one mirror) is removed and the rest is projected onto the
+// filtered SH basis of the start direction.
+//
+// The path's control flow -- hits, BSDF continuations, Russian roulette --
+// does not depend on the basis map (the light samples draw their own
+// counters of the stream, and the throughput is map-independent), so it is
+// traced once per (vertex, sample) and shared by all nc maps:
+//   tm_paths_kernel    one lane per (vertex, sample): closest-hit rays,
+//                      continuations and roulette; records each non-mirror
+//                      hit (position, view direction, normals, material,
+//                      throughput) and the exit ray;
+//   tm_light_kernel    one lane per (vertex, sample, map), consecutive lanes
+//                      on consecutive maps of one path: the light samples
+//                      (shadow rays) at the recorded hits, the exit
+//                      radiance, the removal of the direct part, giving the
+//                      indirect radiance of the (sample, map) pair;
+//   tm_project_kernel  one workgroup per vertex, one lane per (map, channel)
+//                      column: T[k][i][c] = sum over samples, in sample
+//                      order, of Y'_k(start direction) x indirect x 4 pi / N
+//                      (Y' the filtered basis), with the reference's
+//                      T += T / N for a NaN sample.
+// Random counters per (vertex, sample): sample * 256 + {0, 1} for the start
+// direction, + 16 (b + 1) + {0..2} for the light sample at trace b,
+// {3..5} for the BSDF light sample (max_bounce 1), {6..8} for the
+// continuation, 9 for the roulette.
+
+constexpr uint32_t kTmStride = 256;
+// Filtered SH basis of a start direction, padded to a 16-byte multiple.
+constexpr int kTmRow = (kShCoeffs + 3) & ~3;
+
+// A non-mirror hit of a path (64 bytes).  tag = material << 4 | trace index.
+struct TmRecord {
+  float p[3], v[3], ns[3], ng[3], lc[3];
+  int tag;
+};
+// max_bounce 1 only, in the slot after the hit: the BSDF light sample
+// (direction, pdf, BSDF value) and whether its shadow ray is clear.
+struct TmBsdfLight {
+  float l[3], pb, f[3];
+  int visible;
+  float pad[8];
+};
+static_assert(sizeof(TmRecord) == 64 && sizeof(TmBsdfLight) == 64, "record size");
+// How a path ends (32 bytes): the exit ray's direction, throughput and BSDF
+// pdf; info = records | traces << 4 | left the scene << 9 | mirror before
+// any other hit << 10.
+struct TmPath {
+  float dir[3], lc[3], pdf;
+  int info;
+};
+
+__host__ __device__ inline int tm_slots(int max_bounce) { return max_bounce == 1 ? 2 : max_bounce; }
+
+__host__ __device__ inline size_t tm_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct TmWork {
+  TmRecord *rec;  // [pair][slots]
+  TmPath *path;   // [pair]
+  float *y;       // [pair][kTmRow]: filtered basis of the start direction
+  float *ind;     // [pair][map][3]: indirect radiance
+};
+__host__ __device__ inline TmWork tm_work(void *base, long long pairs, int slots) {
+  char *p = (char *)base;
+  TmWork w;
+  w.rec = (TmRecord *)p;
+  p += tm_align((size_t)pairs * slots * sizeof(TmRecord));
+  w.path = (TmPath *)p;
+  p += tm_align((size_t)pairs * sizeof(TmPath));
+  w.y = (float *)p;
+  p += tm_align((size_t)pairs * kTmRow * sizeof(float));
+  w.ind = (float *)p;
+  return w;
+}
+
+// sample_sphere (shaderUtils.h:151-158).
+__device__ inline float3 tm_sphere(float z1, float z2) {
+  const float theta = 2.0f * 3.14159265358979f * z1, phi = acosf(1.0f - 2.0f * z2);
+  const float sp = sinf(phi);
+  return v3(sp * cosf(theta), sp * sinf(theta), cosf(phi));
+}
+
+__device__ inline float3 tm_bsdf_sample(const Surface &s, float3 v, uint32_t seed, uint32_t vtx,
+                                        uint32_t c) {
+  return s.type == 0 ? rp_sample(s, v, rng_uniform(seed, vtx, c), rng_uniform(seed, vtx, c + 1))
+                     : ct_sample(s, v, rng_uniform(seed, vtx, c), rng_uniform(seed, vtx, c + 1),
+                                 rng_uniform(seed, vtx, c + 2));
+}
+
+__device__ inline Surface tm_surface(const TmArgs &a, int mat, float3 ns, float3 ng) {
+  const MaterialRec m = a.materials[mat];
+  return Surface{v3(m.kd), v3(m.ks), m.roughness, ns, ng, m.type};
+}
+
+__global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
+  __shared__ int stack_lds[kBvhMaxDepth * 64];
+  const int lane = threadIdx.x;
+  const long long pair = (long long)blockIdx.x * 64 + lane;
+  if (pair >= a.n_verts * a.samples) return;  // no barriers below
+  int *stack = stack_lds + lane;
+  const int slots = tm_slots(a.max_bounce);
+  const TmWork w = tm_work(a.work, a.n_verts * a.samples, slots);
+  TmRecord *rec = w.rec + pair * slots;
+  const long long vl = pair / a.samples;
+  const int s = (int)(pair - vl * a.samples);
+  const uint32_t vtx = (uint32_t)(a.v_begin + vl);
+  const uint32_t base = (uint32_t)s * kTmStride;
+  float3 d = tm_sphere(rng_uniform(a.seed, vtx, base), rng_uniform(a.seed, vtx, base + 1));
+  float3 o = v3(a.vertices + 3 * (size_t)(a.v_begin + vl));
+  float3 lc = v3(1.0f, 1.0f, 1.0f);
+  float pdf = 0.0f;
+  bool first_hit = false, mirror_first = false, left = false, done = false;
+  int depth = 0, nrec = 0;
+  TmPath out{};
+  for (;;) {
+    const uint32_t c = base + 16u * (uint32_t)(depth + 1);
+    const float3 rd = normalize(d);
+    const Hit h = trace(a.nodes, a.tris, o, d, stack);
+    if (h.prim == INT_MAX) {
+      out.dir[0] = d.x, out.dir[1] = d.y, out.dir[2] = d.z;
+      out.lc[0] = lc.x, out.lc[1] = lc.y, out.lc[2] = lc.z;
+      out.pdf = pdf;
+      left = done = true;
+    } else {
+      const int mat = a.tri_material ? a.tri_material[h.prim] : 0;
+      const float4 *tr = reinterpret_cast<const float4 *>(a.tris + h.slot);
+      const float3 ng = normalize(cross(v3(tr[1]), v3(tr[2])));
+      float3 ns = ng;
+      if (a.normals) {
+        const int *tri = a.indices + 3 * (size_t)h.prim;
+        const float ww = 1.0f - h.u - h.v;
+        ns = normalize(add(add(scale(v3(a.normals + 3 * (size_t)tri[0]), ww),
+                               scale(v3(a.normals + 3 * (size_t)tri[1]), h.u)),
+                           scale(v3(a.normals + 3 * (size_t)tri[2]), h.v)));
+      }
+      const float3 p = add(add(o, scale(rd, h.t)), v3(0.001f, 0.001f, 0.001f));
+      o = p;
+      if (a.materials[mat].type == NWPRT_MATERIAL_MIRROR) {
+        d = reflect(rd, ns);
+        if (!first_hit) mirror_first = true;
+      } else {
+        const Surface sf = tm_surface(a, mat, ns, ng);
+        const float3 v = scale(rd, -1.0f);
+        TmRecord &r = rec[nrec++];
+        r.p[0] = p.x, r.p[1] = p.y, r.p[2] = p.z;
+        r.v[0] = v.x, r.v[1] = v.y, r.v[2] = v.z;
+        r.ns[0] = ns.x, r.ns[1] = ns.y, r.ns[2] = ns.z;
+        r.ng[0] = ng.x, r.ng[1] = ng.y, r.ng[2] = ng.z;
+        r.lc[0] = lc.x, r.lc[1] = lc.y, r.lc[2] = lc.z;
+        r.tag = mat << 4 | depth;
+        if (a.max_bounce == 1) {  // direct mode: BSDF light sample (devicePrograms.cu:202-247)
+          TmBsdfLight &bl = *reinterpret_cast<TmBsdfLight *>(rec + 1);
+          const float3 l = tm_bsdf_sample(sf, v, a.seed, vtx, c + 3);
+          bl.visible = 0;
+          if (l.x != -2.0f) {
+            const float3 f = bsdf_eval(sf, l, v);
+            bl.l[0] = l.x, bl.l[1] = l.y, bl.l[2] = l.z;
+            bl.pb = bsdf_pdf(sf, l, v);
+            bl.f[0] = f.x, bl.f[1] = f.y, bl.f[2] = f.z;
+            bl.visible = !occluded(a.nodes, a.tris, p, l, stack);
+          }
+        }
+        const float3 nd = tm_bsdf_sample(sf, v, a.seed, vtx, c + 6);
+        if (nd.x == -2.0f) {
+          done = true;
+        } else {
+          pdf = bsdf_pdf(sf, nd, v);
+          if (pdf <= 1e-4f) {
+            done = true;
+          } else {
+            const float3 f = bsdf_eval(sf, nd, v);
+            d = nd;
+            lc = v3(lc.x * f.x / pdf, lc.y * f.y / pdf, lc.z * f.z / pdf);
+          }
+        }
+        first_hit = true;
+      }
+    }
+    ++depth;
+    if (!done && depth > 3) {  // Russian roulette (cameraShTmatrix.cu:139-151)
+      const float z = rng_uniform(a.seed, vtx, c + 9);
+      const float rr = fminf(fmaxf(lc.x, fmaxf(lc.y, lc.z)), 0.95f);
+      if (z < rr) {
+        const float q = fmaxf(rr, 1e-10f);
+        lc = v3(lc.x / q, lc.y / q, lc.z / q);
+      } else {
+        done = true;
+      }
+    }
+    if (done || depth >= a.max_bounce) break;
+  }
+  out.info = nrec | depth << 4 | (left ? 1 << 9 : 0) | (mirror_first ? 1 << 10 : 0);
+  w.path[pair] = out;
+  // Filtered basis 1 / (1 + 0.005 l^2 (l+1)^2) Y_k of the start direction
+  // (cameraShTmatrix.cu:159-166).
+  float y[kShCoeffs];
+  sh_eval(tm_sphere(rng_uniform(a.seed, vtx, base), rng_uniform(a.seed, vtx, base + 1)), y);
+  float4 *yo = reinterpret_cast<float4 *>(w.y + pair * kTmRow);
+#pragma unroll
+  for (int k4 = 0; k4 < kTmRow / 4; ++k4) {
+    float q[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * k4 + e;
+      if (k < kShCoeffs) {
+        int l = 0;
+        while ((l + 1) * (l + 1) <= k) ++l;
+        const float fl = (float)l;
+        q[e] = 1.0f / (1.0f + 0.005f * fl * fl * (fl + 1.0f) * (fl + 1.0f)) * y[k];
+      } else {
+        q[e] = 0.0f;
+      }
+    }
+    yo[k4] = make_float4(q[0], q[1], q[2], q[3]);
+  }
+}
+
+__device__ inline bool tm_nan3(float3 c) { return isnan(c.x) || isnan(c.y) || isnan(c.z); }
+
+__global__ __launch_bounds__(64) void tm_light_kernel(TmArgs a) {
+  __shared__ int stack_lds[kBvhMaxDepth * 64];
+  const int lane = threadIdx.x;
+  const int nc = (a.degree + 1) * (a.degree + 1);
+  const long long q = (long long)blockIdx.x * 64 + lane;
+  const long long pairs = a.n_verts * a.samples;
+  if (q >= pairs * nc) return;  // no barriers below
+  int *stack = stack_lds + lane;
+  const int slots = tm_slots(a.max_bounce);
+  const TmWork w = tm_work(a.work, pairs, slots);
+  const long long pair = q / nc;
+  const int i = (int)(q - pair * nc);
+  const long long vl = pair / a.samples;
+  const int s = (int)(pair - vl * a.samples);
+  const uint32_t vtx = (uint32_t)(a.v_begin + vl);
+  const uint32_t base = (uint32_t)s * kTmStride;
+  const float *faces = a.basis + (size_t)i * 6 * a.n * a.n * 3;
+  const double *tab = a.tabs + (size_t)i * a.tab_doubles;
+  const TmPath path = w.path[pair];
+  const int nrec = path.info & 15, traces = (path.info >> 4) & 31;
+  const bool left = (path.info >> 9) & 1, mirror_first = (path.info >> 10) & 1;
+  const TmRecord *rec = w.rec + pair * slots;
+  float3 color = v3(0.0f, 0.0f, 0.0f);
+  for (int r = 0; r < nrec; ++r) {
+    const TmRecord h = rec[r];
+    const uint32_t c = base + 16u * (uint32_t)((h.tag & 15) + 1);
+    const float3 p = v3(h.p), v = v3(h.v), lc = v3(h.lc);
+    const float3 l = env_sample(tab, a.n, rng_uniform(a.seed, vtx, c),
+                                rng_uniform(a.seed, vtx, c + 1), rng_uniform(a.seed, vtx, c + 2));
+    const float pe = env_pdf(tab, a.n, l);
+    if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
+      const Surface sf = tm_surface(a, h.tag >> 4, v3(h.ns), v3(h.ng));
+      const float pb = bsdf_pdf(sf, l, v);
+      const float3 f = bsdf_eval(sf, l, v);
+      const float3 le = env_radiance(faces, a.n, l);
+      const float wt = pe / fmaxf(pe * pe + pb * pb, 1e-8f);
+      const float3 inc = v3(f.x * le.x * lc.x * wt, f.y * le.y * lc.y * wt, f.z * le.z * lc.z * wt);
+      if (!tm_nan3(inc)) color = add(color, inc);
+    }
+    if (a.max_bounce == 1) {
+      const TmBsdfLight bl = *reinterpret_cast<const TmBsdfLight *>(rec + 1);
+      if (bl.visible) {
+        const float3 l2 = v3(bl.l);
+        const float pe2 = env_pdf(tab, a.n, l2);
+        if (pe2 > 1e-4f) {
+          const float3 le = env_radiance(faces, a.n, l2);
+          const float wt = bl.pb / fmaxf(pe2 * pe2 + bl.pb * bl.pb, 1e-8f);
+          const float3 inc = v3(bl.f[0] * le.x * lc.x * wt, bl.f[1] * le.y * lc.y * wt,
+                                bl.f[2] * le.z * lc.z * wt);
+          if (!tm_nan3(inc)) color = add(color, inc);
+        }
+      }
+    }
+  }
+  if (left) {  // __miss__radiance (devicePrograms.cu:517-541)
+    const float3 d = v3(path.dir), lc = v3(path.lc);
+    const float3 le = env_radiance(faces, a.n, d);
+    if (traces == 1) {
+      color = le;
+    } else if (path.pdf < 0.0f) {
+      color = add(v3(le.x * lc.x, le.y * lc.y, le.z * lc.z), color);
+    } else {
+      const float pe = env_pdf(tab, a.n, d);
+      const float wt = path.pdf * path.pdf / fmaxf(path.pdf * path.pdf + pe * pe, 1e-14f);
+      color = add(v3(le.x * lc.x * wt, le.y * lc.y * wt, le.z * lc.z * wt), color);
+    }
+  }
+  // The direct part: the first ray, or the ray after a first mirror, that
+  // left the scene (cameraShTmatrix.cu:126-136).
+  float3 tent = v3(0.0f, 0.0f, 0.0f);
+  if (left && !tm_nan3(color) && (traces == 1 || (traces == 2 && mirror_first))) tent = color;
+  float *o = w.ind + q * 3;
+  o[0] = color.x - tent.x;
+  o[1] = color.y - tent.y;
+  o[2] = color.z - tent.z;
+}
+
+// The projection stages the filtered basis of 64 samples at a time in LDS;
+// lane (map, channel) then walks them in sample order.
+constexpr int kTmProjectSamples = 64;
+
+__global__ __launch_bounds__(256) void tm_project_kernel(TmArgs a) {
+  __shared__ float ylds[kTmProjectSamples][kTmRow];
+  const int t = threadIdx.x;
+  const int nc = (a.degree + 1) * (a.degree + 1), cols = 3 * nc;
+  const long long vl = blockIdx.x;
+  const long long pairs = a.n_verts * a.samples;
+  const TmWork w = tm_work(a.work, pairs, tm_slots(a.max_bounce));
+  const float *ind = w.ind + (size_t)vl * a.samples * cols;
+  const float fn = (float)a.samples, scale_n = 4.0f * 3.14159265358979f / fn;
+  float acc[kShCoeffs];
+#pragma unroll
+  for (int k = 0; k < kShCoeffs; ++k) acc[k] = 0.0f;
+  const int i = t / 3;
+  for (int s0 = 0; s0 < a.samples; s0 += kTmProjectSamples) {
+    const int m = min(kTmProjectSamples, a.samples - s0);
+    __syncthreads();
+    const float *ysrc = w.y + ((size_t)vl * a.samples + s0) * kTmRow;
+    for (int e = t; e < m * kTmRow; e += 256) (&ylds[0][0])[e] = ysrc[e];
+    __syncthreads();
+    if (t < cols) {
+      for (int j = 0; j < m; ++j) {
+        const float *row = ind + (size_t)(s0 + j) * cols;
+        const float x = row[t] * scale_n;
+        if (isnan(row[3 * i]) || isnan(row[3 * i + 1]) || isnan(row[3 * i + 2])) {
+#pragma unroll
+          for (int k = 0; k < kShCoeffs; ++k)
+            if (k < nc) acc[k] += __fdiv_rn(acc[k], fn);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kShCoeffs; ++k)
+            if (k < nc) acc[k] = fmaf(ylds[j][k], x, acc[k]);
+        }
+      }
+    }
+  }
+  if (t < cols) {
+    float *o = a.transfer + (size_t)(a.v_begin + vl) * nc * cols;
+#pragma unroll
+    for (int k = 0; k < kShCoeffs; ++k)
+      if (k < nc) o[(size_t)k * cols + t] = acc[k];
+  }
+}
+
 }  // namespace
 
 size_t env_table_doubles(int n) { return kTabHead + 6 * env_face_doubles(n); }
@@ -1107,6 +1458,33 @@ void launch_sh_render(const ShArgs &a, hipStream_t s) {
   const long long tiles = (long long)((a.width + 7) / 8) * ((a.height + 7) / 8);
   hipLaunchKernelGGL(sh_pixel_kernel, dim3((unsigned)tiles), dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "sh_pixel_kernel launch");
+}
+
+size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce) {
+  const long long pairs = n_verts * samples;
+  const int nc = (degree + 1) * (degree + 1);
+  return tm_align((size_t)pairs * tm_slots(max_bounce) * sizeof(TmRecord)) +
+         tm_align((size_t)pairs * sizeof(TmPath)) + tm_align((size_t)pairs * kTmRow * sizeof(float)) +
+         tm_align((size_t)pairs * nc * 3 * sizeof(float));
+}
+
+void launch_sh_transfer(const TmArgs &a, hipStream_t s) {
+  require(a.nodes && a.tris && a.indices && a.vertices && a.materials && a.basis && a.tabs &&
+              a.work && a.transfer,
+          NWPRT_ERR_INVALID_ARGUMENT, "sh_transfer: null buffer");
+  require(a.degree >= 0 && a.degree <= kShMaxDegree && a.samples >= 1 && a.n >= 1 &&
+              a.max_bounce >= 1 && a.max_bounce <= kTmMaxBounce && a.n_verts >= 1,
+          NWPRT_ERR_INVALID_ARGUMENT, "sh_transfer: bad degree, samples, face size or bounces");
+  const long long pairs = a.n_verts * a.samples;
+  const int nc = (a.degree + 1) * (a.degree + 1);
+  require(pairs * nc / 64 < (1ll << 31), NWPRT_ERR_INVALID_ARGUMENT,
+          "sh_transfer: too many (vertex, sample, map) triples for one launch");
+  hipLaunchKernelGGL(tm_paths_kernel, dim3((unsigned)((pairs + 63) / 64)), dim3(64), 0, s, a);
+  hip_check(hipGetLastError(), "tm_paths_kernel launch");
+  hipLaunchKernelGGL(tm_light_kernel, dim3((unsigned)((pairs * nc + 63) / 64)), dim3(64), 0, s, a);
+  hip_check(hipGetLastError(), "tm_light_kernel launch");
+  hipLaunchKernelGGL(tm_project_kernel, dim3((unsigned)a.n_verts), dim3(256), 0, s, a);
+  hip_check(hipGetLastError(), "tm_project_kernel launch");
 }
 
 }  // namespace nwprt

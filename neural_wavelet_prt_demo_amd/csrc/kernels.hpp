@@ -250,6 +250,31 @@ struct ShArgs {
 };
 void launch_sh_render(const ShArgs &a, hipStream_t s);
 
+// SH transfer matrices (gbuffer_kernels.hip): paths from the vertices
+// [v_begin, v_begin + n_verts) traced once per (vertex, sample), their
+// light samples and exit radiance under each of the nc basis maps, then
+// the projection onto the filtered SH basis of the start directions.
+constexpr int kTmMaxBounce = 15;
+struct TmArgs {
+  const BvhNode *nodes;
+  const BvhTri *tris;
+  const int32_t *indices;       // [n_tri][3]
+  const float *vertices;        // [n_vertices][3]
+  const float *normals;         // [n_vertices][3] or null
+  const int32_t *tri_material;  // or null
+  const MaterialRec *materials;
+  const float *basis;           // [nc][6][n][n][3], nc = (degree + 1)^2
+  const double *tabs;           // [nc][tab_doubles] sampling tables
+  size_t tab_doubles;           // env_table_doubles(n)
+  int n, degree, samples, max_bounce;
+  uint32_t seed;
+  long long v_begin, n_verts;
+  void *work;                   // tm_work_bytes(n_verts, samples, degree, max_bounce)
+  float *transfer;              // [n_vertices][nc * nc][3]
+};
+size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce);
+void launch_sh_transfer(const TmArgs &a, hipStream_t s);
+
 // Persistent batch form: work item = the 64 pixels [base, base + 64) of job
 // `job` (base < the job's pixel count).
 struct BatchItem {

@@ -404,6 +404,7 @@ nwprt_status nwprt_mesh_get_info(nwprt_ctx ctx, int32_t mesh_id, nwprt_mesh_info
     out->triangles = m.n_tris;
     out->bvh_nodes = m.n_nodes;
     out->bvh_depth = m.depth;
+    out->vertices = m.n_vertices;
   });
 }
 
@@ -544,6 +545,72 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
     a.env_n = view_env_n;
     a.color = color;
     launch_sh_render(a, pick(ctx, stream));
+  });
+}
+
+nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const float *basis_faces,
+                                       int32_t face_n, int32_t degree, int32_t samples,
+                                       int32_t max_bounce, uint32_t seed, int64_t vertex_begin,
+                                       int64_t vertex_count, float *transfer, void *stream) {
+  return guarded([&] {
+    bind(ctx);
+    const Mesh &m = get_mesh(ctx, mesh_id);
+    require(basis_faces && transfer, NWPRT_ERR_INVALID_ARGUMENT, "sh_compute_transfer: null argument");
+    require(degree >= 0 && degree <= kShMaxDegree, NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_compute_transfer: SH degree must be in [0, 8]");
+    require(face_n >= 1 && face_n <= NWPRT_MAX_FACE_SIZE, NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_compute_transfer: face size must be in [1, 128]");
+    require(samples >= 1 && samples <= (1 << 20), NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_compute_transfer: samples must be in [1, 2^20]");
+    require(max_bounce >= 1 && max_bounce <= kTmMaxBounce, NWPRT_ERR_INVALID_ARGUMENT,
+            "sh_compute_transfer: max_bounce must be in [1, 15]");
+    if (vertex_count < 0) vertex_count = m.n_vertices - vertex_begin;
+    require(vertex_begin >= 0 && vertex_count >= 0 && vertex_begin + vertex_count <= m.n_vertices,
+            NWPRT_ERR_INVALID_ARGUMENT, "sh_compute_transfer: vertex range outside the mesh");
+    require(m.direct_ok, NWPRT_ERR_UNSUPPORTED,
+            "sh_compute_transfer: DisneyBSDF (type 1) materials are not supported");
+    check_aligned(basis_faces, 4, "basis faces");
+    check_aligned(transfer, 4, "transfer");
+    if (vertex_count == 0) return;
+    const hipStream_t st = pick(ctx, stream);
+    const int nc = (degree + 1) * (degree + 1);
+    const size_t td = env_table_doubles(face_n), face_floats = (size_t)6 * face_n * face_n * 3;
+    ctx->sh_tables.ensure(td * nc * sizeof(double));
+    for (int i = 0; i < nc; ++i)
+      launch_env_tables(basis_faces + face_floats * i, face_n, ctx->sh_tables.as<double>() + td * i,
+                        st);
+    // Vertices per launch: the path records, start-direction bases and
+    // per-(sample, map) radiance of a launch stay within kTmWorkBudget, and
+    // the (vertex, sample, map) triples within one grid.
+    constexpr size_t kTmWorkBudget = (size_t)2 << 30;
+    const size_t per_vertex = tm_work_bytes(1, samples, degree, max_bounce);
+    long long chunk = (long long)std::max<size_t>(1, kTmWorkBudget / per_vertex);
+    chunk = std::min<long long>(chunk, ((1ll << 31) - 1) * 64 / ((long long)samples * nc));
+    chunk = std::max<long long>(1, std::min<long long>(chunk, vertex_count));
+    ctx->sh_work.ensure(tm_work_bytes(chunk, samples, degree, max_bounce));
+    TmArgs a{};
+    a.nodes = m.nodes.as<BvhNode>();
+    a.tris = m.tris.as<BvhTri>();
+    a.indices = m.indices.as<int32_t>();
+    a.vertices = m.vertices.as<float>();
+    a.normals = m.has_normals ? m.normals.as<float>() : nullptr;
+    a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
+    a.materials = m.materials.as<MaterialRec>();
+    a.basis = basis_faces;
+    a.tabs = ctx->sh_tables.as<double>();
+    a.tab_doubles = td;
+    a.n = face_n;
+    a.degree = degree;
+    a.samples = samples;
+    a.max_bounce = max_bounce;
+    a.seed = seed;
+    a.work = ctx->sh_work.p;
+    a.transfer = transfer;
+    for (long long v0 = 0; v0 < vertex_count; v0 += chunk) {
+      a.v_begin = vertex_begin + v0;
+      a.n_verts = std::min<long long>(chunk, vertex_count - v0);
+      launch_sh_transfer(a, st);
+    }
   });
 }
 

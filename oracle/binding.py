@@ -98,6 +98,10 @@ def lib():
                                     c_int, c_int, c_uint32, c_void_p, c_int, c_void_p, c_void_p,
                                     c_void_p]
         l.orc_sh_render.restype = None
+        l.orc_sh_transfer.argtypes = [POINTER(OrcMesh), c_void_p, c_int, c_int, c_int, c_int,
+                                      c_int, c_uint32, c_void_p, ctypes.c_int64, c_void_p,
+                                      c_void_p, c_void_p]
+        l.orc_sh_transfer.restype = None
         l.orc_rng_uniform.argtypes = [c_uint32, c_uint32, c_uint32]
         l.orc_rng_uniform.restype = c_float
         l.orc_tonemap.restype = c_uint32
@@ -378,4 +382,32 @@ def sh_render(vertices, indices, normals, materials, camera, width: int, height:
     lib().orc_sh_render(ctypes.byref(mesh), _p(cam), width, height, _p(t), _p(ls), degree,
                         samples, seed & 0xFFFFFFFF, _p(env_faces), env_n, _p(out["color"]),
                         _p(out["prim"]), _p(out["margin"]))
+    return out
+
+
+def sh_transfer(vertices, indices, materials, basis, degree: int, *, normals=None,
+                material_ids=None, samples: int = 4096, max_bounce: int = 12, rr_depth: int = 3,
+                seed: int = 0, verts=None) -> Dict[str, np.ndarray]:
+    """SH transfer matrices by FP64 replay (orc_sh_transfer).  basis: the
+    nc = (degree + 1)^2 basis maps, float32 [nc][6][n][n][3]; verts: vertex
+    indices (all by default).  Returns T float64 [len(verts)][nc * nc][3]
+    (row k * nc + i), abs_sum (same shape: sums of |terms|) and margin
+    float64 [len(verts)][nc]."""
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+    nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+    nc = (degree + 1) ** 2
+    b = np.ascontiguousarray(basis, np.float32)
+    assert b.ndim == 5 and b.shape[0] == nc and b.shape[1] == 6 and b.shape[4] == 3
+    n = b.shape[2]
+    vs = np.arange(len(v), dtype=np.int64) if verts is None else np.ascontiguousarray(verts, np.int64)
+    mesh = OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats))
+    out = {"T": np.empty((len(vs), nc * nc, 3), np.float64),
+           "abs_sum": np.empty((len(vs), nc * nc, 3), np.float64),
+           "margin": np.empty((len(vs), nc), np.float64)}
+    lib().orc_sh_transfer(ctypes.byref(mesh), _p(b), n, degree, samples, max_bounce, rr_depth,
+                          seed & 0xFFFFFFFF, _p(vs), len(vs), _p(out["T"]), _p(out["abs_sum"]),
+                          _p(out["margin"]))
     return out

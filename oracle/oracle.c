@@ -1466,3 +1466,285 @@ void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const f
   }
   free(corner);
 }
+
+/* ---- SH transfer matrices ------------------------------------------------
+ * RenderEngine::shComputeTMatrix (Renderer/RenderEngine.cpp:410-446) with its
+ * ray generation program __raygen__ComputeTMatrix (Renderer/device_functions/
+ * cameraShTmatrix.cu:87-178) and the radiance closest-hit and miss programs
+ * it traces (devicePrograms.cu:47-311, 517-564), in FP64, literally: for
+ * each basis map i (which is at once the view, lighting and sampling
+ * environment, RenderEngine.cpp:427-428) and each vertex, `samples` paths
+ * start at the vertex in uniform sphere directions (sample_sphere,
+ * shaderUtils.h:151-158).  A path adds, at every non-mirror hit, one
+ * environment-importance sample (shadow-tested, power-heuristic weight)
+ * times the path throughput, and on leaving the scene after the first ray
+ * the environment radiance with the BSDF-sample weight; it is cut by
+ * Russian roulette after depth 3 and at max_bounce.  Radiance that reached
+ * the vertex straight from the environment (first ray) or through one
+ * mirror (first two rays) is removed; the rest, projected onto the
+ * filtered SH basis 1 / (1 + 0.005 l^2 (l+1)^2) Y_k of the start direction
+ * times 4 pi / samples, accumulates into T[v][k][i]; a path whose result is
+ * NaN scales the vertex's row by 1 + 1 / samples instead (the reference's
+ * T += T / N).  Output layout [vertex][k * nc + i][3] as RenderEngine.cpp:
+ * 432-438 writes it.  Random numbers are the product's counter-based stream
+ * keyed by (seed, vertex) with counters
+ *   sample * 256 + {0, 1}                     start direction,
+ *   sample * 256 + 16 (b + 1) + {0, 1, 2}      environment sample at trace b,
+ *                               + {3, 4, 5}      BSDF light sample (max_bounce 1),
+ *                               + {6, 7, 8}      continuation sample,
+ *                               + 9              Russian roulette,
+ * identical for every basis map, as the reference's per-vertex stream is
+ * (one seed per launch, RenderEngine.cpp:427-429). */
+
+/* Occlusion with a robustness figure: lowers *margin to the barycentric
+ * distance to an edge of any triangle the ray passes close to within
+ * (tmin, inf) and to the relative distance of a hit to tmin. */
+static int g_occluded_m(const orc_mesh *m, const double *o, const double *d, double *margin) {
+  const double tmin = 1e-2;
+  int hit = 0;
+  for (int64_t t = 0; t < m->n_tris; ++t) {
+    const float *A = m->vertices + 3 * (size_t)m->indices[3 * t];
+    const float *B = m->vertices + 3 * (size_t)m->indices[3 * t + 1];
+    const float *C = m->vertices + 3 * (size_t)m->indices[3 * t + 2];
+    double e1[3], e2[3], s[3], p[3], q[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = (double)B[k] - A[k];
+      e2[k] = (double)C[k] - A[k];
+      s[k] = o[k] - A[k];
+    }
+    g_cross(d, e2, p);
+    const double det = g_dot(e1, p);
+    if (det == 0.0) continue;
+    const double u = g_dot(s, p) / det;
+    g_cross(s, e1, q);
+    const double v = g_dot(d, q) / det;
+    const double tt = g_dot(e2, q) / det;
+    if (!(tt > tmin * (1.0 - 1e-6))) continue;
+    const double e = fmin(u, fmin(v, 1.0 - u - v));
+    if (fabs(e) < *margin) *margin = fabs(e);
+    if (fabs(tt - tmin) / tmin < *margin) *margin = fabs(tt - tmin) / tmin;
+    if (e >= 0.0 && tt > tmin) hit = 1;
+  }
+  return hit;
+}
+
+static double g_rel_gap(double a, double b) {
+  const double s = fmax(fabs(a), fabs(b));
+  return s > 0 ? fabs(a - b) / s : 1.0;
+}
+
+/* Lobe-choice robustness of a BSDF sample drawn with uniform r (the first
+ * uniform picks the lobe against the specular probability). */
+static double g_lobe_gap(const g_surface *s, float r) {
+  double spec;
+  if (s->type == 0) {
+    const double ls = g_lum(s->ks), lr = g_lum(s->kd);
+    if (ls + lr <= 0) return 1.0;
+    spec = ls / (ls + lr);
+  } else if (!g_spec_probability(s, &spec)) {
+    return 1.0;
+  }
+  return fabs((double)r - spec);
+}
+
+static int g_bsdf_sample(const g_surface *s, const double *v, uint32_t seed, uint32_t vtx,
+                         uint32_t c, double *out, double *margin) {
+  const float r0 = orc_rng_uniform(seed, vtx, c), r1 = orc_rng_uniform(seed, vtx, c + 1);
+  const double g = g_lobe_gap(s, r0);
+  if (g < *margin) *margin = g;
+  return s->type == 0 ? g_rp_sample(s, v, r0, r1, out)
+                      : g_ct_sample(s, v, r0, r1, orc_rng_uniform(seed, vtx, c + 2), out);
+}
+
+/* One path of basis map i from vertex position o0 (sample s); returns the
+ * indirect radiance in ind[3] and its start direction in d0[3]. */
+static void g_transfer_path(const orc_mesh *m, const float *faces, const double *tab, int n,
+                            int max_bounce, int rr_depth, uint32_t seed, uint32_t vtx, int s,
+                            const double *o0, double *d0, double *ind, double *margin) {
+  const uint32_t base = (uint32_t)s * 256u;
+  {
+    const float z1 = orc_rng_uniform(seed, vtx, base), z2 = orc_rng_uniform(seed, vtx, base + 1);
+    const double theta = 2.0 * M_PI * z1, phi = acos(1.0 - 2.0 * z2);
+    d0[0] = sin(phi) * cos(theta);
+    d0[1] = sin(phi) * sin(theta);
+    d0[2] = cos(phi);
+  }
+  double o[3] = {o0[0], o0[1], o0[2]}, d[3] = {d0[0], d0[1], d0[2]};
+  double color[3] = {0, 0, 0}, lc[3] = {1, 1, 1}, tent[3] = {0, 0, 0}, pdf = 0.0;
+  int first_hit = 0, mirror_first = 0, env_hit = 0, done = 0, depth = 0;
+  for (;;) {
+    const uint32_t c = base + 16u * (uint32_t)(depth + 1);
+    double t, u, v, rd[3] = {d[0], d[1], d[2]};
+    g_norm(rd);
+    const int hit = g_closest(m, o, d, &t, &u, &v, margin);
+    if (hit < 0) {
+      double rad[3];
+      g_env(faces, n, d, rad);
+      if (depth == 0) {
+        for (int a = 0; a < 3; ++a) color[a] = rad[a];
+      } else if (pdf < 0) {
+        for (int a = 0; a < 3; ++a) color[a] += rad[a] * lc[a];
+      } else {
+        const double pe = g_env_pdf(tab, n, d);
+        const double w = pdf * pdf / fmax(pdf * pdf + pe * pe, 1e-14);
+        for (int a = 0; a < 3; ++a) color[a] += rad[a] * lc[a] * w;
+      }
+      done = 1;
+      env_hit = 1;
+    } else {
+      const int *tri = m->indices + 3 * (size_t)hit;
+      const float *A = m->vertices + 3 * (size_t)tri[0];
+      const float *B = m->vertices + 3 * (size_t)tri[1];
+      const float *C = m->vertices + 3 * (size_t)tri[2];
+      double e1[3], e2[3], ng[3], ns[3], p[3];
+      for (int a = 0; a < 3; ++a) {
+        e1[a] = (double)B[a] - A[a];
+        e2[a] = (double)C[a] - A[a];
+      }
+      g_cross(e1, e2, ng);
+      g_norm(ng);
+      if (m->normals) {
+        const double ww = 1.0 - u - v;
+        for (int a = 0; a < 3; ++a)
+          ns[a] = ww * m->normals[3 * (size_t)tri[0] + a] + u * m->normals[3 * (size_t)tri[1] + a] +
+                  v * m->normals[3 * (size_t)tri[2] + a];
+        g_norm(ns);
+      } else {
+        for (int a = 0; a < 3; ++a) ns[a] = ng[a];
+      }
+      for (int a = 0; a < 3; ++a) p[a] = o[a] + rd[a] * t + 0.001;
+      for (int a = 0; a < 3; ++a) o[a] = p[a];
+      const int mid = m->material_ids ? m->material_ids[hit] : 0;
+      const float *mr = m->materials + 8 * (size_t)mid;
+      if ((int)mr[7] == 3) {
+        const double nd = g_dot(ns, rd);
+        for (int a = 0; a < 3; ++a) d[a] = rd[a] - 2.0 * nd * ns[a];
+        if (!first_hit) mirror_first = 1;
+      } else {
+        g_surface sf;
+        g_surface_of(mr, ns, ng, &sf);
+        const double vv[3] = {-rd[0], -rd[1], -rd[2]};
+        double l[3], f[3], rad[3];
+        /* environment-importance light sample */
+        g_env_sample(tab, n, orc_rng_uniform(seed, vtx, c), orc_rng_uniform(seed, vtx, c + 1),
+                     orc_rng_uniform(seed, vtx, c + 2), l);
+        double pe = g_env_pdf(tab, n, l);
+        {
+          const double g = g_rel_gap(pe, 1e-4);
+          if (g < *margin) *margin = g;
+        }
+        if (!g_occluded_m(m, p, l, margin) && pe > 1e-4) {
+          const double pb = g_bsdf_pdf(&sf, l, vv);
+          g_bsdf_eval(&sf, l, vv, f);
+          g_env(faces, n, l, rad);
+          const double w = pe / fmax(pe * pe + pb * pb, 1e-8);
+          double inc[3];
+          for (int a = 0; a < 3; ++a) inc[a] = f[a] * rad[a] * lc[a] * w;
+          if (!(isnan(inc[0]) || isnan(inc[1]) || isnan(inc[2])))
+            for (int a = 0; a < 3; ++a) color[a] += inc[a];
+        }
+        /* BSDF light sample: direct mode only (devicePrograms.cu:202-247) */
+        if (max_bounce == 1 && g_bsdf_sample(&sf, vv, seed, vtx, c + 3, l, margin)) {
+          const double pb = g_bsdf_pdf(&sf, l, vv);
+          pe = g_env_pdf(tab, n, l);
+          const double g = g_rel_gap(pe, 1e-4);
+          if (g < *margin) *margin = g;
+          if (!g_occluded_m(m, p, l, margin) && pe > 1e-4) {
+            g_bsdf_eval(&sf, l, vv, f);
+            g_env(faces, n, l, rad);
+            const double w = pb / fmax(pe * pe + pb * pb, 1e-8);
+            double inc[3];
+            for (int a = 0; a < 3; ++a) inc[a] = f[a] * rad[a] * lc[a] * w;
+            if (!(isnan(inc[0]) || isnan(inc[1]) || isnan(inc[2])))
+              for (int a = 0; a < 3; ++a) color[a] += inc[a];
+          }
+        }
+        /* continuation */
+        if (!g_bsdf_sample(&sf, vv, seed, vtx, c + 6, l, margin)) {
+          done = 1;
+        } else {
+          pdf = g_bsdf_pdf(&sf, l, vv);
+          const double g = g_rel_gap(pdf, 1e-4);
+          if (g < *margin) *margin = g;
+          if (pdf <= 1e-4) {
+            done = 1;
+          } else {
+            g_bsdf_eval(&sf, l, vv, f);
+            for (int a = 0; a < 3; ++a) {
+              d[a] = l[a];
+              lc[a] = lc[a] * f[a] / pdf;
+            }
+          }
+        }
+        first_hit = 1;
+      }
+    }
+    ++depth;
+    const int ok = !(isnan(color[0]) || isnan(color[1]) || isnan(color[2]));
+    if (ok && env_hit && (depth == 1 || (depth == 2 && mirror_first)))
+      for (int a = 0; a < 3; ++a) tent[a] = color[a];
+    if (!done && depth > rr_depth) {
+      const float z = orc_rng_uniform(seed, vtx, c + 9);
+      const double rr = fmin(fmax(lc[0], fmax(lc[1], lc[2])), 0.95);
+      const double g = g_rel_gap((double)z, rr);
+      if (g < *margin) *margin = g;
+      if ((double)z < rr) {
+        for (int a = 0; a < 3; ++a) lc[a] /= fmax(rr, 1e-10);
+      } else {
+        done = 1;
+      }
+    }
+    if (done || depth >= max_bounce) break;
+  }
+  for (int a = 0; a < 3; ++a) ind[a] = color[a] - tent[a];
+}
+
+void orc_sh_transfer(const orc_mesh *m, const float *basis, int n, int degree, int samples,
+                     int max_bounce, int rr_depth, uint32_t seed, const int64_t *verts,
+                     int64_t n_verts, double *T, double *abs_sum, double *margin) {
+  const int nc = (degree + 1) * (degree + 1);
+  const size_t td = orc_env_table_doubles(n), face_floats = (size_t)6 * n * n * 3;
+  double *tabs = (double *)malloc(sizeof(double) * td * nc);
+  for (int i = 0; i < nc; ++i) orc_env_tables(basis + face_floats * i, n, tabs + td * i);
+  double filt[81];
+  for (int l = 0; l <= 8; ++l)
+    for (int mm = -l; mm <= l; ++mm)
+      filt[l * l + l + mm] = 1.0 / (1.0 + 0.005 * l * l * (l + 1.0) * (l + 1.0));
+  const int64_t jobs = n_verts * nc;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t job = 0; job < jobs; ++job) {
+    const int64_t vi = job / nc;
+    const int i = (int)(job % nc);
+    const uint32_t vtx = (uint32_t)verts[vi];
+    const double o0[3] = {m->vertices[3 * (size_t)verts[vi]], m->vertices[3 * (size_t)verts[vi] + 1],
+                          m->vertices[3 * (size_t)verts[vi] + 2]};
+    double *Tv = T + (size_t)vi * nc * nc * 3, *Av = abs_sum ? abs_sum + (size_t)vi * nc * nc * 3 : NULL;
+    double mg = 1.0;
+    for (int k = 0; k < nc; ++k)
+      for (int a = 0; a < 3; ++a) {
+        Tv[((size_t)k * nc + i) * 3 + a] = 0.0;
+        if (Av) Av[((size_t)k * nc + i) * 3 + a] = 0.0;
+      }
+    for (int s = 0; s < samples; ++s) {
+      double d0[3], ind[3], y[81];
+      g_transfer_path(m, basis + face_floats * i, tabs + td * i, n, max_bounce, rr_depth, seed,
+                      vtx, s, o0, d0, ind, &mg);
+      orc_sh_eval(d0, y);
+      const int bad = isnan(ind[0]) || isnan(ind[1]) || isnan(ind[2]);
+      for (int k = 0; k < nc; ++k)
+        for (int a = 0; a < 3; ++a) {
+          const size_t e = ((size_t)k * nc + i) * 3 + a;
+          if (bad) {
+            Tv[e] += Tv[e] / samples;
+            if (Av) Av[e] += Av[e] / samples;
+          } else {
+            const double x = filt[k] * y[k] * ind[a] * 4.0 * M_PI / samples;
+            Tv[e] += x;
+            if (Av) Av[e] += fabs(x);
+          }
+        }
+    }
+    if (margin) margin[vi * nc + i] = mg;
+  }
+  free(tabs);
+}

@@ -179,6 +179,18 @@ void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const f
                    const float *env_faces, int env_n, double *color, int32_t *prim,
                    double *margin);
 
+/* SH transfer matrices (RenderEngine::shComputeTMatrix with
+ * __raygen__ComputeTMatrix), FP64, replaying the product's random stream.
+ * basis [nc][6][n][n][3] (nc = (degree+1)^2 basis maps); verts: the vertex
+ * indices to compute; T double [n_verts][nc * nc][3] (row k * nc + i: SH
+ * coefficient k of the start direction, basis map i); abs_sum (or NULL):
+ * the same sums of absolute terms; margin (or NULL) double [n_verts][nc]:
+ * the smallest robustness figure met by the vertex's paths under map i.
+ * rr_depth: Russian roulette after this depth (3 in the reference). */
+void orc_sh_transfer(const orc_mesh *m, const float *basis, int n, int degree, int samples,
+                     int max_bounce, int rr_depth, uint32_t seed, const int64_t *verts,
+                     int64_t n_verts, double *T, double *abs_sum, double *margin);
+
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,
                             const orc_gbuffer *gb, int pixel_begin, int n_pixels,

@@ -316,7 +316,7 @@ def test_full_size_properties(nw, ctx):
     normals and reflections are unit vectors and obey the reflection law
     against the pinhole ray; the direct term is finite, non-negative,
     repeatable for a seed and different for another; the SH render is
-    finite and non-negative."""
+    finite and non-negative, on transfer matrices computed for every vertex."""
     import bench
     verts, norms, idx, mids, mats = bench.gbuffer_bench_mesh()
     mesh = ctx.mesh_create(verts, idx, mats, normals=norms, material_ids=mids)
@@ -364,14 +364,18 @@ def test_full_size_properties(nw, ctx):
                      direct_samples=2, seed=6)
     ctx.sync()
     assert not np.array_equal(get(f2, "direct"), get(f1, "direct"))
-    # SH render at degree 2 over the same scene (identity transfer).
+    # SH-PRT at degree 2 over the same scene: transfer matrices of all
+    # vertices from the 9 real-SH basis maps (16 paths per vertex), then the
+    # render that reads them.
     nc = 9
-    T = np.zeros((len(verts), nc, nc, 3), np.float32)
-    T[:, np.arange(nc), np.arange(nc), :] = 1.0
+    T = ctx.sh_compute_transfer(mesh, nw.sh_basis_faces(2, 16), 16, 2, samples=16, seed=1)
+    ctx.sync()
+    Th = T.download(np.float32, (len(verts), nc * nc, 3))
+    assert np.all(np.isfinite(Th)) and np.count_nonzero(Th) > Th.size // 4
     L = np.zeros((nc, 3), np.float32)
     L[0] = 1.0
     out = nw.DeviceBuffer(P * 16)
-    ctx.sh_render(mesh, cam, W, H, nw.DeviceBuffer.from_array(T), nw.DeviceBuffer.from_array(L),
+    ctx.sh_render(mesh, cam, W, H, T, nw.DeviceBuffer.from_array(L),
                   2, out, brdf_samples=8, seed=1, env_faces=df, env_n=32)
     ctx.sync()
     img = out.download(np.float32, (P, 4))

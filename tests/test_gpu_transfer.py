@@ -1,0 +1,219 @@
+"""GPU tests of nwprt_sh_compute_transfer (RenderEngine::shComputeTMatrix,
+Renderer/RenderEngine.cpp:410-446, with __raygen__ComputeTMatrix,
+Renderer/device_functions/cameraShTmatrix.cu:87-178) through the C ABI.
+
+The product draws the oracle's counter-based random numbers, so the FP64
+replay follows the same paths: per (vertex, basis map) the two differ by
+rounding, except where a decision of some path (a hit near a triangle edge,
+a shadow ray near an edge or at tmin, a lobe choice, the roulette, the
+1e-4 pdf cut-offs) sits within rounding of its threshold; the oracle
+reports the smallest such margin per (vertex, map) and those columns are
+left out.  Bars (per column of T, i.e. one vertex and one basis map):
+|GPU - oracle| <= 2e-4 * A + 1e-7 * scale for 97 % of the kept columns and
+<= 0.05 * max(A) for all of them, with A the oracle's sum of |terms| of the
+entry (the FP32 rounding error grows with it) and scale the largest |T|.
+Statistical checks against the closed form of the floor scene, exact
+power-of-two scaling, vertex ranges and determinism complete the picture."""
+import numpy as np
+import pytest
+
+from transfer_scenes import (floor_basis, floor_expectation, floor_scene, panel_scene,
+                             smooth_basis)
+
+pytestmark = pytest.mark.gpu
+
+MARGIN = 1e-5
+
+
+def _mesh(ctx, v, ix, mats, mids, normals=None):
+    return ctx.mesh_create(v, ix, mats, normals=normals, material_ids=mids)
+
+
+def _transfer(nw, ctx, mesh, basis, deg, n_vertices, **kw):
+    nc = (deg + 1) ** 2
+    out = ctx.sh_compute_transfer(mesh, basis, basis.shape[2], deg, **kw)
+    ctx.sync()
+    return out.download(np.float32, (n_vertices, nc * nc, 3)).astype(np.float64)
+
+
+def _compare(got, ref, label):
+    """got / ref: T rows of the compared vertices; ref from orc.sh_transfer."""
+    nv, rows, _ = got.shape
+    nc = int(round(np.sqrt(rows)))
+    T, A, mg = ref["T"], ref["abs_sum"], ref["margin"]
+    scale = max(np.abs(T).max(), 1e-12)
+    g = got.reshape(nv, nc, nc, 3)      # [v][k][i][c]
+    r = T.reshape(nv, nc, nc, 3)
+    a = A.reshape(nv, nc, nc, 3)
+    keep = mg >= MARGIN                 # [v][i]
+    assert keep.mean() >= 0.5, keep.mean()
+    err = np.abs(g - r)                 # [v][k][i][c]
+    tight = err <= 2e-4 * a + 1e-7 * scale
+    col_ok = tight.all(axis=(1, 3))     # [v][i]
+    frac = col_ok[keep].mean()
+    loose = (err.max(axis=(1, 3)) <= 0.05 * a.max(axis=(1, 3)) + 1e-7 * scale)[keep]
+    worst = (err / (a + 1e-30)).max()
+    print(f"{label}: kept {keep.mean():.3f} of columns, tight {frac:.4f}, "
+          f"worst err/A {worst:.2e}, scale {scale:.3e}")
+    assert frac >= 0.97 and loose.all(), (frac, loose.mean())
+
+
+@pytest.mark.parametrize("deg,samples,max_bounce", [(2, 512, 12), (1, 256, 1), (8, 64, 12)])
+def test_transfer_replays_oracle(nw, orc, ctx, deg, samples, max_bounce):
+    nc = (deg + 1) ** 2
+    v, ix, mids, mats, pts = panel_scene()
+    if deg == 8:
+        pts = pts[::3]
+    basis = smooth_basis(nc, seed=deg)
+    mesh = _mesh(ctx, v, ix, mats, mids)
+    print(f"replay deg {deg} samples {samples} max_bounce {max_bounce}", flush=True)
+    got = _transfer(nw, ctx, mesh, basis, deg, len(v), samples=samples, max_bounce=max_bounce,
+                    seed=11)
+    ref = orc.sh_transfer(v, ix, mats, basis, deg, material_ids=mids, samples=samples,
+                          max_bounce=max_bounce, seed=11, verts=pts)
+    assert np.abs(ref["T"]).max() > 0
+    _compare(got[pts], ref, f"deg {deg} mb {max_bounce}")
+    ctx.mesh_destroy(mesh)
+
+
+def test_transfer_with_vertex_normals_replays_oracle(nw, orc, ctx):
+    """Smooth-shaded geometry: a faceted bowl (interpolated normals differ
+    from the geometric ones) with a floor; degree 2, 384 samples."""
+    rng = np.random.default_rng(4)
+    nu, nvr = 12, 5
+    th = np.linspace(0.25 * np.pi, 0.5 * np.pi, nvr)
+    ph = np.linspace(0, 2 * np.pi, nu, endpoint=False)
+    T_, P_ = np.meshgrid(th, ph, indexing="ij")
+    nrm = np.stack([np.sin(T_) * np.cos(P_), np.sin(T_) * np.sin(P_), -np.cos(T_)], -1).reshape(-1, 3)
+    pos = 0.8 * nrm + np.array([0, 0, 0.9])
+    a = (np.arange(nvr - 1)[:, None] * nu + np.arange(nu)[None, :]).reshape(-1)
+    b = (np.arange(nvr - 1)[:, None] * nu + (np.arange(nu)[None, :] + 1) % nu).reshape(-1)
+    tris = np.concatenate([np.stack([a, a + nu, b], 1), np.stack([b, a + nu, b + nu], 1)])
+    g = np.array([[-3, -3, 0], [3, -3, 0], [3, 3, 0], [-3, 3, 0]], np.float32)
+    v = np.concatenate([pos, g]).astype(np.float32)
+    normals = np.concatenate([-nrm, np.tile([0, 0, 1.0], (4, 1))]).astype(np.float32)
+    base = len(pos)
+    ix = np.concatenate([tris, [[base, base + 1, base + 2], [base, base + 2, base + 3]]]).astype(np.int32)
+    mids = np.concatenate([np.zeros(len(tris)), [1, 1]]).astype(np.int32)
+    mats = [[0.7, 0.6, 0.5, 0.2, 0.2, 0.2, 0.35, 4], [0.5, 0.5, 0.5, 0.0, 0.0, 0.0, 0.8, 2]]
+    mesh = _mesh(ctx, v, ix, mats, mids, normals=normals)
+    pts = rng.choice(len(pos), 10, replace=False)
+    basis = smooth_basis(9, seed=7)
+    got = _transfer(nw, ctx, mesh, basis, 2, len(v), samples=384, seed=5)
+    ref = orc.sh_transfer(v, ix, mats, basis, 2, normals=normals, material_ids=mids, samples=384,
+                          seed=5, verts=pts)
+    _compare(got[pts], ref, "bowl")
+    ctx.mesh_destroy(mesh)
+
+
+def test_transfer_floor_matches_quadrature(nw, orc, ctx):
+    """The floor scene's closed form (transfer_scenes) from 24 vertices x
+    65536 samples: mean within 5 standard errors (+ 2e-3 of the scale)."""
+    deg = 2
+    nc = (deg + 1) ** 2
+    v, ix, mids, mats, pts = floor_scene()
+    basis = floor_basis(nc)
+    mesh = _mesh(ctx, v, ix, mats, mids)
+    got = _transfer(nw, ctx, mesh, basis, deg, len(v), samples=65536, seed=9)
+    T = got[pts].reshape(len(pts), nc, nc, 3)
+    est, sem = T.mean(0), T.std(0, ddof=1) / np.sqrt(len(pts))
+    want, pe_max = floor_expectation(orc, deg, basis)
+    assert pe_max < 1e-4
+    err = np.abs(est - want)
+    print(f"floor: max err / sem {(err / (sem + 1e-12)).max():.2f}")
+    assert np.all(err <= 5 * sem + 2e-3 * np.abs(want).max())
+    ctx.mesh_destroy(mesh)
+
+
+def test_transfer_scales_exactly_with_the_maps(nw, ctx):
+    """Every radiance term is linear in the maps and the sampling tables are
+    normalised, so maps x 2^s give T x 2^s bit for bit."""
+    v, ix, mids, mats, pts = panel_scene()
+    basis = smooth_basis(4, seed=3)
+    mesh = _mesh(ctx, v, ix, mats, mids)
+    base = _transfer(nw, ctx, mesh, basis, 1, len(v), samples=300, seed=2)
+    assert np.count_nonzero(base[pts]) > base[pts].size // 2
+    for s in (5, -3):
+        got = _transfer(nw, ctx, mesh, (basis * np.float32(2.0 ** s)).astype(np.float32), 1,
+                        len(v), samples=300, seed=2)
+        assert np.array_equal(got, base * 2.0 ** s), s
+    ctx.mesh_destroy(mesh)
+
+
+def test_transfer_ranges_determinism_and_errors(nw, ctx):
+    from neural_wavelet_prt_demo_amd import DeviceBuffer
+    v, ix, mids, mats, pts = panel_scene()
+    nc = 4
+    basis = smooth_basis(nc, seed=5)
+    mesh = _mesh(ctx, v, ix, mats, mids)
+    nv = len(v)
+    full = _transfer(nw, ctx, mesh, basis, 1, nv, samples=200, seed=4)
+    again = _transfer(nw, ctx, mesh, basis, 1, nv, samples=200, seed=4)
+    assert np.array_equal(full, again)
+    other = _transfer(nw, ctx, mesh, basis, 1, nv, samples=200, seed=5)
+    assert not np.array_equal(full[pts], other[pts])
+    sentinel = np.full((nv, nc * nc, 3), -7.0, np.float32)
+    out = DeviceBuffer.from_array(sentinel)
+    db = DeviceBuffer.from_array(basis)
+    ctx.sh_compute_transfer(mesh, db, 8, 1, out, samples=200, seed=4, vertex_begin=13,
+                            vertex_count=5)
+    ctx.sh_compute_transfer(mesh, db, 8, 1, out, samples=200, seed=4, vertex_begin=18)
+    ctx.sync()
+    part = out.download(np.float32, (nv, nc * nc, 3))
+    assert np.all(part[:13] == -7.0)
+    assert np.array_equal(part[13:], full[13:].astype(np.float32))
+    bad = [dict(degree=9), dict(samples=0), dict(max_bounce=0), dict(max_bounce=16),
+           dict(vertex_begin=nv + 1), dict(vertex_begin=2, vertex_count=nv), dict(face_n=0)]
+    for kw in bad:
+        args = dict(face_n=8, degree=1, samples=16, max_bounce=12, vertex_begin=0, vertex_count=-1)
+        args.update(kw)
+        with pytest.raises(nw.NwprtError) as e:
+            ctx.sh_compute_transfer(mesh, db, args["face_n"], args["degree"], out,
+                                    samples=args["samples"], max_bounce=args["max_bounce"],
+                                    vertex_begin=args["vertex_begin"],
+                                    vertex_count=args["vertex_count"])
+        assert e.value.status == nw.ERR_INVALID_ARGUMENT, kw
+    ctx.mesh_destroy(mesh)
+    disney = [list(m) for m in mats]
+    disney[0][7] = 1
+    mesh = _mesh(ctx, v, ix, disney, mids)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.sh_compute_transfer(mesh, db, 8, 1, out, samples=16)
+    assert e.value.status == nw.ERR_UNSUPPORTED
+    ctx.mesh_destroy(mesh)
+
+
+def test_sh_render_reads_computed_transfer(nw, orc, ctx):
+    """The SH-PRT render on matrices computed here (the reference's
+    shComputeTMatrix -> shLoadTMatrix -> shRender chain) equals the oracle's
+    render of the same matrices."""
+    from neural_wavelet_prt_demo_amd import DeviceBuffer
+    v, ix, mids, mats, _ = panel_scene()
+    mats = [m for m in mats]
+    mats[2] = [0.6, 0.6, 0.6, 0.0, 0.0, 0.0, 0.7, 2]   # the render has no mirror BSDF
+    normals = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (len(v), 1))
+    normals[4:8] = [0.0, 0.0, -1.0]
+    normals[8:12] = [-1.0, 0.0, 0.0]
+    mesh = _mesh(ctx, v, ix, mats, mids, normals=normals)
+    deg, nc = 2, 9
+    T = ctx.sh_compute_transfer(mesh, smooth_basis(nc, seed=8), 8, deg, samples=256, seed=1)
+    rng = np.random.default_rng(2)
+    L = rng.normal(0, 0.3, (nc, 3)).astype(np.float32)
+    L[0] = [1.5, 1.4, 1.2]
+    W, H = 48, 32
+    cam = ([0.0, -2.5, 0.3], [0.0, 0.0, 0.3], [0.0, 0.0, 1.0])
+    out = DeviceBuffer(W * H * 16)
+    ctx.sh_render(mesh, cam, W, H, T, DeviceBuffer.from_array(L), deg, out, brdf_samples=64, seed=3)
+    ctx.sync()
+    got = out.download(np.float32, (W * H, 4))
+    Th = T.download(np.float32, (len(v), nc, nc, 3))
+    ref = orc.sh_render(v, ix, normals, mats, np.concatenate(cam), W, H, Th, L, deg,
+                        material_ids=mids, samples=64, seed=3)
+    safe = ref["margin"] >= 1e-4
+    hit = safe & (ref["prim"] >= 0)
+    assert hit.mean() > 0.1 and np.isfinite(got).all()
+    assert (ref["color"][hit, :3] > 0).any()
+    top = np.abs(ref["color"][hit, :3]).max()
+    err = np.abs(got[hit, :3] - ref["color"][hit, :3]) / (np.abs(ref["color"][hit, :3]) + 1e-2 * top)
+    assert err.max() <= 1e-3, err.max()
+    ctx.mesh_destroy(mesh)
