@@ -1278,7 +1278,7 @@ __global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
 
 __device__ inline bool tm_nan3(float3 c) { return isnan(c.x) || isnan(c.y) || isnan(c.z); }
 
-__global__ __launch_bounds__(64) void tm_light_kernel(TmArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void tm_light_kernel(TmArgs a) {
   __shared__ int stack_lds[kBvhMaxDepth * 64];
   const int lane = threadIdx.x;
   const int nc = (a.degree + 1) * (a.degree + 1);
@@ -1308,14 +1308,20 @@ __global__ __launch_bounds__(64) void tm_light_kernel(TmArgs a) {
     const float3 l = env_sample(tab, a.n, rng_uniform(a.seed, vtx, c),
                                 rng_uniform(a.seed, vtx, c + 1), rng_uniform(a.seed, vtx, c + 2));
     const float pe = env_pdf(tab, a.n, l);
-    if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
+    if (pe > 1e-4f) {
+      // The term is formed before its shadow ray is cast: a zero term (a
+      // direction below the surface, where the BSDF is zero) or a NaN one
+      // adds nothing whether or not the ray is blocked, so only a non-zero
+      // finite term pays for the ray.
       const Surface sf = tm_surface(a, h.tag >> 4, v3(h.ns), v3(h.ng));
       const float pb = bsdf_pdf(sf, l, v);
       const float3 f = bsdf_eval(sf, l, v);
       const float3 le = env_radiance(faces, a.n, l);
       const float wt = pe / fmaxf(pe * pe + pb * pb, 1e-8f);
       const float3 inc = v3(f.x * le.x * lc.x * wt, f.y * le.y * lc.y * wt, f.z * le.z * lc.z * wt);
-      if (!tm_nan3(inc)) color = add(color, inc);
+      if (!tm_nan3(inc) && (inc.x != 0.0f || inc.y != 0.0f || inc.z != 0.0f) &&
+          !occluded(a.nodes, a.tris, p, l, stack))
+        color = add(color, inc);
     }
     if (a.max_bounce == 1) {
       const TmBsdfLight bl = *reinterpret_cast<const TmBsdfLight *>(rec + 1);
