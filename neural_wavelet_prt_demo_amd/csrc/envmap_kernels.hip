@@ -422,24 +422,23 @@ __global__ __launch_bounds__(256) void prepare_selection_kernel(
 // R rotates by the yaw angle (x' = cos x + sin z, z' = -sin x + cos z).
 // Face and (u, v) conventions of EnvUVToDirec / EnvDirecToUV
 // (Renderer/device_functions/cubeMap.cu:23-115); row 0 is v = 0.  The
-// direction arithmetic is FP64 with explicit roundings (no contraction), so
-// the texel choice is reproducible on the host.
-__device__ __forceinline__ void uv_to_dir(double u, double v, int f, double &x, double &y,
-                                          double &z) {
-  const double uc = __dsub_rn(__dmul_rn(2.0, u), 1.0), vc = __dsub_rn(__dmul_rn(2.0, v), 1.0);
+// direction arithmetic is FP32 with every operation rounded explicitly (no
+// contraction), so the texel choice is reproducible bit for bit on the host.
+__device__ __forceinline__ void uv_to_dir(float u, float v, int f, float &x, float &y, float &z) {
+  const float uc = __fsub_rn(__fmul_rn(2.0f, u), 1.0f), vc = __fsub_rn(__fmul_rn(2.0f, v), 1.0f);
   switch (f) {
-    case 0: x = 1.0; y = vc; z = uc; break;
-    case 1: x = -1.0; y = vc; z = -uc; break;
-    case 2: x = uc; y = 1.0; z = vc; break;
-    case 3: x = uc; y = -1.0; z = -vc; break;
-    case 4: x = uc; y = vc; z = -1.0; break;
-    default: x = -uc; y = vc; z = 1.0; break;
+    case 0: x = 1.0f; y = vc; z = uc; break;
+    case 1: x = -1.0f; y = vc; z = -uc; break;
+    case 2: x = uc; y = 1.0f; z = vc; break;
+    case 3: x = uc; y = -1.0f; z = -vc; break;
+    case 4: x = uc; y = vc; z = -1.0f; break;
+    default: x = -uc; y = vc; z = 1.0f; break;
   }
 }
 
-__device__ __forceinline__ int dir_to_uv(double x, double y, double z, double &u, double &v) {
-  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
-  double m = 1.0, uc = 0.0, vc = 0.0;
+__device__ __forceinline__ int dir_to_uv(float x, float y, float z, float &u, float &v) {
+  const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+  float m = 1.0f, uc = 0.0f, vc = 0.0f;
   int f = 0;
   // The reference tests the faces in this order and keeps the last match.
   if (x > 0 && ax >= ay && ax >= az) { m = ax; uc = z; vc = y; f = 0; }
@@ -448,25 +447,26 @@ __device__ __forceinline__ int dir_to_uv(double x, double y, double z, double &u
   if (!(y > 0) && ay >= ax && ay >= az) { m = ay; uc = x; vc = -z; f = 3; }
   if (!(z > 0) && az >= ax && az >= ay) { m = az; uc = x; vc = y; f = 4; }
   if (z > 0 && az >= ax && az >= ay) { m = az; uc = -x; vc = y; f = 5; }
-  u = __dmul_rn(0.5, __dadd_rn(__ddiv_rn(uc, m), 1.0));
-  v = __dmul_rn(0.5, __dadd_rn(__ddiv_rn(vc, m), 1.0));
+  u = __fmul_rn(0.5f, __fadd_rn(__fdiv_rn(uc, m), 1.0f));
+  v = __fmul_rn(0.5f, __fadd_rn(__fdiv_rn(vc, m), 1.0f));
   return f;
 }
 
 __global__ __launch_bounds__(256) void rotate_cubemap_kernel(const float *__restrict__ src,
                                                              float *__restrict__ dst, int n,
-                                                             double cs, double sn) {
+                                                             float cs, float sn) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= 6 * n * n) return;
   const int f = t / (n * n), r = (t / n) % n, c = t % n;
-  double x, y, z;
-  uv_to_dir(__ddiv_rn(c + 0.5, (double)n), __ddiv_rn(r + 0.5, (double)n), f, x, y, z);
-  const double xs = __dsub_rn(__dmul_rn(cs, x), __dmul_rn(sn, z));
-  const double zs = __dadd_rn(__dmul_rn(sn, x), __dmul_rn(cs, z));
-  double u, v;
+  float x, y, z;
+  uv_to_dir(__fdiv_rn((float)c + 0.5f, (float)n), __fdiv_rn((float)r + 0.5f, (float)n), f, x, y,
+            z);
+  const float xs = __fsub_rn(__fmul_rn(cs, x), __fmul_rn(sn, z));
+  const float zs = __fadd_rn(__fmul_rn(sn, x), __fmul_rn(cs, z));
+  float u, v;
   const int fs = dir_to_uv(xs, y, zs, u, v);
-  const int cs_ = min(n - 1, max(0, (int)floor(__dmul_rn(u, (double)n))));
-  const int rs = min(n - 1, max(0, (int)floor(__dmul_rn(v, (double)n))));
+  const int cs_ = min(n - 1, max(0, (int)floorf(__fmul_rn(u, (float)n))));
+  const int rs = min(n - 1, max(0, (int)floorf(__fmul_rn(v, (float)n))));
   const float *p = src + (((size_t)fs * n + rs) * n + cs_) * 3;
   float *q = dst + (size_t)t * 3;
   q[0] = p[0];
@@ -482,7 +482,7 @@ void launch_rotate_cubemap(const float *src, float *dst, int n, double yaw, hipS
   require(src != dst, NWPRT_ERR_INVALID_ARGUMENT, "rotate: source and destination must differ");
   const int m = 6 * n * n;
   hipLaunchKernelGGL(rotate_cubemap_kernel, dim3((m + 255) / 256), dim3(256), 0, s, src, dst, n,
-                     std::cos(yaw), std::sin(yaw));
+                     (float)std::cos(yaw), (float)std::sin(yaw));
   hip_check(hipGetLastError(), "rotate_cubemap_kernel launch");
 }
 

@@ -154,23 +154,25 @@ void orc_envmap_haar(const float *faces, int n, float *coef) {
 /* --------------------------------------------------------- yaw rotation -- */
 
 /* Face and (u, v) conventions of EnvUVToDirec / EnvDirecToUV
- * (Renderer/device_functions/cubeMap.cu:99-115, 23-96), in FP64. */
-static void orc_uv_to_dir(double u, double v, int f, double d[3]) {
-  const double uc = 2.0 * u - 1.0, vc = 2.0 * v - 1.0;
+ * (Renderer/device_functions/cubeMap.cu:99-115, 23-96), in FP32: this file
+ * is compiled without contraction, so each operation rounds once, as the
+ * product's explicitly rounded device arithmetic does. */
+static void orc_uv_to_dir(float u, float v, int f, float d[3]) {
+  const float uc = 2.0f * u - 1.0f, vc = 2.0f * v - 1.0f;
   switch (f) {
-    case 0: d[0] = 1.0; d[1] = vc; d[2] = uc; break;
-    case 1: d[0] = -1.0; d[1] = vc; d[2] = -uc; break;
-    case 2: d[0] = uc; d[1] = 1.0; d[2] = vc; break;
-    case 3: d[0] = uc; d[1] = -1.0; d[2] = -vc; break;
-    case 4: d[0] = uc; d[1] = vc; d[2] = -1.0; break;
-    default: d[0] = -uc; d[1] = vc; d[2] = 1.0; break;
+    case 0: d[0] = 1.0f; d[1] = vc; d[2] = uc; break;
+    case 1: d[0] = -1.0f; d[1] = vc; d[2] = -uc; break;
+    case 2: d[0] = uc; d[1] = 1.0f; d[2] = vc; break;
+    case 3: d[0] = uc; d[1] = -1.0f; d[2] = -vc; break;
+    case 4: d[0] = uc; d[1] = vc; d[2] = -1.0f; break;
+    default: d[0] = -uc; d[1] = vc; d[2] = 1.0f; break;
   }
 }
 
-static int orc_dir_to_uv(const double d[3], double *u, double *v) {
-  const double x = d[0], y = d[1], z = d[2];
-  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
-  double m = 1.0, uc = 0.0, vc = 0.0;
+static int orc_dir_to_uv(const float d[3], float *u, float *v) {
+  const float x = d[0], y = d[1], z = d[2];
+  const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+  float m = 1.0f, uc = 0.0f, vc = 0.0f;
   int f = 0;
   /* Tested in the reference's order; the last matching face wins. */
   if (x > 0 && ax >= ay && ax >= az) { m = ax; uc = z; vc = y; f = 0; }
@@ -179,26 +181,27 @@ static int orc_dir_to_uv(const double d[3], double *u, double *v) {
   if (!(y > 0) && ay >= ax && ay >= az) { m = ay; uc = x; vc = -z; f = 3; }
   if (!(z > 0) && az >= ax && az >= ay) { m = az; uc = x; vc = y; f = 4; }
   if (z > 0 && az >= ax && az >= ay) { m = az; uc = -x; vc = y; f = 5; }
-  *u = 0.5 * (uc / m + 1.0);
-  *v = 0.5 * (vc / m + 1.0);
+  *u = 0.5f * (uc / m + 1.0f);
+  *v = 0.5f * (vc / m + 1.0f);
   return f;
 }
 
 /* Yaw rotation about +y: texel (f, r, c) of dst takes the src texel nearest
  * to R^T d, d the direction of its centre ((c + 0.5) / N, (r + 0.5) / N), with
- * x' = cos x - sin z, z' = sin x + cos z (cos, sin of yaw in FP64). */
+ * x' = cos x - sin z, z' = sin x + cos z (cos, sin of yaw in FP64, rounded to
+ * FP32; the rest in FP32). */
 void orc_rotate_cubemap(const float *src, int n, double yaw, float *dst) {
-  const double cs = cos(yaw), sn = sin(yaw);
+  const float cs = (float)cos(yaw), sn = (float)sin(yaw);
   for (int f = 0; f < 6; ++f)
     for (int r = 0; r < n; ++r)
       for (int c = 0; c < n; ++c) {
-        double d[3], e[3], u, v;
-        orc_uv_to_dir((c + 0.5) / (double)n, (r + 0.5) / (double)n, f, d);
+        float d[3], e[3], u, v;
+        orc_uv_to_dir(((float)c + 0.5f) / (float)n, ((float)r + 0.5f) / (float)n, f, d);
         e[0] = cs * d[0] - sn * d[2];
         e[1] = d[1];
         e[2] = sn * d[0] + cs * d[2];
         const int fs = orc_dir_to_uv(e, &u, &v);
-        int cc = (int)floor(u * n), rr = (int)floor(v * n);
+        int cc = (int)floorf(u * (float)n), rr = (int)floorf(v * (float)n);
         cc = cc < 0 ? 0 : (cc > n - 1 ? n - 1 : cc);
         rr = rr < 0 ? 0 : (rr > n - 1 ? n - 1 : rr);
         memcpy(dst + (((size_t)f * n + r) * n + c) * 3, src + (((size_t)fs * n + rr) * n + cc) * 3,

@@ -113,3 +113,46 @@ def test_hashed_level_indices_match_published_hash(orc):
                     h = (cx ^ ((cy * 2654435761) & 0xFFFFFFFF) ^ ((cz * 805459861) & 0xFFFFFFFF))
                     want = (h & 0xFFFFFFFF) % int(lv["size"][l])
                 assert idx[l, c] == want, (l, c)
+
+
+def _corner_map(orc, sc, lv, level, positions):
+    """(grid coordinate of corner c -> entry index) seen at `positions`,
+    corner c taking offset (c >> d) & 1 along dimension d."""
+    seen = {}
+    scale = np.float32(lv["scale"][level])
+    for x in positions:
+        idx, _ = sc.hash_encode(x)
+        p = (x.astype(np.float64) * float(scale) + 0.5).astype(np.float32)
+        g = np.floor(p).astype(int)
+        for c in range(8):
+            key = tuple(int(g[d] + ((c >> d) & 1)) for d in range(3))
+            v = int(idx[level, c])
+            assert seen.setdefault(key, v) == v, (level, key)  # a function of the corner
+    return seen
+
+
+def test_corner_indices_lie_in_their_level_and_dense_levels_are_injective(orc):
+    """Properties the level layout fixes whatever the hash: every corner
+    index lies in [0, size) of its level, and a dense level (res^3 <= T)
+    gives distinct grid vertices of [0, res)^3 distinct entries."""
+    levels, log2T = 8, 15
+    rng = np.random.default_rng(3)
+    sc, lv = _scene(orc, levels, log2T, np.zeros((1 << 15) * levels, np.float32).reshape(-1, 2))
+    pos = rng.random((3000, 3)).astype(np.float32)
+    pos[:8] = [[0, 0, 0], [1, 1, 1], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [0.5, 0.5, 0.5],
+               [1e-7, 1 - 1e-7, 0.25]]
+    for x in pos[:400]:
+        idx, _ = sc.hash_encode(x)
+        assert np.all(idx < lv["size"][:, None]), x
+    n_dense = 0
+    for level in range(levels):
+        if not lv["dense"][level]:
+            continue
+        n_dense += 1
+        res = int(lv["resolution"][level])
+        seen = _corner_map(orc, sc, lv, level, pos)
+        inside = {k: v for k, v in seen.items() if all(0 <= c < res for c in k)}
+        assert len(inside) > min(2000, res ** 3 // 2)
+        assert len(set(inside.values())) == len(inside), level  # injective
+        assert max(inside.values()) < lv["size"][level]
+    assert 0 < n_dense < levels  # both kinds of level are exercised
