@@ -106,5 +106,15 @@ __device__ __forceinline__ uint32_t tonemap_rgba8(float r, float g, float b) {
   return tonemap_byte(r) | (tonemap_byte(g) << 8) | (tonemap_byte(b) << 16) | (255u << 24);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin; this remaps workgroup
+// `bid` of `nb` so that each XCD receives a contiguous run of tiles (a
+// bijection of [0, nb)).  Neighbouring pixels then share their coarse
+// hash-grid entries in one XCD's L2 instead of fetching them into eight.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, local = bid >> 3;
+  return x * q + (x < r ? x : r) + local;
+}
+
 }  // namespace dev
 }  // namespace nwprt

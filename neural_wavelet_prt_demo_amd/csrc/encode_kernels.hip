@@ -14,15 +14,6 @@ namespace {
 
 constexpr int kEncodeBlock = 256;
 
-// Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
-// receives a contiguous run of pixel tiles, whose coarse-level hash entries
-// then stay in that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nb) {
-  const int q = nb >> 3, r = nb & 7;
-  const int x = bid & 7, local = bid >> 3;
-  return x * q + (x < r ? x : r) + local;
-}
-
 // Row p of a [n][W] array in element type OUT (W a multiple of 8).
 template <Elem OUT, int W>
 __device__ __forceinline__ void store_row(void *base, int p, const float *v) {
@@ -82,7 +73,7 @@ __global__ __launch_bounds__(kEncodeBlock) void encode_kernel(EncodeArgs a) {
     for (int i = threadIdx.x; i < 32 * 32; i += kEncodeBlock) proj[i] = a.projection[i];
   __syncthreads();
 
-  const int p = xcd_tile(blockIdx.x, gridDim.x) * kEncodeBlock + threadIdx.x;
+  const int p = dev::xcd_tile(blockIdx.x, gridDim.x) * kEncodeBlock + threadIdx.x;
   if (p >= a.n_pixels) return;
   const bool hit = a.env_flag == nullptr || a.env_flag[p] != 0;
 

@@ -304,14 +304,15 @@ __device__ __forceinline__ void load_w4(const RelightArgs &a, i32x8 *w4s, int la
   w4s[64 + lane] = src[64 + lane];
 }
 
-// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
-// pixels [64 (4 b + w), +64).
+// Grid: ceil(P / 256) workgroups of 4 waves; wave w of the workgroup with
+// tile t = xcd_tile(b) owns pixels [64 (4 t + w), +64) (one contiguous band
+// per XCD).
 template <bool SOFTPLUS, bool DUMP, bool AUX, bool PIPE>
 __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
   __shared__ i32x8 w4s[4][2 * 64];
   __shared__ uint32_t feat_lds[AUX ? 1 : 4 * 64 * 16];  // fused encode: [wave][pixel][16]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int base = (blockIdx.x * 4 + wave) * 64;
+  const int base = (dev::xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * 64;
   if (base >= a.n_pixels) return;
   load_w4(a, w4s[wave], lane);
   relight_fp8_wave<SOFTPLUS, DUMP, AUX, !AUX, PIPE>(a, base, lane, w4s[wave],

@@ -52,8 +52,9 @@ constexpr MfmaVariant kMfmaVariants[] = {
 };
 constexpr int kNumMfmaVariants = sizeof(kMfmaVariants) / sizeof(kMfmaVariants[0]);
 
-// Grid: ceil(P / 256) workgroups of 4 waves; wave w of workgroup b owns
-// pixels [64 (4 b + w), +64).
+// Grid: ceil(P / 256) workgroups of 4 waves; wave w of the workgroup with
+// tile t = xcd_tile(b) owns pixels [64 (4 t + w), +64), so each XCD relights
+// (and with the fused encode, gathers hash entries for) one contiguous band.
 template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, int MINW>
 __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(RelightArgs a) {
   __shared__ u32x4 w4s[W4LDS ? 4 * 2 * 64 : 1];
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(kRelightBlock, MINW) void relight_mfma_kernel(Relig
     for (int i = threadIdx.x; i < 4 * 2 * 64; i += kRelightBlock) w4s[i] = wt[2 * 8 * 64 + i];
     __syncthreads();
   }
-  const int base = (blockIdx.x * 4 + wave) * 64;
+  const int base = (dev::xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * 64;
   if (base >= a.n_pixels) return;
   relight_wave<E, SOFTPLUS, DUMP, AUX, GP, W4LDS, !AUX>(a, base, lane, w4s,
                                                        feat_lds + (AUX ? 0 : wave * 64 * 16));
