@@ -441,16 +441,34 @@ typedef struct {
   float up[3];
 } nwprt_camera;
 
-/* What a material's PACKINFO callable writes at a first hit (e.g.
- * Renderer/device_functions/roughPlastic.cu:9-17): kd, ks and roughness as
- * they enter the G-buffer, and the Material::MatType (Renderer/Material.h:7;
- * 3 = mirror, which reflects the ray instead of ending it). */
+/* A material as Renderer/Material.h:21-117 holds it (constant values; no
+ * texture maps): kd (DisneyBSDF's base colour), ks, roughness, the
+ * Material::MatType (Renderer/Material.h:7: 0 roughPlastic, 1 DisneyBSDF,
+ * 2 cookTorrance, 3 mirror, which reflects the ray instead of ending it,
+ * 4 simple_ggx), the DisneyBSDF parameters and the index of refraction eta
+ * that roughPlastic and DisneyBSDF use.  nwprt_material_defaults fills the
+ * reference's defaults (Material::init and the per-type constructors,
+ * Material.h:79-155). */
 typedef struct {
   float kd[3];
   float ks[3];
   float roughness;
   int32_t type;
+  float specular_transmission;
+  float metallic;
+  float subsurface;
+  float specular;
+  float specular_tint;
+  float anisotropic;
+  float sheen;
+  float sheen_tint;
+  float clearcoat;
+  float clearcoat_gloss;
+  float eta;
 } nwprt_material;
+
+/* Host utility (no GPU): the reference's default material of a type. */
+nwprt_status nwprt_material_defaults(int32_t type, nwprt_material *out);
 
 #define NWPRT_MATERIAL_MIRROR 3
 
@@ -535,9 +553,9 @@ typedef struct {
  * sample (face and texel tables of EnvironmentMap.cpp:153-190) and one BSDF
  * sample, each shadow-tested and weighted by the power heuristic, averaged
  * over direct_samples; misses store the view radiance, as kd.  Materials
- * may be roughPlastic (type 0, index of refraction fixed at the reference
- * default 1.49 / 1.000277), cookTorrance (2), mirrors (3) or simple_ggx
- * (4); DisneyBSDF (1) gives NWPRT_ERR_UNSUPPORTED.  Random numbers come from a counter-based generator keyed by
+ * may be roughPlastic (type 0, with the material's eta), DisneyBSDF (1,
+ * disneyBSDF.cu: diffuse, subsurface blend, sheen, metal, clearcoat,
+ * glass), cookTorrance (2), mirrors (3) or simple_ggx (4).  Random numbers come from a counter-based generator keyed by
  * (seed, pixel, sample), not the reference's per-pixel LCG: estimates agree
  * with the reference's in expectation, not sample by sample.  The sampling
  * tables live in a work buffer of the context: casts with a lighting map
@@ -565,7 +583,9 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
  * not once per pixel: the same expectation.
  * transfer: device [n_vertices][nc][nc][3]; light_sh: device [nc][3];
  * color: device float4 [height][width].  The mesh needs vertex normals;
- * materials must be roughPlastic, cookTorrance or simple_ggx.  The light
+ * materials must be roughPlastic, DisneyBSDF, cookTorrance or simple_ggx
+ * (the reference's SH program calls the material's own callables; it has
+ * none for mirrors).  The light
  * responses and vertex colours live in work buffers of the context: calls
  * issued on different streams must not overlap. */
 nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera *camera,
@@ -598,10 +618,9 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
  * counter-based generator keyed by (seed, vertex), the same for every
  * basis map, as the reference's per-vertex stream is: estimates agree with
  * the reference's in expectation, not sample by sample.  Materials:
- * roughPlastic, cookTorrance, mirror, simple_ggx (DisneyBSDF gives
- * NWPRT_ERR_UNSUPPORTED).  The sampling tables and path records live in work
- * buffers of the context: calls issued on different streams must not
- * overlap. */
+ * roughPlastic, DisneyBSDF, cookTorrance, mirror, simple_ggx.  The sampling
+ * tables and path records live in work buffers of the context: calls issued
+ * on different streams must not overlap. */
 nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const float *basis_faces,
                                        int32_t face_n, int32_t degree, int32_t samples,
                                        int32_t max_bounce, uint32_t seed, int64_t vertex_begin,

@@ -125,7 +125,28 @@ class Camera(Structure):
 
 class Material(Structure):
     _fields_ = [("kd", c_float * 3), ("ks", c_float * 3), ("roughness", c_float),
-                ("type", c_int32)]
+                ("type", c_int32), ("specular_transmission", c_float), ("metallic", c_float),
+                ("subsurface", c_float), ("specular", c_float), ("specular_tint", c_float),
+                ("anisotropic", c_float), ("sheen", c_float), ("sheen_tint", c_float),
+                ("clearcoat", c_float), ("clearcoat_gloss", c_float), ("eta", c_float)]
+
+
+# Material rows as arrays: kd[3], ks[3], roughness, type, then the
+# DisneyBSDF parameters and eta in the struct's order.  Rows of 8 take
+# Material::init's values for the rest (Renderer/Material.h:79-97).
+MATERIAL_ROW = 19
+MATERIAL_TAIL_DEFAULTS = (0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.5, 0.0, 1.0,
+                          float(np.float32(1.49) / np.float32(1.000277)))
+
+
+def material_row(m) -> np.ndarray:
+    """A material row of 8 or 19 values as the full 19 (float64)."""
+    r = np.asarray(m, np.float64).reshape(-1)
+    if r.size == 8:
+        r = np.concatenate([r, MATERIAL_TAIL_DEFAULTS])
+    if r.size != MATERIAL_ROW:
+        raise ValueError(f"a material row has 8 or {MATERIAL_ROW} values, not {r.size}")
+    return r
 
 
 class MeshInfo(Structure):
@@ -229,6 +250,7 @@ _SIGNATURES = {
     "nwprt_sh_compute_transfer": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_int32, c_int32,
                                           c_int32, c_uint32, ctypes.c_int64, ctypes.c_int64,
                                           c_void_p, c_void_p]),
+    "nwprt_material_defaults": (c_int, [c_int32, c_void_p]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -350,8 +372,8 @@ def synth_gbuffer(seed: int, width: int, height: int,
 
 def obj_read(path: str) -> Dict[str, np.ndarray]:
     """Wavefront OBJ (+ MTL) as mesh arrays: vertices [n][3], normals [n][3]
-    or None, indices [t][3], material_ids [t], materials [m][8] rows
-    (kd[3], ks[3], roughness, type)."""
+    or None, indices [t][3], material_ids [t], materials [m][19] rows
+    (kd[3], ks[3], roughness, type, the DisneyBSDF parameters, eta)."""
     return _mesh_read(lib().nwprt_obj_read, path)
 
 
@@ -372,10 +394,10 @@ def _mesh_read(fn, path) -> Dict[str, np.ndarray]:
                        if nr.value else None),
            "indices": _take_host(ix, nt.value * 3, np.int32).reshape(-1, 3),
            "material_ids": _take_host(mid, nt.value, np.int32)}
-    raw = _take_host(mats, nm.value * 8, np.float32).reshape(-1, 8)
-    # The struct's last field is an int32 type: reinterpret that column.
+    raw = _take_host(mats, nm.value * MATERIAL_ROW, np.float32).reshape(-1, MATERIAL_ROW)
+    # The struct's eighth field is an int32 type: reinterpret that column.
     out["materials"] = np.concatenate(
-        [raw[:, :7], raw[:, 7:8].view(np.int32).astype(np.float32)], axis=1)
+        [raw[:, :7], raw[:, 7:8].view(np.int32).astype(np.float32), raw[:, 8:]], axis=1)
     return out
 
 
@@ -427,6 +449,16 @@ def hdr_write(path: str, img: np.ndarray) -> None:
     img = np.ascontiguousarray(img, np.float32)
     _check(lib().nwprt_hdr_write(str(path).encode(), _ptr(img), img.shape[1], img.shape[0],
                                  img.shape[2]))
+
+
+def material_defaults(mat_type: int) -> np.ndarray:
+    """The reference's default material of a type (nwprt_material_defaults)
+    as a 19-value row."""
+    m = Material()
+    _check(lib().nwprt_material_defaults(mat_type, ctypes.byref(m)))
+    return np.array(list(m.kd) + list(m.ks) + [m.roughness, m.type, m.specular_transmission,
+                    m.metallic, m.subsurface, m.specular, m.specular_tint, m.anisotropic,
+                    m.sheen, m.sheen_tint, m.clearcoat, m.clearcoat_gloss, m.eta], np.float64)
 
 
 def cross_to_faces(cross: np.ndarray) -> np.ndarray:
@@ -783,7 +815,8 @@ class Context:
                     normals: Optional[np.ndarray] = None,
                     material_ids: Optional[np.ndarray] = None) -> int:
         """Uploads a world-space triangle mesh and builds its BVH.  materials:
-        sequence of (kd[3], ks[3], roughness, type) rows or Material."""
+        sequence of Material or rows (kd[3], ks[3], roughness, type[, the
+        DisneyBSDF parameters, eta]) of 8 or 19 values."""
         v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
         nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
@@ -793,9 +826,9 @@ class Context:
             if isinstance(m, Material):
                 mats[i] = m
             else:
-                m = np.asarray(m, np.float64).reshape(8)
+                m = material_row(m)
                 mats[i] = Material((c_float * 3)(*m[0:3]), (c_float * 3)(*m[3:6]), float(m[6]),
-                                   int(m[7]))
+                                   int(m[7]), *[float(x) for x in m[8:]])
         out = c_int32()
         _check(lib().nwprt_mesh_create(self.h, _ptr(v), len(v), _ptr(nr), _ptr(ix), len(ix),
                                        _ptr(mid), mats, len(materials), ctypes.byref(out)))

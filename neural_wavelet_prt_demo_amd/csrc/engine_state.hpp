@@ -97,8 +97,8 @@ struct Mesh {
   int64_t n_tris = 0, n_nodes = 0, n_vertices = 0;
   int depth = 0;
   bool has_normals = false, has_tri_material = false;
-  bool direct_ok = true;  // every material has a direct-light BSDF here
-  bool sh_ok = true;      // every material has an SH-render BSDF here
+  bool sh_ok = true;    // no mirror material (the SH render has no mirror BSDF)
+  bool disney = false;  // some material the triangles use is DisneyBSDF
 };
 
 // Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,

@@ -271,8 +271,17 @@ struct Surface {
   float3 kd, ks;
   float rough;
   float3 ns, ng;  // unit shading and geometric normals
-  int type;       // 0 roughPlastic, 2 cookTorrance, 4 simple_ggx
+  int type;       // 0 roughPlastic, 1 DisneyBSDF, 2 cookTorrance, 4 simple_ggx
+  // DisneyBSDF parameters; eta also for roughPlastic.
+  float st, metallic, subsurface, specular, spec_tint, aniso, sheen, sheen_tint, clearcoat,
+      cc_gloss, eta;
 };
+
+__device__ inline Surface make_surface(const MaterialRec &m, float3 ns, float3 ng) {
+  return Surface{v3(m.kd), v3(m.ks), m.roughness, ns, ng, m.type, m.specular_transmission,
+                 m.metallic, m.subsurface, m.specular, m.specular_tint, m.anisotropic, m.sheen,
+                 m.sheen_tint, m.clearcoat, m.clearcoat_gloss, m.eta};
+}
 
 __device__ float3 ct_eval(const Surface &s, float3 l, float3 v) {
   if (dot(s.ng, v) < 0.0f || dot(s.ng, l) < 0.0f) return v3(0.0f, 0.0f, 0.0f);
@@ -357,9 +366,7 @@ __device__ float3 ct_sample(const Surface &s, float3 v, float rw, float ru, floa
 
 // roughPlastic (roughPlastic.cu:25-174): a GGX coat over a diffuse base,
 // both weighted by the dielectric Fresnel term at the material's index of
-// refraction, which this build fixes at the reference's default
-// 1.49 / 1.000277 (Material.h:94-96).
-constexpr float kEta = 1.49f / 1.000277f;
+// refraction eta (Material.h:55, default 1.49 / 1.000277, :94-97).
 
 __device__ inline float fresnel_dielectric(float n_dot_i, float eta) {
   const float t2 = 1.0f - (1.0f - n_dot_i * n_dot_i) / (eta * eta);
@@ -376,11 +383,11 @@ __device__ float3 rp_eval(const Surface &s, float3 l, float3 v) {
   const float n_h = dot(n, h), n_in = dot(n, v), n_out = dot(n, l);
   if (n_out <= 0.0f || n_h <= 0.0f) return v3(0.0f, 0.0f, 0.0f);
   const float r = fminf(fmaxf(s.rough, 0.01f), 1.0f), a = r * r;
-  const float fo = fresnel_dielectric(dot(h, l), kEta);
+  const float fo = fresnel_dielectric(dot(h, l), s.eta);
   const float d = gtr2(to_local(h, n), a, a);
   const float g = smith_gtr2(to_local(v, n), a, a) * smith_gtr2(to_local(l, n), a, a);
   const float k = g * fo * d / (4.0f * n_in * n_out);
-  const float fi = fresnel_dielectric(dot(h, v), kEta);
+  const float fi = fresnel_dielectric(dot(h, v), s.eta);
   const float base = (1.0f - fo) * (1.0f - fi) / 3.14159265358979f;
   return v3((s.kd.x * base + s.ks.x * k) * n_out, (s.kd.y * base + s.ks.y * k) * n_out,
             (s.kd.z * base + s.ks.z * k) * n_out);
@@ -450,11 +457,237 @@ __device__ float3 sg_eval(const Surface &s, float3 l, float3 v) {
   return v3(sp + s.kd.x * dif, sp + s.kd.y * dif, sp + s.kd.z * dif);
 }
 
+// DisneyBSDF (disneyBSDF.cu:44-492): diffuse with the subsurface blend,
+// sheen, anisotropic GTR2 metal, GTR1 clearcoat and a rough dielectric
+// glass lobe (reflection and refraction), mixed by the weights of the
+// reference, with its pdf and sampler as they are written -- including the
+// diffuse pdf's cosine of the view direction and the lobe probabilities
+// left unscaled in the refraction case, which the estimators' expectations
+// depend on.
+__device__ inline float gtr1(float n_dot_h, float a) {
+  const float a2 = a * a, t = 1.0f + (a2 - 1.0f) * n_dot_h * n_dot_h;
+  return (a2 - 1.0f) / (3.14159265358979f * logf(a2) * t);
+}
+__device__ inline float schlick5(float c) { return powf(fmaxf(1.0f - c, 0.0f), 5.0f); }
+
+struct DisneyLobes {
+  float3 n;  // shading normal flipped to the view's geometric side
+  float ax, ay, ac, cc;
+  float dw, mw, gw, cw;  // lobe weights
+};
+__device__ inline DisneyLobes disney_lobes(const Surface &s, float3 v) {
+  DisneyLobes d;
+  d.n = dot(s.ns, v) * dot(s.ng, v) < 0.0f ? scale(s.ns, -1.0f) : s.ns;
+  d.cc = 0.25f * s.clearcoat;
+  const float aspect = sqrtf(1.0f - s.aniso * 0.9f);
+  d.ax = fmaxf(0.0001f, s.rough * s.rough / aspect);
+  d.ay = fmaxf(0.0001f, s.rough * s.rough * aspect);
+  d.ac = (1.0f - s.cc_gloss) * 0.1f + s.cc_gloss * 0.001f;
+  d.dw = (1.0f - s.metallic) * (1.0f - s.st);
+  d.mw = 1.0f - s.st * (1.0f - s.metallic);
+  d.gw = (1.0f - s.metallic) * s.st;
+  d.cw = d.cc;
+  return d;
+}
+__device__ inline float3 tint_of(float3 c) {
+  const float l = luminance(c);
+  return l > 0.0f ? scale(c, 1.0f / l) : v3(1.0f, 1.0f, 1.0f);
+}
+
+__device__ float3 ds_eval(const Surface &s, float3 l, float3 v) {
+  const DisneyLobes d = disney_lobes(s, v);
+  const float3 n = d.n;
+  const bool reflect = dot(s.ng, l) * dot(s.ng, v) > 0.0f;
+  if (reflect) {
+    const float n_in = dot(n, v), n_out = dot(n, l);
+    float3 h = normalize(add(v, l));
+    if (dot(h, n) < 0.0f) h = scale(h, -1.0f);
+    const float n_h = dot(s.ns, h), h_in = dot(h, v), h_out = dot(h, l);
+    float3 r = v3(0.0f, 0.0f, 0.0f);
+    if (dot(s.ng, v) >= 0.0f && dot(s.ng, l) >= 0.0f && n_out > 0.0f) {
+      const float fd90 = 0.5f + 2.0f * s.rough * h_out * h_out;
+      const float so = powf(1.0f - n_out, 5.0f), si = powf(1.0f - n_in, 5.0f);
+      const float sh = powf(1.0f - h_out, 5.0f);
+      const float base = (1.0f + (fd90 - 1.0f) * so) * (1.0f + (fd90 - 1.0f) * si);
+      const float fss90 = h_out * h_out * s.rough;
+      const float fss = (1.0f + (fss90 - 1.0f) * so) * (1.0f + (fss90 - 1.0f) * si);
+      const float ss = 1.25f * (fss * (1.0f / (n_out + n_in) - 0.5f) + 0.5f);
+      const float kdiff = (1.0f - s.st) * (1.0f - s.metallic) *
+                          ((base * (1.0f - s.subsurface) + ss * s.subsurface) / 3.14159265358979f) *
+                          n_out;
+      r = add(r, scale(s.kd, kdiff));
+      const float3 ct = tint_of(s.kd);
+      const float3 csheen = add(v3(1.0f - s.sheen_tint, 1.0f - s.sheen_tint, 1.0f - s.sheen_tint),
+                                scale(ct, s.sheen_tint));
+      r = add(r, scale(csheen, (1.0f - s.metallic) * s.sheen * sh * n_out));
+      if (n_in > 0.0f && h_out > 0.0f && n_h > 0.0f) {
+        const float f0 = (s.eta - 1.0f) * (s.eta - 1.0f) / ((s.eta + 1.0f) * (s.eta + 1.0f));
+        const float3 sc = add(v3(1.0f - s.spec_tint, 1.0f - s.spec_tint, 1.0f - s.spec_tint),
+                              scale(ct, s.spec_tint));
+        const float3 c0 = add(scale(sc, s.specular * f0 * (1.0f - s.metallic)), scale(s.kd, s.metallic));
+        const float sw = 1.0f - s.st * (1.0f - s.metallic);
+        const float fw = schlick5(h_out);
+        const float dg = gtr2(to_local(h, n), d.ax, d.ay) * smith_gtr2(to_local(v, n), d.ax, d.ay) *
+                         smith_gtr2(to_local(l, n), d.ax, d.ay);
+        const float k = sw * dg / (4.0f * n_in);
+        r = add(r, v3((c0.x + (1.0f - c0.x) * fw) * k, (c0.y + (1.0f - c0.y) * fw) * k,
+                      (c0.z + (1.0f - c0.z) * fw) * k));
+      }
+      if (n_in > 0.0f && n_h > 0.0f) {
+        const float fc = 0.04f + 0.96f * schlick5(h_out);
+        const float gc = smith_gtr2(to_local(v, n), 0.25f, 0.25f) *
+                         smith_gtr2(to_local(l, n), 0.25f, 0.25f);
+        const float c = d.cc * fc * gtr1(n_h, d.ac) * gc / (4.0f * n_in);
+        r = add(r, v3(c, c, c));
+      }
+    }
+    const float eta = dot(s.ng, v) > 0.0f ? s.eta : 1.0f / s.eta;
+    const float fg = fresnel_dielectric(h_in, eta);
+    const float dg = gtr2(to_local(h, n), d.ax, d.ay) * smith_gtr2(to_local(v, n), d.ax, d.ay) *
+                     smith_gtr2(to_local(l, n), d.ax, d.ay);
+    const float g = (1.0f - s.metallic) * s.st * (fg * dg) / (4.0f * fabsf(n_in));
+    return add(r, scale(s.kd, g));
+  }
+  const float eta = dot(s.ng, v) > 0.0f ? s.eta : 1.0f / s.eta;
+  float3 h = normalize(add(v, scale(l, eta)));
+  if (dot(h, n) < 0.0f) h = scale(h, -1.0f);
+  const float ef = 1.0f / (eta * eta);
+  const float h_in = dot(h, v), h_out = dot(h, l), sd = h_in + eta * h_out;
+  const float fg = fresnel_dielectric(h_in, eta);
+  const float dg = gtr2(to_local(h, n), d.ax, d.ay) * smith_gtr2(to_local(v, n), d.ax, d.ay) *
+                   smith_gtr2(to_local(l, n), d.ax, d.ay);
+  const float t = (1.0f - s.metallic) * s.st *
+                  (ef * (1.0f - fg) * dg * eta * eta * fabsf(h_out * h_in)) /
+                  (fabsf(dot(n, v)) * sd * sd);
+  return v3(sqrtf(s.kd.x) * t, sqrtf(s.kd.y) * t, sqrtf(s.kd.z) * t);
+}
+
+__device__ float ds_pdf(const Surface &s, float3 l, float3 v) {
+  const DisneyLobes d = disney_lobes(s, v);
+  const float3 n = d.n;
+  const bool reflect = dot(s.ng, l) * dot(s.ng, v) > 0.0f;
+  const float tot = d.dw + d.mw + d.gw + d.cw;
+  float dp = d.dw / tot, mp = d.mw / tot, gp = d.gw / tot, cp = d.cw / tot;
+  if (dot(s.ng, v) < 0.0f) {
+    dp = mp = cp = 0.0f;
+    if (gp > 0.0f) gp = 1.0f;
+  }
+  const float eta = dot(s.ng, v) > 0.0f ? s.eta : 1.0f / s.eta;
+  if (reflect) {
+    float3 h = normalize(add(v, l));
+    if (dot(h, n) < 0.0f) h = scale(h, -1.0f);
+    const float n_in = dot(n, v), n_h = dot(h, s.ns), h_in = dot(h, v), h_out = dot(h, l);
+    if (dot(s.ng, l) >= 0.0f && dot(s.ng, v) >= 0.0f) {
+      dp *= fmaxf(dot(n, v), 0.0f) / 3.14159265358979f;
+      if (n_in > 0.0f)
+        mp *= gtr2(to_local(h, n), d.ax, d.ay) * smith_gtr2(to_local(v, n), d.ax, d.ay) /
+              (4.0f * n_in);
+      else
+        mp = 0.0f;
+      if (n_h > 0.0f && h_out > 0.0f)
+        cp *= gtr1(n_h, d.ac) * n_h / (4.0f * h_out);
+      else
+        cp = 0.0f;
+    }
+    gp *= fresnel_dielectric(h_in, eta) * gtr2(to_local(h, n), d.ax, d.ay) *
+          smith_gtr2(to_local(v, n), d.ax, d.ay) / (4.0f * fabsf(n_in));
+  } else {
+    float3 h = normalize(add(v, scale(l, eta)));
+    if (dot(h, n) < 0.0f) h = scale(h, -1.0f);
+    const float h_in = dot(h, v), h_out = dot(h, l), sd = h_in + eta * h_out;
+    const float dh = eta * eta * h_out / (sd * sd);
+    gp *= (1.0f - fresnel_dielectric(h_in, eta)) * gtr2(to_local(h, n), d.ax, d.ay) *
+          smith_gtr2(to_local(v, n), d.ax, d.ay) * fabsf(dh * h_in / dot(n, v));
+  }
+  return dp + mp + gp + cp;
+}
+
+// Heitz's visible-normal sample for an incoming direction below the
+// surface flips it (shaderUtils.h:160-213).
+__device__ inline float3 visible_normal(float3 li, float ax, float ay, float u1, float u2) {
+  bool below = li.z < 0.0f;
+  if (below) li = scale(li, -1.0f);
+  const float3 hemi = normalize(v3(ax * li.x, ay * li.y, li.z));
+  const float r = sqrtf(u1), phi = 2.0f * 3.14159265358979f * u2;
+  const float t1 = r * cosf(phi);
+  float t2 = r * sinf(phi);
+  const float sm = (1.0f + hemi.z) * 0.5f;
+  t2 = (1.0f - sm) * sqrtf(1.0f - t1 * t1) + sm * t2;
+  const float3 disk = v3(t1, t2, sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2)));
+  const float3 hn = to_world(disk, hemi);
+  const float3 m = normalize(v3(ax * hn.x, ay * hn.y, fmaxf(0.0f, hn.z)));
+  return below ? scale(m, -1.0f) : m;
+}
+
+__device__ inline float3 reflect_about(float3 v, float3 h) {
+  return normalize(add(scale(v, -1.0f), scale(h, 2.0f * dot(v, h))));
+}
+
+// Glass lobe: reflect or refract about a visible normal, by the Fresnel
+// term against u.  Total internal reflection returns `tir`.
+__device__ inline float3 ds_glass(const Surface &s, const DisneyLobes &d, float3 v, float u,
+                                  float ru, float rv, float3 tir) {
+  float3 h = to_world(visible_normal(to_local(v, s.ns), d.ax, d.ay, ru, rv), s.ns);
+  if (dot(h, d.n) < 0.0f) h = scale(h, -1.0f);
+  const float h_in = dot(h, v);
+  const float eta = dot(s.ng, v) > 0.0f ? s.eta : 1.0f / s.eta;
+  if (u <= fresnel_dielectric(h_in, eta)) return reflect_about(v, h);
+  const float hos = 1.0f - (1.0f - h_in * h_in) / (eta * eta);
+  if (hos <= 0.0f) return tir;
+  if (h_in < 0.0f) h = scale(h, -1.0f);
+  const float ho = sqrtf(hos);
+  return add(scale(v, -1.0f / eta), scale(h, fabsf(h_in) / eta - ho));
+}
+
+__device__ float3 ds_sample(const Surface &s, float3 v, float rw, float ru, float rv) {
+  const DisneyLobes d = disney_lobes(s, v);
+  if (dot(s.ng, v) >= 0.0f) {
+    const float tot = d.dw + d.mw + d.gw + d.cw;
+    const float dp = d.dw / tot, mp = d.mw / tot, gp = d.gw / tot;
+    if (rw <= dp) {
+      const float phi = 2.0f * 3.14159265358979f * ru;
+      const float tmp = sqrtf(fminf(fmaxf(1.0f - rv, 0.0f), 1.0f));
+      return to_world(v3(cosf(phi) * tmp, sinf(phi) * tmp, sqrtf(fminf(fmaxf(rv, 0.0f), 1.0f))), d.n);
+    }
+    if (rw <= dp + mp)
+      return reflect_about(v, to_world(visible_normal(to_local(v, s.ns), d.ax, d.ay, ru, rv), s.ns));
+    if (rw <= dp + mp + gp) {
+      if (gp <= 0.0f) return v3(-2.0f, -2.0f, -2.0f);
+      return ds_glass(s, d, v, (rw - (dp + mp)) / gp, ru, rv, v3(-2.0f, -2.0f, -2.0f));
+    }
+    const float a2 = d.ac * d.ac;
+    const float ch = sqrtf(fmaxf(0.0f, (1.0f - powf(a2, 1.0f - ru)) / (1.0f - a2)));
+    const float sh = sqrtf(fmaxf(1.0f - ch * ch, 0.0f)), az = 2.0f * 3.14159265358979f * rv;
+    return reflect_about(v, to_world(v3(sh * cosf(az), sh * sinf(az), ch), d.n));
+  }
+  // From inside: the glass lobes only; total internal reflection gives the
+  // reference's zero vector here.
+  return ds_glass(s, d, v, rw, ru, rv, v3(0.0f, 0.0f, 0.0f));
+}
+
+// DS: the kernel instance carries the DisneyBSDF code (a mesh with a
+// DisneyBSDF material); instances without it keep the register budget of
+// the simpler families.
+template <bool DS>
 __device__ inline float3 bsdf_eval(const Surface &s, float3 l, float3 v) {
+  if (DS && s.type == 1) return ds_eval(s, l, v);
   return s.type == 0 ? rp_eval(s, l, v) : s.type == 4 ? sg_eval(s, l, v) : ct_eval(s, l, v);
 }
+template <bool DS>
 __device__ inline float bsdf_pdf(const Surface &s, float3 l, float3 v) {
+  if (DS && s.type == 1) return ds_pdf(s, l, v);
   return s.type == 0 ? rp_pdf(s, l, v) : ct_pdf(s, l, v);
+}
+// The material's sampler from the stream at counter c: roughPlastic draws
+// two uniforms, the others three.  x = -2 marks no sample.
+template <bool DS>
+__device__ inline float3 bsdf_sample(const Surface &s, float3 v, uint32_t seed, uint32_t key,
+                                     uint32_t c) {
+  const float r0 = rng_uniform(seed, key, c), r1 = rng_uniform(seed, key, c + 1);
+  if (s.type == 0) return rp_sample(s, v, r0, r1);
+  const float r2 = rng_uniform(seed, key, c + 2);
+  if (DS && s.type == 1) return ds_sample(s, v, r0, r1, r2);
+  return ct_sample(s, v, r0, r1, r2);
 }
 
 // Cube face and (u, v) of a direction (EnvDirecToUV, later tests win).
@@ -542,6 +775,7 @@ __device__ inline bool finite3(float3 c) {
 
 // Direct light at a first hit p seen along d (unit), averaged over the
 // pixel's samples.
+template <bool DS>
 __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, float3 d,
                                uint32_t pixel, int *stack) {
   const float3 v = scale(d, -1.0f);
@@ -554,8 +788,8 @@ __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, fl
                                   rng_uniform(a.seed, pixel, c + 1), rng_uniform(a.seed, pixel, c + 2));
       const float pe = env_pdf(a.env_tab, a.light_n, l);
       if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
-        const float pb = bsdf_pdf(s, l, v);
-        const float3 f = bsdf_eval(s, l, v);
+        const float pb = bsdf_pdf<DS>(s, l, v);
+        const float3 f = bsdf_eval<DS>(s, l, v);
         const float3 le = env_radiance(a.light_faces, a.light_n, l);
         const float w = pe / fmaxf(pe * pe + pb * pb, 1e-8f);
         const float3 r = v3(f.x * le.x * w, f.y * le.y * w, f.z * le.z * w);
@@ -568,16 +802,12 @@ __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, fl
     }
     // BSDF sample, power-heuristic weight.
     {
-      const float3 l =
-          s.type == 0 ? rp_sample(s, v, rng_uniform(a.seed, pixel, c + 3),
-                                  rng_uniform(a.seed, pixel, c + 4))
-                      : ct_sample(s, v, rng_uniform(a.seed, pixel, c + 3),
-                                  rng_uniform(a.seed, pixel, c + 4), rng_uniform(a.seed, pixel, c + 5));
+      const float3 l = bsdf_sample<DS>(s, v, a.seed, pixel, c + 3);
       if (l.x != -2.0f) {
-        const float pb = bsdf_pdf(s, l, v);
+        const float pb = bsdf_pdf<DS>(s, l, v);
         const float pe = env_pdf(a.env_tab, a.light_n, l);
         if (pe > 1e-4f && !occluded(a.nodes, a.tris, p, l, stack)) {
-          const float3 f = bsdf_eval(s, l, v);
+          const float3 f = bsdf_eval<DS>(s, l, v);
           const float3 le = env_radiance(a.light_faces, a.light_n, l);
           const float w = pb / fmaxf(pe * pe + pb * pb, 1e-8f);
           const float3 r = v3(f.x * le.x * w, f.y * le.y * w, f.z * le.z * w);
@@ -713,7 +943,7 @@ __global__ void env_face_table_kernel(double *__restrict__ tab) {
 // DIRECT: the direct-light estimate is computed and stored (a separate
 // instance, so the plain cast keeps its lower register count).
 // At most 128 VGPRs (4 waves per SIMD, the LDS stack allows 5).
-template <bool DIRECT>
+template <bool DIRECT, bool DS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gbuffer_cast_kernel(
     CastArgs a) {
   __shared__ int stack_lds[kBvhMaxDepth * 64];
@@ -773,11 +1003,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gb
     refl = normalize(reflect(d, ns));
     kd = v3(mat.kd);
     ks = v3(mat.ks);
+    if (mat.type == 1) {  // DisneyBSDF's PACKINFO (disneyBSDF.cu:10-25)
+      const float3 ct = tint_of(kd);
+      const float f0 = (mat.eta - 1.0f) * (mat.eta - 1.0f) / ((mat.eta + 1.0f) * (mat.eta + 1.0f));
+      const float3 sc = add(v3(1.0f - mat.specular_tint, 1.0f - mat.specular_tint,
+                               1.0f - mat.specular_tint),
+                            scale(ct, mat.specular_tint));
+      ks = add(scale(sc, mat.specular * f0 * (1.0f - mat.metallic)), scale(kd, mat.metallic));
+    }
     rough = mat.roughness;
     flag = 1;
     if constexpr (want_direct) {
-      const Surface sf{kd, ks, rough, ns, ng, mat.type};
-      direct = direct_light(a, sf, p, d, (uint32_t)(iy * a.width + ix), stack);
+      const Surface sf = make_surface(mat, ns, ng);
+      direct = direct_light<DS>(a, sf, p, d, (uint32_t)(iy * a.width + ix), stack);
     }
     break;
   }
@@ -964,6 +1202,7 @@ __global__ __launch_bounds__(256) void sh_light_response_kernel(ShArgs a) {
 // chip; their partial sums meet in a shuffle reduction.
 constexpr int kShPairLanes = 16;
 
+template <bool DS>
 __global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
   const int g = threadIdx.x & (kShPairLanes - 1);
   const long long u = (long long)blockIdx.x * (64 / kShPairLanes) + threadIdx.x / kShPairLanes;
@@ -975,7 +1214,7 @@ __global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
   const float3 pv = v3(a.vertices + 3 * (size_t)vi);
   const float3 n = normalize(v3(a.normals + 3 * (size_t)vi));
   const float3 view = normalize(sub(v3(a.cam_pos), pv));
-  const Surface sf{v3(mat.kd), v3(mat.ks), mat.roughness, n, n, mat.type};
+  const Surface sf = make_surface(mat, n, n);
   const int nc = (a.degree + 1) * (a.degree + 1);
   const float *lr = a.light_response + (size_t)vi * nc * 3;
   const uint32_t seed = a.seed ^ (0x9E3779B9u * (uint32_t)(mid + 1));
@@ -983,16 +1222,11 @@ __global__ __launch_bounds__(64) void sh_corner_kernel(ShArgs a) {
   int valid = 0;
   for (int k = g; live && k < a.samples; k += kShPairLanes) {
     const uint32_t q = (uint32_t)k * 4u;
-    const float3 l =
-        sf.type == 0 ? rp_sample(sf, view, rng_uniform(seed, (uint32_t)vi, q),
-                                 rng_uniform(seed, (uint32_t)vi, q + 1))
-                     : ct_sample(sf, view, rng_uniform(seed, (uint32_t)vi, q),
-                                 rng_uniform(seed, (uint32_t)vi, q + 1),
-                                 rng_uniform(seed, (uint32_t)vi, q + 2));
+    const float3 l = bsdf_sample<DS>(sf, view, seed, (uint32_t)vi, q);
     if (l.x == -2.0f) continue;
-    const float pdf = bsdf_pdf(sf, l, view);
+    const float pdf = bsdf_pdf<DS>(sf, l, view);
     if (pdf < 1e-6f) continue;
-    const float3 f = bsdf_eval(sf, l, view);
+    const float3 f = bsdf_eval<DS>(sf, l, view);
     if (!finite3(f)) continue;
     float y[kShCoeffs];
     sh_eval(l, y);
@@ -1141,18 +1375,17 @@ __device__ inline float3 tm_sphere(float z1, float z2) {
   return v3(sp * cosf(theta), sp * sinf(theta), cosf(phi));
 }
 
+template <bool DS>
 __device__ inline float3 tm_bsdf_sample(const Surface &s, float3 v, uint32_t seed, uint32_t vtx,
                                         uint32_t c) {
-  return s.type == 0 ? rp_sample(s, v, rng_uniform(seed, vtx, c), rng_uniform(seed, vtx, c + 1))
-                     : ct_sample(s, v, rng_uniform(seed, vtx, c), rng_uniform(seed, vtx, c + 1),
-                                 rng_uniform(seed, vtx, c + 2));
+  return bsdf_sample<DS>(s, v, seed, vtx, c);
 }
 
 __device__ inline Surface tm_surface(const TmArgs &a, int mat, float3 ns, float3 ng) {
-  const MaterialRec m = a.materials[mat];
-  return Surface{v3(m.kd), v3(m.ks), m.roughness, ns, ng, m.type};
+  return make_surface(a.materials[mat], ns, ng);
 }
 
+template <bool DS>
 __global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
   __shared__ int stack_lds[kBvhMaxDepth * 64];
   const int lane = threadIdx.x;
@@ -1211,25 +1444,25 @@ __global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
         r.tag = mat << 4 | depth;
         if (a.max_bounce == 1) {  // direct mode: BSDF light sample (devicePrograms.cu:202-247)
           TmBsdfLight &bl = *reinterpret_cast<TmBsdfLight *>(rec + 1);
-          const float3 l = tm_bsdf_sample(sf, v, a.seed, vtx, c + 3);
+          const float3 l = tm_bsdf_sample<DS>(sf, v, a.seed, vtx, c + 3);
           bl.visible = 0;
           if (l.x != -2.0f) {
-            const float3 f = bsdf_eval(sf, l, v);
+            const float3 f = bsdf_eval<DS>(sf, l, v);
             bl.l[0] = l.x, bl.l[1] = l.y, bl.l[2] = l.z;
-            bl.pb = bsdf_pdf(sf, l, v);
+            bl.pb = bsdf_pdf<DS>(sf, l, v);
             bl.f[0] = f.x, bl.f[1] = f.y, bl.f[2] = f.z;
             bl.visible = !occluded(a.nodes, a.tris, p, l, stack);
           }
         }
-        const float3 nd = tm_bsdf_sample(sf, v, a.seed, vtx, c + 6);
+        const float3 nd = tm_bsdf_sample<DS>(sf, v, a.seed, vtx, c + 6);
         if (nd.x == -2.0f) {
           done = true;
         } else {
-          pdf = bsdf_pdf(sf, nd, v);
+          pdf = bsdf_pdf<DS>(sf, nd, v);
           if (pdf <= 1e-4f) {
             done = true;
           } else {
-            const float3 f = bsdf_eval(sf, nd, v);
+            const float3 f = bsdf_eval<DS>(sf, nd, v);
             d = nd;
             lc = v3(lc.x * f.x / pdf, lc.y * f.y / pdf, lc.z * f.z / pdf);
           }
@@ -1278,6 +1511,7 @@ __global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
 
 __device__ inline bool tm_nan3(float3 c) { return isnan(c.x) || isnan(c.y) || isnan(c.z); }
 
+template <bool DS>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void tm_light_kernel(TmArgs a) {
   __shared__ int stack_lds[kBvhMaxDepth * 64];
   const int lane = threadIdx.x;
@@ -1314,8 +1548,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void tm
       // adds nothing whether or not the ray is blocked, so only a non-zero
       // finite term pays for the ray.
       const Surface sf = tm_surface(a, h.tag >> 4, v3(h.ns), v3(h.ng));
-      const float pb = bsdf_pdf(sf, l, v);
-      const float3 f = bsdf_eval(sf, l, v);
+      const float pb = bsdf_pdf<DS>(sf, l, v);
+      const float3 f = bsdf_eval<DS>(sf, l, v);
       const float3 le = env_radiance(faces, a.n, l);
       const float wt = pe / fmaxf(pe * pe + pb * pb, 1e-8f);
       const float3 inc = v3(f.x * le.x * lc.x * wt, f.y * le.y * lc.y * wt, f.z * le.z * lc.z * wt);
@@ -1434,10 +1668,14 @@ void launch_gbuffer_cast(const CastArgs &a, hipStream_t s) {
               (a.env_tab && a.light_n >= 1 && a.light_n <= 128 && a.direct_samples >= 1),
           NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: bad direct-light arguments");
   const long long tiles = (long long)((a.width + 7) / 8) * ((a.height + 7) / 8);
-  if (a.gb.direct && a.light_faces)
-    hipLaunchKernelGGL(gbuffer_cast_kernel<true>, dim3((unsigned)tiles), dim3(64), 0, s, a);
-  else
-    hipLaunchKernelGGL(gbuffer_cast_kernel<false>, dim3((unsigned)tiles), dim3(64), 0, s, a);
+  if (a.gb.direct && a.light_faces) {
+    if (a.disney)
+      hipLaunchKernelGGL((gbuffer_cast_kernel<true, true>), dim3((unsigned)tiles), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL((gbuffer_cast_kernel<true, false>), dim3((unsigned)tiles), dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((gbuffer_cast_kernel<false, false>), dim3((unsigned)tiles), dim3(64), 0, s, a);
+  }
   hip_check(hipGetLastError(), "gbuffer_cast_kernel launch");
 }
 
@@ -1457,9 +1695,11 @@ void launch_sh_render(const ShArgs &a, hipStream_t s) {
                      dim3(256), 0, s, a);
   hip_check(hipGetLastError(), "sh_light_response_kernel launch");
   constexpr int pairs_per_block = 64 / kShPairLanes;
-  hipLaunchKernelGGL(sh_corner_kernel,
-                     dim3((unsigned)((a.n_keys + pairs_per_block - 1) / pairs_per_block)),
-                     dim3(64), 0, s, a);
+  const dim3 corner_grid((unsigned)((a.n_keys + pairs_per_block - 1) / pairs_per_block));
+  if (a.disney)
+    hipLaunchKernelGGL(sh_corner_kernel<true>, corner_grid, dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(sh_corner_kernel<false>, corner_grid, dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "sh_corner_kernel launch");
   const long long tiles = (long long)((a.width + 7) / 8) * ((a.height + 7) / 8);
   hipLaunchKernelGGL(sh_pixel_kernel, dim3((unsigned)tiles), dim3(64), 0, s, a);
@@ -1485,9 +1725,16 @@ void launch_sh_transfer(const TmArgs &a, hipStream_t s) {
   const int nc = (a.degree + 1) * (a.degree + 1);
   require(pairs * nc / 64 < (1ll << 31), NWPRT_ERR_INVALID_ARGUMENT,
           "sh_transfer: too many (vertex, sample, map) triples for one launch");
-  hipLaunchKernelGGL(tm_paths_kernel, dim3((unsigned)((pairs + 63) / 64)), dim3(64), 0, s, a);
+  const dim3 path_grid((unsigned)((pairs + 63) / 64)), light_grid((unsigned)((pairs * nc + 63) / 64));
+  if (a.disney)
+    hipLaunchKernelGGL(tm_paths_kernel<true>, path_grid, dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(tm_paths_kernel<false>, path_grid, dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "tm_paths_kernel launch");
-  hipLaunchKernelGGL(tm_light_kernel, dim3((unsigned)((pairs * nc + 63) / 64)), dim3(64), 0, s, a);
+  if (a.disney)
+    hipLaunchKernelGGL(tm_light_kernel<true>, light_grid, dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(tm_light_kernel<false>, light_grid, dim3(64), 0, s, a);
   hip_check(hipGetLastError(), "tm_light_kernel launch");
   hipLaunchKernelGGL(tm_project_kernel, dim3((unsigned)a.n_verts), dim3(256), 0, s, a);
   hip_check(hipGetLastError(), "tm_project_kernel launch");

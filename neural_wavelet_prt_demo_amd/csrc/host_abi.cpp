@@ -184,9 +184,54 @@ struct ObjMesh {
   bool has_normals = false;
 };
 
+}  // namespace
+
+// Material::init (Renderer/Material.h:79-115) and the per-type constructors
+// (:119-155).
+nwprt_material material_defaults(int type) {
+  nwprt_material m{};
+  for (int c = 0; c < 3; ++c) m.kd[c] = m.ks[c] = 0.25f;
+  m.roughness = 0.5f;
+  m.type = type;
+  m.specular_transmission = 0.f;
+  m.metallic = 0.f;
+  m.subsurface = 0.f;
+  m.specular = 0.5f;
+  m.specular_tint = 0.f;
+  m.anisotropic = 0.f;
+  m.sheen = 0.f;
+  m.sheen_tint = 0.5f;
+  m.clearcoat = 0.f;
+  m.clearcoat_gloss = 1.f;
+  m.eta = 1.49f / 1.000277f;
+  switch (type) {
+    case 0:  // roughPlasticMaterial
+      for (int c = 0; c < 3; ++c) m.kd[c] = 0.5f, m.ks[c] = 1.0f;
+      m.roughness = 0.1f;
+      break;
+    case 1:  // DisneyBSDFMaterial
+      for (int c = 0; c < 3; ++c) m.kd[c] = 0.5f;
+      m.roughness = 0.5f;
+      break;
+    case 2:  // cookTorranceMaterial
+      for (int c = 0; c < 3; ++c) m.kd[c] = m.ks[c] = 0.5f;
+      m.roughness = 1.0f;
+      break;
+    case 4:  // simple_ggxMaterial
+      for (int c = 0; c < 3; ++c) m.kd[c] = 0.5f;
+      m.roughness = 1.0f;
+      break;
+    default:
+      break;
+  }
+  return m;
+}
+
+namespace {
+
 nwprt_material cook_torrance(float kd0, float kd1, float kd2, float ks0, float ks1, float ks2,
                              float ns) {
-  nwprt_material m{};
+  nwprt_material m = material_defaults(2);
   m.kd[0] = kd0, m.kd[1] = kd1, m.kd[2] = kd2;
   m.ks[0] = ks0, m.ks[1] = ks1, m.ks[2] = ks2;
   m.roughness = ns == 0.f ? 0.f : (float)std::pow(1.0 / ns, 2.0);
@@ -532,5 +577,13 @@ extern "C" nwprt_status nwprt_obj_read(const char *path, float **vertices, int64
     *n_vertices = (int64_t)(m.vertices.size() / 3);
     *n_triangles = (int64_t)(m.indices.size() / 3);
     *n_materials = (int32_t)m.materials.size();
+  });
+}
+
+extern "C" nwprt_status nwprt_material_defaults(int32_t type, nwprt_material *out) {
+  return host_guarded([&] {
+    require(out && type >= 0 && type <= 4, NWPRT_ERR_INVALID_ARGUMENT,
+            "material_defaults: null output or type outside 0..4");
+    *out = material_defaults(type);
   });
 }

@@ -187,6 +187,11 @@ struct alignas(16) MaterialRec {
   float roughness;
   float ks[3];
   int32_t type;
+  // DisneyBSDF parameters (Renderer/Material.h:45-55) and the index of
+  // refraction roughPlastic and DisneyBSDF use.
+  float specular_transmission, metallic, subsurface, specular;
+  float specular_tint, anisotropic, sheen, sheen_tint;
+  float clearcoat, clearcoat_gloss, eta, pad;
 };
 struct CastArgs {
   const BvhNode *nodes;
@@ -209,6 +214,7 @@ struct CastArgs {
   const double *env_tab;
   int light_n, direct_samples;
   uint32_t seed;
+  bool disney;  // some material is DisneyBSDF (selects the kernel instance)
 };
 // Doubles of the environment sampling tables for face size n: face pmf[6],
 // face cdf[7], then per face cdf_rows[n][n+1], pdf_rows[n][n],
@@ -247,6 +253,7 @@ struct ShArgs {
   const float *env_faces;       // view environment or null
   int env_n;
   float *color;                 // float4 [height][width]
+  bool disney;                  // some material is DisneyBSDF
 };
 void launch_sh_render(const ShArgs &a, hipStream_t s);
 
@@ -271,6 +278,7 @@ struct TmArgs {
   long long v_begin, n_verts;
   void *work;                   // tm_work_bytes(n_verts, samples, degree, max_bounce)
   float *transfer;              // [n_vertices][nc * nc][3]
+  bool disney;                  // some material is DisneyBSDF
 };
 size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce);
 void launch_sh_transfer(const TmArgs &a, hipStream_t s);

@@ -298,14 +298,31 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     std::vector<MaterialRec> mats(n_materials);
     for (int32_t i = 0; i < n_materials; ++i) {
       const nwprt_material &m = materials[i];
-      require(finite3(m.kd) && finite3(m.ks) && std::isfinite(m.roughness) && m.type >= 0 &&
-                  m.type <= 4,
-              NWPRT_ERR_INVALID_ARGUMENT, "mesh_create: bad material " + std::to_string(i));
+      const float params[11] = {m.specular_transmission, m.metallic, m.subsurface, m.specular,
+                                m.specular_tint, m.anisotropic, m.sheen, m.sheen_tint,
+                                m.clearcoat, m.clearcoat_gloss, m.eta};
+      bool finite = finite3(m.kd) && finite3(m.ks) && std::isfinite(m.roughness);
+      for (float x : params) finite = finite && std::isfinite(x);
+      require(finite && m.type >= 0 && m.type <= 4 && m.eta > 0.0f,
+              NWPRT_ERR_INVALID_ARGUMENT,
+              "mesh_create: bad material " + std::to_string(i) +
+                  " (non-finite value, type outside 0..4 or eta <= 0)");
       std::memset(&mats[i], 0, sizeof(MaterialRec));
       std::memcpy(mats[i].kd, m.kd, sizeof m.kd);
       std::memcpy(mats[i].ks, m.ks, sizeof m.ks);
       mats[i].roughness = m.roughness;
       mats[i].type = m.type;
+      mats[i].specular_transmission = m.specular_transmission;
+      mats[i].metallic = m.metallic;
+      mats[i].subsurface = m.subsurface;
+      mats[i].specular = m.specular;
+      mats[i].specular_tint = m.specular_tint;
+      mats[i].anisotropic = m.anisotropic;
+      mats[i].sheen = m.sheen;
+      mats[i].sheen_tint = m.sheen_tint;
+      mats[i].clearcoat = m.clearcoat;
+      mats[i].clearcoat_gloss = m.clearcoat_gloss;
+      mats[i].eta = m.eta;
     }
 
     auto mesh = std::make_unique<Mesh>();
@@ -386,8 +403,8 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
       used[0] = 1;
     for (size_t i = 0; i < mats.size(); ++i) {
       if (!used[i]) continue;
-      if (mats[i].type == 1) mesh->direct_ok = false;  // DisneyBSDF
-      if (mats[i].type == 1 || mats[i].type == NWPRT_MATERIAL_MIRROR) mesh->sh_ok = false;
+      if (mats[i].type == NWPRT_MATERIAL_MIRROR) mesh->sh_ok = false;
+      if (mats[i].type == 1) mesh->disney = true;
     }
     const int id = ctx->next_id++;
     ctx->meshes[id] = std::move(mesh);
@@ -444,6 +461,7 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
     a.normals = m.has_normals ? m.normals.as<float>() : nullptr;
     a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
     a.materials = m.materials.as<MaterialRec>();
+    a.disney = m.disney;
     const float *lo = params->bounds ? params->bounds : m.lower;
     const float *hi = params->bounds ? params->bounds + 3 : m.upper;
     for (int k = 0; k < 3; ++k) {
@@ -465,8 +483,6 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
               NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: light face size must be in [1, 128]");
       require(params->direct_samples >= 1 && params->direct_samples <= (1 << 20),
               NWPRT_ERR_INVALID_ARGUMENT, "gbuffer_cast: direct_samples must be in [1, 2^20]");
-      require(m.direct_ok, NWPRT_ERR_UNSUPPORTED,
-              "gbuffer_cast: the direct term does not support DisneyBSDF (type 1) materials");
       check_aligned(gbuffer->direct, 16, "G-buffer direct");
       ctx->env_tables.ensure(env_table_doubles(params->light_env_n) * sizeof(double));
       launch_env_tables(params->light_env_faces, params->light_env_n,
@@ -509,7 +525,7 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
     require(m.has_normals, NWPRT_ERR_INVALID_ARGUMENT,
             "sh_render: the mesh needs vertex normals");
     require(m.sh_ok, NWPRT_ERR_UNSUPPORTED,
-            "sh_render: materials must be roughPlastic, cookTorrance or simple_ggx");
+            "sh_render: mirror materials have no SH-render BSDF");
     require(finite3(camera->from) && finite3(camera->at) && finite3(camera->up),
             NWPRT_ERR_INVALID_ARGUMENT, "sh_render: non-finite camera");
     check_aligned(color, 16, "sh_render colour");
@@ -526,6 +542,7 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
     a.normals = m.normals.as<float>();
     a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
     a.materials = m.materials.as<MaterialRec>();
+    a.disney = m.disney;
     a.n_tris = m.n_tris;
     a.n_vertices = m.n_vertices;
     a.transfer = transfer;
@@ -567,8 +584,6 @@ nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const flo
     if (vertex_count < 0) vertex_count = m.n_vertices - vertex_begin;
     require(vertex_begin >= 0 && vertex_count >= 0 && vertex_begin + vertex_count <= m.n_vertices,
             NWPRT_ERR_INVALID_ARGUMENT, "sh_compute_transfer: vertex range outside the mesh");
-    require(m.direct_ok, NWPRT_ERR_UNSUPPORTED,
-            "sh_compute_transfer: DisneyBSDF (type 1) materials are not supported");
     check_aligned(basis_faces, 4, "basis faces");
     check_aligned(transfer, 4, "transfer");
     if (vertex_count == 0) return;
@@ -596,6 +611,7 @@ nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const flo
     a.normals = m.has_normals ? m.normals.as<float>() : nullptr;
     a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
     a.materials = m.materials.as<MaterialRec>();
+    a.disney = m.disney;
     a.basis = basis_faces;
     a.tabs = ctx->sh_tables.as<double>();
     a.tab_doubles = td;

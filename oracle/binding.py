@@ -52,6 +52,26 @@ def build() -> Path:
     return LIB_PATH
 
 
+# Material rows: kd[3], ks[3], roughness, type, then the DisneyBSDF
+# parameters and eta (ORC_MAT_FLOATS = 19 values); rows of 8 take
+# Material::init's values for the rest (Renderer/Material.h:79-97).
+MAT_FLOATS = 19
+_MAT_TAIL = np.array([0.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.5, 0.0, 1.0,
+                      np.float32(1.49) / np.float32(1.000277)], np.float32)
+
+
+def mat_rows(materials) -> np.ndarray:
+    """Material rows (8 or 19 values each) as float32 [n][19]."""
+    rows = []
+    for m in np.atleast_2d(np.asarray(materials, np.float32)):
+        m = m.reshape(-1)
+        if m.size == 8:
+            m = np.concatenate([m, _MAT_TAIL])
+        assert m.size == MAT_FLOATS, m.size
+        rows.append(m)
+    return np.ascontiguousarray(np.stack(rows), np.float32)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -102,6 +122,11 @@ def lib():
                                       c_int, c_uint32, c_void_p, ctypes.c_int64, c_void_p,
                                       c_void_p, c_void_p]
         l.orc_sh_transfer.restype = None
+        for name in ("orc_bsdf_sample_many", "orc_bsdf_eval_many", "orc_bsdf_pdf_many"):
+            getattr(l, name).restype = None
+        l.orc_bsdf_sample_many.argtypes = [c_void_p] * 4 + [ctypes.c_int64] + [c_void_p] * 3
+        l.orc_bsdf_eval_many.argtypes = [c_void_p] * 4 + [ctypes.c_int64] + [c_void_p] * 2
+        l.orc_bsdf_pdf_many.argtypes = [c_void_p] * 4 + [ctypes.c_int64] + [c_void_p] * 2
         l.orc_rng_uniform.argtypes = [c_uint32, c_uint32, c_uint32]
         l.orc_rng_uniform.restype = c_float
         l.orc_tonemap.restype = c_uint32
@@ -278,7 +303,7 @@ def gbuffer_cast(vertices: np.ndarray, indices: np.ndarray, materials, camera, w
     ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
     nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
     mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
-    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+    mats = mat_rows(materials)
     cam = np.ascontiguousarray(camera, np.float32).reshape(9)
     if bounds is None:
         bounds = np.concatenate([v.min(0), v.max(0)])
@@ -334,7 +359,7 @@ def _d3(x):
 
 def ct_eval(mat, ns, ng, l, v) -> np.ndarray:
     """cookTorrance BSDF value times the cosine, for mat = (kd, ks, roughness, type)."""
-    m = np.ascontiguousarray(mat, np.float32).reshape(8)
+    m = mat_rows([mat])[0]
     a = [_d3(x) for x in (ns, ng, l, v)]
     out = np.empty(3, np.float64)
     lib().orc_ct_eval(_p(m), *[_p(x) for x in a], _p(out))
@@ -342,7 +367,7 @@ def ct_eval(mat, ns, ng, l, v) -> np.ndarray:
 
 
 def ct_pdf(mat, ns, ng, l, v) -> float:
-    m = np.ascontiguousarray(mat, np.float32).reshape(8)
+    m = mat_rows([mat])[0]
     a = [_d3(x) for x in (ns, ng, l, v)]
     return lib().orc_ct_pdf(_p(m), *[_p(x) for x in a])
 
@@ -367,7 +392,7 @@ def sh_render(vertices, indices, normals, materials, camera, width: int, height:
     ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
     nr = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
     mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
-    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+    mats = mat_rows(materials)
     cam = np.ascontiguousarray(camera, np.float32).reshape(9)
     t = np.ascontiguousarray(transfer, np.float32)
     ls = np.ascontiguousarray(light_sh, np.float32)
@@ -397,7 +422,7 @@ def sh_transfer(vertices, indices, materials, basis, degree: int, *, normals=Non
     ix = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
     nr = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
     mid = None if material_ids is None else np.ascontiguousarray(material_ids, np.int32)
-    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 8)
+    mats = mat_rows(materials)
     nc = (degree + 1) ** 2
     b = np.ascontiguousarray(basis, np.float32)
     assert b.ndim == 5 and b.shape[0] == nc and b.shape[1] == 6 and b.shape[4] == 3
@@ -410,4 +435,34 @@ def sh_transfer(vertices, indices, materials, basis, degree: int, *, normals=Non
     lib().orc_sh_transfer(ctypes.byref(mesh), _p(b), n, degree, samples, max_bounce, rr_depth,
                           seed & 0xFFFFFFFF, _p(vs), len(vs), _p(out["T"]), _p(out["abs_sum"]),
                           _p(out["margin"]))
+    return out
+
+
+def _bsdf_args(mat, ns, ng, v):
+    return (mat_rows([mat])[0], _d3(ns), _d3(ng), _d3(v))
+
+
+def bsdf_sample_many(mat, ns, ng, v, u):
+    """The material's sampler on uniforms u [n][3]: (directions [n][3], ok [n])."""
+    m, a, b, c = _bsdf_args(mat, ns, ng, v)
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+    out = np.empty((len(u), 3), np.float64)
+    ok = np.empty(len(u), np.int32)
+    lib().orc_bsdf_sample_many(_p(m), _p(a), _p(b), _p(c), len(u), _p(u), _p(out), _p(ok))
+    return out, ok.astype(bool)
+
+
+def bsdf_eval_many(mat, ns, ng, v, l):
+    m, a, b, c = _bsdf_args(mat, ns, ng, v)
+    l = np.ascontiguousarray(l, np.float64).reshape(-1, 3)
+    out = np.empty((len(l), 3), np.float64)
+    lib().orc_bsdf_eval_many(_p(m), _p(a), _p(b), _p(c), len(l), _p(l), _p(out))
+    return out
+
+
+def bsdf_pdf_many(mat, ns, ng, v, l):
+    m, a, b, c = _bsdf_args(mat, ns, ng, v)
+    l = np.ascontiguousarray(l, np.float64).reshape(-1, 3)
+    out = np.empty(len(l), np.float64)
+    lib().orc_bsdf_pdf_many(_p(m), _p(a), _p(b), _p(c), len(l), _p(l), _p(out))
     return out

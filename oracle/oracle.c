@@ -513,6 +513,10 @@ static void g_env(const float *faces, int n, const double *d, double *rgb) {
   if (!(d[1] > 0) && ay >= ax && ay >= az) { m = ay; uc = d[0]; vc = -d[2]; f = 3; }
   if (!(d[2] > 0) && az >= ax && az >= ay) { m = az; uc = d[0]; vc = d[1]; f = 4; }
   if (d[2] > 0 && az >= ax && az >= ay) { m = az; uc = -d[0]; vc = d[1]; f = 5; }
+  if (!(m > 0)) { /* no direction: no radiance, as the product's lookup */
+    rgb[0] = rgb[1] = rgb[2] = 0.0;
+    return;
+  }
   const double u = 0.5 * (uc / m + 1.0), v = 0.5 * (vc / m + 1.0);
   const double x = u * n - 0.5, y = v * n - 0.5;
   const double fx = floor(x), fy = floor(y), wx = x - fx, wy = y - fy;
@@ -618,6 +622,8 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
                      const double *p, const double *d, const double *ns, const double *ng,
                      const float *mat, int samples, uint32_t seed, uint32_t pixel, double *out);
 
+static void g_tint(const double *c, double *t);
+
 static void g_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
                    const float *bounds, const float *env_faces, int env_n, double *position,
                    double *normal, double *reflect, double *kd, double *ks, double *roughness,
@@ -665,7 +671,7 @@ static void g_cast(const orc_mesh *m, const float *camera, int w, int h, int max
         break;
       }
       const int mat = m->material_ids ? m->material_ids[hit] : 0;
-      const float *mr = m->materials + 8 * (size_t)mat;
+      const float *mr = m->materials + ORC_MAT_FLOATS * (size_t)mat;
       const int *tri = m->indices + 3 * (size_t)hit;
       double ns[3], ng[3];
       {
@@ -716,6 +722,14 @@ static void g_cast(const orc_mesh *m, const float *camera, int w, int h, int max
         rfl[a] = r[a];
         kdv[a] = mr[a];
         ksv[a] = mr[3 + a];
+      }
+      if ((int)mr[7] == 1) { /* DisneyBSDF's PACKINFO (disneyBSDF.cu:10-25) */
+        double ct[3];
+        g_tint(kdv, ct);
+        const double eta = mr[18];
+        const double f0 = (eta - 1.0) * (eta - 1.0) / ((eta + 1.0) * (eta + 1.0));
+        for (int a = 0; a < 3; ++a)
+          ksv[a] = mr[11] * f0 * (1.0 - mr[9]) * ((1.0 - mr[12]) + mr[12] * ct[a]) + mr[9] * kdv[a];
       }
       rough = mr[6];
       flag = 1;
@@ -963,8 +977,37 @@ static double g_smith(const double *v, double ax, double ay) {
 
 typedef struct {
   double kd[3], ks[3], rough, ns[3], ng[3];
-  int type; /* 0 roughPlastic, 2 cookTorrance, 4 simple_ggx */
+  int type; /* 0 roughPlastic, 1 DisneyBSDF, 2 cookTorrance, 4 simple_ggx */
+  /* DisneyBSDF parameters (Renderer/Material.h:45-55); eta also for
+   * roughPlastic */
+  double st, metallic, subsurface, specular, spec_tint, aniso, sheen, sheen_tint, clearcoat,
+      cc_gloss, eta;
 } g_surface;
+
+/* Material row (ORC_MAT_FLOATS values): kd[3], ks[3], roughness, type, then
+ * specular_transmission, metallic, subsurface, specular, specular_tint,
+ * anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss, eta. */
+static void g_surface_of(const float *mat, const double *ns, const double *ng, g_surface *s) {
+  for (int a = 0; a < 3; ++a) {
+    s->kd[a] = mat[a];
+    s->ks[a] = mat[3 + a];
+    s->ns[a] = ns[a];
+    s->ng[a] = ng[a];
+  }
+  s->rough = mat[6];
+  s->type = (int)mat[7];
+  s->st = mat[8];
+  s->metallic = mat[9];
+  s->subsurface = mat[10];
+  s->specular = mat[11];
+  s->spec_tint = mat[12];
+  s->aniso = mat[13];
+  s->sheen = mat[14];
+  s->sheen_tint = mat[15];
+  s->clearcoat = mat[16];
+  s->cc_gloss = mat[17];
+  s->eta = mat[18];
+}
 
 static void g_shading_normal(const g_surface *s, const double *v, double *n) {
   const double sg = g_dot(s->ns, v) < 0 ? -1.0 : 1.0;
@@ -1072,10 +1115,8 @@ static int g_ct_sample(const g_surface *s, const double *v, float rw, float ru, 
   return 1;
 }
 
-/* roughPlastic (Renderer/device_functions/roughPlastic.cu:25-174), index
- * of refraction at the reference default 1.49 / 1.000277
- * (Renderer/Material.h:94-96). */
-static const double g_eta = (double)(1.49f / 1.000277f);
+/* roughPlastic (Renderer/device_functions/roughPlastic.cu:25-174) at the
+ * material's index of refraction eta (Renderer/Material.h:55, 94-97). */
 
 static double g_fresnel(double n_dot_i, double eta) {
   const double t2 = 1.0 - (1.0 - n_dot_i * n_dot_i) / (eta * eta);
@@ -1094,7 +1135,7 @@ static void g_rp_eval(const g_surface *s, const double *l, const double *v, doub
   const double n_h = g_dot(n, h), n_in = g_dot(n, v), n_out = g_dot(n, l);
   if (n_out <= 0 || n_h <= 0) return;
   const double r = fmin(fmax(s->rough, 0.01), 1.0), a = r * r;
-  const double fo = g_fresnel(g_dot(h, l), g_eta), fi = g_fresnel(g_dot(h, v), g_eta);
+  const double fo = g_fresnel(g_dot(h, l), s->eta), fi = g_fresnel(g_dot(h, v), s->eta);
   g_to_local(h, n, hl);
   g_to_local(v, n, vl);
   g_to_local(l, n, ll);
@@ -1182,16 +1223,282 @@ static void g_sg_eval(const g_surface *s, const double *l, const double *v, doub
   for (int c = 0; c < 3; ++c) out[c] = frac / nom * n_l + s->kd[c] / M_PI * n_l;
 }
 
+/* DisneyBSDF (Renderer/device_functions/disneyBSDF.cu:44-492) with the
+ * helpers of shaderUtils.h:55-213: value, pdf and sampler as the reference
+ * writes them, including the diffuse pdf's cosine of the view direction and
+ * the lobe probabilities that the refraction case leaves unscaled. */
+typedef struct {
+  double n[3]; /* shading normal flipped to the view's geometric side */
+  double ax, ay, ac, cc, dw, mw, gw, cw;
+} g_dlobes;
+
+static void g_disney_lobes(const g_surface *s, const double *v, g_dlobes *d) {
+  const double sg = g_dot(s->ns, v) * g_dot(s->ng, v) < 0 ? -1.0 : 1.0;
+  for (int a = 0; a < 3; ++a) d->n[a] = sg * s->ns[a];
+  d->cc = 0.25 * s->clearcoat;
+  const double aspect = sqrt(1.0 - s->aniso * 0.9);
+  d->ax = fmax(0.0001, s->rough * s->rough / aspect);
+  d->ay = fmax(0.0001, s->rough * s->rough * aspect);
+  d->ac = (1.0 - s->cc_gloss) * 0.1 + s->cc_gloss * 0.001;
+  d->dw = (1.0 - s->metallic) * (1.0 - s->st);
+  d->mw = 1.0 - s->st * (1.0 - s->metallic);
+  d->gw = (1.0 - s->metallic) * s->st;
+  d->cw = d->cc;
+}
+
+static double g_gtr1(double n_dot_h, double a) {
+  const double a2 = a * a, t = 1.0 + (a2 - 1.0) * n_dot_h * n_dot_h;
+  return (a2 - 1.0) / (M_PI * log(a2) * t);
+}
+
+static void g_tint(const double *c, double *t) {
+  const double l = g_lum(c);
+  for (int a = 0; a < 3; ++a) t[a] = l > 0 ? c[a] / l : 1.0;
+}
+
+/* G(v) G(l) D(h) in the frame of n with alphas (ax, ay). */
+static double g_dg(const double *h, const double *v, const double *l, const double *n, double ax,
+                   double ay, int with_l) {
+  double hl[3], vl[3], ll[3];
+  g_to_local(h, n, hl);
+  g_to_local(v, n, vl);
+  double r = g_gtr2(hl, ax, ay) * g_smith(vl, ax, ay);
+  if (with_l) {
+    g_to_local(l, n, ll);
+    r *= g_smith(ll, ax, ay);
+  }
+  return r;
+}
+
+static void g_ds_eval(const g_surface *s, const double *l, const double *v, double *out) {
+  g_dlobes d;
+  g_disney_lobes(s, v, &d);
+  const double *n = d.n;
+  out[0] = out[1] = out[2] = 0.0;
+  const int reflect = g_dot(s->ng, l) * g_dot(s->ng, v) > 0;
+  if (reflect) {
+    const double n_in = g_dot(n, v), n_out = g_dot(n, l);
+    double h[3];
+    for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+    g_norm(h);
+    if (g_dot(h, n) < 0)
+      for (int a = 0; a < 3; ++a) h[a] = -h[a];
+    const double n_h = g_dot(s->ns, h), h_in = g_dot(h, v), h_out = g_dot(h, l);
+    if (g_dot(s->ng, v) >= 0 && g_dot(s->ng, l) >= 0 && n_out > 0) {
+      const double fd90 = 0.5 + 2.0 * s->rough * h_out * h_out;
+      const double so = pow(1.0 - n_out, 5.0), si = pow(1.0 - n_in, 5.0);
+      const double sh = pow(1.0 - h_out, 5.0);
+      const double base = (1.0 + (fd90 - 1.0) * so) * (1.0 + (fd90 - 1.0) * si);
+      const double fss90 = h_out * h_out * s->rough;
+      const double fss = (1.0 + (fss90 - 1.0) * so) * (1.0 + (fss90 - 1.0) * si);
+      const double ss = 1.25 * (fss * (1.0 / (n_out + n_in) - 0.5) + 0.5);
+      const double kdiff = (1.0 - s->st) * (1.0 - s->metallic) *
+                           ((base * (1.0 - s->subsurface) + ss * s->subsurface) / M_PI) * n_out;
+      double ct[3];
+      g_tint(s->kd, ct);
+      for (int a = 0; a < 3; ++a) {
+        out[a] += s->kd[a] * kdiff;
+        out[a] += ((1.0 - s->sheen_tint) + s->sheen_tint * ct[a]) * (1.0 - s->metallic) *
+                  s->sheen * sh * n_out;
+      }
+      if (n_in > 0 && h_out > 0 && n_h > 0) {
+        const double f0 = (s->eta - 1.0) * (s->eta - 1.0) / ((s->eta + 1.0) * (s->eta + 1.0));
+        const double sw = 1.0 - s->st * (1.0 - s->metallic);
+        const double fw = pow(fmax(1.0 - h_out, 0.0), 5.0);
+        const double k = sw * g_dg(h, v, l, n, d.ax, d.ay, 1) / (4.0 * n_in);
+        for (int a = 0; a < 3; ++a) {
+          const double sc = (1.0 - s->spec_tint) + s->spec_tint * ct[a];
+          const double c0 = s->specular * f0 * (1.0 - s->metallic) * sc + s->metallic * s->kd[a];
+          out[a] += (c0 + (1.0 - c0) * fw) * k;
+        }
+      }
+      if (n_in > 0 && n_h > 0) {
+        const double fc = 0.04 + 0.96 * pow(fmax(1.0 - h_out, 0.0), 5.0);
+        double vl[3], ll[3];
+        g_to_local(v, n, vl);
+        g_to_local(l, n, ll);
+        const double gc = g_smith(vl, 0.25, 0.25) * g_smith(ll, 0.25, 0.25);
+        const double c = d.cc * fc * g_gtr1(n_h, d.ac) * gc / (4.0 * n_in);
+        for (int a = 0; a < 3; ++a) out[a] += c;
+      }
+    }
+    const double eta = g_dot(s->ng, v) > 0 ? s->eta : 1.0 / s->eta;
+    const double g = (1.0 - s->metallic) * s->st * g_fresnel(h_in, eta) *
+                     g_dg(h, v, l, n, d.ax, d.ay, 1) / (4.0 * fabs(n_in));
+    for (int a = 0; a < 3; ++a) out[a] += s->kd[a] * g;
+    return;
+  }
+  const double eta = g_dot(s->ng, v) > 0 ? s->eta : 1.0 / s->eta;
+  double h[3];
+  for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a] * eta;
+  g_norm(h);
+  if (g_dot(h, n) < 0)
+    for (int a = 0; a < 3; ++a) h[a] = -h[a];
+  const double h_in = g_dot(h, v), h_out = g_dot(h, l), sd = h_in + eta * h_out;
+  const double t = (1.0 - s->metallic) * s->st *
+                   ((1.0 / (eta * eta)) * (1.0 - g_fresnel(h_in, eta)) *
+                    g_dg(h, v, l, n, d.ax, d.ay, 1) * eta * eta * fabs(h_out * h_in)) /
+                   (fabs(g_dot(n, v)) * sd * sd);
+  for (int a = 0; a < 3; ++a) out[a] = sqrt(s->kd[a]) * t;
+}
+
+static double g_ds_pdf(const g_surface *s, const double *l, const double *v) {
+  g_dlobes d;
+  g_disney_lobes(s, v, &d);
+  const double *n = d.n;
+  const int reflect = g_dot(s->ng, l) * g_dot(s->ng, v) > 0;
+  const double tot = d.dw + d.mw + d.gw + d.cw;
+  double dp = d.dw / tot, mp = d.mw / tot, gp = d.gw / tot, cp = d.cw / tot;
+  if (g_dot(s->ng, v) < 0) {
+    dp = mp = cp = 0.0;
+    if (gp > 0) gp = 1.0;
+  }
+  const double eta = g_dot(s->ng, v) > 0 ? s->eta : 1.0 / s->eta;
+  double h[3];
+  if (reflect) {
+    for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a];
+    g_norm(h);
+    if (g_dot(h, n) < 0)
+      for (int a = 0; a < 3; ++a) h[a] = -h[a];
+    const double n_in = g_dot(n, v), n_h = g_dot(h, s->ns), h_in = g_dot(h, v),
+                 h_out = g_dot(h, l);
+    if (g_dot(s->ng, l) >= 0 && g_dot(s->ng, v) >= 0) {
+      dp *= fmax(g_dot(n, v), 0.0) / M_PI;
+      mp = n_in > 0 ? mp * g_dg(h, v, l, n, d.ax, d.ay, 0) / (4.0 * n_in) : 0.0;
+      cp = (n_h > 0 && h_out > 0) ? cp * g_gtr1(n_h, d.ac) * n_h / (4.0 * h_out) : 0.0;
+    }
+    gp *= g_fresnel(h_in, eta) * g_dg(h, v, l, n, d.ax, d.ay, 0) / (4.0 * fabs(n_in));
+  } else {
+    for (int a = 0; a < 3; ++a) h[a] = v[a] + l[a] * eta;
+    g_norm(h);
+    if (g_dot(h, n) < 0)
+      for (int a = 0; a < 3; ++a) h[a] = -h[a];
+    const double h_in = g_dot(h, v), h_out = g_dot(h, l), sd = h_in + eta * h_out;
+    const double dh = eta * eta * h_out / (sd * sd);
+    gp *= (1.0 - g_fresnel(h_in, eta)) * g_dg(h, v, l, n, d.ax, d.ay, 0) *
+          fabs(dh * h_in / g_dot(n, v));
+  }
+  return dp + mp + gp + cp;
+}
+
+/* Heitz's visible-normal sample with alphas (ax, ay); an incoming direction
+ * below the surface is flipped (shaderUtils.h:160-213). */
+static void g_vndf(const double *li0, double ax, double ay, float u1, float u2, double *m) {
+  double li[3] = {li0[0], li0[1], li0[2]};
+  const int below = li[2] < 0;
+  if (below)
+    for (int a = 0; a < 3; ++a) li[a] = -li[a];
+  double hemi[3] = {ax * li[0], ay * li[1], li[2]}, disk[3], hn[3];
+  g_norm(hemi);
+  const double r = sqrt((double)u1), phi = 2.0 * M_PI * u2;
+  const double t1 = r * cos(phi);
+  double t2 = r * sin(phi);
+  const double sm = (1.0 + hemi[2]) / 2.0;
+  t2 = (1.0 - sm) * sqrt(1.0 - t1 * t1) + sm * t2;
+  disk[0] = t1;
+  disk[1] = t2;
+  disk[2] = sqrt(fmax(0.0, 1.0 - t1 * t1 - t2 * t2));
+  g_to_world(disk, hemi, hn);
+  m[0] = ax * hn[0];
+  m[1] = ay * hn[1];
+  m[2] = fmax(0.0, hn[2]);
+  g_norm(m);
+  if (below)
+    for (int a = 0; a < 3; ++a) m[a] = -m[a];
+}
+
+static void g_reflect_about(const double *v, const double *h, double *out) {
+  const double vh = g_dot(v, h);
+  for (int a = 0; a < 3; ++a) out[a] = -v[a] + 2.0 * vh * h[a];
+  g_norm(out);
+}
+
+/* Glass lobe; total internal reflection returns `tir` (0: no sample, 1: the
+ * zero vector). */
+static int g_ds_glass(const g_surface *s, const g_dlobes *d, const double *v, double u, float ru,
+                      float rv, int tir, double *out) {
+  double li[3], m[3], h[3];
+  g_to_local(v, s->ns, li);
+  g_vndf(li, d->ax, d->ay, ru, rv, m);
+  g_to_world(m, s->ns, h);
+  if (g_dot(h, d->n) < 0)
+    for (int a = 0; a < 3; ++a) h[a] = -h[a];
+  const double h_in = g_dot(h, v);
+  const double eta = g_dot(s->ng, v) > 0 ? s->eta : 1.0 / s->eta;
+  if (u <= g_fresnel(h_in, eta)) {
+    g_reflect_about(v, h, out);
+    return 1;
+  }
+  const double hos = 1.0 - (1.0 - h_in * h_in) / (eta * eta);
+  if (hos <= 0) {
+    out[0] = out[1] = out[2] = 0.0;
+    return tir;
+  }
+  if (h_in < 0)
+    for (int a = 0; a < 3; ++a) h[a] = -h[a];
+  const double ho = sqrt(hos);
+  for (int a = 0; a < 3; ++a) out[a] = -v[a] / eta + (fabs(h_in) / eta - ho) * h[a];
+  return 1;
+}
+
+static int g_ds_sample(const g_surface *s, const double *v, float rw, float ru, float rv,
+                       double *out) {
+  g_dlobes d;
+  g_disney_lobes(s, v, &d);
+  if (g_dot(s->ng, v) >= 0) {
+    const double tot = d.dw + d.mw + d.gw + d.cw;
+    const double dp = d.dw / tot, mp = d.mw / tot, gp = d.gw / tot;
+    if (rw <= dp) {
+      const double phi = 2.0 * M_PI * ru;
+      const double tmp = sqrt(fmin(fmax(1.0 - rv, 0.0), 1.0));
+      const double loc[3] = {cos(phi) * tmp, sin(phi) * tmp, sqrt(fmin(fmax((double)rv, 0.0), 1.0))};
+      g_to_world(loc, d.n, out);
+      return 1;
+    }
+    if (rw <= dp + mp) {
+      double li[3], m[3], h[3];
+      g_to_local(v, s->ns, li);
+      g_vndf(li, d.ax, d.ay, ru, rv, m);
+      g_to_world(m, s->ns, h);
+      g_reflect_about(v, h, out);
+      return 1;
+    }
+    if (rw <= dp + mp + gp) {
+      if (gp <= 0) return 0;
+      return g_ds_glass(s, &d, v, (rw - (dp + mp)) / gp, ru, rv, 0, out);
+    }
+    const double a2 = d.ac * d.ac;
+    const double ch = sqrt(fmax(0.0, (1.0 - pow(a2, 1.0 - ru)) / (1.0 - a2)));
+    const double sh = sqrt(fmax(1.0 - ch * ch, 0.0)), az = 2.0 * M_PI * rv;
+    const double m[3] = {sh * cos(az), sh * sin(az), ch};
+    double h[3];
+    g_to_world(m, d.n, h);
+    g_reflect_about(v, h, out);
+    return 1;
+  }
+  return g_ds_glass(s, &d, v, rw, ru, rv, 1, out);
+}
+
 static void g_bsdf_eval(const g_surface *s, const double *l, const double *v, double *out) {
   if (s->type == 0)
     g_rp_eval(s, l, v, out);
+  else if (s->type == 1)
+    g_ds_eval(s, l, v, out);
   else if (s->type == 4)
     g_sg_eval(s, l, v, out);
   else
     g_ct_eval(s, l, v, out);
 }
 static double g_bsdf_pdf(const g_surface *s, const double *l, const double *v) {
-  return s->type == 0 ? g_rp_pdf(s, l, v) : g_ct_pdf(s, l, v);
+  return s->type == 0 ? g_rp_pdf(s, l, v) : s->type == 1 ? g_ds_pdf(s, l, v) : g_ct_pdf(s, l, v);
+}
+/* The material's sampler on uniforms (r0, r1, r2): roughPlastic uses two.
+ * Returns 0 for the reference's (-2, -2, -2). */
+static int g_bsdf_sample_r(const g_surface *s, const double *v, float r0, float r1, float r2,
+                           double *out) {
+  if (s->type == 0) return g_rp_sample(s, v, r0, r1, out);
+  if (s->type == 1) return g_ds_sample(s, v, r0, r1, r2, out);
+  return g_ct_sample(s, v, r0, r1, r2, out);
 }
 
 static int g_occluded(const orc_mesh *m, const double *o, const double *d) {
@@ -1222,14 +1529,7 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
                      const double *p, const double *d, const double *ns, const double *ng,
                      const float *mat, int samples, uint32_t seed, uint32_t pixel, double *out) {
   g_surface s;
-  for (int a = 0; a < 3; ++a) {
-    s.kd[a] = mat[a];
-    s.ks[a] = mat[3 + a];
-    s.ns[a] = ns[a];
-    s.ng[a] = ng[a];
-  }
-  s.rough = mat[6];
-  s.type = (int)mat[7];
+  g_surface_of(mat, ns, ng, &s);
   const double v[3] = {-d[0], -d[1], -d[2]};
   double acc[3] = {0, 0, 0};
   for (int k = 0; k < samples; ++k) {
@@ -1248,11 +1548,9 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
         if (!isnan(r)) acc[a] += r;
       }
     }
-    const int ok = s.type == 0 ? g_rp_sample(&s, v, orc_rng_uniform(seed, pixel, c + 3),
-                                             orc_rng_uniform(seed, pixel, c + 4), l)
-                               : g_ct_sample(&s, v, orc_rng_uniform(seed, pixel, c + 3),
-                                             orc_rng_uniform(seed, pixel, c + 4),
-                                             orc_rng_uniform(seed, pixel, c + 5), l);
+    const int ok = g_bsdf_sample_r(&s, v, orc_rng_uniform(seed, pixel, c + 3),
+                                   orc_rng_uniform(seed, pixel, c + 4),
+                                   orc_rng_uniform(seed, pixel, c + 5), l);
     if (ok) {
       const double pb = g_bsdf_pdf(&s, l, v);
       pe = g_env_pdf(tab, n, l);
@@ -1283,17 +1581,7 @@ void orc_gbuffer_cast_direct(const orc_mesh *m, const float *camera, int w, int 
   free(tab);
 }
 
-/* Test taps of the direct-light pieces (mat: kd[3], ks[3], roughness, type). */
-static void g_surface_of(const float *mat, const double *ns, const double *ng, g_surface *s) {
-  for (int a = 0; a < 3; ++a) {
-    s->kd[a] = mat[a];
-    s->ks[a] = mat[3 + a];
-    s->ns[a] = ns[a];
-    s->ng[a] = ng[a];
-  }
-  s->rough = mat[6];
-  s->type = (int)mat[7];
-}
+/* Test taps of the direct-light pieces (mat: a material row). */
 void orc_ct_eval(const float *mat, const double *ns, const double *ng, const double *l,
                  const double *v, double *out) {
   g_surface s;
@@ -1387,7 +1675,7 @@ void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const f
     const int64_t tri = c / 3;
     const int vi = m->indices[c];
     const int mid = m->material_ids ? m->material_ids[tri] : 0;
-    const float *mat = m->materials + 8 * (size_t)mid;
+    const float *mat = m->materials + ORC_MAT_FLOATS * (size_t)mid;
     g_surface s;
     double nrm[3], view[3];
     for (int a = 0; a < 3; ++a) {
@@ -1412,12 +1700,9 @@ void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const f
     int valid = 0;
     for (int k = 0; k < samples; ++k) {
       const uint32_t q = (uint32_t)k * 4u;
-      const int ok = s.type == 0
-                         ? g_rp_sample(&s, view, orc_rng_uniform(sd, (uint32_t)vi, q),
-                                       orc_rng_uniform(sd, (uint32_t)vi, q + 1), l)
-                         : g_ct_sample(&s, view, orc_rng_uniform(sd, (uint32_t)vi, q),
-                                       orc_rng_uniform(sd, (uint32_t)vi, q + 1),
-                                       orc_rng_uniform(sd, (uint32_t)vi, q + 2), l);
+      const int ok = g_bsdf_sample_r(&s, view, orc_rng_uniform(sd, (uint32_t)vi, q),
+                                     orc_rng_uniform(sd, (uint32_t)vi, q + 1),
+                                     orc_rng_uniform(sd, (uint32_t)vi, q + 2), l);
       if (!ok) continue;
       const double pdf = g_bsdf_pdf(&s, l, view);
       if (pdf < 1e-6) continue;
@@ -1540,6 +1825,14 @@ static double g_rel_gap(double a, double b) {
  * uniform picks the lobe against the specular probability). */
 static double g_lobe_gap(const g_surface *s, float r) {
   double spec;
+  if (s->type == 1) { /* DisneyBSDF: the cumulative lobe probabilities */
+    g_dlobes d;
+    const double vv[3] = {s->ns[0], s->ns[1], s->ns[2]};
+    g_disney_lobes(s, vv, &d);
+    const double tot = d.dw + d.mw + d.gw + d.cw;
+    const double c1 = d.dw / tot, c2 = c1 + d.mw / tot, c3 = c2 + d.gw / tot;
+    return fmin(fabs((double)r - c1), fmin(fabs((double)r - c2), fabs((double)r - c3)));
+  }
   if (s->type == 0) {
     const double ls = g_lum(s->ks), lr = g_lum(s->kd);
     if (ls + lr <= 0) return 1.0;
@@ -1555,8 +1848,7 @@ static int g_bsdf_sample(const g_surface *s, const double *v, uint32_t seed, uin
   const float r0 = orc_rng_uniform(seed, vtx, c), r1 = orc_rng_uniform(seed, vtx, c + 1);
   const double g = g_lobe_gap(s, r0);
   if (g < *margin) *margin = g;
-  return s->type == 0 ? g_rp_sample(s, v, r0, r1, out)
-                      : g_ct_sample(s, v, r0, r1, orc_rng_uniform(seed, vtx, c + 2), out);
+  return g_bsdf_sample_r(s, v, r0, r1, orc_rng_uniform(seed, vtx, c + 2), out);
 }
 
 /* One path of basis map i from vertex position o0 (sample s); returns the
@@ -1618,7 +1910,7 @@ static void g_transfer_path(const orc_mesh *m, const float *faces, const double 
       for (int a = 0; a < 3; ++a) p[a] = o[a] + rd[a] * t + 0.001;
       for (int a = 0; a < 3; ++a) o[a] = p[a];
       const int mid = m->material_ids ? m->material_ids[hit] : 0;
-      const float *mr = m->materials + 8 * (size_t)mid;
+      const float *mr = m->materials + ORC_MAT_FLOATS * (size_t)mid;
       if ((int)mr[7] == 3) {
         const double nd = g_dot(ns, rd);
         for (int a = 0; a < 3; ++a) d[a] = rd[a] - 2.0 * nd * ns[a];
@@ -1750,4 +2042,29 @@ void orc_sh_transfer(const orc_mesh *m, const float *basis, int n, int degree, i
     if (margin) margin[vi * nc + i] = mg;
   }
   free(tabs);
+}
+
+/* ---- BSDF test taps over many directions ------------------------------
+ * The material's sampler on n uniform triples u[n][3] (ok[i] = 0 where it
+ * returns the reference's (-2, -2, -2)), and its value (with the cosine)
+ * and pdf at n directions l[n][3], for view direction v; mat is a material
+ * row. */
+void orc_bsdf_sample_many(const float *mat, const double *ns, const double *ng, const double *v,
+                          int64_t n, const float *u, double *out, int32_t *ok) {
+  g_surface s;
+  g_surface_of(mat, ns, ng, &s);
+  for (int64_t i = 0; i < n; ++i)
+    ok[i] = g_bsdf_sample_r(&s, v, u[3 * i], u[3 * i + 1], u[3 * i + 2], out + 3 * i);
+}
+void orc_bsdf_eval_many(const float *mat, const double *ns, const double *ng, const double *v,
+                        int64_t n, const double *l, double *out) {
+  g_surface s;
+  g_surface_of(mat, ns, ng, &s);
+  for (int64_t i = 0; i < n; ++i) g_bsdf_eval(&s, l + 3 * i, v, out + 3 * i);
+}
+void orc_bsdf_pdf_many(const float *mat, const double *ns, const double *ng, const double *v,
+                       int64_t n, const double *l, double *out) {
+  g_surface s;
+  g_surface_of(mat, ns, ng, &s);
+  for (int64_t i = 0; i < n; ++i) out[i] = g_bsdf_pdf(&s, l + 3 * i, v);
 }

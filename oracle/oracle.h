@@ -121,8 +121,13 @@ void orc_denoise(const float *color, const float *normal, const float *albedo, i
                  int iterations, double sigma_color, double sigma_normal, double sigma_albedo,
                  double blend, double *out);
 
-/* Triangle mesh for orc_gbuffer_cast.  materials: [n][8] = kd[3], ks[3],
- * roughness, type (as a float; 3 = mirror). */
+/* Material row: kd[3], ks[3], roughness, type (as a float; 1 = DisneyBSDF,
+ * 3 = mirror), specular_transmission, metallic, subsurface, specular,
+ * specular_tint, anisotropic, sheen, sheen_tint, clearcoat,
+ * clearcoat_gloss, eta (Renderer/Material.h:40-55). */
+#define ORC_MAT_FLOATS 19
+
+/* Triangle mesh for orc_gbuffer_cast.  materials: [n][ORC_MAT_FLOATS]. */
 typedef struct {
   const float *vertices;     /* [n_vertices][3]        */
   const float *normals;      /* [n_vertices][3] or NULL */
@@ -190,6 +195,15 @@ void orc_sh_render(const orc_mesh *m, const float *camera, int w, int h, const f
 void orc_sh_transfer(const orc_mesh *m, const float *basis, int n, int degree, int samples,
                      int max_bounce, int rr_depth, uint32_t seed, const int64_t *verts,
                      int64_t n_verts, double *T, double *abs_sum, double *margin);
+
+/* BSDF taps over many directions (mat: a material row; u [n][3] uniforms;
+ * l, out [n][3]). */
+void orc_bsdf_sample_many(const float *mat, const double *ns, const double *ng, const double *v,
+                          int64_t n, const float *u, double *out, int32_t *ok);
+void orc_bsdf_eval_many(const float *mat, const double *ns, const double *ng, const double *v,
+                        int64_t n, const double *l, double *out);
+void orc_bsdf_pdf_many(const float *mat, const double *ns, const double *ng, const double *v,
+                       int64_t n, const double *l, double *out);
 
 /* FP32 restatement timed as the CPU baseline (cpu_baseline.c). */
 double orc_cpu_relight_fp32(const orc_scene *sc, const int32_t *sel, const float *light, int k,

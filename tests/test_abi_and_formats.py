@@ -306,9 +306,12 @@ def test_obj_reader(nw, tmp_path):
     np.testing.assert_array_equal(m["normals"][ix[0]], [[0, 0, 0]] * 3)  # corners without vn
     mats, mid = m["materials"], m["material_ids"]
     assert list(mid) == [2, 0, 0, 1]
-    np.testing.assert_allclose(mats[0], [0.8, 0.1, 0.1, 0.2, 0.2, 0.2, 1 / 16, 2], rtol=1e-6)
-    np.testing.assert_allclose(mats[1], [0.3, 0.3, 0.3, 0, 0, 0, 0, 2])
-    np.testing.assert_allclose(mats[2], [0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1, 2])
+    np.testing.assert_allclose(mats[0, :8], [0.8, 0.1, 0.1, 0.2, 0.2, 0.2, 1 / 16, 2], rtol=1e-6)
+    np.testing.assert_allclose(mats[1, :8], [0.3, 0.3, 0.3, 0, 0, 0, 0, 2])
+    np.testing.assert_allclose(mats[2, :8], [0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1, 2])
+    # the Disney parameters and eta keep Material::init's values
+    for row in mats:
+        np.testing.assert_allclose(row[8:], nw.MATERIAL_TAIL_DEFAULTS, rtol=1e-7)
     (tmp_path / "bad.obj").write_text("v 0 0 0\nf 1 2 3\n")
     with pytest.raises(nw.NwprtError) as e:
         nw.obj_read(tmp_path / "bad.obj")
@@ -333,7 +336,8 @@ def test_ply_reader(nw, tmp_path):
     assert m["vertices"].shape == (5, 3) and m["normals"].shape == (5, 3)
     np.testing.assert_array_equal(m["indices"], [[0, 1, 2], [0, 2, 3], [0, 1, 4]])
     np.testing.assert_array_equal(m["normals"][4], [1, 0, 0])
-    np.testing.assert_allclose(m["materials"], [[0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1, 2]])
+    np.testing.assert_allclose(m["materials"], [nw.material_row([0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1, 2])],
+                               rtol=1e-7)
     assert list(m["material_ids"]) == [0, 0, 0]
     # Binary little-endian: doubles, an extra property and an extra element.
     hdr = ("ply\nformat binary_little_endian 1.0\nelement vertex 3\nproperty double x\n"
@@ -361,3 +365,19 @@ def test_ply_reader(nw, tmp_path):
     with pytest.raises(nw.NwprtError) as e:
         nw.ply_read(tmp_path / "missing.ply")
     assert e.value.status == nw.ERR_IO
+
+
+def test_material_defaults_follow_the_reference_constructors(nw):
+    """Material::init and the per-type constructors (Renderer/Material.h:79-155)."""
+    tail = nw.MATERIAL_TAIL_DEFAULTS
+    want = {0: [0.5, 0.5, 0.5, 1, 1, 1, 0.1], 1: [0.5, 0.5, 0.5, 0.25, 0.25, 0.25, 0.5],
+            2: [0.5, 0.5, 0.5, 0.5, 0.5, 0.5, 1.0], 3: [0.25] * 6 + [0.5],
+            4: [0.5, 0.5, 0.5, 0.25, 0.25, 0.25, 1.0]}
+    for t, head in want.items():
+        m = nw.material_defaults(t)
+        np.testing.assert_allclose(m[:7], head, rtol=1e-7)
+        assert m[7] == t
+        np.testing.assert_allclose(m[8:], tail, rtol=1e-7)
+    assert float(tail[-1]) == float(np.float32(1.49) / np.float32(1.000277))
+    with pytest.raises(nw.NwprtError):
+        nw.material_defaults(5)

@@ -62,8 +62,10 @@ def _compare(got, want, tol_enc=TOL_ENC, min_safe=0.98):
     assert np.array_equal(got["env_flag"][safe], want["env_flag"][safe])
     hit = safe & (want["env_flag"] == 1)
     miss = safe & (want["env_flag"] == 0)
-    for k in ("kd", "ks"):
-        assert np.array_equal(got[k][hit].astype(np.float32), want[k][hit].astype(np.float32)), k
+    assert np.array_equal(got["kd"][hit].astype(np.float32), want["kd"][hit].astype(np.float32))
+    # ks is the material's own value, or DisneyBSDF's packed specular colour
+    # (computed: FP32 against FP64)
+    np.testing.assert_allclose(got["ks"][hit], want["ks"][hit], rtol=2e-6, atol=1e-7)
     assert np.array_equal(got["roughness"][safe].astype(np.float32),
                           want["roughness"][safe].astype(np.float32))
     for k in ("position", "normal", "reflect"):
@@ -228,15 +230,33 @@ def test_cast_feeds_relight(nw, orc, ctx):
     ctx.mesh_destroy(mesh)
 
 
+# DisneyBSDF parameter sets (specular_transmission, metallic, subsurface,
+# specular, specular_tint, anisotropic, sheen, sheen_tint, clearcoat,
+# clearcoat_gloss, eta): a plastic with sheen and clearcoat, a tinted metal,
+# a rough glass.
+DISNEY = [(0.0, 0.0, 0.4, 0.5, 0.2, 0.0, 0.7, 0.5, 0.6, 0.3, 1.5),
+          (0.0, 0.9, 0.0, 0.5, 0.5, 0.6, 0.0, 0.5, 0.0, 1.0, 1.45),
+          (0.8, 0.0, 0.0, 0.5, 0.0, 0.2, 0.0, 0.5, 0.3, 0.8, 1.5)]
+
+
 def _direct_scene(seed, mirror=True, kinds=None):
     """kinds: BSDF types for the three non-mirror materials (None keeps the
-    scene's roughPlastic x 2 + cookTorrance)."""
+    scene's roughPlastic x 2 + cookTorrance); DisneyBSDF materials take the
+    parameter sets of DISNEY in order."""
     s = mesh_scene(seed, mirror=mirror)
+    mats = np.array([nw_material_row(m) for m in s["materials"]], np.float32)
     if kinds is not None:
-        mats = s["materials"].copy()
         mats[:3, 7] = kinds
-        s["materials"] = mats
+        for i, k in enumerate(kinds):
+            if k == 1:
+                mats[i, 8:] = DISNEY[i]
+    s["materials"] = mats
     return s
+
+
+def nw_material_row(m):
+    import neural_wavelet_prt_demo_amd as nw
+    return nw.material_row(m)
 
 
 @pytest.mark.parametrize("seed,w,h,samples,bounce,kinds", [(0, 32, 24, 64, 1, (2, 2, 2)),
@@ -244,7 +264,9 @@ def _direct_scene(seed, mirror=True, kinds=None):
                                                             (2, 1, 1, 256, 1, (2, 2, 2)),
                                                             (3, 32, 24, 64, 1, None),
                                                             (4, 23, 19, 32, 2, None),
-                                                            (5, 32, 24, 48, 1, (4, 0, 4))])
+                                                            (5, 32, 24, 48, 1, (4, 0, 4)),
+                                                            (6, 32, 24, 64, 1, (1, 1, 1)),
+                                                            (7, 29, 17, 32, 2, (1, 0, 2))])
 def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce, kinds):
     """The direct term draws the same counter-based random numbers as the
     oracle's FP64 replay, so per pixel the two estimates differ only by
@@ -286,19 +308,16 @@ def test_direct_light_replays_oracle(nw, orc, ctx, seed, w, h, samples, bounce, 
 
 
 def test_direct_light_errors_and_skips(nw, ctx):
-    s = mesh_scene(0)
-    mats = s["materials"].copy()
-    mats[0, 7] = 1  # DisneyBSDF: no direct-light BSDF here
-    mesh = ctx.mesh_create(s["vertices"], s["indices"], mats, normals=s["normals"],
-                           material_ids=s["material_ids"])
+    s = _direct_scene(0, kinds=(1, 2, 2))
     frame = nw.DeviceFrame(8, 8)
     light = nw.DeviceBuffer.from_array(np.ones((6, 4, 4, 3), np.float32))
-    with pytest.raises(nw.NwprtError) as e:
-        ctx.gbuffer_cast(mesh, np.reshape(s["camera"], (3, 3)), frame, bounds=s["bounds"],
-                         light_faces=light, light_n=4)
-    assert e.value.status == nw.ERR_UNSUPPORTED
-    ctx.mesh_destroy(mesh)
-    s = _direct_scene(0, kinds=(2, 2, 2))
+    for col, bad in ((18, 0.0), (18, -1.5), (9, np.nan)):  # eta <= 0, a NaN parameter
+        mats = s["materials"].copy()
+        mats[0, col] = bad
+        with pytest.raises(nw.NwprtError) as e:
+            ctx.mesh_create(s["vertices"], s["indices"], mats, normals=s["normals"],
+                            material_ids=s["material_ids"])
+        assert e.value.status == nw.ERR_INVALID_ARGUMENT
     mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
                            material_ids=s["material_ids"])
     for kw in ({"light_n": 0}, {"light_n": 200}, {"light_n": 4, "direct_samples": 0}):
@@ -381,3 +400,15 @@ def test_full_size_properties(nw, ctx):
     img = out.download(np.float32, (P, 4))
     assert np.all(np.isfinite(img)) and np.all(img[:, :3] >= 0) and img[flag, :3].mean() > 0
     ctx.mesh_destroy(mesh)
+
+
+@pytest.mark.parametrize("seed,bounce", [(8, 1), (9, 2)])
+def test_cast_disney_first_hit_encodings(nw, orc, ctx, seed, bounce):
+    """DisneyBSDF materials at the first hit: kd is the base colour, ks the
+    packed specular colour (disneyBSDF.cu:10-25), as the oracle computes it."""
+    s = _direct_scene(seed, kinds=(1, 1, 1))
+    faces = np.random.default_rng(seed).lognormal(0, 1, (6, 8, 8, 3)).astype(np.float32)
+    got, _ = _cast(nw, ctx, s, 48, 40, bounce, faces)
+    want = _oracle(orc, s, 48, 40, bounce, faces)
+    n_hit, _ = _compare(got, want, TOL_ENC if bounce == 1 else 1e-3)
+    assert n_hit > 100

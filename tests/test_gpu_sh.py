@@ -22,11 +22,20 @@ def _scene(seed, sphere=(48, 24), soup=40):
 @pytest.mark.parametrize("seed,w,h,degree,samples,kinds", [(0, 32, 24, 2, 32, None),
                                                            (1, 29, 17, 0, 16, None),
                                                            (2, 24, 16, 8, 8, None),
-                                                           (3, 16, 12, 4, 16, (4, 2, 0))])
+                                                           (3, 16, 12, 4, 16, (4, 2, 0)),
+                                                           (4, 24, 16, 2, 16, (1, 1, 2))])
 def test_sh_render_replays_oracle(nw, orc, ctx, seed, w, h, degree, samples, kinds):
     s = _scene(seed, sphere=(16, 8) if degree == 8 else (48, 24), soup=10 if degree == 8 else 40)
     if kinds is not None:
-        s["materials"][:3, 7] = kinds
+        # DisneyBSDF rows take a plastic-with-sheen and a tinted-metal set
+        disney = [(0.0, 0.0, 0.4, 0.5, 0.2, 0.0, 0.7, 0.5, 0.6, 0.3, 1.5),
+                  (0.0, 0.9, 0.0, 0.5, 0.5, 0.6, 0.0, 0.5, 0.0, 1.0, 1.45)]
+        mats = np.array([nw.material_row(m) for m in s["materials"]], np.float32)
+        mats[:3, 7] = kinds
+        for i, k in enumerate(kinds):
+            if k == 1:
+                mats[i, 8:] = disney[i % 2]
+        s["materials"] = mats
     rng = np.random.default_rng(seed)
     nc = (degree + 1) ** 2
     nv = len(s["vertices"])
