@@ -12,9 +12,9 @@
 //
 // The filter, for an image of P = W x H pixels:
 // - HDR normalisation: kappa = 2^-m with m the mean of log2(Y) over pixels
-//   whose luminance Y = 0.2126 R + 0.7152 G + 0.0722 B (FP64) is > 1e-8
-//   (kappa = 1 if none), so colour distances are relative to the image's
-//   log-average luminance;
+//   whose luminance Y = 0.2126 R + 0.7152 G + 0.0722 B is > 1e-8 (kappa = 1
+//   if none), so colour distances are relative to the image's log-average
+//   luminance;
 // - passes i = 0 .. L-1 with step s = 2^i over the 5 x 5 B3-spline taps
 //   h(dx) h(dy), h = (1, 4, 6, 4, 1) / 16, skipping taps outside the image:
 //     w(p, q) = h exp(-(kappa^2 |c_q - c_p|^2 4^i / sc^2
@@ -23,7 +23,10 @@
 //   the colour sigma halving each pass as in the paper;
 // - output RGB = blend * input + (1 - blend) * c after pass L-1, alpha =
 //   the input's alpha.
-// Passes run in FP32 (one exp2 per tap); the normalisation runs in FP64.
+// Everything runs in FP32: one exp2 per tap in the passes, and the
+// normalisation's sum of log2(Y) is compensated (each partial carries the
+// rounding error of its additions, Knuth's TwoSum), with exact integer
+// counts, reduced in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -35,49 +38,73 @@ constexpr int kTile = 16;            // 16 x 16 pixels per workgroup
 constexpr int kIntensityBlocks = 512;
 constexpr int kIntensityThreads = 256;
 
+// A compensated FP32 sum: hi + lo carries the exact sum of what was added up
+// to the rounding of lo, which only collects the TwoSum errors.
+struct CSum {
+  float hi, lo;
+};
+
+__device__ __forceinline__ CSum csum_add(CSum a, float b) {
+  const float s = a.hi + b, bb = s - a.hi;
+  const float e = (a.hi - (s - bb)) + (b - bb);
+  return {s, a.lo + e};
+}
+
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+  CSum r = csum_add(a, b.hi);
+  r.lo += b.lo;
+  return r;
+}
+
 // Per-block partial sums of log2(Y) and counts over lit pixels.
 __global__ __launch_bounds__(kIntensityThreads) void intensity_partials_kernel(
-    const float4 *__restrict__ color, long long n, double2 *__restrict__ partials) {
-  double s = 0.0, c = 0.0;
+    const float4 *__restrict__ color, long long n, float4 *__restrict__ partials) {
+  CSum s = {0.0f, 0.0f};
+  unsigned c = 0;
   for (long long p = (long long)blockIdx.x * kIntensityThreads + threadIdx.x; p < n;
        p += (long long)kIntensityBlocks * kIntensityThreads) {
     const float4 v = color[p];
-    const double y = 0.2126 * (double)v.x + 0.7152 * (double)v.y + 0.0722 * (double)v.z;
-    if (y > 1e-8) {
-      s += log2(y);
-      c += 1.0;
+    const float y = fmaf(0.0722f, v.z, fmaf(0.7152f, v.y, 0.2126f * v.x));
+    if (y > 1e-8f) {
+      s = csum_add(s, log2f(y));
+      ++c;
     }
   }
-  __shared__ double ss[kIntensityThreads], sc[kIntensityThreads];
+  __shared__ CSum ss[kIntensityThreads];
+  __shared__ unsigned sc[kIntensityThreads];
   ss[threadIdx.x] = s;
   sc[threadIdx.x] = c;
   __syncthreads();
   for (int h = kIntensityThreads / 2; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h) {
-      ss[threadIdx.x] += ss[threadIdx.x + h];
+      ss[threadIdx.x] = csum_merge(ss[threadIdx.x], ss[threadIdx.x + h]);
       sc[threadIdx.x] += sc[threadIdx.x + h];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = make_double2(ss[0], sc[0]);
+  if (threadIdx.x == 0)
+    partials[blockIdx.x] = make_float4(ss[0].hi, ss[0].lo, __uint_as_float(sc[0]), 0.0f);
 }
 
-// kappa = 2^-(sum / count) from the partials, in a fixed order.
+// kappa = 2^-(sum / count) from the partials, in a fixed order.  Counts stay
+// below 2^32 (images are limited to 2^31 pixels).
 __global__ __launch_bounds__(kIntensityBlocks) void intensity_finish_kernel(
-    const double2 *__restrict__ partials, float *__restrict__ kappa) {
-  __shared__ double ss[kIntensityBlocks], sc[kIntensityBlocks];
-  const double2 v = partials[threadIdx.x];
-  ss[threadIdx.x] = v.x;
-  sc[threadIdx.x] = v.y;
+    const float4 *__restrict__ partials, float *__restrict__ kappa) {
+  __shared__ CSum ss[kIntensityBlocks];
+  __shared__ unsigned sc[kIntensityBlocks];
+  const float4 v = partials[threadIdx.x];
+  ss[threadIdx.x] = {v.x, v.y};
+  sc[threadIdx.x] = __float_as_uint(v.z);
   __syncthreads();
   for (int h = kIntensityBlocks / 2; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h) {
-      ss[threadIdx.x] += ss[threadIdx.x + h];
+      ss[threadIdx.x] = csum_merge(ss[threadIdx.x], ss[threadIdx.x + h]);
       sc[threadIdx.x] += sc[threadIdx.x + h];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) kappa[0] = sc[0] > 0.0 ? (float)exp2(-ss[0] / sc[0]) : 1.0f;
+  if (threadIdx.x == 0)
+    kappa[0] = sc[0] > 0 ? exp2f(-((ss[0].hi + ss[0].lo) / (float)sc[0])) : 1.0f;
 }
 
 __device__ __forceinline__ float sq3(float4 a, float4 b) {
@@ -171,7 +198,7 @@ __global__ __launch_bounds__(kTile * kTile) void atrous_kernel(
 }  // namespace
 
 size_t denoise_scratch_bytes(int w, int h) {
-  return 2 * (size_t)w * h * sizeof(float4) + kIntensityBlocks * sizeof(double2) + 256;
+  return 2 * (size_t)w * h * sizeof(float4) + kIntensityBlocks * sizeof(float4) + 256;
 }
 
 void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s) {
@@ -181,13 +208,13 @@ void launch_denoise(const DenoiseArgs &a, void *scratch, hipStream_t s) {
           NWPRT_ERR_INVALID_ARGUMENT, "denoise: bad size or iteration count");
   const size_t P = (size_t)a.width * a.height;
   float4 *t0 = (float4 *)scratch, *t1 = t0 + P;
-  double2 *partials = (double2 *)(t1 + P);
+  float4 *partials = (float4 *)(t1 + P);
   float *kappa = (float *)(partials + kIntensityBlocks);
   hipLaunchKernelGGL(intensity_partials_kernel, dim3(kIntensityBlocks), dim3(kIntensityThreads),
                      0, s, (const float4 *)a.color, (long long)P, partials);
   hip_check(hipGetLastError(), "intensity_partials_kernel launch");
   hipLaunchKernelGGL(intensity_finish_kernel, dim3(1), dim3(kIntensityBlocks), 0, s,
-                     (const double2 *)partials, kappa);
+                     (const float4 *)partials, kappa);
   hip_check(hipGetLastError(), "intensity_finish_kernel launch");
   const dim3 block(kTile, kTile);
   const float4 *orig = (const float4 *)a.color;

@@ -1,8 +1,8 @@
 // Wave-level pieces of the BF16 / FP16 fused relight (v_mfma_f32_16x16x32):
 // the MFMA wrappers, ReLU + pack, one wavelet's four layers (plain and
 // software-pipelined), the fused encode's operand read and the per-wave
-// relight.  Shared by the BF16 / FP16 kernels and by the FP8 kernels, whose
-// waves fall back to this form when FP8 overflows.
+// relight.  Shared by the BF16 / FP16 one-call, batch and allocation-free
+// kernels and by the FP8 form's calibration pass.
 //
 // MFMA data layout (v_mfma_f32_16x16x32_{bf16,f16}; lane l, g = l/16,
 // n = l%16): the A operand holds row n, k-slots 8g..8g+7; B holds column n,

@@ -3,7 +3,7 @@ OptixBackend::denoise, Renderer/OptixBackend.cpp:841-906) against the FP64
 oracle (orc_denoise).
 
 Tolerance: the passes run in FP32 with one hardware exp2 per tap, the HDR
-normalisation in FP64.  Per pixel and channel,
+normalisation as a compensated FP32 sum of hardware log2 values.  Per pixel and channel,
 |gpu - oracle| <= 1e-4 * (|oracle| + 1e-3 * max|oracle|)."""
 import numpy as np
 import pytest
