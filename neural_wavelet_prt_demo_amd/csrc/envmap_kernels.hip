@@ -14,6 +14,12 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 
+// No floating-point contraction in this file: the Haar scaling sums, the
+// top-K scores and the rotation's direction arithmetic are each defined as
+// separately rounded operations, the way the oracle evaluates them, so that
+// index sets and texel choices agree bit for bit.
+#pragma clang fp contract(off)
+
 namespace nwprt {
 
 void hip_check(hipError_t e, const char *what) {
@@ -159,8 +165,7 @@ constexpr int kTopkBlock = 1024;
 // n is a power of two (face sizes are), so the position math is shifts.
 __device__ __forceinline__ uint32_t score_bits(const float *__restrict__ coef, int t, int n) {
   const float r = coef[3 * (size_t)t], g = coef[3 * (size_t)t + 1], b = coef[3 * (size_t)t + 2];
-  const float lum = __fadd_rn(__fadd_rn(__fmul_rn(r, 0.212671f), __fmul_rn(g, 0.715160f)),
-                              __fmul_rn(b, 0.072169f));
+  const float lum = r * 0.212671f + g * 0.715160f + b * 0.072169f;
   const int log2n = __ffs(n) - 1;
   const int q = t & (n * n - 1);
   const int row = q >> log2n, col = q & (n - 1);
@@ -422,10 +427,11 @@ __global__ __launch_bounds__(256) void prepare_selection_kernel(
 // R rotates by the yaw angle (x' = cos x + sin z, z' = -sin x + cos z).
 // Face and (u, v) conventions of EnvUVToDirec / EnvDirecToUV
 // (Renderer/device_functions/cubeMap.cu:23-115); row 0 is v = 0.  The
-// direction arithmetic is FP32 with every operation rounded explicitly (no
-// contraction), so the texel choice is reproducible bit for bit on the host.
+// direction arithmetic is FP32 with every operation rounded on its own
+// (contraction is off in this file), so the texel choice is reproducible bit
+// for bit on the host.
 __device__ __forceinline__ void uv_to_dir(float u, float v, int f, float &x, float &y, float &z) {
-  const float uc = __fsub_rn(__fmul_rn(2.0f, u), 1.0f), vc = __fsub_rn(__fmul_rn(2.0f, v), 1.0f);
+  const float uc = 2.0f * u - 1.0f, vc = 2.0f * v - 1.0f;
   switch (f) {
     case 0: x = 1.0f; y = vc; z = uc; break;
     case 1: x = -1.0f; y = vc; z = -uc; break;
@@ -447,8 +453,8 @@ __device__ __forceinline__ int dir_to_uv(float x, float y, float z, float &u, fl
   if (!(y > 0) && ay >= ax && ay >= az) { m = ay; uc = x; vc = -z; f = 3; }
   if (!(z > 0) && az >= ax && az >= ay) { m = az; uc = x; vc = y; f = 4; }
   if (z > 0 && az >= ax && az >= ay) { m = az; uc = -x; vc = y; f = 5; }
-  u = __fmul_rn(0.5f, __fadd_rn(__fdiv_rn(uc, m), 1.0f));
-  v = __fmul_rn(0.5f, __fadd_rn(__fdiv_rn(vc, m), 1.0f));
+  u = 0.5f * (uc / m + 1.0f);
+  v = 0.5f * (vc / m + 1.0f);
   return f;
 }
 
@@ -459,14 +465,13 @@ __global__ __launch_bounds__(256) void rotate_cubemap_kernel(const float *__rest
   if (t >= 6 * n * n) return;
   const int f = t / (n * n), r = (t / n) % n, c = t % n;
   float x, y, z;
-  uv_to_dir(__fdiv_rn((float)c + 0.5f, (float)n), __fdiv_rn((float)r + 0.5f, (float)n), f, x, y,
-            z);
-  const float xs = __fsub_rn(__fmul_rn(cs, x), __fmul_rn(sn, z));
-  const float zs = __fadd_rn(__fmul_rn(sn, x), __fmul_rn(cs, z));
+  uv_to_dir(((float)c + 0.5f) / (float)n, ((float)r + 0.5f) / (float)n, f, x, y, z);
+  const float xs = cs * x - sn * z;
+  const float zs = sn * x + cs * z;
   float u, v;
   const int fs = dir_to_uv(xs, y, zs, u, v);
-  const int cs_ = min(n - 1, max(0, (int)floorf(__fmul_rn(u, (float)n))));
-  const int rs = min(n - 1, max(0, (int)floorf(__fmul_rn(v, (float)n))));
+  const int cs_ = min(n - 1, max(0, (int)floorf(u * (float)n)));
+  const int rs = min(n - 1, max(0, (int)floorf(v * (float)n)));
   const float *p = src + (((size_t)fs * n + rs) * n + cs_) * 3;
   float *q = dst + (size_t)t * 3;
   q[0] = p[0];

@@ -223,9 +223,11 @@ __device__ inline float rng_uniform(uint32_t seed, uint32_t pixel, uint32_t coun
   return (float)(uint32_t)(z >> 40) * 5.9604644775390625e-8f;  // 2^-24
 }
 
+// Separately rounded products and sums (no contraction), as the oracle
+// evaluates the reference's FP32 luminance.
 __device__ inline float luminance(float3 c) {
-  return __fadd_rn(__fadd_rn(__fmul_rn(c.x, 0.212671f), __fmul_rn(c.y, 0.715160f)),
-                   __fmul_rn(c.z, 0.072169f));
+#pragma clang fp contract(off)
+  return c.x * 0.212671f + c.y * 0.715160f + c.z * 0.072169f;
 }
 
 // Orthonormal frame around a unit normal (Frisvad 2012), as the reference's
@@ -740,6 +742,7 @@ __device__ inline int first_above(const double *cdf, int count, float r) {
 
 // __direct_callable__sampleEnvironmapLight (cubeMap.cu:166-174).
 __device__ float3 env_sample(const double *tab, int n, float r0, float r1, float r2) {
+#pragma clang fp contract(off)
   const int f = min(max(first_above(tab + 6, 6, r0) - 1, 0), 5);
   const double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
   const double *cdf_rows = base, *cdf_m = base + (size_t)n * (n + 1) + (size_t)n * n;
@@ -757,6 +760,7 @@ __device__ float3 env_sample(const double *tab, int n, float r0, float r1, float
 // (cubeMap.cu:9-19), in the same FP32 operations: a density per unit face
 // (u, v) area times the face probability.
 __device__ float env_pdf(const double *tab, int n, float3 d) {
+#pragma clang fp contract(off)
   float u, v;
   const int f = dir_to_face(d, u, v);
   const double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
@@ -765,8 +769,8 @@ __device__ float env_pdf(const double *tab, int n, float3 d) {
   const int x = (int)fminf(fmaxf(u * (float)n, 0.0f), (float)(n - 1));
   const int y = (int)fminf(fmaxf(v * (float)n, 0.0f), (float)(n - 1));
   const float py = (float)pdf_m[y], px = (float)pdf_rows[(size_t)y * n + x];
-  const float p = __fmul_rn(__fmul_rn(__fmul_rn(py, px), (float)n), (float)n);
-  return __fmul_rn((float)tab[f], p);
+  const float p = py * px * (float)n * (float)n;
+  return (float)tab[f] * p;
 }
 
 __device__ inline bool finite3(float3 c) {
@@ -834,6 +838,7 @@ __device__ float3 direct_light(const CastArgs &a, const Surface &s, float3 p, fl
 // luminance sum are parked in the face's marginal arrays for pass 2.
 __global__ __launch_bounds__(128) void env_rows_kernel(const float *__restrict__ faces, int n,
                                                       double *__restrict__ tab) {
+#pragma clang fp contract(off)
   const int f = blockIdx.x / n, r = blockIdx.x % n, c = threadIdx.x;
   double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
   double *cr = base + (size_t)r * (n + 1);
@@ -848,12 +853,11 @@ __global__ __launch_bounds__(128) void env_rows_kernel(const float *__restrict__
     const double sa = area(U - inv, V - inv) - area(U - inv, V + inv) - area(U + inv, V - inv) +
                       area(U + inv, V + inv);
     const float saf = (float)sa;
-    const float kScale = __fdiv_rn(__fmul_rn(2.0f, 3.14159265358979323846f), 3.0f);
+    const float kScale = 2.0f * 3.14159265358979323846f / 3.0f;
     const float *px = faces + (((size_t)f * n + r) * n + c) * 3;
     const float3 t = v3(px[0], px[1], px[2]);
-    const float3 q = v3(__fdiv_rn(__fmul_rn(fabsf(t.x), saf), kScale),
-                        __fdiv_rn(__fmul_rn(fabsf(t.y), saf), kScale),
-                        __fdiv_rn(__fmul_rn(fabsf(t.z), saf), kScale));
+    const float3 q = v3(fabsf(t.x) * saf / kScale, fabsf(t.y) * saf / kScale,
+                        fabsf(t.z) * saf / kScale);
     fv[c] = (double)luminance(q);
     lv[c] = (double)luminance(t);
   }
@@ -885,6 +889,7 @@ __global__ __launch_bounds__(128) void env_rows_kernel(const float *__restrict__
 // Pass 2: one workgroup per face: marginals in row order, the face's
 // luminance, and each row's last CDF entry set to 1 (0 for a black row).
 __global__ __launch_bounds__(128) void env_marginals_kernel(int n, double *__restrict__ tab) {
+#pragma clang fp contract(off)
   const int f = blockIdx.x, r = threadIdx.x;
   double *base = tab + kTabHead + (size_t)f * env_face_doubles(n);
   double *cdf_m = base + (size_t)n * (n + 1) + (size_t)n * n, *pdf_m = cdf_m + (n + 1);
@@ -924,6 +929,7 @@ __global__ __launch_bounds__(128) void env_marginals_kernel(int n, double *__res
 // Face distribution (EnvironmentMap.cpp:180-189, make_table_dist_1d,
 // table_dist.cpp:4-27); an all-black map gets the uniform distribution.
 __global__ void env_face_table_kernel(double *__restrict__ tab) {
+#pragma clang fp contract(off)
   if (threadIdx.x != 0) return;
   double total = 0.0, f[6];
   for (int i = 0; i < 6; ++i) total += tab[13 + i];
@@ -1625,7 +1631,7 @@ __global__ __launch_bounds__(256) void tm_project_kernel(TmArgs a) {
         if (isnan(row[3 * i]) || isnan(row[3 * i + 1]) || isnan(row[3 * i + 2])) {
 #pragma unroll
           for (int k = 0; k < kShCoeffs; ++k)
-            if (k < nc) acc[k] += __fdiv_rn(acc[k], fn);
+            if (k < nc) acc[k] += acc[k] / fn;
         } else {
 #pragma unroll
           for (int k = 0; k < kShCoeffs; ++k)

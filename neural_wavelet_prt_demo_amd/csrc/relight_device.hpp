@@ -62,6 +62,13 @@ struct LightTerm {
   static constexpr float kSumScale = SOFTPLUS ? 0.693147180559945309f : 1.0f;
   // Per wavelet and channel: the light as the term uses it.
   __device__ static __forceinline__ float light(float l) { return SOFTPLUS ? relu_f32(l) : l; }
+  // SOFTPLUS: the transport divided by kSumScale, the factor the light
+  // multiplies in term().
+  __device__ static __forceinline__ float factor(float o) {
+    const float y = o * 1.44269504088896341f;
+    const float sp = __builtin_amdgcn_logf(1.0f + __builtin_amdgcn_exp2f(y));
+    return o > 15.0f ? y : sp;
+  }
   // Per pair: the term divided by kSumScale.
   __device__ static __forceinline__ float term(float light, float o) {
     if constexpr (SOFTPLUS) {
@@ -76,8 +83,11 @@ struct LightTerm {
 
 // Kahan-compensated accumulation of non-negative terms: s - c is the
 // running sum (c holds the low-order part lost by the last addition, with
-// its sign flipped).
+// its sign flipped).  Contraction is off: a product that produced x is
+// never fused into x - c, whatever the caller's code shape, so kernel forms
+// that order their work differently sum identically.
 __device__ __forceinline__ void kahan_add(float &s, float &c, float x) {
+#pragma clang fp contract(off)
   const float y = x - c;
   const float t = s + y;
   c = (t - s) - y;

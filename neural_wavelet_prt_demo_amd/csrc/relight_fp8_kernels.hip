@@ -135,7 +135,22 @@ __device__ __forceinline__ u32x4 scale_bf16x8(u32x4 v, float s) {
 // pixel's aux inputs, is pre-multiplied by 2^ea2, so it adds in the scaled
 // domain of that result.  FUSE: the fused-encode path is compiled in (taken
 // when a.fused_encode).
-template <bool SOFTPLUS, bool DUMP, bool AUX, bool FUSE, bool PIPE>
+// term(k) + term(k + 1) of one channel, the second product fused into the
+// addition (one rounding), as every FP8 variant forms the pair.
+template <bool SOFTPLUS>
+__device__ __forceinline__ float pair_sum(float first, float light, float o) {
+  using LT = dev::LightTerm<SOFTPLUS>;
+  if constexpr (SOFTPLUS) return fmaf(light, LT::factor(o), first);
+  else return first + LT::term(light, o);
+}
+
+// Wave forms (MODE): 0 one wavelet's four layers for both pixel groups at
+// a time, two wavelets per iteration; 1 the same with the two groups
+// software-pipelined; 2 ("seq", for three waves per SIMD) one wavelet per
+// iteration, the groups taken through layers 2-4 one after the other, and
+// the next wavelet's first-layer tiles loaded as soon as layer 1 is done.
+// Every form computes the same per-lane values in the same order.
+template <bool SOFTPLUS, bool DUMP, bool AUX, bool FUSE, int MODE>
 __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base, int lane,
                                                  const i32x8 *w4s, uint32_t *feat_lds) {
   const int h = lane >> 5, col = lane & 31;
@@ -242,7 +257,7 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
       return o;
     };
     auto step = [&](const u32x4(&t)[4]) -> f32x16 {
-      if constexpr (PIPE) return step_pipe(t);
+      if constexpr (MODE == 1) return step_pipe(t);
       const f32x16 z = {};
       f32x16 d[2][2];
       i32x8 x[2];
@@ -274,7 +289,53 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
     // Wavelets two at a time: the clamped terms of k and k + 1 are added and
     // the pair sum enters the Kahan-compensated running sum.
     using LT = dev::LightTerm<SOFTPLUS>;
-    for (int k = 0; k < a.k; k += 2) {
+    if constexpr (MODE == 2) {
+      float first[3] = {0.f, 0.f, 0.f};
+      for (int k = 0; k < a.k; ++k) {
+        const f32x16 z = {};
+        i32x8 x[2];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          const f32x16 d0 = mfma_bf16(ta[1], f[g][1], mfma_bf16(ta[0], f[g][0], z));
+          const f32x16 d1 = mfma_bf16(ta[3], f[g][1], mfma_bf16(ta[2], f[g][0], z));
+          x[g] = relu_fp8(d0, d1);
+        }
+        if (k + 1 < a.k) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) ta[q] = mk[((size_t)(k + 1) * 4 + q) * 64 + lane];
+        }
+        f32x16 o = z;
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          f32x16 d0 = mfma_fp8(w2[0], x[g], z, sa2, sb2);
+          if (AUX) d0 = mfma_bf16(waux[0], auxb[g], d0);
+          f32x16 d1 = mfma_fp8(w2[1], x[g], z, sa2, sb2);
+          if (AUX) d1 = mfma_bf16(waux[1], auxb[g], d1);
+          i32x8 y = relu_fp8(d0, d1);
+          d0 = mfma_fp8(w3[0], y, z, sa3, sb3);
+          d1 = mfma_fp8(w3[1], y, z, sa3, sb3);
+          y = relu_fp8(d0, d1);
+          o = mfma_fp8(w4s[64 * g + lane], y, o, sa4, sb4);
+        }
+        const bool odd = (k & 1) != 0;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+          const float l = LT::light(a.light[3 * k + ch]);
+          if (odd) {
+            dev::kahan_add(sum[ch], comp[ch], pair_sum<SOFTPLUS>(first[ch], l, o[ch]));
+          } else {
+            first[ch] = LT::term(l, o[ch]);
+            if (k + 1 == a.k) dev::kahan_add(sum[ch], comp[ch], first[ch]);
+          }
+          if (DUMP) {
+            const int q = my_px - a.dump_begin;
+            if (mine && q >= 0 && q < a.dump_count)
+              a.dump[((size_t)q * a.k + k) * 3 + ch] = dev::transport<SOFTPLUS>(o[ch]);
+          }
+        }
+      }
+    }
+    for (int k = 0; MODE != 2 && k < a.k; k += 2) {
       const bool two = k + 1 < a.k;
       if (two) {
 #pragma unroll
@@ -295,7 +356,7 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
         float x = LT::term(LT::light(a.light[3 * k + ch]), o0[ch]);
-        if (two) x += LT::term(LT::light(a.light[3 * (k + 1) + ch]), o1[ch]);
+        if (two) x = pair_sum<SOFTPLUS>(x, LT::light(a.light[3 * (k + 1) + ch]), o1[ch]);
         dev::kahan_add(sum[ch], comp[ch], x);
         if (DUMP) {
           const int q = my_px - a.dump_begin;
@@ -325,15 +386,15 @@ __device__ __forceinline__ void load_w4(const RelightArgs &a, i32x8 *w4s, int la
 // Grid: ceil(P / 256) workgroups of 4 waves; wave w of the workgroup with
 // tile t = xcd_tile(b) owns pixels [64 (4 t + w), +64) (one contiguous band
 // per XCD).
-template <bool SOFTPLUS, bool DUMP, bool AUX, bool PIPE>
-__global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
+template <bool SOFTPLUS, bool DUMP, bool AUX, int MODE>
+__global__ __launch_bounds__(kBlock, MODE == 2 ? 3 : 2) void relight_fp8_kernel(RelightArgs a) {
   __shared__ i32x8 w4s[4][2 * 64];
   __shared__ uint32_t feat_lds[AUX ? 1 : 4 * 64 * 16];  // fused encode: [wave][pixel][16]
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int base = (dev::xcd_tile(blockIdx.x, gridDim.x) * 4 + wave) * 64;
   if (base >= a.n_pixels) return;
   load_w4(a, w4s[wave], lane);
-  relight_fp8_wave<SOFTPLUS, DUMP, AUX, !AUX, PIPE>(a, base, lane, w4s[wave],
+  relight_fp8_wave<SOFTPLUS, DUMP, AUX, !AUX, MODE>(a, base, lane, w4s[wave],
                                                     feat_lds + (AUX ? 0 : wave * 64 * 16));
 }
 
@@ -342,8 +403,8 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_kernel(RelightArgs a) {
 // item reloads its job's output-layer tiles into the wave's own LDS copy.
 // HAS_AUX: jobs with aux inputs take the aux path (their features come from
 // the standalone encode); the others use the fused encode.
-template <bool SOFTPLUS, bool HAS_AUX, bool PIPE>
-__global__ __launch_bounds__(kBlock, 2) void relight_fp8_batch_kernel(
+template <bool SOFTPLUS, bool HAS_AUX, int MODE>
+__global__ __launch_bounds__(kBlock, MODE == 2 ? 3 : 2) void relight_fp8_batch_kernel(
     const RelightArgs *__restrict__ jobs, const BatchItem *__restrict__ items,
     const int *__restrict__ n_items_p, unsigned int *__restrict__ counter) {
   __shared__ i32x8 w4s[4][2 * 64];
@@ -365,16 +426,16 @@ __global__ __launch_bounds__(kBlock, 2) void relight_fp8_batch_kernel(
     const RelightArgs a = jobs[item.job];
     load_w4(a, w4s[wave], lane);
     if (HAS_AUX && a.aux)
-      relight_fp8_wave<SOFTPLUS, false, true, false, PIPE>(a, item.base, lane, w4s[wave], my_lds);
+      relight_fp8_wave<SOFTPLUS, false, true, false, MODE>(a, item.base, lane, w4s[wave], my_lds);
     else
-      relight_fp8_wave<SOFTPLUS, false, false, true, PIPE>(a, item.base, lane, w4s[wave], my_lds);
+      relight_fp8_wave<SOFTPLUS, false, false, true, MODE>(a, item.base, lane, w4s[wave], my_lds);
   }
 }
 
 struct Fp8Variant {
   const char *name;
 };
-constexpr Fp8Variant kFp8Variants[] = {{"fp8.l1bf16.occ2"}, {"fp8.pipe.occ2"}};
+constexpr Fp8Variant kFp8Variants[] = {{"fp8.l1bf16.occ2"}, {"fp8.pipe.occ2"}, {"fp8.seq.occ3"}};
 constexpr int kNumFp8Variants = sizeof(kFp8Variants) / sizeof(kFp8Variants[0]);
 
 // Exponent e with amax * 2^e in (224, 448], 0 for a zero or non-finite amax.
@@ -463,9 +524,10 @@ void launch_relight_fp8(const RelightArgs &a, int variant, hipStream_t s) {
   const dim3 grid((a.n_pixels + kBlock - 1) / kBlock), block(kBlock);
   const bool aux = a.aux != nullptr, sp = a.softplus != 0, dump = a.dump != nullptr;
   auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, s, a); };
-#define NWPRT_FP8_PICK(SP, DUMP, AUX)                               \
-  if (variant == 1) go(relight_fp8_kernel<SP, DUMP, AUX, true>);    \
-  else go(relight_fp8_kernel<SP, DUMP, AUX, false>);
+#define NWPRT_FP8_PICK(SP, DUMP, AUX)                            \
+  if (variant == 1) go(relight_fp8_kernel<SP, DUMP, AUX, 1>);    \
+  else if (variant == 2) go(relight_fp8_kernel<SP, DUMP, AUX, 2>); \
+  else go(relight_fp8_kernel<SP, DUMP, AUX, 0>);
   if (sp) {
     if (aux) {
       if (dump) { NWPRT_FP8_PICK(true, true, true) } else { NWPRT_FP8_PICK(true, false, true) }
@@ -490,13 +552,16 @@ void launch_relight_fp8_batch(const RelightArgs *jobs, const BatchItem *items, c
           "batch: missing operand");
   require(variant >= 0 && variant < kNumFp8Variants, NWPRT_ERR_INVALID_ARGUMENT,
           "batch: unknown FP8 kernel variant");
-  const dim3 grid(n_cu * 2), block(kBlock);
+  // A machine-sized grid: as many 4-wave workgroups per CU as the form
+  // keeps resident (two, three for the seq form).
+  const dim3 grid(n_cu * (variant == 2 ? 3 : 2)), block(kBlock);
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, grid, block, 0, s, jobs, items, n_items, counter);
   };
-#define NWPRT_FP8B_PICK(SP, AUX)                                 \
-  if (variant == 1) go(relight_fp8_batch_kernel<SP, AUX, true>); \
-  else go(relight_fp8_batch_kernel<SP, AUX, false>);
+#define NWPRT_FP8B_PICK(SP, AUX)                               \
+  if (variant == 1) go(relight_fp8_batch_kernel<SP, AUX, 1>);  \
+  else if (variant == 2) go(relight_fp8_batch_kernel<SP, AUX, 2>); \
+  else go(relight_fp8_batch_kernel<SP, AUX, 0>);
   if (softplus) {
     if (has_aux) { NWPRT_FP8B_PICK(true, true) } else { NWPRT_FP8B_PICK(true, false) }
   } else {

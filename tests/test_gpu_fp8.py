@@ -113,9 +113,9 @@ def test_fp8_fused_encode_equals_p1_p2(nw, orc, ctx):
 
 
 def test_fp8_variants_agree(nw, orc, ctx):
-    """The plain and the software-pipelined FP8 kernels run the same
-    per-lane arithmetic in the same order: identical outputs, also with aux
-    inputs."""
+    """The FP8 kernel forms (plain, software-pipelined, groups in sequence
+    at three waves per SIMD) run the same per-lane arithmetic in the same
+    order: identical outputs, also with aux inputs and an odd K."""
     for aux in (False, True):
         case = make_case(nw, orc, width=150, height=37, k=45, aux=aux, seed=21)
         scene = ctx.scene_create(case.desc, case.params)
