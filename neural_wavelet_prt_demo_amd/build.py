@@ -20,6 +20,7 @@ OBJ = PKG / "_build"
 LIB = PKG / "libnwprt.so"
 OFFLINE = PKG / "nwprt_offline"
 CEILING = PKG / "mfma_ceiling"
+VALU_RATE = PKG / "valu_rate"
 
 SOURCES = [
     "host_util.cpp",
@@ -101,8 +102,15 @@ def build(verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"mfma_ceiling build failed\n{' '.join(cmd)}\n{r.stderr}")
     os.replace(exe_tmp, CEILING)
+    # The vector-ALU issue-cost probe (standalone; no library dependency).
+    exe_tmp = VALU_RATE.with_suffix(".tmp")
+    cmd = [hipcc(), *HIPCC_FLAGS, str(CSRC / "valu_rate.hip"), "-o", str(exe_tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"valu_rate build failed\n{' '.join(cmd)}\n{r.stderr}")
+    os.replace(exe_tmp, VALU_RATE)
     if verbose:
-        print(f"[build] {LIB}\n[build] {OFFLINE}\n[build] {CEILING}")
+        print(f"[build] {LIB}\n[build] {OFFLINE}\n[build] {CEILING}\n[build] {VALU_RATE}")
     return LIB
 
 
