@@ -470,6 +470,17 @@ typedef struct {
 /* Host utility (no GPU): the reference's default material of a type. */
 nwprt_status nwprt_material_defaults(int32_t type, nwprt_material *out);
 
+/* Host utility (no GPU): an 8-bit material texture as
+ * EightBitTexture::loadTexture reads it (Renderer/textures.cpp:102-127,
+ * through stb_image): RGBA8, row 0 at the bottom of the picture.  PNG (all
+ * colour types and bit depths, palette / tRNS, Adam7), TGA (true-colour or
+ * grey, raw or RLE), BMP (24/32-bit uncompressed), binary PGM / PPM; grey
+ * expands to (g, g, g), missing alpha is 255, 16-bit samples keep their
+ * high byte.  Other formats (JPEG among them) -> NWPRT_ERR_UNSUPPORTED.
+ * *rgba [height][width][4] is released with nwprt_free_host. */
+nwprt_status nwprt_image_read_rgba8(const char *path, uint8_t **rgba, int32_t *width,
+                                    int32_t *height);
+
 #define NWPRT_MATERIAL_MIRROR 3
 
 typedef struct {

@@ -251,6 +251,8 @@ _SIGNATURES = {
                                           c_int32, c_uint32, ctypes.c_int64, ctypes.c_int64,
                                           c_void_p, c_void_p]),
     "nwprt_material_defaults": (c_int, [c_int32, c_void_p]),
+    "nwprt_image_read_rgba8": (c_int, [c_char_p, POINTER(POINTER(ctypes.c_uint8)),
+                                       POINTER(c_int32), POINTER(c_int32)]),
     "nwprt_debug_hash_encode": (c_int, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
                                         c_void_p]),
     "nwprt_debug_transport": (c_int, [c_void_p, c_int32, c_int32, POINTER(GBuffer), c_int32,
@@ -459,6 +461,19 @@ def material_defaults(mat_type: int) -> np.ndarray:
     return np.array(list(m.kd) + list(m.ks) + [m.roughness, m.type, m.specular_transmission,
                     m.metallic, m.subsurface, m.specular, m.specular_tint, m.anisotropic,
                     m.sheen, m.sheen_tint, m.clearcoat, m.clearcoat_gloss, m.eta], np.float64)
+
+
+def image_read_rgba8(path: str) -> np.ndarray:
+    """An 8-bit texture image (nwprt_image_read_rgba8) as uint8
+    [height][width][4], row 0 at the bottom of the picture."""
+    p = POINTER(ctypes.c_uint8)()
+    w, h = c_int32(), c_int32()
+    _check(lib().nwprt_image_read_rgba8(str(path).encode(), ctypes.byref(p), ctypes.byref(w),
+                                        ctypes.byref(h)))
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 4)).copy()
+    finally:
+        lib().nwprt_free_host(p)
 
 
 def cross_to_faces(cross: np.ndarray) -> np.ndarray:

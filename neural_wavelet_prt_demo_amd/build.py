@@ -25,6 +25,7 @@ VALU_RATE = PKG / "valu_rate"
 SOURCES = [
     "host_util.cpp",
     "formats.cpp",
+    "image.cpp",
     "synth.cpp",
     "model_folder.cpp",
     "host_abi.cpp",

@@ -580,6 +580,17 @@ extern "C" nwprt_status nwprt_obj_read(const char *path, float **vertices, int64
   });
 }
 
+extern "C" nwprt_status nwprt_image_read_rgba8(const char *path, uint8_t **rgba, int32_t *width,
+                                               int32_t *height) {
+  return host_guarded([&] {
+    require(path && rgba && width && height, NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    const Image img = read_image_rgba8(path);
+    *rgba = copy_out(img.rgba);
+    *width = img.width;
+    *height = img.height;
+  });
+}
+
 extern "C" nwprt_status nwprt_material_defaults(int32_t type, nwprt_material *out) {
   return host_guarded([&] {
     require(out && type >= 0 && type <= 4, NWPRT_ERR_INVALID_ARGUMENT,

@@ -65,6 +65,13 @@ void hdr_write(const std::string &path, const float *data, int width,
                int height, int channels);
 void cross_to_faces(const float *cross, int width, int height, float *faces);
 
+// An 8-bit RGBA picture with row 0 at the bottom (image.cpp).
+struct Image {
+  int width = 0, height = 0;
+  std::vector<uint8_t> rgba;
+};
+Image read_image_rgba8(const std::string &path);
+
 // Minimal JSON value for config_comp.json.
 struct Json {
   enum Type { Null, Bool, Number, String, Array, Object } type = Null;

@@ -191,6 +191,35 @@ def test_transfer_ranges_determinism_and_errors(nw, ctx):
     ctx.mesh_destroy(mesh)
 
 
+def test_transfer_launch_chunks_equal_single_vertex_calls(nw, ctx):
+    """One call is split into launches of at most 2 GiB of path records:
+    at degree 0, max_bounce 1 and 2^20 samples a vertex needs about 465 MB,
+    so 9 vertices take three launches (4, 4, 1).  They must equal nine
+    one-vertex calls bit for bit (the random stream is keyed by vertex)."""
+    from neural_wavelet_prt_demo_amd import DeviceBuffer
+    v, ix, mids, mats, pts = panel_scene()
+    basis = smooth_basis(1, seed=9)
+    mesh = _mesh(ctx, v, ix, mats, mids)
+    first = int(pts[0])
+    count = 9
+    assert first + count <= len(v)
+    db = DeviceBuffer.from_array(basis)
+    kw = dict(samples=1 << 20, max_bounce=1, seed=12)
+    whole = DeviceBuffer(len(v) * 3 * 4)
+    ctx.sh_compute_transfer(mesh, db, 8, 0, whole, vertex_begin=first, vertex_count=count, **kw)
+    single = DeviceBuffer(len(v) * 3 * 4)
+    for i in range(count):
+        ctx.sh_compute_transfer(mesh, db, 8, 0, single, vertex_begin=first + i, vertex_count=1,
+                                **kw)
+    ctx.sync()
+    a = whole.download(np.float32, (len(v), 3))[first:first + count]
+    b = single.download(np.float32, (len(v), 3))[first:first + count]
+    print(f"chunked transfer rows {a[:, 0]}")
+    assert np.all(np.isfinite(a)) and np.count_nonzero(a) > 0
+    assert np.array_equal(a, b)
+    ctx.mesh_destroy(mesh)
+
+
 def test_sh_render_reads_computed_transfer(nw, orc, ctx):
     """The SH-PRT render on matrices computed here (the reference's
     shComputeTMatrix -> shLoadTMatrix -> shRender chain) equals the oracle's

@@ -1,0 +1,213 @@
+"""CPU tests of nwprt_image_read_rgba8, the reader of 8-bit material
+textures (EightBitTexture::loadTexture, Renderer/textures.cpp:102-127, which
+reads through stb_image into RGBA8 with row 0 at the bottom of the
+picture).
+
+The files are written here from known pixel arrays, PNGs through Python's
+zlib (an independent deflate implementation) with every scanline filter,
+colour type, bit depth, palette / tRNS and Adam7 interlacing, and the
+decoded pixels must equal the arrays after stb_image's RGBA expansion
+(grey g -> g, g, g; alpha 255 when absent; 16-bit samples -> high byte;
+1/2/4-bit grey scaled to 0..255)."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+
+import neural_wavelet_prt_demo_amd as nw
+
+ADAM7 = [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2),
+         (0, 1, 1, 2)]
+
+
+def _chunk(t, data):
+    return struct.pack(">I", len(data)) + t + data + struct.pack(">I", zlib.crc32(t + data))
+
+
+def _paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    return a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+
+
+def _filter_rows(rows, bpp, rng):
+    """rows: list of bytes (unfiltered scanlines); each gets a random filter."""
+    out = bytearray()
+    prev = bytes(len(rows[0])) if rows else b""
+    for row in rows:
+        f = int(rng.integers(5))
+        out.append(f)
+        for i, x in enumerate(row):
+            a = row[i - bpp] if i >= bpp else 0
+            b = prev[i]
+            c = prev[i - bpp] if i >= bpp else 0
+            pred = [0, a, b, (a + b) >> 1, _paeth(a, b, c)][f]
+            out.append((x - pred) & 0xFF)
+        prev = row
+    return bytes(out)
+
+
+def _pack_rows(samples, depth):
+    """samples: [h][w*channels] ints -> list of packed scanlines."""
+    rows = []
+    for r in samples:
+        if depth == 8:
+            rows.append(bytes(int(v) for v in r))
+        elif depth == 16:
+            rows.append(b"".join(struct.pack(">H", int(v)) for v in r))
+        else:
+            per = 8 // depth
+            b = bytearray((len(r) + per - 1) // per)
+            for i, v in enumerate(r):
+                b[i // per] |= int(v) << (8 - depth * (i % per + 1))
+            rows.append(bytes(b))
+    return rows
+
+
+def _png(samples, ctype, depth, rng, plte=None, trns=None, interlace=False, level=6):
+    """samples: [h][w][channels] integer array in the file's sample range."""
+    h, w, ch = samples.shape
+    bits_pp = ch * depth
+    bpp = max(1, bits_pp // 8)
+    if not interlace:
+        raw = _filter_rows(_pack_rows(samples.reshape(h, w * ch), depth), bpp, rng)
+    else:
+        raw = b""
+        for x0, y0, dx, dy in ADAM7:
+            sub = samples[y0::dy, x0::dx]
+            if sub.size == 0:
+                continue
+            raw += _filter_rows(_pack_rows(sub.reshape(sub.shape[0], -1), depth), bpp, rng)
+    ihdr = struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 1 if interlace else 0)
+    out = b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", ihdr)
+    if plte is not None:
+        out += _chunk(b"PLTE", bytes(np.asarray(plte, np.uint8).ravel()))
+    if trns is not None:
+        out += _chunk(b"tRNS", bytes(trns))
+    z = zlib.compress(raw, level)
+    # Split IDAT in two to exercise chunk concatenation.
+    out += _chunk(b"IDAT", z[: len(z) // 2]) + _chunk(b"IDAT", z[len(z) // 2:])
+    return out + _chunk(b"IEND", b"")
+
+
+def _read(tmp_path, name, data):
+    p = tmp_path / name
+    p.write_bytes(data)
+    return nw.image_read_rgba8(str(p))
+
+
+def _bottom_up(rgba):
+    return rgba[::-1]
+
+
+@pytest.mark.parametrize("interlace", [False, True])
+@pytest.mark.parametrize("level", [0, 1, 9])
+def test_png_rgba_and_rgb_8bit(tmp_path, interlace, level):
+    rng = np.random.default_rng(3 + level + interlace)
+    for ctype, ch in ((6, 4), (2, 3)):
+        h, w = 13, 17
+        if level == 9:  # long runs, so the encoder emits back-references
+            px = np.repeat(rng.integers(0, 256, (h, 3, ch)), 6, axis=1)[:, :w]
+        else:
+            px = rng.integers(0, 256, (h, w, ch))
+        got = _read(tmp_path, f"a{ctype}.png", _png(px, ctype, 8, rng, interlace=interlace,
+                                                    level=level))
+        want = np.full((h, w, 4), 255, np.uint8)
+        want[..., :ch] = px
+        assert got.shape == (h, w, 4)
+        assert np.array_equal(got, _bottom_up(want)), (ctype, interlace, level)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4, 8, 16])
+def test_png_grey_depths_and_colour_key(tmp_path, depth):
+    rng = np.random.default_rng(depth)
+    h, w = 9, 23
+    px = rng.integers(0, 2 ** depth, (h, w, 1))
+    key = int(px[2, 3, 0])
+    got = _read(tmp_path, "g.png", _png(px, 0, depth, rng, trns=struct.pack(">H", key),
+                                        interlace=depth == 4))
+    v = px[..., 0]
+    g8 = (v >> 8) if depth == 16 else (v * 255 // (2 ** depth - 1)) if depth < 8 else v
+    want = np.empty((h, w, 4), np.uint8)
+    want[..., 0] = want[..., 1] = want[..., 2] = g8
+    want[..., 3] = np.where(v == key, 0, 255)
+    assert np.array_equal(got, _bottom_up(want))
+
+
+def test_png_grey_alpha_and_16bit_rgba(tmp_path):
+    rng = np.random.default_rng(11)
+    px = rng.integers(0, 256, (7, 5, 2))
+    got = _read(tmp_path, "ga.png", _png(px, 4, 8, rng))
+    want = np.stack([px[..., 0]] * 3 + [px[..., 1]], -1).astype(np.uint8)
+    assert np.array_equal(got, _bottom_up(want))
+    px16 = rng.integers(0, 65536, (6, 11, 4))
+    got = _read(tmp_path, "c16.png", _png(px16, 6, 16, rng, interlace=True))
+    assert np.array_equal(got, _bottom_up((px16 >> 8).astype(np.uint8)))
+
+
+@pytest.mark.parametrize("depth", [1, 2, 4, 8])
+def test_png_palette_with_transparency(tmp_path, depth):
+    rng = np.random.default_rng(40 + depth)
+    n = 2 ** depth
+    plte = rng.integers(0, 256, (n, 3))
+    alpha = rng.integers(0, 256, n // 2 + 1)  # tRNS shorter than the palette
+    idx = rng.integers(0, n, (10, 19, 1))
+    got = _read(tmp_path, "p.png", _png(idx, 3, depth, rng, plte=plte, trns=bytes(alpha.tolist())))
+    want = np.empty((10, 19, 4), np.uint8)
+    want[..., :3] = plte[idx[..., 0]]
+    a = np.full(n, 255)
+    a[: len(alpha)] = alpha
+    want[..., 3] = a[idx[..., 0]]
+    assert np.array_equal(got, _bottom_up(want))
+
+
+def test_tga_bmp_pnm(tmp_path):
+    rng = np.random.default_rng(5)
+    h, w = 6, 7
+    rgba = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+    # TGA 32-bit, raw, bottom-up (origin bit clear): file rows are bottom first.
+    hdr = struct.pack("<BBBHHBHHHHBB", 0, 0, 2, 0, 0, 0, 0, 0, w, h, 32, 8)
+    body = rgba[::-1][..., [2, 1, 0, 3]].tobytes()
+    assert np.array_equal(_read(tmp_path, "a.tga", hdr + body), _bottom_up(rgba))
+    # TGA 24-bit RLE, top-down (origin bit set): one run packet per row.
+    row_px = rgba[:, :1, :3].repeat(w, axis=1)
+    hdr = struct.pack("<BBBHHBHHHHBB", 0, 0, 10, 0, 0, 0, 0, 0, w, h, 24, 0x20)
+    body = b"".join(bytes([0x80 | (w - 1)]) + bytes(row_px[y, 0][[2, 1, 0]].tolist())
+                    for y in range(h))
+    want = np.concatenate([row_px, np.full((h, w, 1), 255, np.uint8)], -1)
+    assert np.array_equal(_read(tmp_path, "b.tga", hdr + body), _bottom_up(want))
+    # BMP 24-bit, positive height (stored bottom-up), rows padded to 4 bytes.
+    stride = (w * 3 + 3) & ~3
+    rows = b"".join(rgba[::-1][y][:, [2, 1, 0]].tobytes() + bytes(stride - 3 * w)
+                    for y in range(h))
+    bmp = (b"BM" + struct.pack("<IHHI", 54 + len(rows), 0, 0, 54) +
+           struct.pack("<IiiHHIIiiII", 40, w, h, 1, 24, 0, len(rows), 0, 0, 0, 0) + rows)
+    want = rgba.copy()
+    want[..., 3] = 255
+    assert np.array_equal(_read(tmp_path, "c.bmp", bmp), _bottom_up(want))
+    # PPM (P6, maxval 255) and PGM (P5, maxval 1000: rescaled to 0..255).
+    ppm = b"P6\n# comment\n%d %d\n255\n" % (w, h) + rgba[..., :3].tobytes()
+    assert np.array_equal(_read(tmp_path, "d.ppm", ppm), _bottom_up(want))
+    grey = rng.integers(0, 1001, (h, w))
+    pgm = b"P5 %d %d 1000\n" % (w, h) + b"".join(struct.pack(">H", int(v)) for v in grey.ravel())
+    g8 = (grey * 255 + 500) // 1000
+    want = np.stack([g8, g8, g8, np.full_like(g8, 255)], -1).astype(np.uint8)
+    assert np.array_equal(_read(tmp_path, "e.pgm", pgm), _bottom_up(want))
+
+
+def test_image_errors(tmp_path):
+    rng = np.random.default_rng(1)
+    good = _png(rng.integers(0, 256, (4, 4, 3)), 2, 8, rng)
+    cases = {"cut.png": (good[:40], nw.ERR_FORMAT),
+             "photo.jpg": (b"\xff\xd8\xff\xe0" + bytes(100), nw.ERR_UNSUPPORTED),
+             "what.xyz": (b"abcdefgh", nw.ERR_UNSUPPORTED)}
+    for name, (data, status) in cases.items():
+        p = tmp_path / name
+        p.write_bytes(data)
+        with pytest.raises(nw.NwprtError) as e:
+            nw.image_read_rgba8(str(p))
+        assert e.value.status == status, name
+    with pytest.raises(nw.NwprtError) as e:
+        nw.image_read_rgba8(str(tmp_path / "missing.png"))
+    assert e.value.status == nw.ERR_IO
