@@ -483,6 +483,25 @@ nwprt_status nwprt_image_read_rgba8(const char *path, uint8_t **rgba, int32_t *w
 
 #define NWPRT_MATERIAL_MIRROR 3
 
+/* Material texture fields (the SBT fields devicePrograms.cu:121-147 reads
+ * from textures when the mesh has texture coordinates): kd and ks take the
+ * texel's RGB, the others its first channel. */
+#define NWPRT_TEX_KD 0
+#define NWPRT_TEX_KS 1
+#define NWPRT_TEX_ROUGHNESS 2
+#define NWPRT_TEX_SPECULAR_TRANSMISSION 3
+#define NWPRT_TEX_METALLIC 4
+#define NWPRT_TEX_SUBSURFACE 5
+#define NWPRT_TEX_SPECULAR 6
+#define NWPRT_TEX_SPECULAR_TINT 7
+#define NWPRT_TEX_ANISOTROPIC 8
+#define NWPRT_TEX_SHEEN 9
+#define NWPRT_TEX_SHEEN_TINT 10
+#define NWPRT_TEX_CLEARCOAT 11
+#define NWPRT_TEX_CLEARCOAT_GLOSS 12
+#define NWPRT_TEX_ETA 13
+#define NWPRT_TEX_FIELDS 14
+
 typedef struct {
   float lower[3], upper[3]; /* bounds of the vertices (Scene::bounds)       */
   int64_t triangles;
@@ -524,6 +543,34 @@ nwprt_status nwprt_ply_read(const char *path, float **vertices, int64_t *n_verti
                             int32_t **material_ids, nwprt_material **materials,
                             int32_t *n_materials);
 nwprt_status nwprt_mesh_destroy(nwprt_ctx ctx, int32_t mesh_id);
+
+/* A material texture in device memory (EightBitTexture::toCUDA,
+ * Renderer/textures.cpp:8-47 and textures.h:74-90): RGBA8 texels [height]
+ * [width][4] with row 0 at v = 0 (as nwprt_image_read_rgba8 returns them),
+ * sampled like the reference's CUDA texture object: normalized coordinates
+ * wrapped in both directions, bilinear filtering between texel centres
+ * ((i + 0.5) / width), bytes read as b / 255 with the sRGB decode on the
+ * colour channels (tex_desc.sRGB is set), no mipmaps.  Host pixels,
+ * copied. */
+nwprt_status nwprt_texture_create(nwprt_ctx ctx, const uint8_t *rgba, int32_t width,
+                                  int32_t height, int32_t *texture_id);
+/* NWPRT_ERR_STATE while a mesh uses the texture. */
+nwprt_status nwprt_texture_destroy(nwprt_ctx ctx, int32_t texture_id);
+/* Texture coordinates and material textures of a mesh (the SBT's texcoord
+ * array and per-field hasTexture / texture, OptixBackend.cpp:7-8):
+ * texcoords [n_vertices][2] (host; NULL: none, and then, as without
+ * hasTexcoord, no texture is read); material_textures [n_materials]
+ * [NWPRT_TEX_FIELDS] texture ids, -1 for the material's constant (NULL:
+ * none).  At a hit the coordinates are interpolated with the barycentrics,
+ * (1 - u - v) t0 + u t1 + v t2, and each textured field replaces the
+ * material's value in the first-hit encodings, the direct term and the
+ * transfer computation.  Replaces earlier textures of the mesh; the mesh
+ * holds its textures until it is destroyed or its textures are replaced.
+ * The SH render does not take a textured mesh (NWPRT_ERR_UNSUPPORTED): the
+ * reference reads its textures per pixel, which its per-vertex colours
+ * cannot represent. */
+nwprt_status nwprt_mesh_set_textures(nwprt_ctx ctx, int32_t mesh_id, const float *texcoords,
+                                     const int32_t *material_textures);
 
 typedef struct {
   int32_t max_bounce;          /* samplingParameter.maximumBounce: 1 in

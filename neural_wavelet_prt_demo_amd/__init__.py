@@ -242,6 +242,9 @@ _SIGNATURES = {
                                POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int32)]),
     "nwprt_mesh_destroy": (c_int, [c_void_p, c_int32]),
+    "nwprt_texture_create": (c_int, [c_void_p, c_void_p, c_int32, c_int32, POINTER(c_int32)]),
+    "nwprt_texture_destroy": (c_int, [c_void_p, c_int32]),
+    "nwprt_mesh_set_textures": (c_int, [c_void_p, c_int32, c_void_p, c_void_p]),
     "nwprt_gbuffer_cast": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32,
                                    POINTER(CastParams), POINTER(GBuffer), c_void_p]),
     "nwprt_sh_render": (c_int, [c_void_p, c_int32, POINTER(Camera), c_int32, c_int32, c_void_p,
@@ -461,6 +464,13 @@ def material_defaults(mat_type: int) -> np.ndarray:
     return np.array(list(m.kd) + list(m.ks) + [m.roughness, m.type, m.specular_transmission,
                     m.metallic, m.subsurface, m.specular, m.specular_tint, m.anisotropic,
                     m.sheen, m.sheen_tint, m.clearcoat, m.clearcoat_gloss, m.eta], np.float64)
+
+
+# Material texture fields (NWPRT_TEX_*).
+TEX_FIELD_NAMES = ("kd", "ks", "roughness", "specular_transmission", "metallic", "subsurface",
+                   "specular", "specular_tint", "anisotropic", "sheen", "sheen_tint", "clearcoat",
+                   "clearcoat_gloss", "eta")
+TEX_FIELDS = len(TEX_FIELD_NAMES)
 
 
 def image_read_rgba8(path: str) -> np.ndarray:
@@ -858,6 +868,30 @@ class Context:
 
     def mesh_destroy(self, mesh: int) -> None:
         _check(lib().nwprt_mesh_destroy(self.h, mesh))
+
+    def texture_create(self, rgba: np.ndarray) -> int:
+        """A material texture (nwprt_texture_create) from uint8 [height]
+        [width][4] texels, row 0 at v = 0 (as image_read_rgba8 returns
+        them)."""
+        t = np.ascontiguousarray(rgba, np.uint8)
+        assert t.ndim == 3 and t.shape[2] == 4, t.shape
+        tid = c_int32()
+        _check(lib().nwprt_texture_create(self.h, t.ctypes.data, t.shape[1], t.shape[0],
+                                          ctypes.byref(tid)))
+        return tid.value
+
+    def texture_destroy(self, texture: int) -> None:
+        _check(lib().nwprt_texture_destroy(self.h, texture))
+
+    def mesh_set_textures(self, mesh: int, texcoords: Optional[np.ndarray],
+                          material_textures: Optional[np.ndarray]) -> None:
+        """texcoords: [n_vertices][2]; material_textures: int [n_materials]
+        [TEX_FIELDS] texture ids or -1 (None for either: untextured)."""
+        tc = None if texcoords is None else np.ascontiguousarray(texcoords, np.float32)
+        mt = None if material_textures is None else np.ascontiguousarray(material_textures,
+                                                                         np.int32)
+        _check(lib().nwprt_mesh_set_textures(self.h, mesh, None if tc is None else tc.ctypes.data,
+                                             None if mt is None else mt.ctypes.data))
 
     def gbuffer_cast(self, mesh: int, camera, frame, *, max_bounce: int = 1, bounds=None,
                      env_faces=None, env_n: int = 0, light_faces=None, light_n: int = 0,

@@ -99,6 +99,19 @@ struct Mesh {
   bool has_normals = false, has_tri_material = false;
   bool sh_ok = true;    // no mirror material (the SH render has no mirror BSDF)
   bool disney = false;  // some material the triangles use is DisneyBSDF
+  int32_t n_materials = 0;
+  // Material textures (nwprt_mesh_set_textures): texcoords [n_vertices][2]
+  // and TexRef [n_materials][kTexFields]; the texture ids the mesh holds.
+  DevBuf texcoords, mat_tex;
+  std::vector<int32_t> texture_ids;
+  bool textured = false;
+};
+
+// A material texture (nwprt_texture_create): RGBA8 texels, row 0 at v = 0.
+struct TextureObj {
+  DevBuf rgba;
+  int32_t width = 0, height = 0;
+  int uses = 0;  // meshes holding it
 };
 
 // Work buffers of one frame slot (0, 1: the caller's two-frame pipeline,
@@ -172,6 +185,7 @@ struct nwprt_context {
   std::map<int, nwprt::ServerPtr> servers;
   int n_cu = 0;  // compute units of the device (persistent grid size)
   nwprt::DevBuf denoise_scratch;  // denoiser ping-pong images + normalisation
+  std::map<int, std::unique_ptr<nwprt::TextureObj>> textures;
   std::map<int, std::unique_ptr<nwprt::Mesh>> meshes;
   nwprt::DevBuf env_tables;  // direct-light sampling tables of the last cast
   nwprt::DevBuf sh_response, sh_colors;  // SH-PRT render work buffers

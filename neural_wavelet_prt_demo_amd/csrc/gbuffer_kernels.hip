@@ -22,6 +22,7 @@
 #include <climits>
 
 #include "kernels.hpp"
+#include "srgb_table.hpp"
 
 namespace nwprt {
 namespace {
@@ -278,6 +279,78 @@ struct Surface {
   float st, metallic, subsurface, specular, spec_tint, aniso, sheen, sheen_tint, clearcoat,
       cc_gloss, eta;
 };
+
+// One texel of a material texture as the reference's CUDA texture object
+// reads it: b / 255, the colour channels through the sRGB decode.
+__device__ inline float4 texel(const TexRef &t, int i, int j) {
+  const uint32_t q = reinterpret_cast<const uint32_t *>(t.rgba)[(size_t)j * t.w + i];
+  return make_float4(kSrgbToLinear[q & 0xffu], kSrgbToLinear[(q >> 8) & 0xffu],
+                     kSrgbToLinear[(q >> 16) & 0xffu], (float)(q >> 24) * (1.0f / 255.0f));
+}
+
+// tex2D with normalized coordinates, wrap addressing and bilinear filtering
+// (Renderer/textures.cpp:8-47): the coordinate's fractional part times the
+// size, minus half a texel, between the centres of the four neighbours.
+__device__ inline float4 tex_sample(const TexRef &t, float u, float v) {
+  float fu = u - floorf(u), fv = v - floorf(v);
+  if (!(fu >= 0.0f && fu < 1.0f)) fu = 0.0f;  // NaN or inf coordinates
+  if (!(fv >= 0.0f && fv < 1.0f)) fv = 0.0f;
+  const float x = fu * (float)t.w - 0.5f, y = fv * (float)t.h - 0.5f;
+  const float x0f = floorf(x), y0f = floorf(y);
+  const float ax = x - x0f, ay = y - y0f;
+  int x0 = (int)x0f, y0 = (int)y0f;  // in [-1, w - 1] and [-1, h - 1]
+  if (x0 < 0) x0 += t.w;
+  if (y0 < 0) y0 += t.h;
+  const int x1 = x0 + 1 == t.w ? 0 : x0 + 1, y1 = y0 + 1 == t.h ? 0 : y0 + 1;
+  const float4 a = texel(t, x0, y0), b = texel(t, x1, y0), c = texel(t, x0, y1),
+               d = texel(t, x1, y1);
+  const float w00 = (1.0f - ax) * (1.0f - ay), w10 = ax * (1.0f - ay), w01 = (1.0f - ax) * ay,
+              w11 = ax * ay;
+  return make_float4(w00 * a.x + w10 * b.x + w01 * c.x + w11 * d.x,
+                     w00 * a.y + w10 * b.y + w01 * c.y + w11 * d.y,
+                     w00 * a.z + w10 * b.z + w01 * c.z + w11 * d.z,
+                     w00 * a.w + w10 * b.w + w01 * c.w + w11 * d.w);
+}
+
+// The texture coordinates of a hit: (1 - u - v) t0 + u t1 + v t2
+// (devicePrograms.cu:66-71).
+__device__ inline float2 hit_texcoord(const float *tc, const int32_t *indices, int prim, float u,
+                                      float v) {
+  const int *tri = indices + 3 * (size_t)prim;
+  const float w = 1.0f - u - v;
+  const float2 t0 = reinterpret_cast<const float2 *>(tc)[tri[0]];
+  const float2 t1 = reinterpret_cast<const float2 *>(tc)[tri[1]];
+  const float2 t2 = reinterpret_cast<const float2 *>(tc)[tri[2]];
+  return make_float2(w * t0.x + u * t1.x + v * t2.x, w * t0.y + u * t1.y + v * t2.y);
+}
+
+// The material with its textured fields read at tc (devicePrograms.cu:
+// 121-147): kd and ks take the texel's RGB, the others its first channel.
+__device__ inline MaterialRec textured(MaterialRec m, const TexRef *mt, float2 tc) {
+#pragma unroll 1
+  for (int f = 0; f < kTexFields; ++f) {
+    const TexRef t = mt[f];
+    if (!t.rgba) continue;
+    const float4 x = tex_sample(t, tc.x, tc.y);
+    switch (f) {
+      case NWPRT_TEX_KD: m.kd[0] = x.x, m.kd[1] = x.y, m.kd[2] = x.z; break;
+      case NWPRT_TEX_KS: m.ks[0] = x.x, m.ks[1] = x.y, m.ks[2] = x.z; break;
+      case NWPRT_TEX_ROUGHNESS: m.roughness = x.x; break;
+      case NWPRT_TEX_SPECULAR_TRANSMISSION: m.specular_transmission = x.x; break;
+      case NWPRT_TEX_METALLIC: m.metallic = x.x; break;
+      case NWPRT_TEX_SUBSURFACE: m.subsurface = x.x; break;
+      case NWPRT_TEX_SPECULAR: m.specular = x.x; break;
+      case NWPRT_TEX_SPECULAR_TINT: m.specular_tint = x.x; break;
+      case NWPRT_TEX_ANISOTROPIC: m.anisotropic = x.x; break;
+      case NWPRT_TEX_SHEEN: m.sheen = x.x; break;
+      case NWPRT_TEX_SHEEN_TINT: m.sheen_tint = x.x; break;
+      case NWPRT_TEX_CLEARCOAT: m.clearcoat = x.x; break;
+      case NWPRT_TEX_CLEARCOAT_GLOSS: m.clearcoat_gloss = x.x; break;
+      default: m.eta = x.x; break;
+    }
+  }
+  return m;
+}
 
 __device__ inline Surface make_surface(const MaterialRec &m, float3 ns, float3 ng) {
   return Surface{v3(m.kd), v3(m.ks), m.roughness, ns, ng, m.type, m.specular_transmission,
@@ -985,7 +1058,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gb
       rough = 1.0f;
       break;
     }
-    const MaterialRec mat = a.materials[a.tri_material ? a.tri_material[h.prim] : 0];
+    const int mid = a.tri_material ? a.tri_material[h.prim] : 0;
+    MaterialRec mat = a.materials[mid];
     const float4 *rec = reinterpret_cast<const float4 *>(a.tris + h.slot);
     const float3 ng = normalize(cross(v3(rec[1]), v3(rec[2])));
     float3 ns = ng;
@@ -1004,6 +1078,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void gb
       d = normalize(reflect(d, ns));
       continue;
     }
+    if (a.mat_tex)
+      mat = textured(mat, a.mat_tex + (size_t)mid * kTexFields,
+                     hit_texcoord(a.texcoords, a.indices, h.prim, h.u, h.v));
     pos = p;
     nrm = ns;
     refl = normalize(reflect(d, ns));
@@ -1360,8 +1437,9 @@ struct TmWork {
   TmPath *path;   // [pair]
   float *y;       // [pair][kTmRow]: filtered basis of the start direction
   float *ind;     // [pair][map][3]: indirect radiance
+  float2 *tc;     // [pair][slots]: a record's texture coordinates (textured meshes)
 };
-__host__ __device__ inline TmWork tm_work(void *base, long long pairs, int slots) {
+__host__ __device__ inline TmWork tm_work(void *base, long long pairs, int slots, int nc) {
   char *p = (char *)base;
   TmWork w;
   w.rec = (TmRecord *)p;
@@ -1371,6 +1449,8 @@ __host__ __device__ inline TmWork tm_work(void *base, long long pairs, int slots
   w.y = (float *)p;
   p += tm_align((size_t)pairs * kTmRow * sizeof(float));
   w.ind = (float *)p;
+  p += tm_align((size_t)pairs * nc * 3 * sizeof(float));
+  w.tc = (float2 *)p;  // only addressed for textured meshes
   return w;
 }
 
@@ -1387,7 +1467,10 @@ __device__ inline float3 tm_bsdf_sample(const Surface &s, float3 v, uint32_t see
   return bsdf_sample<DS>(s, v, seed, vtx, c);
 }
 
-__device__ inline Surface tm_surface(const TmArgs &a, int mat, float3 ns, float3 ng) {
+// The surface of a path vertex; tc: its texture coordinates (read only for
+// textured meshes).
+__device__ inline Surface tm_surface(const TmArgs &a, int mat, float3 ns, float3 ng, float2 tc) {
+  if (a.mat_tex) return make_surface(textured(a.materials[mat], a.mat_tex + (size_t)mat * kTexFields, tc), ns, ng);
   return make_surface(a.materials[mat], ns, ng);
 }
 
@@ -1399,7 +1482,7 @@ __global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
   if (pair >= a.n_verts * a.samples) return;  // no barriers below
   int *stack = stack_lds + lane;
   const int slots = tm_slots(a.max_bounce);
-  const TmWork w = tm_work(a.work, a.n_verts * a.samples, slots);
+  const TmWork w = tm_work(a.work, a.n_verts * a.samples, slots, (a.degree + 1) * (a.degree + 1));
   TmRecord *rec = w.rec + pair * slots;
   const long long vl = pair / a.samples;
   const int s = (int)(pair - vl * a.samples);
@@ -1439,7 +1522,12 @@ __global__ __launch_bounds__(64) void tm_paths_kernel(TmArgs a) {
         d = reflect(rd, ns);
         if (!first_hit) mirror_first = true;
       } else {
-        const Surface sf = tm_surface(a, mat, ns, ng);
+        float2 tc = make_float2(0.0f, 0.0f);
+        if (a.mat_tex) {
+          tc = hit_texcoord(a.texcoords, a.indices, h.prim, h.u, h.v);
+          w.tc[pair * slots + nrec] = tc;
+        }
+        const Surface sf = tm_surface(a, mat, ns, ng, tc);
         const float3 v = scale(rd, -1.0f);
         TmRecord &r = rec[nrec++];
         r.p[0] = p.x, r.p[1] = p.y, r.p[2] = p.z;
@@ -1527,7 +1615,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void tm
   if (q >= pairs * nc) return;  // no barriers below
   int *stack = stack_lds + lane;
   const int slots = tm_slots(a.max_bounce);
-  const TmWork w = tm_work(a.work, pairs, slots);
+  const TmWork w = tm_work(a.work, pairs, slots, nc);
   const long long pair = q / nc;
   const int i = (int)(q - pair * nc);
   const long long vl = pair / a.samples;
@@ -1553,7 +1641,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void tm
       // direction below the surface, where the BSDF is zero) or a NaN one
       // adds nothing whether or not the ray is blocked, so only a non-zero
       // finite term pays for the ray.
-      const Surface sf = tm_surface(a, h.tag >> 4, v3(h.ns), v3(h.ng));
+      const Surface sf = tm_surface(a, h.tag >> 4, v3(h.ns), v3(h.ng),
+                                    a.mat_tex ? w.tc[pair * slots + r] : make_float2(0.0f, 0.0f));
       const float pb = bsdf_pdf<DS>(sf, l, v);
       const float3 f = bsdf_eval<DS>(sf, l, v);
       const float3 le = env_radiance(faces, a.n, l);
@@ -1611,7 +1700,7 @@ __global__ __launch_bounds__(256) void tm_project_kernel(TmArgs a) {
   const int nc = (a.degree + 1) * (a.degree + 1), cols = 3 * nc;
   const long long vl = blockIdx.x;
   const long long pairs = a.n_verts * a.samples;
-  const TmWork w = tm_work(a.work, pairs, tm_slots(a.max_bounce));
+  const TmWork w = tm_work(a.work, pairs, tm_slots(a.max_bounce), nc);
   const float *ind = w.ind + (size_t)vl * a.samples * cols;
   const float fn = (float)a.samples, scale_n = 4.0f * 3.14159265358979f / fn;
   float acc[kShCoeffs];
@@ -1712,12 +1801,13 @@ void launch_sh_render(const ShArgs &a, hipStream_t s) {
   hip_check(hipGetLastError(), "sh_pixel_kernel launch");
 }
 
-size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce) {
+size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce, bool textured) {
   const long long pairs = n_verts * samples;
   const int nc = (degree + 1) * (degree + 1);
   return tm_align((size_t)pairs * tm_slots(max_bounce) * sizeof(TmRecord)) +
          tm_align((size_t)pairs * sizeof(TmPath)) + tm_align((size_t)pairs * kTmRow * sizeof(float)) +
-         tm_align((size_t)pairs * nc * 3 * sizeof(float));
+         tm_align((size_t)pairs * nc * 3 * sizeof(float)) +
+         (textured ? tm_align((size_t)pairs * tm_slots(max_bounce) * sizeof(float2)) : 0);
 }
 
 void launch_sh_transfer(const TmArgs &a, hipStream_t s) {

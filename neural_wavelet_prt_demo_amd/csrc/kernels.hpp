@@ -193,6 +193,16 @@ struct alignas(16) MaterialRec {
   float specular_tint, anisotropic, sheen, sheen_tint;
   float clearcoat, clearcoat_gloss, eta, pad;
 };
+// A material texture as the G-buffer kernels sample it: RGBA8 texels, row 0
+// at v = 0; rgba null: the field keeps the material's constant.  Fields
+// (NWPRT_TEX_*): kd, ks (RGB), then roughness and the DisneyBSDF
+// parameters and eta (first channel).
+struct TexRef {
+  const uint8_t *rgba;
+  int32_t w, h;
+};
+static_assert(sizeof(TexRef) == 16, "TexRef layout");
+constexpr int kTexFields = NWPRT_TEX_FIELDS;
 struct CastArgs {
   const BvhNode *nodes;
   const BvhTri *tris;
@@ -215,6 +225,10 @@ struct CastArgs {
   int light_n, direct_samples;
   uint32_t seed;
   bool disney;  // some material is DisneyBSDF (selects the kernel instance)
+  // Material textures: texcoords [n_vertices][2] and mat_tex
+  // [n_materials][kTexFields], both null for an untextured mesh.
+  const float *texcoords;
+  const TexRef *mat_tex;
 };
 // Doubles of the environment sampling tables for face size n: face pmf[6],
 // face cdf[7], then per face cdf_rows[n][n+1], pdf_rows[n][n],
@@ -276,11 +290,14 @@ struct TmArgs {
   int n, degree, samples, max_bounce;
   uint32_t seed;
   long long v_begin, n_verts;
-  void *work;                   // tm_work_bytes(n_verts, samples, degree, max_bounce)
+  void *work;                   // tm_work_bytes(n_verts, samples, degree, max_bounce, textured)
   float *transfer;              // [n_vertices][nc * nc][3]
   bool disney;                  // some material is DisneyBSDF
+  const float *texcoords;       // as CastArgs (null: untextured)
+  const TexRef *mat_tex;
 };
-size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce);
+// textured: the work also holds each path record's texture coordinates.
+size_t tm_work_bytes(long long n_verts, int samples, int degree, int max_bounce, bool textured);
 void launch_sh_transfer(const TmArgs &a, hipStream_t s);
 
 // Persistent batch form: work item = the 64 pixels [base, base + 64) of job

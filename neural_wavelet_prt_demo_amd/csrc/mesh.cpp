@@ -221,6 +221,18 @@ Mesh &get_mesh(nwprt_ctx ctx, int id) {
   return *it->second;
 }
 
+// Drops the mesh's hold on its textures and its texture buffers.
+void release_textures(nwprt_ctx ctx, Mesh &m) {
+  for (int32_t id : m.texture_ids) {
+    auto it = ctx->textures.find(id);
+    if (it != ctx->textures.end()) --it->second->uses;
+  }
+  m.texture_ids.clear();
+  m.texcoords.release();
+  m.mat_tex.release();
+  m.textured = false;
+}
+
 bool finite3(const float *p) {
   return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]);
 }
@@ -395,6 +407,7 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
     mesh->depth = bld.max_depth;
     mesh->has_normals = normals != nullptr;
     mesh->has_tri_material = material_ids != nullptr;
+    mesh->n_materials = n_materials;
     // Capabilities of the materials the triangles use.
     std::vector<char> used(mats.size(), 0);
     if (material_ids)
@@ -428,9 +441,72 @@ nwprt_status nwprt_mesh_get_info(nwprt_ctx ctx, int32_t mesh_id, nwprt_mesh_info
 nwprt_status nwprt_mesh_destroy(nwprt_ctx ctx, int32_t mesh_id) {
   return guarded([&] {
     bind(ctx);
-    get_mesh(ctx, mesh_id);
+    Mesh &m = get_mesh(ctx, mesh_id);
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    release_textures(ctx, m);
     ctx->meshes.erase(mesh_id);
+  });
+}
+
+nwprt_status nwprt_texture_create(nwprt_ctx ctx, const uint8_t *rgba, int32_t width,
+                                  int32_t height, int32_t *texture_id) {
+  return guarded([&] {
+    bind(ctx);
+    require(rgba && texture_id, NWPRT_ERR_INVALID_ARGUMENT, "texture_create: null argument");
+    require(width >= 1 && height >= 1 && (int64_t)width * height <= (1ll << 28),
+            NWPRT_ERR_INVALID_ARGUMENT, "texture_create: size must be 1..2^28 texels");
+    auto t = std::make_unique<TextureObj>();
+    t->rgba.upload(rgba, (size_t)width * height * 4);
+    t->width = width;
+    t->height = height;
+    const int id = ctx->next_id++;
+    ctx->textures[id] = std::move(t);
+    *texture_id = id;
+  });
+}
+
+nwprt_status nwprt_texture_destroy(nwprt_ctx ctx, int32_t texture_id) {
+  return guarded([&] {
+    bind(ctx);
+    auto it = ctx->textures.find(texture_id);
+    require(it != ctx->textures.end(), NWPRT_ERR_NOT_FOUND,
+            "unknown texture id " + std::to_string(texture_id));
+    require(it->second->uses == 0, NWPRT_ERR_STATE, "texture_destroy: a mesh uses the texture");
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    ctx->textures.erase(it);
+  });
+}
+
+nwprt_status nwprt_mesh_set_textures(nwprt_ctx ctx, int32_t mesh_id, const float *texcoords,
+                                     const int32_t *material_textures) {
+  return guarded([&] {
+    bind(ctx);
+    Mesh &m = get_mesh(ctx, mesh_id);
+    std::vector<TexRef> refs;
+    std::vector<int32_t> held;
+    if (texcoords && material_textures) {
+      refs.resize((size_t)m.n_materials * kTexFields, TexRef{nullptr, 0, 0});
+      for (size_t i = 0; i < refs.size(); ++i) {
+        const int32_t id = material_textures[i];
+        if (id < 0) continue;
+        auto it = ctx->textures.find(id);
+        require(it != ctx->textures.end(), NWPRT_ERR_NOT_FOUND,
+                "mesh_set_textures: unknown texture id " + std::to_string(id));
+        refs[i] = TexRef{it->second->rgba.as<uint8_t>(), it->second->width, it->second->height};
+        held.push_back(id);
+      }
+      for (int64_t v = 0; v < 2 * m.n_vertices; ++v)
+        require(std::isfinite(texcoords[v]), NWPRT_ERR_INVALID_ARGUMENT,
+                "mesh_set_textures: texture coordinates must be finite");
+    }
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");  // kernels may read the old ones
+    release_textures(ctx, m);
+    if (held.empty()) return;  // no texture read: the mesh stays untextured
+    m.texcoords.upload(texcoords, (size_t)m.n_vertices * 2 * sizeof(float));
+    m.mat_tex.upload(refs.data(), refs.size() * sizeof(TexRef));
+    for (int32_t id : held) ++ctx->textures[id]->uses;
+    m.texture_ids = std::move(held);
+    m.textured = true;
   });
 }
 
@@ -462,6 +538,8 @@ nwprt_status nwprt_gbuffer_cast(nwprt_ctx ctx, int32_t mesh_id, const nwprt_came
     a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
     a.materials = m.materials.as<MaterialRec>();
     a.disney = m.disney;
+    a.texcoords = m.textured ? m.texcoords.as<float>() : nullptr;
+    a.mat_tex = m.textured ? m.mat_tex.as<TexRef>() : nullptr;
     const float *lo = params->bounds ? params->bounds : m.lower;
     const float *hi = params->bounds ? params->bounds + 3 : m.upper;
     for (int k = 0; k < 3; ++k) {
@@ -526,6 +604,9 @@ nwprt_status nwprt_sh_render(nwprt_ctx ctx, int32_t mesh_id, const nwprt_camera 
             "sh_render: the mesh needs vertex normals");
     require(m.sh_ok, NWPRT_ERR_UNSUPPORTED,
             "sh_render: mirror materials have no SH-render BSDF");
+    require(!m.textured, NWPRT_ERR_UNSUPPORTED,
+            "sh_render: textured meshes are read per pixel by the reference; the per-vertex "
+            "colours of this render cannot carry them");
     require(finite3(camera->from) && finite3(camera->at) && finite3(camera->up),
             NWPRT_ERR_INVALID_ARGUMENT, "sh_render: non-finite camera");
     check_aligned(color, 16, "sh_render colour");
@@ -598,11 +679,11 @@ nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const flo
     // per-(sample, map) radiance of a launch stay within kTmWorkBudget, and
     // the (vertex, sample, map) triples within one grid.
     constexpr size_t kTmWorkBudget = (size_t)2 << 30;
-    const size_t per_vertex = tm_work_bytes(1, samples, degree, max_bounce);
+    const size_t per_vertex = tm_work_bytes(1, samples, degree, max_bounce, m.textured);
     long long chunk = (long long)std::max<size_t>(1, kTmWorkBudget / per_vertex);
     chunk = std::min<long long>(chunk, ((1ll << 31) - 1) * 64 / ((long long)samples * nc));
     chunk = std::max<long long>(1, std::min<long long>(chunk, vertex_count));
-    ctx->sh_work.ensure(tm_work_bytes(chunk, samples, degree, max_bounce));
+    ctx->sh_work.ensure(tm_work_bytes(chunk, samples, degree, max_bounce, m.textured));
     TmArgs a{};
     a.nodes = m.nodes.as<BvhNode>();
     a.tris = m.tris.as<BvhTri>();
@@ -612,6 +693,8 @@ nwprt_status nwprt_sh_compute_transfer(nwprt_ctx ctx, int32_t mesh_id, const flo
     a.tri_material = m.has_tri_material ? m.tri_material.as<int32_t>() : nullptr;
     a.materials = m.materials.as<MaterialRec>();
     a.disney = m.disney;
+    a.texcoords = m.textured ? m.texcoords.as<float>() : nullptr;
+    a.mat_tex = m.textured ? m.mat_tex.as<TexRef>() : nullptr;
     a.basis = basis_faces;
     a.tabs = ctx->sh_tables.as<double>();
     a.tab_doubles = td;

@@ -34,7 +34,40 @@ class OrcGbuffer(Structure):
 
 class OrcMesh(Structure):
     _fields_ = [("vertices", c_void_p), ("normals", c_void_p), ("indices", c_void_p),
-                ("n_tris", ctypes.c_int64), ("material_ids", c_void_p), ("materials", c_void_p)]
+                ("n_tris", ctypes.c_int64), ("material_ids", c_void_p), ("materials", c_void_p),
+                ("texcoords", c_void_p), ("mat_tex", c_void_p), ("tex_texels", c_void_p),
+                ("tex_offset", c_void_p), ("tex_w", c_void_p), ("tex_h", c_void_p)]
+
+
+TEX_FIELDS = 14
+
+
+def _with_textures(mesh, keep, texcoords=None, mat_tex=None, textures=None):
+    """Fills the texture fields of an OrcMesh.  texcoords [n_vertices][2];
+    mat_tex [n_materials][14] indices into `textures` or -1; textures: list
+    of uint8 [h][w][4] arrays, row 0 at v = 0.  `keep` holds the arrays."""
+    if texcoords is None or mat_tex is None or not textures:
+        return mesh
+    tc = np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+    mt = np.ascontiguousarray(mat_tex, np.int32).reshape(-1, TEX_FIELDS)
+    ts = [np.ascontiguousarray(t, np.uint8) for t in textures]
+    texels = np.concatenate([t.ravel() for t in ts])
+    off = np.cumsum([0] + [t.size for t in ts[:-1]]).astype(np.int64)
+    tw = np.array([t.shape[1] for t in ts], np.int32)
+    th = np.array([t.shape[0] for t in ts], np.int32)
+    keep += [tc, mt, texels, off, tw, th]
+    mesh.texcoords, mesh.mat_tex, mesh.tex_texels = _p(tc), _p(mt), _p(texels)
+    mesh.tex_offset, mesh.tex_w, mesh.tex_h = _p(off), _p(tw), _p(th)
+    return mesh
+
+
+def tex_sample(texture: np.ndarray, u: float, v: float) -> np.ndarray:
+    """orc_tex_sample of one uint8 [h][w][4] texture at (u, v)."""
+    keep = []
+    mesh = _with_textures(OrcMesh(), keep, np.zeros((1, 2)), -np.ones((1, TEX_FIELDS)), [texture])
+    out = np.zeros(4)
+    lib().orc_tex_sample(ctypes.byref(mesh), 0, float(u), float(v), _p(out))
+    return out
 
 
 def _gbuf(g):
@@ -122,6 +155,8 @@ def lib():
                                       c_int, c_uint32, c_void_p, ctypes.c_int64, c_void_p,
                                       c_void_p, c_void_p]
         l.orc_sh_transfer.restype = None
+        l.orc_tex_sample.argtypes = [POINTER(OrcMesh), c_int, c_double, c_double, c_void_p]
+        l.orc_tex_sample.restype = None
         for name in ("orc_bsdf_sample_many", "orc_bsdf_eval_many", "orc_bsdf_pdf_many"):
             getattr(l, name).restype = None
         l.orc_bsdf_sample_many.argtypes = [c_void_p] * 4 + [ctypes.c_int64] + [c_void_p] * 3
@@ -294,8 +329,9 @@ def gbuffer_cast(vertices: np.ndarray, indices: np.ndarray, materials, camera, w
                  material_ids: Optional[np.ndarray] = None, max_bounce: int = 1,
                  bounds=None, env_faces: Optional[np.ndarray] = None,
                  light_faces: Optional[np.ndarray] = None, samples: int = 1,
-                 seed: int = 0) -> Dict[str, np.ndarray]:
-    """First-hit G-buffer by FP64 brute force.  materials: rows (kd[3], ks[3],
+                 seed: int = 0, texcoords=None, mat_tex=None,
+                 textures=None) -> Dict[str, np.ndarray]:
+    """First-hit G-buffer by FP64 brute force (textures: see _with_textures).  materials: rows (kd[3], ks[3],
     roughness, type); camera: (from, at, up); bounds: (lower, upper) or the
     vertices' own.  Returns float64 [P][4] position / normal / reflect / kd /
     ks, [P] roughness, uint8 env_flag, int32 prim (-1 none), float64 margin."""
@@ -312,7 +348,9 @@ def gbuffer_cast(vertices: np.ndarray, indices: np.ndarray, materials, camera, w
     if env_faces is not None:
         env_faces = np.ascontiguousarray(env_faces, np.float32)
         env_n = env_faces.shape[1]
-    mesh = OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats))
+    keep = []
+    mesh = _with_textures(OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats)), keep,
+                          texcoords, mat_tex, textures)
     P = width * height
     out = {k: np.empty((P, 4), np.float64) for k in ("position", "normal", "reflect", "kd", "ks")}
     out["roughness"] = np.empty(P, np.float64)
@@ -412,8 +450,10 @@ def sh_render(vertices, indices, normals, materials, camera, width: int, height:
 
 def sh_transfer(vertices, indices, materials, basis, degree: int, *, normals=None,
                 material_ids=None, samples: int = 4096, max_bounce: int = 12, rr_depth: int = 3,
-                seed: int = 0, verts=None) -> Dict[str, np.ndarray]:
-    """SH transfer matrices by FP64 replay (orc_sh_transfer).  basis: the
+                seed: int = 0, verts=None, texcoords=None, mat_tex=None,
+                textures=None) -> Dict[str, np.ndarray]:
+    """SH transfer matrices by FP64 replay (orc_sh_transfer; textures: see
+    _with_textures).  basis: the
     nc = (degree + 1)^2 basis maps, float32 [nc][6][n][n][3]; verts: vertex
     indices (all by default).  Returns T float64 [len(verts)][nc * nc][3]
     (row k * nc + i), abs_sum (same shape: sums of |terms|) and margin
@@ -428,7 +468,9 @@ def sh_transfer(vertices, indices, materials, basis, degree: int, *, normals=Non
     assert b.ndim == 5 and b.shape[0] == nc and b.shape[1] == 6 and b.shape[4] == 3
     n = b.shape[2]
     vs = np.arange(len(v), dtype=np.int64) if verts is None else np.ascontiguousarray(verts, np.int64)
-    mesh = OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats))
+    keep = []
+    mesh = _with_textures(OrcMesh(_p(v), _p(nr), _p(ix), len(ix), _p(mid), _p(mats)), keep,
+                          texcoords, mat_tex, textures)
     out = {"T": np.empty((len(vs), nc * nc, 3), np.float64),
            "abs_sum": np.empty((len(vs), nc * nc, 3), np.float64),
            "margin": np.empty((len(vs), nc), np.float64)}

@@ -623,6 +623,8 @@ static void g_direct(const orc_mesh *m, const double *tab, int n, const float *l
                      const float *mat, int samples, uint32_t seed, uint32_t pixel, double *out);
 
 static void g_tint(const double *c, double *t);
+static const float *g_material_row(const orc_mesh *m, int mat, const int *tri, double u,
+                                   double v, float *buf);
 
 static void g_cast(const orc_mesh *m, const float *camera, int w, int h, int max_bounce,
                    const float *bounds, const float *env_faces, int env_n, double *position,
@@ -671,8 +673,9 @@ static void g_cast(const orc_mesh *m, const float *camera, int w, int h, int max
         break;
       }
       const int mat = m->material_ids ? m->material_ids[hit] : 0;
-      const float *mr = m->materials + ORC_MAT_FLOATS * (size_t)mat;
       const int *tri = m->indices + 3 * (size_t)hit;
+      float mbuf[ORC_MAT_FLOATS];
+      const float *mr = g_material_row(m, mat, tri, u, v, mbuf);
       double ns[3], ng[3];
       {
         const float *A = m->vertices + 3 * (size_t)tri[0];
@@ -987,6 +990,67 @@ typedef struct {
 /* Material row (ORC_MAT_FLOATS values): kd[3], ks[3], roughness, type, then
  * specular_transmission, metallic, subsurface, specular, specular_tint,
  * anisotropic, sheen, sheen_tint, clearcoat, clearcoat_gloss, eta. */
+/* Texture sampling for the G-buffer producer (Renderer/textures.cpp:8-47,
+ * EightBitTexture::toCUDA): the IEC 61966-2-1 sRGB decode of a byte, and
+ * tex2D with normalized coordinates, wrap addressing and bilinear
+ * filtering, all in FP64. */
+static double g_srgb(int b) {
+  const double c = b / 255.0;
+  return c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4);
+}
+
+void orc_tex_sample(const orc_mesh *m, int t, double u, double v, double *out) {
+  const int w = m->tex_w[t], h = m->tex_h[t];
+  const uint8_t *px = m->tex_texels + m->tex_offset[t];
+  double fu = u - floor(u), fv = v - floor(v);
+  if (!(fu >= 0.0 && fu < 1.0)) fu = 0.0;
+  if (!(fv >= 0.0 && fv < 1.0)) fv = 0.0;
+  const double x = fu * w - 0.5, y = fv * h - 0.5, x0f = floor(x), y0f = floor(y);
+  const double ax = x - x0f, ay = y - y0f;
+  int x0 = (int)x0f, y0 = (int)y0f;
+  if (x0 < 0) x0 += w;
+  if (y0 < 0) y0 += h;
+  const int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
+  const int xs[4] = {x0, x1, x0, x1}, ys[4] = {y0, y0, y1, y1};
+  const double ws[4] = {(1 - ax) * (1 - ay), ax * (1 - ay), (1 - ax) * ay, ax * ay};
+  for (int c = 0; c < 4; ++c) {
+    double acc = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      const int b = px[((size_t)ys[k] * w + xs[k]) * 4 + c];
+      acc += ws[k] * (c < 3 ? g_srgb(b) : b / 255.0);
+    }
+    out[c] = acc;
+  }
+}
+
+/* The material row of a hit with its textured fields read at the hit's
+ * texture coordinates (1 - u - v) t0 + u t1 + v t2 (devicePrograms.cu:
+ * 66-71, 121-147); the row itself when the mesh has no textures. */
+static const float *g_material_row(const orc_mesh *m, int mat, const int *tri, double u,
+                                   double v, float *buf) {
+  const float *mr = m->materials + ORC_MAT_FLOATS * (size_t)mat;
+  if (!m->texcoords || !m->mat_tex) return mr;
+  memcpy(buf, mr, ORC_MAT_FLOATS * sizeof(float));
+  const double w = 1.0 - u - v;
+  double tc[2];
+  for (int a = 0; a < 2; ++a)
+    tc[a] = w * m->texcoords[2 * (size_t)tri[0] + a] + u * m->texcoords[2 * (size_t)tri[1] + a] +
+            v * m->texcoords[2 * (size_t)tri[2] + a];
+  for (int f = 0; f < ORC_TEX_FIELDS; ++f) {
+    const int t = m->mat_tex[(size_t)mat * ORC_TEX_FIELDS + f];
+    if (t < 0) continue;
+    double x[4];
+    orc_tex_sample(m, t, tc[0], tc[1], x);
+    if (f == 0 || f == 1)
+      for (int a = 0; a < 3; ++a) buf[3 * f + a] = (float)x[a];
+    else if (f == 2)
+      buf[6] = (float)x[0];
+    else
+      buf[8 + (f - 3)] = (float)x[0];
+  }
+  return buf;
+}
+
 static void g_surface_of(const float *mat, const double *ns, const double *ng, g_surface *s) {
   for (int a = 0; a < 3; ++a) {
     s->kd[a] = mat[a];
@@ -1910,7 +1974,8 @@ static void g_transfer_path(const orc_mesh *m, const float *faces, const double 
       for (int a = 0; a < 3; ++a) p[a] = o[a] + rd[a] * t + 0.001;
       for (int a = 0; a < 3; ++a) o[a] = p[a];
       const int mid = m->material_ids ? m->material_ids[hit] : 0;
-      const float *mr = m->materials + ORC_MAT_FLOATS * (size_t)mid;
+      float mbuf[ORC_MAT_FLOATS];
+      const float *mr = g_material_row(m, mid, tri, u, v, mbuf);
       if ((int)mr[7] == 3) {
         const double nd = g_dot(ns, rd);
         for (int a = 0; a < 3; ++a) d[a] = rd[a] - 2.0 * nd * ns[a];

@@ -127,7 +127,16 @@ void orc_denoise(const float *color, const float *normal, const float *albedo, i
  * clearcoat_gloss, eta (Renderer/Material.h:40-55). */
 #define ORC_MAT_FLOATS 19
 
-/* Triangle mesh for orc_gbuffer_cast.  materials: [n][ORC_MAT_FLOATS]. */
+/* Texture fields of a material (devicePrograms.cu:121-147): kd, ks (RGB),
+ * roughness, then the DisneyBSDF parameters and eta (first channel). */
+#define ORC_TEX_FIELDS 14
+
+/* Triangle mesh for orc_gbuffer_cast.  materials: [n][ORC_MAT_FLOATS].
+ * Optional textures (all four of texcoords, mat_tex, tex_texels and the
+ * sizes, or texcoords NULL): texcoords [n_vertices][2]; mat_tex
+ * [n_materials][ORC_TEX_FIELDS] texture indices or -1; texture t is
+ * tex_w[t] x tex_h[t] RGBA8 texels at tex_texels + tex_offset[t] (bytes),
+ * row 0 at v = 0. */
 typedef struct {
   const float *vertices;     /* [n_vertices][3]        */
   const float *normals;      /* [n_vertices][3] or NULL */
@@ -135,7 +144,17 @@ typedef struct {
   int64_t n_tris;
   const int32_t *material_ids; /* [n_tris] or NULL     */
   const float *materials;
+  const float *texcoords;
+  const int32_t *mat_tex;
+  const uint8_t *tex_texels;
+  const int64_t *tex_offset;
+  const int32_t *tex_w, *tex_h;
 } orc_mesh;
+
+/* tex2D of texture t at (u, v) as the reference's CUDA texture objects read
+ * 8-bit textures (Renderer/textures.cpp:8-47): wrap, bilinear between texel
+ * centres, b / 255 with the sRGB decode on RGB.  out[4]. */
+void orc_tex_sample(const orc_mesh *m, int t, double u, double v, double *out);
 
 /* First-hit G-buffer of a w x h view, FP64 brute force (one ray per pixel,
  * jitter at its mean).  camera: from[3], at[3], up[3]; bounds: lower[3],

@@ -412,3 +412,154 @@ def test_cast_disney_first_hit_encodings(nw, orc, ctx, seed, bounce):
     want = _oracle(orc, s, 48, 40, bounce, faces)
     n_hit, _ = _compare(got, want, TOL_ENC if bounce == 1 else 1e-3)
     assert n_hit > 100
+
+
+# ---------------------------------------------------------------- textures --
+# Material textures (nwprt_texture_create / nwprt_mesh_set_textures; the
+# reference's tex2D reads, devicePrograms.cu:66-71, 121-147) against the
+# oracle's FP64 sampling.  Texture coordinates are random per vertex over
+# [-1.5, 2.5] (so sampling wraps), the textures random bytes of odd sizes.
+# Bar for textured fields: within 1e-3 (FP32 bilinear at FP32 barycentrics
+# against FP64: on the 16-unit ground quad, whose random texture
+# coordinates span 4 wraps, the barycentrics' rounding moves a sample by up
+# to a few 1e-4; a wrong texel, channel or orientation errs by ~0.1).
+
+def _texture_set(rng):
+    return [rng.integers(0, 256, (6, 8, 4), dtype=np.uint8),     # 8 wide, 6 high
+            np.array([[[200, 120, 40, 255]]], np.uint8),          # 1 x 1
+            rng.integers(0, 256, (16, 16, 4), dtype=np.uint8),
+            rng.integers(0, 256, (5, 3, 4), dtype=np.uint8),
+            rng.integers(200, 256, (4, 4, 4), dtype=np.uint8)]   # eta of about 0.58-1
+
+
+def _textured_scene(nw, seed, kinds):
+    s = _direct_scene(seed, kinds=kinds)
+    rng = np.random.default_rng(100 + seed)
+    f = {n: i for i, n in enumerate(nw.TEX_FIELD_NAMES)}
+    tc = rng.uniform(-1.5, 2.5, (len(s["vertices"]), 2)).astype(np.float32)
+    mt = -np.ones((len(s["materials"]), nw.TEX_FIELDS), np.int32)
+    mt[0, f["kd"]], mt[0, f["roughness"]] = 0, 3
+    mt[1, f["ks"]], mt[1, f["kd"]] = 2, 1
+    mt[2, f["kd"]], mt[2, f["metallic"]], mt[2, f["specular_tint"]] = 2, 3, 1
+    mt[2, f["eta"]], mt[2, f["clearcoat"]], mt[2, f["sheen"]] = 4, 0, 3
+    return s, tc, mt, _texture_set(rng)
+
+
+def _textured_mesh(nw, ctx, s, tc, mt, texs):
+    mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
+                           material_ids=s["material_ids"])
+    ids = [ctx.texture_create(t) for t in texs]
+    ctx.mesh_set_textures(mesh, tc, np.where(mt >= 0, np.take(ids, np.maximum(mt, 0)), -1))
+    return mesh, ids
+
+
+@pytest.mark.parametrize("seed,w,h,kinds", [(20, 64, 48, None), (21, 41, 33, (1, 0, 1)),
+                                            (22, 1, 1, (2, 2, 2))])
+def test_cast_textured_matches_oracle(nw, orc, ctx, seed, w, h, kinds):
+    s, tc, mt, texs = _textured_scene(nw, seed, kinds)
+    mesh, ids = _textured_mesh(nw, ctx, s, tc, mt, texs)
+    frame = nw.DeviceFrame(w, h)
+    ctx.gbuffer_cast(mesh, np.reshape(s["camera"], (3, 3)), frame, bounds=s["bounds"])
+    ctx.sync()
+    got = _download(frame)
+    want = orc.gbuffer_cast(s["vertices"], s["indices"], s["materials"], s["camera"], w, h,
+                            normals=s["normals"], material_ids=s["material_ids"],
+                            bounds=s["bounds"], texcoords=tc, mat_tex=mt, textures=texs)
+    safe = want["margin"] >= MARGIN
+    assert safe.mean() >= 0.98
+    assert np.array_equal(got["env_flag"][safe], want["env_flag"][safe])
+    hit = safe & (want["env_flag"] == 1)
+    for k in ("kd", "ks"):
+        err = np.abs(got[k][hit, :3] - want[k][hit, :3]).max() if hit.any() else 0.0
+        assert err <= 1e-3, (k, err)
+    assert np.abs(got["roughness"][hit] - want["roughness"][hit]).max(initial=0.0) <= 1e-3
+    for k in ("position", "normal", "reflect"):
+        err = (np.abs(got[k][safe] - want[k][safe]) /
+               np.maximum(1.0, np.abs(want[k][safe]))).max() if safe.any() else 0.0
+        assert err <= TOL_ENC, (k, err)
+    if w * h > 100:  # the textures are in use: kd varies across the hits
+        assert len(np.unique(np.round(got["kd"][hit, 0], 3))) > 10
+    ctx.mesh_destroy(mesh)
+    for t in ids:
+        ctx.texture_destroy(t)
+
+
+@pytest.mark.parametrize("seed,w,h,samples,kinds", [(30, 32, 24, 48, None),
+                                                    (31, 29, 17, 32, (1, 0, 2))])
+def test_direct_light_textured_replays_oracle(nw, orc, ctx, seed, w, h, samples, kinds):
+    """The direct term with textured materials, same bar as the untextured
+    replay (98 % of pixels within 1e-3 of |oracle| + 1e-2 max)."""
+    s, tc, mt, texs = _textured_scene(nw, seed, kinds)
+    mesh, ids = _textured_mesh(nw, ctx, s, tc, mt, texs)
+    rng = np.random.default_rng(seed)
+    light = rng.lognormal(0, 1, (6, 8, 8, 3)).astype(np.float32)
+    light[2, 3, 4] = 200.0
+    frame = nw.DeviceFrame(w, h)
+    dl = nw.DeviceBuffer.from_array(light)
+    ctx.gbuffer_cast(mesh, np.reshape(s["camera"], (3, 3)), frame, bounds=s["bounds"],
+                     light_faces=dl, light_n=8, direct_samples=samples, seed=77 + seed)
+    ctx.sync()
+    got = frame.bufs["direct"].download(np.float32, (w * h, 4)).astype(np.float64)
+    want = orc.gbuffer_cast(s["vertices"], s["indices"], s["materials"], s["camera"], w, h,
+                            normals=s["normals"], material_ids=s["material_ids"],
+                            bounds=s["bounds"], light_faces=light, samples=samples,
+                            seed=77 + seed, texcoords=tc, mat_tex=mt, textures=texs)
+    ref = want["direct"]
+    safe = want["margin"] >= MARGIN
+    top = max(np.abs(ref[:, :3]).max(), 1e-6)
+    err = np.abs(got[safe, :3] - ref[safe, :3]) / (np.abs(ref[safe, :3]) + 1e-2 * top)
+    per_pixel = err.max(axis=1)
+    print(f"textured direct seed {seed}: within 1e-3 {np.mean(per_pixel <= 1e-3):.4f}, "
+          f"worst {per_pixel.max():.2e}")
+    assert np.mean(per_pixel <= 1e-3) >= 0.98
+    assert np.abs(got[safe, :3] - ref[safe, :3]).max() <= 0.25 * top
+    assert (ref[want["env_flag"] == 1, :3] > 0).any()
+    ctx.mesh_destroy(mesh)
+    for t in ids:
+        ctx.texture_destroy(t)
+
+
+def test_texture_lifetime_and_errors(nw, ctx):
+    s, tc, mt, texs = _textured_scene(nw, 40, (2, 2, 2))
+    mesh, ids = _textured_mesh(nw, ctx, s, tc, mt, texs)
+    with pytest.raises(nw.NwprtError) as e:  # held by the mesh
+        ctx.texture_destroy(ids[0])
+    assert e.value.status == nw.ERR_STATE
+    bad = np.where(mt >= 0, 987654, -1).astype(np.int32)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.mesh_set_textures(mesh, tc, bad)
+    assert e.value.status == nw.ERR_NOT_FOUND
+    with pytest.raises(nw.NwprtError) as e:  # the mesh keeps its textures
+        ctx.texture_destroy(ids[0])
+    assert e.value.status == nw.ERR_STATE
+    frame = nw.DeviceFrame(8, 8)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.texture_create(np.zeros((0, 4, 4), np.uint8))
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.mesh_set_textures(mesh, np.full_like(tc, np.nan), np.where(mt >= 0, ids[0], -1))
+    assert e.value.status == nw.ERR_INVALID_ARGUMENT
+    ctx.mesh_set_textures(mesh, None, None)  # untextured again: the textures are free
+    for t in ids:
+        ctx.texture_destroy(t)
+    ctx.gbuffer_cast(mesh, np.reshape(s["camera"], (3, 3)), frame, bounds=s["bounds"])
+    ctx.sync()
+    ctx.mesh_destroy(mesh)
+
+
+def test_sh_render_rejects_textured_meshes(nw, ctx):
+    s = mesh_scene(41, mirror=False)
+    mesh = ctx.mesh_create(s["vertices"], s["indices"], s["materials"], normals=s["normals"],
+                           material_ids=s["material_ids"])
+    tid = ctx.texture_create(np.full((2, 2, 4), 128, np.uint8))
+    mt = -np.ones((len(s["materials"]), nw.TEX_FIELDS), np.int32)
+    mt[0, 0] = tid
+    ctx.mesh_set_textures(mesh, np.zeros((len(s["vertices"]), 2), np.float32), mt)
+    tm = nw.DeviceBuffer(len(s["vertices"]) * 9 * 3 * 4)
+    sh = nw.DeviceBuffer.from_array(np.ones((9, 3), np.float32))
+    out = nw.DeviceBuffer(8 * 8 * 16)
+    with pytest.raises(nw.NwprtError) as e:
+        ctx.sh_render(mesh, np.reshape(s["camera"], (3, 3)), 8, 8, tm, sh, 2, out, brdf_samples=4)
+    assert e.value.status == nw.ERR_UNSUPPORTED
+    ctx.mesh_destroy(mesh)
+    ctx.texture_destroy(tid)

@@ -220,6 +220,33 @@ def test_transfer_launch_chunks_equal_single_vertex_calls(nw, ctx):
     ctx.mesh_destroy(mesh)
 
 
+def test_transfer_textured_replays_oracle(nw, orc, ctx):
+    """Textured materials in the transfer paths: each path vertex reads its
+    material's textures at the hit's texture coordinates (kd and roughness
+    of the cookTorrance floor, ks and kd of the roughPlastic ceiling); same
+    bars as the untextured replay."""
+    v, ix, mids, mats, pts = panel_scene()
+    rng = np.random.default_rng(17)
+    tc = rng.uniform(-1.0, 2.0, (len(v), 2)).astype(np.float32)
+    texs = [rng.integers(0, 256, (5, 7, 4), dtype=np.uint8),
+            rng.integers(0, 256, (3, 2, 4), dtype=np.uint8),
+            rng.integers(0, 256, (8, 8, 4), dtype=np.uint8)]
+    mt = -np.ones((len(mats), nw.TEX_FIELDS), np.int32)
+    mt[0, 0], mt[0, 2] = 0, 1      # kd, roughness
+    mt[1, 1], mt[1, 0] = 2, 1      # ks, kd
+    basis = smooth_basis(9, seed=21)
+    mesh = _mesh(ctx, v, ix, mats, mids)
+    ids = [ctx.texture_create(t) for t in texs]
+    ctx.mesh_set_textures(mesh, tc, np.where(mt >= 0, np.take(ids, np.maximum(mt, 0)), -1))
+    got = _transfer(nw, ctx, mesh, basis, 2, len(v), samples=256, max_bounce=12, seed=8)
+    ref = orc.sh_transfer(v, ix, mats, basis, 2, material_ids=mids, samples=256, max_bounce=12,
+                          seed=8, verts=pts, texcoords=tc, mat_tex=mt, textures=texs)
+    _compare(got[pts], ref, "textured deg 2")
+    ctx.mesh_destroy(mesh)
+    for t in ids:
+        ctx.texture_destroy(t)
+
+
 def test_sh_render_reads_computed_transfer(nw, orc, ctx):
     """The SH-PRT render on matrices computed here (the reference's
     shComputeTMatrix -> shLoadTMatrix -> shRender chain) equals the oracle's
