@@ -524,15 +524,30 @@ nwprt_status nwprt_mesh_create(nwprt_ctx ctx, const float *vertices, int64_t n_v
                                int32_t *mesh_id);
 nwprt_status nwprt_mesh_get_info(nwprt_ctx ctx, int32_t mesh_id, nwprt_mesh_info *out);
 /* Host utility (no GPU): Wavefront OBJ (+ MTL) reader with what loadOBJ
- * (Renderer/Model.cpp:73-146) keeps: polygons fanned into triangles, one
- * vertex per distinct (position, normal) corner, materials as cookTorrance
- * (type 2: kd = Kd, ks = Ks, roughness = (1 / Ns)^2, 0 for Ns = 0; faces
- * without a material: kd = ks = 0.5, roughness 1).  *normals is NULL when
+ * (Renderer/Model.cpp:73-146) keeps: polygons fanned into triangles,
+ * materials as cookTorrance (type 2: kd = Kd, ks = Ks, roughness =
+ * (1 / Ns)^2, 0 for Ns = 0; faces without a material: kd = ks = 0.5,
+ * roughness 1).  Vertices in the reference's scene order: one mesh per
+ * (o / g shape, material), materials ascending, each mesh's distinct
+ * (position, normal, texcoord) corners by first use (addVertex,
+ * Model.cpp:34-70), the meshes one after the other.  *normals is NULL when
  * the file has none.  Arrays are released with nwprt_free_host. */
 nwprt_status nwprt_obj_read(const char *path, float **vertices, int64_t *n_vertices,
                             float **normals, int32_t **indices, int64_t *n_triangles,
                             int32_t **material_ids, nwprt_material **materials,
                             int32_t *n_materials);
+/* Host utility (no GPU): nwprt_obj_read plus the texture coordinates
+ * ([n_vertices][2], NULL when the file has none) and, per material, the
+ * files of its kd, ks and roughness maps (map_Kd, map_Ks, map_Ns;
+ * Model.cpp:133-135) resolved against the OBJ's folder: *map_paths holds
+ * n_materials x 3 NUL-terminated strings, "" for no map (*map_bytes in
+ * all).  The reference loads a map only when its file exists
+ * (textureManager::addTexture, textures.cpp:129-152). */
+nwprt_status nwprt_obj_read_textured(const char *path, float **vertices, int64_t *n_vertices,
+                                     float **normals, float **texcoords, int32_t **indices,
+                                     int64_t *n_triangles, int32_t **material_ids,
+                                     nwprt_material **materials, int32_t *n_materials,
+                                     char **map_paths, int64_t *map_bytes);
 /* Host utility (no GPU): Stanford PLY reader (ASCII or binary little-endian)
  * with what loadPly (Renderer/Model.cpp:152-220) keeps: vertex x y z,
  * optional nx ny nz, face vertex_indices (polygons fanned).  One material:

@@ -238,6 +238,10 @@ _SIGNATURES = {
     "nwprt_obj_read": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int32)]),
+    "nwprt_obj_read_textured": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64),
+                                        POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p),
+                                        POINTER(c_int64), POINTER(c_void_p), POINTER(c_void_p),
+                                        POINTER(c_int32), POINTER(c_void_p), POINTER(c_int64)]),
     "nwprt_ply_read": (c_int, [c_char_p, POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
                                POINTER(c_void_p), POINTER(c_int32)]),
@@ -380,6 +384,30 @@ def obj_read(path: str) -> Dict[str, np.ndarray]:
     or None, indices [t][3], material_ids [t], materials [m][19] rows
     (kd[3], ks[3], roughness, type, the DisneyBSDF parameters, eta)."""
     return _mesh_read(lib().nwprt_obj_read, path)
+
+
+def obj_read_textured(path: str) -> Dict[str, np.ndarray]:
+    """obj_read plus texcoords [n][2] (or None) and maps: per material the
+    kd / ks / roughness map files (map_Kd, map_Ks, map_Ns; "" for none)."""
+    v, nr, tc, ix, mid, mats, mp = (c_void_p() for _ in range(7))
+    nv, nt, nm, nb = c_int64(), c_int64(), c_int32(), c_int64()
+    _check(lib().nwprt_obj_read_textured(str(path).encode(), ctypes.byref(v), ctypes.byref(nv),
+                                         ctypes.byref(nr), ctypes.byref(tc), ctypes.byref(ix),
+                                         ctypes.byref(nt), ctypes.byref(mid), ctypes.byref(mats),
+                                         ctypes.byref(nm), ctypes.byref(mp), ctypes.byref(nb)))
+    out = {"vertices": _take_host(v, nv.value * 3, np.float32).reshape(-1, 3),
+           "normals": (_take_host(nr, nv.value * 3, np.float32).reshape(-1, 3)
+                       if nr.value else None),
+           "texcoords": (_take_host(tc, nv.value * 2, np.float32).reshape(-1, 2)
+                         if tc.value else None),
+           "indices": _take_host(ix, nt.value * 3, np.int32).reshape(-1, 3),
+           "material_ids": _take_host(mid, nt.value, np.int32)}
+    raw = _take_host(mats, nm.value * MATERIAL_ROW, np.float32).reshape(-1, MATERIAL_ROW)
+    out["materials"] = np.concatenate(
+        [raw[:, :7], raw[:, 7:8].view(np.int32).astype(np.float32), raw[:, 8:]], axis=1)
+    names = _take_host(mp, nb.value, np.uint8).tobytes().split(b"\0")[:-1]
+    out["maps"] = [[n.decode() for n in names[3 * i:3 * i + 3]] for i in range(nm.value)]
+    return out
 
 
 def ply_read(path: str) -> Dict[str, np.ndarray]:

@@ -1,8 +1,10 @@
 // C ABI of the host-only utilities (no GPU use): level table, parameter
 // counts, seeded inputs and file formats (npy, hdr, cube crosses, OBJ).
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
+#include <limits>
 #include <map>
 #include <sstream>
 
@@ -169,19 +171,29 @@ nwprt_status nwprt_cubemap_cross_to_faces(const float *cross, int32_t width, int
 // ------------------------------------------------------------------ OBJ --
 // Wavefront OBJ + MTL reader for the G-buffer producer's meshes, following
 // what loadOBJ (Renderer/Model.cpp:73-146, tinyobjloader with triangulation)
-// keeps: positions, normals, polygon faces fanned into triangles, one
-// vertex per distinct (position, normal) pair, and per-face materials made
-// into cookTorrance materials (kd = Kd, ks = Ks, roughness = (1 / Ns)^2, 0
-// for Ns = 0; faces without a material get cookTorranceMaterial's defaults
-// kd = ks = 0.5, roughness 1, Renderer/Material.h:136-143).  Texture maps
-// are not read.
+// keeps: positions, normals, texture coordinates, polygon faces fanned into
+// triangles, and per-face materials made into cookTorrance materials (kd =
+// Kd, ks = Ks, roughness = (1 / Ns)^2, 0 for Ns = 0; faces without a
+// material get cookTorranceMaterial's defaults kd = ks = 0.5, roughness 1,
+// Renderer/Material.h:136-143) with their map_Kd / map_Ks / map_Ns files
+// (Model.cpp:133-135).  Vertices come in the reference's order: one mesh
+// per (shape, material), shapes split at o / g lines, materials ascending,
+// each mesh numbering its distinct (position, normal, texcoord) corners by
+// first use (addVertex, Model.cpp:34-70, normals and texcoords back-filled
+// the same way).  The reference keeps one corner map per shape across its
+// materials' meshes, so a corner shared by two materials of one shape
+// indexes the first mesh's vertices from the second; here each mesh keys
+// its own corners.
 namespace {
 
 struct ObjMesh {
-  std::vector<float> vertices, normals;
+  std::vector<float> vertices, normals, texcoords;
   std::vector<int32_t> indices, material_ids;
   std::vector<nwprt_material> materials;
-  bool has_normals = false;
+  // Per material: the kd, ks and roughness map files (map_Kd, map_Ks,
+  // map_Ns), resolved against the OBJ's folder; empty for none.
+  std::vector<std::string> maps;
+  bool has_normals = false, has_texcoords = false;
 };
 
 }  // namespace
@@ -246,8 +258,8 @@ std::vector<std::string> split_ws(const std::string &line) {
   return out;
 }
 
-void read_mtl(const std::string &path, std::map<std::string, int> &names,
-              std::vector<nwprt_material> &mats) {
+void read_mtl(const std::string &path, const std::string &dir, std::map<std::string, int> &names,
+              std::vector<nwprt_material> &mats, std::vector<std::string> &maps) {
   std::ifstream in(path);
   require((bool)in, NWPRT_ERR_IO, "obj: cannot open material library " + path);
   int cur = -1;
@@ -263,15 +275,108 @@ void read_mtl(const std::string &path, std::map<std::string, int> &names,
       names[t[1]] = cur = (int)mats.size();
       // tinyobjloader's material defaults: Kd = Ks = 0, Ns = 1.
       mats.push_back(cook_torrance(0, 0, 0, 0, 0, 0, 1.f));
+      maps.resize(mats.size() * 3);
       ns = 1.f;
     } else if (cur >= 0 && (t[0] == "Kd" || t[0] == "Ks") && t.size() >= 4) {
       float *dst = t[0] == "Kd" ? mats[(size_t)cur].kd : mats[(size_t)cur].ks;
       for (int k = 0; k < 3; ++k) dst[k] = std::stof(t[(size_t)k + 1]);
     } else if (cur >= 0 && t[0] == "Ns" && t.size() >= 2) {
       ns = std::stof(t[1]);
+    } else if (cur >= 0 && (t[0] == "map_Kd" || t[0] == "map_Ks" || t[0] == "map_Ns") &&
+               t.size() >= 2) {
+      // The file name is the last token (options such as -s come first).
+      const int slot = t[0] == "map_Kd" ? 0 : t[0] == "map_Ks" ? 1 : 2;
+      const std::string &f = t.back();
+      maps[(size_t)cur * 3 + slot] = f.size() && f[0] == '/' ? f : dir + f;
     }
   }
   close();
+}
+
+struct Corner {
+  int v, t, n;  // position, texcoord, normal (-1: none)
+  bool operator<(const Corner &o) const {  // tinyobj::index_t order
+    if (v != o.v) return v < o.v;
+    if (n != o.n) return n < o.n;
+    return t < o.t;
+  }
+};
+
+// W. R. Franklin's crossing-number test: is (x, y) inside the polygon?
+bool inside(int n, const float *px, const float *py, float x, float y) {
+  bool c = false;
+  for (int i = 0, j = n - 1; i < n; j = i++)
+    if ((py[i] > y) != (py[j] > y) && x < (px[j] - px[i]) * (y - py[i]) / (py[j] - py[i]) + px[i])
+      c = !c;
+  return c;
+}
+
+// Triangles of one face as tinyobjloader's LoadObj(triangulate = true)
+// makes them (common/3rdParty/tiny_obj_loader.h:1385-1590): a triangle
+// stays as it is; a polygon is projected on the two axes the normal of its
+// first non-degenerate corner favours and cut by ear clipping from its first
+// corner on (a convex polygon becomes the fan of its first corner).  A
+// polygon that runs out of ears keeps the triangles cut so far.
+std::vector<std::array<Corner, 3>> triangulate(const std::vector<Corner> &poly,
+                                               const std::vector<float> &pos) {
+  std::vector<std::array<Corner, 3>> out;
+  const size_t n = poly.size();
+  if (n == 3) {
+    out.push_back({poly[0], poly[1], poly[2]});
+    return out;
+  }
+  auto p = [&](const Corner &c, int axis) { return pos[3 * (size_t)c.v + (size_t)axis]; };
+  int ax = 1, ay = 2;  // projection axes
+  for (size_t k = 0; k < n; ++k) {
+    const Corner &a = poly[k], &b = poly[(k + 1) % n], &c = poly[(k + 2) % n];
+    float e0[3], e1[3];
+    for (int i = 0; i < 3; ++i) e0[i] = p(b, i) - p(a, i), e1[i] = p(c, i) - p(b, i);
+    const float cx = std::fabs(e0[1] * e1[2] - e0[2] * e1[1]);
+    const float cy = std::fabs(e0[2] * e1[0] - e0[0] * e1[2]);
+    const float cz = std::fabs(e0[0] * e1[1] - e0[1] * e1[0]);
+    const float eps = std::numeric_limits<float>::epsilon();
+    if (cx > eps || cy > eps || cz > eps) {
+      if (!(cx > cy && cx > cz)) ax = 0, ay = cz > cx && cz > cy ? 1 : 2;
+      break;
+    }
+  }
+  float area = 0.f;  // signed area of the projection
+  for (size_t k = 0; k < n; ++k) {
+    const Corner &a = poly[k], &b = poly[(k + 1) % n];
+    area += (p(a, ax) * p(b, ay) - p(a, ay) * p(b, ax)) * 0.5f;
+  }
+  std::vector<Corner> rest = poly;
+  size_t guess = 0, budget = n, last = n;
+  while (rest.size() > 3 && budget > 0) {
+    const size_t m = rest.size();
+    if (guess >= m) guess -= m;
+    if (last != m) {
+      last = m;
+      budget = m;
+    } else {
+      --budget;
+    }
+    Corner tri[3];
+    float x[3], y[3];
+    for (int k = 0; k < 3; ++k) {
+      tri[k] = rest[(guess + (size_t)k) % m];
+      x[k] = p(tri[k], ax), y[k] = p(tri[k], ay);
+    }
+    const float turn = (x[1] - x[0]) * (y[2] - y[1]) - (y[1] - y[0]) * (x[2] - x[1]);
+    bool ear = !(turn * area < 0.f);  // a reflex corner is no ear
+    for (size_t o = 3; ear && o < m; ++o) {
+      const Corner &q = rest[(guess + o) % m];
+      if (inside(3, x, y, p(q, ax), p(q, ay))) ear = false;
+    }
+    if (!ear) {
+      ++guess;
+      continue;
+    }
+    out.push_back({tri[0], tri[1], tri[2]});
+    rest.erase(rest.begin() + (long)((guess + 1) % m));
+  }
+  if (rest.size() == 3) out.push_back({rest[0], rest[1], rest[2]});
+  return out;
 }
 
 ObjMesh read_obj(const std::string &path) {
@@ -281,15 +386,17 @@ ObjMesh read_obj(const std::string &path) {
     const size_t s = path.find_last_of('/');
     return s == std::string::npos ? std::string() : path.substr(0, s + 1);
   }();
-  std::vector<float> pos, nrm;
+  std::vector<float> pos, nrm, tex;
   std::map<std::string, int> names;
   std::vector<nwprt_material> mats;
-  std::map<std::pair<int, int>, int32_t> known;
-  ObjMesh m;
-  int cur_mat = -1, default_mat = -1;
-  bool any_normal = false;
-  std::vector<std::pair<int, int>> corner_keys;
-  std::vector<int> face_mat;
+  std::vector<std::string> maps;
+  struct Face {
+    Corner c[3];
+    int shape, mat;
+  };
+  std::vector<Face> faces;
+  int cur_mat = -1, shape = 0;
+  bool shape_has_faces = false;
   auto resolve = [](long v, size_t count) -> int {
     const long n = (long)count;
     const long i = v > 0 ? v - 1 : n + v;
@@ -306,32 +413,35 @@ ObjMesh read_obj(const std::string &path) {
         for (int k = 1; k <= 3; ++k) pos.push_back(std::stof(t[(size_t)k]));
       } else if (t[0] == "vn" && t.size() >= 4) {
         for (int k = 1; k <= 3; ++k) nrm.push_back(std::stof(t[(size_t)k]));
+      } else if (t[0] == "vt" && t.size() >= 3) {
+        for (int k = 1; k <= 2; ++k) tex.push_back(std::stof(t[(size_t)k]));
       } else if (t[0] == "mtllib" && t.size() > 1) {
-        read_mtl(dir + t[1], names, mats);
+        read_mtl(dir + t[1], dir, names, mats, maps);
       } else if (t[0] == "usemtl" && t.size() > 1) {
         auto it = names.find(t[1]);
         cur_mat = it == names.end() ? -1 : it->second;
+      } else if (t[0] == "o" || t[0] == "g") {
+        // tinyobj starts a new shape at an object or group line.
+        if (shape_has_faces) ++shape;
+        shape_has_faces = false;
       } else if (t[0] == "f" && t.size() >= 4) {
-        std::vector<std::pair<int, int>> poly;
+        std::vector<Corner> poly;
         for (size_t c = 1; c < t.size(); ++c) {
           const std::string &s = t[c];
           const size_t a = s.find('/');
-          const int vi = resolve(std::stol(s.substr(0, a)), pos.size() / 3);
-          int ni = -1;
+          Corner k{resolve(std::stol(s.substr(0, a)), pos.size() / 3), -1, -1};
           if (a != std::string::npos) {
             const size_t b = s.find('/', a + 1);
+            const std::string ts = s.substr(a + 1, b == std::string::npos ? std::string::npos : b - a - 1);
+            if (!ts.empty()) k.t = resolve(std::stol(ts), tex.size() / 2);
             if (b != std::string::npos && b + 1 < s.size())
-              ni = resolve(std::stol(s.substr(b + 1)), nrm.size() / 3);
+              k.n = resolve(std::stol(s.substr(b + 1)), nrm.size() / 3);
           }
-          if (ni >= 0) any_normal = true;
-          poly.emplace_back(vi, ni);
+          poly.push_back(k);
         }
-        for (size_t c = 1; c + 1 < poly.size(); ++c) {
-          corner_keys.push_back(poly[0]);
-          corner_keys.push_back(poly[c]);
-          corner_keys.push_back(poly[c + 1]);
-          face_mat.push_back(cur_mat);
-        }
+        for (const auto &tri : triangulate(poly, pos))
+          faces.push_back(Face{{tri[0], tri[1], tri[2]}, shape, cur_mat});
+        shape_has_faces = true;
       }
     } catch (const Error &) {
       throw;
@@ -339,32 +449,75 @@ ObjMesh read_obj(const std::string &path) {
       fail(NWPRT_ERR_FORMAT, "obj: bad line " + std::to_string(line_no) + " of " + path);
     }
   }
-  require(!face_mat.empty(), NWPRT_ERR_FORMAT, "obj: no faces in " + path);
-  for (int &fm : face_mat)
-    if (fm < 0) {
-      if (default_mat < 0) {
-        default_mat = (int)mats.size();
-        mats.push_back(cook_torrance(0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 1.f));
+  require(!faces.empty(), NWPRT_ERR_FORMAT, "obj: no faces in " + path);
+  // Faces without a material take cookTorrance's defaults (kd = ks = 0.5,
+  // roughness 1), as one more material.
+  int default_mat = -1;
+  for (const Face &f : faces)
+    if (f.mat < 0 && default_mat < 0) {
+      default_mat = (int)mats.size();
+      mats.push_back(cook_torrance(0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 0.5f, 1.f));
+      maps.resize(mats.size() * 3);
+    }
+  maps.resize(mats.size() * 3);
+  // Meshes in the reference's order: per shape, one mesh per material in
+  // ascending material id (faces without one first), faces in file order;
+  // each mesh numbers its vertices by first use of a distinct (position,
+  // normal, texcoord) corner, and the scene's vertices are the meshes'
+  // vertices one after the other (Model.cpp:73-146, OptixBackend.cpp:700-752).
+  ObjMesh m;
+  const bool any_n = [&] {
+    for (const Face &f : faces)
+      for (const Corner &c : f.c)
+        if (c.n >= 0) return true;
+    return false;
+  }();
+  const bool any_t = [&] {
+    for (const Face &f : faces)
+      for (const Corner &c : f.c)
+        if (c.t >= 0) return true;
+    return false;
+  }();
+  std::map<std::pair<int, int>, std::vector<size_t>> groups;  // (shape, material) -> faces
+  for (size_t i = 0; i < faces.size(); ++i) groups[{faces[i].shape, faces[i].mat}].push_back(i);
+  for (const auto &g : groups) {  // shape, then material id (-1 first)
+    std::map<Corner, int32_t> known;
+    const size_t first = m.vertices.size() / 3;
+    std::vector<float> gn, gt;  // this mesh's normals / texcoords, back-filled
+    for (size_t fi : g.second) {
+      for (const Corner &c : faces[fi].c) {
+        auto it = known.find(c);
+        if (it == known.end()) {
+          const int32_t id = (int32_t)(m.vertices.size() / 3);
+          for (int k = 0; k < 3; ++k) m.vertices.push_back(pos[3 * (size_t)c.v + k]);
+          const size_t local = (size_t)id - first + 1;
+          // addVertex: a corner with a normal / texcoord fills every vertex
+          // of the mesh so far that has none; the arrays then match the
+          // vertex count (zeros for trailing vertices without one).
+          if (c.n >= 0)
+            while (gn.size() < 3 * local)
+              for (int k = 0; k < 3; ++k) gn.push_back(nrm[3 * (size_t)c.n + k]);
+          if (c.t >= 0)
+            while (gt.size() < 2 * local)
+              for (int k = 0; k < 2; ++k) gt.push_back(tex[2 * (size_t)c.t + k]);
+          if (!gt.empty()) gt.resize(2 * local, 0.f);
+          if (!gn.empty()) gn.resize(3 * local, 0.f);
+          it = known.emplace(c, id).first;
+        }
+        m.indices.push_back(it->second);
       }
-      fm = default_mat;
+      m.material_ids.push_back(g.first.second < 0 ? default_mat : g.first.second);
     }
-  for (const auto &key : corner_keys) {
-    auto it = known.find(key);
-    if (it == known.end()) {
-      const int32_t id = (int32_t)(m.vertices.size() / 3);
-      for (int k = 0; k < 3; ++k) m.vertices.push_back(pos[3 * (size_t)key.first + k]);
-      if (any_normal)
-        for (int k = 0; k < 3; ++k)
-          // Corners without a normal take the zero vector (geometric
-          // fallback is per triangle, not per vertex).
-          m.normals.push_back(key.second >= 0 ? nrm[3 * (size_t)key.second + k] : 0.f);
-      it = known.emplace(key, id).first;
-    }
-    m.indices.push_back(it->second);
+    const size_t count = m.vertices.size() / 3 - first;
+    gn.resize(3 * count, 0.f);
+    gt.resize(2 * count, 0.f);
+    if (any_n) m.normals.insert(m.normals.end(), gn.begin(), gn.end());
+    if (any_t) m.texcoords.insert(m.texcoords.end(), gt.begin(), gt.end());
   }
-  m.material_ids.assign(face_mat.begin(), face_mat.end());
   m.materials = mats;
-  m.has_normals = any_normal;
+  m.maps = maps;
+  m.has_normals = any_n;
+  m.has_texcoords = any_t;
   return m;
 }
 
@@ -588,6 +741,36 @@ extern "C" nwprt_status nwprt_image_read_rgba8(const char *path, uint8_t **rgba,
     *rgba = copy_out(img.rgba);
     *width = img.width;
     *height = img.height;
+  });
+}
+
+extern "C" nwprt_status nwprt_obj_read_textured(const char *path, float **vertices,
+                                                int64_t *n_vertices, float **normals,
+                                                float **texcoords, int32_t **indices,
+                                                int64_t *n_triangles, int32_t **material_ids,
+                                                nwprt_material **materials, int32_t *n_materials,
+                                                char **map_paths, int64_t *map_bytes) {
+  return host_guarded([&] {
+    require(path && vertices && n_vertices && normals && texcoords && indices && n_triangles &&
+                material_ids && materials && n_materials && map_paths && map_bytes,
+            NWPRT_ERR_INVALID_ARGUMENT, "null argument");
+    const ObjMesh m = read_obj(path);
+    std::vector<char> block;
+    for (const std::string &f : m.maps) {
+      block.insert(block.end(), f.begin(), f.end());
+      block.push_back('\0');
+    }
+    *vertices = copy_out(m.vertices);
+    *normals = m.has_normals ? copy_out(m.normals) : nullptr;
+    *texcoords = m.has_texcoords ? copy_out(m.texcoords) : nullptr;
+    *indices = copy_out(m.indices);
+    *material_ids = copy_out(m.material_ids);
+    *materials = copy_out(m.materials);
+    *map_paths = copy_out(block);
+    *map_bytes = (int64_t)block.size();
+    *n_vertices = (int64_t)(m.vertices.size() / 3);
+    *n_triangles = (int64_t)(m.indices.size() / 3);
+    *n_materials = (int32_t)m.materials.size();
   });
 }
 

@@ -16,7 +16,9 @@
 // the direct-light buffer gets samplesPerPixel x directSamplesPerLight
 // estimates per pixel (trajectory keywords, defaults 1024 x 1 as
 // renderConfig's, Renderer/RenderEngine.h:30-31) with random stream
-// (--seed + i).  Without --obj, frame i's G-buffer is the seeded synthetic
+// (--seed + i).  An OBJ's material maps (map_Kd, map_Ks, map_Ns of its
+// MTL, relative to the OBJ's folder) become textures of the mesh when the
+// files exist and the OBJ has texture coordinates.  Without --obj, frame i's G-buffer is the seeded synthetic
 // one with seed (--seed + i) and the camera columns are parsed and ignored.
 //
 //   nwprt_offline <results_dir> -w <model_dir> -m <env_folder> -t <trajectory>
@@ -28,6 +30,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <sstream>
 #include <vector>
 
@@ -105,6 +108,48 @@ void writeFlipped(const fs::path &file, std::vector<float> rgba, int w, int h) {
     throw std::runtime_error(std::string("writing ") + file.string() + ": " + nwprt_last_error());
 }
 
+// Texture ids per material for the kd / ks / roughness maps the MTL names,
+// as textureManager::addTexture gives them (Renderer/textures.cpp:134-153,
+// Model.cpp:133-136): a name that is no file, or a file that does not
+// decode, leaves the field constant (-1); one texture per distinct absolute
+// file name.
+std::vector<int32_t> loadTextures(nwprt_ctx ctx, const char *maps, int64_t bytes, int32_t nm,
+                                  std::map<std::string, int32_t> &known) {
+  std::vector<std::string> names;
+  for (int64_t i = 0; i < bytes;) {
+    names.emplace_back(maps + i);
+    i += (int64_t)names.back().size() + 1;
+  }
+  if ((int64_t)names.size() != 3 * (int64_t)nm)
+    throw std::runtime_error("material map list does not match the materials");
+  const int field[3] = {NWPRT_TEX_KD, NWPRT_TEX_KS, NWPRT_TEX_ROUGHNESS};
+  std::vector<int32_t> ids((size_t)nm * NWPRT_TEX_FIELDS, -1);
+  for (int32_t m = 0; m < nm; ++m)
+    for (int slot = 0; slot < 3; ++slot) {
+      const std::string &name = names[(size_t)m * 3 + (size_t)slot];
+      std::error_code ec;
+      if (name.empty() || !fs::is_regular_file(name, ec)) continue;
+      const std::string key = fs::absolute(name, ec).string();
+      auto it = known.find(key);
+      if (it == known.end()) {
+        uint8_t *rgba = nullptr;
+        int32_t w = 0, h = 0, id = -1;
+        if (nwprt_image_read_rgba8(key.c_str(), &rgba, &w, &h) != NWPRT_OK) {
+          std::cerr << "nwprt_offline: texture " << key << " not read: " << nwprt_last_error()
+                    << "\n";
+          continue;
+        }
+        const nwprt_status st = nwprt_texture_create(ctx, rgba, w, h, &id);
+        nwprt_free_host(rgba);
+        if (st != NWPRT_OK)
+          throw std::runtime_error("texture " + key + ": " + nwprt_last_error());
+        it = known.emplace(key, id).first;
+      }
+      ids[(size_t)m * NWPRT_TEX_FIELDS + (size_t)field[slot]] = it->second;
+    }
+  return ids;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -155,6 +200,7 @@ int main(int argc, char **argv) {
     engine.resize(w, h);
     engine.initTCNN(model);
     int32_t mesh = -1;
+    std::map<std::string, int32_t> textures;  // absolute file name -> texture id
     if (!obj.empty()) {
       float *v = nullptr, *n = nullptr;
       int32_t *ix = nullptr, *mid = nullptr, nm = 0;
@@ -164,12 +210,24 @@ int main(int argc, char **argv) {
         if (st != NWPRT_OK) throw std::runtime_error(std::string(what) + ": " + nwprt_last_error());
       };
       const bool ply = obj.size() > 4 && obj.compare(obj.size() - 4, 4, ".ply") == 0;
-      need((ply ? nwprt_ply_read : nwprt_obj_read)(obj.c_str(), &v, &nv, &n, &ix, &nt, &mid,
-                                                   &mats, &nm),
-           "reading the mesh");
-      const nwprt_status st =
-          nwprt_mesh_create(engine.context(), v, nv, n, ix, nt, mid, mats, nm, &mesh);
-      for (void *p : {(void *)v, (void *)n, (void *)ix, (void *)mid, (void *)mats})
+      float *tc = nullptr;
+      char *maps = nullptr;
+      int64_t map_bytes = 0;
+      if (ply)
+        need(nwprt_ply_read(obj.c_str(), &v, &nv, &n, &ix, &nt, &mid, &mats, &nm),
+             "reading the mesh");
+      else
+        need(nwprt_obj_read_textured(obj.c_str(), &v, &nv, &n, &tc, &ix, &nt, &mid, &mats, &nm,
+                                     &maps, &map_bytes),
+             "reading the mesh");
+      nwprt_status st = nwprt_mesh_create(engine.context(), v, nv, n, ix, nt, mid, mats, nm, &mesh);
+      if (st == NWPRT_OK && tc) {
+        const std::vector<int32_t> matTex =
+            loadTextures(engine.context(), maps, map_bytes, nm, textures);
+        st = nwprt_mesh_set_textures(engine.context(), mesh, tc, matTex.data());
+      }
+      for (void *p : {(void *)v, (void *)n, (void *)tc, (void *)ix, (void *)mid, (void *)mats,
+                      (void *)maps})
         nwprt_free_host(p);
       need(st, "uploading the mesh");
       engine.setMesh(mesh);
@@ -220,6 +278,7 @@ int main(int argc, char **argv) {
     }
     engine.destroyLights();
     if (mesh >= 0) nwprt_mesh_destroy(engine.context(), mesh);
+    for (const auto &t : textures) nwprt_texture_destroy(engine.context(), t.second);
     if (envFaces) (void)hipFree(envFaces);
   } catch (const std::exception &e) {
     std::cerr << "nwprt_offline: " << e.what() << "\n";

@@ -290,7 +290,7 @@ def test_obj_reader(nw, tmp_path):
     (tmp_path / "s.obj").write_text(
         "# test\nmtllib m.mtl\n"
         "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0 0 1\n"
-        "vn 0 0 1\nvn 0 1 0\n"
+        "vn 0 0 1\nvn 0 1 0\nvt 0.25 0.75\n"
         "f 1 2 3\n"                      # before any usemtl: default material
         "usemtl red\nf 1//1 2//1 3//1 4//1\n"   # quad -> 2 triangles
         "usemtl flat\nf -5/1/2 -4/1/2 -1/1/2\n")  # negative indices, v/vt/vn
@@ -319,6 +319,81 @@ def test_obj_reader(nw, tmp_path):
     with pytest.raises(nw.NwprtError) as e:
         nw.obj_read(tmp_path / "missing.obj")
     assert e.value.status == nw.ERR_IO
+
+
+def test_obj_read_textured(nw, tmp_path):
+    """nwprt_obj_read_textured: texcoords and map_Kd / map_Ks / map_Ns file
+    names (relative to the OBJ's folder; absolute names kept), and the
+    reference's vertex order (Renderer/Model.cpp:73-146): per shape ("o" /
+    "g"), one mesh per material in ascending id with faces without one first,
+    each mesh numbering distinct (position, normal, texcoord) corners by first
+    use; a corner with a normal or texcoord fills the mesh's earlier vertices
+    that have none (addVertex, Model.cpp:38-70).  Polygons are cut by
+    tinyobjloader's ear clipping, so a concave one does not fan from a
+    reflex corner."""
+    (tmp_path / "t.mtl").write_text(
+        "newmtl a\nKd 1 0 0\nmap_Kd -s 1 1 1 wood.png\nmap_Ns /abs/r.png\n"
+        "newmtl b\nKd 0 1 0\nmap_Ks spec.tga\n")
+    (tmp_path / "t.obj").write_text(
+        "mtllib t.mtl\n"
+        "v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0.5 0.25 0\n"
+        "vt 0 0\nvt 1 0\nvt 1 1\nvt 0 1\nvn 0 0 1\n"
+        "o first\n"
+        "usemtl b\nf 1/1/1 2/2/1 3/3/1\n"      # shape 0, material b
+        "usemtl a\nf 1 3 4/4\n"                 # shape 0, material a: texcoord back-fill
+        "usemtl b\nf 3/3/1 4/4/1 1/1/1\n"      # shape 0, material b again
+        "o second\n"
+        "f 2 3 4 1 5\n")                       # shape 1, material b (kept): concave pentagon
+    m = nw.obj_read_textured(tmp_path / "t.obj")
+    mats, ix, v, tc = m["materials"], m["indices"], m["vertices"], m["texcoords"]
+    assert len(mats) == 2 and m["maps"] == [[str(tmp_path / "wood.png"), "", "/abs/r.png"],
+                                           ["", str(tmp_path / "spec.tga"), ""]]
+    np.testing.assert_allclose(mats[0, :3], [1, 0, 0])
+    # Mesh order: (shape 0, a), (shape 0, b), (shape 1, b).
+    assert list(m["material_ids"]) == [0, 1, 1] + [1] * 3
+    # Mesh (0, a): corners (1,-,-), (3,-,-), (4,t4,-): the texcoord fills all three.
+    np.testing.assert_array_equal(ix[0], [0, 1, 2])
+    np.testing.assert_array_equal(tc[:3], [[0, 1]] * 3)
+    np.testing.assert_array_equal(m["normals"][:3], np.zeros((3, 3)))  # no vn in this mesh
+    # Mesh (0, b): 1/1/1 2/2/1 3/3/1 then 3/3/1 4/4/1 1/1/1 -> vertices 3..6.
+    np.testing.assert_array_equal(ix[1:3], [[3, 4, 5], [5, 6, 3]])
+    np.testing.assert_array_equal(tc[3:7], [[0, 0], [1, 0], [1, 1], [0, 1]])
+    np.testing.assert_array_equal(m["normals"][3:7], [[0, 0, 1]] * 4)
+    # Mesh (1, b): the pentagon 2 3 4 1 5 has a reflex corner at 5.  Ear
+    # clipping cuts (2,3,4); (2,4,1) holds corner 5, so (4,1,5); then (2,4,5).
+    tris = [sorted(map(tuple, v[t])) for t in ix[3:]]
+    want = [[(1, 0, 0), (1, 1, 0), (0, 1, 0)], [(0, 1, 0), (0, 0, 0), (0.5, 0.25, 0)],
+            [(1, 0, 0), (0, 1, 0), (0.5, 0.25, 0)]]
+    assert tris == [sorted(w) for w in want]
+    assert len(v) == 7 + 5
+    # Total area equals the pentagon's (shoelace), so no triangle overlaps.
+    a = sum(0.5 * abs(np.cross(v[t[1]] - v[t[0]], v[t[2]] - v[t[0]])[2]) for t in ix[3:])
+    poly = np.array([[1, 0], [1, 1], [0, 1], [0, 0], [0.5, 0.25]])
+    x, y = poly[:, 0], poly[:, 1]
+    assert a == pytest.approx(0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1))))
+    # Texcoord index out of range is a format error (tinyobj only warns).
+    (tmp_path / "bad.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1/1 2/1 3/1\n")
+    with pytest.raises(nw.NwprtError) as e:
+        nw.obj_read_textured(tmp_path / "bad.obj")
+    assert e.value.status == nw.ERR_FORMAT
+
+
+def test_obj_concave_ear_order(nw, tmp_path):
+    """Ear clipping on a polygon whose first corner is reflex: the cut moves
+    on to the next corner (tiny_obj_loader.h:1463-1566), in the projection on
+    the axes the face normal favours (here x-z, a face in the y = 2 plane)."""
+    (tmp_path / "c.obj").write_text(
+        "v 0 2 0\nv 0.5 2 0.25\nv 1 2 0\nv 1 2 1\nv 0 2 1\nf 1 2 3 4 5\n")
+    m = nw.obj_read(tmp_path / "c.obj")
+    v, ix = m["vertices"], m["indices"]
+    assert len(ix) == 3
+    area = sum(0.5 * np.linalg.norm(np.cross(v[t[1]] - v[t[0]], v[t[2]] - v[t[0]])) for t in ix)
+    # Unit square minus the notch; a fan from corner 1 would cover 1.125.
+    assert area == pytest.approx(1.0 - 0.5 * 0.25)
+    # Corner 2 is reflex, so the first ear is (2, 3, 4), then (2, 4, 5), (1, 2, 5).
+    np.testing.assert_array_equal(v[ix[0]], [[0.5, 2, 0.25], [1, 2, 0], [1, 2, 1]])
+    np.testing.assert_array_equal(v[ix[1]], [[0.5, 2, 0.25], [1, 2, 1], [0, 2, 1]])
+    np.testing.assert_array_equal(v[ix[2]], [[0, 2, 0], [0.5, 2, 0.25], [0, 2, 1]])
 
 
 def test_ply_reader(nw, tmp_path):

@@ -279,3 +279,102 @@ def test_offline_driver_with_obj_scene(nw, ctx, tmp_path):
             got = nw.hdr_read(out / folder / f"{i}_a.hdr")
             np.testing.assert_array_equal(got, nw.hdr_read(want))
     ctx.mesh_destroy(mesh)
+
+
+def _png_rgba(path, rgba):
+    """8-bit RGBA PNG, top row first, no filtering."""
+    import struct
+    import zlib
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d))
+    h, w, _ = rgba.shape
+    raw = b"".join(b"\0" + rgba[y].astype(np.uint8).tobytes() for y in range(h))
+    path.write_bytes(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0))
+                     + chunk(b"IDAT", zlib.compress(raw)) + chunk(b"IEND", b""))
+
+
+def test_offline_driver_with_textured_obj(nw, ctx, tmp_path):
+    """--obj with texture coordinates and material maps: map_Kd and map_Ns
+    files that exist become the kd and roughness textures of their materials
+    (textureManager::addTexture, Renderer/textures.cpp:134-153; one texture
+    for a file two materials name); a map_Ks naming no file leaves ks
+    constant.  The images equal obj_read_textured -> mesh_create ->
+    texture_create(image_read_rgba8) -> mesh_set_textures -> gbuffer_cast
+    -> relight through the C ABI."""
+    import subprocess
+    from pathlib import Path
+    from cases import mesh_scene
+    exe = Path(nw.__file__).resolve().with_name("nwprt_offline")
+    rng = np.random.default_rng(16)
+    n_face, K = 8, 16
+    _write_model_folder(tmp_path / "model", rng, n_face)
+    env_dir = tmp_path / "env"
+    (env_dir / "cube").mkdir(parents=True)
+    nw.hdr_write(env_dir / "cube" / "a.hdr", _faces_to_cross(nw.synth_cubemap(7, n_face)))
+    s = mesh_scene(8, mirror=False)
+    vt = s["vertices"][:, [0, 2]] * 0.37 + 0.1  # planar projection, wraps
+    lines = ["mtllib scene.mtl"]
+    lines += [f"v {a:.9g} {b:.9g} {c:.9g}" for a, b, c in s["vertices"]]
+    lines += [f"vt {a:.9g} {b:.9g}" for a, b in vt]
+    lines += [f"vn {a:.9g} {b:.9g} {c:.9g}" for a, b, c in s["normals"]]
+    cur = None
+    for t, m in zip(s["indices"] + 1, s["material_ids"]):
+        if m != cur:
+            lines.append(f"usemtl m{m}")
+            cur = m
+        lines.append("f " + " ".join(f"{i}/{i}/{i}" for i in t))
+    (tmp_path / "scene.obj").write_text("\n".join(lines) + "\n")
+    (tmp_path / "tex").mkdir()
+    _png_rgba(tmp_path / "tex" / "kd.png", rng.integers(0, 256, (5, 7, 4)))
+    _png_rgba(tmp_path / "tex" / "rough.png", rng.integers(0, 256, (4, 3, 4)))
+    mtl = []
+    for i, row in enumerate(s["materials"]):
+        ns = 1.0 / np.sqrt(row[6]) if row[6] > 0 else 0.0
+        mtl += [f"newmtl m{i}", "Kd %.9g %.9g %.9g" % tuple(row[0:3]),
+                "Ks %.9g %.9g %.9g" % tuple(row[3:6]), f"Ns {ns:.9g}",
+                "map_Kd tex/kd.png", "map_Ks tex/none.png"]
+        if i % 2:
+            mtl.append("map_Ns tex/rough.png")
+    (tmp_path / "scene.mtl").write_text("\n".join(mtl) + "\n")
+    cams = [(0.5, 1.0, 6.0, 0.0, 0.0, -1.5)]
+    traj = tmp_path / "traj.txt"
+    traj.write_text(f"envmapFolder {env_dir}\nsamplesPerPixel 2\ndirectSamplesPerLight 2\n"
+                    "trajectoryBegin\n" + "".join(" ".join(map(str, c)) + " a\n" for c in cams))
+    out = tmp_path / "out"
+    r = subprocess.run([str(exe), str(out), "-w", str(tmp_path / "model"), "-t", str(traj),
+                        "-k", str(K), "--precision", "fp32", "--obj", str(tmp_path / "scene.obj")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    m = nw.obj_read_textured(tmp_path / "scene.obj")
+    assert m["texcoords"] is not None and len(m["materials"]) >= 2
+    mesh = ctx.mesh_create(m["vertices"], m["indices"], m["materials"], normals=m["normals"],
+                           material_ids=m["material_ids"])
+    kd = ctx.texture_create(nw.image_read_rgba8(str(tmp_path / "tex" / "kd.png")))
+    rough = ctx.texture_create(nw.image_read_rgba8(str(tmp_path / "tex" / "rough.png")))
+    mat_tex = np.full((len(m["materials"]), nw.TEX_FIELDS), -1, np.int32)
+    for i, (fkd, fks, fns) in enumerate(m["maps"]):
+        assert fkd.endswith("kd.png") and fks.endswith("none.png")
+        mat_tex[i, nw.TEX_FIELD_NAMES.index("kd")] = kd
+        if fns:
+            mat_tex[i, nw.TEX_FIELD_NAMES.index("roughness")] = rough
+    ctx.mesh_set_textures(mesh, m["texcoords"], mat_tex)
+    scene = ctx.initTCNN(tmp_path / "model")
+    env = ctx.envmap_load_folder(env_dir, "a", K)
+    faces = nw.cross_to_faces(nw.hdr_read(env_dir / "cube" / "a.hdr"))
+    dfaces = nw.DeviceBuffer.from_array(np.ascontiguousarray(faces, np.float32))
+    frame = nw.DeviceFrame(512, 512)
+    ctx.gbuffer_cast(mesh, (cams[0][0:3], cams[0][3:6], (0, 1, 0)), frame, env_faces=dfaces,
+                     env_n=n_face, light_faces=dfaces, light_n=n_face, direct_samples=4, seed=0)
+    ctx.relight(scene, env, frame, nw.PREC_FP32)
+    ctx.sync()
+    res = frame.results()
+    for key, folder in (("indirect", "predictedIndirect"),
+                        ("indirect_with_direct", "predictedWithDirect")):
+        want = tmp_path / f"want_{key}.hdr"
+        nw.hdr_write(want, res[key].reshape(512, 512, 4)[::-1])
+        got = nw.hdr_read(out / folder / "0_a.hdr")
+        np.testing.assert_array_equal(got, nw.hdr_read(want))
+    ctx.mesh_destroy(mesh)
+    ctx.texture_destroy(kd)
+    ctx.texture_destroy(rough)
