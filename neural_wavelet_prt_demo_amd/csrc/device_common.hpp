@@ -106,6 +106,26 @@ __device__ __forceinline__ uint32_t tonemap_rgba8(float r, float g, float b) {
   return tonemap_byte(r) | (tonemap_byte(g) << 8) | (tonemap_byte(b) << 16) | (255u << 24);
 }
 
+// Pointer typed as global memory.  Device buffers that reach a kernel
+// through memory (a batch's job table) cannot be told from LDS by the
+// compiler, which then emits flat loads; a flat load
+// counts against both the vector-memory and the LDS / scalar counters, so
+// each wait for an LDS read or a scalar load then also waits for the tile
+// prefetches in flight.  Loads through a gptr are global loads.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+
+// Load of memory no thread writes while the kernel runs (the selected
+// wavelets' light coefficients), through the constant address space, so a
+// wave-uniform address is served by a scalar load.  Without it a kernel
+// that loops over tiles and stores outputs in between cannot prove the
+// memory unchanged, issues vector loads, and their vmcnt waits also wait for
+// the tile prefetches in flight.
+template <typename T>
+__device__ __forceinline__ T load_invariant(const T *p) {
+  return *(const __attribute__((address_space(4))) T *)p;
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin; this remaps workgroup
 // `bid` of `nb` so that each XCD receives a contiguous run of tiles (a
 // bijection of [0, nb)).  Neighbouring pixels then share their coarse

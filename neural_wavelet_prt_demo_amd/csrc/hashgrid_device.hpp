@@ -62,9 +62,9 @@ struct Entry2 {
   typedef typename std::conditional<HF16, uint32_t, float2>::type raw;
   __device__ static __forceinline__ raw load(const void *hash, size_t e) {
     if constexpr (HF16)
-      return *((const uint32_t *)hash + e);
+      return *((gptr<uint32_t>)hash + e);
     else
-      return *((const float2 *)hash + e);
+      return __builtin_bit_cast(raw, *((gptr<unsigned long long>)hash + e));
   }
   __device__ static __forceinline__ void add(float w, raw v, float *acc) {
     if constexpr (HF16) {

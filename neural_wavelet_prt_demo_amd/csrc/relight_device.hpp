@@ -33,12 +33,12 @@ __device__ __forceinline__ void write_outputs(const RelightArgs &a, int p, float
 // 8-byte reads from planes slot0 / 4 and slot0 / 4 + 1 (plane stride P).
 __device__ __forceinline__ u32x4 load_features8(const RelightArgs &a, int P, int p, int slot0) {
   if (a.features_planar) {
-    const uint2 *pl = (const uint2 *)a.features;
-    const uint2 lo = pl[(size_t)(slot0 >> 2) * P + p];
-    const uint2 hi = pl[(size_t)((slot0 >> 2) + 1) * P + p];
-    return u32x4{lo.x, lo.y, hi.x, hi.y};
+    const gptr<uint64_t> pl = (gptr<uint64_t>)a.features;
+    const uint64_t lo = pl[(size_t)(slot0 >> 2) * P + p];
+    const uint64_t hi = pl[(size_t)((slot0 >> 2) + 1) * P + p];
+    return u32x4{(uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32)};
   }
-  return *(const u32x4 *)((const uint16_t *)a.features + (size_t)p * 32 + slot0);
+  return *(gptr<u32x4>)((const uint16_t *)a.features + (size_t)p * 32 + slot0);
 }
 
 // Transport of one output (softplus or identity); its clamped light term is
@@ -138,8 +138,8 @@ __device__ __forceinline__ void fused_encode_lds(const RelightArgs &a, bool acti
 #pragma unroll
   for (int i = 0; i < 16; ++i) pk[i] = 0u;
   if (active) {
-    const float4 x4 = *(const float4 *)(a.position + 4 * (size_t)my_px);
-    const float x[3] = {x4.x, x4.y, x4.z};
+    const f32x4 x4 = *(gptr<f32x4>)(a.position + 4 * (size_t)my_px);
+    const float x[3] = {x4[0], x4[1], x4[2]};
     // Four levels per batch: 32 gathers in flight before the first is used
     // (FP16 entries; two levels of FP32 entries per half-batch).  The table
     // type is tested per batch: that branch also keeps the compiler from

@@ -157,7 +157,8 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
   const int P = a.n_pixels;
   const int my_px = base + lane;
   const bool mine = my_px < P;
-  const bool hit = mine && (a.env_flag == nullptr || a.env_flag[my_px] != 0);
+  const bool hit =
+      mine && (a.env_flag == nullptr || *((dev::gptr<uint8_t>)a.env_flag + my_px) != 0);
   const bool any_hit = __ballot(hit) != 0;
 
   float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
@@ -196,19 +197,19 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
     for (int g = 0; g < 2; ++g) {
       const int p = base + 32 * g + col;
       auxb[g] = AUX && p < P
-                    ? scale_bf16x8(*(const u32x4 *)((const uint16_t *)a.aux + (size_t)p * 16 + 8 * h),
+                    ? scale_bf16x8(*(dev::gptr<u32x4>)((const uint16_t *)a.aux + (size_t)p * 16 + 8 * h),
                                    __uint_as_float((uint32_t)(ea2 + 127) << 23))
                     : u32x4{0u, 0u, 0u, 0u};
-      waux[g] = AUX ? ((const u32x4 *)a.waux32_tiles)[g * 64 + lane] : u32x4{0u, 0u, 0u, 0u};
+      waux[g] = AUX ? ((dev::gptr<u32x4>)a.waux32_tiles)[g * 64 + lane] : u32x4{0u, 0u, 0u, 0u};
     }
-    const i32x8 *w8 = (const i32x8 *)a.w8_tiles;
+    const dev::gptr<i32x8> w8 = (dev::gptr<i32x8>)a.w8_tiles;
     i32x8 w2[2], w3[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       w2[b] = w8[(0 * 2 + b) * 64 + lane];
       w3[b] = w8[(1 * 2 + b) * 64 + lane];
     }
-    const u32x4 *mk = (const u32x4 *)a.mk;
+    const dev::gptr<u32x4> mk = (dev::gptr<u32x4>)a.mk;
     u32x4 ta[4], tb[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) ta[q] = mk[q * 64 + lane];
@@ -320,7 +321,7 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
         const bool odd = (k & 1) != 0;
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-          const float l = LT::light(a.light[3 * k + ch]);
+          const float l = LT::light(dev::load_invariant(a.light + 3 * k + ch));
           if (odd) {
             dev::kahan_add(sum[ch], comp[ch], pair_sum<SOFTPLUS>(first[ch], l, o[ch]));
           } else {
@@ -355,8 +356,8 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
       }
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        float x = LT::term(LT::light(a.light[3 * k + ch]), o0[ch]);
-        if (two) x = pair_sum<SOFTPLUS>(x, LT::light(a.light[3 * (k + 1) + ch]), o1[ch]);
+        float x = LT::term(LT::light(dev::load_invariant(a.light + 3 * k + ch)), o0[ch]);
+        if (two) x = pair_sum<SOFTPLUS>(x, LT::light(dev::load_invariant(a.light + 3 * (k + 1) + ch)), o1[ch]);
         dev::kahan_add(sum[ch], comp[ch], x);
         if (DUMP) {
           const int q = my_px - a.dump_begin;
@@ -378,7 +379,7 @@ __device__ __forceinline__ void relight_fp8_wave(const RelightArgs &a, int base,
 
 // The wave's copy of the output layer's two tiles (lane l: entries l, 64 + l).
 __device__ __forceinline__ void load_w4(const RelightArgs &a, i32x8 *w4s, int lane) {
-  const i32x8 *src = (const i32x8 *)a.w8_tiles + 2 * 2 * 64;
+  const dev::gptr<i32x8> src = (dev::gptr<i32x8>)a.w8_tiles + 2 * 2 * 64;
   w4s[lane] = src[lane];
   w4s[64 + lane] = src[64 + lane];
 }

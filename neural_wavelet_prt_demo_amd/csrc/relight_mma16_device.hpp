@@ -242,11 +242,12 @@ template <typename E, bool SOFTPLUS, bool DUMP, bool AUX, int GP, bool W4LDS, bo
 __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int lane,
                                              const u32x4 *w4s, uint32_t *feat_lds) {
   const int g = lane >> 4, n = lane & 15;
-  const u32x4 *wt = (const u32x4 *)a.w_tiles;
+  const dev::gptr<u32x4> wt = (dev::gptr<u32x4>)a.w_tiles;
   const int P = a.n_pixels;
   const int my_px = base + lane;
   const bool mine = my_px < P;
-  const bool hit = mine && (a.env_flag == nullptr || a.env_flag[my_px] != 0);
+  const bool hit =
+      mine && (a.env_flag == nullptr || *((dev::gptr<uint8_t>)a.env_flag + my_px) != 0);
   const bool any_hit = __ballot(hit) != 0;
 
   float sum[3] = {0.f, 0.f, 0.f}, comp[3] = {0.f, 0.f, 0.f};
@@ -282,7 +283,7 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
         w3[b][kb] = wt[(1 * 8 + b * 2 + kb) * 64 + lane];
         if (!W4LDS) w4[b][kb] = wt[(2 * 8 + b * 2 + kb) * 64 + lane];
       }
-    const u32x4 *mk = (const u32x4 *)a.mk;
+    const dev::gptr<u32x4> mk = (dev::gptr<u32x4>)a.mk;
     u32x4 ta[4], tb[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) ta[b] = mk[b * 64 + lane];
@@ -312,8 +313,8 @@ __device__ __forceinline__ void relight_wave(const RelightArgs &a, int base, int
       }
 #pragma unroll
       for (int ch = 0; ch < 3; ++ch) {
-        float x = LT::term(LT::light(a.light[3 * k + ch]), o0[ch]);
-        if (two) x += LT::term(LT::light(a.light[3 * (k + 1) + ch]), o1[ch]);
+        float x = LT::term(LT::light(dev::load_invariant(a.light + 3 * k + ch)), o0[ch]);
+        if (two) x += LT::term(LT::light(dev::load_invariant(a.light + 3 * (k + 1) + ch)), o1[ch]);
         dev::kahan_add(sum[ch], comp[ch], x);
         if (DUMP) {
           const int q = my_px - a.dump_begin;
