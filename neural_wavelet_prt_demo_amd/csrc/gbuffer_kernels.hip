@@ -283,7 +283,10 @@ struct Surface {
 // One texel of a material texture as the reference's CUDA texture object
 // reads it: b / 255, the colour channels through the sRGB decode.
 __device__ inline float4 texel(const TexRef &t, int i, int j) {
-  const uint32_t q = reinterpret_cast<const uint32_t *>(t.rgba)[(size_t)j * t.w + i];
+  // The texel pointer comes from the mesh's texture table in memory: typed
+  // as global so the fetch is a global, not a flat, load.
+  typedef const __attribute__((address_space(1))) uint32_t *global_u32;
+  const uint32_t q = ((global_u32)t.rgba)[(size_t)j * t.w + i];
   return make_float4(kSrgbToLinear[q & 0xffu], kSrgbToLinear[(q >> 8) & 0xffu],
                      kSrgbToLinear[(q >> 16) & 0xffu], (float)(q >> 24) * (1.0f / 255.0f));
 }
