@@ -115,7 +115,7 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, np.float64), q))
 
 
-def bench_serving(nw, ctx, prec, n_requests=600, rates=(8, 12, 20, 30, 60, 120),
+def bench_serving(nw, ctx, prec, n_requests=600, rates=(4, 8, 20, 30, 60, 120),
                   mix_4k=True):
     """Served stream: Poisson arrivals, 4 resident scenes (face 128, FP16 hash
     grid 16 x 2^19 x 2), K = 300; 70 % of requests turn a resident 6x128x128
