@@ -25,6 +25,14 @@
 namespace nwprt {
 namespace {
 
+// Largest image any reader accepts (the texture limit): checked before a
+// pixel buffer is allocated, so a corrupt header cannot ask for gigabytes.
+constexpr int64_t kMaxPixels = 1ll << 28;
+void check_size(int64_t w, int64_t h, const char *what) {
+  require(w > 0 && h > 0 && w * h <= kMaxPixels, NWPRT_ERR_FORMAT,
+          std::string(what) + ": bad size (1 to 2^28 pixels)");
+}
+
 // ------------------------------------------------------------- inflate --
 
 class BitReader {
@@ -278,8 +286,8 @@ Image read_png(const std::vector<uint8_t> &f) {
     }
     p += 12 + (size_t)len;
   }
-  require(have_ihdr && w > 0 && h > 0 && (int64_t)w * h <= (1ll << 28), NWPRT_ERR_FORMAT,
-          "png: missing IHDR or bad size");
+  require(have_ihdr, NWPRT_ERR_FORMAT, "png: missing IHDR");
+  check_size(w, h, "png");
   int channels = 0;
   switch (ctype) {
     case 0: channels = 1; break;
@@ -392,7 +400,7 @@ Image read_tga(const std::vector<uint8_t> &f) {
   const bool grey = type == 3 || type == 11;
   require(grey ? bits == 8 : (bits == 24 || bits == 32), NWPRT_ERR_UNSUPPORTED,
           "tga: 8-bit grey or 24/32-bit colour only");
-  require(w > 0 && h > 0, NWPRT_ERR_FORMAT, "tga: bad size");
+  check_size(w, h, "tga");
   size_t p = 18 + id_len + (size_t)cmap_len * ((cmap_bits + 7) / 8);
   const int bpp = bits / 8;
   std::vector<uint8_t> px((size_t)w * h * bpp);
@@ -446,19 +454,21 @@ Image read_bmp(const std::vector<uint8_t> &f) {
   auto le32 = [&](size_t p) { return (uint32_t)f[p] | (f[p + 1] << 8) | (f[p + 2] << 16) | ((uint32_t)f[p + 3] << 24); };
   require(f.size() >= 54 && f[0] == 'B' && f[1] == 'M', NWPRT_ERR_FORMAT, "bmp: bad header");
   const size_t off = le32(10);
-  const int w = (int)le32(18), hs = (int)le32(22), bits = f[28] | (f[29] << 8);
+  const int64_t w = (int32_t)le32(18), hs = (int32_t)le32(22);
+  const int bits = f[28] | (f[29] << 8);
   const uint32_t comp = le32(30);
-  require(w > 0 && hs != 0 && (bits == 24 || bits == 32) && (comp == 0 || (comp == 3 && bits == 32)),
+  require((bits == 24 || bits == 32) && (comp == 0 || (comp == 3 && bits == 32)),
           NWPRT_ERR_UNSUPPORTED, "bmp: 24/32-bit uncompressed only");
-  const int h = hs < 0 ? -hs : hs;
+  const int64_t h = hs < 0 ? -hs : hs;
+  check_size(w, h, "bmp");
   const size_t stride = (((size_t)w * bits / 8) + 3) & ~(size_t)3;
   require(off + stride * h <= f.size(), NWPRT_ERR_FORMAT, "bmp: pixel data too short");
   Image img;
-  img.width = w;
-  img.height = h;
+  img.width = (int)w;
+  img.height = (int)h;
   img.rgba.resize((size_t)w * h * 4);
   for (int y = 0; y < h; ++y) {
-    const int Y = hs > 0 ? y : h - 1 - y;  // positive height: stored bottom-up
+    const int Y = hs > 0 ? y : (int)h - 1 - y;  // positive height: stored bottom-up
     for (int x = 0; x < w; ++x) {
       const uint8_t *s = &f[off + (size_t)y * stride + (size_t)x * (bits / 8)];
       uint8_t *o = &img.rgba[((size_t)Y * w + x) * 4];
@@ -488,13 +498,17 @@ Image read_pnm(const std::vector<uint8_t> &f) {
     }
     long v = 0;
     require(std::isdigit(f[p]), NWPRT_ERR_FORMAT, "pnm: bad header number");
-    while (p < f.size() && std::isdigit(f[p])) v = v * 10 + (f[p++] - '0');
+    while (p < f.size() && std::isdigit(f[p])) {
+      v = v * 10 + (f[p++] - '0');
+      require(v <= (1l << 30), NWPRT_ERR_FORMAT, "pnm: header number too large");
+    }
     return (int)v;
   };
   const bool colour = f[1] == '6';
   const int w = token(), h = token(), maxv = token();
   ++p;  // the single whitespace byte before the raster
-  require(w > 0 && h > 0 && maxv > 0 && maxv < 65536, NWPRT_ERR_FORMAT, "pnm: bad size or maxval");
+  check_size(w, h, "pnm");
+  require(maxv > 0 && maxv < 65536, NWPRT_ERR_FORMAT, "pnm: bad maxval");
   const int ch = colour ? 3 : 1, bps = maxv > 255 ? 2 : 1;
   require(p + (size_t)w * h * ch * bps <= f.size(), NWPRT_ERR_FORMAT, "pnm: raster too short");
   Image img;

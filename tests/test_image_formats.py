@@ -211,3 +211,19 @@ def test_image_errors(tmp_path):
     with pytest.raises(nw.NwprtError) as e:
         nw.image_read_rgba8(str(tmp_path / "missing.png"))
     assert e.value.status == nw.ERR_IO
+
+
+def test_oversized_headers_rejected(tmp_path):
+    """Headers that ask for more than 2^28 pixels (or carry absurd numbers)
+    are format errors before any pixel buffer is allocated."""
+    tga = struct.pack("<BBBHHBHHHHBB", 0, 0, 10, 0, 0, 0, 0, 0, 65535, 65535, 32, 0)
+    bmp = (b"BM" + struct.pack("<IHHI", 54, 0, 0, 54) +
+           struct.pack("<IiiHHIIiiII", 40, 100000, -(2 ** 31), 1, 24, 0, 0, 0, 0, 0, 0))
+    pnm = b"P6 99999999999999999999 2 255\n"
+    png = b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", struct.pack(">IIBBBBB", 1 << 20, 1 << 20, 8, 6, 0, 0, 0))
+    for name, data in (("big.tga", tga), ("big.bmp", bmp), ("big.ppm", pnm), ("big.png", png)):
+        p = tmp_path / name
+        p.write_bytes(data + bytes(64))
+        with pytest.raises(nw.NwprtError) as e:
+            nw.image_read_rgba8(str(p))
+        assert e.value.status == nw.ERR_FORMAT, name
